@@ -1,0 +1,224 @@
+/*
+ * mife.h - C ABI of the MI355X-native finite-element assembly and sparse-solve
+ * library (libmi_fe.so) that stands in for OpenCMISS-Iron's
+ *   FiniteElementCalculate -> EquationsMatrices element-add -> DistributedMatrix CSR
+ *   -> Dirichlet reduction -> KSP (CG / GMRES + Jacobi) -> back-substitution
+ * path.  Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status:
+ *   0  MIFE_OK                         (CMFE_NO_ERROR, bindings/generate_bindings/c.py:31)
+ *  -1  MIFE_ERR_POINTER_IS_NULL        (CMFE_POINTER_IS_NULL, c.py:32)
+ *  -3  MIFE_ERR_COULD_NOT_ALLOCATE     (CMFE_COULD_NOT_ALLOCATE_POINTER, c.py:34)
+ *  >0  errors raised by the library itself, as iron's FlagError (err=1,
+ *      src/base_routines.F90:673-741) with a message from mife_last_error().
+ *
+ * Buffers are device pointers owned by the caller unless stated otherwise;
+ * work is ordered on the caller's hipStream_t (passed as void*; NULL = default
+ * stream).  Index arrays are 0-based int32 (iron's CSR is 1-based; the
+ * conversion is a constant -1).  Calls that read results back to the host
+ * (iteration counts, norms) say so.
+ */
+#ifndef MIFE_H
+#define MIFE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MIFE_OK 0
+#define MIFE_ERR_POINTER_IS_NULL (-1)
+#define MIFE_ERR_COULD_NOT_ALLOCATE (-3)
+#define MIFE_ERR 1
+#define MIFE_ERR_ARG 2
+#define MIFE_ERR_HIP 3
+#define MIFE_ERR_NOT_CONVERGED 4
+#define MIFE_ERR_PATTERN 5
+
+/* Value precision selector for CSR values and element kernels. */
+#define MIFE_F64 0
+#define MIFE_F32 1
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Library version string (static storage). */
+const char* mife_version(void);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated).
+ * Replaces cmfe_ExtractErrorMessage (bindings/python/opencmiss/iron/_utils.in.py:66-77). */
+int mife_last_error(char* buf, int n);
+
+/* Selects the HIP device and warms the runtime.  Replaces the device-free
+ * cmfe_Initialise_ (src/cmiss.F90:237). */
+int mife_init(int device);
+
+/* ------------------------------------------------- sparsity and DOF maps */
+
+/* Builds the FEM CSR sparsity of an equations matrix on the GPU:
+ * row d holds the sorted unique union of the DOF lists of every element whose
+ * DOF list contains d.  Replaces EquationsMatrix_StructureCalculate (FEM/CSR,
+ * src/equations_matrices_routines.F90:4399-4461) with LIST_REMOVE_DUPLICATES
+ * (src/lists.F90:2652-2700).
+ *   elem_dofs [n_elem*n_edof]  element DOF lists (device)
+ *   row_ptr   [n_dofs+1]       output (device)
+ * Pass 1 (col == NULL): fills row_ptr and writes the nnz to *nnz_out (host read).
+ * Pass 2 (col != NULL): fills col[nnz] (device). work must hold
+ * mife_csr_pattern_work_bytes(n_elem, n_edof, n_dofs) bytes and be preserved
+ * between the two passes. */
+size_t mife_csr_pattern_work_bytes(int n_elem, int n_edof, int n_dofs);
+int mife_csr_pattern(const int32_t* elem_dofs, int n_elem, int n_edof, int n_dofs,
+                     int32_t* row_ptr, int32_t* col, int64_t* nnz_out,
+                     void* work, void* stream);
+
+/* Per-element CSR position map: pos[e][i][j] = position of column
+ * elem_dofs[e][j] in row elem_dofs[e][i].  Replaces the per-entry
+ * MATRIX_STORAGE_LOCATION_FIND bisection (src/matrix_vector.F90:1724-1854)
+ * done inside every MATRIX_VALUES_ADD_DP2 (:3119) with a one-off table.
+ * Returns MIFE_ERR_PATTERN if an entry is missing (checked on device, read back). */
+int mife_scatter_map(const int32_t* elem_dofs, int n_elem, int n_edof,
+                     const int32_t* row_ptr, const int32_t* col,
+                     int32_t* pos, void* stream);
+
+/* Greedy element colouring on the host (no two elements of one colour share a
+ * DOF), so that coloured scatter-adds are race-free and deterministic.
+ * elem_dofs is a HOST array here.  colour[n_elem] is written; returns the
+ * number of colours in *n_colours. */
+int mife_colour_elements(const int32_t* elem_dofs_host, int n_elem, int n_edof,
+                         int n_dofs, int32_t* colour_host, int* n_colours);
+
+/* Reduced (solver) pattern over the free DOFs: keeps entries whose row and
+ * column are both free, renumbers them by rank among free DOFs and records the
+ * source position of each kept entry.  Replaces SOLVER_MAPPING_CALCULATE row and
+ * column inclusion (src/solver_mapping_routines.F90:422-547, :1445-1511) and
+ * SOLVER_MATRIX_STRUCTURE_CALCULATE (src/solver_matrices_routines.F90:1612-2431).
+ *   new_idx [n]    rank of each DOF among free DOFs, -1 if fixed (device)
+ * Pass 1 (s_col == NULL): s_row_ptr[n_free+1] and *nnz_out (host read).
+ * Pass 2: s_col[nnz_f], src_pos[nnz_f]. */
+int mife_reduce_pattern(int n, const int32_t* row_ptr, const int32_t* col,
+                        const int32_t* new_idx, int n_free,
+                        int32_t* s_row_ptr, int32_t* s_col, int32_t* src_pos,
+                        int64_t* nnz_out, void* work, void* stream);
+
+/* ------------------------------------------------------------- assembly */
+
+/* Element-kernel geometry and basis description shared by the assembly calls.
+ * Tables are device arrays in Gauss-point-major order. */
+typedef struct mife_element_desc {
+  int dim;          /* number of xi = number of x (2 or 3) */
+  int n_nodes;      /* element nodes of the (single) basis: 4, 8, 9, 27 */
+  int n_gauss;      /* Gauss points */
+  const double* dN; /* [n_gauss][n_nodes][dim] dphi/dxi (GAUSS_BASIS_FNS(:,PART_DERIV_Si,:)) */
+  const double* N;  /* [n_gauss][n_nodes] phi (GAUSS_BASIS_FNS(:,NO_PART_DERIV,:)) or NULL */
+  const double* w;  /* [n_gauss] Gauss weights */
+} mife_element_desc;
+
+/* Scalar Laplace / generalised-Laplace / constant-source Poisson assembly with
+ * deterministic coloured scatter into CSR values.  Replaces, per element,
+ * LaplaceEquation_FiniteElementCalculate (src/Laplace_equations_routines.F90:482-747),
+ * POISSON_EQUATION_FINITE_ELEMENT_CALCULATE constant source
+ * (src/Poisson_equations_routines.F90:3347-3418) and EquationsMatrices_ElementAdd
+ * (src/equations_matrices_routines.F90:1140-1265).
+ *   coords   [n_nodes_geom][dim] FP64 node coordinates (device)
+ *   elem_nodes [n_elem][n_nodes] geometric node of each element node (device)
+ *   elem_dofs  [n_elem][n_nodes] row/column DOF of each element node (device)
+ *   kappa    [n_elem] element conductivity or NULL (=1)
+ *   source   [n_elem] element source f or NULL (no source vector)
+ *   pos      [n_elem][n_nodes][n_nodes] scatter map
+ *   order    [n_elem] element ids grouped by colour; colour_ptr[n_colours+1]
+ *            (host array) delimits each colour; order == NULL means atomics.
+ *   vals     CSR values (FP64 or FP32 per `prec`), accumulated into (+=)
+ *   src_vec  FP64 source vector, accumulated into (+=), may be NULL */
+int mife_assemble_scalar(const mife_element_desc* desc, int n_elem,
+                         const double* coords, const int32_t* elem_nodes,
+                         const int32_t* elem_dofs, const double* kappa,
+                         const double* source, const int32_t* pos,
+                         const int32_t* order, const int32_t* colour_ptr_host,
+                         int n_colours, int prec, void* vals, double* src_vec,
+                         void* stream);
+
+/* 3-D linear elasticity (orthotropic C from E1,E2,E3,v13,v23,v12 per element),
+ * element DOFs component-blocked [u1(1..nn),u2,u3].  Replaces
+ * LINEAR_ELASTICITY_FINITE_ELEMENT_CALCULATE (src/linear_elasticity_routines.F90:1109-1403)
+ * and LINEAR_ELASTICITY_TENSOR (:1410-1510) plus EquationsMatrices_ElementAdd.
+ *   material [n_elem][6] FP64 (E1,E2,E3,v13,v23,v12)
+ *   elem_dofs [n_elem][3*n_nodes] */
+int mife_assemble_elastic(const mife_element_desc* desc, int n_elem,
+                          const double* coords, const int32_t* elem_nodes,
+                          const int32_t* elem_dofs, const double* material,
+                          const int32_t* pos, const int32_t* order,
+                          const int32_t* colour_ptr_host, int n_colours, int prec,
+                          void* vals, void* stream);
+
+/* --------------------------------------------------- sparse operations */
+
+/* y = A x (prec selects FP64 or FP32 values and vectors). */
+int mife_spmv(int n_rows, const int32_t* row_ptr, const int32_t* col,
+              const void* vals, const void* x, void* y, int prec, void* stream);
+
+/* dst[k] = src[src_pos[k]] (FP64 -> FP64 or FP32 per dst_prec).  The per-solve
+ * copy of K_ff into the solver matrix, SOLVER_MATRIX_EQUATIONS_MATRIX_ADD
+ * (src/solver_matrices_routines.F90:806-1023). */
+int mife_gather_values(int64_t nnz, const int32_t* src_pos, const double* src,
+                       void* dst, int dst_prec, void* stream);
+
+/* Solver RHS over the free rows (SOLVER_MATRICES_STATIC_ASSEMBLE RHS part,
+ * src/solver_routines.F90:14640-14850):
+ *   b[f] = src[i] + rhs[i] - sum_{j fixed, |u_j|>=ZT, |K_ij|>=ZT} K_ij u_j,  i = free_idx[f]
+ * src, rhs may be NULL.  ZT = 5*DBL_EPSILON (src/constants.F90:69). */
+int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr,
+                    const int32_t* col, const double* vals, const int32_t* new_idx,
+                    const double* u, const double* src, const double* rhs,
+                    double* b, void* stream);
+
+/* u[free_idx[f]] = x[f] (SOLVER_VARIABLES_FIELD_UPDATE, src/solver_routines.F90:22814-23001). */
+int mife_field_update(int n_free, const int32_t* free_idx, const double* x,
+                      double* u, void* stream);
+
+/* Reactions / flux on the RHS variable, EQUATIONS_SET_BACKSUBSTITUTE CSR branch
+ * (src/equations_set_routines.F90:1784-1830).  iron_compat=1 reproduces the
+ * reference's u[k - rowstart] read (:1806); 0 computes the correct K u.
+ * rhs_fixed (uint8 [n] or NULL) marks RHS DOFs with a fixed condition.
+ * R[i] = (free ? sum : 0) - (src ? src[i] : 0). */
+int mife_backsubstitute(int n, const int32_t* row_ptr, const int32_t* col,
+                        const double* vals, const double* u, const double* src,
+                        const uint8_t* rhs_fixed, int iron_compat, double* R,
+                        void* stream);
+
+/* ----------------------------------------------------- Krylov solvers */
+
+/* Workspace bytes for mife_cg / mife_refine / mife_gmres on n rows. */
+size_t mife_solver_work_bytes(int n, int restart);
+
+/* Jacobi-preconditioned CG in the precision of `prec` (x, b and vals alike),
+ * stopped when ||b - A x||_2 <= rtol ||b||_2 (recurrence residual, confirmed by
+ * an explicit residual at exit) or after maxit iterations.  Replaces KSPCG+PCJACOBI
+ * (SOLVER_LINEAR_ITERATIVE_CREATE_FINISH / _SOLVE, src/solver_routines.F90:10921-11157,
+ * :11775-11984).  x holds the initial guess on entry.  iters and relres are
+ * host outputs (the call synchronises the stream).  Returns
+ * MIFE_ERR_NOT_CONVERGED when maxit is reached. */
+int mife_cg(int n, const int32_t* row_ptr, const int32_t* col, const void* vals,
+            const void* b, void* x, int prec, double rtol, int maxit,
+            void* work, int* iters, double* relres, void* stream);
+
+/* Mixed-precision solve: FP64 iterative refinement around an FP32
+ * Jacobi-preconditioned CG.  Outer residual r = b - A64 x and the solution
+ * accumulate in FP64 (vals64); the inner CG runs on vals32.  Stops when
+ * ||b - A64 x|| <= rtol ||b||.  inner_rtol is the inner CG reduction per outer
+ * step.  iters = total inner iterations, outer = refinement steps. */
+int mife_refine(int n, const int32_t* row_ptr, const int32_t* col,
+                const double* vals64, const float* vals32, const double* b,
+                double* x, double rtol, double inner_rtol, int maxit,
+                void* work, int* iters, int* outer, double* relres, void* stream);
+
+/* Right-Jacobi-preconditioned restarted GMRES(restart) in FP64 (KSPGMRES default
+ * restart 30, src/solver_routines.F90:11320-11369). */
+int mife_gmres(int n, const int32_t* row_ptr, const int32_t* col, const double* vals,
+               const double* b, double* x, double rtol, int restart, int maxit,
+               void* work, int* iters, double* relres, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIFE_H */

@@ -1,0 +1,119 @@
+"""ctypes binding of libmi_fe.so (declared in include/mife.h).
+
+The library is required: importing this module on a machine without the built
+library raises, and there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libmi_fe.so"
+
+MIFE_OK = 0
+MIFE_ERR_NOT_CONVERGED = 4
+MIFE_F64 = 0
+MIFE_F32 = 1
+
+
+class CMFEError(RuntimeError):
+    """Raised on a nonzero status, as the reference bindings raise CMFEError
+    (bindings/python/opencmiss/iron/_utils.in.py:66-77)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{message} (error code {code})")
+        self.code = code
+
+
+class NotConvergedError(CMFEError):
+    pass
+
+
+class ElementDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n_nodes", C.c_int), ("n_gauss", C.c_int),
+                ("dN", C.c_void_p), ("N", C.c_void_p), ("w", C.c_void_p)]
+
+
+_lib = None
+
+_P = C.c_void_p
+_I = C.c_int
+_I64 = C.c_int64
+_D = C.c_double
+_SIG = {
+    "mife_version": (C.c_char_p, []),
+    "mife_last_error": (_I, [C.c_char_p, _I]),
+    "mife_init": (_I, [_I]),
+    "mife_csr_pattern_work_bytes": (C.c_size_t, [_I, _I, _I]),
+    "mife_csr_pattern": (_I, [_P, _I, _I, _I, _P, _P, C.POINTER(_I64), _P, _P]),
+    "mife_scatter_map": (_I, [_P, _I, _I, _P, _P, _P, _P]),
+    "mife_colour_elements": (_I, [_P, _I, _I, _I, _P, C.POINTER(_I)]),
+    "mife_reduce_pattern": (_I, [_I, _P, _P, _P, _I, _P, _P, _P, C.POINTER(_I64), _P, _P]),
+    "mife_assemble_scalar": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
+    "mife_assemble_elastic": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P]),
+    "mife_spmv": (_I, [_I, _P, _P, _P, _P, _P, _I, _P]),
+    "mife_gather_values": (_I, [_I64, _P, _P, _P, _I, _P]),
+    "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "mife_field_update": (_I, [_I, _P, _P, _P, _P]),
+    "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    "mife_solver_work_bytes": (C.c_size_t, [_I, _I]),
+    "mife_cg": (_I, [_I, _P, _P, _P, _P, _P, _I, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_refine": (_I, [_I, _P, _P, _P, _P, _P, _P, _D, _D, _I, _P, C.POINTER(_I), C.POINTER(_I),
+                         C.POINTER(_D), _P]),
+    "mife_gmres": (_I, [_I, _P, _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+}
+
+
+def lib():
+    """Load (once) and return the library handle."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m iron_amd.build` (hipcc, gfx950)")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIG.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIG)
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().mife_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(code: int):
+    if code == MIFE_OK:
+        return
+    msg = last_error()
+    if code == MIFE_ERR_NOT_CONVERGED:
+        raise NotConvergedError(code, msg)
+    raise CMFEError(code, msg)
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a torch tensor / numpy array, or None."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def stream_ptr(stream=None) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def call(name: str, *args):
+    check(getattr(lib(), name)(*args))

@@ -1,0 +1,327 @@
+// Element integration + scatter-add into CSR values.
+//
+// One 64-lane wavefront integrates one element: lanes gather the element's
+// node coordinates into LDS, one lane per Gauss point forms dx/dxi, its
+// inverse and |J| (always FP64), then the lanes build the physical gradients
+// B[(g,i)][a] and contract K = B^T W B in the value precision T, and finally
+// add K into the CSR values through the precomputed position map.  Colours
+// are launched one after another, so every CSR entry is updated by at most one
+// wavefront per launch and the summation order is fixed (deterministic).
+#include "mife_common.hpp"
+
+namespace mife {
+
+struct Mat3 {
+  double a[3][3];
+};
+
+// 2x2 / 3x3 inverse by cofactors (maths.F90 InvertFullDP ordering of terms).
+template <int DIM>
+__device__ __forceinline__ double invert(const double (&A)[3][3], double (&B)[3][3]) {
+  if constexpr (DIM == 2) {
+    const double det = A[0][0] * A[1][1] - A[1][0] * A[0][1];
+    B[0][0] = A[1][1] / det;
+    B[0][1] = -A[0][1] / det;
+    B[1][0] = -A[1][0] / det;
+    B[1][1] = A[0][0] / det;
+    return det;
+  } else {
+    const double det = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] -
+                       A[0][0] * A[2][1] * A[1][2] - A[1][0] * A[0][1] * A[2][2] - A[2][0] * A[1][1] * A[0][2];
+    B[0][0] = (A[1][1] * A[2][2] - A[2][1] * A[1][2]) / det;
+    B[1][0] = (A[1][2] * A[2][0] - A[2][2] * A[1][0]) / det;
+    B[2][0] = (A[1][0] * A[2][1] - A[2][0] * A[1][1]) / det;
+    B[0][1] = (A[2][1] * A[0][2] - A[0][1] * A[2][2]) / det;
+    B[1][1] = (A[2][2] * A[0][0] - A[0][2] * A[2][0]) / det;
+    B[2][1] = (A[2][0] * A[0][1] - A[0][0] * A[2][1]) / det;
+    B[0][2] = (A[0][1] * A[1][2] - A[1][1] * A[0][2]) / det;
+    B[1][2] = (A[0][2] * A[1][0] - A[1][2] * A[0][0]) / det;
+    B[2][2] = (A[0][0] * A[1][1] - A[1][0] * A[0][1]) / det;
+    return det;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void add_value(T* p, T v, bool atomic) {
+  if (atomic) atomicAdd(p, v);
+  else *p += v;
+}
+
+// Per-Gauss-point geometry of one element held in LDS by its wavefront.
+template <int DIM, int NN, int NG>
+struct GeomLDS {
+  double x[NN * DIM];
+  double ji[NG][DIM][DIM];  // dxi_j/dx_i stored as ji[g][j][i]
+  double rwg[NG];           // |J| * w_g
+};
+
+template <int DIM, int NN, int NG>
+__device__ __forceinline__ void element_geometry(GeomLDS<DIM, NN, NG>& G, const double* __restrict__ coords,
+                                                 const int32_t* __restrict__ enodes, long long e,
+                                                 const double* __restrict__ dN, const double* __restrict__ w,
+                                                 int lane, bool active) {
+  if (active && lane < NN) {
+    const long long node = enodes[e * NN + lane];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) G.x[lane * DIM + d] = coords[node * DIM + d];
+  }
+  __syncthreads();
+  if (active && lane < NG) {
+    const int g = lane;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < NN; ++a) {
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        const double dn = dN[(g * NN + a) * DIM + j];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) J[d][j] += G.x[a * DIM + d] * dn;
+      }
+    }
+    double Ji[3][3];
+    const double det = invert<DIM>(J, Ji);
+#pragma unroll
+    for (int j = 0; j < DIM; ++j)
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) G.ji[g][j][i] = Ji[j][i];
+    G.rwg[g] = fabs(det) * w[g];
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------ scalar field
+template <int DIM, int NN, int NG, typename T, int WPB>
+__global__ __launch_bounds__(64 * WPB) void scalar_assemble_kernel(
+    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
+    const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
+    const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
+    const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
+    bool atomic) {
+  constexpr int KD = NG * DIM;
+  __shared__ GeomLDS<DIM, NN, NG> geo[WPB];
+  __shared__ T sB[WPB][KD * NN];
+  __shared__ T sW[WPB][KD];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = blockIdx.x * WPB + wv;
+  const bool active = idx < elem_count;
+  long long e = 0;
+  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
+  GeomLDS<DIM, NN, NG>& G = geo[wv];
+  element_geometry<DIM, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
+  const double ke = (active && kappa) ? kappa[e] : 1.0;
+  if (active) {
+    for (int t = lane; t < NG * NN; t += 64) {
+      const int g = t / NN, a = t - (t / NN) * NN;
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        double b = 0.0;
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) b += dN[(g * NN + a) * DIM + j] * G.ji[g][j][i];
+        sB[wv][(g * DIM + i) * NN + a] = (T)b;
+      }
+    }
+    for (int k = lane; k < KD; k += 64) sW[wv][k] = (T)(G.rwg[k / DIM] * ke);
+  }
+  __syncthreads();
+  if (active) {
+    const T* B = sB[wv];
+    const T* W = sW[wv];
+    for (int t = lane; t < NN * NN; t += 64) {
+      const int m = t / NN, n = t - (t / NN) * NN;
+      T acc = 0;
+#pragma unroll 9
+      for (int k = 0; k < KD; ++k) acc += W[k] * B[k * NN + m] * B[k * NN + n];
+      const int p = pos[e * (NN * NN) + t];
+      add_value(&vals[p], acc, atomic);
+    }
+    if (source && svec && lane < NN) {
+      double s = 0.0;
+      for (int g = 0; g < NG; ++g) s += N[g * NN + lane] * G.rwg[g];
+      s *= source[e];
+      add_value(&svec[edofs[e * NN + lane]], s, atomic);
+    }
+  }
+}
+
+// ------------------------------------------------------- linear elasticity
+// Iron's per-block coefficient use (linear_elasticity_routines.F90:1255-1286):
+// diagonal block c uses JRWG*diagC[c][d] on dphi_d*dphi_d; off-diagonal block
+// (c1<c2) uses Ca*dphi_n,c1*dphi_m,c2 + Cb*dphi_n,c2*dphi_m,c1.
+template <int NN, int NG, typename T, int WPB>
+__global__ __launch_bounds__(64 * WPB) void elastic_assemble_kernel(
+    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const double* __restrict__ material,
+    const int32_t* __restrict__ pos, const int32_t* __restrict__ order, int elem_begin, int elem_count,
+    const double* __restrict__ dN, const double* __restrict__ w, T* __restrict__ vals, bool atomic) {
+  constexpr int ND = 3 * NN;
+  __shared__ GeomLDS<3, NN, NG> geo[WPB];
+  __shared__ T sB[WPB][NG][NN][3];
+  __shared__ T sC[WPB][NG][12];  // per GP: diag[3][3] (9) + off Ca,Cb handled below
+  __shared__ T sOff[WPB][NG][6];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = blockIdx.x * WPB + wv;
+  const bool active = idx < elem_count;
+  long long e = 0;
+  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
+  GeomLDS<3, NN, NG>& G = geo[wv];
+  element_geometry<3, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
+  if (active) {
+    for (int t = lane; t < NG * NN; t += 64) {
+      const int g = t / NN, a = t - (t / NN) * NN;
+      // dphidx(a,mi) = sum_ni DXI_DX(mi,ni) dphi/dxi_ni   (linear_elasticity_routines.F90:1245-1252)
+#pragma unroll
+      for (int mi = 0; mi < 3; ++mi) {
+        double b = 0.0;
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni) b += G.ji[g][mi][ni] * dN[(g * NN + a) * 3 + ni];
+        sB[wv][g][a][mi] = (T)b;
+      }
+    }
+    if (lane < NG) {
+      const int g = lane;
+      const double* mat = material + e * 6;
+      const double E1 = mat[0], E2 = mat[1], E3 = mat[2], v13 = mat[3], v23 = mat[4], v12 = mat[5];
+      const double v31 = v13, v32 = v23, v21 = v12;
+      const double gama = 1.0 / (1.0 - v12 * v21 - v23 * v32 - v31 * v13 - 2.0 * v21 * v32 * v13);
+      const double C11 = E1 * (1.0 - v23 * v32) * gama, C22 = E2 * (1.0 - v13 * v31) * gama;
+      const double C33 = E3 * (1.0 - v12 * v21) * gama;
+      const double C12 = E1 * (v21 + v31 * v23) * gama, C13 = E1 * (v31 + v21 * v32) * gama;
+      const double C23 = E2 * (v32 + v12 * v31) * gama;
+      const double C44 = E2 / (2.0 * (1.0 + v23)), C55 = E1 / (2.0 * (1.0 + v13)), C66 = E3 / (2.0 * (1.0 + v12));
+      const double jr = G.rwg[g];
+      const double diag[9] = {C11, C66, C44, C66, C22, C55, C44, C55, C33};
+      for (int k = 0; k < 9; ++k) sC[wv][g][k] = (T)(jr * diag[k]);
+      const double off[6] = {C12, C66, C13, C44, C23, C55};
+      for (int k = 0; k < 6; ++k) sOff[wv][g][k] = (T)(jr * off[k]);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (int t = lane; t < ND * ND; t += 64) {
+      const int r = t / ND, s = t - (t / ND) * ND;
+      int c1 = r / NN, n = r - (r / NN) * NN;
+      int c2 = s / NN, m = s - (s / NN) * NN;
+      T acc = 0;
+      if (c1 == c2) {
+        for (int g = 0; g < NG; ++g) {
+          const T* D = &sC[wv][g][c1 * 3];
+          acc += D[0] * sB[wv][g][n][0] * sB[wv][g][m][0] + D[1] * sB[wv][g][n][1] * sB[wv][g][m][1] +
+                 D[2] * sB[wv][g][n][2] * sB[wv][g][m][2];
+        }
+      } else {
+        // order the pair so that (c1 < c2); the lower blocks are the transpose
+        if (c1 > c2) {
+          int tc = c1; c1 = c2; c2 = tc;
+          int tn = n; n = m; m = tn;
+        }
+        const int blk = (c1 == 0) ? (c2 == 1 ? 0 : 1) : 2;
+        for (int g = 0; g < NG; ++g) {
+          const T Ca = sOff[wv][g][2 * blk], Cb = sOff[wv][g][2 * blk + 1];
+          acc += Ca * sB[wv][g][n][c1] * sB[wv][g][m][c2] + Cb * sB[wv][g][n][c2] * sB[wv][g][m][c1];
+        }
+      }
+      const int p = pos[e * (ND * ND) + t];
+      add_value(&vals[p], acc, atomic);
+    }
+  }
+}
+
+template <int DIM, int NN, int NG, typename T>
+static int launch_scalar(const mife_element_desc* d, int n_elem, const double* coords, const int32_t* enodes,
+                         const int32_t* edofs, const double* kappa, const double* source, const int32_t* pos,
+                         const int32_t* order, const int32_t* cptr, int ncol, T* vals, double* svec,
+                         hipStream_t s) {
+  constexpr int WPB = (sizeof(T) * NG * DIM * NN > 12000) ? 1 : 4;
+  if (!order) {
+    scalar_assemble_kernel<DIM, NN, NG, T, WPB><<<div_up(n_elem, WPB), 64 * WPB, 0, s>>>(
+        coords, enodes, edofs, kappa, source, pos, nullptr, 0, n_elem, d->dN, d->N, d->w, vals, svec, true);
+  } else {
+    for (int c = 0; c < ncol; ++c) {
+      const int cnt = cptr[c + 1] - cptr[c];
+      if (cnt <= 0) continue;
+      scalar_assemble_kernel<DIM, NN, NG, T, WPB><<<div_up(cnt, WPB), 64 * WPB, 0, s>>>(
+          coords, enodes, edofs, kappa, source, pos, order, cptr[c], cnt, d->dN, d->N, d->w, vals, svec, false);
+    }
+  }
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+template <int NN, int NG, typename T>
+static int launch_elastic(const mife_element_desc* d, int n_elem, const double* coords, const int32_t* enodes,
+                          const double* material, const int32_t* pos, const int32_t* order, const int32_t* cptr,
+                          int ncol, T* vals, hipStream_t s) {
+  constexpr int WPB = 2;
+  if (!order) {
+    elastic_assemble_kernel<NN, NG, T, WPB><<<div_up(n_elem, WPB), 64 * WPB, 0, s>>>(
+        coords, enodes, material, pos, nullptr, 0, n_elem, d->dN, d->w, vals, true);
+  } else {
+    for (int c = 0; c < ncol; ++c) {
+      const int cnt = cptr[c + 1] - cptr[c];
+      if (cnt <= 0) continue;
+      elastic_assemble_kernel<NN, NG, T, WPB><<<div_up(cnt, WPB), 64 * WPB, 0, s>>>(
+          coords, enodes, material, pos, order, cptr[c], cnt, d->dN, d->w, vals, false);
+    }
+  }
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+}  // namespace mife
+
+using namespace mife;
+
+extern "C" int mife_assemble_scalar(const mife_element_desc* d, int n_elem, const double* coords,
+                                    const int32_t* elem_nodes, const int32_t* elem_dofs, const double* kappa,
+                                    const double* source, const int32_t* pos, const int32_t* order,
+                                    const int32_t* colour_ptr_host, int n_colours, int prec, void* vals,
+                                    double* src_vec, void* stream) {
+  MIFE_REQUIRE(d && coords && elem_nodes && elem_dofs && pos && vals && d->dN && d->w, MIFE_ERR_POINTER_IS_NULL,
+               "null pointer argument");
+  MIFE_REQUIRE(!source || d->N, MIFE_ERR_POINTER_IS_NULL, "source given without basis values N");
+  MIFE_REQUIRE(!order || colour_ptr_host, MIFE_ERR_POINTER_IS_NULL, "colour order without colour_ptr");
+  MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
+  if (n_elem == 0) return MIFE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int key = d->dim * 10000 + d->n_nodes * 100 + d->n_gauss;
+#define SCALAR_CASE(DIM, NN, NG)                                                                            \
+  case DIM * 10000 + NN * 100 + NG:                                                                        \
+    if (prec == MIFE_F64)                                                                                   \
+      return launch_scalar<DIM, NN, NG, double>(d, n_elem, coords, elem_nodes, elem_dofs, kappa, source,   \
+                                                pos, order, colour_ptr_host, n_colours, (double*)vals,     \
+                                                src_vec, s);                                               \
+    return launch_scalar<DIM, NN, NG, float>(d, n_elem, coords, elem_nodes, elem_dofs, kappa, source, pos, \
+                                             order, colour_ptr_host, n_colours, (float*)vals, src_vec, s);
+  switch (key) {
+    SCALAR_CASE(2, 4, 4)
+    SCALAR_CASE(2, 9, 9)
+    SCALAR_CASE(3, 8, 8)
+    SCALAR_CASE(3, 27, 27)
+    SCALAR_CASE(3, 8, 27)
+    default:
+      set_error("unsupported scalar element (dim=%d, nodes=%d, gauss=%d)", d->dim, d->n_nodes, d->n_gauss);
+      return MIFE_ERR_ARG;
+  }
+#undef SCALAR_CASE
+}
+
+extern "C" int mife_assemble_elastic(const mife_element_desc* d, int n_elem, const double* coords,
+                                     const int32_t* elem_nodes, const int32_t* elem_dofs, const double* material,
+                                     const int32_t* pos, const int32_t* order, const int32_t* colour_ptr_host,
+                                     int n_colours, int prec, void* vals, void* stream) {
+  (void)elem_dofs;
+  MIFE_REQUIRE(d && coords && elem_nodes && material && pos && vals && d->dN && d->w, MIFE_ERR_POINTER_IS_NULL,
+               "null pointer argument");
+  MIFE_REQUIRE(!order || colour_ptr_host, MIFE_ERR_POINTER_IS_NULL, "colour order without colour_ptr");
+  MIFE_REQUIRE(d->dim == 3, MIFE_ERR_ARG, "linear elasticity kernel is 3-D only");
+  MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
+  if (n_elem == 0) return MIFE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (d->n_nodes == 8 && d->n_gauss == 8) {
+    if (prec == MIFE_F64)
+      return launch_elastic<8, 8, double>(d, n_elem, coords, elem_nodes, material, pos, order, colour_ptr_host,
+                                          n_colours, (double*)vals, s);
+    return launch_elastic<8, 8, float>(d, n_elem, coords, elem_nodes, material, pos, order, colour_ptr_host,
+                                       n_colours, (float*)vals, s);
+  }
+  set_error("unsupported elastic element (nodes=%d, gauss=%d)", d->n_nodes, d->n_gauss);
+  return MIFE_ERR_ARG;
+}

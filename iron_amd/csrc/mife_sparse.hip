@@ -1,0 +1,140 @@
+// CSR sparse operations on the solve path: SpMV, solver-matrix gather,
+// Dirichlet lifting into the solver RHS, field update and back-substitution.
+#include "mife_common.hpp"
+#include "mife_sparse.hpp"
+
+namespace mife {
+
+__global__ void gather_kernel(long long nnz, const int32_t* __restrict__ src_pos, const double* __restrict__ src,
+                              double* __restrict__ dst64, float* __restrict__ dst32) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x) {
+    const double v = src[src_pos[k]];
+    if (dst64) dst64[k] = v;
+    else dst32[k] = (float)v;
+  }
+}
+
+// One wavefront per free row: b[f] = src + rhs - sum over fixed columns of K_ij u_j,
+// skipping |u_j| < ZT or |K_ij| < ZT as SOLVER_MATRICES_STATIC_ASSEMBLE does.
+__global__ void solver_rhs_kernel(int n_free, const int32_t* __restrict__ free_idx,
+                                  const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                                  const double* __restrict__ vals, const int32_t* __restrict__ new_idx,
+                                  const double* __restrict__ u, const double* __restrict__ src,
+                                  const double* __restrict__ rhs, double* __restrict__ b) {
+  const double ZT = 5.0 * 2.220446049250313e-16;
+  const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (f >= n_free) return;
+  const int i = free_idx[f];
+  double lift = 0.0;
+  for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64) {
+    const int j = col[k];
+    if (new_idx[j] < 0) {
+      const double uj = u[j], kv = vals[k];
+      if (fabs(uj) >= ZT && fabs(kv) >= ZT) lift += -1.0 * kv * uj;
+    }
+  }
+  lift = wave_sum(lift);
+  if (lane == 0) {
+    double v = 0.0;
+    if (src) v += 1.0 * src[i];
+    if (rhs) v += rhs[i];
+    b[f] = v + lift;
+  }
+}
+
+__global__ void field_update_kernel(int n_free, const int32_t* __restrict__ free_idx, const double* __restrict__ x,
+                                    double* __restrict__ u) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n_free) u[free_idx[f]] = x[f];
+}
+
+__global__ void backsub_kernel(int n, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                               const double* __restrict__ vals, const double* __restrict__ u,
+                               const double* __restrict__ src, const uint8_t* __restrict__ rhs_fixed,
+                               int iron_compat, double* __restrict__ R) {
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  double acc = 0.0;
+  const bool free_row = !(rhs_fixed && rhs_fixed[i]);
+  if (free_row) {
+    const int k0 = row_ptr[i];
+    for (int k = k0 + lane; k < row_ptr[i + 1]; k += 64) {
+      // equations_set_routines.F90:1806 reads the dependent value at the position
+      // within the row; the correct product reads it at the column.
+      const int j = iron_compat ? (k - k0) : col[k];
+      acc += vals[k] * u[j];
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) R[i] = acc - (src ? src[i] : 0.0);
+}
+
+}  // namespace mife
+
+using namespace mife;
+
+extern "C" {
+
+int mife_spmv(int n_rows, const int32_t* row_ptr, const int32_t* col, const void* vals, const void* x, void* y,
+              int prec, void* stream) {
+  MIFE_REQUIRE(row_ptr && col && vals && x && y, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
+  if (n_rows == 0) return MIFE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  int32_t nnz = 0;
+  MIFE_CHECK_HIP(hipMemcpyAsync(&nnz, row_ptr + n_rows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
+  const int L = spmv_lanes(nnz, n_rows);
+  if (prec == MIFE_F64)
+    launch_spmv<double>(n_rows, row_ptr, col, (const double*)vals, (const double*)x, (double*)y, L, s);
+  else
+    launch_spmv<float>(n_rows, row_ptr, col, (const float*)vals, (const float*)x, (float*)y, L, s);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+int mife_gather_values(int64_t nnz, const int32_t* src_pos, const double* src, void* dst, int dst_prec,
+                       void* stream) {
+  MIFE_REQUIRE(src_pos && src && dst, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  if (nnz == 0) return MIFE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int grid = (int)std::min<long long>(div_up(nnz, 256), 8192);
+  gather_kernel<<<grid, 256, 0, s>>>(nnz, src_pos, src, dst_prec == MIFE_F64 ? (double*)dst : nullptr,
+                                     dst_prec == MIFE_F64 ? nullptr : (float*)dst);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr, const int32_t* col,
+                    const double* vals, const int32_t* new_idx, const double* u, const double* src,
+                    const double* rhs, double* b, void* stream) {
+  MIFE_REQUIRE(free_idx && row_ptr && col && vals && new_idx && u && b, MIFE_ERR_POINTER_IS_NULL,
+               "null pointer argument");
+  if (n_free == 0) return MIFE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  solver_rhs_kernel<<<div_up(n_free, 4), 256, 0, s>>>(n_free, free_idx, row_ptr, col, vals, new_idx, u, src, rhs, b);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+int mife_field_update(int n_free, const int32_t* free_idx, const double* x, double* u, void* stream) {
+  MIFE_REQUIRE(free_idx && x && u, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  if (n_free == 0) return MIFE_OK;
+  field_update_kernel<<<div_up(n_free, 256), 256, 0, (hipStream_t)stream>>>(n_free, free_idx, x, u);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+int mife_backsubstitute(int n, const int32_t* row_ptr, const int32_t* col, const double* vals, const double* u,
+                        const double* src, const uint8_t* rhs_fixed, int iron_compat, double* R, void* stream) {
+  MIFE_REQUIRE(row_ptr && col && vals && u && R, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  if (n == 0) return MIFE_OK;
+  backsub_kernel<<<div_up(n, 4), 256, 0, (hipStream_t)stream>>>(n, row_ptr, col, vals, u, src, rhs_fixed,
+                                                                  iron_compat, R);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+"""Device-resident finite-element linear systems on top of libmi_fe.so.
+
+This is the engine behind the iron-compatible API (iron_amd.iron): it holds the
+mesh, the CSR structure, the scatter map and the Dirichlet reduction on the GPU
+and drives the HIP kernels through the C ABI.  PyTorch only provides device
+memory and the stream.
+
+Pipeline of a static linear solve (reference call stack in parentheses):
+  assemble()      element integrals + coloured scatter-add into the equations CSR
+                  (EquationsSet_AssembleStaticLinearFEM -> ElementAdd)
+  reduce.gather() K_ff copy into the solver matrix (SOLVER_MATRIX_EQUATIONS_MATRIX_ADD)
+  reduce.rhs()    b_f = s_f + f_f - K_fd u_d (SOLVER_MATRICES_STATIC_ASSEMBLE RHS)
+  solve_*()       Jacobi CG / mixed refinement / GMRES (Petsc_KSPSolve)
+  field_update()  u_f = x (SOLVER_VARIABLES_FIELD_UPDATE)
+  backsub()       reactions on the RHS variable (EQUATIONS_SET_BACKSUBSTITUTE)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ElementDesc, call, ptr
+from .basis import LagrangeBasis
+
+F64, F32 = _lib.MIFE_F64, _lib.MIFE_F32
+
+
+def _dev(device=None):
+    if not torch.cuda.is_available():
+        raise RuntimeError("iron_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch.device(device if device is not None else "cuda:0")
+
+
+def _stream():
+    return _lib.stream_ptr()
+
+
+class DeviceBasis:
+    """Basis tables uploaded once: dN [ng][nn][dim], N [ng][nn], w [ng]."""
+
+    def __init__(self, b: LagrangeBasis, device):
+        self.basis = b
+        self.dN = torch.as_tensor(np.ascontiguousarray(b.dN), dtype=torch.float64, device=device)
+        self.N = torch.as_tensor(np.ascontiguousarray(b.N), dtype=torch.float64, device=device)
+        self.w = torch.as_tensor(np.ascontiguousarray(b.weights), dtype=torch.float64, device=device)
+        self.desc = ElementDesc(b.num_xi, b.num_nodes, b.num_gauss, ptr(self.dN), ptr(self.N), ptr(self.w))
+
+
+class CSRStructure:
+    """FEM sparsity of an equations matrix plus the per-element scatter map and
+    the element colouring used for race-free, deterministic scatter-adds."""
+
+    def __init__(self, elem_dofs: np.ndarray, n_dofs: int, device=None, colours: np.ndarray | None = None):
+        dev = _dev(device)
+        self.device = dev
+        self.n = int(n_dofs)
+        ed = np.ascontiguousarray(elem_dofs, dtype=np.int32)
+        self.n_elem, self.n_edof = ed.shape
+        self.elem_dofs = torch.as_tensor(ed, device=dev)
+        s = _stream()
+        L = _lib.lib()
+        wbytes = L.mife_csr_pattern_work_bytes(self.n_elem, self.n_edof, self.n)
+        work = torch.empty(int(wbytes), dtype=torch.uint8, device=dev)
+        self.row_ptr = torch.empty(self.n + 1, dtype=torch.int32, device=dev)
+        nnz = C.c_int64(0)
+        call("mife_csr_pattern", ptr(self.elem_dofs), self.n_elem, self.n_edof, self.n, ptr(self.row_ptr), None,
+             C.byref(nnz), ptr(work), s)
+        self.nnz = int(nnz.value)
+        self.col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        call("mife_csr_pattern", ptr(self.elem_dofs), self.n_elem, self.n_edof, self.n, ptr(self.row_ptr),
+             ptr(self.col), None, ptr(work), s)
+        del work
+        self.pos = torch.empty(self.n_elem * self.n_edof * self.n_edof, dtype=torch.int32, device=dev)
+        call("mife_scatter_map", ptr(self.elem_dofs), self.n_elem, self.n_edof, ptr(self.row_ptr), ptr(self.col),
+             ptr(self.pos), s)
+        if colours is None:
+            col_h = np.empty(self.n_elem, dtype=np.int32)
+            nc = C.c_int(0)
+            call("mife_colour_elements", ed.ctypes.data, self.n_elem, self.n_edof, self.n, col_h.ctypes.data,
+                 C.byref(nc))
+            colours = col_h
+        colours = np.asarray(colours, dtype=np.int32)
+        order = np.argsort(colours, kind="stable").astype(np.int32)
+        counts = np.bincount(colours, minlength=int(colours.max()) + 1)
+        self.colour_ptr = np.zeros(len(counts) + 1, dtype=np.int32)
+        self.colour_ptr[1:] = np.cumsum(counts)
+        self.n_colours = len(counts)
+        self.order = torch.as_tensor(order, device=dev)
+
+    def zeros(self, prec=F64):
+        return torch.zeros(max(self.nnz, 1), dtype=torch.float64 if prec == F64 else torch.float32, device=self.device)
+
+    def to_host(self):
+        """(row_ptr, col) as int64 numpy arrays."""
+        return self.row_ptr.cpu().numpy().astype(np.int64), self.col[: self.nnz].cpu().numpy().astype(np.int64)
+
+
+class DirichletReduction:
+    """Free-DOF solver system: rows/columns of free DOFs in ascending order."""
+
+    def __init__(self, csr: CSRStructure, fixed_mask: np.ndarray):
+        dev = csr.device
+        self.csr = csr
+        fixed = np.asarray(fixed_mask, dtype=bool)
+        free = np.nonzero(~fixed)[0].astype(np.int32)
+        new_idx = np.full(csr.n, -1, dtype=np.int32)
+        new_idx[free] = np.arange(len(free), dtype=np.int32)
+        self.n_free = len(free)
+        self.fixed_mask = fixed
+        self.free_idx = torch.as_tensor(free, device=dev)
+        self.new_idx = torch.as_tensor(new_idx, device=dev)
+        s = _stream()
+        wbytes = _lib.lib().mife_csr_pattern_work_bytes(1, 1, csr.n)
+        work = torch.empty(int(wbytes), dtype=torch.uint8, device=dev)
+        self.row_ptr = torch.empty(self.n_free + 1, dtype=torch.int32, device=dev)
+        nnz = C.c_int64(0)
+        call("mife_reduce_pattern", csr.n, ptr(csr.row_ptr), ptr(csr.col), ptr(self.new_idx), self.n_free,
+             ptr(self.row_ptr), None, None, C.byref(nnz), ptr(work), s)
+        self.nnz = int(nnz.value)
+        self.col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        self.src_pos = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        call("mife_reduce_pattern", csr.n, ptr(csr.row_ptr), ptr(csr.col), ptr(self.new_idx), self.n_free,
+             ptr(self.row_ptr), ptr(self.col), ptr(self.src_pos), None, ptr(work), s)
+
+    def gather(self, vals64: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
+        if out is None:
+            out = torch.empty(max(self.nnz, 1), dtype=torch.float64 if prec == F64 else torch.float32,
+                              device=vals64.device)
+        call("mife_gather_values", self.nnz, ptr(self.src_pos), ptr(vals64), ptr(out), prec, _stream())
+        return out
+
+    def rhs(self, vals64, u, src=None, rhs=None, out=None) -> torch.Tensor:
+        if out is None:
+            out = torch.empty(max(self.n_free, 1), dtype=torch.float64, device=vals64.device)
+        call("mife_solver_rhs", self.n_free, ptr(self.free_idx), ptr(self.csr.row_ptr), ptr(self.csr.col),
+             ptr(vals64), ptr(self.new_idx), ptr(u), ptr(src), ptr(rhs), ptr(out), _stream())
+        return out
+
+    def field_update(self, x, u):
+        call("mife_field_update", self.n_free, ptr(self.free_idx), ptr(x), ptr(u), _stream())
+
+
+@dataclass
+class SolveInfo:
+    iterations: int
+    relres: float
+    outer: int = 0
+
+
+class KrylovWorkspace:
+    def __init__(self, n: int, device, restart: int = 30):
+        self.n = n
+        self.restart = restart
+        self.buf = torch.empty(int(_lib.lib().mife_solver_work_bytes(n, restart)), dtype=torch.uint8, device=device)
+
+
+def cg(n, row_ptr, col, vals, b, x, rtol, maxit, ws: KrylovWorkspace, prec=F64) -> SolveInfo:
+    it, rr = C.c_int(0), C.c_double(0.0)
+    call("mife_cg", n, ptr(row_ptr), ptr(col), ptr(vals), ptr(b), ptr(x), prec, float(rtol), int(maxit),
+         ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
+    return SolveInfo(it.value, rr.value)
+
+
+def refine(n, row_ptr, col, vals64, vals32, b, x, rtol, maxit, ws: KrylovWorkspace, inner_rtol=1e-4) -> SolveInfo:
+    it, outer, rr = C.c_int(0), C.c_int(0), C.c_double(0.0)
+    call("mife_refine", n, ptr(row_ptr), ptr(col), ptr(vals64), ptr(vals32), ptr(b), ptr(x), float(rtol),
+         float(inner_rtol), int(maxit), ptr(ws.buf), C.byref(it), C.byref(outer), C.byref(rr), _stream())
+    return SolveInfo(it.value, rr.value, outer.value)
+
+
+def gmres(n, row_ptr, col, vals, b, x, rtol, maxit, ws: KrylovWorkspace, restart=30) -> SolveInfo:
+    it, rr = C.c_int(0), C.c_double(0.0)
+    call("mife_gmres", n, ptr(row_ptr), ptr(col), ptr(vals), ptr(b), ptr(x), float(rtol), int(restart),
+         int(maxit), ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
+    return SolveInfo(it.value, rr.value)
+
+
+def spmv(row_ptr, col, vals, x, y, prec=F64):
+    call("mife_spmv", len(row_ptr) - 1, ptr(row_ptr), ptr(col), ptr(vals), ptr(x), ptr(y), prec, _stream())
+    return y
+
+
+def backsubstitute(csr: CSRStructure, vals64, u, src=None, rhs_fixed=None, iron_compat=False, out=None):
+    if out is None:
+        out = torch.empty(csr.n, dtype=torch.float64, device=vals64.device)
+    call("mife_backsubstitute", csr.n, ptr(csr.row_ptr), ptr(csr.col), ptr(vals64), ptr(u), ptr(src),
+         ptr(rhs_fixed), 1 if iron_compat else 0, ptr(out), _stream())
+    return out
+
+
+class ScalarAssembler:
+    """Laplace / generalised Laplace / constant-source Poisson on one basis."""
+
+    def __init__(self, basis: LagrangeBasis, coords: np.ndarray, elem_nodes: np.ndarray, csr: CSRStructure):
+        dev = csr.device
+        self.csr = csr
+        self.db = DeviceBasis(basis, dev)
+        self.coords = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64), device=dev)
+        self.elem_nodes = torch.as_tensor(np.ascontiguousarray(elem_nodes, dtype=np.int32), device=dev)
+        self.n_elem = elem_nodes.shape[0]
+
+    def assemble(self, vals, kappa=None, source=None, src_vec=None, prec=F64, atomic=False):
+        cs = self.csr
+        call("mife_assemble_scalar", C.byref(self.db.desc), self.n_elem, ptr(self.coords), ptr(self.elem_nodes),
+             ptr(cs.elem_dofs), ptr(kappa), ptr(source), ptr(cs.pos), None if atomic else ptr(cs.order),
+             None if atomic else cs.colour_ptr.ctypes.data, cs.n_colours, prec, ptr(vals), ptr(src_vec), _stream())
+        return vals
+
+
+class ElasticAssembler:
+    """3-D linear elasticity on one basis (all three components on it)."""
+
+    def __init__(self, basis: LagrangeBasis, coords: np.ndarray, elem_nodes: np.ndarray, csr: CSRStructure):
+        dev = csr.device
+        self.csr = csr
+        self.db = DeviceBasis(basis, dev)
+        self.coords = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64), device=dev)
+        self.elem_nodes = torch.as_tensor(np.ascontiguousarray(elem_nodes, dtype=np.int32), device=dev)
+        self.n_elem = elem_nodes.shape[0]
+
+    def assemble(self, vals, material, prec=F64, atomic=False):
+        cs = self.csr
+        call("mife_assemble_elastic", C.byref(self.db.desc), self.n_elem, ptr(self.coords), ptr(self.elem_nodes),
+             ptr(cs.elem_dofs), ptr(material), ptr(cs.pos), None if atomic else ptr(cs.order),
+             None if atomic else cs.colour_ptr.ctypes.data, cs.n_colours, prec, ptr(vals), _stream())
+        return vals

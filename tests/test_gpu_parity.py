@@ -1,0 +1,232 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Tolerances: CSR pattern, DOF maps and scatter positions bit-exact; FP64
+assembly within 1e-13 of the largest entry (summation order differs); FP32
+assembly within 2e-6 (FP32 contraction on FP64 geometry); solutions within the
+stated relative residual of the FP64 system.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import fe_oracle as fo  # noqa: E402
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _fem():
+    from iron_amd import fem, mesh, basis  # noqa: F401
+    return fem, mesh, basis
+
+
+def _build(ne, order, ncomp=1, extent=None, gauss=None):
+    fem, mesh, basis = _fem()
+    d = len(ne)
+    extent = extent or (1.0,) * d
+    gx = gauss or order + 1
+    pb = basis.basis(d, {1: 1, 2: 2}[order], gx)
+    pm = mesh.RegularMesh(ne, [pb], extent)
+    pd = mesh.DofMap(pm, [0] * ncomp)
+    ob = fo.Basis(nxi=d, order=order, ngauss_xi=gx)
+    om = fo.regular_mesh(ne, [ob], extent)
+    od = fo.DofMap(om, [0] * ncomp)
+    return pb, pm, pd, ob, om, od
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("ne,order,ncomp", [((3, 2, 2), 1, 1), ((2, 3, 2), 2, 1), ((3, 2, 2), 1, 3), ((4, 3), 1, 1),
+                                           ((3, 2), 2, 1)])
+def test_pattern_and_dofs_bit_exact(ne, order, ncomp):
+    fem, _, _ = _fem()
+    pb, pm, pd, ob, om, od = _build(ne, order, ncomp)
+    assert np.array_equal(pd.element_dofs(), od.element_dofs())
+    csr = fem.CSRStructure(pd.element_dofs(), pd.num_dofs)
+    rp, col = csr.to_host()
+    orp, ocol = fo.csr_pattern(od)
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    # scatter map: position of (row i, col j) of every element entry
+    ed = od.element_dofs()
+    pos = csr.pos.cpu().numpy().reshape(len(ed), ed.shape[1], ed.shape[1])
+    for e in [0, len(ed) // 2, len(ed) - 1]:
+        for i in range(ed.shape[1]):
+            for j in range(ed.shape[1]):
+                assert pos[e, i, j] == fo.storage_location_find(orp, ocol, ed[e, i], ed[e, j])
+    # colouring: no DOF shared inside a colour
+    order_ = csr.order.cpu().numpy()
+    for c in range(csr.n_colours):
+        els = order_[csr.colour_ptr[c]:csr.colour_ptr[c + 1]]
+        dofs = pd.element_dofs()[els].ravel()
+        assert len(np.unique(dofs)) == len(dofs)
+
+
+@pytest.mark.parametrize("ne,order", [((4, 3, 3), 1), ((3, 2, 3), 2), ((5, 4), 1), ((3, 3), 2)])
+def test_scalar_assembly_fp64_fp32(ne, order):
+    fem, _, _ = _fem()
+    rng = np.random.default_rng(7)
+    pb, pm, pd, ob, om, od = _build(ne, order, extent=tuple(1.0 + 0.5 * k for k in range(len(ne))))
+    kappa = np.exp(rng.standard_normal(pm.num_elements))
+    csr = fem.CSRStructure(pd.element_dofs(), pd.num_dofs)
+    asm = fem.ScalarAssembler(pb, pm.coords, pm.elem_nodes[0], csr)
+    kt = torch.as_tensor(kappa, device="cuda")
+    ft = torch.full((pm.num_elements,), 1.0, dtype=torch.float64, device="cuda")
+    v64 = csr.zeros(fem.F64)
+    s64 = torch.zeros(pd.num_dofs, dtype=torch.float64, device="cuda")
+    asm.assemble(v64, kappa=kt, source=ft, src_vec=s64, prec=fem.F64)
+    v32 = csr.zeros(fem.F32)
+    asm.assemble(v32, kappa=kt, prec=fem.F32)
+    rp, col = fo.csr_pattern(od)
+    xe = om.coords[om.elem_nodes[0] - 1]
+    ref = fo.assemble_csr(rp, col, od.element_dofs(), fo.laplace_element_matrices(xe, ob, kappa), od.num_dofs)
+    sref = fo.assemble_vector(od.element_dofs(), fo.poisson_source_vectors(xe, ob, 1.0), od.num_dofs)
+    assert _rel(v64.cpu().numpy()[: len(ref)], ref) < 1e-13
+    assert _rel(s64.cpu().numpy(), sref) < 1e-13
+    assert _rel(v32.cpu().numpy()[: len(ref)].astype(np.float64), ref) < 2e-6
+    # determinism of the coloured scatter and agreement with the atomic path
+    v64b = csr.zeros(fem.F64)
+    asm.assemble(v64b, kappa=kt, prec=fem.F64)
+    assert torch.equal(v64, v64b)
+    va = csr.zeros(fem.F64)
+    asm.assemble(va, kappa=kt, prec=fem.F64, atomic=True)
+    assert _rel(va.cpu().numpy()[: len(ref)], ref) < 1e-13
+
+
+def test_elastic_assembly_fp64_fp32():
+    fem, _, _ = _fem()
+    rng = np.random.default_rng(3)
+    pb, pm, pd, ob, om, od = _build((4, 2, 3), 1, ncomp=3, extent=(3.0, 1.0, 1.5))
+    E = 10.0 * np.exp(0.3 * rng.standard_normal(pm.num_elements))
+    mat = np.stack([E, E, E, 0.3 * np.ones_like(E), 0.3 * np.ones_like(E), 0.3 * np.ones_like(E)], axis=1)
+    csr = fem.CSRStructure(pd.element_dofs(), pd.num_dofs)
+    asm = fem.ElasticAssembler(pb, pm.coords, pm.elem_nodes[0], csr)
+    mt = torch.as_tensor(np.ascontiguousarray(mat), device="cuda")
+    v64 = csr.zeros(fem.F64)
+    asm.assemble(v64, mt, prec=fem.F64)
+    v32 = csr.zeros(fem.F32)
+    asm.assemble(v32, mt, prec=fem.F32)
+    rp, col = fo.csr_pattern(od)
+    xe = om.coords[om.elem_nodes[0] - 1]
+    C = fo.linear_elasticity_tensor(*mat.T)
+    ref = fo.assemble_csr(rp, col, od.element_dofs(), fo.linear_elasticity_element_matrices(xe, ob, C), od.num_dofs)
+    assert _rel(v64.cpu().numpy()[: len(ref)], ref) < 1e-13
+    assert _rel(v32.cpu().numpy()[: len(ref)].astype(np.float64), ref) < 2e-6
+
+
+def _poisson_system(ne, order, seed=11):
+    """Lognormal-kappa Poisson with f=1 and u=0 on every boundary face (oracle)."""
+    rng = np.random.default_rng(seed)
+    pb, pm, pd, ob, om, od = _build(ne, order)
+    kappa = np.exp(rng.standard_normal(pm.num_elements))
+    X = om.coords
+    fixed = np.any((np.abs(X) < 1e-12) | (np.abs(X - 1.0) < 1e-12), axis=1)
+    return pb, pm, pd, ob, om, od, kappa, fixed
+
+
+def _gpu_linear(pb, pm, pd, kappa, fixed, ufix, source=1.0):
+    fem, _, _ = _fem()
+    csr = fem.CSRStructure(pd.element_dofs(), pd.num_dofs)
+    asm = fem.ScalarAssembler(pb, pm.coords, pm.elem_nodes[0], csr)
+    kt = torch.as_tensor(kappa, device="cuda")
+    ft = torch.full((pm.num_elements,), float(source), dtype=torch.float64, device="cuda")
+    v64 = csr.zeros(fem.F64)
+    src = torch.zeros(pd.num_dofs, dtype=torch.float64, device="cuda")
+    asm.assemble(v64, kappa=kt, source=ft if source else None, src_vec=src if source else None, prec=fem.F64)
+    v32 = csr.zeros(fem.F32)
+    asm.assemble(v32, kappa=kt, prec=fem.F32)
+    red = fem.DirichletReduction(csr, fixed)
+    u = torch.as_tensor(np.where(fixed, ufix, 0.0), device="cuda")
+    b = red.rhs(v64, u, src=src if source else None)
+    return fem, csr, red, v64, v32, src, u, b
+
+
+def test_reduction_rhs_and_cg_fp64_against_oracle():
+    pb, pm, pd, ob, om, od, kappa, fixed = _poisson_system((4, 3, 3), 2)
+    ufix = np.zeros(pd.num_dofs)
+    fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, kappa, fixed, ufix)
+    rp, col = fo.csr_pattern(od)
+    xe = om.coords[om.elem_nodes[0] - 1]
+    vals = fo.assemble_csr(rp, col, od.element_dofs(), fo.laplace_element_matrices(xe, ob, kappa), od.num_dofs)
+    sref = fo.assemble_vector(od.element_dofs(), fo.poisson_source_vectors(xe, ob, 1.0), od.num_dofs)
+    free, srp, scol, sv, bref = fo.reduce_dirichlet(rp, col, vals, fixed, ufix, sref)
+    assert np.array_equal(red.row_ptr.cpu().numpy(), srp) and np.array_equal(red.col.cpu().numpy()[: len(scol)], scol)
+    sv_gpu = red.gather(v64, fem.F64)
+    assert _rel(sv_gpu.cpu().numpy()[: len(sv)], sv) < 1e-13
+    assert _rel(b.cpu().numpy(), bref) < 1e-13
+    ws = fem.KrylovWorkspace(red.n_free, "cuda")
+    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    info = fem.cg(red.n_free, red.row_ptr, red.col, sv_gpu, b, x, 1e-10, 10000, ws, prec=fem.F64)
+    assert info.relres <= 1e-10
+    xo, _, _ = fo.cg_jacobi(srp, scol, sv, bref, rtol=1e-13)
+    assert _rel(x.cpu().numpy(), xo) < 1e-8
+    # mixed precision: FP32 inner CG, FP64 refinement to 1e-8 of the FP64 system
+    sv32 = red.gather(v64, fem.F32)
+    xm = torch.zeros_like(x)
+    info_m = fem.refine(red.n_free, red.row_ptr, red.col, sv_gpu, sv32, b, xm, 1e-8, 10000, ws)
+    A = fo._csr_scipy(srp, scol, sv)
+    assert np.linalg.norm(bref - A @ xm.cpu().numpy()) / np.linalg.norm(bref) <= 1e-8
+    assert info_m.relres <= 1e-8 and info_m.outer >= 1
+    # GMRES(30) + Jacobi
+    xg = torch.zeros_like(x)
+    info_g = fem.gmres(red.n_free, red.row_ptr, red.col, sv_gpu, b, xg, 1e-10, 20000, ws)
+    assert info_g.relres <= 1e-10
+    assert _rel(xg.cpu().numpy(), xo) < 1e-7
+
+
+def test_unit_cube_laplace_u_equals_x_and_reactions():
+    fem, _, _ = _fem()
+    pb, pm, pd, ob, om, od = _build((6, 5, 4), 1)
+    X = pm.coords
+    fixed = (np.abs(X[:, 0]) < 1e-12) | (np.abs(X[:, 0] - 1.0) < 1e-12)
+    ufix = np.where(np.abs(X[:, 0] - 1.0) < 1e-12, 1.0, 0.0)
+    fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, np.ones(pm.num_elements), fixed, ufix, source=0.0)
+    sv = red.gather(v64, fem.F64)
+    ws = fem.KrylovWorkspace(red.n_free, "cuda")
+    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    info = fem.cg(red.n_free, red.row_ptr, red.col, sv, b, x, 1e-10, 10000, ws)
+    red.field_update(x, u)
+    un = u.cpu().numpy()
+    assert np.max(np.abs(un - X[:, 0])) < 1e-8 and info.relres <= 1e-10
+    R = fem.backsubstitute(csr, v64, u).cpu().numpy()
+    assert abs(R[np.abs(X[:, 0]) < 1e-12].sum() + 1.0) < 1e-8
+    assert abs(R[np.abs(X[:, 0] - 1) < 1e-12].sum() - 1.0) < 1e-8
+
+
+def test_golden_analytic_laplace_through_gpu():
+    """The reference's AnalyticLaplaceLinearLagrange.opanal reproduced by the HIP path."""
+    fem, mesh, basis = _fem()
+    data = json.loads((GOLDEN / "analytic_laplace_linear_lagrange.json").read_text())
+    o = -data["setup"]["origin_pi"] / 2.0
+    pb = basis.basis(2, 1, 2)
+    pm = mesh.RegularMesh((2, 2), [pb], (2.0, 2.0), (o, o))
+    pd = mesh.DofMap(pm, [0])
+    X = pm.coords
+    ua = np.cos(X[:, 0]) * np.cosh(X[:, 1])
+    fixed = np.ones(9, bool)
+    fixed[4] = False
+    fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, np.ones(4), fixed, ua, source=0.0)
+    sv = red.gather(v64, fem.F64)
+    ws = fem.KrylovWorkspace(red.n_free, "cuda")
+    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    fem.cg(red.n_free, red.row_ptr, red.col, sv, b, x, 1e-14, 100, ws)
+    red.field_update(x, u)
+    un = u.cpu().numpy()
+    for node, (num, _) in data["nodal"]["Phi"].items():
+        assert float(f"{un[int(node) - 1]:.4e}") == pytest.approx(num, rel=1e-4, abs=1e-15)
+    R = fem.backsubstitute(csr, v64, u, iron_compat=True).cpu().numpy()
+    for node, (num, _) in data["nodal"]["del Phi/del n"].items():
+        i = int(node) - 1
+        if abs(num) < 1e-10:
+            assert abs(R[i]) < 1e-12
+        else:
+            assert R[i] == pytest.approx(num, rel=1e-4)
+    rp, col = fo.csr_pattern(fo.DofMap(fo.regular_mesh((2, 2), [fo.Basis(2, 1, 2)], (2.0, 2.0), (o, o)), [0]))
+    vals = v64.cpu().numpy()[: len(col)]
+    Rt = fem.backsubstitute(csr, v64, u, iron_compat=False).cpu().numpy()
+    assert np.allclose(Rt, fo.backsubstitute(rp, col, vals, un), atol=1e-13)
