@@ -1,0 +1,288 @@
+#!/usr/bin/env python
+"""Benchmark of iron's element-assembly + sparse-solve hot path on one MI355X.
+
+Workload (default ``diffusion48``): 3-D Poisson-type diffusion on a 48x48x48
+triquadratic Lagrange hex mesh (110,592 elements, 97^3 nodes, 3x3x3 Gauss),
+per-element conductivity lognormal(0,1) from a fixed seed, source f = 1, u = 0 on
+every boundary face.  One step is one full pass of the path:
+  FP32 (served inner operator) and FP64 (outer operator) element integration with
+  coloured scatter into the CSR -> K_ff gather + RHS -> FP32 Jacobi CG inside FP64
+  iterative refinement to ||b - A x|| <= 1e-8 ||b|| -> field update.
+The plain FP64 drop-in (FP64 assembly + FP64 Jacobi CG to 1e-8) is timed in the
+same run, and the C restatement of iron's CPU algorithm on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16|elastic]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="diffusion48", choices=["diffusion48", "laplace16", "elastic"])
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    return ap.parse_args()
+
+
+def build_problem(args):
+    import torch
+    from iron_amd import basis, fem, mesh
+    if args.workload == "diffusion48":
+        ne, interp, extent, rtol = (48, 48, 48), basis.QUADRATIC_LAGRANGE, (1.0, 1.0, 1.0), 1e-8
+    elif args.workload == "laplace16":
+        ne, interp, extent, rtol = (16, 16, 16), basis.LINEAR_LAGRANGE, (1.0, 1.0, 1.0), 1e-10
+    else:
+        raise SystemExit("elastic workload is run by tests/bench of the elasticity path")
+    b = basis.basis(3, interp)
+    m = mesh.RegularMesh(ne, [b], extent)
+    dm = mesh.DofMap(m, [0])
+    dev = torch.device("cuda:0")
+    E = m.num_elements
+    if args.workload == "diffusion48":
+        g = torch.Generator().manual_seed(args.seed)
+        kappa = torch.randn(E, dtype=torch.float64, generator=g).exp()
+        X = m.coords
+        fixed = np.any((np.abs(X) < 1e-12) | (np.abs(X - 1.0) < 1e-12), axis=1)
+        ufix = np.zeros(dm.num_dofs)
+        source = torch.ones(E, dtype=torch.float64)
+    else:
+        kappa = torch.ones(E, dtype=torch.float64)
+        X = m.coords
+        fixed = (np.abs(X[:, 0]) < 1e-12) | (np.abs(X[:, 0] - 1.0) < 1e-12)
+        ufix = np.where(np.abs(X[:, 0] - 1.0) < 1e-12, 1.0, 0.0)
+        source = None
+    csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, dev, colours=mesh.structured_colours(m))
+    asm = fem.ScalarAssembler(b, m.coords, m.elem_nodes[0], csr)
+    red = fem.DirichletReduction(csr, fixed)
+    P = dict(m=m, dm=dm, b=b, csr=csr, asm=asm, red=red, rtol=rtol, fixed=fixed, ufix=ufix,
+             kappa=kappa.to(dev), source=None if source is None else source.to(dev), kappa_h=kappa.numpy(),
+             ws=fem.KrylovWorkspace(red.n_free, dev))
+    P["v64"] = csr.zeros(fem.F64)
+    P["v32"] = csr.zeros(fem.F32)
+    P["src"] = torch.zeros(dm.num_dofs, dtype=torch.float64, device=dev)
+    P["u"] = torch.as_tensor(np.where(fixed, ufix, 0.0), device=dev)
+    P["sv64"] = torch.empty(max(red.nnz, 1), dtype=torch.float64, device=dev)
+    P["sv32"] = torch.empty(max(red.nnz, 1), dtype=torch.float32, device=dev)
+    P["b"] = torch.empty(red.n_free, dtype=torch.float64, device=dev)
+    P["x"] = torch.zeros(red.n_free, dtype=torch.float64, device=dev)
+    return P
+
+
+def step(P, mixed: bool, inner_rtol: float):
+    from iron_amd import fem
+    v64, v32, src = P["v64"], P["v32"], P["src"]
+    v64.zero_()
+    src.zero_()
+    P["asm"].assemble(v64, kappa=P["kappa"], source=P["source"], src_vec=src if P["source"] is not None else None,
+                      prec=fem.F64)
+    red = P["red"]
+    red.gather(v64, fem.F64, out=P["sv64"])
+    red.rhs(v64, P["u"], src=src if P["source"] is not None else None, out=P["b"])
+    x = P["x"]
+    x.zero_()
+    if mixed:
+        v32.zero_()
+        P["asm"].assemble(v32, kappa=P["kappa"], prec=fem.F32)
+        red.gather(v32, fem.F32, out=P["sv32"])
+        info = fem.refine(red.n_free, red.row_ptr, red.col, P["sv64"], P["sv32"], P["b"], x, P["rtol"], 200000,
+                          P["ws"], inner_rtol=inner_rtol)
+    else:
+        info = fem.cg(red.n_free, red.row_ptr, red.col, P["sv64"], P["b"], x, P["rtol"], 200000, P["ws"],
+                      prec=fem.F64)
+    red.field_update(x, P["u"])
+    return info
+
+
+def timed_steps(P, k, mixed, inner_rtol):
+    import torch
+    lat, infos = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        t = time.perf_counter()
+        infos.append(step(P, mixed, inner_rtol))
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, lat, infos
+
+
+def cpu_baseline(P, fp64_iters, budget_s):
+    """Times the C restatement of iron's algorithm (oracle/liboracle_fe.so) on one
+    core over a bounded sample and scales it to one full pass of the workload."""
+    import ctypes as C
+    import torch
+    sys.path.insert(0, str(ROOT))
+    so = ROOT / "oracle" / "liboracle_fe.so"
+    if not so.exists():
+        return None
+    lib = C.CDLL(str(so))
+    m, b, csr, red = P["m"], P["b"], P["csr"], P["red"]
+    E = m.num_elements
+    rp = csr.row_ptr.cpu().numpy().astype(np.int32)
+    col = csr.col[: csr.nnz].cpu().numpy().astype(np.int32)
+    ed = P["dm"].element_dofs()
+    en = np.ascontiguousarray(m.elem_nodes[0], dtype=np.int32)
+    coords = np.ascontiguousarray(m.coords, dtype=np.float64)
+    dN = np.ascontiguousarray(b.dN, dtype=np.float64)
+    N = np.ascontiguousarray(b.N, dtype=np.float64)
+    w = np.ascontiguousarray(b.weights, dtype=np.float64)
+    kap = np.ascontiguousarray(P["kappa_h"], dtype=np.float64)
+    f = np.ones(E) if P["source"] is not None else None
+    vals = np.zeros(csr.nnz)
+    src = np.zeros(csr.n)
+    P_ = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    lib.oracle_assemble_scalar3.restype = C.c_int
+    # assembly sample: elements [0, Es) chosen to fit ~40% of the budget
+    t = time.perf_counter()
+    probe = min(E, 512)
+    lib.oracle_assemble_scalar3(E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed),
+                                P_(kap), P_(f), P_(rp), P_(col), P_(vals), P_(src), 0, probe)
+    per_elem = (time.perf_counter() - t) / probe
+    Es = int(min(E, max(probe, 0.4 * budget_s / max(per_elem, 1e-9))))
+    vals[:] = 0.0
+    src[:] = 0.0
+    t = time.perf_counter()
+    lib.oracle_assemble_scalar3(E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed),
+                                P_(kap), P_(f), P_(rp), P_(col), P_(vals), P_(src), 0, Es)
+    t_asm = (time.perf_counter() - t) * (E / Es)
+    if Es < E:  # complete the matrix (untimed) so the solve runs on the real system
+        lib.oracle_assemble_scalar3(E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed),
+                                    P_(kap), P_(f), P_(rp), P_(col), P_(vals), P_(src), Es, E)
+    new_idx = red.new_idx.cpu().numpy().astype(np.int32)
+    u = P["u"].cpu().numpy()
+    srp = np.zeros(red.n_free + 1, dtype=np.int32)
+    scol = np.zeros(max(red.nnz, 1), dtype=np.int32)
+    sv = np.zeros(max(red.nnz, 1))
+    bb = np.zeros(red.n_free)
+    lib.oracle_reduce.restype = C.c_int64
+    t = time.perf_counter()
+    lib.oracle_reduce(csr.n, P_(rp), P_(col), P_(vals), P_(new_idx), P_(u), P_(src if f is not None else None),
+                      P_(srp), P_(scol), P_(sv), P_(bb))
+    t_red = time.perf_counter() - t
+    x = np.zeros(red.n_free)
+    rr = C.c_double(0.0)
+    lib.oracle_cg.restype = C.c_int
+    lib.oracle_cg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                              C.c_int, C.c_int, C.POINTER(C.c_double)]
+    t = time.perf_counter()
+    it_probe = 5
+    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, it_probe, 1, C.byref(rr))
+    per_it = (time.perf_counter() - t) / it_probe
+    n_it = int(max(it_probe, min(fp64_iters, 0.5 * budget_s / max(per_it, 1e-9))))
+    x[:] = 0.0
+    t = time.perf_counter()
+    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, n_it, 1, C.byref(rr))
+    per_it = (time.perf_counter() - t) / n_it
+    t_solve = per_it * fp64_iters
+    t_total = t_asm + t_red + t_solve
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": E / t_total, "unit": "elems/s", "cores": 1, "kind": "port",
+        "sample": (f"C restatement of iron's FP64 path on 1 core ({cpu}): Q2 element loop with bisection CSR add "
+                   f"over {Es} of {E} elements (x{E / Es:.2f}), K_ff/RHS reduction in full, {n_it} Jacobi-CG "
+                   f"iterations scaled to the {fp64_iters} iterations the FP64 solve needs"),
+        "assembly_s": t_asm, "reduce_s": t_red, "cg_iter_ms": per_it * 1e3, "solve_s": t_solve,
+        "solves_per_s": 1.0 / t_total,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box here has one GPU; "
+              "requests shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, str(ROOT))
+    import torch
+    from iron_amd import _lib, fem
+    torch.cuda.init()
+    P = build_problem(args)
+    red, csr = P["red"], P["csr"]
+    E = P["m"].num_elements
+    for _ in range(args.warmup):
+        step(P, True, args.inner_rtol)
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    wall, lat, infos = timed_steps(P, args.steps, True, args.inner_rtol)
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    # plain FP64 drop-in, same problem, same GPU
+    step(P, False, args.inner_rtol)
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    wall64, lat64, infos64 = timed_steps(P, max(1, min(args.steps, 3)), False, args.inner_rtol)
+    prof64 = _lib.profile_read()
+    _lib.profile_enable(False)
+
+    ms_step = 1e3 * wall / args.steps
+    lat_ms = np.array(lat) * 1e3
+    n_cg, ms_cg = prof.get("cg_spmv_f32", (0, 0.0))
+    nnz_f, n_f = red.nnz, red.n_free
+    bytes_spmv32 = 8.0 * nnz_f + 24.0 * n_f + 4.0
+    avg_cg_ms = ms_cg / max(n_cg, 1)
+    achieved = bytes_spmv32 / (avg_cg_ms * 1e-3) / 1e9 if n_cg else 0.0
+    n_a32, ms_a32 = prof.get("assemble_scalar_f32", (0, 0.0))
+    n_a64, ms_a64 = prof.get("assemble_scalar", (0, 0.0))
+    asm32_ms = ms_a32 / max(n_a32, 1)
+    asm64_ms = ms_a64 / max(n_a64, 1)
+    out = {
+        "metric": "elems/s assembled; solves/s mixed vs FP64 at 1e-8 rel resid; p50/p99",
+        "value": E * args.steps / wall,
+        "unit": "elems/s (assembled and solved to 1e-8, full pass)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 outer",
+        "data": "synthetic (generated mesh, lognormal conductivity from a fixed seed)",
+        "config": {"workload": f"{args.workload}: 48^3 Q2 hex, 3x3x3 Gauss, kappa~lognormal(0,1) seed {args.seed}, "
+                               "f=1, u=0 on all faces, FP32 inner CG + FP64 refinement to 1e-8",
+                   "elements": E, "dofs": csr.n, "free_dofs": n_f, "solver_nnz": nnz_f, "eq_nnz": csr.nnz},
+        "solves_per_s_mixed": args.steps / wall,
+        "solves_per_s_fp64": len(lat64) / wall64,
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "p50_ms_fp64": float(np.percentile(np.array(lat64) * 1e3, 50)),
+        "assembly_elems_per_s_f32": E / (asm32_ms * 1e-3) if asm32_ms > 0 else None,
+        "assembly_elems_per_s_f64": E / (asm64_ms * 1e-3) if asm64_ms > 0 else None,
+        "dof_per_s": csr.n * args.steps / wall,
+        "iterations_mixed_inner": infos[-1].iterations, "refinement_steps": infos[-1].outer,
+        "iterations_fp64": infos64[-1].iterations,
+        "relres_mixed": infos[-1].relres, "relres_fp64": infos64[-1].relres,
+        "roofline": {"kernel": "cg_spmv<float> (fused SpMV + p/q update + p.q)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_spmv32, "avg_launch_ms": avg_cg_ms, "launches": n_cg},
+        "kernel_ms": {k: v for k, v in prof.items() if v[0]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -156,10 +156,10 @@ int mife_assemble_elastic(const mife_element_desc* desc, int n_elem,
 int mife_spmv(int n_rows, const int32_t* row_ptr, const int32_t* col,
               const void* vals, const void* x, void* y, int prec, void* stream);
 
-/* dst[k] = src[src_pos[k]] (FP64 -> FP64 or FP32 per dst_prec).  The per-solve
- * copy of K_ff into the solver matrix, SOLVER_MATRIX_EQUATIONS_MATRIX_ADD
- * (src/solver_matrices_routines.F90:806-1023). */
-int mife_gather_values(int64_t nnz, const int32_t* src_pos, const double* src,
+/* dst[k] = src[src_pos[k]], src and dst each FP64 or FP32 (src_prec, dst_prec).
+ * The per-solve copy of K_ff into the solver matrix,
+ * SOLVER_MATRIX_EQUATIONS_MATRIX_ADD (src/solver_matrices_routines.F90:806-1023). */
+int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int src_prec,
                        void* dst, int dst_prec, void* stream);
 
 /* Solver RHS over the free rows (SOLVER_MATRICES_STATIC_ASSEMBLE RHS part,
@@ -216,6 +216,18 @@ int mife_refine(int n, const int32_t* row_ptr, const int32_t* col,
 int mife_gmres(int n, const int32_t* row_ptr, const int32_t* col, const double* vals,
                const double* b, double* x, double rtol, int restart, int maxit,
                void* work, int* iters, double* relres, void* stream);
+
+/* ------------------------------------------------------------- profiling */
+
+/* Kernel timing with HIP events recorded on the launching stream around each
+ * hot kernel (CG SpMV / update, assembly colour passes, FP64 residual).  Off by
+ * default; no reference counterpart (replaces the TAUPROF / timing-output
+ * instrumentation, src/equations_set_routines.F90:1239-1353).  mife_profile_read
+ * synchronises the recorded events and returns per-kernel launch counts and
+ * total milliseconds; names are written as n_max slots of 48 chars. */
+int mife_profile_enable(int on);
+int mife_profile_reset(void);
+int mife_profile_read(int n_max, char* names, int* counts, double* total_ms, int* n_out);
 
 #ifdef __cplusplus
 }

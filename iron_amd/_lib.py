@@ -54,7 +54,7 @@ _SIG = {
     "mife_assemble_scalar": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
     "mife_assemble_elastic": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P]),
     "mife_spmv": (_I, [_I, _P, _P, _P, _P, _P, _I, _P]),
-    "mife_gather_values": (_I, [_I64, _P, _P, _P, _I, _P]),
+    "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
     "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mife_field_update": (_I, [_I, _P, _P, _P, _P]),
     "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
@@ -63,6 +63,9 @@ _SIG = {
     "mife_refine": (_I, [_I, _P, _P, _P, _P, _P, _P, _D, _D, _I, _P, C.POINTER(_I), C.POINTER(_I),
                          C.POINTER(_D), _P]),
     "mife_gmres": (_I, [_I, _P, _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_profile_enable": (_I, [_I]),
+    "mife_profile_reset": (_I, []),
+    "mife_profile_read": (_I, [_I, C.c_char_p, _P, _P, C.POINTER(_I)]),
 }
 
 
@@ -117,3 +120,27 @@ def stream_ptr(stream=None) -> int:
 
 def call(name: str, *args):
     check(getattr(lib(), name)(*args))
+
+
+def profile_enable(on: bool = True):
+    call("mife_profile_enable", 1 if on else 0)
+
+
+def profile_reset():
+    call("mife_profile_reset")
+
+
+def profile_read() -> dict:
+    """{kernel: (launches, total_ms)} of the HIP-event timings since the last reset."""
+    import numpy as np
+    n_max = 32
+    names = C.create_string_buffer(48 * n_max)
+    counts = np.zeros(n_max, dtype=np.int32)
+    ms = np.zeros(n_max, dtype=np.float64)
+    n = C.c_int(0)
+    call("mife_profile_read", n_max, names, counts.ctypes.data, ms.ctypes.data, C.byref(n))
+    out = {}
+    for k in range(n.value):
+        name = names.raw[48 * k: 48 * (k + 1)].split(b"\0", 1)[0].decode()
+        out[name] = (int(counts[k]), float(ms[k]))
+    return out

@@ -8,6 +8,7 @@
 // are launched one after another, so every CSR entry is updated by at most one
 // wavefront per launch and the summation order is fixed (deterministic).
 #include "mife_common.hpp"
+#include "mife_profile.hpp"
 
 namespace mife {
 
@@ -230,6 +231,7 @@ static int launch_scalar(const mife_element_desc* d, int n_elem, const double* c
                          const int32_t* order, const int32_t* cptr, int ncol, T* vals, double* svec,
                          hipStream_t s) {
   constexpr int WPB = (sizeof(T) * NG * DIM * NN > 12000) ? 1 : 4;
+  const int tk = prof::begin(s);
   if (!order) {
     scalar_assemble_kernel<DIM, NN, NG, T, WPB><<<div_up(n_elem, WPB), 64 * WPB, 0, s>>>(
         coords, enodes, edofs, kappa, source, pos, nullptr, 0, n_elem, d->dN, d->N, d->w, vals, svec, true);
@@ -241,6 +243,7 @@ static int launch_scalar(const mife_element_desc* d, int n_elem, const double* c
           coords, enodes, edofs, kappa, source, pos, order, cptr[c], cnt, d->dN, d->N, d->w, vals, svec, false);
     }
   }
+  prof::end(tk, s, sizeof(T) == 8 ? prof::kAssembleScalar : prof::kAssembleScalar32);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }
@@ -250,6 +253,7 @@ static int launch_elastic(const mife_element_desc* d, int n_elem, const double* 
                           const double* material, const int32_t* pos, const int32_t* order, const int32_t* cptr,
                           int ncol, T* vals, hipStream_t s) {
   constexpr int WPB = 2;
+  const int tk = prof::begin(s);
   if (!order) {
     elastic_assemble_kernel<NN, NG, T, WPB><<<div_up(n_elem, WPB), 64 * WPB, 0, s>>>(
         coords, enodes, material, pos, nullptr, 0, n_elem, d->dN, d->w, vals, true);
@@ -261,6 +265,7 @@ static int launch_elastic(const mife_element_desc* d, int n_elem, const double* 
           coords, enodes, material, pos, order, cptr[c], cnt, d->dN, d->w, vals, false);
     }
   }
+  prof::end(tk, s, sizeof(T) == 8 ? prof::kAssembleElastic : prof::kAssembleElastic32);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }

@@ -11,6 +11,7 @@
 // iterations.  Kernels return at once after convergence is flagged.
 #include "mife_common.hpp"
 #include "mife_sparse.hpp"
+#include "mife_profile.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -325,24 +326,32 @@ static void launch_cg_init(int n, int L, const int32_t* rp, const int32_t* col, 
 #undef CASE_L
 }
 
+// Launches one CG iteration; with profiling on, the two kernels' event tickets
+// are returned through tk[0..1] (else -1).
 template <typename T>
 static void launch_cg_iter(int n, int L, const int32_t* rp, const int32_t* col, const T* vals, T* x, T* r, T* z,
-                           T* p, T* q, const T* dinv, CgState* st, hipStream_t s) {
+                           T* p, T* q, const T* dinv, CgState* st, hipStream_t s, int* tk) {
   const int grid = std::min(spmv_grid(n, L), kRedBlocks);
+  tk[0] = prof::begin(s);
 #define CASE_L(LL) \
   case LL: cg_spmv_kernel<T, LL><<<grid, 256, 0, s>>>(n, rp, col, vals, z, p, q, st); break;
   switch (L) { CASE_L(4) CASE_L(8) CASE_L(16) CASE_L(32) default: CASE_L(64) }
 #undef CASE_L
+  prof::end(tk[0], s, sizeof(T) == 8 ? prof::kCgSpmv : prof::kCgSpmv32);
+  tk[1] = prof::begin(s);
   cg_update_kernel<T><<<vec_grid(n), 256, 0, s>>>(n, x, r, z, p, q, dinv, st);
+  prof::end(tk[1], s, sizeof(T) == 8 ? prof::kCgUpdate : prof::kCgUpdate32);
 }
 
 static void launch_residual64(int n, int L, const int32_t* rp, const int32_t* col, const double* vals,
                               const double* b, const double* x, double* r, CgState* st, hipStream_t s) {
   const int grid = std::min(spmv_grid(n, L), kRedBlocks);
+  const int tk = prof::begin(s);
 #define CASE_L(LL) \
   case LL: residual64_kernel<LL><<<grid, 256, 0, s>>>(n, rp, col, vals, b, x, r, st); break;
   switch (L) { CASE_L(4) CASE_L(8) CASE_L(16) CASE_L(32) default: CASE_L(64) }
 #undef CASE_L
+  prof::end(tk, s, prof::kResidual64);
 }
 
 struct HostState {
@@ -373,12 +382,21 @@ static int run_cg(int n, int L, const int32_t* rp, const int32_t* col, const T* 
   MIFE_CHECK_HIP(hipGetLastError());
   int chunk = 8;
   int launched = 0;
+  std::vector<int> tickets;
   while (true) {
-    for (int k = 0; k < chunk; ++k) launch_cg_iter<T>(n, L, rp, col, vals, x, r, z, p, q, dinv, st, s);
-    launched += chunk;
+    tickets.assign(2 * chunk, -1);
+    for (int k = 0; k < chunk; ++k)
+      launch_cg_iter<T>(n, L, rp, col, vals, x, r, z, p, q, dinv, st, s, &tickets[2 * k]);
     MIFE_CHECK_HIP(hipGetLastError());
     int rc = read_state(st, hs, s);
     if (rc) return rc;
+    // iterations at or past the converged count returned at once: not timed
+    for (int k = 0; k < chunk; ++k)
+      if (launched + k >= hs->iters) {
+        prof::discard(tickets[2 * k]);
+        prof::discard(tickets[2 * k + 1]);
+      }
+    launched += chunk;
     if (hs->done || launched >= maxit + chunk) break;
     chunk = std::min(chunk * 2, 256);
   }

@@ -5,13 +5,11 @@
 
 namespace mife {
 
-__global__ void gather_kernel(long long nnz, const int32_t* __restrict__ src_pos, const double* __restrict__ src,
-                              double* __restrict__ dst64, float* __restrict__ dst32) {
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x) {
-    const double v = src[src_pos[k]];
-    if (dst64) dst64[k] = v;
-    else dst32[k] = (float)v;
-  }
+template <typename S, typename D>
+__global__ void gather_kernel(long long nnz, const int32_t* __restrict__ src_pos, const S* __restrict__ src,
+                              D* __restrict__ dst) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x)
+    dst[k] = (D)src[src_pos[k]];
 }
 
 // One wavefront per free row: b[f] = src + rhs - sum over fixed columns of K_ij u_j,
@@ -95,14 +93,22 @@ int mife_spmv(int n_rows, const int32_t* row_ptr, const int32_t* col, const void
   return MIFE_OK;
 }
 
-int mife_gather_values(int64_t nnz, const int32_t* src_pos, const double* src, void* dst, int dst_prec,
+int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int src_prec, void* dst, int dst_prec,
                        void* stream) {
   MIFE_REQUIRE(src_pos && src && dst, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE((src_prec == MIFE_F64 || src_prec == MIFE_F32) && (dst_prec == MIFE_F64 || dst_prec == MIFE_F32),
+               MIFE_ERR_ARG, "invalid precision");
   if (nnz == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   const int grid = (int)std::min<long long>(div_up(nnz, 256), 8192);
-  gather_kernel<<<grid, 256, 0, s>>>(nnz, src_pos, src, dst_prec == MIFE_F64 ? (double*)dst : nullptr,
-                                     dst_prec == MIFE_F64 ? nullptr : (float*)dst);
+  if (src_prec == MIFE_F64 && dst_prec == MIFE_F64)
+    gather_kernel<double, double><<<grid, 256, 0, s>>>(nnz, src_pos, (const double*)src, (double*)dst);
+  else if (src_prec == MIFE_F64)
+    gather_kernel<double, float><<<grid, 256, 0, s>>>(nnz, src_pos, (const double*)src, (float*)dst);
+  else if (dst_prec == MIFE_F32)
+    gather_kernel<float, float><<<grid, 256, 0, s>>>(nnz, src_pos, (const float*)src, (float*)dst);
+  else
+    gather_kernel<float, double><<<grid, 256, 0, s>>>(nnz, src_pos, (const float*)src, (double*)dst);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }

@@ -126,11 +126,13 @@ class DirichletReduction:
         call("mife_reduce_pattern", csr.n, ptr(csr.row_ptr), ptr(csr.col), ptr(self.new_idx), self.n_free,
              ptr(self.row_ptr), ptr(self.col), ptr(self.src_pos), None, ptr(work), s)
 
-    def gather(self, vals64: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
+    def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
+        """K_ff values from equations-CSR values (FP64 or FP32) in precision ``prec``."""
         if out is None:
             out = torch.empty(max(self.nnz, 1), dtype=torch.float64 if prec == F64 else torch.float32,
-                              device=vals64.device)
-        call("mife_gather_values", self.nnz, ptr(self.src_pos), ptr(vals64), ptr(out), prec, _stream())
+                              device=vals.device)
+        src_prec = F64 if vals.dtype == torch.float64 else F32
+        call("mife_gather_values", self.nnz, ptr(self.src_pos), ptr(vals), src_prec, ptr(out), prec, _stream())
         return out
 
     def rhs(self, vals64, u, src=None, rhs=None, out=None) -> torch.Tensor:

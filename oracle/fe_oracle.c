@@ -1,0 +1,214 @@
+/*
+ * C restatement of OpenCMISS-Iron's CPU path for the scalar (Laplace /
+ * generalised Laplace / constant-source Poisson) and linear-elasticity
+ * assembly-and-solve stage, FP64, one core per call unless OpenMP threads are
+ * requested.
+ *
+ * TEST INFRASTRUCTURE ONLY: the timed CPU baseline in bench.py and a checker in
+ * tests/.  It is never linked into, or called by, the product library.
+ *
+ * Follows, per element and per Gauss point, the loops of
+ *   LaplaceEquation_FiniteElementCalculate   Laplace_equations_routines.F90:482-579
+ *   (materials scaling of the generalised subtype :581-747, isotropic kappa*I)
+ *   POISSON constant source vector            Poisson_equations_routines.F90:3407-3415
+ *   COORDINATE_METRICS_CALCULATE (GL, GU, |J|) coordinate_routines.F90:639-826
+ *   InvertFullDP                               maths.F90:2807-2868
+ * and adds every element entry into the CSR values the way
+ *   MATRIX_VALUES_ADD_DP2 -> MATRIX_STORAGE_LOCATION_FIND does
+ *   (bisection down to 10 entries, then a linear scan; matrix_vector.F90:184,1724-1854,3119).
+ * The solve is PETSc-style CG with Jacobi preconditioning on the reduced
+ * (free x free) CSR, stopped on the true relative residual.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static double inv3(const double A[3][3], double B[3][3]) {
+  const double det = A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] -
+                     A[0][0] * A[2][1] * A[1][2] - A[1][0] * A[0][1] * A[2][2] - A[2][0] * A[1][1] * A[0][2];
+  B[0][0] = (A[1][1] * A[2][2] - A[2][1] * A[1][2]) / det;
+  B[1][0] = (A[1][2] * A[2][0] - A[2][2] * A[1][0]) / det;
+  B[2][0] = (A[1][0] * A[2][1] - A[2][0] * A[1][1]) / det;
+  B[0][1] = (A[2][1] * A[0][2] - A[0][1] * A[2][2]) / det;
+  B[1][1] = (A[2][2] * A[0][0] - A[0][2] * A[2][0]) / det;
+  B[2][1] = (A[2][0] * A[0][1] - A[0][0] * A[2][1]) / det;
+  B[0][2] = (A[0][1] * A[1][2] - A[1][1] * A[0][2]) / det;
+  B[1][2] = (A[0][2] * A[1][0] - A[1][2] * A[0][0]) / det;
+  B[2][2] = (A[0][0] * A[1][1] - A[1][0] * A[0][1]) / det;
+  return det;
+}
+
+/* MATRIX_STORAGE_LOCATION_FIND, CSR: bisection to <= 10 entries then linear. */
+static int64_t location_find(const int32_t* rp, const int32_t* col, int row, int target) {
+  int64_t lo = rp[row], hi = (int64_t)rp[row + 1] - 1;
+  while (hi - lo + 1 > 10) {
+    const int64_t mid = (lo + hi) / 2;
+    if (col[mid] == target) return mid;
+    if (col[mid] < target) lo = mid + 1; else hi = mid - 1;
+  }
+  for (int64_t k = lo; k <= hi; ++k)
+    if (col[k] == target) return k;
+  return -1;
+}
+
+/* Scalar 3-D assembly.  dN[ng][nn][3], N[ng][nn], w[ng]; coords[node][3];
+ * enodes/edofs [E][nn]; kappa/source per element (NULL = 1 / none).
+ * vals (CSR values) and src (vector) are accumulated into.  Returns -1 if an
+ * entry is missing from the pattern. */
+int oracle_assemble_scalar3(int E, int nn, int ng, const double* dN, const double* N, const double* w,
+                            const double* coords, const int32_t* enodes, const int32_t* edofs,
+                            const double* kappa, const double* source, const int32_t* rp, const int32_t* col,
+                            double* vals, double* src, int e_begin, int e_end) {
+  double* Ke = (double*)malloc(sizeof(double) * nn * nn);
+  double* se = (double*)malloc(sizeof(double) * nn);
+  double* xe = (double*)malloc(sizeof(double) * nn * 3);
+  int bad = 0;
+  for (int e = e_begin; e < e_end && e < E; ++e) {
+    memset(Ke, 0, sizeof(double) * nn * nn);
+    memset(se, 0, sizeof(double) * nn);
+    for (int a = 0; a < nn; ++a)
+      for (int d = 0; d < 3; ++d) xe[a * 3 + d] = coords[(int64_t)enodes[(int64_t)e * nn + a] * 3 + d];
+    const double ke = kappa ? kappa[e] : 1.0;
+    for (int g = 0; g < ng; ++g) {
+      double J[3][3] = {{0}};
+      for (int a = 0; a < nn; ++a)
+        for (int j = 0; j < 3; ++j) {
+          const double dn = dN[((int64_t)g * nn + a) * 3 + j];
+          for (int d = 0; d < 3; ++d) J[d][j] += xe[a * 3 + d] * dn;
+        }
+      double GL[3][3], GU[3][3];
+      for (int mi = 0; mi < 3; ++mi)
+        for (int ni = 0; ni < 3; ++ni) GL[mi][ni] = J[0][mi] * J[0][ni] + J[1][mi] * J[1][ni] + J[2][mi] * J[2][ni];
+      const double detgl = inv3(GL, GU);
+      const double rwg = sqrt(fabs(detgl)) * w[g];
+      for (int ms = 0; ms < nn; ++ms) {
+        const double* pm = &dN[((int64_t)g * nn + ms) * 3];
+        for (int ns = 0; ns < nn; ++ns) {
+          const double* pn = &dN[((int64_t)g * nn + ns) * 3];
+          double sum = 0.0;
+          for (int mi = 0; mi < 3; ++mi)
+            for (int ni = 0; ni < 3; ++ni) sum += pm[mi] * pn[ni] * GU[mi][ni];
+          Ke[ms * nn + ns] += sum * rwg;
+        }
+        if (source) se[ms] += source[e] * N[(int64_t)g * nn + ms] * rwg;
+      }
+    }
+    for (int ms = 0; ms < nn; ++ms) {
+      const int row = edofs[(int64_t)e * nn + ms];
+      for (int ns = 0; ns < nn; ++ns) {
+        const int64_t p = location_find(rp, col, row, edofs[(int64_t)e * nn + ns]);
+        if (p < 0) { bad = 1; continue; }
+        vals[p] += ke * Ke[ms * nn + ns];
+      }
+      if (source && src) src[row] += se[ms];
+    }
+  }
+  free(Ke);
+  free(se);
+  free(xe);
+  return bad ? -1 : 0;
+}
+
+/* K_ff extraction + RHS lifting over free rows (SOLVER_MATRICES_STATIC_ASSEMBLE). */
+int64_t oracle_reduce(int n, const int32_t* rp, const int32_t* col, const double* vals, const int32_t* new_idx,
+                      const double* u, const double* src, int32_t* srp, int32_t* scol, double* svals, double* b) {
+  const double ZT = 5.0 * 2.220446049250313e-16;
+  int64_t k2 = 0;
+  int f = 0;
+  srp[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    if (new_idx[i] < 0) continue;
+    double bi = src ? src[i] : 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+      const int j = col[k];
+      if (new_idx[j] >= 0) {
+        scol[k2] = new_idx[j];
+        svals[k2] = vals[k];
+        ++k2;
+      } else if (fabs(u[j]) >= ZT && fabs(vals[k]) >= ZT) {
+        bi += -1.0 * vals[k] * u[j];
+      }
+    }
+    b[f] = bi;
+    srp[++f] = (int32_t)k2;
+  }
+  return k2;
+}
+
+static double dot(int n, const double* a, const double* b, int threads) {
+  double s = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : s) num_threads(threads) if (threads > 1)
+#endif
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  (void)threads;
+  return s;
+}
+
+static void matvec(int n, const int32_t* rp, const int32_t* col, const double* v, const double* x, double* y,
+                   int threads) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+#endif
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[col[k]];
+    y[i] = s;
+  }
+  (void)threads;
+}
+
+/* Jacobi-preconditioned CG, FP64.  Runs at most maxit iterations; returns the
+ * iteration count, and the true relative residual in *relres. */
+int oracle_cg(int n, const int32_t* rp, const int32_t* col, const double* vals, const double* b, double* x,
+              double rtol, int maxit, int threads, double* relres) {
+  double* r = (double*)malloc(sizeof(double) * n);
+  double* z = (double*)malloc(sizeof(double) * n);
+  double* p = (double*)malloc(sizeof(double) * n);
+  double* q = (double*)malloc(sizeof(double) * n);
+  double* dinv = (double*)malloc(sizeof(double) * n);
+  for (int i = 0; i < n; ++i) {
+    double d = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k)
+      if (col[k] == i) d = vals[k];
+    dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+  }
+  matvec(n, rp, col, vals, x, q, threads);
+  for (int i = 0; i < n; ++i) {
+    r[i] = b[i] - q[i];
+    z[i] = dinv[i] * r[i];
+    p[i] = z[i];
+  }
+  const double bn = sqrt(dot(n, b, b, threads));
+  double rz = dot(n, r, z, threads);
+  int it = 0;
+  while (it < maxit && sqrt(dot(n, r, r, threads)) > rtol * bn) {
+    matvec(n, rp, col, vals, p, q, threads);
+    const double alpha = rz / dot(n, p, q, threads);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) if (threads > 1)
+#endif
+    for (int i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+      z[i] = dinv[i] * r[i];
+    }
+    const double rz2 = dot(n, r, z, threads);
+    const double beta = rz2 / rz;
+    rz = rz2;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) if (threads > 1)
+#endif
+    for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    ++it;
+  }
+  matvec(n, rp, col, vals, x, q, threads);
+  for (int i = 0; i < n; ++i) r[i] = b[i] - q[i];
+  if (relres) *relres = bn > 0 ? sqrt(dot(n, r, r, threads)) / bn : 0.0;
+  free(r); free(z); free(p); free(q); free(dinv);
+  return it;
+}
