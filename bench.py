@@ -39,7 +39,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--format", default="sell", choices=["sell", "csr"], help="solver-matrix storage")
+    ap.add_argument("--inner-rtol", type=float, default=0.1,
+                    help="FP32 residual reduction between FP64 residual replacements")
     return ap.parse_args()
 
 
@@ -73,15 +75,16 @@ def build_problem(args):
     csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, dev, colours=mesh.structured_colours(m))
     asm = fem.ScalarAssembler(b, m.coords, m.elem_nodes[0], csr)
     red = fem.DirichletReduction(csr, fixed)
-    P = dict(m=m, dm=dm, b=b, csr=csr, asm=asm, red=red, rtol=rtol, fixed=fixed, ufix=ufix,
+    S = fem.SolverMatrix(red, args.format)
+    P = dict(m=m, dm=dm, basis=b, csr=csr, asm=asm, red=red, S=S, rtol=rtol, fixed=fixed, ufix=ufix,
              kappa=kappa.to(dev), source=None if source is None else source.to(dev), kappa_h=kappa.numpy(),
              ws=fem.KrylovWorkspace(red.n_free, dev))
     P["v64"] = csr.zeros(fem.F64)
     P["v32"] = csr.zeros(fem.F32)
     P["src"] = torch.zeros(dm.num_dofs, dtype=torch.float64, device=dev)
     P["u"] = torch.as_tensor(np.where(fixed, ufix, 0.0), device=dev)
-    P["sv64"] = torch.empty(max(red.nnz, 1), dtype=torch.float64, device=dev)
-    P["sv32"] = torch.empty(max(red.nnz, 1), dtype=torch.float32, device=dev)
+    P["sv64"] = torch.empty(max(S.stored, 1), dtype=torch.float64, device=dev)
+    P["sv32"] = torch.empty(max(S.stored, 1), dtype=torch.float32, device=dev)
     P["b"] = torch.empty(red.n_free, dtype=torch.float64, device=dev)
     P["x"] = torch.zeros(red.n_free, dtype=torch.float64, device=dev)
     return P
@@ -94,20 +97,18 @@ def step(P, mixed: bool, inner_rtol: float):
     src.zero_()
     P["asm"].assemble(v64, kappa=P["kappa"], source=P["source"], src_vec=src if P["source"] is not None else None,
                       prec=fem.F64)
-    red = P["red"]
-    red.gather(v64, fem.F64, out=P["sv64"])
+    red, S = P["red"], P["S"]
+    S.gather(v64, fem.F64, out=P["sv64"])
     red.rhs(v64, P["u"], src=src if P["source"] is not None else None, out=P["b"])
     x = P["x"]
     x.zero_()
     if mixed:
         v32.zero_()
         P["asm"].assemble(v32, kappa=P["kappa"], prec=fem.F32)
-        red.gather(v32, fem.F32, out=P["sv32"])
-        info = fem.refine(red.n_free, red.row_ptr, red.col, P["sv64"], P["sv32"], P["b"], x, P["rtol"], 200000,
-                          P["ws"], inner_rtol=inner_rtol)
+        S.gather(v32, fem.F32, out=P["sv32"])
+        info = fem.refine(S, P["sv64"], P["sv32"], P["b"], x, P["rtol"], 200000, P["ws"], inner_rtol=inner_rtol)
     else:
-        info = fem.cg(red.n_free, red.row_ptr, red.col, P["sv64"], P["b"], x, P["rtol"], 200000, P["ws"],
-                      prec=fem.F64)
+        info = fem.cg(S, P["sv64"], P["b"], x, P["rtol"], 200000, P["ws"], prec=fem.F64)
     red.field_update(x, P["u"])
     return info
 
@@ -136,7 +137,7 @@ def cpu_baseline(P, fp64_iters, budget_s):
     if not so.exists():
         return None
     lib = C.CDLL(str(so))
-    m, b, csr, red = P["m"], P["b"], P["csr"], P["red"]
+    m, b, csr, red = P["m"], P["basis"], P["csr"], P["red"]
     E = m.num_elements
     rp = csr.row_ptr.cpu().numpy().astype(np.int32)
     col = csr.col[: csr.nnz].cpu().numpy().astype(np.int32)
@@ -245,6 +246,8 @@ def main():
     lat_ms = np.array(lat) * 1e3
     n_cg, ms_cg = prof.get("cg_spmv_f32", (0, 0.0))
     nnz_f, n_f = red.nnz, red.n_free
+    # algorithmic bytes of one fused SpMV launch (FP32): values + columns of the
+    # solver matrix, row pointers, z read once, p and q read and written
     bytes_spmv32 = 8.0 * nnz_f + 24.0 * n_f + 4.0
     avg_cg_ms = ms_cg / max(n_cg, 1)
     achieved = bytes_spmv32 / (avg_cg_ms * 1e-3) / 1e9 if n_cg else 0.0
@@ -278,6 +281,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bytes_spmv32, "avg_launch_ms": avg_cg_ms, "launches": n_cg},
         "kernel_ms": {k: v for k, v in prof.items() if v[0]},
+        "kernel_ms_fp64": {k: v for k, v in prof64.items() if v[0]},
+        "solver_format": args.format, "sell_padding": getattr(P["S"], "padding", 0.0),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)

@@ -40,6 +40,27 @@ extern "C" {
 #define MIFE_F64 0
 #define MIFE_F32 1
 
+/* Solver-matrix storage formats. */
+#define MIFE_FMT_CSR 0
+#define MIFE_FMT_SELL 1
+
+/* A square solver matrix (values passed separately, FP64 or FP32).
+ * CSR: ptr = row_ptr[n+1], col = column indices, perm = NULL.
+ * SELL (slices of 64 rows, one row per lane of a wavefront): slice row k is
+ * original row perm[k]; ptr = slice_ptr[n_slices+1] (entry offsets, multiples
+ * of 64); entry j of row k of slice s sits at ptr[s] + 64 j + (k - 64 s); col
+ * holds column indices in the slice numbering (padding: value 0).  Solvers take
+ * b and x in the original numbering and permute internally. */
+typedef struct mife_matrix {
+  int n;
+  int format;
+  const int32_t* ptr;
+  const int32_t* col;
+  const int32_t* perm;
+  int n_slices;
+  int64_t nnz;
+} mife_matrix;
+
 /* ---------------------------------------------------------------- runtime */
 
 /* Library version string (static storage). */
@@ -100,6 +121,17 @@ int mife_reduce_pattern(int n, const int32_t* row_ptr, const int32_t* col,
                         int32_t* s_row_ptr, int32_t* s_col, int32_t* src_pos,
                         int64_t* nnz_out, void* work, void* stream);
 
+/* SELL-64 layout of a CSR matrix: given the row permutation (perm, iperm) and
+ * slice_ptr computed from the row lengths, writes sell_col (slice numbering)
+ * and sell_src (position of each stored entry in the CSR values, -1 for
+ * padding) so that mife_gather_values fills SELL values straight from the
+ * equations matrix.  No reference counterpart: the reference's solver matrix
+ * is a PETSc AIJ (CSR) matrix (src/distributed_matrix_vector.F90:2465-2581). */
+int mife_sell_build(int n, const int32_t* row_ptr, const int32_t* col, const int32_t* src_pos,
+                    const int32_t* perm, const int32_t* iperm, int n_slices,
+                    const int32_t* slice_ptr, int32_t* sell_col, int32_t* sell_src,
+                    void* stream);
+
 /* ------------------------------------------------------------- assembly */
 
 /* Element-kernel geometry and basis description shared by the assembly calls.
@@ -152,11 +184,13 @@ int mife_assemble_elastic(const mife_element_desc* desc, int n_elem,
 
 /* --------------------------------------------------- sparse operations */
 
-/* y = A x (prec selects FP64 or FP32 values and vectors). */
-int mife_spmv(int n_rows, const int32_t* row_ptr, const int32_t* col,
-              const void* vals, const void* x, void* y, int prec, void* stream);
+/* y = A x (prec selects FP64 or FP32 values and vectors).  For SELL, x and y
+ * are in the slice numbering. */
+int mife_spmv(const mife_matrix* A, const void* vals, const void* x, void* y, int prec,
+              void* stream);
 
-/* dst[k] = src[src_pos[k]], src and dst each FP64 or FP32 (src_prec, dst_prec).
+/* dst[k] = src[src_pos[k]] (0 where src_pos[k] < 0), src and dst each FP64 or
+ * FP32 (src_prec, dst_prec).
  * The per-solve copy of K_ff into the solver matrix,
  * SOLVER_MATRIX_EQUATIONS_MATRIX_ADD (src/solver_matrices_routines.F90:806-1023). */
 int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int src_prec,
@@ -197,25 +231,27 @@ size_t mife_solver_work_bytes(int n, int restart);
  * :11775-11984).  x holds the initial guess on entry.  iters and relres are
  * host outputs (the call synchronises the stream).  Returns
  * MIFE_ERR_NOT_CONVERGED when maxit is reached. */
-int mife_cg(int n, const int32_t* row_ptr, const int32_t* col, const void* vals,
-            const void* b, void* x, int prec, double rtol, int maxit,
-            void* work, int* iters, double* relres, void* stream);
+int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int prec,
+            double rtol, int maxit, void* work, int* iters, double* relres, void* stream);
 
-/* Mixed-precision solve: FP64 iterative refinement around an FP32
- * Jacobi-preconditioned CG.  Outer residual r = b - A64 x and the solution
- * accumulate in FP64 (vals64); the inner CG runs on vals32.  Stops when
- * ||b - A64 x|| <= rtol ||b||.  inner_rtol is the inner CG reduction per outer
- * step.  iters = total inner iterations, outer = refinement steps. */
-int mife_refine(int n, const int32_t* row_ptr, const int32_t* col,
-                const double* vals64, const float* vals32, const double* b,
-                double* x, double rtol, double inner_rtol, int maxit,
+/* Mixed-precision solve: Jacobi CG on the FP32 operator vals32 whose residual is
+ * replaced by the FP64 residual r = b - A64 x each time the FP32 recurrence
+ * residual has fallen by the factor inner_rtol since the last replacement
+ * ("reliable updates"); the FP32 correction is folded into the FP64 solution x
+ * at each replacement, and the search direction is kept, so FP64 accuracy is
+ * won back without restarting the Krylov space.  Stops when
+ * ||b - A64 x|| <= rtol ||b||.  iters = FP32 iterations, outer = FP64
+ * replacements.  No reference counterpart: iron solves in FP64 only
+ * (Petsc_KSPSolve, src/solver_routines.F90:11869). */
+int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
+                const double* b, double* x, double rtol, double inner_rtol, int maxit,
                 void* work, int* iters, int* outer, double* relres, void* stream);
 
 /* Right-Jacobi-preconditioned restarted GMRES(restart) in FP64 (KSPGMRES default
  * restart 30, src/solver_routines.F90:11320-11369). */
-int mife_gmres(int n, const int32_t* row_ptr, const int32_t* col, const double* vals,
-               const double* b, double* x, double rtol, int restart, int maxit,
-               void* work, int* iters, double* relres, void* stream);
+int mife_gmres(const mife_matrix* A, const double* vals, const double* b, double* x,
+               double rtol, int restart, int maxit, void* work, int* iters,
+               double* relres, void* stream);
 
 /* ------------------------------------------------------------- profiling */
 

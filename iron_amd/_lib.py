@@ -31,6 +31,16 @@ class NotConvergedError(CMFEError):
     pass
 
 
+class MatrixDesc(C.Structure):
+    """mife_matrix: a square solver matrix in CSR or SELL-64 storage."""
+    _fields_ = [("n", C.c_int), ("format", C.c_int), ("ptr", C.c_void_p), ("col", C.c_void_p),
+                ("perm", C.c_void_p), ("n_slices", C.c_int), ("nnz", C.c_int64)]
+
+
+MIFE_FMT_CSR = 0
+MIFE_FMT_SELL = 1
+
+
 class ElementDesc(C.Structure):
     _fields_ = [("dim", C.c_int), ("n_nodes", C.c_int), ("n_gauss", C.c_int),
                 ("dN", C.c_void_p), ("N", C.c_void_p), ("w", C.c_void_p)]
@@ -53,16 +63,17 @@ _SIG = {
     "mife_reduce_pattern": (_I, [_I, _P, _P, _P, _I, _P, _P, _P, C.POINTER(_I64), _P, _P]),
     "mife_assemble_scalar": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
     "mife_assemble_elastic": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P]),
-    "mife_spmv": (_I, [_I, _P, _P, _P, _P, _P, _I, _P]),
+    "mife_spmv": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _P]),
+    "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
     "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mife_field_update": (_I, [_I, _P, _P, _P, _P]),
     "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "mife_solver_work_bytes": (C.c_size_t, [_I, _I]),
-    "mife_cg": (_I, [_I, _P, _P, _P, _P, _P, _I, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
-    "mife_refine": (_I, [_I, _P, _P, _P, _P, _P, _P, _D, _D, _I, _P, C.POINTER(_I), C.POINTER(_I),
+    "mife_cg": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_refine": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _D, _I, _P, C.POINTER(_I), C.POINTER(_I),
                          C.POINTER(_D), _P]),
-    "mife_gmres": (_I, [_I, _P, _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_profile_enable": (_I, [_I]),
     "mife_profile_reset": (_I, []),
     "mife_profile_read": (_I, [_I, C.c_char_p, _P, _P, C.POINTER(_I)]),

@@ -1,17 +1,20 @@
-// Krylov solvers: Jacobi-preconditioned CG (FP64 or FP32), FP64 iterative
-// refinement around an FP32 inner CG, and right-Jacobi restarted GMRES (FP64).
+// Krylov solvers: Jacobi-preconditioned CG (FP64 or FP32), mixed-precision CG
+// with FP64 residual replacement ("reliable updates"), and right-Jacobi
+// restarted GMRES (FP64).
 //
 // CG runs two fused kernels per iteration and keeps every scalar on the
 // device.  With q = A p maintained by the recurrence q = A z + beta q, the
 // search direction update folds into the SpMV kernel:
 //   K_A: p = z + beta p ; q = A z + beta q ; partial (p.q)   -> alpha = rz / pq
 //   K_B: x += alpha p ; r -= alpha q ; z = D^-1 r ; partial (r.z), (r.r) -> beta
-// The last block of each kernel reduces the per-block partials in a fixed
-// order (deterministic), so the host only looks at the residual every few
-// iterations.  Kernels return at once after convergence is flagged.
+// Each bandwidth kernel leaves one partial sum per block; a one-block
+// finisher kernel reduces them in a fixed order (deterministic) and updates the
+// device-side state, so no device-scope fence runs inside the bandwidth
+// kernels and the host only looks at the state every few iterations.  Kernels
+// return at once once the state says stop.
 #include "mife_common.hpp"
-#include "mife_sparse.hpp"
 #include "mife_profile.hpp"
+#include "mife_sparse.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -19,25 +22,24 @@
 
 namespace mife {
 
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = kMaxGrid;
 
 struct CgState {
   double rz, pq, alpha, beta, rr, bb, tol2, scale;
-  int iters, done, maxit, pad;
+  double rz_prev, maxrr, delta2, rr64;
+  int iters, done, maxit, pause;
   unsigned int cnt_a, cnt_b, cnt_c, pad2;
   // per-block partials of up to 4 reduced values each (value v of block b at v*kRedBlocks+b)
   double part_a[4 * kRedBlocks];
   double part_b[4 * kRedBlocks];
 };
 
-// Fixed-order reduction of per-block partials by the last block to finish.
-// Returns true in the last block (all threads); sums in out[0..n_vals).
+// Block-level sums of up to 4 values; thread 0 stores them as this block's
+// partials (value v of block b at parts[v * kRedBlocks + b]).
 template <int NV>
-__device__ __forceinline__ bool last_block_reduce(double (&mine)[NV], double* parts, unsigned int* counter,
-                                                  double (&out)[NV]) {
+__device__ __forceinline__ void block_partials(double (&mine)[NV], double* parts) {
   static_assert(NV <= 4, "at most 4 reduced values per kernel");
   __shared__ double red[NV][8];
-  __shared__ bool is_last;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = wave_sum(mine[v]);
@@ -49,125 +51,131 @@ __device__ __forceinline__ bool last_block_reduce(double (&mine)[NV], double* pa
     for (int v = 0; v < NV; ++v) {
       double s = 0.0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[v][w];
-      __hip_atomic_store(&parts[v * kRedBlocks + blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      parts[v * kRedBlocks + blockIdx.x] = s;
     }
-    __threadfence();
-    const unsigned int ticket = atomicAdd(counter, 1u);
-    is_last = (ticket == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// In a one-block finisher: fixed-order sums of a producer's nb partials.
+template <int NV>
+__device__ __forceinline__ void finish_sums(const double* __restrict__ parts, int nb, double (&out)[NV]) {
+  __shared__ double red[NV][8];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) s += __hip_atomic_load(&parts[v * kRedBlocks + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += parts[v * kRedBlocks + b];
     s = wave_sum(s);
-    __syncthreads();
     if ((threadIdx.x & 63) == 0) red[v][threadIdx.x >> 6] = s;
-    __syncthreads();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
     double t = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[v][w];
     out[v] = t;
   }
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
 }
 
 template <typename T>
-__global__ void diag_inv_kernel(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                const T* __restrict__ vals, T* __restrict__ dinv) {
+__global__ void diag_inv_kernel(MatArgs A, const T* __restrict__ vals, T* __restrict__ dinv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int lo = rp[i], hi = rp[i + 1];
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (col[mid] < i) lo = mid + 1; else hi = mid;
+  if (i >= A.n) return;
+  T d = 0;
+  if (A.fmt == MIFE_FMT_SELL) {
+    const int s = i / kSlice, lane = i - s * kSlice;
+    const int base = A.ptr[s], len = (A.ptr[s + 1] - base) / kSlice;
+    for (int j = 0; j < len; ++j) {
+      const int e = base + j * kSlice + lane;
+      if (A.col[e] == i && vals[e] != (T)0) { d = vals[e]; break; }
+    }
+  } else {
+    int lo = A.ptr[i], hi = A.ptr[i + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (A.col[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    if (lo < A.ptr[i + 1] && A.col[lo] == i) d = vals[lo];
   }
-  const T d = (lo < rp[i + 1] && col[lo] == i) ? vals[lo] : (T)0;
   dinv[i] = (d != (T)0) ? (T)1 / d : (T)1;
 }
 
-// r = b - A x (x may be skipped when zero), z = D^-1 r, p = 0, q = 0; reduces
-// rz, rr, bb and initialises the state.
-template <typename T, int L>
-__global__ __launch_bounds__(256) void cg_init_kernel(int n, const int32_t* __restrict__ rp,
-                                                      const int32_t* __restrict__ col, const T* __restrict__ vals,
+// r = b - A x (x == NULL: x = 0), z = D^-1 r, p = q = 0; reduces rz, rr, bb and
+// initialises the state.
+template <typename T, int FMT, int L>
+__global__ __launch_bounds__(256) void cg_init_kernel(MatArgs A, const T* __restrict__ vals,
                                                       const T* __restrict__ b, const T* __restrict__ x,
                                                       const T* __restrict__ dinv, T* __restrict__ r,
                                                       T* __restrict__ z, T* __restrict__ p, T* __restrict__ q,
-                                                      CgState* st, double tol2, int maxit) {
-  const int gpb = 256 / L, sub = threadIdx.x % L;
+                                                      CgState* st) {
   double m[3] = {0.0, 0.0, 0.0};
-  for (int row = blockIdx.x * gpb + threadIdx.x / L; row < n; row += gridDim.x * gpb) {
-    T ax = 0;
-    if (x) {
-      for (int k = rp[row] + sub; k < rp[row + 1]; k += L) ax += vals[k] * x[col[k]];
-#pragma unroll
-      for (int off = L / 2; off > 0; off >>= 1) ax += __shfl_xor(ax, off, 64);
-    }
-    if (sub == 0) {
-      const T ri = b[row] - ax;
-      const T zi = dinv[row] * ri;
-      r[row] = ri;
-      z[row] = zi;
-      p[row] = 0;
-      q[row] = 0;
-      m[0] += (double)ri * (double)zi;
-      m[1] += (double)ri * (double)ri;
-      m[2] += (double)b[row] * (double)b[row];
-    }
-  }
+  for_each_row<T, FMT, L>(A, vals, x, [&](int row, T ax) {
+    const T ri = b[row] - ax;
+    const T zi = dinv[row] * ri;
+    r[row] = ri;
+    z[row] = zi;
+    p[row] = 0;
+    q[row] = 0;
+    m[0] += (double)ri * (double)zi;
+    m[1] += (double)ri * (double)ri;
+    m[2] += (double)b[row] * (double)b[row];
+  });
+  block_partials<3>(m, st->part_b);
+}
+
+__global__ void fin_init_kernel(CgState* st, int nb, double tol2, int maxit) {
   double out[3];
-  if (last_block_reduce<3>(m, st->part_b, &st->cnt_b, out) && threadIdx.x == 0) {
+  finish_sums<3>(st->part_b, nb, out);
+  if (threadIdx.x == 0) {
     st->rz = out[0];
     st->rr = out[1];
     st->bb = out[2];
+    st->maxrr = out[1];
     st->beta = 0.0;
     st->alpha = 0.0;
     st->iters = 0;
     st->maxit = maxit;
     st->tol2 = tol2;
+    st->pause = 0;
     st->done = (out[1] <= tol2 * out[2]) ? 1 : 0;
   }
 }
 
-template <typename T, int L>
-__global__ __launch_bounds__(256) void cg_spmv_kernel(int n, const int32_t* __restrict__ rp,
-                                                      const int32_t* __restrict__ col, const T* __restrict__ vals,
+template <typename T, int FMT, int L>
+__global__ __launch_bounds__(256) void cg_spmv_kernel(MatArgs A, const T* __restrict__ vals,
                                                       const T* __restrict__ z, T* __restrict__ p,
                                                       T* __restrict__ q, CgState* st) {
-  if (st->done) return;
+  if (st->done || st->pause) return;
   const T beta = (T)st->beta;
-  const int gpb = 256 / L, sub = threadIdx.x % L;
   double m[1] = {0.0};
-  for (int row = blockIdx.x * gpb + threadIdx.x / L; row < n; row += gridDim.x * gpb) {
-    T az = 0;
-    const int k1 = rp[row + 1];
-    for (int k = rp[row] + sub; k < k1; k += L) az += vals[k] * z[col[k]];
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) az += __shfl_xor(az, off, 64);
-    if (sub == 0) {
-      const T pi = z[row] + beta * p[row];
-      const T qi = az + beta * q[row];
-      p[row] = pi;
-      q[row] = qi;
-      m[0] += (double)pi * (double)qi;
-    }
-  }
+  for_each_row<T, FMT, L>(A, vals, z, [&](int row, T az) {
+    const T pi = z[row] + beta * p[row];
+    const T qi = az + beta * q[row];
+    p[row] = pi;
+    q[row] = qi;
+    m[0] += (double)pi * (double)qi;
+  });
+  block_partials<1>(m, st->part_a);
+}
+
+__global__ void fin_alpha_kernel(CgState* st, int nb) {
+  if (st->done || st->pause) return;
   double out[1];
-  if (last_block_reduce<1>(m, st->part_a, &st->cnt_a, out) && threadIdx.x == 0) {
+  finish_sums<1>(st->part_a, nb, out);
+  if (threadIdx.x == 0) {
     st->pq = out[0];
     st->alpha = st->rz / out[0];
   }
 }
 
+// With delta2 > 0 (mixed precision) the kernel raises `pause` once the FP32
+// residual has fallen by delta since the last FP64 replacement, or below the
+// target; the host then replaces it with the FP64 residual.
 template <typename T>
 __global__ __launch_bounds__(256) void cg_update_kernel(int n, T* __restrict__ x, T* __restrict__ r,
                                                         T* __restrict__ z, const T* __restrict__ p,
                                                         const T* __restrict__ q, const T* __restrict__ dinv,
                                                         CgState* st) {
-  if (st->done) return;
+  if (st->done || st->pause) return;
   const T alpha = (T)st->alpha;
   double m[2] = {0.0, 0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -179,56 +187,109 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, T* __restrict__ x
     m[0] += (double)ri * (double)zi;
     m[1] += (double)ri * (double)ri;
   }
+  block_partials<2>(m, st->part_b);
+}
+
+__global__ void fin_beta_kernel(CgState* st, int nb) {
+  if (st->done || st->pause) return;
   double out[2];
-  if (last_block_reduce<2>(m, st->part_b, &st->cnt_b, out) && threadIdx.x == 0) {
+  finish_sums<2>(st->part_b, nb, out);
+  if (threadIdx.x == 0) {
+    st->rz_prev = st->rz;
     st->beta = out[0] / st->rz;
     st->rz = out[0];
     st->rr = out[1];
     st->iters += 1;
-    st->done = (out[1] <= st->tol2 * st->bb || st->iters >= st->maxit) ? 1 : 0;
+    st->maxrr = fmax(st->maxrr, out[1]);
+    if (st->delta2 > 0.0) {
+      if (out[1] <= st->delta2 * st->maxrr || out[1] <= st->tol2 * st->bb) st->pause = 1;
+      if (st->iters >= st->maxit) st->done = 1;
+    } else {
+      st->done = (out[1] <= st->tol2 * st->bb || st->iters >= st->maxit) ? 1 : 0;
+    }
   }
 }
 
-// FP64 residual r = b - A x with ||r||^2 and ||b||^2 (refinement / exit checks).
-template <int L>
-__global__ __launch_bounds__(256) void residual64_kernel(int n, const int32_t* __restrict__ rp,
-                                                         const int32_t* __restrict__ col,
-                                                         const double* __restrict__ vals,
+// FP64 residual r = b - A x with ||r||^2 and ||b||^2.
+template <int FMT, int L>
+__global__ __launch_bounds__(256) void residual64_kernel(MatArgs A, const double* __restrict__ vals,
                                                          const double* __restrict__ b, const double* __restrict__ x,
                                                          double* __restrict__ r, CgState* st) {
-  const int gpb = 256 / L, sub = threadIdx.x % L;
   double m[2] = {0.0, 0.0};
-  for (int row = blockIdx.x * gpb + threadIdx.x / L; row < n; row += gridDim.x * gpb) {
-    double ax = 0;
-    for (int k = rp[row] + sub; k < rp[row + 1]; k += L) ax += vals[k] * x[col[k]];
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) ax += __shfl_xor(ax, off, 64);
-    if (sub == 0) {
-      const double ri = b[row] - ax;
-      if (r) r[row] = ri;
-      m[0] += ri * ri;
-      m[1] += b[row] * b[row];
-    }
-  }
+  for_each_row<double, FMT, L>(A, vals, x, [&](int row, double ax) {
+    const double ri = b[row] - ax;
+    if (r) r[row] = ri;
+    m[0] += ri * ri;
+    m[1] += b[row] * b[row];
+  });
+  block_partials<2>(m, st->part_a);
+}
+
+__global__ void fin_residual_kernel(CgState* st, int nb) {
   double out[2];
-  if (last_block_reduce<2>(m, st->part_a, &st->cnt_a, out) && threadIdx.x == 0) {
+  finish_sums<2>(st->part_a, nb, out);
+  if (threadIdx.x == 0) {
+    st->rr64 = out[0];
     st->rr = out[0];
     st->bb = out[1];
     st->scale = sqrt(out[0]);
   }
 }
 
-// r32 = r64 / ||r64||
-__global__ void scale_to_f32_kernel(int n, const double* __restrict__ r, float* __restrict__ r32,
-                                    const CgState* st) {
-  const double inv = st->scale > 0.0 ? 1.0 / st->scale : 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) r32[i] = (float)(r[i] * inv);
+// dst[k] = src[perm[k]] (gather into slice numbering); identity when perm == NULL
+template <typename T>
+__global__ void permute_in_kernel(int n, const int32_t* __restrict__ perm, const T* __restrict__ src,
+                                  T* __restrict__ dst) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    dst[k] = src[perm ? perm[k] : k];
 }
 
-// x64 += ||r|| * d32
-__global__ void accumulate_kernel(int n, double* __restrict__ x, const float* __restrict__ d, const CgState* st) {
-  const double s = st->scale;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] += s * (double)d[i];
+// dst[perm[k]] = src[k]
+template <typename T>
+__global__ void permute_out_kernel(int n, const int32_t* __restrict__ perm, const T* __restrict__ src,
+                                   T* __restrict__ dst) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    dst[perm ? perm[k] : k] = src[k];
+}
+
+// Reliable update, part 1: x64 += x32 ; x32 = 0.
+__global__ void fold_correction_kernel(int n, double* __restrict__ x64, float* __restrict__ x32) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x64[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// Reliable update, part 2 (after residual64 into r64 and `ro`):
+// r32 = r64, z = D^-1 r32; the FP32 recurrence restarts from the true residual
+// while p and q = A p are kept, beta = (r.z)_new / (r.z)_before-the-step.
+__global__ __launch_bounds__(256) void replace_residual_kernel(int n, const double* __restrict__ r64,
+                                                               float* __restrict__ r, float* __restrict__ z,
+                                                               const float* __restrict__ dinv, CgState* st) {
+  double m[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float ri = (float)r64[i];
+    const float zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    m[0] += (double)ri * (double)zi;
+    m[1] += (double)ri * (double)ri;
+  }
+  block_partials<2>(m, st->part_b);
+}
+
+__global__ void fin_replace_kernel(CgState* st, const CgState* ro, int nb, int first) {
+  double out[2];
+  finish_sums<2>(st->part_b, nb, out);
+  if (threadIdx.x == 0) {
+    st->beta = first ? 0.0 : out[0] / st->rz_prev;
+    st->rz = out[0];
+    st->rr = out[1];
+    st->maxrr = out[1];
+    st->bb = ro->bb;
+    st->pause = 0;
+    st->done = (ro->rr64 <= st->tol2 * ro->bb) ? 1 : 0;
+  }
 }
 
 template <typename T>
@@ -237,7 +298,7 @@ __global__ void fill_kernel(int n, T* __restrict__ v, T val) {
 }
 
 // ------------------------------------------------------------------ GMRES
-// h[j] partials of V_j . w for j < m over a fixed grid; reduced by gmres_reduce.
+// Partials of V_j . w for j < m over a fixed grid; reduced by multidot_reduce.
 __global__ __launch_bounds__(256) void multidot_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
                                                        const double* __restrict__ w, double* __restrict__ parts) {
   __shared__ double red[32][4];
@@ -259,19 +320,18 @@ __global__ __launch_bounds__(256) void multidot_kernel(int n, int m, const doubl
   }
 }
 
-__global__ void multidot_reduce(int m, int nblocks, const double* __restrict__ parts, double* __restrict__ h,
-                                int accumulate) {
+__global__ void multidot_reduce(int m, int nblocks, const double* __restrict__ parts, double* __restrict__ h) {
   __shared__ double red[4];
   for (int j = 0; j < m; ++j) {
     double s = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[j * kRedBlocks + b];
     s = block_sum(s, red);
-    if (threadIdx.x == 0) h[j] = accumulate ? h[j] + s : s;
+    if (threadIdx.x == 0) h[j] = s;
     __syncthreads();
   }
 }
 
-// w -= sum_j V_j h_j   (h = hsrc, m terms)
+// w -= sum_j V_j h_j
 __global__ void orth_update_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
                                    const double* __restrict__ h, double* __restrict__ w) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -315,48 +375,66 @@ struct Work {
 
 static int vec_grid(int n) { return std::max(1, std::min(div_up(n, 256), kRedBlocks)); }
 
-template <typename T>
-static void launch_cg_init(int n, int L, const int32_t* rp, const int32_t* col, const T* vals, const T* b,
-                           const T* x, const T* dinv, T* r, T* z, T* p, T* q, CgState* st, double tol2, int maxit,
-                           hipStream_t s) {
-  const int grid = std::min(spmv_grid(n, L), kRedBlocks);
-#define CASE_L(LL) \
-  case LL: cg_init_kernel<T, LL><<<grid, 256, 0, s>>>(n, rp, col, vals, b, x, dinv, r, z, p, q, st, tol2, maxit); break;
-  switch (L) { CASE_L(4) CASE_L(8) CASE_L(16) CASE_L(32) default: CASE_L(64) }
-#undef CASE_L
+struct Mat {
+  MatArgs a;
+  const int32_t* perm;
+  int L;
+};
+
+static int make_mat(const mife_matrix* A, Mat* m) {
+  MIFE_REQUIRE(A && A->ptr && A->col, MIFE_ERR_POINTER_IS_NULL, "null matrix");
+  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL, MIFE_ERR_ARG, "invalid matrix format");
+  MIFE_REQUIRE(A->n > 0, MIFE_ERR_ARG, "empty matrix");
+  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || (A->perm && A->n_slices == div_up(A->n, kSlice)), MIFE_ERR_ARG,
+               "SELL matrix needs perm and n_slices = ceil(n/64)");
+  m->a = MatArgs{A->n, A->format, A->ptr, A->col, A->n_slices};
+  m->perm = A->format == MIFE_FMT_SELL ? A->perm : nullptr;
+  m->L = spmv_lanes(A->nnz, A->n);
+  return MIFE_OK;
 }
 
-// Launches one CG iteration; with profiling on, the two kernels' event tickets
-// are returned through tk[0..1] (else -1).
 template <typename T>
-static void launch_cg_iter(int n, int L, const int32_t* rp, const int32_t* col, const T* vals, T* x, T* r, T* z,
-                           T* p, T* q, const T* dinv, CgState* st, hipStream_t s, int* tk) {
-  const int grid = std::min(spmv_grid(n, L), kRedBlocks);
-  tk[0] = prof::begin(s);
-#define CASE_L(LL) \
-  case LL: cg_spmv_kernel<T, LL><<<grid, 256, 0, s>>>(n, rp, col, vals, z, p, q, st); break;
-  switch (L) { CASE_L(4) CASE_L(8) CASE_L(16) CASE_L(32) default: CASE_L(64) }
-#undef CASE_L
-  prof::end(tk[0], s, sizeof(T) == 8 ? prof::kCgSpmv : prof::kCgSpmv32);
-  tk[1] = prof::begin(s);
-  cg_update_kernel<T><<<vec_grid(n), 256, 0, s>>>(n, x, r, z, p, q, dinv, st);
-  prof::end(tk[1], s, sizeof(T) == 8 ? prof::kCgUpdate : prof::kCgUpdate32);
-}
-
-static void launch_residual64(int n, int L, const int32_t* rp, const int32_t* col, const double* vals,
-                              const double* b, const double* x, double* r, CgState* st, hipStream_t s) {
-  const int grid = std::min(spmv_grid(n, L), kRedBlocks);
+static void launch_cg_init(const Mat& M, const T* vals, const T* b, const T* x, const T* dinv, T* r, T* z, T* p,
+                           T* q, CgState* st, double tol2, int maxit, hipStream_t s) {
   const int tk = prof::begin(s);
-#define CASE_L(LL) \
-  case LL: residual64_kernel<LL><<<grid, 256, 0, s>>>(n, rp, col, vals, b, x, r, st); break;
-  switch (L) { CASE_L(4) CASE_L(8) CASE_L(16) CASE_L(32) default: CASE_L(64) }
-#undef CASE_L
+  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  MIFE_DISPATCH_FMT(M.a, M.L, (cg_init_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, dinv, r, z, p, q,
+                                                                                  st)));
+  fin_init_kernel<<<1, 256, 0, s>>>(st, grid, tol2, maxit);
+  prof::end(tk, s, prof::kCgInit);
+}
+
+// Launches one CG iteration (SpMV kernel + finisher, update kernel + finisher);
+// with profiling on, the two bandwidth kernels' event tickets are returned
+// through tk[0..1] (else -1).
+template <typename T>
+static void launch_cg_iter(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
+                           hipStream_t s, int* tk) {
+  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  const int vg = vec_grid(M.a.n);
+  tk[0] = prof::begin(s);
+  MIFE_DISPATCH_FMT(M.a, M.L, (cg_spmv_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, p, q, st)));
+  prof::end(tk[0], s, sizeof(T) == 8 ? prof::kCgSpmv : prof::kCgSpmv32);
+  fin_alpha_kernel<<<1, 256, 0, s>>>(st, grid);
+  tk[1] = prof::begin(s);
+  cg_update_kernel<T><<<vg, 256, 0, s>>>(M.a.n, x, r, z, p, q, dinv, st);
+  prof::end(tk[1], s, sizeof(T) == 8 ? prof::kCgUpdate : prof::kCgUpdate32);
+  fin_beta_kernel<<<1, 256, 0, s>>>(st, vg);
+}
+
+static void launch_residual64(const Mat& M, const double* vals, const double* b, const double* x, double* r,
+                              CgState* st, hipStream_t s) {
+  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  const int tk = prof::begin(s);
+  MIFE_DISPATCH_FMT(M.a, M.L, (residual64_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, r, st)));
   prof::end(tk, s, prof::kResidual64);
+  fin_residual_kernel<<<1, 256, 0, s>>>(st, grid);
 }
 
 struct HostState {
   double rz, pq, alpha, beta, rr, bb, tol2, scale;
-  int iters, done, maxit, pad;
+  double rz_prev, maxrr, delta2, rr64;
+  int iters, done, maxit, pause;
 };
 
 static int read_state(const CgState* st, HostState* h, hipStream_t s) {
@@ -365,41 +443,64 @@ static int read_state(const CgState* st, HostState* h, hipStream_t s) {
   return MIFE_OK;
 }
 
-static int nnz_of(const int32_t* rp, int n, hipStream_t s, int32_t* nnz) {
-  MIFE_CHECK_HIP(hipMemcpyAsync(nnz, rp + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  MIFE_CHECK_HIP(hipStreamSynchronize(s));
-  return MIFE_OK;
-}
-
-// Runs CG iterations until the device flags convergence; checks every `chunk`.
+// Launches CG iterations in growing chunks until the device state says stop
+// (done, or pause in mixed mode); iterations launched after the stop return at
+// once and are excluded from the profile.
 template <typename T>
-static int run_cg(int n, int L, const int32_t* rp, const int32_t* col, const T* vals, const T* b, T* x,
-                  bool x_zero, double rtol, int maxit, T* r, T* z, T* p, T* q, T* dinv, CgState* st,
-                  hipStream_t s, HostState* hs) {
-  MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
-  diag_inv_kernel<T><<<div_up(n, 256), 256, 0, s>>>(n, rp, col, vals, dinv);
-  launch_cg_init<T>(n, L, rp, col, vals, b, x_zero ? nullptr : x, dinv, r, z, p, q, st, rtol * rtol, maxit, s);
-  MIFE_CHECK_HIP(hipGetLastError());
-  int chunk = 8;
-  int launched = 0;
+static int run_chunks(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
+                      hipStream_t s, HostState* hs, int* chunk) {
   std::vector<int> tickets;
   while (true) {
-    tickets.assign(2 * chunk, -1);
-    for (int k = 0; k < chunk; ++k)
-      launch_cg_iter<T>(n, L, rp, col, vals, x, r, z, p, q, dinv, st, s, &tickets[2 * k]);
+    const int it0 = hs->iters;
+    tickets.assign(2 * *chunk, -1);
+    for (int k = 0; k < *chunk; ++k) launch_cg_iter<T>(M, vals, x, r, z, p, q, dinv, st, s, &tickets[2 * k]);
     MIFE_CHECK_HIP(hipGetLastError());
     int rc = read_state(st, hs, s);
     if (rc) return rc;
-    // iterations at or past the converged count returned at once: not timed
-    for (int k = 0; k < chunk; ++k)
-      if (launched + k >= hs->iters) {
+    for (int k = 0; k < *chunk; ++k)
+      if (it0 + k >= hs->iters) {
         prof::discard(tickets[2 * k]);
         prof::discard(tickets[2 * k + 1]);
       }
-    launched += chunk;
-    if (hs->done || launched >= maxit + chunk) break;
-    chunk = std::min(chunk * 2, 256);
+    if (hs->done || hs->pause) return MIFE_OK;
+    *chunk = std::min(*chunk * 2, 64);
   }
+}
+
+template <typename T>
+static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double rtol, int maxit, Work& w,
+                   hipStream_t s, int* iters, double* relres, bool* converged, const double* vals64_check) {
+  const int n = M.a.n;
+  const int vg = vec_grid(n);
+  CgState* st = w.take<CgState>(1);
+  T *b = w.take<T>(n), *x = w.take<T>(n), *r = w.take<T>(n), *z = w.take<T>(n), *p = w.take<T>(n),
+    *q = w.take<T>(n), *dinv = w.take<T>(n);
+  permute_in_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
+  permute_in_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
+  MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
+  diag_inv_kernel<T><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
+  launch_cg_init<T>(M, vals, b, x, dinv, r, z, p, q, st, rtol * rtol, maxit, s);
+  HostState hs{};
+  int rc = read_state(st, &hs, s);
+  if (rc) return rc;
+  int chunk = 8;
+  if (!hs.done) rc = run_chunks<T>(M, vals, x, r, z, p, q, dinv, st, s, &hs, &chunk);
+  if (rc) return rc;
+  permute_out_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, x, x_io);
+  double rel = hs.bb > 0 ? sqrt(hs.rr / hs.bb) : 0.0;
+  if (vals64_check) {  // explicit FP64 residual at exit
+    CgState* st2 = w.take<CgState>(1);
+    MIFE_CHECK_HIP(hipMemsetAsync(st2, 0, sizeof(CgState), s));
+    launch_residual64(M, vals64_check, (const double*)b, (const double*)x, nullptr, st2, s);
+    HostState h2{};
+    rc = read_state(st2, &h2, s);
+    if (rc) return rc;
+    rel = h2.bb > 0 ? sqrt(h2.rr64 / h2.bb) : 0.0;
+  }
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
+  *iters = hs.iters;
+  *relres = rel;
+  *converged = rel <= rtol;
   return MIFE_OK;
 }
 
@@ -413,119 +514,122 @@ size_t mife_solver_work_bytes(int n, int restart) {
   const size_t nn = (size_t)std::max(n, 1);
   const size_t m = (size_t)std::max(restart, 1) + 1;
   size_t bytes = 0;
-  bytes += 2 * (sizeof(CgState) + 256);
-  bytes += (8 * nn + 256) * (8 + m);            // FP64 vectors incl. GMRES basis
-  bytes += (4 * nn + 256) * 8;                  // FP32 vectors
-  bytes += (8 * (size_t)kRedBlocks * 32 + 256); // GMRES multidot partials
+  bytes += 3 * (sizeof(CgState) + 256);
+  bytes += (8 * nn + 256) * (10 + m);            // FP64 vectors incl. GMRES basis
+  bytes += (4 * nn + 256) * 8;                   // FP32 vectors
+  bytes += (8 * (size_t)kRedBlocks * 32 + 256);  // GMRES multidot partials
   bytes += 8 * 64 * 4 + 65536;
   return bytes;
 }
 
-int mife_cg(int n, const int32_t* row_ptr, const int32_t* col, const void* vals, const void* b, void* x, int prec,
-            double rtol, int maxit, void* work, int* iters, double* relres, void* stream) {
-  MIFE_REQUIRE(row_ptr && col && vals && b && x && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol, int maxit,
+            void* work, int* iters, double* relres, void* stream) {
+  MIFE_REQUIRE(vals && b && x && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
-  MIFE_REQUIRE(n > 0 && maxit >= 0 && rtol >= 0.0, MIFE_ERR_ARG, "invalid CG arguments");
-  hipStream_t s = (hipStream_t)stream;
-  int32_t nnz = 0;
-  int rc = nnz_of(row_ptr, n, s, &nnz);
+  MIFE_REQUIRE(maxit >= 0 && rtol >= 0.0, MIFE_ERR_ARG, "invalid CG arguments");
+  Mat M;
+  int rc = make_mat(A, &M);
   if (rc) return rc;
-  const int L = spmv_lanes(nnz, n);
+  hipStream_t s = (hipStream_t)stream;
   Work w{(char*)work, 0};
-  CgState* st = w.take<CgState>(1);
-  HostState hs{};
-  if (prec == MIFE_F64) {
-    double *r = w.take<double>(n), *z = w.take<double>(n), *p = w.take<double>(n), *q = w.take<double>(n),
-           *dinv = w.take<double>(n);
-    rc = run_cg<double>(n, L, row_ptr, col, (const double*)vals, (const double*)b, (double*)x, false, rtol, maxit,
-                        r, z, p, q, dinv, st, s, &hs);
-    if (rc) return rc;
-    // explicit FP64 residual at exit
-    CgState* st2 = w.take<CgState>(1);
-    MIFE_CHECK_HIP(hipMemsetAsync(st2, 0, sizeof(CgState), s));
-    launch_residual64(n, L, row_ptr, col, (const double*)vals, (const double*)b, (const double*)x, nullptr, st2, s);
-    HostState h2{};
-    rc = read_state(st2, &h2, s);
-    if (rc) return rc;
-    if (iters) *iters = hs.iters;
-    if (relres) *relres = h2.bb > 0 ? sqrt(h2.rr / h2.bb) : 0.0;
-  } else {
-    float *r = w.take<float>(n), *z = w.take<float>(n), *p = w.take<float>(n), *q = w.take<float>(n),
-          *dinv = w.take<float>(n);
-    rc = run_cg<float>(n, L, row_ptr, col, (const float*)vals, (const float*)b, (float*)x, false, rtol, maxit, r, z,
-                       p, q, dinv, st, s, &hs);
-    if (rc) return rc;
-    if (iters) *iters = hs.iters;
-    if (relres) *relres = hs.bb > 0 ? sqrt(hs.rr / hs.bb) : 0.0;
-  }
-  if (hs.iters >= maxit && !(hs.rr <= hs.tol2 * hs.bb)) {
-    set_error("CG reached the maximum of %d iterations", maxit);
+  int it = 0;
+  double rel = 0.0;
+  bool conv = false;
+  if (prec == MIFE_F64)
+    rc = cg_impl<double>(M, (const double*)vals, (const double*)b, (double*)x, rtol, maxit, w, s, &it, &rel, &conv,
+                         (const double*)vals);
+  else
+    rc = cg_impl<float>(M, (const float*)vals, (const float*)b, (float*)x, rtol, maxit, w, s, &it, &rel, &conv,
+                        nullptr);
+  if (rc) return rc;
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  if (!conv) {
+    set_error("CG stopped at relative residual %.3e after %d iterations (maximum %d)", rel, it, maxit);
     return MIFE_ERR_NOT_CONVERGED;
   }
   return MIFE_OK;
 }
 
-int mife_refine(int n, const int32_t* row_ptr, const int32_t* col, const double* vals64, const float* vals32,
-                const double* b, double* x, double rtol, double inner_rtol, int maxit, void* work, int* iters,
-                int* outer, double* relres, void* stream) {
-  MIFE_REQUIRE(row_ptr && col && vals64 && vals32 && b && x && work, MIFE_ERR_POINTER_IS_NULL,
-               "null pointer argument");
-  MIFE_REQUIRE(n > 0 && rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0, MIFE_ERR_ARG, "invalid arguments");
-  hipStream_t s = (hipStream_t)stream;
-  int32_t nnz = 0;
-  int rc = nnz_of(row_ptr, n, s, &nnz);
+int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32, const double* b_in, double* x_io,
+                double rtol, double inner_rtol, int maxit, void* work, int* iters, int* outer, double* relres,
+                void* stream) {
+  MIFE_REQUIRE(vals64 && vals32 && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0 && maxit > 0, MIFE_ERR_ARG, "invalid arguments");
+  Mat M;
+  int rc = make_mat(A, &M);
   if (rc) return rc;
-  const int L = spmv_lanes(nnz, n);
+  hipStream_t s = (hipStream_t)stream;
+  const int n = M.a.n;
   Work w{(char*)work, 0};
   CgState* st = w.take<CgState>(1);
   CgState* sto = w.take<CgState>(1);
-  double* r64 = w.take<double>(n);
-  float *b32 = w.take<float>(n), *d32 = w.take<float>(n), *r = w.take<float>(n), *z = w.take<float>(n),
-        *p = w.take<float>(n), *q = w.take<float>(n), *dinv = w.take<float>(n);
-  HostState hs{}, ho{};
-  int total = 0, nout = 0;
+  double *r64 = w.take<double>(n), *b = w.take<double>(n), *x = w.take<double>(n);
+  float *x32 = w.take<float>(n), *r = w.take<float>(n), *z = w.take<float>(n), *p = w.take<float>(n),
+        *q = w.take<float>(n), *dinv = w.take<float>(n);
+  const int vg = vec_grid(n);
+  permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
+  permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
+  MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
   MIFE_CHECK_HIP(hipMemsetAsync(sto, 0, sizeof(CgState), s));
-  launch_residual64(n, L, row_ptr, col, vals64, b, x, r64, sto, s);
-  rc = read_state(sto, &ho, s);
-  if (rc) return rc;
-  const double bb = ho.bb;
-  double rel = bb > 0 ? sqrt(ho.rr / bb) : 0.0;
-  while (rel > rtol && total < maxit) {
-    scale_to_f32_kernel<<<vec_grid(n), 256, 0, s>>>(n, r64, b32, sto);
-    fill_kernel<float><<<vec_grid(n), 256, 0, s>>>(n, d32, 0.0f);
-    // the inner tolerance never asks for more than the outer target needs
-    const double itol = std::max(inner_rtol, 0.5 * rtol / rel);
-    rc = run_cg<float>(n, L, row_ptr, col, vals32, b32, d32, true, itol, maxit - total, r, z, p, q, dinv, st, s,
-                       &hs);
+  diag_inv_kernel<float><<<div_up(n, 256), 256, 0, s>>>(M.a, vals32, dinv);
+  fill_kernel<float><<<vg, 256, 0, s>>>(n, x32, 0.0f);
+  fill_kernel<float><<<vg, 256, 0, s>>>(n, p, 0.0f);
+  fill_kernel<float><<<vg, 256, 0, s>>>(n, q, 0.0f);
+  HostState init{};
+  init.tol2 = rtol * rtol;
+  init.delta2 = inner_rtol * inner_rtol;
+  init.maxit = maxit;
+  MIFE_CHECK_HIP(hipMemcpyAsync(st, &init, sizeof(HostState), hipMemcpyHostToDevice, s));
+  HostState hs{}, ho{};
+  int chunk = 8, nrep = 0;
+  bool first = true;
+  while (true) {
+    // FP64 replacement: x64 += x32, r64 = b - A64 x64, r32 = r64, z = D^-1 r32
+    if (!first) fold_correction_kernel<<<vg, 256, 0, s>>>(n, x, x32);
+    launch_residual64(M, vals64, b, x, r64, sto, s);
+    replace_residual_kernel<<<vg, 256, 0, s>>>(n, r64, r, z, dinv, st);
+    fin_replace_kernel<<<1, 256, 0, s>>>(st, sto, vg, first ? 1 : 0);
+    MIFE_CHECK_HIP(hipGetLastError());
+    rc = read_state(st, &hs, s);
     if (rc) return rc;
-    total += hs.iters;
-    accumulate_kernel<<<vec_grid(n), 256, 0, s>>>(n, x, d32, sto);
-    launch_residual64(n, L, row_ptr, col, vals64, b, x, r64, sto, s);
     rc = read_state(sto, &ho, s);
     if (rc) return rc;
-    rel = sqrt(ho.rr / bb);
-    ++nout;
-    if (hs.iters == 0) break;
+    first = false;
+    ++nrep;
+    if (hs.done || hs.iters >= maxit) break;
+    rc = run_chunks<float>(M, vals32, x32, r, z, p, q, dinv, st, s, &hs, &chunk);
+    if (rc) return rc;
+    if (hs.done && !hs.pause) {  // iteration cap inside the chunk: fold what was computed
+      fold_correction_kernel<<<vg, 256, 0, s>>>(n, x, x32);
+      launch_residual64(M, vals64, b, x, r64, sto, s);
+      rc = read_state(sto, &ho, s);
+      if (rc) return rc;
+      break;
+    }
   }
-  if (iters) *iters = total;
-  if (outer) *outer = nout;
+  permute_out_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x, x_io);
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
+  const double rel = ho.bb > 0 ? sqrt(ho.rr64 / ho.bb) : 0.0;
+  if (iters) *iters = hs.iters;
+  if (outer) *outer = nrep;
   if (relres) *relres = rel;
   if (rel > rtol) {
-    set_error("mixed-precision refinement stopped at relative residual %.3e", rel);
+    set_error("mixed-precision CG stopped at relative residual %.3e after %d iterations", rel, hs.iters);
     return MIFE_ERR_NOT_CONVERGED;
   }
   return MIFE_OK;
 }
 
-int mife_gmres(int n, const int32_t* row_ptr, const int32_t* col, const double* vals, const double* b, double* x,
-               double rtol, int restart, int maxit, void* work, int* iters, double* relres, void* stream) {
-  MIFE_REQUIRE(row_ptr && col && vals && b && x && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
-  MIFE_REQUIRE(n > 0 && restart >= 1 && restart <= 31 && rtol > 0.0, MIFE_ERR_ARG, "invalid GMRES arguments");
-  hipStream_t s = (hipStream_t)stream;
-  int32_t nnz = 0;
-  int rc = nnz_of(row_ptr, n, s, &nnz);
+int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, double rtol, int restart,
+               int maxit, void* work, int* iters, double* relres, void* stream) {
+  MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(restart >= 1 && restart <= 31 && rtol > 0.0, MIFE_ERR_ARG, "invalid GMRES arguments");
+  Mat M;
+  int rc = make_mat(A, &M);
   if (rc) return rc;
-  const int L = spmv_lanes(nnz, n);
+  hipStream_t s = (hipStream_t)stream;
+  const int n = M.a.n;
   const int m = restart;
   Work w{(char*)work, 0};
   CgState* st = w.take<CgState>(1);
@@ -533,21 +637,25 @@ int mife_gmres(int n, const int32_t* row_ptr, const int32_t* col, const double* 
   double* r = w.take<double>(n);
   double* t = w.take<double>(n);
   double* wv = w.take<double>(n);
+  double* b = w.take<double>(n);
+  double* x = w.take<double>(n);
   const size_t ldv = ((size_t)n + 63) & ~size_t(63);
   double* V = w.take<double>(ldv * (m + 1));
   double* parts = w.take<double>((size_t)kRedBlocks * 32);
   double* hdev = w.take<double>(64);
   double* ydev = w.take<double>(64);
-  diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(n, row_ptr, col, vals, dinv);
+  const int vg = vec_grid(n);
+  permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
+  permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
+  diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
   MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
-  launch_residual64(n, L, row_ptr, col, vals, b, x, r, st, s);
+  launch_residual64(M, vals, b, x, r, st, s);
   HostState hs{};
   rc = read_state(st, &hs, s);
   if (rc) return rc;
   const double bn = sqrt(hs.bb);
-  double beta = sqrt(hs.rr);
+  double beta = sqrt(hs.rr64);
   int it = 0;
-  const int vg = vec_grid(n);
   std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 2);
   while (bn > 0 && beta / bn > rtol && it < maxit) {
     scale_copy_kernel<<<vg, 256, 0, s>>>(n, r, V, 1.0 / beta);
@@ -557,15 +665,15 @@ int mife_gmres(int n, const int32_t* row_ptr, const int32_t* col, const double* 
     int k_used = 0;
     for (int k = 0; k < m; ++k) {
       mul_kernel<<<vg, 256, 0, s>>>(n, dinv, V + k * ldv, t);
-      launch_spmv<double>(n, row_ptr, col, vals, t, wv, L, s);
+      launch_spmv<double>(M.a, vals, t, wv, M.L, s);
       // classical Gram-Schmidt, two passes
       for (int pass = 0; pass < 2; ++pass) {
         multidot_kernel<<<vg, 256, 0, s>>>(n, k + 1, V, ldv, wv, parts);
-        multidot_reduce<<<1, 256, 0, s>>>(k + 1, vg, parts, pass == 0 ? hdev : ydev, 0);
+        multidot_reduce<<<1, 256, 0, s>>>(k + 1, vg, parts, pass == 0 ? hdev : ydev);
         orth_update_kernel<<<vg, 256, 0, s>>>(n, k + 1, V, ldv, pass == 0 ? hdev : ydev, wv);
       }
       multidot_kernel<<<vg, 256, 0, s>>>(n, 1, wv, ldv, wv, parts);
-      multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, hdev + 32, 0);
+      multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, hdev + 32);
       MIFE_CHECK_HIP(hipGetLastError());
       double hh[33], h2[32];
       MIFE_CHECK_HIP(hipMemcpyAsync(hh, hdev, sizeof(double) * 33, hipMemcpyDeviceToHost, s));
@@ -600,11 +708,13 @@ int mife_gmres(int n, const int32_t* row_ptr, const int32_t* col, const double* 
     }
     MIFE_CHECK_HIP(hipMemcpyAsync(ydev, y.data(), sizeof(double) * k_used, hipMemcpyHostToDevice, s));
     gmres_update_kernel<<<vg, 256, 0, s>>>(n, k_used, V, ldv, ydev, dinv, x);
-    launch_residual64(n, L, row_ptr, col, vals, b, x, r, st, s);
+    launch_residual64(M, vals, b, x, r, st, s);
     rc = read_state(st, &hs, s);
     if (rc) return rc;
-    beta = sqrt(hs.rr);
+    beta = sqrt(hs.rr64);
   }
+  permute_out_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x, x_io);
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
   if (iters) *iters = it;
   if (relres) *relres = bn > 0 ? beta / bn : 0.0;
   if (bn > 0 && beta / bn > rtol) {

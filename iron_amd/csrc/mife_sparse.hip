@@ -8,8 +8,41 @@ namespace mife {
 template <typename S, typename D>
 __global__ void gather_kernel(long long nnz, const int32_t* __restrict__ src_pos, const S* __restrict__ src,
                               D* __restrict__ dst) {
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x)
-    dst[k] = (D)src[src_pos[k]];
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x) {
+    const int p = src_pos[k];
+    dst[k] = p >= 0 ? (D)src[p] : (D)0;
+  }
+}
+
+// One wavefront per slice, one row per lane: entry j of slice row k.
+__global__ void sell_build_kernel(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                  const int32_t* __restrict__ src_pos, const int32_t* __restrict__ perm,
+                                  const int32_t* __restrict__ iperm, int n_slices,
+                                  const int32_t* __restrict__ slice_ptr, int32_t* __restrict__ sell_col,
+                                  int32_t* __restrict__ sell_src) {
+  const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= n_slices) return;
+  const int k = s * kSlice + lane;
+  const int base = slice_ptr[s];
+  const int len = (slice_ptr[s + 1] - base) / kSlice;
+  int r0 = 0, rl = 0;
+  if (k < n) {
+    const int row = perm[k];
+    r0 = rp[row];
+    rl = rp[row + 1] - r0;
+  }
+  for (int j = 0; j < len; ++j) {
+    const int e = base + j * kSlice + lane;
+    if (j < rl) {
+      const int p = r0 + j;
+      sell_col[e] = iperm[col[p]];
+      sell_src[e] = src_pos ? src_pos[p] : p;
+    } else {
+      sell_col[e] = k < n ? k : 0;
+      sell_src[e] = -1;
+    }
+  }
 }
 
 // One wavefront per free row: b[f] = src + rhs - sum over fixed columns of K_ij u_j,
@@ -75,20 +108,31 @@ using namespace mife;
 
 extern "C" {
 
-int mife_spmv(int n_rows, const int32_t* row_ptr, const int32_t* col, const void* vals, const void* x, void* y,
-              int prec, void* stream) {
-  MIFE_REQUIRE(row_ptr && col && vals && x && y, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+int mife_spmv(const mife_matrix* A, const void* vals, const void* x, void* y, int prec, void* stream) {
+  MIFE_REQUIRE(A && A->ptr && A->col && vals && x && y, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
-  if (n_rows == 0) return MIFE_OK;
+  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL, MIFE_ERR_ARG, "invalid matrix format");
+  if (A->n == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
-  int32_t nnz = 0;
-  MIFE_CHECK_HIP(hipMemcpyAsync(&nnz, row_ptr + n_rows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  MIFE_CHECK_HIP(hipStreamSynchronize(s));
-  const int L = spmv_lanes(nnz, n_rows);
+  const MatArgs M{A->n, A->format, A->ptr, A->col, A->n_slices};
+  const int L = spmv_lanes(A->nnz, A->n);
   if (prec == MIFE_F64)
-    launch_spmv<double>(n_rows, row_ptr, col, (const double*)vals, (const double*)x, (double*)y, L, s);
+    launch_spmv<double>(M, (const double*)vals, (const double*)x, (double*)y, L, s);
   else
-    launch_spmv<float>(n_rows, row_ptr, col, (const float*)vals, (const float*)x, (float*)y, L, s);
+    launch_spmv<float>(M, (const float*)vals, (const float*)x, (float*)y, L, s);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+int mife_sell_build(int n, const int32_t* row_ptr, const int32_t* col, const int32_t* src_pos, const int32_t* perm,
+                    const int32_t* iperm, int n_slices, const int32_t* slice_ptr, int32_t* sell_col,
+                    int32_t* sell_src, void* stream) {
+  MIFE_REQUIRE(row_ptr && col && perm && iperm && slice_ptr && sell_col && sell_src, MIFE_ERR_POINTER_IS_NULL,
+               "null pointer argument");
+  MIFE_REQUIRE(n_slices == div_up(n, kSlice), MIFE_ERR_ARG, "n_slices must be ceil(n/64)");
+  if (n == 0) return MIFE_OK;
+  sell_build_kernel<<<div_up(n_slices, 4), 256, 0, (hipStream_t)stream>>>(n, row_ptr, col, src_pos, perm, iperm,
+                                                                          n_slices, slice_ptr, sell_col, sell_src);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import ElementDesc, call, ptr
+from ._lib import ElementDesc, MatrixDesc, call, ptr
 from .basis import LagrangeBasis
 
 F64, F32 = _lib.MIFE_F64, _lib.MIFE_F32
@@ -146,6 +146,69 @@ class DirichletReduction:
         call("mife_field_update", self.n_free, ptr(self.free_idx), ptr(x), ptr(u), _stream())
 
 
+class SolverMatrix:
+    """Storage of the reduced (free x free) solver matrix.
+
+    ``fmt="csr"`` keeps iron's solver pattern as CSR; ``fmt="sell"`` (default)
+    stores it in slices of 64 rows, one row per wavefront lane, rows sorted by
+    length inside windows of ``sigma`` rows so a slice is nearly rectangular.
+    Either way the values are gathered straight from the equations-matrix CSR.
+    """
+
+    def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 4096):
+        dev = red.csr.device
+        self.red = red
+        self.n = red.n_free
+        self.fmt = fmt
+        if fmt == "csr":
+            self.src = red.src_pos
+            self.stored = red.nnz
+            self.padding = 0.0
+            self.desc = MatrixDesc(self.n, _lib.MIFE_FMT_CSR, ptr(red.row_ptr), ptr(red.col), None, 0, red.nnz)
+            return
+        if fmt != "sell":
+            raise ValueError(f"unknown solver matrix format {fmt!r}")
+        n = self.n
+        lengths = np.diff(red.row_ptr.cpu().numpy().astype(np.int64))
+        idx = np.arange(n)
+        perm = np.lexsort((idx, -lengths, idx // sigma)).astype(np.int32)
+        iperm = np.empty(n, dtype=np.int32)
+        iperm[perm] = np.arange(n, dtype=np.int32)
+        n_slices = (n + 63) // 64
+        lp = np.zeros(n_slices * 64, dtype=np.int64)
+        lp[:n] = lengths[perm]
+        slice_len = lp.reshape(n_slices, 64).max(axis=1)
+        slice_ptr = np.zeros(n_slices + 1, dtype=np.int64)
+        slice_ptr[1:] = np.cumsum(slice_len * 64)
+        if slice_ptr[-1] >= 2 ** 31:
+            raise ValueError("SELL storage exceeds the int32 index range")
+        self.stored = int(slice_ptr[-1])
+        self.padding = self.stored / max(red.nnz, 1) - 1.0
+        self.perm = torch.as_tensor(perm, device=dev)
+        self.iperm = torch.as_tensor(iperm, device=dev)
+        self.slice_ptr = torch.as_tensor(slice_ptr.astype(np.int32), device=dev)
+        self.col = torch.empty(max(self.stored, 1), dtype=torch.int32, device=dev)
+        self.src = torch.empty(max(self.stored, 1), dtype=torch.int32, device=dev)
+        call("mife_sell_build", n, ptr(red.row_ptr), ptr(red.col), ptr(red.src_pos), ptr(self.perm), ptr(self.iperm),
+             n_slices, ptr(self.slice_ptr), ptr(self.col), ptr(self.src), _stream())
+        self.desc = MatrixDesc(n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), ptr(self.col), ptr(self.perm), n_slices,
+                               self.stored)
+
+    def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Solver-matrix values (this storage) from equations-CSR values (FP64 or FP32)."""
+        if out is None:
+            out = torch.empty(max(self.stored, 1), dtype=torch.float64 if prec == F64 else torch.float32,
+                              device=vals.device)
+        src_prec = F64 if vals.dtype == torch.float64 else F32
+        call("mife_gather_values", self.stored, ptr(self.src), ptr(vals), src_prec, ptr(out), prec, _stream())
+        return out
+
+    def spmv(self, vals, x, y, prec=F64):
+        """y = A x in this storage's row numbering (SELL: slice numbering)."""
+        call("mife_spmv", C.byref(self.desc), ptr(vals), ptr(x), ptr(y), prec, _stream())
+        return y
+
+
 @dataclass
 class SolveInfo:
     iterations: int
@@ -160,30 +223,27 @@ class KrylovWorkspace:
         self.buf = torch.empty(int(_lib.lib().mife_solver_work_bytes(n, restart)), dtype=torch.uint8, device=device)
 
 
-def cg(n, row_ptr, col, vals, b, x, rtol, maxit, ws: KrylovWorkspace, prec=F64) -> SolveInfo:
+def cg(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, prec=F64) -> SolveInfo:
+    """Jacobi CG on A (b, x in the original free-DOF numbering)."""
     it, rr = C.c_int(0), C.c_double(0.0)
-    call("mife_cg", n, ptr(row_ptr), ptr(col), ptr(vals), ptr(b), ptr(x), prec, float(rtol), int(maxit),
-         ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
+    call("mife_cg", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), prec, float(rtol), int(maxit), ptr(ws.buf),
+         C.byref(it), C.byref(rr), _stream())
     return SolveInfo(it.value, rr.value)
 
 
-def refine(n, row_ptr, col, vals64, vals32, b, x, rtol, maxit, ws: KrylovWorkspace, inner_rtol=1e-4) -> SolveInfo:
+def refine(A: SolverMatrix, vals64, vals32, b, x, rtol, maxit, ws: KrylovWorkspace, inner_rtol=0.1) -> SolveInfo:
+    """FP32 Jacobi CG with FP64 residual replacement every ``inner_rtol`` reduction."""
     it, outer, rr = C.c_int(0), C.c_int(0), C.c_double(0.0)
-    call("mife_refine", n, ptr(row_ptr), ptr(col), ptr(vals64), ptr(vals32), ptr(b), ptr(x), float(rtol),
-         float(inner_rtol), int(maxit), ptr(ws.buf), C.byref(it), C.byref(outer), C.byref(rr), _stream())
+    call("mife_refine", C.byref(A.desc), ptr(vals64), ptr(vals32), ptr(b), ptr(x), float(rtol), float(inner_rtol),
+         int(maxit), ptr(ws.buf), C.byref(it), C.byref(outer), C.byref(rr), _stream())
     return SolveInfo(it.value, rr.value, outer.value)
 
 
-def gmres(n, row_ptr, col, vals, b, x, rtol, maxit, ws: KrylovWorkspace, restart=30) -> SolveInfo:
+def gmres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, restart=30) -> SolveInfo:
     it, rr = C.c_int(0), C.c_double(0.0)
-    call("mife_gmres", n, ptr(row_ptr), ptr(col), ptr(vals), ptr(b), ptr(x), float(rtol), int(restart),
-         int(maxit), ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
+    call("mife_gmres", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), float(rtol), int(restart), int(maxit),
+         ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
     return SolveInfo(it.value, rr.value)
-
-
-def spmv(row_ptr, col, vals, x, y, prec=F64):
-    call("mife_spmv", len(row_ptr) - 1, ptr(row_ptr), ptr(col), ptr(vals), ptr(x), ptr(y), prec, _stream())
-    return y
 
 
 def backsubstitute(csr: CSRStructure, vals64, u, src=None, rhs_fixed=None, iron_compat=False, out=None):

@@ -156,27 +156,49 @@ def test_reduction_rhs_and_cg_fp64_against_oracle():
     sref = fo.assemble_vector(od.element_dofs(), fo.poisson_source_vectors(xe, ob, 1.0), od.num_dofs)
     free, srp, scol, sv, bref = fo.reduce_dirichlet(rp, col, vals, fixed, ufix, sref)
     assert np.array_equal(red.row_ptr.cpu().numpy(), srp) and np.array_equal(red.col.cpu().numpy()[: len(scol)], scol)
-    sv_gpu = red.gather(v64, fem.F64)
+    sv_gpu = fem.SolverMatrix(red, "csr").gather(v64, fem.F64)
     assert _rel(sv_gpu.cpu().numpy()[: len(sv)], sv) < 1e-13
     assert _rel(b.cpu().numpy(), bref) < 1e-13
     ws = fem.KrylovWorkspace(red.n_free, "cuda")
-    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
-    info = fem.cg(red.n_free, red.row_ptr, red.col, sv_gpu, b, x, 1e-10, 10000, ws, prec=fem.F64)
-    assert info.relres <= 1e-10
     xo, _, _ = fo.cg_jacobi(srp, scol, sv, bref, rtol=1e-13)
-    assert _rel(x.cpu().numpy(), xo) < 1e-8
-    # mixed precision: FP32 inner CG, FP64 refinement to 1e-8 of the FP64 system
-    sv32 = red.gather(v64, fem.F32)
-    xm = torch.zeros_like(x)
-    info_m = fem.refine(red.n_free, red.row_ptr, red.col, sv_gpu, sv32, b, xm, 1e-8, 10000, ws)
     A = fo._csr_scipy(srp, scol, sv)
-    assert np.linalg.norm(bref - A @ xm.cpu().numpy()) / np.linalg.norm(bref) <= 1e-8
-    assert info_m.relres <= 1e-8 and info_m.outer >= 1
-    # GMRES(30) + Jacobi
-    xg = torch.zeros_like(x)
-    info_g = fem.gmres(red.n_free, red.row_ptr, red.col, sv_gpu, b, xg, 1e-10, 20000, ws)
-    assert info_g.relres <= 1e-10
-    assert _rel(xg.cpu().numpy(), xo) < 1e-7
+    for fmt in ("csr", "sell"):
+        S = fem.SolverMatrix(red, fmt)
+        s64 = S.gather(v64, fem.F64)
+        x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+        info = fem.cg(S, s64, b, x, 1e-10, 10000, ws, prec=fem.F64)
+        assert info.relres <= 1e-10, fmt
+        assert _rel(x.cpu().numpy(), xo) < 1e-8, fmt
+        # mixed precision: FP32 CG with FP64 residual replacement, to 1e-8 of the FP64 system
+        s32 = S.gather(v32, fem.F32)
+        xm = torch.zeros_like(x)
+        info_m = fem.refine(S, s64, s32, b, xm, 1e-8, 10000, ws)
+        assert np.linalg.norm(bref - A @ xm.cpu().numpy()) / np.linalg.norm(bref) <= 1e-8, fmt
+        assert info_m.relres <= 1e-8 and info_m.outer >= 2, fmt
+        # GMRES(30) + Jacobi
+        xg = torch.zeros_like(x)
+        info_g = fem.gmres(S, s64, b, xg, 1e-10, 20000, ws)
+        assert info_g.relres <= 1e-10, fmt
+        assert _rel(xg.cpu().numpy(), xo) < 1e-7, fmt
+
+
+def test_sell_storage_matches_csr_product():
+    """SELL-64 values/columns reproduce the CSR solver matrix product exactly in FP64."""
+    fem, _, _ = _fem()
+    pb, pm, pd, ob, om, od, kappa, fixed = _poisson_system((5, 4, 3), 2, seed=5)
+    fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, kappa, fixed, np.zeros(pd.num_dofs))
+    Sc = fem.SolverMatrix(red, "csr")
+    Ss = fem.SolverMatrix(red, "sell", sigma=128)
+    vc = Sc.gather(v64, fem.F64)
+    vs = Ss.gather(v64, fem.F64)
+    xv = torch.rand(red.n_free, dtype=torch.float64, device="cuda")
+    yc = Sc.spmv(vc, xv, torch.empty_like(xv))
+    perm = Ss.perm.long()
+    ys = Ss.spmv(vs, xv[perm].contiguous(), torch.empty_like(xv))
+    y_back = torch.empty_like(xv)
+    y_back[perm] = ys
+    assert _rel(y_back.cpu().numpy(), yc.cpu().numpy()) < 1e-14
+    assert Ss.stored >= red.nnz and Ss.stored % 64 == 0
 
 
 def test_unit_cube_laplace_u_equals_x_and_reactions():
@@ -186,10 +208,11 @@ def test_unit_cube_laplace_u_equals_x_and_reactions():
     fixed = (np.abs(X[:, 0]) < 1e-12) | (np.abs(X[:, 0] - 1.0) < 1e-12)
     ufix = np.where(np.abs(X[:, 0] - 1.0) < 1e-12, 1.0, 0.0)
     fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, np.ones(pm.num_elements), fixed, ufix, source=0.0)
-    sv = red.gather(v64, fem.F64)
+    S = fem.SolverMatrix(red)
+    sv = S.gather(v64, fem.F64)
     ws = fem.KrylovWorkspace(red.n_free, "cuda")
     x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
-    info = fem.cg(red.n_free, red.row_ptr, red.col, sv, b, x, 1e-10, 10000, ws)
+    info = fem.cg(S, sv, b, x, 1e-10, 10000, ws)
     red.field_update(x, u)
     un = u.cpu().numpy()
     assert np.max(np.abs(un - X[:, 0])) < 1e-8 and info.relres <= 1e-10
@@ -211,10 +234,11 @@ def test_golden_analytic_laplace_through_gpu():
     fixed = np.ones(9, bool)
     fixed[4] = False
     fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, np.ones(4), fixed, ua, source=0.0)
-    sv = red.gather(v64, fem.F64)
+    S = fem.SolverMatrix(red)
+    sv = S.gather(v64, fem.F64)
     ws = fem.KrylovWorkspace(red.n_free, "cuda")
     x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
-    fem.cg(red.n_free, red.row_ptr, red.col, sv, b, x, 1e-14, 100, ws)
+    fem.cg(S, sv, b, x, 1e-14, 100, ws)
     red.field_update(x, u)
     un = u.cpu().numpy()
     for node, (num, _) in data["nodal"]["Phi"].items():
