@@ -16,6 +16,8 @@
 #include "mife_profile.hpp"
 #include "mife_sparse.hpp"
 
+extern "C" int mife_profile_enable(int on);
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -23,6 +25,7 @@
 namespace mife {
 
 constexpr int kRedBlocks = kMaxGrid;
+constexpr int kFinThreads = 1024;  // one-block finishers: two partials per thread
 
 struct CgState {
   double rz, pq, alpha, beta, rr, bb, tol2, scale;
@@ -59,7 +62,7 @@ __device__ __forceinline__ void block_partials(double (&mine)[NV], double* parts
 // In a one-block finisher: fixed-order sums of a producer's nb partials.
 template <int NV>
 __device__ __forceinline__ void finish_sums(const double* __restrict__ parts, int nb, double (&out)[NV]) {
-  __shared__ double red[NV][8];
+  __shared__ double red[NV][16];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = 0.0;
@@ -167,9 +170,6 @@ __global__ void fin_alpha_kernel(CgState* st, int nb) {
   }
 }
 
-// With delta2 > 0 (mixed precision) the kernel raises `pause` once the FP32
-// residual has fallen by delta since the last FP64 replacement, or below the
-// target; the host then replaces it with the FP64 residual.
 template <typename T>
 __global__ __launch_bounds__(256) void cg_update_kernel(int n, T* __restrict__ x, T* __restrict__ r,
                                                         T* __restrict__ z, const T* __restrict__ p,
@@ -201,12 +201,7 @@ __global__ void fin_beta_kernel(CgState* st, int nb) {
     st->rr = out[1];
     st->iters += 1;
     st->maxrr = fmax(st->maxrr, out[1]);
-    if (st->delta2 > 0.0) {
-      if (out[1] <= st->delta2 * st->maxrr || out[1] <= st->tol2 * st->bb) st->pause = 1;
-      if (st->iters >= st->maxit) st->done = 1;
-    } else {
-      st->done = (out[1] <= st->tol2 * st->bb || st->iters >= st->maxit) ? 1 : 0;
-    }
+    st->done = (out[1] <= st->tol2 * st->bb || st->iters >= st->maxit) ? 1 : 0;
   }
 }
 
@@ -278,7 +273,7 @@ __global__ __launch_bounds__(256) void replace_residual_kernel(int n, const doub
   block_partials<2>(m, st->part_b);
 }
 
-__global__ void fin_replace_kernel(CgState* st, const CgState* ro, int nb, int first) {
+__global__ void fin_replace_kernel(CgState* st, const CgState* ro, int nb, int first, double tol2) {
   double out[2];
   finish_sums<2>(st->part_b, nb, out);
   if (threadIdx.x == 0) {
@@ -288,7 +283,7 @@ __global__ void fin_replace_kernel(CgState* st, const CgState* ro, int nb, int f
     st->maxrr = out[1];
     st->bb = ro->bb;
     st->pause = 0;
-    st->done = (ro->rr64 <= st->tol2 * ro->bb) ? 1 : 0;
+    st->done = (ro->rr64 <= tol2 * ro->bb) ? 1 : 0;
   }
 }
 
@@ -400,7 +395,7 @@ static void launch_cg_init(const Mat& M, const T* vals, const T* b, const T* x, 
   const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
   MIFE_DISPATCH_FMT(M.a, M.L, (cg_init_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, dinv, r, z, p, q,
                                                                                   st)));
-  fin_init_kernel<<<1, 256, 0, s>>>(st, grid, tol2, maxit);
+  fin_init_kernel<<<1, kFinThreads, 0, s>>>(st, grid, tol2, maxit);
   prof::end(tk, s, prof::kCgInit);
 }
 
@@ -415,11 +410,11 @@ static void launch_cg_iter(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, 
   tk[0] = prof::begin(s);
   MIFE_DISPATCH_FMT(M.a, M.L, (cg_spmv_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, p, q, st)));
   prof::end(tk[0], s, sizeof(T) == 8 ? prof::kCgSpmv : prof::kCgSpmv32);
-  fin_alpha_kernel<<<1, 256, 0, s>>>(st, grid);
+  fin_alpha_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
   tk[1] = prof::begin(s);
   cg_update_kernel<T><<<vg, 256, 0, s>>>(M.a.n, x, r, z, p, q, dinv, st);
   prof::end(tk[1], s, sizeof(T) == 8 ? prof::kCgUpdate : prof::kCgUpdate32);
-  fin_beta_kernel<<<1, 256, 0, s>>>(st, vg);
+  fin_beta_kernel<<<1, kFinThreads, 0, s>>>(st, vg);
 }
 
 static void launch_residual64(const Mat& M, const double* vals, const double* b, const double* x, double* r,
@@ -428,7 +423,7 @@ static void launch_residual64(const Mat& M, const double* vals, const double* b,
   const int tk = prof::begin(s);
   MIFE_DISPATCH_FMT(M.a, M.L, (residual64_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, r, st)));
   prof::end(tk, s, prof::kResidual64);
-  fin_residual_kernel<<<1, 256, 0, s>>>(st, grid);
+  fin_residual_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
 }
 
 struct HostState {
@@ -443,28 +438,57 @@ static int read_state(const CgState* st, HostState* h, hipStream_t s) {
   return MIFE_OK;
 }
 
-// Launches CG iterations in growing chunks until the device state says stop
-// (done, or pause in mixed mode); iterations launched after the stop return at
-// once and are excluded from the profile.
+// Launches exactly `count` CG iterations (iterations after a device-side stop
+// return at once).  With profiling on, one iteration in 8 carries event pairs,
+// which keeps the timing overhead off the solve.
 template <typename T>
-static int run_chunks(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
-                      hipStream_t s, HostState* hs, int* chunk) {
-  std::vector<int> tickets;
-  while (true) {
-    const int it0 = hs->iters;
-    tickets.assign(2 * *chunk, -1);
-    for (int k = 0; k < *chunk; ++k) launch_cg_iter<T>(M, vals, x, r, z, p, q, dinv, st, s, &tickets[2 * k]);
-    MIFE_CHECK_HIP(hipGetLastError());
-    int rc = read_state(st, hs, s);
-    if (rc) return rc;
-    for (int k = 0; k < *chunk; ++k)
-      if (it0 + k >= hs->iters) {
-        prof::discard(tickets[2 * k]);
-        prof::discard(tickets[2 * k + 1]);
-      }
-    if (hs->done || hs->pause) return MIFE_OK;
-    *chunk = std::min(*chunk * 2, 64);
+static int run_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
+                     hipStream_t s, HostState* hs, int count) {
+  std::vector<int> tickets(2 * count, -1);
+  const int it0 = hs->iters;
+  const bool on = prof::enabled();
+  for (int k = 0; k < count; ++k) {
+    const bool sample = on && ((it0 + k) % 8) == 0;
+    if (on && !sample) mife_profile_enable(0);
+    launch_cg_iter<T>(M, vals, x, r, z, p, q, dinv, st, s, &tickets[2 * k]);
+    if (on && !sample) mife_profile_enable(1);
   }
+  MIFE_CHECK_HIP(hipGetLastError());
+  int rc = read_state(st, hs, s);
+  if (rc) return rc;
+  for (int k = 0; k < count; ++k)
+    if (it0 + k >= hs->iters) {
+      prof::discard(tickets[2 * k]);
+      prof::discard(tickets[2 * k + 1]);
+    }
+  return MIFE_OK;
+}
+
+// Iterations the CG residual needs to fall from rr to target at the rate
+// (log reduction per iteration, < 0) seen so far; clamped to [lo, hi].
+static int predict_iters(double rr, double target, double log_rate, int lo, int hi) {
+  if (!(log_rate < 0.0) || !(rr > target)) return lo;
+  const double need = std::log(target / rr) / log_rate;
+  return (int)std::min((double)hi, std::max((double)lo, std::ceil(need)));
+}
+
+// Runs CG until the device state says done, in chunks sized from the observed
+// convergence rate so that few launches land after convergence.
+template <typename T>
+static int run_to_convergence(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv,
+                              CgState* st, hipStream_t s, HostState* hs) {
+  int chunk = 16;
+  double log_rate = 0.0;
+  while (!hs->done) {
+    const int it0 = hs->iters;
+    const double rr0 = hs->rr;
+    int rc = run_iters<T>(M, vals, x, r, z, p, q, dinv, st, s, hs, chunk);
+    if (rc) return rc;
+    const int ran = hs->iters - it0;
+    if (ran > 0 && rr0 > 0.0 && hs->rr > 0.0 && hs->rr < rr0) log_rate = std::log(hs->rr / rr0) / ran;
+    chunk = predict_iters(hs->rr, hs->tol2 * hs->bb, log_rate, 2, 64);
+  }
+  return MIFE_OK;
 }
 
 template <typename T>
@@ -483,8 +507,7 @@ static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double r
   HostState hs{};
   int rc = read_state(st, &hs, s);
   if (rc) return rc;
-  int chunk = 8;
-  if (!hs.done) rc = run_chunks<T>(M, vals, x, r, z, p, q, dinv, st, s, &hs, &chunk);
+  if (!hs.done) rc = run_to_convergence<T>(M, vals, x, r, z, p, q, dinv, st, s, &hs);
   if (rc) return rc;
   permute_out_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, x, x_io);
   double rel = hs.bb > 0 ? sqrt(hs.rr / hs.bb) : 0.0;
@@ -576,20 +599,25 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
   fill_kernel<float><<<vg, 256, 0, s>>>(n, x32, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, p, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, q, 0.0f);
+  // the inner FP32 recurrence never stops on its own (tol2 = 0 on the device):
+  // the host sizes every segment and the FP64 residual decides convergence
   HostState init{};
-  init.tol2 = rtol * rtol;
-  init.delta2 = inner_rtol * inner_rtol;
+  init.tol2 = 0.0;
   init.maxit = maxit;
+  const double tol2 = rtol * rtol;
   MIFE_CHECK_HIP(hipMemcpyAsync(st, &init, sizeof(HostState), hipMemcpyHostToDevice, s));
+  const double delta2 = inner_rtol * inner_rtol;  // FP32 reduction per segment
   HostState hs{}, ho{};
-  int chunk = 8, nrep = 0;
+  int nrep = 0;
   bool first = true;
+  double log_rate = 0.0;
+  int seg = 16;
   while (true) {
     // FP64 replacement: x64 += x32, r64 = b - A64 x64, r32 = r64, z = D^-1 r32
     if (!first) fold_correction_kernel<<<vg, 256, 0, s>>>(n, x, x32);
     launch_residual64(M, vals64, b, x, r64, sto, s);
     replace_residual_kernel<<<vg, 256, 0, s>>>(n, r64, r, z, dinv, st);
-    fin_replace_kernel<<<1, 256, 0, s>>>(st, sto, vg, first ? 1 : 0);
+    fin_replace_kernel<<<1, kFinThreads, 0, s>>>(st, sto, vg, first ? 1 : 0, tol2);
     MIFE_CHECK_HIP(hipGetLastError());
     rc = read_state(st, &hs, s);
     if (rc) return rc;
@@ -598,15 +626,19 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
     first = false;
     ++nrep;
     if (hs.done || hs.iters >= maxit) break;
-    rc = run_chunks<float>(M, vals32, x32, r, z, p, q, dinv, st, s, &hs, &chunk);
-    if (rc) return rc;
-    if (hs.done && !hs.pause) {  // iteration cap inside the chunk: fold what was computed
-      fold_correction_kernel<<<vg, 256, 0, s>>>(n, x, x32);
-      launch_residual64(M, vals64, b, x, r64, sto, s);
-      rc = read_state(sto, &ho, s);
-      if (rc) return rc;
-      break;
+    // the next segment: until the FP32 residual has fallen by inner_rtol, or to
+    // a quarter of the target when that is nearer
+    if (nrep > 1) {
+      const double target = std::max(delta2 * hs.rr, 0.25 * tol2 * hs.bb);
+      seg = predict_iters(hs.rr, target, log_rate, 4, 256);
     }
+    seg = std::min(seg, maxit - hs.iters);
+    const int it0 = hs.iters;
+    const double rr0 = hs.rr;
+    rc = run_iters<float>(M, vals32, x32, r, z, p, q, dinv, st, s, &hs, seg);
+    if (rc) return rc;
+    const int ran = hs.iters - it0;
+    if (ran > 0 && rr0 > 0.0 && hs.rr > 0.0 && hs.rr < rr0) log_rate = std::log(hs.rr / rr0) / ran;
   }
   permute_out_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x, x_io);
   MIFE_CHECK_HIP(hipStreamSynchronize(s));

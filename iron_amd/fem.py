@@ -155,7 +155,7 @@ class SolverMatrix:
     Either way the values are gathered straight from the equations-matrix CSR.
     """
 
-    def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 4096):
+    def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 256):
         dev = red.csr.device
         self.red = red
         self.n = red.n_free
