@@ -198,12 +198,15 @@ int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int
 
 /* Solver RHS over the free rows (SOLVER_MATRICES_STATIC_ASSEMBLE RHS part,
  * src/solver_routines.F90:14640-14850):
- *   b[f] = src[i] + rhs[i] - sum_{j fixed, |u_j|>=ZT, |K_ij|>=ZT} K_ij u_j,  i = free_idx[f]
- * src, rhs may be NULL.  ZT = 5*DBL_EPSILON (src/constants.F90:69). */
+ *   b[f] = src[i] + rhs[i] - lift_i,  i = free_idx[f],
+ *   lift_i = sum_{j fixed, |u_j|>=ZT, |K_ij|>=ZT} K_ij u_j for rows whose RHS DOF is
+ *   free; rows with rhs_fixed[i] != 0 (a Neumann or fixed condition on the RHS
+ *   variable) get no lifting, as in the reference (:14850).
+ * src, rhs, rhs_fixed may be NULL.  ZT = 5*DBL_EPSILON (src/constants.F90:69). */
 int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr,
                     const int32_t* col, const double* vals, const int32_t* new_idx,
                     const double* u, const double* src, const double* rhs,
-                    double* b, void* stream);
+                    const uint8_t* rhs_fixed, double* b, void* stream);
 
 /* u[free_idx[f]] = x[f] (SOLVER_VARIABLES_FIELD_UPDATE, src/solver_routines.F90:22814-23001). */
 int mife_field_update(int n_free, const int32_t* free_idx, const double* x,

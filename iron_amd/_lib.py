@@ -66,7 +66,7 @@ _SIG = {
     "mife_spmv": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _P]),
     "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
-    "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mife_field_update": (_I, [_I, _P, _P, _P, _P]),
     "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "mife_solver_work_bytes": (C.c_size_t, [_I, _I]),

@@ -51,14 +51,16 @@ __global__ void solver_rhs_kernel(int n_free, const int32_t* __restrict__ free_i
                                   const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
                                   const double* __restrict__ vals, const int32_t* __restrict__ new_idx,
                                   const double* __restrict__ u, const double* __restrict__ src,
-                                  const double* __restrict__ rhs, double* __restrict__ b) {
+                                  const double* __restrict__ rhs, const uint8_t* __restrict__ rhs_fixed,
+                                  double* __restrict__ b) {
   const double ZT = 5.0 * 2.220446049250313e-16;
   const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (f >= n_free) return;
   const int i = free_idx[f];
   double lift = 0.0;
-  for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64) {
+  const bool lifted = !(rhs_fixed && rhs_fixed[i]);
+  for (int k = row_ptr[i] + lane; lifted && k < row_ptr[i + 1]; k += 64) {
     const int j = col[k];
     if (new_idx[j] < 0) {
       const double uj = u[j], kv = vals[k];
@@ -159,12 +161,13 @@ int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int
 
 int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr, const int32_t* col,
                     const double* vals, const int32_t* new_idx, const double* u, const double* src,
-                    const double* rhs, double* b, void* stream) {
+                    const double* rhs, const uint8_t* rhs_fixed, double* b, void* stream) {
   MIFE_REQUIRE(free_idx && row_ptr && col && vals && new_idx && u && b, MIFE_ERR_POINTER_IS_NULL,
                "null pointer argument");
   if (n_free == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
-  solver_rhs_kernel<<<div_up(n_free, 4), 256, 0, s>>>(n_free, free_idx, row_ptr, col, vals, new_idx, u, src, rhs, b);
+  solver_rhs_kernel<<<div_up(n_free, 4), 256, 0, s>>>(n_free, free_idx, row_ptr, col, vals, new_idx, u, src, rhs,
+                                                      rhs_fixed, b);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }

@@ -1,0 +1,280 @@
+"""Problem.Solve() for the iron-compatible API: maps an equations set, its
+fields, boundary conditions and solver settings onto the device engine.
+
+Static linear solve, following PROBLEM_SOLVER_EQUATIONS_SOLVE ->
+Problem_SolverEquationsStaticLinearSolve (problem_routines.F90:2533):
+  assemble (EQUATIONS_SET_ASSEMBLE) -> solver matrix + RHS
+  (SOLVER_MATRICES_STATIC_ASSEMBLE) -> KSP -> field update -> back-substitution
+  (EQUATIONS_SET_BACKSUBSTITUTE, problem_routines.F90:2597).
+Device structures (CSR pattern, scatter map, reduction, solver matrix) are
+cached on the equations set and reused while the mesh and the set of fixed
+DOFs stay the same.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import fem
+from ._lib import CMFEError
+from .basis import LagrangeBasis, face_quadrature
+from .mesh import DofMap, RegularMesh, structured_colours
+
+
+def _iron():
+    from . import iron
+    return iron
+
+
+# ------------------------------------------------------------ Neumann
+def boundary_faces(rm: RegularMesh):
+    """Boundary faces of a regular mesh: (element, xi direction, side 0/1)."""
+    eidx = np.array(np.unravel_index(np.arange(rm.num_elements), rm.n_elem, order="F")).T
+    out = []
+    for k in range(rm.dim):
+        for side in (0, 1):
+            sel = eidx[:, k] == (0 if side == 0 else rm.n_elem[k] - 1)
+            for e in np.nonzero(sel)[0]:
+                out.append((int(e), k, side))
+    return out
+
+
+def neumann_integrate(rm: RegularMesh, dm: DofMap, comp_basis, point_values: dict, geom_comp=0,
+                      integrated_only=frozenset()):
+    """BoundaryConditions_NeumannIntegrate (boundary_condition_routines.F90:2654-2996),
+    3-D node-based case: every NEUMANN_POINT DOF o with value q_o is integrated over
+    every boundary face that contains its node, f_m += q_o * int_face phi_m phi_o dA,
+    for the face nodes m of the same component; a face holding a
+    NEUMANN_INTEGRATED_ONLY DOF of that component is skipped (:2860-2862).
+    ``point_values``: {dof: q}; ``integrated_only``: set of DOFs."""
+    if not point_values:
+        return np.zeros(dm.num_dofs)
+    f = np.zeros(dm.num_dofs)
+    coords = rm.coords
+    by_comp = {}
+    for dof, q in point_values.items():
+        c = int(dm.component_of(np.array([dof]))[0])
+        by_comp.setdefault(c, {})[int(dof)] = float(q)
+    for c, vals in by_comp.items():
+        mc = dm.comp_mesh[c]
+        b: LagrangeBasis = comp_basis[c]
+        en = rm.elem_nodes[mc]
+        geo_en = rm.elem_nodes[geom_comp]
+        last = b.nodes_xi - 1
+        for (e, k, side) in boundary_faces(rm):
+            fmask = b.node_index[:, k] == (0 if side == 0 else last)
+            fnodes = np.nonzero(fmask)[0]
+            dofs = dm.offsets[c] + en[e, fnodes]
+            qs = np.array([vals.get(int(d), None) for d in dofs], dtype=object)
+            has = np.array([q is not None for q in qs])
+            if not has.any():
+                continue
+            if integrated_only and any(int(d) in integrated_only for d in dofs):
+                continue
+            N, dN, wf = face_quadrature(b, k, side)
+            # geometry interpolated with the geometric basis of mesh component 1
+            gb = rm.bases[geom_comp]
+            Ng, dNg, _ = face_quadrature(gb, k, side)
+            xe = coords[geo_en[e]]
+            tang = [j for j in range(rm.dim) if j != k]
+            dx = np.einsum("gaj,ad->gdj", dNg[:, :, tang], xe)  # [ng, dim, 2]
+            gl = np.einsum("gdi,gdj->gij", dx, dx)
+            jac = np.sqrt(np.abs(gl[:, 0, 0] * gl[:, 1, 1] - gl[:, 0, 1] * gl[:, 1, 0]))
+            Mf = np.einsum("ga,gb,g->ab", N[:, fnodes], N[:, fnodes], wf * jac)
+            qv = np.array([q if q is not None else 0.0 for q in qs], dtype=np.float64)
+            f_local = Mf[:, has] @ qv[has]
+            np.add.at(f, dofs, f_local)
+    return f
+
+
+# ------------------------------------------------------------ helpers
+def _dof_map(field, rm: RegularMesh):
+    iron = _iron()
+    v = field._var(iron.FieldVariableTypes.U)
+    return DofMap(rm, [mc - 1 for mc in v.mesh_comp])
+
+
+def _element_values(field, comp: int, rm: RegularMesh, vt=None):
+    """Per-element value of a field component (element-based or constant)."""
+    iron = _iron()
+    vt = vt or iron.FieldVariableTypes.U
+    var = field._var(vt)
+    it = var.interp[comp - 1]
+    vals = field._component(vt, comp)
+    if it == iron.FieldInterpolationTypes.ELEMENT_BASED:
+        return np.ascontiguousarray(vals, dtype=np.float64)
+    if it == iron.FieldInterpolationTypes.CONSTANT:
+        return np.full(rm.num_elements, float(vals[0]))
+    raise CMFEError(1, "node-based material parameters are not supported by the element kernels")
+
+
+def _boundary_conditions(bc, es, dm: DofMap, rm: RegularMesh):
+    iron = _iron()
+    dep = es.dependentField
+    n = dm.num_dofs
+    fixed = np.zeros(n, dtype=bool)
+    rhs_fixed = np.zeros(n, dtype=bool)
+    u = dep._data(iron.FieldVariableTypes.U).copy()
+    neumann = {}
+    increments = {}
+    integrated_only = set()
+    for (field, vt, node, comp, cond, value, add) in bc._entries:
+        if field is not dep and field.__dict__ is not dep.__dict__:
+            raise CMFEError(1, "boundary conditions must be set on the dependent field")
+        mc = dm.comp_mesh[comp - 1]
+        dof = int(dm.offsets[comp - 1] + rm.local_node(node, mc))
+        if vt == iron.FieldVariableTypes.U:
+            if cond in (iron.BoundaryConditionsTypes.FIXED, iron.BoundaryConditionsTypes.FIXED_INCREMENTED):
+                fixed[dof] = True
+                u[dof] = (u[dof] + value) if add else value
+                if cond == iron.BoundaryConditionsTypes.FIXED_INCREMENTED:
+                    increments[dof] = value
+            else:
+                raise CMFEError(1, f"boundary condition type {cond} on U is not supported")
+        elif vt == iron.FieldVariableTypes.DELUDELN:
+            # both make the RHS DOF "fixed" (boundary_condition_routines.F90:1685-1693)
+            rhs_fixed[dof] = True
+            if cond == iron.BoundaryConditionsTypes.NEUMANN_POINT:
+                neumann[dof] = (neumann.get(dof, 0.0) + value) if add else value
+            elif cond == iron.BoundaryConditionsTypes.FIXED:
+                dep._data(iron.FieldVariableTypes.DELUDELN)[dof] = value
+            elif cond in (iron.BoundaryConditionsTypes.NEUMANN_INTEGRATED,
+                          iron.BoundaryConditionsTypes.NEUMANN_INTEGRATED_ONLY):
+                # the integrated value goes straight into the RHS values (:1366-1370)
+                dep._data(iron.FieldVariableTypes.DELUDELN)[dof] += value
+                if cond == iron.BoundaryConditionsTypes.NEUMANN_INTEGRATED_ONLY:
+                    integrated_only.add(dof)
+            else:
+                raise CMFEError(1, f"boundary condition type {cond} on DELUDELN is not supported")
+    return fixed, u, neumann, increments, rhs_fixed, frozenset(integrated_only)
+
+
+class _LinearCache:
+    pass
+
+
+def _linear_setup(es, rm: RegularMesh, dm: DofMap, fixed: np.ndarray):
+    """Device structures for a linear equations set, cached while valid."""
+    cache = getattr(es, "_device_cache", None)
+    key = (id(rm), dm.num_dofs, fixed.tobytes())
+    if cache is not None and cache.key == key:
+        return cache
+    c = _LinearCache()
+    c.key = key
+    dev = torch.device("cuda:0")
+    ed = dm.element_dofs()
+    c.csr = fem.CSRStructure(ed, dm.num_dofs, dev, colours=structured_colours(rm))
+    c.red = fem.DirichletReduction(c.csr, fixed)
+    c.S = fem.SolverMatrix(c.red)
+    c.ws = fem.KrylovWorkspace(c.red.n_free, dev)
+    es._device_cache = c
+    return c
+
+
+# ---------------------------------------------------------------- solve
+def solve(problem, es, solver, bc):
+    kind = es._kind
+    if kind == "mooney_rivlin":
+        from . import finite_elasticity
+        return finite_elasticity.solve(problem, es, solver, bc)
+    return solve_linear(es, solver, bc)
+
+
+def solve_linear(es, solver, bc):
+    iron = _iron()
+    geo = es.geometricField
+    rm: RegularMesh = geo._mesh()
+    dep = es.dependentField
+    dm = _dof_map(dep, rm)
+    mcs = set(dm.comp_mesh)
+    if len(mcs) != 1:
+        raise CMFEError(1, "linear equations sets need every dependent component on one mesh component")
+    mc = dm.comp_mesh[0]
+    b: LagrangeBasis = rm.bases[mc]
+    if mc != 0:
+        raise CMFEError(1, "the dependent field must use the geometric mesh component")
+    fixed, u_host, neumann, _, rhs_fixed, integrated_only = _boundary_conditions(bc, es, dm, rm)
+    c = _linear_setup(es, rm, dm, fixed)
+    dev = c.csr.device
+    gvar = geo._var(iron.FieldVariableTypes.U)
+    coords = np.stack([geo._component(iron.FieldVariableTypes.U, k + 1) for k in range(rm.dim)], axis=1)
+    kind = es._kind
+    mixed = str(getattr(solver, "linearIterativePrecision", "fp64")).lower() == "mixed"
+    v64 = c.csr.zeros(fem.F64)
+    v32 = c.csr.zeros(fem.F32) if mixed else None
+    src = None
+    if kind in ("laplace", "generalised_laplace", "poisson"):
+        asm = fem.ScalarAssembler(b, coords, rm.elem_nodes[mc], c.csr)
+        kappa = None
+        source = None
+        if kind == "generalised_laplace":
+            mf = es.materialsField
+            k11, k22, k33 = (_element_values(mf, j, rm) for j in (1, 2, 3))
+            offd = [_element_values(mf, j, rm) for j in (4, 5, 6)] if mf._var(iron.FieldVariableTypes.U).ncomp >= 6 \
+                else []
+            if not (np.allclose(k11, k22) and np.allclose(k11, k33) and all(np.all(o == 0) for o in offd)):
+                raise CMFEError(1, "only isotropic conductivity (k11 = k22 = k33, no off-diagonals) is supported")
+            kappa = torch.as_tensor(k11, device=dev)
+            if es.sourceField is not None:
+                source = torch.as_tensor(_element_values(es.sourceField, 1, rm), device=dev)
+        elif kind == "poisson":
+            source = torch.as_tensor(_element_values(es.materialsField, 1, rm), device=dev)
+        if source is not None:
+            src = torch.zeros(dm.num_dofs, dtype=torch.float64, device=dev)
+        asm.assemble(v64, kappa=kappa, source=source, src_vec=src, prec=fem.F64)
+        if mixed:
+            asm.assemble(v32, kappa=kappa, prec=fem.F32)
+    elif kind == "linear_elasticity":
+        if rm.dim != 3:
+            raise CMFEError(1, "linear elasticity is implemented for the 3-D subtype")
+        asm = fem.ElasticAssembler(b, coords, rm.elem_nodes[mc], c.csr)
+        mf = es.materialsField
+        mat = np.stack([_element_values(mf, j, rm) for j in range(1, 7)], axis=1)
+        mt = torch.as_tensor(np.ascontiguousarray(mat), device=dev)
+        asm.assemble(v64, mt, prec=fem.F64)
+        if mixed:
+            asm.assemble(v32, mt, prec=fem.F32)
+    else:
+        raise CMFEError(1, f"equations set kind {kind} is not implemented")
+    # RHS: DELUDELN values of rows whose RHS DOF is fixed, plus the integrated
+    # Neumann point values (b = f + N q, solver_routines.F90:14681-14696,14850)
+    rhs_host = np.where(rhs_fixed, dep._data(iron.FieldVariableTypes.DELUDELN), 0.0)
+    if neumann:
+        rhs_host = rhs_host + neumann_integrate(rm, dm, [b] * len(dm.comp_mesh), neumann,
+                                                integrated_only=integrated_only)
+    rhs = torch.as_tensor(rhs_host, device=dev)
+    rfix = torch.as_tensor(rhs_fixed.astype(np.uint8), device=dev) if rhs_fixed.any() else None
+    u = torch.as_tensor(np.where(fixed, u_host, 0.0), device=dev)
+    bvec = c.red.rhs(v64, u, src=src, rhs=rhs, rhs_fixed=rfix)
+    # KSP settings (solver_routines.F90:10921-11157)
+    rtol = float(solver.linearIterativeRelativeTolerance)
+    atol = float(solver.linearIterativeAbsoluteTolerance)
+    maxit = int(solver.linearIterativeMaximumIterations)
+    if int(solver.linearType) == iron.LinearSolverTypes.DIRECT:
+        rtol, atol = 1e-14, 0.0  # no GPU direct factorisation: iterate to round-off
+    bn = float(torch.linalg.vector_norm(bvec).item()) if c.red.n_free else 0.0
+    rt = max(rtol, atol / bn) if bn > 0 else rtol
+    x = torch.as_tensor(u_host[~fixed] if c.red.n_free else np.zeros(0), device=dev).clone()
+    s64 = c.S.gather(v64, fem.F64)
+    ktype = int(solver.linearIterativeType)
+    if c.red.n_free == 0:
+        info = fem.SolveInfo(0, 0.0)
+    elif mixed and ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
+        s32 = c.S.gather(v32, fem.F32)
+        info = fem.refine(c.S, s64, s32, bvec, x, rt, maxit, c.ws)
+    elif ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
+        info = fem.cg(c.S, s64, bvec, x, rt, maxit, c.ws)
+    elif ktype == iron.IterativeLinearSolverTypes.GMRES or int(solver.linearType) == iron.LinearSolverTypes.DIRECT:
+        info = fem.gmres(c.S, s64, bvec, x, rt, maxit, c.ws, restart=int(solver.linearIterativeGMRESRestart))
+    else:
+        raise CMFEError(1, f"iterative solver type {ktype} is not implemented (CG and GMRES are)")
+    solver.last_info = info
+    c.red.field_update(x, u)
+    # back-substitution: iron's DELUDELN values and the correct reactions
+    R_iron = fem.backsubstitute(c.csr, v64, u, src=src, rhs_fixed=rfix, iron_compat=True)
+    R = fem.backsubstitute(c.csr, v64, u, src=src, iron_compat=False)
+    dep._data(iron.FieldVariableTypes.U)[:] = u.cpu().numpy()
+    dep._data(iron.FieldVariableTypes.DELUDELN)[:] = R_iron.cpu().numpy()
+    es.reactions = R.cpu().numpy()
+    es._last = dict(vals=v64, src=src, u=u, info=info)
+    return info

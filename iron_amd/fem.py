@@ -135,11 +135,12 @@ class DirichletReduction:
         call("mife_gather_values", self.nnz, ptr(self.src_pos), ptr(vals), src_prec, ptr(out), prec, _stream())
         return out
 
-    def rhs(self, vals64, u, src=None, rhs=None, out=None) -> torch.Tensor:
+    def rhs(self, vals64, u, src=None, rhs=None, rhs_fixed=None, out=None) -> torch.Tensor:
+        """b_f = src + rhs - K_fd u_d (no lifting on rows flagged in ``rhs_fixed``, uint8)."""
         if out is None:
             out = torch.empty(max(self.n_free, 1), dtype=torch.float64, device=vals64.device)
         call("mife_solver_rhs", self.n_free, ptr(self.free_idx), ptr(self.csr.row_ptr), ptr(self.csr.col),
-             ptr(vals64), ptr(self.new_idx), ptr(u), ptr(src), ptr(rhs), ptr(out), _stream())
+             ptr(vals64), ptr(self.new_idx), ptr(u), ptr(src), ptr(rhs), ptr(rhs_fixed), ptr(out), _stream())
         return out
 
     def field_update(self, x, u):

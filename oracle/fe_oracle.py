@@ -547,7 +547,7 @@ def assemble_vector(edofs, fe, n_rows):
 # ----------------------------------------------------------------------------
 # a15-a16: solver mapping, Dirichlet reduction and RHS lifting
 # ----------------------------------------------------------------------------
-def reduce_dirichlet(row_ptr, col, vals, fixed_mask, u_fixed, rhs_extra):
+def reduce_dirichlet(row_ptr, col, vals, fixed_mask, u_fixed, rhs_extra, rhs_fixed_mask=None):
     """SOLVER_MAPPING_CALCULATE (solver_mapping_routines.F90:422-547, :1445-1511),
     SOLVER_MATRIX_STRUCTURE_CALCULATE (solver_matrices_routines.F90:1612-2431),
     SOLVER_MATRIX_EQUATIONS_MATRIX_ADD (:806-1023) and the RHS of
@@ -555,7 +555,9 @@ def reduce_dirichlet(row_ptr, col, vals, fixed_mask, u_fixed, rhs_extra):
 
     Solver rows/columns are the free DOFs in ascending order.  b_f = rhs_extra_f
     - sum_d K_fd u_d, where a fixed column contributes only if |u_d| >= ZERO_TOLERANCE
-    and |K_fd| >= ZERO_TOLERANCE.  Returns (free, s_row_ptr, s_col, s_vals, b).
+    and |K_fd| >= ZERO_TOLERANCE, and only on rows whose RHS-variable DOF is free (the
+    lifting sits in the BOUNDARY_CONDITION_DOF_FREE branch, :14681-14849).
+    Returns (free, s_row_ptr, s_col, s_vals, b).
     """
     n = len(fixed_mask)
     free = np.nonzero(~fixed_mask)[0]
@@ -571,6 +573,8 @@ def reduce_dirichlet(row_ptr, col, vals, fixed_mask, u_fixed, rhs_extra):
     s_row_ptr = np.cumsum(s_row_ptr)
     b = np.asarray(rhs_extra, dtype=np.float64)[free].copy()
     lift = (~fixed_mask[rows]) & fixed_mask[col]
+    if rhs_fixed_mask is not None:
+        lift &= ~np.asarray(rhs_fixed_mask, dtype=bool)[rows]
     ud = np.asarray(u_fixed, dtype=np.float64)[col[lift]]
     kv = vals[lift]
     ok = (np.abs(ud) >= ZERO_TOLERANCE) & (np.abs(kv) >= ZERO_TOLERANCE)
@@ -706,3 +710,60 @@ def backsubstitute(row_ptr, col, vals, u, source=None, rhs_fixed_mask=None, iron
     if source is not None:
         R = R - source
     return R
+
+
+# ----------------------------------------------------------------------------
+# a20: Neumann point conditions integrated over boundary faces
+# ----------------------------------------------------------------------------
+def neumann_integrate(mesh: RegularMesh, dm: DofMap, point_values: dict, integrated_only=frozenset()):
+    """BoundaryConditions_NeumannIntegrate, 3-D node-based branch,
+    boundary_condition_routines.F90:2654-2996 (faces loop :2836-2920).
+
+    For every NEUMANN_POINT DOF o (value q_o, node n, component c) and every
+    boundary face that contains n, integrationMatrix(m, o) += sum_g phi_m phi_o
+    w_g |J_area| over the face nodes m of component c (face basis = the 2-D
+    Lagrange basis of the element's order, default Gauss scheme); then
+    f = integrationMatrix . q.  Loops are written as the reference nests them.
+    Single-basis meshes (the geometry basis equals the field basis)."""
+    b3 = mesh.bases[0]
+    fb = Basis(nxi=2, order=b3.order, ngauss_xi=b3.ngauss_xi)
+    f = np.zeros(dm.num_dofs)
+    E = mesh.num_elements
+    ne = mesh.n_elem
+    for o, q in point_values.items():
+        c = int(np.searchsorted(np.array(dm.offsets[1:] + [dm.num_dofs]), o, side="right"))
+        node_user = int(mesh.comp_nodes[dm.comp_mesh[c]][o - dm.offsets[c]])
+        for e in range(E):
+            ex = [(e // int(np.prod(ne[:k]))) % ne[k] for k in range(3)]
+            en = mesh.elem_nodes[0][e]
+            if node_user not in en:
+                continue
+            for k in range(3):
+                for side in (0, 1):
+                    if ex[k] != (0 if side == 0 else ne[k] - 1):
+                        continue  # not a boundary face
+                    want = 1 if side == 0 else b3.nodes_xic
+                    face_local = [a for a in range(b3.number_of_nodes) if b3.node_index[a, k] == want]
+                    face_nodes = en[face_local]
+                    if node_user not in face_nodes:
+                        continue
+                    face_dofs = [dm.offsets[c] + dm.node_local[dm.comp_mesh[c]][nd] for nd in face_nodes]
+                    if any(int(d) in integrated_only for d in face_dofs):
+                        continue
+                    ol = list(face_nodes).index(node_user)
+                    xf = mesh.coords[face_nodes - 1]
+                    tang = [j for j in range(3) if j != k]
+                    for g in range(len(fb.gauss_weights)):
+                        J = np.zeros((3, 2))
+                        for a in range(len(face_nodes)):
+                            for t in range(2):
+                                J[:, t] += xf[a] * fb.gauss_basis_fns[a, FIRST_DERIV_MAP[t], g]
+                        GL = J.T @ J
+                        jac = math.sqrt(abs(GL[0, 0] * GL[1, 1] - GL[0, 1] * GL[1, 0]))
+                        phio = fb.gauss_basis_fns[ol, NO_PART_DERIV, g]
+                        for m in range(len(face_nodes)):
+                            phim = fb.gauss_basis_fns[m, NO_PART_DERIV, g]
+                            dof_m = dm.offsets[c] + dm.node_local[dm.comp_mesh[c]][face_nodes[m]]
+                            f[dof_m] += q * phim * phio * fb.gauss_weights[g] * jac
+                    del tang
+    return f

@@ -1,0 +1,59 @@
+"""Host-side behaviour of the iron-compatible API (no GPU needed)."""
+import numpy as np
+import pytest
+
+from iron_amd import iron
+
+
+def test_enum_values_match_iron_constants():
+    # values from src/*_routines.F90 / *_constants.F90 aliased by src/opencmiss_iron.F90
+    assert iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT == 3
+    assert iron.IterativeLinearSolverTypes.GMRES == 5
+    assert iron.IterativePreconditionerTypes.JACOBI == 1
+    assert iron.LinearSolverTypes.ITERATIVE == 2
+    assert iron.EquationsSetClasses.CLASSICAL_FIELD == 4 and iron.EquationsSetClasses.ELASTICITY == 1
+    assert iron.EquationsSetSubtypes.GENERALISED_LAPLACE == 2
+    assert iron.EquationsSetSubtypes.THREE_DIMENSIONAL == 4
+    assert iron.FieldVariableTypes.DELUDELN == 2 and iron.FieldVariableTypes.V == 5
+    assert iron.FieldInterpolationTypes.ELEMENT_BASED == 2
+    assert iron.BoundaryConditionsTypes.NEUMANN_POINT == 8
+    assert iron.BoundaryConditionsTypes.FIXED_INCREMENTED == 13
+    assert iron.BoundaryConditionsTypes.NEUMANN_INTEGRATED_ONLY == 20
+
+
+def test_set_get_method_forms_alias_properties():
+    cs = iron.CoordinateSystem()
+    cs.CreateStart(1)
+    cs.DimensionSet(2)
+    assert cs.dimension == 2 and cs.DimensionGet() == 2
+    s = iron.Solver()
+    s.LinearIterativeRelativeToleranceSet(1e-9)
+    assert s.linearIterativeRelativeTolerance == 1e-9
+    # iron's KSP defaults (solver_routines.F90:11320-11369)
+    assert s.linearIterativeType == iron.IterativeLinearSolverTypes.GMRES
+    assert s.linearIterativeMaximumIterations == 100000
+
+
+def test_unsupported_specification_raises_cmfe_error():
+    es = iron.EquationsSet()
+    with pytest.raises(iron.CMFEError):
+        es.CreateStart(1, None, None, [iron.EquationsSetClasses.CLASSICAL_FIELD, 7, 1], 2, None)
+
+
+def test_field_parameter_access_and_mesh_numbering():
+    cs = iron.CoordinateSystem(); cs.CreateStart(1); cs.CreateFinish()
+    region = iron.Region(); region.CreateStart(1, iron.WorldRegion); region.CreateFinish()
+    basis = iron.Basis(); basis.CreateStart(1)
+    basis.interpolationXi = [iron.BasisInterpolationSpecifications.QUADRATIC_LAGRANGE] * 3
+    basis.CreateFinish()
+    gm = iron.GeneratedMesh(); gm.CreateStart(1, region)
+    gm.basis = [basis]; gm.extent = [2.0, 1.0, 1.0]; gm.numberOfElements = [2, 1, 1]
+    mesh = iron.Mesh(); gm.CreateFinish(1, mesh)
+    dec = iron.Decomposition(); dec.CreateStart(1, mesh); dec.CreateFinish()
+    geo = iron.Field(); geo.CreateStart(1, region); geo.meshDecomposition = dec; geo.CreateFinish()
+    gm.GeometricParametersCalculate(geo)
+    nodes = iron.Nodes(); region.NodesGet(nodes)
+    assert nodes.numberOfNodes == 5 * 3 * 3
+    # node 2 is the mid-edge node at x = 0.5
+    assert geo.ParameterSetGetNodeDP(iron.FieldVariableTypes.U, 1, 1, 1, 2, 1) == 0.5
+    assert geo.ParameterSetGetNodeDP(iron.FieldVariableTypes.U, 1, 1, 1, 45, 3) == 1.0
