@@ -11,6 +11,8 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parents[1]))
+from iron_amd.exfile import read_exelem, read_exnode  # noqa: E402
 
 
 def parse_opanal(path: Path):
@@ -50,6 +52,41 @@ def main(ref_root: str):
     }
     (HERE / "analytic_laplace_linear_lagrange.json").write_text(json.dumps(data, indent=1))
     print("wrote analytic_laplace_linear_lagrange.json")
+
+    # Finite-elasticity cantilevers (Lagrange, element-based pressure): nodal deformed
+    # coordinates and DELUDELN values, element pressures and their DELUDELN values.
+    cases = {
+        "mr_cantilever_python_1x1x1.json": (
+            "bindings/python/tests/output_cantilever/Cantilever.part0",
+            {"comment": "bindings/python/tests/Cantilever.py: 1x1x1 trilinear, extent 60x40x40, "
+                        "element pressure (initial -8), c1=2, c2=2, density 9e-4, gravity (0,0,-9.81), "
+                        "2 load increments, nodes 1,3,5,7 fixed (AddNode FIXED 0)",
+             "n_elem": [1, 1, 1], "extent": [60.0, 40.0, 40.0], "c1": 2.0, "c2": 2.0,
+             "density": 9e-4, "gravity": [0.0, 0.0, -9.81], "increments": 2, "pressure0": -8.0}),
+        "mr_cantilever_linear_3x2x2.json": (
+            "tests/FiniteElasticity/output_cantilever/linear_3x2x2/Cantilever.part0",
+            {"comment": "tests/FiniteElasticity/Cantilever.F90 (defaults, :150-153, :185-206): 3x2x2 "
+                        "trilinear, extent 60x40x40, element pressure (initial -14), c1=2, c2=6, "
+                        "density 9e-4, gravity (0,0,-9.8), 2 load increments, x=0 face fixed",
+             "n_elem": [3, 2, 2], "extent": [60.0, 40.0, 40.0], "c1": 2.0, "c2": 6.0,
+             "density": 9e-4, "gravity": [0.0, 0.0, -9.8], "increments": 2, "pressure0": -14.0}),
+    }
+    for name, (stem, setup) in cases.items():
+        nodes = read_exnode(ref / (stem + ".exnode"))
+        elems = read_exelem(ref / (stem + ".exelem"))
+        data = {
+            "source": stem + ".exnode/.exelem",
+            "setup": setup,
+            "nodes": {n: {"dependent": [rec["Dependent"][c][0] for c in range(3)],
+                          "deludeln": [rec["del U/del n"][c][0] for c in range(3)]}
+                      for n, rec in nodes.items()},
+            "elements": {e: {"nodes": rec["nodes"],
+                             "pressure": rec["grid"][("Dependent", 4)][0],
+                             "deludeln_p": rec["grid"][("del U/del n", 4)][0]}
+                         for e, rec in elems.items()},
+        }
+        (HERE / name).write_text(json.dumps(data, indent=1))
+        print("wrote", name)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,145 @@
+"""Pins the Mooney-Rivlin oracle (oracle/mr_oracle.py) without a GPU.
+
+* reference state: zero residual;
+* the exact Jacobian against central finite differences of the residual;
+* homogeneous uniaxial stretch against its closed form;
+* the reference's golden cantilever outputs (tests/golden/mr_cantilever_*.json).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import mr_oracle as mo
+from oracle.fe_oracle import Basis, regular_mesh
+
+GOLDEN = Path(__file__).parent / "golden"
+FORMS = ("deviatoric", "cauchy")
+
+
+def _problem(n_elem, order, elem_p, form, c1=2.0, c2=6.0, extent=(1.0, 1.0, 1.0)):
+    bu = Basis(3, order, order + 1)
+    bases = [bu] if elem_p else [bu, Basis(3, 1, order + 1)]
+    m = regular_mesh(n_elem, bases, extent)
+    E = m.num_elements
+    return mo.MRProblem(m, elem_p, np.full(E, c1), np.full(E, c2), formulation=form)
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("order,elem_p", [(1, True), (2, False)])
+def test_reference_state_residual_vanishes(form, order, elem_p):
+    prob = _problem((2, 1, 1), order, elem_p, form, extent=(2.0, 1.0, 1.5))
+    # deviatoric: sigma_iso is spherical at F = I and is projected out, so p = 0;
+    # cauchy: sigma = 2(c1 + 2 c2) I + p I vanishes for p = -2(c1 + 2 c2).
+    p0 = 0.0 if form == "deviatoric" else -2.0 * (2.0 + 2.0 * 6.0)
+    R, _ = prob.residual(prob.initial_state(p0))
+    assert np.abs(R).max() < 1e-12
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("order,elem_p", [(1, True), (2, False)])
+def test_jacobian_matches_central_differences(form, order, elem_p):
+    rng = np.random.default_rng(3)
+    prob = _problem((2, 1, 1), order, elem_p, form, extent=(2.0, 1.0, 1.3))
+    prob.c1 = np.array([2.0, 3.0])
+    prob.c2 = np.array([6.0, 1.5])
+    x = prob.initial_state(-1.0)
+    x[:3 * prob.nu] += 0.05 * rng.standard_normal(3 * prob.nu)
+    x[3 * prob.nu:] += rng.standard_normal(prob.npd)
+    K = prob.jacobian(x).toarray()
+    h = 1e-6
+    Kfd = np.zeros_like(K)
+    for j in range(prob.ndofs):
+        xp, xm = x.copy(), x.copy()
+        xp[j] += h
+        xm[j] -= h
+        Kfd[:, j] = (prob.residual(xp)[0] - prob.residual(xm)[0]) / (2 * h)
+    assert np.abs(K - Kfd).max() < 1e-7 * np.abs(K).max()
+    if form == "deviatoric":
+        # the deviatoric form is a potential plus a symmetric constraint: K = K^T
+        assert np.abs(K - K.T).max() < 1e-12 * np.abs(K).max()
+
+
+def _uniaxial_pressure(form, lam, c1, c2):
+    B = np.diag([lam * lam, 1.0 / lam, 1.0 / lam])
+    i1 = np.trace(B)
+    if form == "deviatoric":
+        s = 2.0 * (c1 + c2 * i1) * B - 2.0 * c2 * B @ B
+        return (s[0, 0] - s[1, 1]) / 3.0
+    # cauchy: sigma = 2 c1 B + 2 c2 (I1 B - B^2) + p I with J = 1
+    s = 2.0 * c1 * B + 2.0 * c2 * (i1 * B - B @ B)
+    return -s[1, 1]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("order,elem_p", [(1, True), (2, False)])
+def test_uniaxial_stretch_closed_form(form, order, elem_p):
+    """Rollers on x=0, y=0, z=0 and a prescribed x-stretch on x=1: the homogeneous
+    isochoric stretch (lam, lam^-1/2, lam^-1/2) with constant p solves the discrete
+    problem exactly (linear fields are in the displacement space)."""
+    lam, c1, c2 = 1.2, 2.0, 6.0
+    prob = _problem((2, 2, 2), order, elem_p, form, c1, c2)
+    nu = prob.nu
+    X = prob.geo
+    fixed = np.zeros(prob.ndofs, bool)
+    target = prob.initial_state(0.0)
+    for c in range(3):
+        fixed[c * nu:(c + 1) * nu] |= X[:, c] == 0.0
+    fixed[:nu] |= X[:, 0] == 1.0
+    target[:nu][X[:, 0] == 1.0] = lam
+    x0 = prob.initial_state(0.0 if form == "deviatoric" else -2.0 * (c1 + 2.0 * c2))
+    x, R = mo.solve(prob, x0, fixed, incremented=fixed.copy(), full_value=target, increments=2)
+    z, p = prob.split(x)
+    exact = X * np.array([lam, lam ** -0.5, lam ** -0.5])
+    assert np.abs(z - exact).max() < 1e-10
+    assert np.abs(p - _uniaxial_pressure(form, lam, c1, c2)).max() < 1e-8
+
+
+def _golden_case(name):
+    g = json.loads((GOLDEN / name).read_text())
+    s = g["setup"]
+    bu = Basis(3, 1, 2)
+    m = regular_mesh(tuple(s["n_elem"]), [bu], tuple(s["extent"]))
+    E = m.num_elements
+    prob = mo.MRProblem(m, True, np.full(E, s["c1"]), np.full(E, s["c2"]), s["density"],
+                        tuple(s["gravity"]), "geometric", "cauchy")
+    fixed = np.zeros(prob.ndofs, bool)
+    for c in range(3):
+        fixed[c * prob.nu:(c + 1) * prob.nu] = prob.geo[:, 0] == 0.0
+    return g, prob, fixed
+
+
+@pytest.mark.parametrize("name", ["mr_cantilever_python_1x1x1.json", "mr_cantilever_linear_3x2x2.json"])
+def test_cantilever_goldens(name):
+    """The golden files carry the general-branch (cauchy) solution: deformed nodes and
+    DELUDELN to the reference's Newton tolerance; their pressure is half the pressure
+    of S = ... + p C^-1 (they were written with S = ... + 2 p C^-1)."""
+    g, prob, fixed = _golden_case(name)
+    s = g["setup"]
+    x, R = mo.solve(prob, prob.initial_state(s["pressure0"]), fixed, increments=s["increments"])
+    z, p = prob.split(x)
+    nodes = prob.mesh.comp_nodes[0]
+    gz = np.array([g["nodes"][str(n)]["dependent"] for n in nodes])
+    gr = np.array([g["nodes"][str(n)]["deludeln"] for n in nodes])
+    assert np.abs(z - gz).max() < 1e-9
+    ru = R[:3 * prob.nu].reshape(3, -1).T
+    assert np.abs(ru - gr).max() < 1e-7 * max(1.0, np.abs(gr).max())
+    gp = np.array([g["elements"][str(e + 1)]["pressure"] for e in range(prob.mesh.num_elements)])
+    assert np.abs(p - 2.0 * gp).max() < 1e-9 * np.abs(gp).max()
+    # element connectivity of the golden file equals the generated mesh's
+    for e in range(prob.mesh.num_elements):
+        assert list(prob.mesh.elem_nodes[0][e]) == g["elements"][str(e + 1)]["nodes"]
+
+
+def test_deviatoric_differs_from_goldens():
+    """The MOONEY_RIVLIN subtype branch does not reproduce the golden files (they predate
+    it): the tip of the 3x2x2 cantilever moves by more than 0.1."""
+    g, prob, fixed = _golden_case("mr_cantilever_linear_3x2x2.json")
+    prob.formulation = "deviatoric"
+    s = g["setup"]
+    x, _ = mo.solve(prob, prob.initial_state(0.0), fixed, increments=s["increments"])
+    z, _ = prob.split(x)
+    nodes = prob.mesh.comp_nodes[0]
+    gz = np.array([g["nodes"][str(n)]["dependent"] for n in nodes])
+    assert np.abs(z - gz).max() > 0.1
