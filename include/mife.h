@@ -182,6 +182,46 @@ int mife_assemble_elastic(const mife_element_desc* desc, int n_elem,
                           const int32_t* colour_ptr_host, int n_colours, int prec,
                           void* vals, void* stream);
 
+/* Mooney-Rivlin finite elasticity (3-D, FP64). */
+#define MIFE_MR_DEVIATORIC 0 /* EQUATIONS_SET_MOONEY_RIVLIN_SUBTYPE branch */
+#define MIFE_MR_CAUCHY 1     /* EQUATIONS_SET_NO_SUBTYPE general branch */
+
+typedef struct mife_mr_desc {
+  const mife_element_desc* u; /* displacement (= geometric) basis: dim 3, dN, N, w */
+  int n_pressure;             /* pressure DOFs per element: 1 (element based) or the
+                                 pressure basis' node count (node based) */
+  const double* Np;           /* [n_gauss][n_pressure] pressure basis at the displacement
+                                 Gauss points; NULL when n_pressure == 1 */
+  int formulation;            /* MIFE_MR_DEVIATORIC or MIFE_MR_CAUCHY */
+} mife_mr_desc;
+
+/* Residual, exact Jacobian and gravity vector of the Mooney-Rivlin equations,
+ * accumulated (+=) into caller-zeroed global arrays.  Replaces, per element,
+ * FiniteElasticity_FiniteElementResidualEvaluate (MOONEY_RIVLIN subtype
+ * src/finite_elasticity_routines.F90:1669-1760 with
+ * FINITE_ELASTICITY_GAUSS_STRESS_TENSOR :6985-7058, or the general branch
+ * :1824-2000 with FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR :6036-6068), the gravity
+ * term :3370-3430, FiniteElasticity_FiniteElementJacobianEvaluate (:879-1296; here
+ * the exact derivative of the residual, which iron's FD path
+ * src/equations_set_routines.F90:2839-2962 approximates) and EquationsMatrices_ElementAdd.
+ *   coords     [n_geom_nodes][3] reference coordinates
+ *   elem_nodes [n_elem][n_nodes] geometric node of each displacement element node
+ *   elem_dofs  [n_elem][3*n_nodes + n_pressure] global DOFs: x over the element
+ *              nodes, then y, z, then the pressure DOFs
+ *   state      [n_dofs] current dependent values (deformed coordinates, pressures)
+ *   material   [n_elem][2] c1, c2
+ *   body_force NULL or host [3] = density * gravity * load fraction
+ *   pos        [n_elem][ne][ne] CSR positions (ne = 3*n_nodes + n_pressure), needed
+ *              with jac_vals
+ *   residual, jac_vals, rhs: outputs or NULL; rhs gets the gravity vector. */
+int mife_mr_assemble(const mife_mr_desc* desc, int n_elem, const double* coords,
+                     const int32_t* elem_nodes, const int32_t* elem_dofs,
+                     const double* state, const double* material,
+                     const double* body_force, const int32_t* pos,
+                     const int32_t* order, const int32_t* colour_ptr_host,
+                     int n_colours, double* residual, double* jac_vals, double* rhs,
+                     void* stream);
+
 /* --------------------------------------------------- sparse operations */
 
 /* y = A x (prec selects FP64 or FP32 values and vectors).  For SELL, x and y
@@ -250,11 +290,27 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
                 const double* b, double* x, double rtol, double inner_rtol, int maxit,
                 void* work, int* iters, int* outer, double* relres, void* stream);
 
-/* Right-Jacobi-preconditioned restarted GMRES(restart) in FP64 (KSPGMRES default
- * restart 30, src/solver_routines.F90:11320-11369). */
+/* Right-preconditioned restarted GMRES(restart) in FP64 (KSPGMRES default
+ * restart 30, src/solver_routines.F90:11320-11369), restart <= 31.  The
+ * preconditioner is diagonal: Jacobi 1/a_ii, except rows where pc_inv[i] != 0
+ * (device, caller's row order, may be NULL) which use pc_inv[i] instead, e.g. a
+ * scaled inverse pressure mass for the zero-diagonal rows of a mixed
+ * displacement/pressure Jacobian (PCJACOBI would use 1 there). */
 int mife_gmres(const mife_matrix* A, const double* vals, const double* b, double* x,
-               double rtol, int restart, int maxit, void* work, int* iters,
-               double* relres, void* stream);
+               const double* pc_inv, double rtol, int restart, int maxit, void* work,
+               int* iters, double* relres, void* stream);
+
+/* Preconditioned MINRES (Paige-Saunders) in FP64 for symmetric, possibly
+ * indefinite A, such as the Newton Jacobian of the Mooney-Rivlin deviatoric
+ * formulation.  SPD diagonal preconditioner |1/a_ii|, or pc_inv[i] where nonzero
+ * (as for mife_gmres, absolute value taken).  Stops when the preconditioned
+ * residual norm has fallen by rtol or after maxit iterations; relres returns the
+ * true ||b - A x|| / ||b|| at exit.  No reference counterpart (iron's Newton
+ * solves go through KSP, src/solver_routines.F90:18674-18763).  Workspace:
+ * mife_solver_work_bytes(n, 1). */
+int mife_minres(const mife_matrix* A, const double* vals, const double* b, double* x,
+                const double* pc_inv, double rtol, int maxit, void* work, int* iters,
+                double* relres, void* stream);
 
 /* ------------------------------------------------------------- profiling */
 

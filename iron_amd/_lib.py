@@ -46,6 +46,15 @@ class ElementDesc(C.Structure):
                 ("dN", C.c_void_p), ("N", C.c_void_p), ("w", C.c_void_p)]
 
 
+class MrDesc(C.Structure):
+    """mife_mr_desc: displacement basis, pressure DOFs per element, formulation."""
+    _fields_ = [("u", C.POINTER(ElementDesc)), ("n_pressure", C.c_int), ("Np", C.c_void_p),
+                ("formulation", C.c_int)]
+
+
+MIFE_MR_DEVIATORIC = 0
+MIFE_MR_CAUCHY = 1
+
 _lib = None
 
 _P = C.c_void_p
@@ -73,7 +82,9 @@ _SIG = {
     "mife_cg": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_refine": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _D, _I, _P, C.POINTER(_I), C.POINTER(_I),
                          C.POINTER(_D), _P]),
-    "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_minres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_mr_assemble": (_I, [C.POINTER(MrDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "mife_profile_enable": (_I, [_I]),
     "mife_profile_reset": (_I, []),
     "mife_profile_read": (_I, [_I, C.c_char_p, _P, _P, C.POINTER(_I)]),

@@ -1,0 +1,414 @@
+// Mooney-Rivlin finite elasticity: element residual, exact element Jacobian and
+// gravity vector, scattered into a global residual, CSR values and RHS vector.
+//
+// One 256-thread workgroup integrates one element in FP64:
+//   1. gather the element's reference coordinates and current dependent values
+//      (deformed coordinates, pressures) into LDS;
+//   2. one thread per Gauss point forms dX/dxi, its inverse, |J| w, F = dz/dX,
+//      det F, F^-T and the first Piola stress P;
+//   3. all threads build the 81-entry tangent A = dP/dF of every Gauss point;
+//   4. per Gauss point the threads form T[a][i][j][L] = sum_K dphi_a/dX_K A_iKjL
+//      and accumulate the element Jacobian entries they own in registers;
+//   5. residual/RHS entries, then the coloured (or atomic) scatter.
+//
+// The residual is written in the reference configuration,
+//   R_(i,a) = sum_g w |J_x| dphi_a/dX_K P_iK,   R_q = sum_g w |J_x| (det F - 1) psi_q,
+// which equals the Cauchy-stress form dphi/dz . sigma |J_z| w of
+// finite_elasticity_routines.F90:1669-1756 (P = det F sigma F^-T).
+#include "mife_common.hpp"
+#include "mife_profile.hpp"
+
+namespace mife {
+
+namespace mr {
+
+__device__ __forceinline__ double det3(const double (&A)[3][3]) {
+  return A[0][0] * A[1][1] * A[2][2] + A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] -
+         A[0][0] * A[2][1] * A[1][2] - A[1][0] * A[0][1] * A[2][2] - A[2][0] * A[1][1] * A[0][2];
+}
+
+// B = A^-1 by cofactors; returns det A.
+__device__ __forceinline__ double inv3(const double (&A)[3][3], double (&B)[3][3]) {
+  const double det = det3(A);
+  const double r = 1.0 / det;
+  B[0][0] = (A[1][1] * A[2][2] - A[2][1] * A[1][2]) * r;
+  B[1][0] = (A[1][2] * A[2][0] - A[2][2] * A[1][0]) * r;
+  B[2][0] = (A[1][0] * A[2][1] - A[2][0] * A[1][1]) * r;
+  B[0][1] = (A[2][1] * A[0][2] - A[0][1] * A[2][2]) * r;
+  B[1][1] = (A[2][2] * A[0][0] - A[0][2] * A[2][0]) * r;
+  B[2][1] = (A[2][0] * A[0][1] - A[0][0] * A[2][1]) * r;
+  B[0][2] = (A[0][1] * A[1][2] - A[1][1] * A[0][2]) * r;
+  B[1][2] = (A[0][2] * A[1][0] - A[1][2] * A[0][0]) * r;
+  B[2][2] = (A[0][0] * A[1][1] - A[1][0] * A[0][1]) * r;
+  return det;
+}
+
+// Per-Gauss-point kinematics and stress shared by the element's threads.
+struct GaussState {
+  double F[9], G[9], C[9], B[9], X[9];  // X: I2,F (deviatoric) or S (cauchy)
+  double Ci[9];                          // C^-1 (cauchy)
+  double P[9];                           // first Piola stress
+  double dPdp[9];                        // dP/dp
+  double JG[9];                          // det F * F^-T = d(det F)/dF
+  double s[8];                           // I1, I2, J, a, b, p, c1, c2
+  double xi[9];                          // dxi_m/dX_K as xi[m*3+K]
+  double wj;                             // w |det dX/dxi|
+};
+
+}  // namespace mr
+
+template <int NN, int NP, int NG, int FORM>
+struct MrLDS {
+  double X[NN][3];
+  double Z[NN][3];
+  double p[NP];
+  mr::GaussState gs[NG];
+  double A[NG][81];
+  double GX[NG][NN][3];
+  double T[NN][27];  // T[a][(i*3+j)*3+L] for the current Gauss point
+};
+
+// A[i,K,j,L] = dP_iK / dF_jL of one Gauss point; idx = ((i*3+K)*3+j)*3+L.
+template <int FORM>
+__device__ __forceinline__ double tangent_entry(const mr::GaussState& g, int idx) {
+  const int L = idx % 3, j = (idx / 3) % 3, K = (idx / 9) % 3, i = idx / 27;
+  const double dij = (i == j) ? 1.0 : 0.0, dKL = (K == L) ? 1.0 : 0.0;
+  const double* F = g.F;
+  const double* G = g.G;
+  const double c2 = g.s[7], p = g.s[5];
+  if constexpr (FORM == MIFE_MR_DEVIATORIC) {
+    const double I1 = g.s[0], I2 = g.s[1], J = g.s[2], a = g.s[3], b = g.s[4], c1 = g.s[6];
+    const double GiK = G[i * 3 + K], GjL = G[j * 3 + L], GiL = G[i * 3 + L], GjK = G[j * 3 + K];
+    const double FiK = F[i * 3 + K], FjL = F[j * 3 + L], FiL = F[i * 3 + L], FjK = F[j * 3 + K];
+    const double* I2F = g.X;
+    const double ib1 = -(2.0 / 3.0) * a * GjL * (2.0 * FiK - (2.0 / 3.0) * I1 * GiK) +
+                       a * (2.0 * dij * dKL - (4.0 / 3.0) * GiK * FjL + (2.0 / 3.0) * I1 * GiL * GjK);
+    const double i2ff = 2.0 * (2.0 * FiK * FjL + I1 * dij * dKL - dij * g.C[L * 3 + K] - FiL * FjK -
+                               g.B[i * 3 + j] * dKL);
+    const double ib2 = -(4.0 / 3.0) * b * GjL * (I2F[i * 3 + K] - (4.0 / 3.0) * I2 * GiK) +
+                       b * (i2ff - (4.0 / 3.0) * GiK * I2F[j * 3 + L] + (4.0 / 3.0) * I2 * GiL * GjK);
+    return c1 * ib1 + c2 * ib2 + p * J * (GjL * GiK - GiL * GjK);
+  } else {
+    const double* S = g.X;
+    return dij * S[L * 3 + K] +
+           2.0 * c2 * (2.0 * F[i * 3 + K] * F[j * 3 + L] - F[i * 3 + L] * F[j * 3 + K] - g.B[i * 3 + j] * dKL) -
+           p * (G[i * 3 + L] * G[j * 3 + K] + dij * g.Ci[L * 3 + K]);
+  }
+}
+
+template <int NN, int NP, int NG, int FORM>
+__global__ __launch_bounds__(256) void mr_element_kernel(
+    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
+    const double* __restrict__ state, const double* __restrict__ material, const double* __restrict__ dN,
+    const double* __restrict__ Nu, const double* __restrict__ Np, const double* __restrict__ w, double bfx,
+    double bfy, double bfz, int want_rhs, const int32_t* __restrict__ pos, const int32_t* __restrict__ order,
+    int elem_begin, int elem_count, double* __restrict__ residual, double* __restrict__ jac,
+    double* __restrict__ rhs, bool atomic) {
+  constexpr int NE = 3 * NN + NP;
+  constexpr int NENT = NE * NE;
+  constexpr int PER = (NENT + 255) / 256;
+  __shared__ MrLDS<NN, NP, NG, FORM> S;
+  const int t = threadIdx.x;
+  const int idx = blockIdx.x;
+  if (idx >= elem_count) return;  // whole block exits together
+  const long long e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
+  const int32_t* ed = edofs + e * NE;
+
+  // 1. gather
+  if (t < NN) {
+    const long long node = enodes[e * NN + t];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      S.X[t][d] = coords[node * 3 + d];
+      S.Z[t][d] = state[ed[d * NN + t]];
+    }
+  }
+  if (t < NP) S.p[t] = state[ed[3 * NN + t]];
+  __syncthreads();
+
+  // 2. per-Gauss-point kinematics and stress
+  if (t < NG) {
+    const int g = t;
+    double Jx[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Jz[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int a = 0; a < NN; ++a) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double dn = dN[(g * NN + a) * 3 + m];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          Jx[d][m] += S.X[a][d] * dn;
+          Jz[d][m] += S.Z[a][d] * dn;
+        }
+      }
+    }
+    double xi[3][3];
+    const double detx = mr::inv3(Jx, xi);  // xi[m][K] = dxi_m/dX_K
+    double F[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int K = 0; K < 3; ++K) F[i][K] = Jz[i][0] * xi[0][K] + Jz[i][1] * xi[1][K] + Jz[i][2] * xi[2][K];
+    double Fi[3][3];
+    const double J = mr::inv3(F, Fi);
+    double pg = 0.0;
+    if constexpr (NP == 1) {
+      pg = S.p[0];
+    } else {
+#pragma unroll
+      for (int q = 0; q < NP; ++q) pg += Np[g * NP + q] * S.p[q];
+    }
+    const double c1 = material[e * 2 + 0], c2 = material[e * 2 + 1];
+    mr::GaussState& gs = S.gs[g];
+    double C[3][3], B[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        C[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+        B[i][j] = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2];
+      }
+    const double I1 = C[0][0] + C[1][1] + C[2][2];
+    double CC = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) CC += C[i][j] * C[i][j];
+    const double I2 = 0.5 * (I1 * I1 - CC);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        gs.F[i * 3 + j] = F[i][j];
+        gs.G[i * 3 + j] = Fi[j][i];
+        gs.C[i * 3 + j] = C[i][j];
+        gs.B[i * 3 + j] = B[i][j];
+        gs.JG[i * 3 + j] = J * Fi[j][i];
+        gs.xi[i * 3 + j] = xi[i][j];
+      }
+    if constexpr (FORM == MIFE_MR_DEVIATORIC) {
+      const double a = pow(J, -2.0 / 3.0), b = a * a;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int K = 0; K < 3; ++K) {
+          const double FC = F[i][0] * C[0][K] + F[i][1] * C[1][K] + F[i][2] * C[2][K];
+          const double i2f = 2.0 * (I1 * F[i][K] - FC);
+          const double GiK = Fi[K][i];
+          gs.X[i * 3 + K] = i2f;
+          gs.P[i * 3 + K] = c1 * a * (2.0 * F[i][K] - (2.0 / 3.0) * I1 * GiK) +
+                            c2 * b * (i2f - (4.0 / 3.0) * I2 * GiK) + pg * J * GiK;
+          gs.dPdp[i * 3 + K] = J * GiK;
+        }
+      gs.s[3] = a;
+      gs.s[4] = b;
+    } else {
+      double Ci[3][3];
+      mr::inv3(C, Ci);
+      double Sm[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          Sm[i][j] = 2.0 * (c1 * (i == j ? 1.0 : 0.0) + c2 * ((i == j ? I1 : 0.0) - C[i][j])) + pg * Ci[i][j];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int K = 0; K < 3; ++K) {
+          gs.X[i * 3 + K] = Sm[i][K];
+          gs.Ci[i * 3 + K] = Ci[i][K];
+          gs.P[i * 3 + K] = F[i][0] * Sm[0][K] + F[i][1] * Sm[1][K] + F[i][2] * Sm[2][K];
+          gs.dPdp[i * 3 + K] = Fi[K][i];
+        }
+      gs.s[3] = 0.0;
+      gs.s[4] = 0.0;
+    }
+    gs.s[0] = I1;
+    gs.s[1] = I2;
+    gs.s[2] = J;
+    gs.s[5] = pg;
+    gs.s[6] = c1;
+    gs.s[7] = c2;
+    gs.wj = fabs(detx) * w[g];
+  }
+  __syncthreads();
+
+  // 3. tangents and reference gradients
+  const bool want_jac = jac != nullptr;
+  if (want_jac) {
+    for (int k = t; k < NG * 81; k += 256) {
+      const int g = k / 81;
+      S.A[g][k - g * 81] = tangent_entry<FORM>(S.gs[g], k - g * 81);
+    }
+  }
+  for (int k = t; k < NG * NN; k += 256) {
+    const int g = k / NN, a = k - g * NN;
+    const double* xi = S.gs[g].xi;
+#pragma unroll
+    for (int K = 0; K < 3; ++K) {
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) v += dN[(g * NN + a) * 3 + m] * xi[m * 3 + K];
+      S.GX[g][a][K] = v;
+    }
+  }
+  __syncthreads();
+
+  // 4. element Jacobian, entries r*NE+c owned by thread t: t, t+256, ...
+  double acc[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) acc[k] = 0.0;
+  if (want_jac) {
+    for (int g = 0; g < NG; ++g) {
+      for (int k = t; k < NN * 27; k += 256) {
+        const int a = k / 27, r = k - a * 27;  // r = (i*3+j)*3+L
+        const int i = r / 9, jl = r - i * 9;
+        const double* A = S.A[g];
+        S.T[a][r] = S.GX[g][a][0] * A[(i * 3 + 0) * 9 + jl] + S.GX[g][a][1] * A[(i * 3 + 1) * 9 + jl] +
+                    S.GX[g][a][2] * A[(i * 3 + 2) * 9 + jl];
+      }
+      __syncthreads();
+      const double wj = S.gs[g].wj;
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int ent = t + k * 256;
+        if (ent < NENT) {
+          const int row = ent / NE, col = ent - (ent / NE) * NE;
+          double v;
+          if (row < 3 * NN && col < 3 * NN) {
+            const int i = row / NN, a = row - i * NN, j = col / NN, b = col - j * NN;
+            const double* Tr = &S.T[a][(i * 3 + j) * 3];
+            v = Tr[0] * S.GX[g][b][0] + Tr[1] * S.GX[g][b][1] + Tr[2] * S.GX[g][b][2];
+          } else if (row < 3 * NN) {
+            const int i = row / NN, a = row - i * NN, q = col - 3 * NN;
+            const double* dp = S.gs[g].dPdp;
+            const double psi = (NP == 1) ? 1.0 : Np[g * NP + q];
+            v = (S.GX[g][a][0] * dp[i * 3 + 0] + S.GX[g][a][1] * dp[i * 3 + 1] + S.GX[g][a][2] * dp[i * 3 + 2]) *
+                psi;
+          } else if (col < 3 * NN) {
+            const int q = row - 3 * NN, j = col / NN, b = col - j * NN;
+            const double* jg = S.gs[g].JG;
+            const double psi = (NP == 1) ? 1.0 : Np[g * NP + q];
+            v = (S.GX[g][b][0] * jg[j * 3 + 0] + S.GX[g][b][1] * jg[j * 3 + 1] + S.GX[g][b][2] * jg[j * 3 + 2]) *
+                psi;
+          } else {
+            v = 0.0;
+          }
+          acc[k] += wj * v;
+        }
+      }
+      __syncthreads();
+    }
+    const int32_t* pe = pos + e * (long long)NENT;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int ent = t + k * 256;
+      if (ent < NENT) {
+        double* dst = jac + pe[ent];
+        if (atomic) atomicAdd(dst, acc[k]);
+        else *dst += acc[k];
+      }
+    }
+  }
+
+  // 5. residual and gravity vector
+  if (residual && t < NE) {
+    double r = 0.0;
+    if (t < 3 * NN) {
+      const int i = t / NN, a = t - i * NN;
+      for (int g = 0; g < NG; ++g) {
+        const double* P = S.gs[g].P;
+        r += S.gs[g].wj *
+             (S.GX[g][a][0] * P[i * 3 + 0] + S.GX[g][a][1] * P[i * 3 + 1] + S.GX[g][a][2] * P[i * 3 + 2]);
+      }
+    } else {
+      const int q = t - 3 * NN;
+      for (int g = 0; g < NG; ++g) {
+        const double psi = (NP == 1) ? 1.0 : Np[g * NP + q];
+        r += S.gs[g].wj * (S.gs[g].s[2] - 1.0) * psi;
+      }
+    }
+    double* dst = residual + ed[t];
+    if (atomic) atomicAdd(dst, r);
+    else *dst += r;
+  }
+  if (want_rhs && rhs && t < 3 * NN) {
+    const int i = t / NN, a = t - i * NN;
+    const double bf = i == 0 ? bfx : (i == 1 ? bfy : bfz);
+    double r = 0.0;
+    for (int g = 0; g < NG; ++g) r += Nu[g * NN + a] * S.gs[g].wj;
+    double* dst = rhs + ed[t];
+    if (atomic) atomicAdd(dst, r * bf);
+    else *dst += r * bf;
+  }
+}
+
+template <int NN, int NP, int NG, int FORM>
+static int launch_mr(const mife_mr_desc* d, int n_elem, const double* coords, const int32_t* enodes,
+                     const int32_t* edofs, const double* state, const double* material, const double* body_force,
+                     const int32_t* pos, const int32_t* order, const int32_t* cptr, int ncol, double* residual,
+                     double* jac, double* rhs, hipStream_t s) {
+  const double bx = body_force ? body_force[0] : 0.0, by = body_force ? body_force[1] : 0.0,
+               bz = body_force ? body_force[2] : 0.0;
+  const int want_rhs = (rhs != nullptr && body_force != nullptr) ? 1 : 0;
+  const int tk = prof::begin(s);
+  if (!order) {
+    mr_element_kernel<NN, NP, NG, FORM><<<n_elem, 256, 0, s>>>(coords, enodes, edofs, state, material, d->u->dN,
+                                                               d->u->N, d->Np, d->u->w, bx, by, bz, want_rhs, pos,
+                                                               nullptr, 0, n_elem, residual, jac, rhs, true);
+  } else {
+    for (int c = 0; c < ncol; ++c) {
+      const int cnt = cptr[c + 1] - cptr[c];
+      if (cnt <= 0) continue;
+      mr_element_kernel<NN, NP, NG, FORM><<<cnt, 256, 0, s>>>(coords, enodes, edofs, state, material, d->u->dN,
+                                                              d->u->N, d->Np, d->u->w, bx, by, bz, want_rhs, pos,
+                                                              order, cptr[c], cnt, residual, jac, rhs, false);
+    }
+  }
+  prof::end(tk, s, prof::kFiniteElasticity);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+}  // namespace mife
+
+using namespace mife;
+
+extern "C" int mife_mr_assemble(const mife_mr_desc* d, int n_elem, const double* coords, const int32_t* elem_nodes,
+                                const int32_t* elem_dofs, const double* state, const double* material,
+                                const double* body_force, const int32_t* pos, const int32_t* order,
+                                const int32_t* colour_ptr_host, int n_colours, double* residual, double* jac_vals,
+                                double* rhs, void* stream) {
+  MIFE_REQUIRE(d && d->u && coords && elem_nodes && elem_dofs && state && material, MIFE_ERR_POINTER_IS_NULL,
+               "null pointer argument");
+  MIFE_REQUIRE(d->u->dN && d->u->w, MIFE_ERR_POINTER_IS_NULL, "displacement basis without dN or w");
+  MIFE_REQUIRE(!jac_vals || pos, MIFE_ERR_POINTER_IS_NULL, "Jacobian requested without a position map");
+  MIFE_REQUIRE(!rhs || !body_force || d->u->N, MIFE_ERR_POINTER_IS_NULL, "gravity requested without basis values N");
+  MIFE_REQUIRE(d->n_pressure == 1 || d->Np, MIFE_ERR_POINTER_IS_NULL, "node-based pressure without its basis values");
+  MIFE_REQUIRE(!order || colour_ptr_host, MIFE_ERR_POINTER_IS_NULL, "colour order without colour_ptr");
+  MIFE_REQUIRE(d->u->dim == 3, MIFE_ERR_ARG, "finite elasticity kernel is 3-D only");
+  MIFE_REQUIRE(d->formulation == MIFE_MR_DEVIATORIC || d->formulation == MIFE_MR_CAUCHY, MIFE_ERR_ARG,
+               "invalid formulation %d", d->formulation);
+  if (n_elem == 0 || (!residual && !jac_vals && !rhs)) return MIFE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int key = d->u->n_nodes * 10000 + d->n_pressure * 100 + d->u->n_gauss;
+#define MR_CASE(NN, NP, NG)                                                                                  \
+  case NN * 10000 + NP * 100 + NG:                                                                          \
+    if (d->formulation == MIFE_MR_DEVIATORIC)                                                                \
+      return launch_mr<NN, NP, NG, MIFE_MR_DEVIATORIC>(d, n_elem, coords, elem_nodes, elem_dofs, state,      \
+                                                       material, body_force, jac_vals ? pos : nullptr, order, \
+                                                       colour_ptr_host, n_colours, residual, jac_vals, rhs, s); \
+    return launch_mr<NN, NP, NG, MIFE_MR_CAUCHY>(d, n_elem, coords, elem_nodes, elem_dofs, state, material,   \
+                                                 body_force, jac_vals ? pos : nullptr, order, colour_ptr_host, \
+                                                 n_colours, residual, jac_vals, rhs, s);
+  switch (key) {
+    MR_CASE(8, 1, 8)
+    MR_CASE(8, 8, 8)
+    MR_CASE(27, 1, 27)
+    MR_CASE(27, 8, 27)
+    default:
+      set_error("unsupported finite-elasticity element (nodes=%d, pressure dofs=%d, gauss=%d)", d->u->n_nodes,
+                d->n_pressure, d->u->n_gauss);
+      return MIFE_ERR_ARG;
+  }
+#undef MR_CASE
+}

@@ -20,6 +20,7 @@ extern "C" int mife_profile_enable(int on);
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <vector>
 
 namespace mife {
@@ -355,6 +356,172 @@ __global__ void gmres_update_kernel(int n, int m, const double* __restrict__ V, 
   }
 }
 
+// ------------------------------------------------------------------ MINRES
+// Preconditioned MINRES (Paige & Saunders 1975) for symmetric indefinite
+// systems with an SPD diagonal preconditioner, three launches per iteration
+// and every scalar on the device.  The solution update of iteration k is
+// applied at the start of iteration k+1's SpMV kernel (it only needs row-wise
+// data), so x is complete once one more minres_spmv/fin pair has run.
+struct MinresState {
+  double beta, oldb, beta1, alfa, dbar, epsln, phibar, cs, sn, phi, denom, oldeps, delta, tol, rr, bb;
+  int iters, maxit, done, pend;
+  double part_a[4 * kRedBlocks];
+  double part_b[4 * kRedBlocks];
+};
+
+// Applies a pending w/x update, then v = z/beta, t = A z / beta - (beta/oldb) r1, partial v.t
+template <int FMT, int L>
+__global__ __launch_bounds__(256) void minres_spmv_kernel(MatArgs A, const double* __restrict__ vals,
+                                                          const double* __restrict__ z, double* __restrict__ v,
+                                                          double* __restrict__ t, const double* __restrict__ r1,
+                                                          double* __restrict__ w, double* __restrict__ w1,
+                                                          double* __restrict__ w2, double* __restrict__ x,
+                                                          MinresState* st) {
+  const bool pend = st->pend != 0, done = st->done != 0;
+  if (!pend && done) return;
+  const double ib = 1.0 / st->beta, rb = st->oldb != 0.0 ? st->beta / st->oldb : 0.0;
+  const double oldeps = st->oldeps, delta = st->delta, denom = st->denom, phi = st->phi;
+  double m[1] = {0.0};
+  for_each_row<double, FMT, L>(A, vals, done ? nullptr : z, [&](int row, double az) {
+    if (pend) {
+      const double wo = w[row], w2o = w2[row];
+      const double wn = (v[row] - oldeps * w2o - delta * wo) * denom;
+      w1[row] = w2o;
+      w2[row] = wo;
+      w[row] = wn;
+      x[row] += phi * wn;
+    }
+    if (!done) {
+      const double vi = z[row] * ib;
+      const double ti = az * ib - rb * r1[row];
+      v[row] = vi;
+      t[row] = ti;
+      m[0] += vi * ti;
+    }
+  });
+  if (!done) block_partials<1>(m, st->part_a);
+}
+
+__global__ void minres_fin1_kernel(MinresState* st, int nb) {
+  const bool done = st->done != 0;
+  double out[1] = {0.0};
+  if (!done) finish_sums<1>(st->part_a, nb, out);
+  if (threadIdx.x == 0) {
+    st->pend = 0;
+    if (!done) st->alfa = out[0];
+  }
+}
+
+// t -= (alfa/beta) r2 ; r1 = r2 ; r2 = t ; z = D^-1 r2 ; partial r2.z
+__global__ __launch_bounds__(256) void minres_update_kernel(int n, double* __restrict__ t, double* __restrict__ r1,
+                                                            double* __restrict__ r2, double* __restrict__ z,
+                                                            const double* __restrict__ dinv, MinresState* st) {
+  if (st->done) return;
+  const double ab = st->alfa / st->beta;
+  double m[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double r2o = r2[i];
+    const double ti = t[i] - ab * r2o;
+    const double zi = dinv[i] * ti;
+    r1[i] = r2o;
+    r2[i] = ti;
+    z[i] = zi;
+    m[0] += ti * zi;
+  }
+  block_partials<1>(m, st->part_b);
+}
+
+// beta_{k+1}, the Givens rotation and the pending update coefficients.
+__global__ void minres_fin2_kernel(MinresState* st, int nb) {
+  if (st->done) return;
+  double out[1];
+  finish_sums<1>(st->part_b, nb, out);
+  if (threadIdx.x == 0) {
+    const double beta = sqrt(fmax(out[0], 0.0));
+    st->oldb = st->beta;
+    st->beta = beta;
+    st->oldeps = st->epsln;
+    const double delta = st->cs * st->dbar + st->sn * st->alfa;
+    const double gbar = st->sn * st->dbar - st->cs * st->alfa;
+    st->epsln = st->sn * beta;
+    st->dbar = -st->cs * beta;
+    const double gamma = fmax(sqrt(gbar * gbar + beta * beta), 1e-300);
+    st->cs = gbar / gamma;
+    st->sn = beta / gamma;
+    st->phi = st->cs * st->phibar;
+    st->phibar = st->sn * st->phibar;
+    st->denom = 1.0 / gamma;
+    st->delta = delta;
+    st->pend = 1;
+    st->iters += 1;
+    st->done = (st->phibar <= st->tol * st->beta1 || st->iters >= st->maxit || beta == 0.0) ? 1 : 0;
+  }
+}
+
+// r1 = r2 = b - A x, z = D^-1 r1, w = w2 = 0 ; partials r1.z, b.b
+template <int FMT, int L>
+__global__ __launch_bounds__(256) void minres_init_kernel(MatArgs A, const double* __restrict__ vals,
+                                                          const double* __restrict__ b, const double* __restrict__ x,
+                                                          const double* __restrict__ dinv, double* __restrict__ r1,
+                                                          double* __restrict__ r2, double* __restrict__ z,
+                                                          double* __restrict__ w, double* __restrict__ w2,
+                                                          MinresState* st) {
+  double m[2] = {0.0, 0.0};
+  for_each_row<double, FMT, L>(A, vals, x, [&](int row, double ax) {
+    const double ri = b[row] - ax;
+    const double zi = dinv[row] * ri;
+    r1[row] = ri;
+    r2[row] = ri;
+    z[row] = zi;
+    w[row] = 0.0;
+    w2[row] = 0.0;
+    m[0] += ri * zi;
+    m[1] += b[row] * b[row];
+  });
+  block_partials<2>(m, st->part_b);
+}
+
+__global__ void minres_fin_init_kernel(MinresState* st, int nb, double tol, int maxit) {
+  double out[2];
+  finish_sums<2>(st->part_b, nb, out);
+  if (threadIdx.x == 0) {
+    const double b1 = sqrt(fmax(out[0], 0.0));
+    st->beta1 = b1;
+    st->beta = b1;
+    st->oldb = 0.0;
+    st->alfa = 0.0;
+    st->dbar = 0.0;
+    st->epsln = 0.0;
+    st->phibar = b1;
+    st->cs = -1.0;
+    st->sn = 0.0;
+    st->phi = 0.0;
+    st->denom = 0.0;
+    st->oldeps = 0.0;
+    st->delta = 0.0;
+    st->tol = tol;
+    st->bb = out[1];
+    st->iters = 0;
+    st->maxit = maxit;
+    st->pend = 0;
+    st->done = (b1 == 0.0 || maxit <= 0) ? 1 : 0;
+  }
+}
+
+// Preconditioner diagonal: pc[i] != 0 replaces the Jacobi entry of row i (given in
+// the caller's row order; permuted to slice numbering here).  absval: |1/a_ii|.
+__global__ void pc_override_kernel(int n, const int32_t* __restrict__ perm, const double* __restrict__ pc,
+                                   double* __restrict__ dinv, int absval) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    double d = dinv[k];
+    if (pc) {
+      const double o = pc[perm ? perm[k] : k];
+      if (o != 0.0) d = o;
+    }
+    dinv[k] = absval ? fabs(d) : d;
+  }
+}
+
 // ------------------------------------------------------------- host side
 struct Work {
   char* base;
@@ -653,8 +820,88 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
   return MIFE_OK;
 }
 
-int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, double rtol, int restart,
-               int maxit, void* work, int* iters, double* relres, void* stream) {
+int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, const double* pc_inv,
+                double rtol, int maxit, void* work, int* iters, double* relres, void* stream) {
+  MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(rtol > 0.0 && maxit >= 0, MIFE_ERR_ARG, "invalid MINRES arguments");
+  Mat M;
+  int rc = make_mat(A, &M);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = M.a.n;
+  Work w{(char*)work, 0};
+  MinresState* st = w.take<MinresState>(1);
+  CgState* rs = w.take<CgState>(1);
+  double* dinv = w.take<double>(n);
+  double* b = w.take<double>(n);
+  double* x = w.take<double>(n);
+  double* z = w.take<double>(n);
+  double* v = w.take<double>(n);
+  double* t = w.take<double>(n);
+  double* r1 = w.take<double>(n);
+  double* r2 = w.take<double>(n);
+  double* wv = w.take<double>(n);
+  double* w1 = w.take<double>(n);
+  double* w2 = w.take<double>(n);
+  const int vg = vec_grid(n);
+  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
+  permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
+  diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
+  pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 1);
+  fill_kernel<double><<<vg, 256, 0, s>>>(n, v, 0.0);
+  fill_kernel<double><<<vg, 256, 0, s>>>(n, w1, 0.0);
+  MIFE_DISPATCH_FMT(M.a, M.L, (minres_init_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, dinv, r1, r2, z,
+                                                                                   wv, w2, st)));
+  minres_fin_init_kernel<<<1, kFinThreads, 0, s>>>(st, grid, rtol, maxit);
+  MIFE_CHECK_HIP(hipGetLastError());
+  int done = 0, it = 0;
+  const int chunk = 16;
+  while (!done) {
+    for (int k = 0; k < chunk; ++k) {
+      const int tk = prof::begin(s);
+      MIFE_DISPATCH_FMT(M.a, M.L, (minres_spmv_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, v, t, r1, wv, w1,
+                                                                                       w2, x, st)));
+      prof::end(tk, s, prof::kMinresSpmv);
+      minres_fin1_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
+      minres_update_kernel<<<vg, 256, 0, s>>>(n, t, r1, r2, z, dinv, st);
+      minres_fin2_kernel<<<1, kFinThreads, 0, s>>>(st, vg);
+    }
+    // one more spmv/fin pair applies the last pending update (no-op otherwise)
+    MIFE_DISPATCH_FMT(M.a, M.L, (minres_spmv_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, v, t, r1, wv, w1, w2,
+                                                                                     x, st)));
+    minres_fin1_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
+    MIFE_CHECK_HIP(hipGetLastError());
+    int hd[4];
+    MIFE_CHECK_HIP(hipMemcpyAsync(hd, &st->iters, sizeof(hd), hipMemcpyDeviceToHost, s));
+    MIFE_CHECK_HIP(hipStreamSynchronize(s));
+    it = hd[0];
+    done = hd[2];
+  }
+  // true residual at exit
+  MIFE_CHECK_HIP(hipMemsetAsync(rs, 0, sizeof(CgState), s));
+  launch_residual64(M, vals, b, x, nullptr, rs, s);
+  HostState hs{};
+  rc = read_state(rs, &hs, s);
+  if (rc) return rc;
+  permute_out_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x, x_io);
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
+  const double rel = hs.bb > 0 ? sqrt(hs.rr64 / hs.bb) : 0.0;
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  if (it >= maxit && maxit > 0) {
+    MinresState h{};
+    MIFE_CHECK_HIP(hipMemcpy(&h, st, offsetof(MinresState, part_a), hipMemcpyDeviceToHost));
+    if (h.phibar > h.tol * h.beta1) {
+      set_error("MINRES reached the maximum of %d iterations (relative residual %.3e)", maxit, rel);
+      return MIFE_ERR_NOT_CONVERGED;
+    }
+  }
+  return MIFE_OK;
+}
+
+int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, const double* pc_inv,
+               double rtol, int restart, int maxit, void* work, int* iters, double* relres, void* stream) {
   MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(restart >= 1 && restart <= 31 && rtol > 0.0, MIFE_ERR_ARG, "invalid GMRES arguments");
   Mat M;
@@ -680,6 +927,7 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
+  if (pc_inv) pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 0);
   MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
   launch_residual64(M, vals, b, x, r, st, s);
   HostState hs{};

@@ -12,7 +12,8 @@ static const char* kNames[kNumKernels] = {"cg_spmv",          "cg_update",      
                                           "residual64",       "assemble_scalar",   "assemble_elastic",
                                           "gather_values",    "finite_elasticity", "grouped_spmv",
                                           "grouped_update",   "cg_spmv_f32",       "cg_update_f32",
-                                          "assemble_scalar_f32", "assemble_elastic_f32"};
+                                          "assemble_scalar_f32", "assemble_elastic_f32",
+                                          "minres_spmv"};
 
 struct Interval {
   hipEvent_t a, b;

@@ -20,6 +20,7 @@ enum Kernel : int {
   kCgUpdate32,
   kAssembleScalar32,
   kAssembleElastic32,
+  kMinresSpmv,
   kNumKernels
 };
 

@@ -240,9 +240,20 @@ def refine(A: SolverMatrix, vals64, vals32, b, x, rtol, maxit, ws: KrylovWorkspa
     return SolveInfo(it.value, rr.value, outer.value)
 
 
-def gmres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, restart=30) -> SolveInfo:
+def gmres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, restart=30, pc_inv=None) -> SolveInfo:
+    """Restarted GMRES; pc_inv (FP64, solver-row order) overrides the Jacobi entry
+    of rows where it is nonzero."""
     it, rr = C.c_int(0), C.c_double(0.0)
-    call("mife_gmres", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), float(rtol), int(restart), int(maxit),
+    call("mife_gmres", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), ptr(pc_inv), float(rtol), int(restart),
+         int(maxit), ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
+    return SolveInfo(it.value, rr.value)
+
+
+def minres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, pc_inv=None) -> SolveInfo:
+    """Preconditioned MINRES for a symmetric (indefinite) solver matrix; rtol applies to
+    the preconditioned residual norm, SolveInfo.relres is the true relative residual."""
+    it, rr = C.c_int(0), C.c_double(0.0)
+    call("mife_minres", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), ptr(pc_inv), float(rtol), int(maxit),
          ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
     return SolveInfo(it.value, rr.value)
 

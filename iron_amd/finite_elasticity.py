@@ -1,0 +1,356 @@
+"""Mooney-Rivlin finite elasticity on the GPU: element residual/Jacobian kernel,
+load-increment control loop and Newton iteration.
+
+Reference call stack (paths relative to the reference's root):
+  Problem_Solve -> load-increment control loop (CONTROL_LOOP_LOAD_INCREMENT):
+    EQUATIONS_SET_BOUNDARY_CONDITIONS_INCREMENT (equations_set_routines.F90:6261-6550)
+    FINITE_ELASTICITY_LOAD_INCREMENT_APPLY (finite_elasticity_routines.F90:13315-13360)
+    SOLVER_NEWTON_LINESEARCH_SOLVE -> SNESSolve(b = gravity RHS) (solver_routines.F90:19062-19220)
+      residual: FiniteElasticity_FiniteElementResidualEvaluate (:1382-3506) + ElementAdd
+      Jacobian: FiniteElasticity_FiniteElementJacobianEvaluate (:879-1296) or FD
+      linear solve: KSP on the free rows
+    EQUATIONS_SET_NONLINEAR_RHS_UPDATE (equations_set_routines.F90:1919-2052)
+
+Here the residual, the exact Jacobian and the gravity vector come from one HIP
+kernel (mife_mr_assemble), the Newton system is the free-row reduction of the
+Jacobian in SELL-64 storage, and the Newton step is solved with preconditioned
+MINRES (symmetric deviatoric form) or GMRES (general form) whose diagonal
+preconditioner uses 1/|a_ii| on displacement rows and the element-lumped
+pressure mass scaled by 1/(2(c1+c2)) on pressure rows.  The line search is
+backtracking with the Armijo factor 1e-4 (SNES's default alpha, solver_routines.F90:18942-18991).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, fem
+from ._lib import CMFEError, MrDesc, call, ptr
+from .basis import LagrangeBasis
+from .mesh import RegularMesh, structured_colours
+
+DEVIATORIC = _lib.MIFE_MR_DEVIATORIC
+CAUCHY = _lib.MIFE_MR_CAUCHY
+
+
+class MRSystem:
+    """Device structures of a Mooney-Rivlin problem on a regular mesh.
+
+    Displacement (and geometry) on mesh component 0 (``rm.bases[0]``); pressure
+    either node based on mesh component ``p_mesh_comp`` or element based
+    (``p_mesh_comp=None``).  DOF layout (separated order): x over the
+    displacement nodes, then y, then z, then the pressure DOFs."""
+
+    def __init__(self, rm: RegularMesh, p_mesh_comp: int | None, formulation: int = DEVIATORIC, device=None):
+        if rm.dim != 3:
+            raise CMFEError(1, "finite elasticity is implemented for 3-D meshes")
+        dev = fem._dev(device)
+        self.device = dev
+        self.rm = rm
+        self.formulation = int(formulation)
+        bu: LagrangeBasis = rm.bases[0]
+        self.bu = bu
+        self.nu = rm.num_nodes(0)
+        en_u = rm.elem_nodes[0]
+        E = rm.num_elements
+        if p_mesh_comp is None:
+            self.bp = None
+            self.npd = E
+            en_p = np.arange(E, dtype=np.int32)[:, None]
+            self.n_pressure = 1
+        else:
+            bp: LagrangeBasis = rm.bases[p_mesh_comp]
+            if bp.gauss_xi != bu.gauss_xi:
+                raise CMFEError(1, "the pressure basis must use the displacement basis' Gauss scheme")
+            self.bp = bp
+            self.npd = rm.num_nodes(p_mesh_comp)
+            en_p = rm.elem_nodes[p_mesh_comp]
+            self.n_pressure = bp.num_nodes
+        self.ndofs = 3 * self.nu + self.npd
+        self.elem_dofs = np.ascontiguousarray(
+            np.concatenate([en_u + c * self.nu for c in range(3)] + [en_p + 3 * self.nu], axis=1).astype(np.int32))
+        self.en_p = en_p
+        self.csr = fem.CSRStructure(self.elem_dofs, self.ndofs, dev, colours=structured_colours(rm))
+        self.db = fem.DeviceBasis(bu, dev)
+        self.Np = None
+        if self.bp is not None:
+            self.Np = torch.as_tensor(np.ascontiguousarray(self.bp.N), dtype=torch.float64, device=dev)
+        self.coords = torch.as_tensor(np.ascontiguousarray(rm.coords, dtype=np.float64), device=dev)
+        self.elem_nodes = torch.as_tensor(np.ascontiguousarray(en_u, dtype=np.int32), device=dev)
+        self.desc = MrDesc(C.pointer(self.db.desc), self.n_pressure, ptr(self.Np), self.formulation)
+
+    def assemble(self, state, material, body_force=None, residual=None, jac=None, rhs=None, atomic=False):
+        """Accumulate the residual, the Jacobian (equations-CSR values) and the gravity
+        vector at ``state`` into the given (zeroed) device arrays."""
+        cs = self.csr
+        bf = None
+        if body_force is not None:
+            bf = (C.c_double * 3)(*[float(v) for v in body_force])
+        call("mife_mr_assemble", C.byref(self.desc), self.rm.num_elements, ptr(self.coords), ptr(self.elem_nodes),
+             ptr(cs.elem_dofs), ptr(state), ptr(material), C.cast(bf, C.c_void_p) if bf is not None else None,
+             ptr(cs.pos) if jac is not None else None, None if atomic else ptr(cs.order),
+             None if atomic else cs.colour_ptr.ctypes.data, cs.n_colours, ptr(residual), ptr(jac), ptr(rhs),
+             fem._stream())
+
+    def pressure_mass(self, mu_e: np.ndarray) -> np.ndarray:
+        """Lumped sum_e (1/mu_e) int psi_q dV over the pressure DOFs ([ndofs], zero on
+        displacement DOFs): the diagonal Schur-complement estimate."""
+        bu = self.bu
+        X = self.rm.coords[self.rm.elem_nodes[0]]                  # [E, nn, 3]
+        J = np.einsum("ead,gaj->egdj", X, bu.dN)
+        wj = np.abs(np.linalg.det(J)) * bu.weights[None, :]         # [E, ng]
+        psi = np.ones((bu.num_gauss, 1)) if self.bp is None else self.bp.N
+        m = np.einsum("eg,gq->eq", wj, psi) / mu_e[:, None]
+        out = np.zeros(self.ndofs)
+        np.add.at(out, 3 * self.nu + self.en_p, m)
+        return out
+
+
+@dataclass
+class NewtonOptions:
+    """SNES settings (iron defaults, solver_routines.F90:17960-18030)."""
+    atol: float = 1e-10
+    rtol: float = 1e-5
+    stol: float = 1e-5
+    maxit: int = 50
+    linear_rtol: float = 1e-5
+    linear_maxit: int = 100000
+    linear: str = "auto"          # "minres", "gmres" or "auto" (by formulation)
+    gmres_restart: int = 30
+
+
+@dataclass
+class NewtonInfo:
+    iterations: int = 0
+    linear_iterations: int = 0
+    fnorm0: float = 0.0
+    fnorm: float = 0.0
+    reason: str = ""
+    history: list = field(default_factory=list)
+
+
+class MRNewton:
+    """Newton iteration on the free DOFs of an MRSystem with a fixed set of
+    constrained DOFs (their values are set in the state before each solve)."""
+
+    def __init__(self, sys: MRSystem, fixed_mask: np.ndarray, material: np.ndarray):
+        self.sys = sys
+        dev = sys.device
+        self.red = fem.DirichletReduction(sys.csr, fixed_mask)
+        self.S = fem.SolverMatrix(self.red)
+        self.ws = fem.KrylovWorkspace(max(self.red.n_free, 1), dev, restart=30)
+        self.free = self.red.free_idx.long()
+        mat = np.ascontiguousarray(material, dtype=np.float64).reshape(-1, 2)
+        self.material = torch.as_tensor(mat, device=dev)
+        mu = 2.0 * (mat[:, 0] + mat[:, 1])
+        pm = sys.pressure_mass(np.where(mu > 0, mu, 1.0))[~np.asarray(fixed_mask, dtype=bool)]
+        pc = np.where(pm > 0, 1.0 / np.where(pm > 0, pm, 1.0), 0.0)
+        self.pc_pos = torch.as_tensor(pc, device=dev)           # SPD form (MINRES)
+        self.pc_neg = torch.as_tensor(-pc, device=dev)          # GMRES: negative Schur block
+        self.R = torch.zeros(sys.ndofs, dtype=torch.float64, device=dev)
+        self.f = torch.zeros(sys.ndofs, dtype=torch.float64, device=dev)
+        self.J = sys.csr.zeros()
+        self.Jff = None
+
+    def _eval(self, x, body_force, jac: bool):
+        self.R.zero_()
+        self.f.zero_()
+        if jac:
+            self.J.zero_()
+        self.sys.assemble(x, self.material, body_force=body_force, residual=self.R,
+                          jac=self.J if jac else None, rhs=self.f if body_force is not None else None)
+        return (self.R - self.f).index_select(0, self.free)
+
+    def internal_residual(self, x):
+        self.R.zero_()
+        self.sys.assemble(x, self.material, residual=self.R)
+        return self.R.clone()
+
+    def solve(self, x, body_force=None, opts: NewtonOptions | None = None) -> NewtonInfo:
+        """Newton with backtracking on ||F||, F = R_int(x) - f_gravity(x) on the free rows;
+        x (device, FP64, all DOFs) is updated in place."""
+        o = opts or NewtonOptions()
+        info = NewtonInfo()
+        if self.red.n_free == 0:
+            info.reason = "no free DOFs"
+            return info
+        linear = o.linear
+        if linear == "auto":
+            linear = "minres" if self.sys.formulation == DEVIATORIC else "gmres"
+        F = self._eval(x, body_force, True)
+        fn = float(torch.linalg.vector_norm(F).item())
+        info.fnorm0 = fn
+        info.history.append(fn)
+        while True:
+            if not np.isfinite(fn):
+                raise CMFEError(1, "Nonlinear line search solver did not converge. PETSc diverged F Norm NaN.")
+            if fn <= o.atol:
+                info.reason = "converged F norm absolute"
+                break
+            if info.iterations > 0 and fn <= o.rtol * info.fnorm0:
+                info.reason = "converged F norm relative"
+                break
+            if info.iterations >= o.maxit:
+                raise CMFEError(1, "Nonlinear line search solver did not converge. PETSc diverged maximum "
+                                   "iterations.")
+            self.Jff = self.S.gather(self.J, fem.F64, out=self.Jff)
+            dx = torch.zeros(self.red.n_free, dtype=torch.float64, device=x.device)
+            b = -F
+            if linear == "minres":
+                li = fem.minres(self.S, self.Jff, b, dx, o.linear_rtol, o.linear_maxit, self.ws, pc_inv=self.pc_pos)
+            else:
+                li = fem.gmres(self.S, self.Jff, b, dx, o.linear_rtol, o.linear_maxit, self.ws,
+                               restart=o.gmres_restart, pc_inv=self.pc_neg)
+            info.linear_iterations += li.iterations
+            lam = 1.0
+            xt = x.clone()
+            stalled = False
+            while True:
+                xt.copy_(x)
+                xt.index_add_(0, self.free, dx, alpha=lam)
+                Ft = self._eval(xt, body_force, False)
+                ft = float(torch.linalg.vector_norm(Ft).item())
+                if np.isfinite(ft) and ft <= (1.0 - 1e-4 * lam) * fn:
+                    break
+                lam *= 0.5
+                if lam < 1e-6:
+                    stalled = True
+                    break
+            if stalled:
+                # no decrease along an exact Newton direction: ||F|| sits at round-off
+                if fn <= 1e-10 * max(info.fnorm0, 1.0):
+                    info.reason = "converged F norm at round-off"
+                    break
+                raise CMFEError(1, "Nonlinear line search solver did not converge. PETSc diverged line search.")
+            snorm = lam * float(torch.linalg.vector_norm(dx).item())
+            x.copy_(xt)
+            info.iterations += 1
+            F = self._eval(x, body_force, True)
+            fn = float(torch.linalg.vector_norm(F).item())
+            info.history.append(fn)
+            xnorm = float(torch.linalg.vector_norm(x.index_select(0, self.free)).item())
+            if snorm < o.stol * xnorm and fn > o.atol and fn > o.rtol * info.fnorm0:
+                info.reason = "converged S norm relative"
+                break
+        info.fnorm = fn
+        return info
+
+
+def load_increments(newton: MRNewton, x, n_increments: int, incremented: np.ndarray | None = None,
+                    full_value: np.ndarray | None = None, body_force=None, opts: NewtonOptions | None = None):
+    """The load-increment control loop: iteration i of N moves each FIXED_INCREMENTED
+    DOF to current + (full - current)/(N - i + 1) (equations_set_routines.F90:6340-6350)
+    and scales the gravity body force by i/N when N > 1
+    (finite_elasticity_routines.F90:13333-13350), then runs Newton."""
+    infos = []
+    dev = x.device
+    inc_idx = None
+    if incremented is not None and np.any(incremented):
+        inc_idx = torch.as_tensor(np.nonzero(incremented)[0], device=dev)
+        full = torch.as_tensor(np.asarray(full_value, dtype=np.float64)[incremented], device=dev)
+    for i in range(1, n_increments + 1):
+        if inc_idx is not None:
+            cur = x.index_select(0, inc_idx)
+            x.index_copy_(0, inc_idx, cur + (full - cur) / float(n_increments - i + 1))
+        bf = None
+        if body_force is not None:
+            scale = i / n_increments if n_increments > 1 else 1.0
+            bf = [float(v) * scale for v in body_force]
+        infos.append(newton.solve(x, bf, opts))
+    return infos
+
+
+# ------------------------------------------------------- iron API solve
+def solve(problem, es, solver, bc):
+    """Problem.Solve() for a FINITE_ELASTICITY equations set (MOONEY_RIVLIN subtype,
+    or NO_SUBTYPE for the general branch)."""
+    from . import engine
+    from . import iron
+    rm: RegularMesh = es.geometricField._mesh()
+    dep = es.dependentField
+    U = iron.FieldVariableTypes.U
+    var = dep._var(U)
+    if var.ncomp != 4:
+        raise CMFEError(1, "the finite-elasticity dependent field needs 3 displacement components and a pressure")
+    if any(mc != 1 for mc in var.mesh_comp[:3]) or any(
+            it != iron.FieldInterpolationTypes.NODE_BASED for it in var.interp[:3]):
+        raise CMFEError(1, "displacement components must be node based on mesh component 1")
+    if var.interp[3] == iron.FieldInterpolationTypes.ELEMENT_BASED:
+        p_mc = None
+    elif var.interp[3] == iron.FieldInterpolationTypes.NODE_BASED:
+        p_mc = var.mesh_comp[3] - 1
+    else:
+        raise CMFEError(1, "the pressure must be element based or node based")
+    sub = es.specification[2]
+    form = DEVIATORIC if sub == iron.EquationsSetSubtypes.MOONEY_RIVLIN else CAUCHY
+    cache = getattr(es, "_mr_cache", None)
+    key = (id(rm), p_mc, form)
+    if cache is None or cache[0] != key:
+        sys = MRSystem(rm, p_mc, form)
+        cache = (key, sys, {})
+        es._mr_cache = cache
+    sys = cache[1]
+    # boundary conditions on U: FIXED sets/adds the value, FIXED_INCREMENTED ramps to it
+    n = sys.ndofs
+    if dep._data(U).shape[0] != n:
+        raise CMFEError(1, "dependent field layout does not match the finite-elasticity DOF layout")
+    x_host = dep._data(U).copy()
+    fixed = np.zeros(n, dtype=bool)
+    incremented = np.zeros(n, dtype=bool)
+    full = np.zeros(n)
+    for (fld, vt, node, comp, cond, value, add) in bc._entries:
+        if vt != U:
+            raise CMFEError(1, "finite elasticity supports boundary conditions on the U variable only")
+        if comp > 3 and p_mc is None:
+            raise CMFEError(1, "element-based pressure DOFs cannot carry node boundary conditions")
+        mc = var.mesh_comp[comp - 1] - 1
+        dof = int(dep._offset(U, comp) + rm.local_node(node, mc))
+        fixed[dof] = True
+        if cond == iron.BoundaryConditionsTypes.FIXED:
+            x_host[dof] = (x_host[dof] + value) if add else value
+        elif cond == iron.BoundaryConditionsTypes.FIXED_INCREMENTED:
+            incremented[dof] = True
+            full[dof] = (dep._data(U)[dof] + value) if add else value
+        else:
+            raise CMFEError(1, f"boundary condition type {cond} is not supported for finite elasticity")
+    # materials: c1, c2 per element; density (V) and gravity (source) -> body force
+    mf = es.materialsField
+    mat = np.stack([engine._element_values(mf, 1, rm), engine._element_values(mf, 2, rm)], axis=1)
+    body_force = None
+    if es.sourceField is not None and mf is not None and iron.FieldVariableTypes.V in mf._vars:
+        rho = engine._element_values(mf, 1, rm, vt=iron.FieldVariableTypes.V)
+        g = np.stack([engine._element_values(es.sourceField, c, rm) for c in (1, 2, 3)], axis=1)
+        bf = rho[:, None] * g
+        if not np.allclose(bf, bf[0]):
+            raise CMFEError(1, "a spatially varying body force is not supported")
+        if np.any(bf[0] != 0):
+            body_force = bf[0]
+    fkey = (fixed.tobytes(), mat.tobytes())
+    newton = cache[2].get(fkey)
+    if newton is None:
+        newton = MRNewton(sys, fixed, mat)
+        cache[2].clear()
+        cache[2][fkey] = newton
+    lin = solver._linear
+    opts = NewtonOptions(atol=float(solver.newtonAbsoluteTolerance), rtol=float(solver.newtonRelativeTolerance),
+                         stol=float(getattr(solver, "newtonSolutionTolerance", 1e-5)),
+                         maxit=int(solver.newtonMaximumIterations))
+    if lin is not None:
+        if int(lin.linearType) == iron.LinearSolverTypes.DIRECT:
+            opts.linear_rtol = 1e-12  # no GPU factorisation: iterate to round-off instead
+        else:
+            opts.linear_rtol = float(lin.linearIterativeRelativeTolerance)
+            opts.linear_maxit = int(lin.linearIterativeMaximumIterations)
+            opts.gmres_restart = int(lin.linearIterativeGMRESRestart)
+    x = torch.as_tensor(x_host, device=sys.device)
+    n_inc = max(int(problem._control_loop.maximumIterations), 1)
+    infos = load_increments(newton, x, n_inc, incremented, full, body_force, opts)
+    # EQUATIONS_SET_NONLINEAR_RHS_UPDATE: DELUDELN = assembled residual (RHS-free DOFs)
+    R = newton.internal_residual(x)
+    dep._data(U)[:] = x.cpu().numpy()
+    dep._data(iron.FieldVariableTypes.DELUDELN)[:] = R.cpu().numpy()
+    solver.last_info = infos
+    return infos

@@ -568,7 +568,7 @@ class EquationsSet(_Obj):
               or (cls == EquationsSetClasses.ELASTICITY and typ == EquationsSetTypes.LINEAR_ELASTICITY
                   and sub == 4)
               or (cls == EquationsSetClasses.ELASTICITY and typ == EquationsSetTypes.FINITE_ELASTICITY
-                  and sub == 1))
+                  and sub in (0, 1)))
         if not ok:
             raise CMFEError(1, f"equations set specification {self.specification} is not implemented")
 
@@ -696,8 +696,10 @@ class Solver(_Obj):
         self.linearIterativePrecision = "fp64"
         # Newton defaults, solver_routines.F90:17960-18030
         self.newtonMaximumIterations = 50
+        self.newtonMaximumFunctionEvaluations = 1000
         self.newtonRelativeTolerance = 1.0e-5
         self.newtonAbsoluteTolerance = 1.0e-10
+        self.newtonSolutionTolerance = 1.0e-5
         self.newtonJacobianCalculationType = JacobianCalculationTypes.FD
         self._problem = None
         self._linear = None

@@ -1,0 +1,306 @@
+"""GPU parity of the Mooney-Rivlin path (mife_mr_assemble, MINRES/GMRES Newton)
+against the FP64 oracle (oracle/mr_oracle.py), the closed-form uniaxial stretch and
+the reference's golden cantilever outputs.
+
+Tolerances: element residual/Jacobian/gravity within 1e-12 of the largest entry
+(FP64, different but equivalent arithmetic: reference-configuration P versus the
+Cauchy-stress form); Newton solutions within 1e-9 of the oracle / closed form.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import mr_oracle as mo  # noqa: E402
+from oracle.fe_oracle import Basis, regular_mesh  # noqa: E402
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _systems(ne, order, elem_p, form, extent=(1.0, 1.0, 1.0), c1=2.0, c2=6.0):
+    from iron_amd import basis, finite_elasticity as fe, mesh
+    interp = {1: 1, 2: 2}[order]
+    bu = basis.basis(3, interp, order + 1)
+    bases = [bu] if elem_p else [bu, basis.basis(3, 1, order + 1)]
+    rm = mesh.RegularMesh(ne, bases, extent)
+    sysg = fe.MRSystem(rm, None if elem_p else 1, fe.DEVIATORIC if form == "deviatoric" else fe.CAUCHY)
+    obu = Basis(3, order, order + 1)
+    om = regular_mesh(ne, [obu] if elem_p else [obu, Basis(3, 1, order + 1)], extent)
+    E = om.num_elements
+    prob = mo.MRProblem(om, elem_p, np.full(E, c1), np.full(E, c2), formulation=form)
+    assert sysg.ndofs == prob.ndofs
+    assert np.array_equal(sysg.elem_dofs, prob.edofs)
+    return sysg, prob
+
+
+def _csr_dense(sysg, vals):
+    rp, col = sysg.csr.to_host()
+    v = vals.cpu().numpy()
+    A = np.zeros((sysg.ndofs, sysg.ndofs))
+    for i in range(sysg.ndofs):
+        A[i, col[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    return A
+
+
+@pytest.mark.parametrize("form", ["deviatoric", "cauchy"])
+@pytest.mark.parametrize("order,elem_p", [(1, True), (2, False), (1, False)])
+def test_residual_jacobian_gravity_match_oracle(form, order, elem_p):
+    print(f"case form={form} order={order} elem_p={elem_p}", flush=True)
+    sysg, prob = _systems((2, 2, 1), order, elem_p, form, extent=(2.0, 1.5, 1.0))
+    rng = np.random.default_rng(7)
+    E = prob.mesh.num_elements
+    prob.c1 = rng.uniform(1.0, 3.0, E)
+    prob.c2 = rng.uniform(0.5, 6.0, E)
+    x = prob.initial_state(-3.0)
+    x[:3 * prob.nu] += 0.04 * rng.standard_normal(3 * prob.nu)
+    x[3 * prob.nu:] += rng.standard_normal(prob.npd)
+    prob.density, prob.gravity = 2.0, (0.3, -0.2, -9.8)
+    R_o, f_o = prob.residual(x)
+    K_o = prob.jacobian(x).toarray()
+    dev = sysg.device
+    xs = torch.as_tensor(x, device=dev)
+    mat = torch.as_tensor(np.stack([prob.c1, prob.c2], axis=1), device=dev)
+    R = torch.zeros(sysg.ndofs, dtype=torch.float64, device=dev)
+    f = torch.zeros_like(R)
+    J = sysg.csr.zeros()
+    sysg.assemble(xs, mat, body_force=[2.0 * 0.3, 2.0 * -0.2, 2.0 * -9.8], residual=R, jac=J, rhs=f)
+    torch.cuda.synchronize()
+    Rg = R.cpu().numpy()
+    assert np.abs(Rg - R_o).max() <= 1e-12 * np.abs(R_o).max()
+    assert np.abs(f.cpu().numpy() - f_o).max() <= 1e-12 * np.abs(f_o).max()
+    Kg = _csr_dense(sysg, J)
+    assert np.abs(Kg - K_o).max() <= 1e-12 * np.abs(K_o).max()
+    # atomic scatter agrees with the coloured one
+    R2 = torch.zeros_like(R)
+    J2 = sysg.csr.zeros()
+    sysg.assemble(xs, mat, residual=R2, jac=J2, atomic=True)
+    torch.cuda.synchronize()
+    assert torch.allclose(R2, R, rtol=0, atol=1e-12 * float(R.abs().max()))
+    assert torch.allclose(J2, J, rtol=0, atol=1e-12 * float(J.abs().max()))
+
+
+def _uniaxial(sysg, prob, lam):
+    nu = prob.nu
+    X = prob.geo
+    fixed = np.zeros(prob.ndofs, bool)
+    target = prob.initial_state(0.0)
+    for c in range(3):
+        fixed[c * nu:(c + 1) * nu] |= X[:, c] == 0.0
+    fixed[:nu] |= X[:, 0] == 1.0
+    target[:nu][X[:, 0] == 1.0] = lam
+    return fixed, target
+
+
+@pytest.mark.parametrize("form", ["deviatoric", "cauchy"])
+def test_uniaxial_stretch_gpu_newton(form):
+    from iron_amd import finite_elasticity as fe
+    print(f"case uniaxial 2x2x2 Q2/Q1 form={form}", flush=True)
+    lam, c1, c2 = 1.2, 2.0, 6.0
+    sysg, prob = _systems((2, 2, 2), 2, False, form, c1=c1, c2=c2)
+    fixed, target = _uniaxial(sysg, prob, lam)
+    mat = np.tile([c1, c2], (prob.mesh.num_elements, 1))
+    newton = fe.MRNewton(sysg, fixed, mat)
+    p0 = 0.0 if form == "deviatoric" else -2.0 * (c1 + 2.0 * c2)
+    x = torch.as_tensor(prob.initial_state(p0), device=sysg.device)
+    opts = fe.NewtonOptions(atol=1e-11, rtol=1e-12, stol=0.0, linear_rtol=1e-12)
+    infos = fe.load_increments(newton, x, 2, fixed, target, opts=opts)
+    z, p = prob.split(x.cpu().numpy())
+    exact = prob.geo * np.array([lam, lam ** -0.5, lam ** -0.5])
+    assert np.abs(z - exact).max() < 1e-9
+    B = np.diag([lam * lam, 1 / lam, 1 / lam])
+    i1 = np.trace(B)
+    if form == "deviatoric":
+        s = 2 * (c1 + c2 * i1) * B - 2 * c2 * B @ B
+        pe = (s[0, 0] - s[1, 1]) / 3
+    else:
+        s = 2 * c1 * B + 2 * c2 * (i1 * B - B @ B)
+        pe = -s[1, 1]
+    assert np.abs(p - pe).max() < 1e-8
+    assert all(i.iterations <= 8 for i in infos)
+
+
+def test_newton_matches_oracle_random_bcs():
+    """A non-homogeneous case: clamped x=0, prescribed shear+stretch at x=1, gravity."""
+    from iron_amd import finite_elasticity as fe
+    sysg, prob = _systems((3, 2, 2), 2, False, "deviatoric", extent=(3.0, 1.0, 1.0))
+    nu, X = prob.nu, prob.geo
+    fixed = np.zeros(prob.ndofs, bool)
+    target = prob.initial_state(0.0)
+    for c in range(3):
+        fixed[c * nu:(c + 1) * nu] |= (X[:, 0] == 0.0) | (X[:, 0] == 3.0)
+    end = X[:, 0] == 3.0
+    target[:nu][end] += 0.3
+    target[2 * nu:3 * nu][end] -= 0.2
+    prob.density, prob.gravity = 1.0, (0.0, 0.0, -0.5)
+    x_o, R_o = mo.solve(prob, prob.initial_state(0.0), fixed, incremented=fixed.copy(), full_value=target,
+                        increments=3)
+    mat = np.tile([2.0, 6.0], (prob.mesh.num_elements, 1))
+    newton = fe.MRNewton(sysg, fixed, mat)
+    x = torch.as_tensor(prob.initial_state(0.0), device=sysg.device)
+    opts = fe.NewtonOptions(atol=1e-12, rtol=1e-13, stol=0.0, linear_rtol=1e-13)
+    fe.load_increments(newton, x, 3, fixed, target, body_force=[0.0, 0.0, -0.5], opts=opts)
+    xg = x.cpu().numpy()
+    assert np.abs(xg[:3 * nu] - x_o[:3 * nu]).max() < 1e-9
+    assert np.abs(xg[3 * nu:] - x_o[3 * nu:]).max() < 1e-8 * max(1.0, np.abs(x_o[3 * nu:]).max())
+    Rg = newton.internal_residual(x).cpu().numpy()
+    assert np.abs(Rg - R_o).max() < 1e-7 * np.abs(R_o).max()
+
+
+def _cantilever_script(g):
+    """bindings/python/tests/Cantilever.py / tests/FiniteElasticity/Cantilever.F90 through
+    the iron-compatible API with the NO_SUBTYPE (general-branch) law."""
+    from iron_amd import iron
+    s = g["setup"]
+    nx, ny, nz = s["n_elem"]
+    cs = iron.CoordinateSystem()
+    cs.CreateStart(1)
+    cs.DimensionSet(3)
+    cs.CreateFinish()
+    region = iron.Region()
+    region.CreateStart(1, iron.WorldRegion)
+    region.coordinateSystem = cs
+    region.CreateFinish()
+    basis = iron.Basis()
+    basis.CreateStart(1)
+    basis.type = iron.BasisTypes.LAGRANGE_HERMITE_TP
+    basis.numberOfXi = 3
+    basis.interpolationXi = [iron.BasisInterpolationSpecifications.LINEAR_LAGRANGE] * 3
+    basis.quadratureNumberOfGaussXi = [2] * 3
+    basis.CreateFinish()
+    gm = iron.GeneratedMesh()
+    gm.CreateStart(1, region)
+    gm.type = iron.GeneratedMeshTypes.REGULAR
+    gm.basis = [basis]
+    gm.extent = list(s["extent"])
+    gm.numberOfElements = [nx, ny, nz]
+    mesh = iron.Mesh()
+    gm.CreateFinish(1, mesh)
+    dec = iron.Decomposition()
+    dec.CreateStart(1, mesh)
+    dec.CreateFinish()
+    geo = iron.Field()
+    geo.CreateStart(1, region)
+    geo.MeshDecompositionSet(dec)
+    geo.CreateFinish()
+    gm.GeometricParametersCalculate(geo)
+    es = iron.EquationsSet()
+    esf = iron.Field()
+    es.CreateStart(1, region, geo, [iron.EquationsSetClasses.ELASTICITY, iron.EquationsSetTypes.FINITE_ELASTICITY,
+                                    iron.EquationsSetSubtypes.NONE], 6, esf)
+    es.CreateFinish()
+    dep = iron.Field()
+    es.DependentCreateStart(4, dep)
+    dep.ComponentInterpolationSet(iron.FieldVariableTypes.U, 4, iron.FieldInterpolationTypes.ELEMENT_BASED)
+    dep.ComponentInterpolationSet(iron.FieldVariableTypes.DELUDELN, 4, iron.FieldInterpolationTypes.ELEMENT_BASED)
+    es.DependentCreateFinish()
+    for c in (1, 2, 3):
+        iron.Field.ParametersToFieldParametersComponentCopy(geo, iron.FieldVariableTypes.U,
+                                                           iron.FieldParameterSetTypes.VALUES, c, dep,
+                                                           iron.FieldVariableTypes.U,
+                                                           iron.FieldParameterSetTypes.VALUES, c)
+    # the golden files' pressure is half of this law's p (see the oracle)
+    iron.Field.ComponentValuesInitialiseDP(dep, iron.FieldVariableTypes.U, iron.FieldParameterSetTypes.VALUES, 4,
+                                           2.0 * s["pressure0"])
+    mat = iron.Field()
+    es.MaterialsCreateStart(3, mat)
+    es.MaterialsCreateFinish()
+    mat.ComponentValuesInitialiseDP(iron.FieldVariableTypes.U, iron.FieldParameterSetTypes.VALUES, 1, s["c1"])
+    mat.ComponentValuesInitialiseDP(iron.FieldVariableTypes.U, iron.FieldParameterSetTypes.VALUES, 2, s["c2"])
+    mat.ComponentValuesInitialiseDP(iron.FieldVariableTypes.V, iron.FieldParameterSetTypes.VALUES, 1, s["density"])
+    src = iron.Field()
+    es.SourceCreateStart(5, src)
+    es.SourceCreateFinish()
+    for c in (1, 2, 3):
+        src.ComponentValuesInitialiseDP(iron.FieldVariableTypes.U, iron.FieldParameterSetTypes.VALUES, c,
+                                        s["gravity"][c - 1])
+    eq = iron.Equations()
+    es.EquationsCreateStart(eq)
+    es.EquationsCreateFinish()
+    problem = iron.Problem()
+    problem.CreateStart(1, [iron.ProblemClasses.ELASTICITY, iron.ProblemTypes.FINITE_ELASTICITY,
+                            iron.ProblemSubtypes.NONE])
+    problem.CreateFinish()
+    problem.ControlLoopCreateStart()
+    loop = iron.ControlLoop()
+    problem.ControlLoopGet([iron.ControlLoopIdentifiers.NODE], loop)
+    loop.MaximumIterationsSet(s["increments"])
+    problem.ControlLoopCreateFinish()
+    nls, ls = iron.Solver(), iron.Solver()
+    problem.SolversCreateStart()
+    problem.SolverGet([iron.ControlLoopIdentifiers.NODE], 1, nls)
+    nls.NewtonJacobianCalculationTypeSet(iron.JacobianCalculationTypes.EQUATIONS)
+    nls.newtonAbsoluteTolerance = 1e-12
+    nls.newtonRelativeTolerance = 1e-13
+    nls.newtonSolutionTolerance = 0.0
+    nls.NewtonLinearSolverGet(ls)
+    ls.linearType = iron.LinearSolverTypes.DIRECT
+    problem.SolversCreateFinish()
+    solver, se = iron.Solver(), iron.SolverEquations()
+    problem.SolverEquationsCreateStart()
+    problem.SolverGet([iron.ControlLoopIdentifiers.NODE], 1, solver)
+    solver.SolverEquationsGet(se)
+    se.EquationsSetAdd(es)
+    problem.SolverEquationsCreateFinish()
+    bc = iron.BoundaryConditions()
+    se.BoundaryConditionsCreateStart(bc)
+    rm = geo._mesh()
+    for k, user in enumerate(rm.comp_user[0]):
+        if rm.coords[k, 0] == 0.0:
+            for c in (1, 2, 3):
+                bc.AddNode(dep, iron.FieldVariableTypes.U, 1, 1, int(user), c, iron.BoundaryConditionsTypes.FIXED, 0.0)
+    se.BoundaryConditionsCreateFinish()
+    problem.Solve()
+    return dep, rm
+
+
+@pytest.mark.parametrize("name", ["mr_cantilever_python_1x1x1.json", "mr_cantilever_linear_3x2x2.json"])
+def test_cantilever_goldens_through_api(name):
+    from iron_amd import iron
+    print(f"case {name}", flush=True)
+    g = json.loads((GOLDEN / name).read_text())
+    dep, rm = _cantilever_script(g)
+    nodes = rm.comp_user[0]
+    z = np.stack([dep._component(iron.FieldVariableTypes.U, c) for c in (1, 2, 3)], axis=1)
+    r = np.stack([dep._component(iron.FieldVariableTypes.DELUDELN, c) for c in (1, 2, 3)], axis=1)
+    gz = np.array([g["nodes"][str(n)]["dependent"] for n in nodes])
+    gr = np.array([g["nodes"][str(n)]["deludeln"] for n in nodes])
+    assert np.abs(z - gz).max() < 1e-9
+    assert np.abs(r - gr).max() < 1e-7 * max(1.0, np.abs(gr).max())
+    p = dep._component(iron.FieldVariableTypes.U, 4)
+    gp = np.array([g["elements"][str(e + 1)]["pressure"] for e in range(rm.num_elements)])
+    assert np.abs(p - 2.0 * gp).max() < 1e-9 * np.abs(gp).max()
+
+
+def test_c4_uniaxial_full_size():
+    """12^3 Q2/Q1 (49,072 DOFs), rollers + x-stretch to 1.2 in 10 increments, Newton to a
+    relative residual of 1e-8: every node on the closed-form homogeneous stretch."""
+    from iron_amd import basis, finite_elasticity as fe, mesh
+    print("case C4 12^3 Q2/Q1 uniaxial", flush=True)
+    rm = mesh.RegularMesh((12, 12, 12), [basis.basis(3, 2, 3), basis.basis(3, 1, 3)], (1.0, 1.0, 1.0))
+    sysg = fe.MRSystem(rm, 1, fe.DEVIATORIC)
+    nu = sysg.nu
+    X = rm.coords
+    fixed = np.zeros(sysg.ndofs, bool)
+    target = np.concatenate([X[:, 0], X[:, 1], X[:, 2], np.zeros(sysg.npd)])
+    for c in range(3):
+        fixed[c * nu:(c + 1) * nu] |= X[:, c] == 0.0
+    fixed[:nu] |= X[:, 0] == 1.0
+    target[:nu][X[:, 0] == 1.0] = 1.2
+    mat = np.tile([2.0, 6.0], (rm.num_elements, 1))
+    newton = fe.MRNewton(sysg, fixed, mat)
+    x = torch.as_tensor(np.concatenate([X[:, 0], X[:, 1], X[:, 2], np.zeros(sysg.npd)]), device=sysg.device)
+    opts = fe.NewtonOptions(atol=1e-12, rtol=1e-8, stol=0.0, linear_rtol=1e-6)
+    infos = fe.load_increments(newton, x, 10, fixed, target, opts=opts)
+    xg = x.cpu().numpy()
+    z = xg[:3 * nu].reshape(3, -1).T
+    exact = X * np.array([1.2, 1.2 ** -0.5, 1.2 ** -0.5])
+    assert np.abs(z - exact).max() < 1e-8
+    B = np.diag([1.44, 1 / 1.2, 1 / 1.2])
+    s = 2 * (2.0 + 6.0 * np.trace(B)) * B - 2 * 6.0 * B @ B
+    assert np.abs(xg[3 * nu:] - (s[0, 0] - s[1, 1]) / 3).max() < 1e-7
+    for i in infos:
+        assert i.fnorm <= 1e-8 * i.fnorm0 or i.fnorm <= 1e-12
