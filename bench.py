@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--format", default="sell", choices=["sell", "csr"], help="solver-matrix storage")
     ap.add_argument("--inner-rtol", type=float, default=0.1,
                     help="FP32 residual reduction between FP64 residual replacements")
+    ap.add_argument("--no-mr", action="store_true", help="skip the Mooney-Rivlin (12^3 Q2/Q1) leg")
     return ap.parse_args()
 
 
@@ -214,6 +215,98 @@ def cpu_baseline(P, fp64_iters, budget_s):
     }
 
 
+def mr_flops_per_element(nn, npd, ng):
+    """Algorithmic FP64 flops of one Mooney-Rivlin element (residual + exact Jacobian),
+    counted from the kernel's loops: per Gauss point 72 nn^2 (displacement block:
+    (3nn)^2 entries x 3 FMAs + weight), 54 nn npd (pressure coupling), 216 nn (metrics,
+    reference gradients, T = GX . A) and about 4,200 for kinematics, stress and the
+    81-entry tangent."""
+    return ng * (72 * nn * nn + 54 * nn * npd + 216 * nn + 4200)
+
+
+def mr_leg(runs, cpu_seconds):
+    """12^3 triquadratic/trilinear Mooney-Rivlin cube (c1=2, c2=6), rollers on x=0, y=0,
+    z=0, x=1 face stretched to 1.2 in 10 load increments, Newton to a relative residual
+    of 1e-8 per increment (preconditioned MINRES steps, Eisenstat-Walker tolerances)."""
+    import torch
+    from iron_amd import _lib, basis, finite_elasticity as fe, mesh
+    rm = mesh.RegularMesh((12, 12, 12), [basis.basis(3, 2, 3), basis.basis(3, 1, 3)], (1.0, 1.0, 1.0))
+    sysg = fe.MRSystem(rm, 1, fe.DEVIATORIC)
+    nu, X = sysg.nu, rm.coords
+    fixed = np.zeros(sysg.ndofs, bool)
+    x0 = np.concatenate([X[:, 0], X[:, 1], X[:, 2], np.zeros(sysg.npd)])
+    target = x0.copy()
+    for c in range(3):
+        fixed[c * nu:(c + 1) * nu] |= X[:, c] == 0.0
+    fixed[:nu] |= X[:, 0] == 1.0
+    target[:nu][X[:, 0] == 1.0] = 1.2
+    mat = np.tile([2.0, 6.0], (rm.num_elements, 1))
+    newton = fe.MRNewton(sysg, fixed, mat)
+    opts = fe.NewtonOptions(atol=1e-12, rtol=1e-8, stol=0.0, linear_rtol="ew")
+
+    def run():
+        x = torch.as_tensor(x0, device=sysg.device)
+        infos = fe.load_increments(newton, x, 10, fixed, target, opts=opts)
+        torch.cuda.synchronize()
+        return x, infos
+
+    run()
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        x, infos = run()
+        times.append(time.perf_counter() - t)
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    xe = X * np.array([1.2, 1.2 ** -0.5, 1.2 ** -0.5])
+    err = float(np.abs(x.cpu().numpy()[:3 * nu].reshape(3, -1).T - xe).max())
+    n_mr, ms_mr = prof.get("finite_elasticity", (0, 0.0))
+    avg_mr = ms_mr / max(n_mr, 1)
+    E = rm.num_elements
+    fl = mr_flops_per_element(27, 8, 27) * E
+    out = {
+        "workload": "mr12: 12^3 Q2/Q1 Mooney-Rivlin (deviatoric, c1=2, c2=6), rollers + x-stretch to 1.2 in 10 "
+                    "increments, Newton to 1e-8 per increment, MINRES steps with Eisenstat-Walker tolerances",
+        "elements": E, "dofs": sysg.ndofs, "free_dofs": newton.red.n_free, "jacobian_nnz": sysg.csr.nnz,
+        "solve_s": float(np.median(times)), "solves_per_s": 1.0 / float(np.median(times)),
+        "newton_iterations": sum(i.iterations for i in infos),
+        "minres_iterations": sum(i.linear_iterations for i in infos),
+        "max_node_error_vs_closed_form": err,
+        "assembly_ms": avg_mr, "assemblies": n_mr,
+        "assembly_elems_per_s": E / (avg_mr * 1e-3) if avg_mr > 0 else None,
+        "roofline": {"kernel": "mr_element_kernel<27,8,27> (residual + exact Jacobian, FP64)", "bound": "fp64",
+                     "achieved": fl / (avg_mr * 1e-3) / 1e12 if avg_mr > 0 else None, "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": (fl / (avg_mr * 1e-3) / 1e12) / 78.6 if avg_mr > 0 else None,
+                     "flops_per_element": mr_flops_per_element(27, 8, 27)},
+        "kernel_ms": {k: v for k, v in prof.items() if v[0]},
+    }
+    if cpu_seconds > 0:
+        sys.path.insert(0, str(ROOT))
+        from oracle import mr_oracle as mo
+        from oracle.fe_oracle import Basis, regular_mesh
+        om = regular_mesh((12, 12, 12), [Basis(3, 2, 3), Basis(3, 1, 3)], (1.0, 1.0, 1.0))
+        prob = mo.MRProblem(om, False, np.full(E, 2.0), np.full(E, 6.0))
+        xs = prob.initial_state(0.0)
+        geo_e, dep_e, pe = prob.element_arrays(xs)
+        ns = 16
+        t = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t < 0.5 * cpu_seconds and k + ns <= E:
+            mo.mr_element_residual(geo_e[k:k + ns], dep_e[k:k + ns], pe[k:k + ns], prob.bu, prob.bp,
+                                   prob.c1[k:k + ns], prob.c2[k:k + ns])
+            mo.mr_element_jacobian(geo_e[k:k + ns], dep_e[k:k + ns], pe[k:k + ns], prob.bu, prob.bp,
+                                   prob.c1[k:k + ns], prob.c2[k:k + ns])
+            k += ns
+        per = (time.perf_counter() - t) / max(k, 1)
+        out["cpu_baseline"] = {"value": 1.0 / per, "unit": "elems/s (residual + Jacobian)", "cores": 1,
+                               "kind": "port",
+                               "sample": f"NumPy FP64 oracle element residual + exact Jacobian on {k} of {E} "
+                                         f"elements (vectorised over {ns} elements per call)"}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -286,6 +379,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)
+    if not args.no_mr:
+        del P
+        torch.cuda.empty_cache()
+        out["mooney_rivlin"] = mr_leg(2, 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 10.0))
     print(json.dumps(out))
 
 

@@ -291,7 +291,8 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
                 void* work, int* iters, int* outer, double* relres, void* stream);
 
 /* Right-preconditioned restarted GMRES(restart) in FP64 (KSPGMRES default
- * restart 30, src/solver_routines.F90:11320-11369), restart <= 31.  The
+ * restart 30, src/solver_routines.F90:11320-11369), restart <= 256 (workspace
+ * from mife_solver_work_bytes(n, restart)).  The
  * preconditioner is diagonal: Jacobi 1/a_ii, except rows where pc_inv[i] != 0
  * (device, caller's row order, may be NULL) which use pc_inv[i] instead, e.g. a
  * scaled inverse pressure mass for the zero-diagonal rows of a mixed

@@ -26,6 +26,7 @@ extern "C" int mife_profile_enable(int on);
 namespace mife {
 
 constexpr int kRedBlocks = kMaxGrid;
+constexpr int kMaxRestart = 256;
 constexpr int kFinThreads = 1024;  // one-block finishers: two partials per thread
 
 struct CgState {
@@ -295,24 +296,27 @@ __global__ void fill_kernel(int n, T* __restrict__ v, T val) {
 
 // ------------------------------------------------------------------ GMRES
 // Partials of V_j . w for j < m over a fixed grid; reduced by multidot_reduce.
+// blockIdx.y selects a chunk of 32 basis vectors (m <= 32 * gridDim.y).
 __global__ __launch_bounds__(256) void multidot_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
                                                        const double* __restrict__ w, double* __restrict__ parts) {
   __shared__ double red[32][4];
+  const int j0 = blockIdx.y * 32;
+  const int mc = min(32, m - j0);
   double acc[32];
   for (int j = 0; j < 32; ++j) acc[j] = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double wi = w[i];
-    for (int j = 0; j < m; ++j) acc[j] += V[j * ldv + i] * wi;
+    for (int j = 0; j < mc; ++j) acc[j] += V[(j0 + j) * ldv + i] * wi;
   }
-  for (int j = 0; j < m; ++j) {
+  for (int j = 0; j < mc; ++j) {
     const double s = wave_sum(acc[j]);
     if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
   }
   __syncthreads();
-  if (threadIdx.x < m) {
+  if (threadIdx.x < mc) {
     double s = 0.0;
     for (int wv = 0; wv < 4; ++wv) s += red[threadIdx.x][wv];
-    parts[threadIdx.x * kRedBlocks + blockIdx.x] = s;
+    parts[(size_t)(j0 + threadIdx.x) * kRedBlocks + blockIdx.x] = s;
   }
 }
 
@@ -320,7 +324,7 @@ __global__ void multidot_reduce(int m, int nblocks, const double* __restrict__ p
   __shared__ double red[4];
   for (int j = 0; j < m; ++j) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[j * kRedBlocks + b];
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[(size_t)j * kRedBlocks + b];
     s = block_sum(s, red);
     if (threadIdx.x == 0) h[j] = s;
     __syncthreads();
@@ -358,29 +362,85 @@ __global__ void gmres_update_kernel(int n, int m, const double* __restrict__ V, 
 
 // ------------------------------------------------------------------ MINRES
 // Preconditioned MINRES (Paige & Saunders 1975) for symmetric indefinite
-// systems with an SPD diagonal preconditioner, three launches per iteration
-// and every scalar on the device.  The solution update of iteration k is
-// applied at the start of iteration k+1's SpMV kernel (it only needs row-wise
-// data), so x is complete once one more minres_spmv/fin pair has run.
-struct MinresState {
-  double beta, oldb, beta1, alfa, dbar, epsln, phibar, cs, sn, phi, denom, oldeps, delta, tol, rr, bb;
-  int iters, maxit, done, pend;
-  double part_a[4 * kRedBlocks];
-  double part_b[4 * kRedBlocks];
+// systems with an SPD diagonal preconditioner: two launches per iteration,
+// every scalar on the device, no finisher kernels.  Each block of a launch
+// reduces the previous launch's per-block partials itself, in the same fixed
+// order (the grids are small), so all blocks hold identical scalars; block 0
+// stores them.  The scalars are double-buffered by iteration parity: launch k
+// reads s[k&1] and block 0 writes s[(k+1)&1], so no block reads what another
+// block of the same launch writes.
+//   A(k): reduce r2.z -> beta_k; (k>0) Givens rotation of step k-1 and the
+//         w/x update of step k-1; v = z/beta_k; t = A v - (beta_k/beta_{k-1}) r1;
+//         partial v.t
+//   B(k): reduce v.t -> alfa_k; t -= (alfa/beta) r2; r1 = r2; r2 = t;
+//         z = D^-1 r2; partial r2.z
+struct MinresScalars {
+  double beta, oldb, beta1, alfa, dbar, epsln, phibar, cs, sn, tol, bb;
+  int iters, maxit, done, pad;
 };
 
-// Applies a pending w/x update, then v = z/beta, t = A z / beta - (beta/oldb) r1, partial v.t
+struct MinresState {
+  MinresScalars s[2];
+  double part_a[kRedBlocks];
+  double part_b[2 * kRedBlocks];
+};
+
+// Fixed-order sum of nb partials, computed by every block (all threads get it).
+__device__ __forceinline__ double block_fixed_sum(const double* __restrict__ parts, int nb) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += parts[i];
+  return block_sum(acc, red);
+}
+
 template <int FMT, int L>
-__global__ __launch_bounds__(256) void minres_spmv_kernel(MatArgs A, const double* __restrict__ vals,
-                                                          const double* __restrict__ z, double* __restrict__ v,
-                                                          double* __restrict__ t, const double* __restrict__ r1,
-                                                          double* __restrict__ w, double* __restrict__ w1,
-                                                          double* __restrict__ w2, double* __restrict__ x,
-                                                          MinresState* st) {
-  const bool pend = st->pend != 0, done = st->done != 0;
-  if (!pend && done) return;
-  const double ib = 1.0 / st->beta, rb = st->oldb != 0.0 ? st->beta / st->oldb : 0.0;
-  const double oldeps = st->oldeps, delta = st->delta, denom = st->denom, phi = st->phi;
+__global__ __launch_bounds__(256) void minres_a_kernel(MatArgs A, const double* __restrict__ vals,
+                                                       const double* __restrict__ z, double* __restrict__ v,
+                                                       double* __restrict__ t, const double* __restrict__ r1,
+                                                       double* __restrict__ w, double* __restrict__ w1,
+                                                       double* __restrict__ w2, double* __restrict__ x,
+                                                       MinresState* st, int k, int nb) {
+  MinresScalars S = st->s[k & 1];
+  if (S.done) {  // carry the final state forward so every later launch sees it
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->s[(k + 1) & 1] = S;
+    return;
+  }
+  const double rz = block_fixed_sum(st->part_b, nb);
+  const double beta = sqrt(fmax(rz, 0.0));
+  bool pend = false;
+  double phi = 0.0, denom = 0.0, delta = 0.0, oldeps = 0.0;
+  if (k == 0) {
+    const double bb = block_fixed_sum(st->part_b + kRedBlocks, nb);
+    S.beta1 = beta;
+    S.oldb = 0.0;
+    S.phibar = beta;
+    S.cs = -1.0;
+    S.sn = 0.0;
+    S.dbar = 0.0;
+    S.epsln = 0.0;
+    S.bb = bb;
+    S.iters = 0;
+    S.done = (beta == 0.0 || S.maxit <= 0) ? 1 : 0;
+  } else {
+    oldeps = S.epsln;
+    delta = S.cs * S.dbar + S.sn * S.alfa;
+    const double gbar = S.sn * S.dbar - S.cs * S.alfa;
+    S.epsln = S.sn * beta;
+    S.dbar = -S.cs * beta;
+    const double gamma = fmax(sqrt(gbar * gbar + beta * beta), 1e-300);
+    S.cs = gbar / gamma;
+    S.sn = beta / gamma;
+    phi = S.cs * S.phibar;
+    S.phibar = S.sn * S.phibar;
+    denom = 1.0 / gamma;
+    S.oldb = S.beta;
+    S.iters += 1;
+    S.done = (S.phibar <= S.tol * S.beta1 || S.iters >= S.maxit || beta == 0.0) ? 1 : 0;
+    pend = true;
+  }
+  S.beta = beta;
+  const bool done = S.done != 0;
+  const double ib = beta > 0.0 ? 1.0 / beta : 0.0, rb = S.oldb != 0.0 ? beta / S.oldb : 0.0;
   double m[1] = {0.0};
   for_each_row<double, FMT, L>(A, vals, done ? nullptr : z, [&](int row, double az) {
     if (pend) {
@@ -399,26 +459,24 @@ __global__ __launch_bounds__(256) void minres_spmv_kernel(MatArgs A, const doubl
       m[0] += vi * ti;
     }
   });
-  if (!done) block_partials<1>(m, st->part_a);
-}
-
-__global__ void minres_fin1_kernel(MinresState* st, int nb) {
-  const bool done = st->done != 0;
-  double out[1] = {0.0};
-  if (!done) finish_sums<1>(st->part_a, nb, out);
-  if (threadIdx.x == 0) {
-    st->pend = 0;
-    if (!done) st->alfa = out[0];
+  if (!done) {
+    __shared__ double red[4];
+    double sm = block_sum(m[0], red);
+    if (threadIdx.x == 0) st->part_a[blockIdx.x] = sm;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->s[(k + 1) & 1] = S;
 }
 
-// t -= (alfa/beta) r2 ; r1 = r2 ; r2 = t ; z = D^-1 r2 ; partial r2.z
-__global__ __launch_bounds__(256) void minres_update_kernel(int n, double* __restrict__ t, double* __restrict__ r1,
-                                                            double* __restrict__ r2, double* __restrict__ z,
-                                                            const double* __restrict__ dinv, MinresState* st) {
-  if (st->done) return;
-  const double ab = st->alfa / st->beta;
-  double m[1] = {0.0};
+__global__ __launch_bounds__(256) void minres_b_kernel(int n, double* __restrict__ t, double* __restrict__ r1,
+                                                       double* __restrict__ r2, double* __restrict__ z,
+                                                       const double* __restrict__ dinv, MinresState* st, int k,
+                                                       int nb) {
+  const int sdone = st->s[(k + 1) & 1].done;
+  const double sbeta = st->s[(k + 1) & 1].beta;
+  if (sdone) return;
+  const double alfa = block_fixed_sum(st->part_a, nb);
+  const double ab = alfa / sbeta;
+  double m = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double r2o = r2[i];
     const double ti = t[i] - ab * r2o;
@@ -426,47 +484,23 @@ __global__ __launch_bounds__(256) void minres_update_kernel(int n, double* __res
     r1[i] = r2o;
     r2[i] = ti;
     z[i] = zi;
-    m[0] += ti * zi;
+    m += ti * zi;
   }
-  block_partials<1>(m, st->part_b);
+  __shared__ double red[4];
+  m = block_sum(m, red);
+  if (threadIdx.x == 0) st->part_b[blockIdx.x] = m;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->s[(k + 1) & 1].alfa = alfa;
 }
 
-// beta_{k+1}, the Givens rotation and the pending update coefficients.
-__global__ void minres_fin2_kernel(MinresState* st, int nb) {
-  if (st->done) return;
-  double out[1];
-  finish_sums<1>(st->part_b, nb, out);
-  if (threadIdx.x == 0) {
-    const double beta = sqrt(fmax(out[0], 0.0));
-    st->oldb = st->beta;
-    st->beta = beta;
-    st->oldeps = st->epsln;
-    const double delta = st->cs * st->dbar + st->sn * st->alfa;
-    const double gbar = st->sn * st->dbar - st->cs * st->alfa;
-    st->epsln = st->sn * beta;
-    st->dbar = -st->cs * beta;
-    const double gamma = fmax(sqrt(gbar * gbar + beta * beta), 1e-300);
-    st->cs = gbar / gamma;
-    st->sn = beta / gamma;
-    st->phi = st->cs * st->phibar;
-    st->phibar = st->sn * st->phibar;
-    st->denom = 1.0 / gamma;
-    st->delta = delta;
-    st->pend = 1;
-    st->iters += 1;
-    st->done = (st->phibar <= st->tol * st->beta1 || st->iters >= st->maxit || beta == 0.0) ? 1 : 0;
-  }
-}
-
-// r1 = r2 = b - A x, z = D^-1 r1, w = w2 = 0 ; partials r1.z, b.b
+// r1 = r2 = b - A x, z = D^-1 r1, w = w2 = 0 ; partials r1.z and b.b ; scalars reset
 template <int FMT, int L>
 __global__ __launch_bounds__(256) void minres_init_kernel(MatArgs A, const double* __restrict__ vals,
                                                           const double* __restrict__ b, const double* __restrict__ x,
                                                           const double* __restrict__ dinv, double* __restrict__ r1,
                                                           double* __restrict__ r2, double* __restrict__ z,
                                                           double* __restrict__ w, double* __restrict__ w2,
-                                                          MinresState* st) {
-  double m[2] = {0.0, 0.0};
+                                                          MinresState* st, double tol, int maxit) {
+  double m0 = 0.0, m1 = 0.0;
   for_each_row<double, FMT, L>(A, vals, x, [&](int row, double ax) {
     const double ri = b[row] - ax;
     const double zi = dinv[row] * ri;
@@ -475,36 +509,22 @@ __global__ __launch_bounds__(256) void minres_init_kernel(MatArgs A, const doubl
     z[row] = zi;
     w[row] = 0.0;
     w2[row] = 0.0;
-    m[0] += ri * zi;
-    m[1] += b[row] * b[row];
+    m0 += ri * zi;
+    m1 += b[row] * b[row];
   });
-  block_partials<2>(m, st->part_b);
-}
-
-__global__ void minres_fin_init_kernel(MinresState* st, int nb, double tol, int maxit) {
-  double out[2];
-  finish_sums<2>(st->part_b, nb, out);
+  __shared__ double red[4];
+  m0 = block_sum(m0, red);
+  m1 = block_sum(m1, red);
   if (threadIdx.x == 0) {
-    const double b1 = sqrt(fmax(out[0], 0.0));
-    st->beta1 = b1;
-    st->beta = b1;
-    st->oldb = 0.0;
-    st->alfa = 0.0;
-    st->dbar = 0.0;
-    st->epsln = 0.0;
-    st->phibar = b1;
-    st->cs = -1.0;
-    st->sn = 0.0;
-    st->phi = 0.0;
-    st->denom = 0.0;
-    st->oldeps = 0.0;
-    st->delta = 0.0;
-    st->tol = tol;
-    st->bb = out[1];
-    st->iters = 0;
-    st->maxit = maxit;
-    st->pend = 0;
-    st->done = (b1 == 0.0 || maxit <= 0) ? 1 : 0;
+    st->part_b[blockIdx.x] = m0;
+    st->part_b[kRedBlocks + blockIdx.x] = m1;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    MinresScalars S{};
+    S.tol = tol;
+    S.maxit = maxit;
+    st->s[0] = S;
+    st->s[1] = S;
   }
 }
 
@@ -707,7 +727,8 @@ size_t mife_solver_work_bytes(int n, int restart) {
   bytes += 3 * (sizeof(CgState) + 256);
   bytes += (8 * nn + 256) * (10 + m);            // FP64 vectors incl. GMRES basis
   bytes += (4 * nn + 256) * 8;                   // FP32 vectors
-  bytes += (8 * (size_t)kRedBlocks * 32 + 256);  // GMRES multidot partials
+  bytes += (8 * (size_t)kRedBlocks * (size_t)std::max(m, (size_t)2) + 256);  // GMRES multidot partials
+  bytes += 8 * (3 * m + 8) + 768;                // GMRES coefficient vectors
   bytes += 8 * 64 * 4 + 65536;
   return bytes;
 }
@@ -852,31 +873,29 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
   fill_kernel<double><<<vg, 256, 0, s>>>(n, v, 0.0);
   fill_kernel<double><<<vg, 256, 0, s>>>(n, w1, 0.0);
   MIFE_DISPATCH_FMT(M.a, M.L, (minres_init_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, dinv, r1, r2, z,
-                                                                                   wv, w2, st)));
-  minres_fin_init_kernel<<<1, kFinThreads, 0, s>>>(st, grid, rtol, maxit);
+                                                                                   wv, w2, st, rtol, maxit)));
   MIFE_CHECK_HIP(hipGetLastError());
-  int done = 0, it = 0;
-  const int chunk = 16;
+  int done = 0, it = 0, k = 0;
+  const int chunk = 32;
   while (!done) {
-    for (int k = 0; k < chunk; ++k) {
+    for (int c = 0; c < chunk; ++c, ++k) {
+      const int nb = (k == 0) ? grid : vg;
       const int tk = prof::begin(s);
-      MIFE_DISPATCH_FMT(M.a, M.L, (minres_spmv_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, v, t, r1, wv, w1,
-                                                                                       w2, x, st)));
+      MIFE_DISPATCH_FMT(M.a, M.L, (minres_a_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, v, t, r1, wv, w1, w2,
+                                                                                    x, st, k, nb)));
       prof::end(tk, s, prof::kMinresSpmv);
-      minres_fin1_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
-      minres_update_kernel<<<vg, 256, 0, s>>>(n, t, r1, r2, z, dinv, st);
-      minres_fin2_kernel<<<1, kFinThreads, 0, s>>>(st, vg);
+      minres_b_kernel<<<vg, 256, 0, s>>>(n, t, r1, r2, z, dinv, st, k, grid);
     }
-    // one more spmv/fin pair applies the last pending update (no-op otherwise)
-    MIFE_DISPATCH_FMT(M.a, M.L, (minres_spmv_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, v, t, r1, wv, w1, w2,
-                                                                                     x, st)));
-    minres_fin1_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
     MIFE_CHECK_HIP(hipGetLastError());
-    int hd[4];
-    MIFE_CHECK_HIP(hipMemcpyAsync(hd, &st->iters, sizeof(hd), hipMemcpyDeviceToHost, s));
+    MinresScalars hsc[2];
+    MIFE_CHECK_HIP(hipMemcpyAsync(hsc, st->s, sizeof(hsc), hipMemcpyDeviceToHost, s));
     MIFE_CHECK_HIP(hipStreamSynchronize(s));
-    it = hd[0];
-    done = hd[2];
+    done = hsc[0].done | hsc[1].done;
+    it = std::max(hsc[0].iters, hsc[1].iters);
+    if (done) {
+      const MinresScalars& d = hsc[0].done ? hsc[0] : hsc[1];
+      if (d.iters >= d.maxit && d.maxit > 0 && d.phibar > d.tol * d.beta1) done = 2;
+    }
   }
   // true residual at exit
   MIFE_CHECK_HIP(hipMemsetAsync(rs, 0, sizeof(CgState), s));
@@ -889,13 +908,9 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
   const double rel = hs.bb > 0 ? sqrt(hs.rr64 / hs.bb) : 0.0;
   if (iters) *iters = it;
   if (relres) *relres = rel;
-  if (it >= maxit && maxit > 0) {
-    MinresState h{};
-    MIFE_CHECK_HIP(hipMemcpy(&h, st, offsetof(MinresState, part_a), hipMemcpyDeviceToHost));
-    if (h.phibar > h.tol * h.beta1) {
-      set_error("MINRES reached the maximum of %d iterations (relative residual %.3e)", maxit, rel);
-      return MIFE_ERR_NOT_CONVERGED;
-    }
+  if (done == 2) {
+    set_error("MINRES reached the maximum of %d iterations (relative residual %.3e)", maxit, rel);
+    return MIFE_ERR_NOT_CONVERGED;
   }
   return MIFE_OK;
 }
@@ -903,7 +918,7 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
 int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, const double* pc_inv,
                double rtol, int restart, int maxit, void* work, int* iters, double* relres, void* stream) {
   MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
-  MIFE_REQUIRE(restart >= 1 && restart <= 31 && rtol > 0.0, MIFE_ERR_ARG, "invalid GMRES arguments");
+  MIFE_REQUIRE(restart >= 1 && restart <= kMaxRestart && rtol > 0.0, MIFE_ERR_ARG, "invalid GMRES arguments");
   Mat M;
   int rc = make_mat(A, &M);
   if (rc) return rc;
@@ -920,9 +935,10 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   double* x = w.take<double>(n);
   const size_t ldv = ((size_t)n + 63) & ~size_t(63);
   double* V = w.take<double>(ldv * (m + 1));
-  double* parts = w.take<double>((size_t)kRedBlocks * 32);
-  double* hdev = w.take<double>(64);
-  double* ydev = w.take<double>(64);
+  double* parts = w.take<double>((size_t)kRedBlocks * (m + 1));
+  double* hdev = w.take<double>(m + 2);
+  double* ydev = w.take<double>(m + 2);
+  double* ndev = w.take<double>(2);
   const int vg = vec_grid(n);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
@@ -936,7 +952,8 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   const double bn = sqrt(hs.bb);
   double beta = sqrt(hs.rr64);
   int it = 0;
-  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 2);
+  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 2), hh(m + 1), h2(m + 1);
+  double hnorm = 0.0;
   while (bn > 0 && beta / bn > rtol && it < maxit) {
     scale_copy_kernel<<<vg, 256, 0, s>>>(n, r, V, 1.0 / beta);
     std::fill(H.begin(), H.end(), 0.0);
@@ -947,20 +964,21 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
       mul_kernel<<<vg, 256, 0, s>>>(n, dinv, V + k * ldv, t);
       launch_spmv<double>(M.a, vals, t, wv, M.L, s);
       // classical Gram-Schmidt, two passes
+      const dim3 mg(vg, div_up(k + 1, 32));
       for (int pass = 0; pass < 2; ++pass) {
-        multidot_kernel<<<vg, 256, 0, s>>>(n, k + 1, V, ldv, wv, parts);
+        multidot_kernel<<<mg, 256, 0, s>>>(n, k + 1, V, ldv, wv, parts);
         multidot_reduce<<<1, 256, 0, s>>>(k + 1, vg, parts, pass == 0 ? hdev : ydev);
         orth_update_kernel<<<vg, 256, 0, s>>>(n, k + 1, V, ldv, pass == 0 ? hdev : ydev, wv);
       }
-      multidot_kernel<<<vg, 256, 0, s>>>(n, 1, wv, ldv, wv, parts);
-      multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, hdev + 32);
+      multidot_kernel<<<dim3(vg, 1), 256, 0, s>>>(n, 1, wv, ldv, wv, parts);
+      multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, ndev);
       MIFE_CHECK_HIP(hipGetLastError());
-      double hh[33], h2[32];
-      MIFE_CHECK_HIP(hipMemcpyAsync(hh, hdev, sizeof(double) * 33, hipMemcpyDeviceToHost, s));
-      MIFE_CHECK_HIP(hipMemcpyAsync(h2, ydev, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+      MIFE_CHECK_HIP(hipMemcpyAsync(hh.data(), hdev, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+      MIFE_CHECK_HIP(hipMemcpyAsync(h2.data(), ydev, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+      MIFE_CHECK_HIP(hipMemcpyAsync(&hnorm, ndev, sizeof(double), hipMemcpyDeviceToHost, s));
       MIFE_CHECK_HIP(hipStreamSynchronize(s));
       for (int j = 0; j <= k; ++j) hcol[j] = hh[j] + h2[j];
-      const double hk1 = sqrt(std::max(hh[32], 0.0));
+      const double hk1 = sqrt(std::max(hnorm, 0.0));
       hcol[k + 1] = hk1;
       if (hk1 > 0) scale_copy_kernel<<<vg, 256, 0, s>>>(n, wv, V + (k + 1) * ldv, 1.0 / hk1);
       for (int j = 0; j < k; ++j) {

@@ -116,10 +116,10 @@ class NewtonOptions:
     rtol: float = 1e-5
     stol: float = 1e-5
     maxit: int = 50
-    linear_rtol: float = 1e-5
+    linear_rtol: float = 1e-5       # or "ew": Eisenstat-Walker forcing terms
     linear_maxit: int = 100000
     linear: str = "auto"          # "minres", "gmres" or "auto" (by formulation)
-    gmres_restart: int = 30
+    gmres_restart: int = 100
 
 
 @dataclass
@@ -136,12 +136,15 @@ class MRNewton:
     """Newton iteration on the free DOFs of an MRSystem with a fixed set of
     constrained DOFs (their values are set in the state before each solve)."""
 
-    def __init__(self, sys: MRSystem, fixed_mask: np.ndarray, material: np.ndarray):
+    def __init__(self, sys: MRSystem, fixed_mask: np.ndarray, material: np.ndarray, restart: int = 100):
         self.sys = sys
         dev = sys.device
         self.red = fem.DirichletReduction(sys.csr, fixed_mask)
-        self.S = fem.SolverMatrix(self.red)
-        self.ws = fem.KrylovWorkspace(max(self.red.n_free, 1), dev, restart=30)
+        # ~200 entries per row and few rows: CSR with a wavefront per row group keeps
+        # every CU busy where 64-row slices would not
+        self.S = fem.SolverMatrix(self.red, fmt="csr")
+        self.restart = int(restart)
+        self.ws = fem.KrylovWorkspace(max(self.red.n_free, 1), dev, restart=self.restart)
         self.free = self.red.free_idx.long()
         mat = np.ascontiguousarray(material, dtype=np.float64).reshape(-1, 2)
         self.material = torch.as_tensor(mat, device=dev)
@@ -164,6 +167,24 @@ class MRNewton:
                           jac=self.J if jac else None, rhs=self.f if body_force is not None else None)
         return (self.R - self.f).index_select(0, self.free)
 
+    @staticmethod
+    def _ew_eta(fn, fn_prev, eta_prev, fn0, o):
+        """Eisenstat-Walker forcing term, choice 2 with PETSc's SNES defaults
+        (eta_0 = 0.3, gamma = 1, alpha = (1+sqrt 5)/2, safeguard threshold 0.1,
+        eta_max = 0.9), floored at half the Newton target over ||F|| so the linear
+        solve never goes below what the stopping test needs."""
+        alpha = 0.5 * (1.0 + 5.0 ** 0.5)
+        if fn_prev is None:
+            eta = 0.3
+        else:
+            eta = (fn / fn_prev) ** alpha
+            safe = eta_prev ** alpha
+            if safe > 0.1:
+                eta = max(eta, safe)
+        eta = min(eta, 0.9)
+        target = max(o.atol, o.rtol * fn0)
+        return max(eta, 0.5 * target / max(fn, 1e-300), 1e-14)
+
     def internal_residual(self, x):
         self.R.zero_()
         self.sys.assemble(x, self.material, residual=self.R)
@@ -184,6 +205,8 @@ class MRNewton:
         fn = float(torch.linalg.vector_norm(F).item())
         info.fnorm0 = fn
         info.history.append(fn)
+        ew = o.linear_rtol == "ew"
+        eta_prev, fn_prev = 0.3, None
         while True:
             if not np.isfinite(fn):
                 raise CMFEError(1, "Nonlinear line search solver did not converge. PETSc diverged F Norm NaN.")
@@ -199,11 +222,16 @@ class MRNewton:
             self.Jff = self.S.gather(self.J, fem.F64, out=self.Jff)
             dx = torch.zeros(self.red.n_free, dtype=torch.float64, device=x.device)
             b = -F
-            if linear == "minres":
-                li = fem.minres(self.S, self.Jff, b, dx, o.linear_rtol, o.linear_maxit, self.ws, pc_inv=self.pc_pos)
+            if ew:
+                eta = self._ew_eta(fn, fn_prev, eta_prev, info.fnorm0, o)
+                eta_prev, fn_prev = eta, fn
             else:
-                li = fem.gmres(self.S, self.Jff, b, dx, o.linear_rtol, o.linear_maxit, self.ws,
-                               restart=o.gmres_restart, pc_inv=self.pc_neg)
+                eta = float(o.linear_rtol)
+            if linear == "minres":
+                li = fem.minres(self.S, self.Jff, b, dx, eta, o.linear_maxit, self.ws, pc_inv=self.pc_pos)
+            else:
+                li = fem.gmres(self.S, self.Jff, b, dx, eta, o.linear_maxit, self.ws,
+                               restart=min(o.gmres_restart, self.restart), pc_inv=self.pc_neg)
             info.linear_iterations += li.iterations
             lam = 1.0
             xt = x.clone()
@@ -340,7 +368,9 @@ def solve(problem, es, solver, bc):
                          maxit=int(solver.newtonMaximumIterations))
     if lin is not None:
         if int(lin.linearType) == iron.LinearSolverTypes.DIRECT:
-            opts.linear_rtol = 1e-12  # no GPU factorisation: iterate to round-off instead
+            # no GPU factorisation: iterate to round-off with a long Krylov basis instead
+            opts.linear_rtol = 1e-12
+            opts.gmres_restart = 100
         else:
             opts.linear_rtol = float(lin.linearIterativeRelativeTolerance)
             opts.linear_maxit = int(lin.linearIterativeMaximumIterations)

@@ -304,3 +304,56 @@ def test_c4_uniaxial_full_size():
     assert np.abs(xg[3 * nu:] - (s[0, 0] - s[1, 1]) / 3).max() < 1e-7
     for i in infos:
         assert i.fnorm <= 1e-8 * i.fnorm0 or i.fnorm <= 1e-12
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_minres_and_gmres_solve_saddle_point_jacobian(fmt):
+    """The Newton system of a deformed Q2/Q1 state (symmetric indefinite, zero pressure
+    diagonal) solved by MINRES and by GMRES with the pressure-block preconditioner,
+    against a dense FP64 solve."""
+    from iron_amd import fem, finite_elasticity as fe
+    print(f"case minres/gmres saddle point fmt={fmt}", flush=True)
+    sysg, prob = _systems((2, 2, 2), 2, False, "deviatoric")
+    rng = np.random.default_rng(11)
+    x = prob.initial_state(0.5)
+    x[:3 * prob.nu] += 0.03 * rng.standard_normal(3 * prob.nu)
+    fixed = np.zeros(prob.ndofs, bool)
+    fixed[:3 * prob.nu] = np.tile(prob.geo[:, 0] == 0.0, 3)
+    mat = np.tile([2.0, 6.0], (prob.mesh.num_elements, 1))
+    newton = fe.MRNewton(sysg, fixed, mat)
+    S = fem.SolverMatrix(newton.red, fmt=fmt)
+    xs = torch.as_tensor(x, device=sysg.device)
+    newton._eval(xs, None, True)
+    vals = S.gather(newton.J)
+    K = prob.jacobian(x).toarray()
+    fr = np.nonzero(~fixed)[0]
+    Kff = K[np.ix_(fr, fr)]
+    assert np.abs(Kff - Kff.T).max() < 1e-12 * np.abs(Kff).max()
+    b = rng.standard_normal(len(fr))
+    ref = np.linalg.solve(Kff, b)
+    bt = torch.as_tensor(b, device=sysg.device)
+    for solver in ("minres", "gmres"):
+        dx = torch.zeros(len(fr), dtype=torch.float64, device=sysg.device)
+        if solver == "minres":
+            info = fem.minres(S, vals, bt, dx, 1e-12, 20000, newton.ws, pc_inv=newton.pc_pos)
+        else:
+            info = fem.gmres(S, vals, bt, dx, 1e-12, 20000, newton.ws, restart=100, pc_inv=newton.pc_neg)
+        sol = dx.cpu().numpy()
+        assert info.relres < 1e-9, (solver, info)
+        assert np.abs(sol - ref).max() < 1e-7 * np.abs(ref).max(), solver
+
+
+def test_newton_eisenstat_walker_matches_fixed_tolerance():
+    from iron_amd import finite_elasticity as fe
+    sysg, prob = _systems((2, 2, 2), 2, False, "deviatoric")
+    fixed, target = _uniaxial(sysg, prob, 1.15)
+    mat = np.tile([2.0, 6.0], (prob.mesh.num_elements, 1))
+    newton = fe.MRNewton(sysg, fixed, mat)
+    res = []
+    for lr in ("ew", 1e-10):
+        x = torch.as_tensor(prob.initial_state(0.0), device=sysg.device)
+        opts = fe.NewtonOptions(atol=1e-12, rtol=1e-10, stol=0.0, linear_rtol=lr)
+        infos = fe.load_increments(newton, x, 3, fixed, target, opts=opts)
+        res.append((x.cpu().numpy(), sum(i.linear_iterations for i in infos)))
+    assert np.abs(res[0][0] - res[1][0]).max() < 1e-8
+    assert res[0][1] < res[1][1]

@@ -1,0 +1,127 @@
+"""Full-size checks of the linear configurations through size-independent
+properties (the oracle is too slow at these sizes):
+
+* 16^3 trilinear Laplace, u = 0 at x = 0 and u = 1 at x = 1: the discrete solution is
+  u = x exactly and the boundary fluxes are -1 and +1.
+* 48^3 triquadratic diffusion, lognormal conductivity, f = 1, u = 0 on all faces:
+  the FP64 residual of the mixed FP32/FP64 solve is below 1e-8, the FP32 and FP64
+  assemblies agree, and the reactions balance the source (rows of K sum to zero,
+  so sum of (K u - f) over all DOFs is -int f dV = -1).
+* 96x32x32 trilinear elasticity on [0,3]x[0,1]^2, lognormal Young's modulus, clamp at
+  x = 0, consistent traction t_z = -1 on x = 3: relative residual <= 1e-8 and the clamp
+  reactions balance the applied force in every direction.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _linear(order, ne, ncomp=1, extent=(1.0, 1.0, 1.0)):
+    from iron_amd import basis, fem, mesh
+    b = basis.basis(3, {1: 1, 2: 2}[order])
+    m = mesh.RegularMesh(ne, [b], extent)
+    dm = mesh.DofMap(m, [0] * ncomp)
+    csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, colours=mesh.structured_colours(m))
+    return fem, b, m, dm, csr
+
+
+def test_c1_laplace16_u_equals_x():
+    print("case C1 16^3 Q1 Laplace", flush=True)
+    fem, b, m, dm, csr = _linear(1, (16, 16, 16))
+    X = m.coords
+    fixed = (X[:, 0] == 0.0) | (X[:, 0] == 1.0)
+    asm = fem.ScalarAssembler(b, X, m.elem_nodes[0], csr)
+    v = csr.zeros()
+    asm.assemble(v)
+    red = fem.DirichletReduction(csr, fixed)
+    S = fem.SolverMatrix(red)
+    u = torch.as_tensor(np.where(X[:, 0] == 1.0, 1.0, 0.0), device="cuda")
+    bvec = red.rhs(v, u)
+    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    info = fem.cg(S, S.gather(v), bvec, x, 1e-10, 100000, fem.KrylovWorkspace(red.n_free, "cuda"))
+    red.field_update(x, u)
+    un = u.cpu().numpy()
+    assert info.relres <= 1e-10
+    assert np.abs(un - X[:, 0]).max() < 1e-8
+    R = fem.backsubstitute(csr, v, u).cpu().numpy()
+    assert abs(R[X[:, 0] == 0.0].sum() + 1.0) < 1e-8
+    assert abs(R[X[:, 0] == 1.0].sum() - 1.0) < 1e-8
+
+
+def test_c2_diffusion48_mixed_solve_balance():
+    print("case C2 48^3 Q2 diffusion", flush=True)
+    fem, b, m, dm, csr = _linear(2, (48, 48, 48))
+    X = m.coords
+    E = m.num_elements
+    g = torch.Generator().manual_seed(1234)
+    kappa = torch.randn(E, dtype=torch.float64, generator=g).exp().cuda()
+    fixed = np.any((X == 0.0) | (X == 1.0), axis=1)
+    asm = fem.ScalarAssembler(b, X, m.elem_nodes[0], csr)
+    v64, v32 = csr.zeros(fem.F64), csr.zeros(fem.F32)
+    src = torch.zeros(csr.n, dtype=torch.float64, device="cuda")
+    ones = torch.ones(E, dtype=torch.float64, device="cuda")
+    asm.assemble(v64, kappa=kappa, source=ones, src_vec=src)
+    asm.assemble(v32, kappa=kappa, prec=fem.F32)
+    scale = float(v64.abs().max())
+    assert float((v32.double() - v64).abs().max()) <= 2e-6 * scale
+    assert abs(float(src.sum()) - 1.0) < 1e-12          # int f dV over the unit cube
+    red = fem.DirichletReduction(csr, fixed)
+    S = fem.SolverMatrix(red)
+    u = torch.zeros(csr.n, dtype=torch.float64, device="cuda")
+    bvec = red.rhs(v64, u, src=src)
+    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    info = fem.refine(S, S.gather(v64), S.gather(v32, fem.F32), bvec, x, 1e-8, 200000,
+                      fem.KrylovWorkspace(red.n_free, "cuda"))
+    assert info.relres <= 1e-8
+    # independent FP64 residual through the SELL SpMV (slice numbering)
+    xs = x[S.perm.long()]
+    y = torch.empty_like(xs)
+    S.spmv(S.gather(v64), xs, y)
+    r = bvec[S.perm.long()] - y
+    assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(bvec)) <= 1e-8
+    red.field_update(x, u)
+    R = fem.backsubstitute(csr, v64, u, src=src).cpu().numpy()
+    assert abs(R[fixed].sum() + 1.0) < 1e-6
+    assert np.abs(R[~fixed]).max() < 1e-6 * np.abs(R[fixed]).max()
+
+
+def test_c3_elastic96_equilibrium():
+    from iron_amd import engine
+    print("case C3 96x32x32 Q1 elasticity", flush=True)
+    fem, b, m, dm, csr = _linear(1, (96, 32, 32), 3, (3.0, 1.0, 1.0))
+    X = m.coords
+    E = m.num_elements
+    N = m.num_nodes(0)
+    g = torch.Generator().manual_seed(7)
+    Ee = 10.0 * torch.exp(0.3 * torch.randn(E, dtype=torch.float64, generator=g)).numpy()
+    mat = np.stack([Ee, Ee, Ee, np.full(E, 0.3), np.full(E, 0.3), np.full(E, 0.3)], axis=1)
+    asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], csr)
+    v = csr.zeros()
+    asm.assemble(v, torch.as_tensor(np.ascontiguousarray(mat), device="cuda"))
+    clamp = X[:, 0] == 0.0
+    fixed = np.concatenate([clamp, clamp, clamp])
+    tip = np.nonzero(X[:, 0] == 3.0)[0]
+    f = engine.neumann_integrate(m, dm, [b] * 3, {int(2 * N + i): -1.0 for i in tip})
+    red = fem.DirichletReduction(csr, fixed)
+    S = fem.SolverMatrix(red)
+    u = torch.zeros(csr.n, dtype=torch.float64, device="cuda")
+    ft = torch.as_tensor(f, device="cuda")
+    bvec = red.rhs(v, u, rhs=ft)
+    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    info = fem.cg(S, S.gather(v), bvec, x, 1e-8, 200000, fem.KrylovWorkspace(red.n_free, "cuda"))
+    assert info.relres <= 1e-8
+    red.field_update(x, u)
+    R = fem.backsubstitute(csr, v, u).cpu().numpy() - f
+    fc = f.reshape(3, N).sum(axis=1)
+    # one unit on the x = 3 face plus iron's integration of the edge-node values over the
+    # four side faces that share those nodes: 4 strips of area 1/32, half weight
+    assert abs(fc[2] + 1.0625) < 1e-12 and fc[0] == 0.0 and fc[1] == 0.0
+    for c in range(3):
+        rc = R[c * N:(c + 1) * N]
+        # clamp reactions balance the applied force, free rows are in equilibrium
+        assert abs(rc[clamp].sum() + fc[c]) < 1e-6 * max(1.0, abs(fc[c]))
+        assert np.abs(rc[~clamp]).max() < 1e-6
+    uz = u.cpu().numpy()[2 * N:]
+    assert uz[tip].mean() < 0.0
