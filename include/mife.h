@@ -313,6 +313,61 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b, doubl
                 const double* pc_inv, double rtol, int maxit, void* work, int* iters,
                 double* relres, void* stream);
 
+/* -------------------------------------------- grouped solves (serving) */
+
+/* One independent system of a group: its CSR (local 0-based columns) lives in
+ * shared pools at rp_off (n+1 row pointers), nz_off (columns and values), and
+ * its vectors at row_off of the vector pools.  slot (0 <= slot < max_segments)
+ * names the segment's persistent CG state. */
+typedef struct mife_segment {
+  int32_t n;
+  int32_t maxit;
+  int32_t slot;
+  int32_t pad;
+  int64_t row_off;
+  int64_t rp_off;
+  int64_t nz_off;
+  double rtol;
+} mife_segment;
+
+/* Grouped Jacobi CG: every launch serves all segments of a table; each segment
+ * has its own scalars, its own fixed-order reductions and its own stopping test
+ * ||r|| <= rtol ||b|| (recurrence residual), so a system's iterates do not
+ * depend on which other systems share the launches.  No reference counterpart
+ * (iron solves one system per KSPSolve, src/solver_routines.F90:11869); it is
+ * the batching layer of the serving path.
+ *   mife_grouped_state_bytes / _work_bytes: device buffers for the per-slot
+ *     state and for one table (max_blocks from the sum of mife_grouped_blocks(n)).
+ *   mife_grouped_setup: builds the table of n_seg segments (host staging buffer
+ *     of _work_bytes), synchronises the stream.
+ *   mife_grouped_cg_init: Jacobi diagonal, r = b - A x, z, p = q = 0 and the
+ *     state of the table's segments.
+ *   mife_grouped_cg_iterate: `iterations` CG iterations of every unconverged
+ *     segment; launches only (no allocation, memset or synchronisation), so the
+ *     call can be captured in a HIP graph and replayed.
+ *   mife_grouped_cg_residual: true ||b - A x|| of the table's segments.
+ *   mife_grouped_cg_status: host copy of iterations, done flags and relative
+ *     residuals of all max_segments slots (synchronises).
+ * Vector pools: b, x [pool_rows] and vec5 [5][pool_rows] (D^-1, r, z, p, q). */
+size_t mife_grouped_state_bytes(int max_segments);
+size_t mife_grouped_work_bytes(int max_segments, int max_blocks);
+int mife_grouped_blocks(int n_rows);
+int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segments, int max_blocks,
+                       int* n_blocks_out, void* table_host, void* work, void* stream);
+int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_blocks,
+                         const int32_t* rp_pool, const int32_t* col_pool, const double* val_pool,
+                         const double* b, double* x, double* vec5, int64_t pool_rows, void* work,
+                         void* state, void* stream);
+int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_blocks,
+                            const int32_t* rp_pool, const int32_t* col_pool, const double* val_pool,
+                            double* x, double* vec5, int64_t pool_rows, int iterations, void* work,
+                            void* state, void* stream);
+int mife_grouped_cg_residual(int n_seg, int n_blocks, int max_segments, int max_blocks,
+                             const int32_t* rp_pool, const int32_t* col_pool, const double* val_pool,
+                             const double* b, const double* x, void* work, void* state, void* stream);
+int mife_grouped_cg_status(int max_segments, const void* state, int* iters, int* done, double* relres,
+                           void* stream);
+
 /* ------------------------------------------------------------- profiling */
 
 /* Kernel timing with HIP events recorded on the launching stream around each

@@ -85,6 +85,14 @@ _SIG = {
     "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_minres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_mr_assemble": (_I, [C.POINTER(MrDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "mife_grouped_state_bytes": (C.c_size_t, [_I]),
+    "mife_grouped_work_bytes": (C.c_size_t, [_I, _I]),
+    "mife_grouped_blocks": (_I, [_I]),
+    "mife_grouped_setup": (_I, [_P, _I, _I, _I, C.POINTER(_I), _P, _P, _P]),
+    "mife_grouped_cg_init": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _I64, _P, _P, _P]),
+    "mife_grouped_cg_iterate": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I64, _I, _P, _P, _P]),
+    "mife_grouped_cg_residual": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "mife_grouped_cg_status": (_I, [_I, _P, _P, _P, _P, _P]),
     "mife_profile_enable": (_I, [_I]),
     "mife_profile_reset": (_I, []),
     "mife_profile_read": (_I, [_I, C.c_char_p, _P, _P, C.POINTER(_I)]),

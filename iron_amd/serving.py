@@ -1,0 +1,397 @@
+"""Serving many small finite-element solves on one GPU.
+
+Requests (a diffusion or an elasticity box of its own size and coefficients)
+arrive over time.  Each admitted request is assembled and reduced on the GPU
+(the same kernels as a single solve), and its reduced system is copied into
+shared HBM pools: CSR row pointers, columns and values, and the CG vectors.
+All admitted requests then iterate together in grouped Jacobi CG launches
+(mife_grouped_cg_*): one launch serves every unconverged system, each with its
+own scalars and stopping test.  Between chunks of iterations the server
+retires converged requests (their solution goes back to the host and their
+pool ranges are freed) and admits waiting ones, first come first served, as
+long as their rows and entries fit the pool budget (HBM-budget admission).  A
+chunk of iterations of an unchanged group is captured once in a HIP graph and
+replayed.
+
+No reference counterpart: iron solves one problem per Problem_Solve.  Each
+request's numbers are those of the single-solve path (tests compare every
+request against the FP64 oracle, and a request solved alone against the same
+request solved in a mix, bit for bit).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from collections import OrderedDict
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, fem
+from ._lib import call, ptr
+from .basis import basis
+from .mesh import DofMap, RegularMesh, structured_colours
+
+
+class Segment(C.Structure):
+    """mife_segment"""
+    _fields_ = [("n", C.c_int32), ("maxit", C.c_int32), ("slot", C.c_int32), ("pad", C.c_int32),
+                ("row_off", C.c_int64), ("rp_off", C.c_int64), ("nz_off", C.c_int64), ("rtol", C.c_double)]
+
+
+@dataclass
+class Request:
+    rid: int
+    kind: str                 # "diffusion" or "elasticity"
+    n_elem: tuple
+    seed: int
+    arrival: float = 0.0      # seconds after the start of the run
+    rtol: float = 1e-8
+    # results
+    u: np.ndarray | None = None
+    iterations: int = 0
+    relres: float = 0.0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.arrival
+
+
+# ------------------------------------------------------------------ problems
+class _Structure:
+    """Device structures of one request shape (cached by (kind, n_elem))."""
+
+    def __init__(self, kind: str, n_elem, device):
+        self.kind = kind
+        b = basis(3, 1, 2)
+        self.basis = b
+        extent = (1.0, 1.0, 1.0) if kind == "diffusion" else (3.0, 1.0, 1.0)
+        m = RegularMesh(n_elem, [b], extent)
+        self.mesh = m
+        ncomp = 1 if kind == "diffusion" else 3
+        dm = DofMap(m, [0] * ncomp)
+        self.dm = dm
+        X = m.coords
+        N = m.num_nodes(0)
+        if kind == "diffusion":
+            fixed = (X[:, 0] == 0.0) | (X[:, 0] == 1.0)
+            ufix = np.where(X[:, 0] == 1.0, 1.0, 0.0)
+            rhs = None
+        else:
+            clamp = X[:, 0] == 0.0
+            fixed = np.concatenate([clamp, clamp, clamp])
+            ufix = np.zeros(3 * N)
+            # consistent nodal forces of t_z = -1 on the x = 3 face: int phi dA on the
+            # bilinear face grid is h_y h_z (1/2 at face edges, 1 inside) per direction
+            ny, nz = n_elem[1], n_elem[2]
+            hy, hz = 1.0 / ny, 1.0 / nz
+            cy = np.where((np.arange(ny + 1) == 0) | (np.arange(ny + 1) == ny), 0.5, 1.0)
+            cz = np.where((np.arange(nz + 1) == 0) | (np.arange(nz + 1) == nz), 0.5, 1.0)
+            rhs = np.zeros(3 * N)
+            tip = np.nonzero(X[:, 0] == 3.0)[0]
+            gp = m.node_grid_pos[tip]
+            rhs[2 * N + tip] = -1.0 * hy * hz * cy[gp[:, 1]] * cz[gp[:, 2]]
+        self.fixed = fixed
+        self.ufix = ufix
+        self.csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, device, colours=structured_colours(m))
+        self.red = fem.DirichletReduction(self.csr, fixed)
+        self.rhs = None if rhs is None else torch.as_tensor(rhs, device=device)
+        self.u0 = torch.as_tensor(np.where(fixed, ufix, 0.0), device=device)
+        if kind == "diffusion":
+            self.asm = fem.ScalarAssembler(b, X, m.elem_nodes[0], self.csr)
+        else:
+            self.asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], self.csr)
+
+    def coefficients(self, seed: int, device):
+        """Lognormal(0,1) element coefficients from the request's seed: conductivity, or
+        Young's modulus 10 exp(N(0,1)) with Poisson ratio 0.3."""
+        E = self.mesh.num_elements
+        g = torch.Generator().manual_seed(int(seed))
+        z = torch.randn(E, dtype=torch.float64, generator=g).exp()
+        if self.kind == "diffusion":
+            return z.to(device)
+        Ey = 10.0 * z.numpy()
+        mat = np.stack([Ey, Ey, Ey, np.full(E, 0.3), np.full(E, 0.3), np.full(E, 0.3)], axis=1)
+        return torch.as_tensor(np.ascontiguousarray(mat), device=device)
+
+    def assemble(self, coef, vals):
+        vals.zero_()
+        if self.kind == "diffusion":
+            self.asm.assemble(vals, kappa=coef)
+        else:
+            self.asm.assemble(vals, coef)
+        return vals
+
+
+# ------------------------------------------------------------------ pools
+class _FirstFit:
+    def __init__(self, cap: int):
+        self.cap = int(cap)
+        self.free = [(0, self.cap)]  # sorted (start, size)
+
+    def alloc(self, size: int):
+        size = max(int(size), 1)
+        for i, (s, n) in enumerate(self.free):
+            if n >= size:
+                if n == size:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (s + size, n - size)
+                return s
+        return None
+
+    def release(self, start: int, size: int):
+        size = max(int(size), 1)
+        self.free.append((start, size))
+        self.free.sort()
+        merged = []
+        for s, n in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + n)
+            else:
+                merged.append((s, n))
+        self.free = merged
+
+
+class Server:
+    """Grouped-CG serving of Request objects within an HBM budget (bytes)."""
+
+    BYTES_PER_ROW = 7 * 8 + 4      # b, x, D^-1, r, z, p, q (FP64) + row pointer
+    BYTES_PER_NZ = 8 + 4           # value + column
+
+    def __init__(self, hbm_budget: float = 32e9, max_segments: int = 256, nz_per_row: float = 60.0,
+                 chunk: int = 16, use_graph: bool = True, device=None, cache_shapes: int = 8):
+        self.device = fem._dev(device)
+        self.max_segments = int(max_segments)
+        rows = int(hbm_budget / (self.BYTES_PER_ROW + nz_per_row * self.BYTES_PER_NZ))
+        self.rows_cap = max(rows, 1024)
+        self.nz_cap = int(self.rows_cap * nz_per_row)
+        self.rp_cap = self.rows_cap + self.max_segments
+        dev = self.device
+        self.b = torch.zeros(self.rows_cap, dtype=torch.float64, device=dev)
+        self.x = torch.zeros(self.rows_cap, dtype=torch.float64, device=dev)
+        self.vec5 = torch.zeros(5 * self.rows_cap, dtype=torch.float64, device=dev)
+        self.rp = torch.zeros(self.rp_cap, dtype=torch.int32, device=dev)
+        self.col = torch.zeros(max(self.nz_cap, 1), dtype=torch.int32, device=dev)
+        self.val = torch.zeros(max(self.nz_cap, 1), dtype=torch.float64, device=dev)
+        self.max_blocks = _lib.lib().mife_grouped_blocks(self.rows_cap) + 2 * self.max_segments
+        L = _lib.lib()
+        self.state = torch.zeros(int(L.mife_grouped_state_bytes(self.max_segments)), dtype=torch.uint8, device=dev)
+        wb = int(L.mife_grouped_work_bytes(self.max_segments, self.max_blocks))
+        self.table = torch.zeros(wb, dtype=torch.uint8, device=dev)
+        self.init_table = torch.zeros(wb, dtype=torch.uint8, device=dev)
+        self.staging = np.zeros(wb, dtype=np.uint8)
+        self.alloc_rows = _FirstFit(self.rows_cap)
+        self.alloc_rp = _FirstFit(self.rp_cap)
+        self.alloc_nz = _FirstFit(self.nz_cap)
+        self.free_slots = list(range(self.max_segments))[::-1]
+        self.chunk = int(chunk)
+        self.use_graph = bool(use_graph)
+        self.cache = OrderedDict()
+        self.cache_shapes = int(cache_shapes)
+        self.graph = None
+        self.stats = dict(admitted=0, retired=0, chunks=0, graph_captures=0, graph_replays=0, table_builds=0)
+
+    # -- structures
+    def _structure(self, req: Request) -> _Structure:
+        key = (req.kind, tuple(req.n_elem))
+        st = self.cache.get(key)
+        if st is None:
+            st = _Structure(req.kind, req.n_elem, self.device)
+            self.cache[key] = st
+            while len(self.cache) > self.cache_shapes:
+                self.cache.popitem(last=False)
+        else:
+            self.cache.move_to_end(key)
+        return st
+
+    def fits(self, req: Request) -> bool:
+        st = self._structure(req)
+        r = st.red
+        return (self._largest(self.alloc_rows) >= r.n_free and self._largest(self.alloc_nz) >= r.nnz
+                and self._largest(self.alloc_rp) >= r.n_free + 1 and bool(self.free_slots))
+
+    @staticmethod
+    def _largest(a: _FirstFit) -> int:
+        return max((n for _, n in a.free), default=0)
+
+    # -- admission: assemble, reduce, copy into the pools
+    def admit(self, req: Request, now: float):
+        st = self._structure(req)
+        red = st.red
+        nf, nnz = red.n_free, red.nnz
+        ro = self.alloc_rows.alloc(nf)
+        rpo = self.alloc_rp.alloc(nf + 1)
+        nzo = self.alloc_nz.alloc(nnz)
+        if ro is None or rpo is None or nzo is None:
+            for a, o, n in ((self.alloc_rows, ro, nf), (self.alloc_rp, rpo, nf + 1), (self.alloc_nz, nzo, nnz)):
+                if o is not None:
+                    a.release(o, n)
+            return None
+        slot = self.free_slots.pop()
+        coef = st.coefficients(req.seed, self.device)
+        vals = st.csr.zeros()
+        st.assemble(coef, vals)
+        s = fem._stream()
+        # reduced values straight into the value pool, structure by copy
+        call("mife_gather_values", nnz, ptr(red.src_pos), ptr(vals), fem.F64, ptr(self.val) + 8 * nzo, fem.F64, s)
+        self.rp[rpo:rpo + nf + 1].copy_(red.row_ptr[: nf + 1])
+        self.col[nzo:nzo + nnz].copy_(red.col[:nnz])
+        call("mife_solver_rhs", nf, ptr(red.free_idx), ptr(st.csr.row_ptr), ptr(st.csr.col), ptr(vals),
+             ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(self.b) + 8 * ro, s)
+        self.x[ro:ro + nf].zero_()
+        seg = Segment(nf, 200000, slot, 0, ro, rpo, nzo, req.rtol)
+        req.t_admit = now
+        self.stats["admitted"] += 1
+        return dict(req=req, seg=seg, st=st, nf=nf, nnz=nnz)
+
+    def _build_table(self, entries, table):
+        segs = (Segment * max(len(entries), 1))(*[e["seg"] for e in entries])
+        nb = C.c_int(0)
+        call("mife_grouped_setup", segs, len(entries), self.max_segments, self.max_blocks, C.byref(nb),
+             self.staging.ctypes.data, ptr(table), fem._stream())
+        self.stats["table_builds"] += 1
+        return nb.value
+
+    def _iterate(self, n_seg, nb, iters):
+        call("mife_grouped_cg_iterate", n_seg, nb, self.max_segments, self.max_blocks, ptr(self.rp), ptr(self.col),
+             ptr(self.val), ptr(self.x), ptr(self.vec5), self.rows_cap, int(iters), ptr(self.table),
+             ptr(self.state), fem._stream())
+
+    def _status(self):
+        it = np.zeros(self.max_segments, dtype=np.int32)
+        dn = np.zeros(self.max_segments, dtype=np.int32)
+        rr = np.zeros(self.max_segments, dtype=np.float64)
+        call("mife_grouped_cg_status", self.max_segments, ptr(self.state), it.ctypes.data, dn.ctypes.data,
+             rr.ctypes.data, fem._stream())
+        return it, dn, rr
+
+    def _retire(self, e, now):
+        req, seg, st = e["req"], e["seg"], e["st"]
+        nf = e["nf"]
+        u = st.u0.clone()
+        st.red.field_update(self.x[seg.row_off:seg.row_off + nf], u)
+        req.u = u.cpu().numpy()
+        req.t_done = time.perf_counter() - now
+        self.alloc_rows.release(seg.row_off, nf)
+        self.alloc_rp.release(seg.rp_off, nf + 1)
+        self.alloc_nz.release(seg.nz_off, e["nnz"])
+        self.free_slots.append(seg.slot)
+        self.stats["retired"] += 1
+
+    # -- the serving loop
+    def run(self, requests, max_batch: int | None = None):
+        """Serve ``requests`` (arrival times relative to the call); returns them with
+        results, iterations, relative residuals and completion times filled in."""
+        queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        active = []
+        t0 = time.perf_counter()
+        qi = 0
+        nb = 0
+        changed = True
+        while qi < len(queue) or active:
+            now = time.perf_counter() - t0
+            # admission (FIFO, stop at the first request that does not fit)
+            new = []
+            while qi < len(queue) and queue[qi].arrival <= now and (max_batch is None or len(active) < max_batch):
+                e = self.admit(queue[qi], now) if self.fits(queue[qi]) else None
+                if e is None:
+                    break
+                new.append(e)
+                active.append(e)
+                qi += 1
+            if new:
+                nbi = self._build_table(new, self.init_table)
+                call("mife_grouped_cg_init", len(new), nbi, self.max_segments, self.max_blocks, ptr(self.rp),
+                     ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
+                     ptr(self.init_table), ptr(self.state), fem._stream())
+                changed = True
+            if not active:
+                if qi < len(queue):
+                    time.sleep(max(0.0, min(queue[qi].arrival - (time.perf_counter() - t0), 0.002)))
+                continue
+            if changed:
+                nb = self._build_table(active, self.table)
+                self.graph = None
+                changed = False
+            self._run_chunk(len(active), nb)
+            self.stats["chunks"] += 1
+            it, dn, rr = self._status()
+            finished = [e for e in active if dn[e["seg"].slot]]
+            if finished:
+                tbl = self._build_table(finished, self.init_table)
+                call("mife_grouped_cg_residual", len(finished), tbl, self.max_segments, self.max_blocks, ptr(self.rp),
+                     ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.init_table), ptr(self.state),
+                     fem._stream())
+                it, dn, rr = self._status()
+                for e in finished:
+                    e["req"].iterations = int(it[e["seg"].slot])
+                    e["req"].relres = float(rr[e["seg"].slot])
+                    self._retire(e, t0)
+                active = [e for e in active if not dn[e["seg"].slot]]
+                changed = True
+        return queue
+
+    def _run_chunk(self, n_seg, nb):
+        if not self.use_graph:
+            self._iterate(n_seg, nb, self.chunk)
+            return
+        if self.graph is None:
+            g = torch.cuda.CUDAGraph()
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self._iterate(n_seg, nb, self.chunk)
+            torch.cuda.current_stream().wait_stream(s)
+            self.graph = g
+            self.stats["graph_captures"] += 1
+            # the capture did not run the iterations: replay once now
+        self.graph.replay()
+        self.stats["graph_replays"] += 1
+
+
+def solve_one_fp64(req: Request, structures: dict, device=None) -> Request:
+    """The one-at-a-time FP64 path for a request: assemble, reduce, Jacobi CG, field
+    update (the single-solve engine)."""
+    dev = fem._dev(device)
+    key = (req.kind, tuple(req.n_elem))
+    st = structures.get(key)
+    if st is None:
+        st = _Structure(req.kind, req.n_elem, dev)
+        structures.clear()
+        structures[key] = st
+    coef = st.coefficients(req.seed, dev)
+    vals = st.csr.zeros()
+    st.assemble(coef, vals)
+    red = st.red
+    S = fem.SolverMatrix(red, fmt="csr")
+    b = red.rhs(vals, st.u0, rhs=st.rhs)
+    x = torch.zeros(red.n_free, dtype=torch.float64, device=dev)
+    ws = fem.KrylovWorkspace(red.n_free, dev)
+    info = fem.cg(S, S.gather(vals), b, x, req.rtol, 200000, ws)
+    u = st.u0.clone()
+    red.field_update(x, u)
+    req.u = u.cpu().numpy()
+    req.iterations = info.iterations
+    req.relres = info.relres
+    return req
+
+
+def make_stream(n: int, seed: int = 2024, lo: int = 8, hi: int = 64, frac_diffusion: float = 0.7,
+                rate: float | None = None):
+    """n requests: 70% diffusion / 30% elasticity, N_x, N_y, N_z ~ U{lo..hi}, per-request
+    coefficient seeds, Poisson arrivals at ``rate`` per second (all at t=0 if None)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    t = 0.0
+    for i in range(n):
+        kind = "diffusion" if rng.random() < frac_diffusion else "elasticity"
+        ne = tuple(int(v) for v in rng.integers(lo, hi + 1, size=3))
+        if rate:
+            t += rng.exponential(1.0 / rate)
+        reqs.append(Request(i, kind, ne, int(rng.integers(1 << 30)), arrival=t if rate else 0.0))
+    return reqs

@@ -1,0 +1,104 @@
+"""Serving path (grouped CG, HBM-budget admission, continuous batching, HIP-graph
+replay): every request against the FP64 oracle and the single-solve path, a request
+alone against the same request in a mix (bit for bit), and admission under a tight
+budget."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle import fe_oracle as fo  # noqa: E402
+
+
+def _oracle_solution(req):
+    """Dense FP64 solve of the request's reduced system with the oracle's assembly."""
+    ob = fo.Basis(3, 1, 2)
+    extent = (1.0, 1.0, 1.0) if req.kind == "diffusion" else (3.0, 1.0, 1.0)
+    om = fo.regular_mesh(req.n_elem, [ob], extent)
+    ncomp = 1 if req.kind == "diffusion" else 3
+    od = fo.DofMap(om, [0] * ncomp)
+    ed = od.element_dofs()
+    xe = om.coords[om.elem_nodes[0] - 1]
+    E = om.num_elements
+    g = torch.Generator().manual_seed(int(req.seed))
+    z = torch.randn(E, dtype=torch.float64, generator=g).exp().numpy()
+    X = om.coords
+    N = len(X)
+    if req.kind == "diffusion":
+        Ke = fo.laplace_element_matrices(xe, ob, z)
+        fixed = (X[:, 0] == 0.0) | (X[:, 0] == 1.0)
+        ufix = np.where(X[:, 0] == 1.0, 1.0, 0.0)
+        rhs = np.zeros(N)
+    else:
+        C = fo.linear_elasticity_tensor(10 * z, 10 * z, 10 * z, np.full(E, 0.3), np.full(E, 0.3), np.full(E, 0.3))
+        Ke = fo.linear_elasticity_element_matrices(xe, ob, C)
+        clamp = X[:, 0] == 0.0
+        fixed = np.concatenate([clamp] * 3)
+        ufix = np.zeros(3 * N)
+        ny, nz = req.n_elem[1], req.n_elem[2]
+        rhs = np.zeros(3 * N)
+        tip = np.nonzero(X[:, 0] == 3.0)[0]
+        iy = np.rint(X[tip, 1] * ny).astype(int)
+        iz = np.rint(X[tip, 2] * nz).astype(int)
+        cy = np.where((iy == 0) | (iy == ny), 0.5, 1.0)
+        cz = np.where((iz == 0) | (iz == nz), 0.5, 1.0)
+        rhs[2 * N + tip] = -1.0 * cy * cz / (ny * nz)
+    rp, col = fo.csr_pattern(od)
+    vals = fo.assemble_csr(rp, col, ed, Ke, od.num_dofs)
+    free, srp, scol, sv, b = fo.reduce_dirichlet(rp, col, vals, fixed, ufix, rhs)
+    A = fo._csr_scipy(srp, scol, sv).toarray()
+    u = np.where(fixed, ufix, 0.0)
+    u[free] = np.linalg.solve(A, b)
+    return u
+
+
+def _mix():
+    from iron_amd import serving
+    return [serving.Request(0, "diffusion", (4, 3, 5), 11), serving.Request(1, "elasticity", (6, 2, 2), 12),
+            serving.Request(2, "diffusion", (7, 7, 6), 13), serving.Request(3, "elasticity", (4, 3, 3), 14),
+            serving.Request(4, "diffusion", (3, 3, 3), 15)]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_grouped_requests_match_oracle_and_single_solve(use_graph):
+    from iron_amd import serving
+    print(f"case serving mix graph={use_graph}", flush=True)
+    srv = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8, use_graph=use_graph)
+    reqs = srv.run(_mix())
+    single = {}
+    for r in reqs:
+        assert r.relres <= 1e-8, (r.rid, r.relres)
+        ref = _oracle_solution(r)
+        assert np.abs(r.u - ref).max() <= 1e-6 * max(1.0, np.abs(ref).max()), r.rid
+        one = serving.solve_one_fp64(serving.Request(r.rid, r.kind, r.n_elem, r.seed), single)
+        assert np.abs(one.u - ref).max() <= 1e-6 * max(1.0, np.abs(ref).max())
+    if use_graph:
+        assert srv.stats["graph_replays"] > 0
+
+
+def test_alone_equals_in_mix_bitwise():
+    from iron_amd import serving
+    print("case serving alone vs mix", flush=True)
+    target = serving.Request(9, "elasticity", (5, 3, 4), 99)
+    alone = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8).run(
+        [serving.Request(9, "elasticity", (5, 3, 4), 99)])[0]
+    mix = _mix() + [target]
+    for k, r in enumerate(mix):
+        r.arrival = 0.0 if k % 2 == 0 else 1e-3  # joins at different chunk boundaries
+    out = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8).run(mix)
+    inmix = [r for r in out if r.rid == 9][0]
+    assert inmix.iterations == alone.iterations
+    assert np.array_equal(inmix.u, alone.u)
+
+
+def test_budget_admission_queues_and_completes():
+    from iron_amd import serving
+    print("case serving tight budget", flush=True)
+    reqs = serving.make_stream(12, seed=5, lo=3, hi=8)
+    srv = serving.Server(hbm_budget=3e6, max_segments=4, chunk=4, use_graph=False)
+    out = srv.run(reqs)
+    assert srv.stats["retired"] == 12
+    for r in out:
+        assert r.u is not None and r.relres <= 1e-8
+    assert srv.stats["table_builds"] > 3
