@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--inner-rtol", type=float, default=0.1,
                     help="FP32 residual reduction between FP64 residual replacements")
     ap.add_argument("--no-mr", action="store_true", help="skip the Mooney-Rivlin (12^3 Q2/Q1) leg")
+    ap.add_argument("--serve-requests", type=int, default=128,
+                    help="requests of the served-stream leg (the full stream has 1024); 0 skips it")
     return ap.parse_args()
 
 
@@ -307,6 +309,61 @@ def mr_leg(runs, cpu_seconds):
     return out
 
 
+def serve_leg(n, seed=2024):
+    """Served stream: n requests, 70% diffusion / 30% elasticity trilinear boxes with
+    N_x, N_y, N_z ~ U{8..64}, lognormal(0,1) coefficients (per-request seeds), BCs of
+    the 16^3 Laplace and 96x32x32 elasticity cases, FP64 to 1e-8.  Measured: the
+    grouped server's capacity (all requests at t=0), then latency at Poisson arrivals
+    of 0.7x that capacity (arrival -> solution on the host), and the one-at-a-time FP64
+    path on the same requests."""
+    import torch
+    from iron_amd import serving
+    srv = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=True)
+    warm = serving.make_stream(8, seed=seed + 1, lo=8, hi=16)
+    srv.run(warm)
+    reqs = serving.make_stream(n, seed=seed)
+    t = time.perf_counter()
+    out = srv.run(reqs)
+    torch.cuda.synchronize()
+    wall_sat = time.perf_counter() - t
+    cap = n / wall_sat
+    timing_sat = dict(srv.timing)
+    for k in srv.timing:
+        srv.timing[k] = 0.0
+    reqs_p = serving.make_stream(n, seed=seed, rate=0.7 * cap)
+    t = time.perf_counter()
+    out_p = srv.run(reqs_p)
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - t
+    lat = np.array([r.latency for r in out_p]) * 1e3
+    structures = {}
+    seq = serving.make_stream(n, seed=seed)
+    service = []
+    t = time.perf_counter()
+    for r in seq:
+        t1 = time.perf_counter()
+        serving.solve_one_fp64(r, structures)
+        service.append(time.perf_counter() - t1)
+    torch.cuda.synchronize()
+    wall_seq = time.perf_counter() - t
+    service = np.array(service) * 1e3
+    elems = sum(int(np.prod(r.n_elem)) for r in reqs)
+    return {
+        "workload": f"served stream: {n} requests, 70% diffusion / 30% elasticity Q1 boxes N~U{{8..64}}^3, "
+                    "lognormal(0,1) coefficients, FP64 Jacobi CG to 1e-8, grouped CG + HBM-pool admission",
+        "requests": n, "elements_total": elems,
+        "capacity_solves_per_s": cap, "elems_per_s_at_capacity": elems / wall_sat,
+        "poisson_rate_per_s": 0.7 * cap, "poisson_solves_per_s": n / wall_p,
+        "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
+        "one_at_a_time_fp64_solves_per_s": n / wall_seq,
+        "one_at_a_time_p50_service_ms": float(np.percentile(service, 50)),
+        "one_at_a_time_p99_service_ms": float(np.percentile(service, 99)),
+        "speedup_vs_one_at_a_time": (n / wall_sat) / (n / wall_seq),
+        "max_relres": max(r.relres for r in out + out_p),
+        "server_phase_s_at_capacity": timing_sat, "server_stats": dict(srv.stats),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -379,10 +436,13 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)
+    del P
+    torch.cuda.empty_cache()
     if not args.no_mr:
-        del P
-        torch.cuda.empty_cache()
         out["mooney_rivlin"] = mr_leg(2, 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 10.0))
+        torch.cuda.empty_cache()
+    if args.serve_requests > 0:
+        out["served_stream"] = serve_leg(args.serve_requests)
     print(json.dumps(out))
 
 

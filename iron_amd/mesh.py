@@ -21,6 +21,17 @@ import numpy as np
 from .basis import LagrangeBasis
 
 
+def _lattice(dims):
+    """[prod(dims), len(dims)] multi-indices, first index fastest (Fortran order)."""
+    dims = [int(d) for d in dims]
+    cols, inner = [], 1
+    total = int(np.prod(dims))
+    for d in dims:
+        cols.append(np.tile(np.repeat(np.arange(d, dtype=np.int64), inner), total // (inner * d)))
+        inner *= d
+    return np.stack(cols, axis=1)
+
+
 class RegularMesh:
     def __init__(self, n_elem, bases, extent, origin=None):
         self.n_elem = tuple(int(n) for n in n_elem)
@@ -38,15 +49,22 @@ class RegularMesh:
         b0 = self.bases[0]
         p0 = b0.nodes_xi - 1
         tot0 = [p0 * n + 1 for n in ne]
-        eidx = np.array(np.unravel_index(np.arange(self.num_elements), ne, order="F")).T  # [E, d]
+        eidx = _lattice(ne)  # [E, d]
         strides0 = np.cumprod([1] + tot0[:-1])
         self.user_nodes = []     # [b] -> [E, nn] user numbers (1-based)
         self.comp_user = []      # [b] -> sorted unique user numbers of the mesh component
         self.elem_nodes = []     # [b] -> [E, nn] local node numbers (0-based)
         for bi, b in enumerate(self.bases):
             if bi == 0:
-                gpos = eidx[:, None, :] * p0 + b.node_index[None, :, :]
-                user = 1 + (gpos * strides0[None, None, :]).sum(axis=2)
+                base = (eidx * (p0 * strides0)[None, :]).sum(axis=1)          # [E] first node - 1
+                offs = (b.node_index * strides0[None, :]).sum(axis=1)          # [nn]
+                user = 1 + base[:, None] + offs[None, :]
+                # every lattice node of the first basis belongs to an element, so its
+                # sorted unique user numbers are 1..prod(tot0) and local = user - 1
+                self.user_nodes.append(user)
+                self.comp_user.append(np.arange(1, int(np.prod(tot0)) + 1, dtype=np.int64))
+                self.elem_nodes.append((user - 1).astype(np.int32))
+                continue
             elif b.nodes_xi == b0.nodes_xi:
                 user = self.user_nodes[0].copy()
             elif b.nodes_xi == 2:
@@ -61,7 +79,7 @@ class RegularMesh:
             self.elem_nodes.append(np.searchsorted(uniq, user).astype(np.int32))
         # geometric coordinates on mesh component 1
         delta = [self.extent[k] / float(tot0[k] - 1) for k in range(d)]
-        pos = np.array(np.unravel_index(self.comp_user[0] - 1, tot0, order="F")).T  # [N, d]
+        pos = _lattice(tot0)  # [N, d]: component-1 nodes are the whole lattice, in order
         coords = np.empty((len(self.comp_user[0]), d))
         for c in range(d):
             coords[:, c] = self.origin[c] + pos[:, c].astype(np.float64) * delta[c]

@@ -183,6 +183,8 @@ class Server:
         wb = int(L.mife_grouped_work_bytes(self.max_segments, self.max_blocks))
         self.table = torch.zeros(wb, dtype=torch.uint8, device=dev)
         self.init_table = torch.zeros(wb, dtype=torch.uint8, device=dev)
+        self.resid_table = torch.zeros(wb, dtype=torch.uint8, device=dev)
+        self.side = torch.cuda.Stream()
         self.staging = np.zeros(wb, dtype=np.uint8)
         self.alloc_rows = _FirstFit(self.rows_cap)
         self.alloc_rp = _FirstFit(self.rp_cap)
@@ -194,13 +196,18 @@ class Server:
         self.cache_shapes = int(cache_shapes)
         self.graph = None
         self.stats = dict(admitted=0, retired=0, chunks=0, graph_captures=0, graph_replays=0, table_builds=0)
+        # host-side wall time per phase (seconds): structure builds, admission
+        # (assembly + reduction + pool copies), iteration chunks, status reads, retirement
+        self.timing = dict(structure=0.0, admit=0.0, chunk=0.0, status=0.0, retire=0.0)
 
     # -- structures
     def _structure(self, req: Request) -> _Structure:
         key = (req.kind, tuple(req.n_elem))
         st = self.cache.get(key)
         if st is None:
+            t = time.perf_counter()
             st = _Structure(req.kind, req.n_elem, self.device)
+            self.timing["structure"] += time.perf_counter() - t
             self.cache[key] = st
             while len(self.cache) > self.cache_shapes:
                 self.cache.popitem(last=False)
@@ -283,48 +290,70 @@ class Server:
         self.stats["retired"] += 1
 
     # -- the serving loop
-    def run(self, requests, max_batch: int | None = None):
+    def run(self, requests, max_batch: int | None = None, overlap: bool = True):
         """Serve ``requests`` (arrival times relative to the call); returns them with
-        results, iterations, relative residuals and completion times filled in."""
+        results, iterations, relative residuals and completion times filled in.
+
+        With ``overlap`` the assembly, reduction and pool set-up of arriving requests
+        run on a second stream while the group's iteration chunk runs on the main
+        stream; they join the group at the next chunk boundary."""
         queue = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        active = []
+        active, pending = [], []
         t0 = time.perf_counter()
         qi = 0
         nb = 0
         changed = True
-        while qi < len(queue) or active:
-            now = time.perf_counter() - t0
-            # admission (FIFO, stop at the first request that does not fit)
-            new = []
-            while qi < len(queue) and queue[qi].arrival <= now and (max_batch is None or len(active) < max_batch):
-                e = self.admit(queue[qi], now) if self.fits(queue[qi]) else None
-                if e is None:
-                    break
-                new.append(e)
-                active.append(e)
-                qi += 1
-            if new:
-                nbi = self._build_table(new, self.init_table)
-                call("mife_grouped_cg_init", len(new), nbi, self.max_segments, self.max_blocks, ptr(self.rp),
-                     ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
-                     ptr(self.init_table), ptr(self.state), fem._stream())
+        main = torch.cuda.current_stream()
+        side = self.side if overlap else main
+        while qi < len(queue) or active or pending:
+            if pending:
+                main.wait_stream(side)
+                active += pending
+                pending = []
                 changed = True
-            if not active:
-                if qi < len(queue):
+            launched = False
+            if active:
+                if changed:
+                    nb = self._build_table(active, self.table)
+                    self.graph = None
+                    changed = False
+                tc = time.perf_counter()
+                self._run_chunk(len(active), nb)
+                self.stats["chunks"] += 1
+                self.timing["chunk"] += time.perf_counter() - tc
+                launched = True
+            # admission of arrived requests (FIFO, stop at the first that does not fit)
+            ta = time.perf_counter()
+            with torch.cuda.stream(side):
+                new = []
+                now = time.perf_counter() - t0
+                while qi < len(queue) and queue[qi].arrival <= now and (
+                        max_batch is None or len(active) + len(new) < max_batch):
+                    e = self.admit(queue[qi], now) if self.fits(queue[qi]) else None
+                    if e is None:
+                        break
+                    new.append(e)
+                    qi += 1
+                if new:
+                    nbi = self._build_table(new, self.init_table)
+                    call("mife_grouped_cg_init", len(new), nbi, self.max_segments, self.max_blocks, ptr(self.rp),
+                         ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
+                         ptr(self.init_table), ptr(self.state), fem._stream())
+                    pending += new
+            self.timing["admit"] += time.perf_counter() - ta
+            if not launched:
+                if not pending and qi < len(queue):
                     time.sleep(max(0.0, min(queue[qi].arrival - (time.perf_counter() - t0), 0.002)))
                 continue
-            if changed:
-                nb = self._build_table(active, self.table)
-                self.graph = None
-                changed = False
-            self._run_chunk(len(active), nb)
-            self.stats["chunks"] += 1
+            ts = time.perf_counter()
             it, dn, rr = self._status()
+            self.timing["status"] += time.perf_counter() - ts
             finished = [e for e in active if dn[e["seg"].slot]]
+            tr = time.perf_counter()
             if finished:
-                tbl = self._build_table(finished, self.init_table)
+                tbl = self._build_table(finished, self.resid_table)
                 call("mife_grouped_cg_residual", len(finished), tbl, self.max_segments, self.max_blocks, ptr(self.rp),
-                     ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.init_table), ptr(self.state),
+                     ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.resid_table), ptr(self.state),
                      fem._stream())
                 it, dn, rr = self._status()
                 for e in finished:
@@ -333,6 +362,7 @@ class Server:
                     self._retire(e, t0)
                 active = [e for e in active if not dn[e["seg"].slot]]
                 changed = True
+            self.timing["retire"] += time.perf_counter() - tr
         return queue
 
     def _run_chunk(self, n_seg, nb):
