@@ -50,7 +50,10 @@ extern "C" {
  * original row perm[k]; ptr = slice_ptr[n_slices+1] (entry offsets, multiples
  * of 64); entry j of row k of slice s sits at ptr[s] + 64 j + (k - 64 s); col
  * holds column indices in the slice numbering (padding: value 0).  Solvers take
- * b and x in the original numbering and permute internally. */
+ * b and x in the original numbering and permute internally.
+ * Compressed SELL: when col_base (per-slice base column, [n_slices]) is not
+ * NULL, col points to uint16_t offsets and the column of an entry of slice s is
+ * col_base[s] + offset (mife_sell_compress); CSR ignores col_base. */
 typedef struct mife_matrix {
   int n;
   int format;
@@ -59,6 +62,7 @@ typedef struct mife_matrix {
   const int32_t* perm;
   int n_slices;
   int64_t nnz;
+  const int32_t* col_base;
 } mife_matrix;
 
 /* ---------------------------------------------------------------- runtime */
@@ -131,6 +135,13 @@ int mife_sell_build(int n, const int32_t* row_ptr, const int32_t* col, const int
                     const int32_t* perm, const int32_t* iperm, int n_slices,
                     const int32_t* slice_ptr, int32_t* sell_col, int32_t* sell_src,
                     void* stream);
+
+/* 16-bit SELL columns: col_base[s] = smallest column of slice s, col16[e] =
+ * sell_col[e] - col_base[s].  *ok (host) is 1 when every slice spans fewer than
+ * 65,536 columns (then col16/col_base are valid), else 0.  Synchronises the
+ * stream.  Halves the column bytes the SELL products stream. */
+int mife_sell_compress(int n_slices, const int32_t* slice_ptr, const int32_t* sell_col,
+                       int32_t* col_base, uint16_t* col16, int* ok, void* stream);
 
 /* ------------------------------------------------------------- assembly */
 

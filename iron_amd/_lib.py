@@ -34,7 +34,7 @@ class NotConvergedError(CMFEError):
 class MatrixDesc(C.Structure):
     """mife_matrix: a square solver matrix in CSR or SELL-64 storage."""
     _fields_ = [("n", C.c_int), ("format", C.c_int), ("ptr", C.c_void_p), ("col", C.c_void_p),
-                ("perm", C.c_void_p), ("n_slices", C.c_int), ("nnz", C.c_int64)]
+                ("perm", C.c_void_p), ("n_slices", C.c_int), ("nnz", C.c_int64), ("col_base", C.c_void_p)]
 
 
 MIFE_FMT_CSR = 0
@@ -74,6 +74,7 @@ _SIG = {
     "mife_assemble_elastic": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P]),
     "mife_spmv": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _P]),
     "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "mife_sell_compress": (_I, [_I, _P, _P, _P, _P, C.POINTER(_I), _P]),
     "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
     "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mife_field_update": (_I, [_I, _P, _P, _P, _P]),

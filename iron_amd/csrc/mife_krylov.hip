@@ -86,12 +86,14 @@ __global__ void diag_inv_kernel(MatArgs A, const T* __restrict__ vals, T* __rest
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   T d = 0;
-  if (A.fmt == MIFE_FMT_SELL) {
+  if (A.fmt == MIFE_FMT_SELL || A.fmt == kFmtSell16) {
     const int s = i / kSlice, lane = i - s * kSlice;
     const int base = A.ptr[s], len = (A.ptr[s + 1] - base) / kSlice;
+    const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col);
     for (int j = 0; j < len; ++j) {
       const int e = base + j * kSlice + lane;
-      if (A.col[e] == i && vals[e] != (T)0) { d = vals[e]; break; }
+      const int c = A.fmt == kFmtSell16 ? A.cbase[s] + (int)c16[e] : A.col[e];
+      if (c == i && vals[e] != (T)0) { d = vals[e]; break; }
     }
   } else {
     int lo = A.ptr[i], hi = A.ptr[i + 1];
@@ -569,7 +571,7 @@ static int make_mat(const mife_matrix* A, Mat* m) {
   MIFE_REQUIRE(A->n > 0, MIFE_ERR_ARG, "empty matrix");
   MIFE_REQUIRE(A->format == MIFE_FMT_CSR || (A->perm && A->n_slices == div_up(A->n, kSlice)), MIFE_ERR_ARG,
                "SELL matrix needs perm and n_slices = ceil(n/64)");
-  m->a = MatArgs{A->n, A->format, A->ptr, A->col, A->n_slices};
+  m->a = mat_args(A);
   m->perm = A->format == MIFE_FMT_SELL ? A->perm : nullptr;
   m->L = spmv_lanes(A->nnz, A->n);
   return MIFE_OK;
@@ -579,7 +581,7 @@ template <typename T>
 static void launch_cg_init(const Mat& M, const T* vals, const T* b, const T* x, const T* dinv, T* r, T* z, T* p,
                            T* q, CgState* st, double tol2, int maxit, hipStream_t s) {
   const int tk = prof::begin(s);
-  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   MIFE_DISPATCH_FMT(M.a, M.L, (cg_init_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, dinv, r, z, p, q,
                                                                                   st)));
   fin_init_kernel<<<1, kFinThreads, 0, s>>>(st, grid, tol2, maxit);
@@ -592,7 +594,7 @@ static void launch_cg_init(const Mat& M, const T* vals, const T* b, const T* x, 
 template <typename T>
 static void launch_cg_iter(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
                            hipStream_t s, int* tk) {
-  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   const int vg = vec_grid(M.a.n);
   tk[0] = prof::begin(s);
   MIFE_DISPATCH_FMT(M.a, M.L, (cg_spmv_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, p, q, st)));
@@ -606,7 +608,7 @@ static void launch_cg_iter(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, 
 
 static void launch_residual64(const Mat& M, const double* vals, const double* b, const double* x, double* r,
                               CgState* st, hipStream_t s) {
-  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   const int tk = prof::begin(s);
   MIFE_DISPATCH_FMT(M.a, M.L, (residual64_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, r, st)));
   prof::end(tk, s, prof::kResidual64);
@@ -865,7 +867,7 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
   double* w1 = w.take<double>(n);
   double* w2 = w.take<double>(n);
   const int vg = vec_grid(n);
-  const int grid = row_grid(M.a, M.a.fmt == MIFE_FMT_SELL ? 64 : M.L);
+  const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);

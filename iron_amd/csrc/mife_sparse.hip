@@ -3,6 +3,9 @@
 #include "mife_common.hpp"
 #include "mife_sparse.hpp"
 
+#include <algorithm>
+#include <vector>
+
 namespace mife {
 
 template <typename S, typename D>
@@ -39,7 +42,7 @@ __global__ void sell_build_kernel(int n, const int32_t* __restrict__ rp, const i
       sell_col[e] = iperm[col[p]];
       sell_src[e] = src_pos ? src_pos[p] : p;
     } else {
-      sell_col[e] = k < n ? k : 0;
+      sell_col[e] = k < n ? k : n - 1;  // a column inside the slice's range
       sell_src[e] = -1;
     }
   }
@@ -104,6 +107,34 @@ __global__ void backsub_kernel(int n, const int32_t* __restrict__ row_ptr, const
   if (lane == 0) R[i] = acc - (src ? src[i] : 0.0);
 }
 
+// One wavefront per slice: base = min column of the slice, 16-bit offsets, and
+// the slice's span (max - min) into span[s].
+__global__ void sell_compress_kernel(int n_slices, const int32_t* __restrict__ slice_ptr,
+                                     const int32_t* __restrict__ sell_col, int32_t* __restrict__ col_base,
+                                     uint16_t* __restrict__ col16, int* __restrict__ span) {
+  const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= n_slices) return;
+  const int base = slice_ptr[s], end = slice_ptr[s + 1];
+  int lo = 0x7fffffff, hi = -1;
+  for (int e = base + lane; e < end; e += kSlice) {
+    const int c = sell_col[e];
+    lo = min(lo, c);
+    hi = max(hi, c);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off, 64));
+    hi = max(hi, __shfl_xor(hi, off, 64));
+  }
+  if (end == base) lo = hi = 0;
+  if (lane == 0) {
+    col_base[s] = lo;
+    span[s] = hi - lo;
+  }
+  for (int e = base + lane; e < end; e += kSlice) col16[e] = (uint16_t)min(sell_col[e] - lo, 65535);
+}
+
 }  // namespace mife
 
 using namespace mife;
@@ -116,7 +147,7 @@ int mife_spmv(const mife_matrix* A, const void* vals, const void* x, void* y, in
   MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL, MIFE_ERR_ARG, "invalid matrix format");
   if (A->n == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
-  const MatArgs M{A->n, A->format, A->ptr, A->col, A->n_slices};
+  const MatArgs M = mat_args(A);
   const int L = spmv_lanes(A->nnz, A->n);
   if (prec == MIFE_F64)
     launch_spmv<double>(M, (const double*)vals, (const double*)x, (double*)y, L, s);
@@ -136,6 +167,26 @@ int mife_sell_build(int n, const int32_t* row_ptr, const int32_t* col, const int
   sell_build_kernel<<<div_up(n_slices, 4), 256, 0, (hipStream_t)stream>>>(n, row_ptr, col, src_pos, perm, iperm,
                                                                           n_slices, slice_ptr, sell_col, sell_src);
   MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+int mife_sell_compress(int n_slices, const int32_t* slice_ptr, const int32_t* sell_col, int32_t* col_base,
+                       uint16_t* col16, int* ok, void* stream) {
+  MIFE_REQUIRE(slice_ptr && sell_col && col_base && col16 && ok, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  *ok = 0;
+  if (n_slices == 0) {
+    *ok = 1;
+    return MIFE_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int* span = nullptr;
+  MIFE_CHECK_HIP(hipMallocAsync((void**)&span, sizeof(int) * (size_t)n_slices, s));
+  sell_compress_kernel<<<div_up(n_slices, 4), 256, 0, s>>>(n_slices, slice_ptr, sell_col, col_base, col16, span);
+  std::vector<int> h(n_slices);
+  MIFE_CHECK_HIP(hipMemcpyAsync(h.data(), span, sizeof(int) * (size_t)n_slices, hipMemcpyDeviceToHost, s));
+  MIFE_CHECK_HIP(hipFreeAsync(span, s));
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
+  *ok = *std::max_element(h.begin(), h.end()) < 65536 ? 1 : 0;
   return MIFE_OK;
 }
 

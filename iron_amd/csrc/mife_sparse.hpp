@@ -20,13 +20,22 @@ namespace mife {
 constexpr int kSlice = 64;
 constexpr int kMaxGrid = 2048;  // resident grid: 8 blocks of 256 threads per CU on 256 CUs
 
+// Internal format tag of SELL with 16-bit slice-relative columns.
+constexpr int kFmtSell16 = 2;
+
 struct MatArgs {
   int n;
-  int fmt;  // MIFE_FMT_CSR / MIFE_FMT_SELL
+  int fmt;  // MIFE_FMT_CSR / MIFE_FMT_SELL / kFmtSell16
   const int32_t* ptr;
-  const int32_t* col;
+  const int32_t* col;  // kFmtSell16: uint16 offsets
   int n_slices;
+  const int32_t* cbase;  // kFmtSell16: per-slice base column
 };
+
+inline MatArgs mat_args(const mife_matrix* A) {
+  const bool c16 = A->format == MIFE_FMT_SELL && A->col_base != nullptr;
+  return MatArgs{A->n, c16 ? kFmtSell16 : A->format, A->ptr, A->col, A->n_slices, c16 ? A->col_base : nullptr};
+}
 
 template <typename T>
 __device__ __forceinline__ T stream_load(const T* p) {
@@ -61,7 +70,8 @@ __device__ __forceinline__ T row_partial(const int32_t* __restrict__ rp, const i
   return a0 + a1;
 }
 
-// SELL-64 row sum of slice `s`, lane `lane` (one row per lane, complete sum).
+// SELL-64 row sum of slice `s`, lane `lane` (one row per lane, complete sum),
+// eight entries in flight per lane.
 template <typename T>
 __device__ __forceinline__ T sell_row(const int32_t* __restrict__ sp, const int32_t* __restrict__ col,
                                       const T* __restrict__ vals, const T* __restrict__ x, int s, int lane) {
@@ -69,36 +79,87 @@ __device__ __forceinline__ T sell_row(const int32_t* __restrict__ sp, const int3
   const int len = (sp[s + 1] - base) / kSlice;
   const int32_t* c = col + base + lane;
   const T* v = vals + base + lane;
-  T a0 = 0, a1 = 0;
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   int j = 0;
-  for (; j + 3 < len; j += 4) {
-    const int c0 = stream_load(c), c1 = stream_load(c + kSlice), c2 = stream_load(c + 2 * kSlice),
-              c3 = stream_load(c + 3 * kSlice);
-    const T v0 = stream_load(v), v1 = stream_load(v + kSlice), v2 = stream_load(v + 2 * kSlice),
-            v3 = stream_load(v + 3 * kSlice);
-    a0 += v0 * x[c0];
-    a1 += v1 * x[c1];
-    a0 += v2 * x[c2];
-    a1 += v3 * x[c3];
-    c += 4 * kSlice;
-    v += 4 * kSlice;
+  for (; j + 7 < len; j += 8) {
+    int ci[8];
+    T vi[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ci[u] = stream_load(c + u * kSlice);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) vi[u] = stream_load(v + u * kSlice);
+    a0 += vi[0] * x[ci[0]];
+    a1 += vi[1] * x[ci[1]];
+    a2 += vi[2] * x[ci[2]];
+    a3 += vi[3] * x[ci[3]];
+    a0 += vi[4] * x[ci[4]];
+    a1 += vi[5] * x[ci[5]];
+    a2 += vi[6] * x[ci[6]];
+    a3 += vi[7] * x[ci[7]];
+    c += 8 * kSlice;
+    v += 8 * kSlice;
   }
   for (; j < len; ++j) {
     a0 += stream_load(v) * x[stream_load(c)];
     c += kSlice;
     v += kSlice;
   }
-  return a0 + a1;
+  return (a0 + a1) + (a2 + a3);
+}
+
+// As sell_row with 16-bit columns relative to the slice's base column; eight
+// entries in flight per lane (the 2-byte columns leave registers for it).
+template <typename T>
+__device__ __forceinline__ T sell16_row(const int32_t* __restrict__ sp, const uint16_t* __restrict__ col,
+                                        const int32_t* __restrict__ cbase, const T* __restrict__ vals,
+                                        const T* __restrict__ x, int s, int lane) {
+  const int base = sp[s];
+  const int len = (sp[s + 1] - base) / kSlice;
+  const T* xs = x + cbase[s];
+  const uint16_t* c = col + base + lane;
+  const T* v = vals + base + lane;
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int j = 0;
+  for (; j + 7 < len; j += 8) {
+    int ci[8];
+    T vi[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ci[u] = stream_load(c + u * kSlice);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) vi[u] = stream_load(v + u * kSlice);
+    a0 += vi[0] * xs[ci[0]];
+    a1 += vi[1] * xs[ci[1]];
+    a2 += vi[2] * xs[ci[2]];
+    a3 += vi[3] * xs[ci[3]];
+    a0 += vi[4] * xs[ci[4]];
+    a1 += vi[5] * xs[ci[5]];
+    a2 += vi[6] * xs[ci[6]];
+    a3 += vi[7] * xs[ci[7]];
+    c += 8 * kSlice;
+    v += 8 * kSlice;
+  }
+  for (; j < len; ++j) {
+    a0 += stream_load(v) * xs[(int)stream_load(c)];
+    c += kSlice;
+    v += kSlice;
+  }
+  return (a0 + a1) + (a2 + a3);
 }
 
 // Calls f(row, (A x)_row) once per row (by one lane), over the whole grid.
 template <typename T, int FMT, int L, typename F>
 __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restrict__ vals, const T* __restrict__ x,
                                              F&& f) {
-  if constexpr (FMT == MIFE_FMT_SELL) {
+  if constexpr (FMT == MIFE_FMT_SELL || FMT == kFmtSell16) {
     const int wpb = blockDim.x >> 6, lane = threadIdx.x & 63;
     for (int s = blockIdx.x * wpb + (threadIdx.x >> 6); s < A.n_slices; s += gridDim.x * wpb) {
-      const T ax = x ? sell_row<T>(A.ptr, A.col, vals, x, s, lane) : (T)0;
+      T ax = (T)0;
+      if (x) {
+        if constexpr (FMT == kFmtSell16)
+          ax = sell16_row<T>(A.ptr, reinterpret_cast<const uint16_t*>(A.col), A.cbase, vals, x, s, lane);
+        else
+          ax = sell_row<T>(A.ptr, A.col, vals, x, s, lane);
+      }
       const int row = s * kSlice + lane;
       if (row < A.n) f(row, ax);
     }
@@ -123,7 +184,7 @@ inline int spmv_lanes(long long nnz, int n) {
 }
 
 inline int row_grid(const MatArgs& A, int L) {
-  const int units = A.fmt == MIFE_FMT_SELL ? div_up(A.n_slices, 4) : div_up(A.n, 256 / L);
+  const int units = A.fmt != MIFE_FMT_CSR ? div_up(A.n_slices, 4) : div_up(A.n, 256 / L);
   return std::max(1, std::min(units, kMaxGrid));
 }
 
@@ -132,6 +193,9 @@ inline int row_grid(const MatArgs& A, int L) {
   do {                                                                          \
     if ((A).fmt == MIFE_FMT_SELL) {                                             \
       constexpr int FMT_ = MIFE_FMT_SELL, L_ = 64;                              \
+      STMT;                                                                     \
+    } else if ((A).fmt == kFmtSell16) {                                         \
+      constexpr int FMT_ = kFmtSell16, L_ = 64;                                 \
       STMT;                                                                     \
     } else {                                                                    \
       switch (L) {                                                              \

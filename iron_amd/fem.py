@@ -156,7 +156,7 @@ class SolverMatrix:
     Either way the values are gathered straight from the equations-matrix CSR.
     """
 
-    def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 256):
+    def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 256, compress: bool = True):
         dev = red.csr.device
         self.red = red
         self.n = red.n_free
@@ -165,7 +165,7 @@ class SolverMatrix:
             self.src = red.src_pos
             self.stored = red.nnz
             self.padding = 0.0
-            self.desc = MatrixDesc(self.n, _lib.MIFE_FMT_CSR, ptr(red.row_ptr), ptr(red.col), None, 0, red.nnz)
+            self.desc = MatrixDesc(self.n, _lib.MIFE_FMT_CSR, ptr(red.row_ptr), ptr(red.col), None, 0, red.nnz, None)
             return
         if fmt != "sell":
             raise ValueError(f"unknown solver matrix format {fmt!r}")
@@ -192,8 +192,21 @@ class SolverMatrix:
         self.src = torch.empty(max(self.stored, 1), dtype=torch.int32, device=dev)
         call("mife_sell_build", n, ptr(red.row_ptr), ptr(red.col), ptr(red.src_pos), ptr(self.perm), ptr(self.iperm),
              n_slices, ptr(self.slice_ptr), ptr(self.col), ptr(self.src), _stream())
-        self.desc = MatrixDesc(n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), ptr(self.col), ptr(self.perm), n_slices,
-                               self.stored)
+        self.col16 = None
+        self.col_base = None
+        colp = ptr(self.col)
+        if compress:
+            # 16-bit slice-relative columns when every slice spans < 65,536 columns
+            col16 = torch.empty(max(self.stored, 1), dtype=torch.int16, device=dev)
+            cbase = torch.empty(max(n_slices, 1), dtype=torch.int32, device=dev)
+            ok = C.c_int(0)
+            call("mife_sell_compress", n_slices, ptr(self.slice_ptr), ptr(self.col), ptr(cbase), ptr(col16),
+                 C.byref(ok), _stream())
+            if ok.value:
+                self.col16, self.col_base = col16, cbase
+                colp = ptr(col16)
+        self.desc = MatrixDesc(n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), colp, ptr(self.perm), n_slices,
+                               self.stored, ptr(self.col_base))
 
     def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
         """Solver-matrix values (this storage) from equations-CSR values (FP64 or FP32)."""

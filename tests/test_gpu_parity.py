@@ -182,13 +182,22 @@ def test_reduction_rhs_and_cg_fp64_against_oracle():
         assert _rel(xg.cpu().numpy(), xo) < 1e-7, fmt
 
 
-def test_sell_storage_matches_csr_product():
-    """SELL-64 values/columns reproduce the CSR solver matrix product exactly in FP64."""
+@pytest.mark.parametrize("compress", [False, True])
+def test_sell_storage_matches_csr_product(compress):
+    """SELL-64 values/columns (int32 or 16-bit slice-relative) reproduce the CSR solver
+    matrix product in FP64, and CG on either storage converges to the same solution."""
     fem, _, _ = _fem()
     pb, pm, pd, ob, om, od, kappa, fixed = _poisson_system((5, 4, 3), 2, seed=5)
     fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, kappa, fixed, np.zeros(pd.num_dofs))
     Sc = fem.SolverMatrix(red, "csr")
-    Ss = fem.SolverMatrix(red, "sell", sigma=128)
+    Ss = fem.SolverMatrix(red, "sell", sigma=128, compress=compress)
+    assert (Ss.col16 is not None) == compress
+    if compress:
+        cb = Ss.col_base.cpu().numpy()
+        c16 = Ss.col16.cpu().numpy().view(np.uint16).astype(np.int64)
+        sp = Ss.slice_ptr.cpu().numpy()
+        full = np.concatenate([cb[k] + c16[sp[k]:sp[k + 1]] for k in range(len(sp) - 1)])
+        assert np.array_equal(full, Ss.col[: Ss.stored].cpu().numpy())
     vc = Sc.gather(v64, fem.F64)
     vs = Ss.gather(v64, fem.F64)
     xv = torch.rand(red.n_free, dtype=torch.float64, device="cuda")
@@ -199,6 +208,13 @@ def test_sell_storage_matches_csr_product():
     y_back[perm] = ys
     assert _rel(y_back.cpu().numpy(), yc.cpu().numpy()) < 1e-14
     assert Ss.stored >= red.nnz and Ss.stored % 64 == 0
+    ws = fem.KrylovWorkspace(red.n_free, "cuda")
+    xc = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    xs = torch.zeros_like(xc)
+    ic = fem.cg(Sc, vc, b, xc, 1e-12, 10000, ws)
+    is_ = fem.cg(Ss, vs, b, xs, 1e-12, 10000, ws)
+    assert ic.relres <= 1e-12 and is_.relres <= 1e-12
+    assert _rel(xs.cpu().numpy(), xc.cpu().numpy()) < 1e-10
 
 
 def test_unit_cube_laplace_u_equals_x_and_reactions():
