@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--inner-rtol", type=float, default=0.1,
                     help="FP32 residual reduction between FP64 residual replacements")
     ap.add_argument("--no-mr", action="store_true", help="skip the Mooney-Rivlin (12^3 Q2/Q1) leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run three streaming passes y = 1.0 * x over 256 MiB FP32 arrays (reads and "
+                         "writes 268,435,456 bytes each: the reference kernel for HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=128,
                     help="requests of the served-stream leg (the full stream has 1024); 0 skips it")
     return ap.parse_args()
@@ -374,6 +377,13 @@ def main():
     import torch
     from iron_amd import _lib, fem
     torch.cuda.init()
+    if args.calibrate_copy:
+        a = torch.ones(1 << 26, dtype=torch.float32, device="cuda")
+        bcopy = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=bcopy)
+        torch.cuda.synchronize()
+        del a, bcopy
     P = build_problem(args)
     red, csr = P["red"], P["csr"]
     E = P["m"].num_elements

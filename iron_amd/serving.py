@@ -356,11 +356,25 @@ class Server:
                      ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.resid_table), ptr(self.state),
                      fem._stream())
                 it, dn, rr = self._status()
+                redo = []
                 for e in finished:
-                    e["req"].iterations = int(it[e["seg"].slot])
-                    e["req"].relres = float(rr[e["seg"].slot])
-                    self._retire(e, t0)
-                active = [e for e in active if not dn[e["seg"].slot]]
+                    slot = e["seg"].slot
+                    e["req"].iterations += int(it[slot])
+                    e["req"].relres = float(rr[slot])
+                    if rr[slot] > e["req"].rtol and e.get("restarts", 0) < 4:
+                        # the recurrence drifted below the true residual: restart CG from it
+                        e["restarts"] = e.get("restarts", 0) + 1
+                        redo.append(e)
+                    else:
+                        self._retire(e, t0)
+                if redo:
+                    nbr = self._build_table(redo, self.resid_table)
+                    call("mife_grouped_cg_init", len(redo), nbr, self.max_segments, self.max_blocks, ptr(self.rp),
+                         ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
+                         ptr(self.resid_table), ptr(self.state), fem._stream())
+                redo_ids = {id(e) for e in redo}
+                retired = {id(e) for e in finished if id(e) not in redo_ids}
+                active = [e for e in active if id(e) not in retired]
                 changed = True
             self.timing["retire"] += time.perf_counter() - tr
         return queue
