@@ -53,7 +53,12 @@ extern "C" {
  * b and x in the original numbering and permute internally.
  * Compressed SELL: when col_base (per-slice base column, [n_slices]) is not
  * NULL, col points to uint16_t offsets and the column of an entry of slice s is
- * col_base[s] + offset (mife_sell_compress); CSR ignores col_base. */
+ * col_base[s] + offset (mife_sell_compress); CSR ignores col_base.
+ * SELL schedule: when slice_order is not NULL it holds 8 ranges of order_span
+ * entries (mife_sell_order): range k lists the slices of the k-th eighth of the
+ * matrix, longest first, padded with -1.  SELL products then give range k to
+ * the blocks that the dispatcher places on XCD k and start the longest slices
+ * first; NULL keeps the plain slice order.  CSR ignores both fields. */
 typedef struct mife_matrix {
   int n;
   int format;
@@ -63,6 +68,8 @@ typedef struct mife_matrix {
   int n_slices;
   int64_t nnz;
   const int32_t* col_base;
+  const int32_t* slice_order;
+  int order_span;
 } mife_matrix;
 
 /* ---------------------------------------------------------------- runtime */
@@ -142,6 +149,18 @@ int mife_sell_build(int n, const int32_t* row_ptr, const int32_t* col, const int
  * stream.  Halves the column bytes the SELL products stream. */
 int mife_sell_compress(int n_slices, const int32_t* slice_ptr, const int32_t* sell_col,
                        int32_t* col_base, uint16_t* col16, int* ok, void* stream);
+
+/* Length (entries) of the slice_order array of a SELL matrix with n_slices
+ * slices: 8 * order_span, order_span = 4 * ceil(ceil(n_slices / 8) / 4). */
+int mife_sell_order_len(int n_slices);
+
+/* SELL schedule (see mife_matrix): splits the slices into 8 contiguous ranges
+ * and lists each range by decreasing slice length (ties by slice index),
+ * padding with -1.  order has mife_sell_order_len(n_slices) entries (device);
+ * *order_span (host) receives the per-range length.  The ranking runs on the
+ * host from a copy of slice_ptr; synchronises the stream.  No reference
+ * counterpart (PETSc's MatMult walks CSR rows in order). */
+int mife_sell_order(int n_slices, const int32_t* slice_ptr, int32_t* order, int* order_span, void* stream);
 
 /* ------------------------------------------------------------- assembly */
 

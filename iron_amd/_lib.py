@@ -34,7 +34,8 @@ class NotConvergedError(CMFEError):
 class MatrixDesc(C.Structure):
     """mife_matrix: a square solver matrix in CSR or SELL-64 storage."""
     _fields_ = [("n", C.c_int), ("format", C.c_int), ("ptr", C.c_void_p), ("col", C.c_void_p),
-                ("perm", C.c_void_p), ("n_slices", C.c_int), ("nnz", C.c_int64), ("col_base", C.c_void_p)]
+                ("perm", C.c_void_p), ("n_slices", C.c_int), ("nnz", C.c_int64), ("col_base", C.c_void_p),
+                ("slice_order", C.c_void_p), ("order_span", C.c_int)]
 
 
 MIFE_FMT_CSR = 0
@@ -75,6 +76,8 @@ _SIG = {
     "mife_spmv": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _P]),
     "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "mife_sell_compress": (_I, [_I, _P, _P, _P, _P, C.POINTER(_I), _P]),
+    "mife_sell_order_len": (_I, [_I]),
+    "mife_sell_order": (_I, [_I, _P, _P, C.POINTER(_I), _P]),
     "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
     "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mife_field_update": (_I, [_I, _P, _P, _P, _P]),

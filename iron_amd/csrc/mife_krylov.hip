@@ -25,7 +25,8 @@ extern "C" int mife_profile_enable(int on);
 
 namespace mife {
 
-constexpr int kRedBlocks = kMaxGrid;
+constexpr int kRedBlocks = kMaxSellGrid;  // partial slots per reduced value (largest row grid)
+constexpr int kDotBlocks = kMaxGrid;     // GMRES multidot partial slots (vector grid)
 constexpr int kMaxRestart = 256;
 constexpr int kFinThreads = 1024;  // one-block finishers: two partials per thread
 
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(256) void multidot_kernel(int n, int m, const doubl
   if (threadIdx.x < mc) {
     double s = 0.0;
     for (int wv = 0; wv < 4; ++wv) s += red[threadIdx.x][wv];
-    parts[(size_t)(j0 + threadIdx.x) * kRedBlocks + blockIdx.x] = s;
+    parts[(size_t)(j0 + threadIdx.x) * kDotBlocks + blockIdx.x] = s;
   }
 }
 
@@ -340,7 +341,7 @@ __global__ void multidot_reduce(int m, int nblocks, const double* __restrict__ p
   __shared__ double red[4];
   for (int j = 0; j < m; ++j) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[(size_t)j * kRedBlocks + b];
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[(size_t)j * kDotBlocks + b];
     s = block_sum(s, red);
     if (threadIdx.x == 0) h[j] = s;
     __syncthreads();
@@ -571,7 +572,7 @@ struct Work {
   }
 };
 
-static int vec_grid(int n) { return std::max(1, std::min(div_up(n, 256), kRedBlocks)); }
+static int vec_grid(int n) { return std::max(1, std::min(div_up(n, 256), kMaxGrid)); }
 
 struct Mat {
   MatArgs a;
@@ -754,7 +755,7 @@ size_t mife_solver_work_bytes(int n, int restart) {
   bytes += 3 * (sizeof(CgState) + 256);
   bytes += (8 * nn + 256) * (10 + m);            // FP64 vectors incl. GMRES basis
   bytes += (4 * nn + 256) * 8;                   // FP32 vectors
-  bytes += (8 * (size_t)kRedBlocks * (size_t)std::max(m, (size_t)2) + 256);  // GMRES multidot partials
+  bytes += (8 * (size_t)kDotBlocks * (size_t)std::max(m, (size_t)2) + 256);  // GMRES multidot partials
   bytes += 8 * (3 * m + 8) + 768;                // GMRES coefficient vectors
   bytes += 8 * 64 * 4 + 65536;
   return bytes;
@@ -962,7 +963,7 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   double* x = w.take<double>(n);
   const size_t ldv = ((size_t)n + 63) & ~size_t(63);
   double* V = w.take<double>(ldv * (m + 1));
-  double* parts = w.take<double>((size_t)kRedBlocks * (m + 1));
+  double* parts = w.take<double>((size_t)kDotBlocks * (m + 1));
   double* hdev = w.take<double>(m + 2);
   double* ydev = w.take<double>(m + 2);
   double* ndev = w.take<double>(2);

@@ -190,6 +190,31 @@ int mife_sell_compress(int n_slices, const int32_t* slice_ptr, const int32_t* se
   return MIFE_OK;
 }
 
+int mife_sell_order_len(int n_slices) { return n_slices > 0 ? 8 * sell_order_span(n_slices) : 0; }
+
+int mife_sell_order(int n_slices, const int32_t* slice_ptr, int32_t* order, int* order_span, void* stream) {
+  MIFE_REQUIRE(slice_ptr && order && order_span, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(n_slices >= 0, MIFE_ERR_ARG, "negative slice count");
+  *order_span = 0;
+  if (n_slices == 0) return MIFE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<int32_t> sp(n_slices + 1);
+  MIFE_CHECK_HIP(hipMemcpyAsync(sp.data(), slice_ptr, sizeof(int32_t) * sp.size(), hipMemcpyDeviceToHost, s));
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
+  const int range = div_up(n_slices, 8), span = sell_order_span(n_slices);
+  std::vector<int32_t> h((size_t)8 * span, -1);
+  for (int k = 0; k < 8; ++k) {
+    const int s0 = std::min(k * range, n_slices), s1 = std::min(s0 + range, n_slices);
+    int32_t* out = h.data() + (size_t)k * span;
+    for (int i = s0; i < s1; ++i) out[i - s0] = i;
+    std::stable_sort(out, out + (s1 - s0), [&](int32_t a, int32_t b) { return sp[a + 1] - sp[a] > sp[b + 1] - sp[b]; });
+  }
+  MIFE_CHECK_HIP(hipMemcpyAsync(order, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, s));
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));
+  *order_span = span;
+  return MIFE_OK;
+}
+
 int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int src_prec, void* dst, int dst_prec,
                        void* stream) {
   MIFE_REQUIRE(src_pos && src && dst, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");

@@ -19,9 +19,15 @@ namespace mife {
 
 constexpr int kSlice = 64;
 constexpr int kMaxGrid = 2048;  // resident grid: 8 blocks of 256 threads per CU on 256 CUs
+// Largest grid of a scheduled SELL product (one slice per wavefront, 4 per block).
+constexpr int kMaxSellGrid = 8192;
+constexpr int kXcds = 8;
 
 // Internal format tag of SELL with 16-bit slice-relative columns.
 constexpr int kFmtSell16 = 2;
+
+// Per-XCD range length of a SELL schedule: ceil(n_slices / 8) rounded up to 4.
+inline int sell_order_span(int n_slices) { return 4 * div_up(div_up(n_slices, kXcds), 4); }
 
 struct MatArgs {
   int n;
@@ -30,11 +36,16 @@ struct MatArgs {
   const int32_t* col;  // kFmtSell16: uint16 offsets
   int n_slices;
   const int32_t* cbase;  // kFmtSell16: per-slice base column
+  const int32_t* order;  // SELL schedule (8 ranges of `span` slices, -1 padded) or null
+  int span;
 };
 
 inline MatArgs mat_args(const mife_matrix* A) {
-  const bool c16 = A->format == MIFE_FMT_SELL && A->col_base != nullptr;
-  return MatArgs{A->n, c16 ? kFmtSell16 : A->format, A->ptr, A->col, A->n_slices, c16 ? A->col_base : nullptr};
+  const bool sell = A->format == MIFE_FMT_SELL;
+  const bool c16 = sell && A->col_base != nullptr;
+  const bool sched = sell && A->slice_order != nullptr && A->order_span > 0;
+  return MatArgs{A->n, c16 ? kFmtSell16 : A->format, A->ptr, A->col, A->n_slices, c16 ? A->col_base : nullptr,
+                 sched ? A->slice_order : nullptr, sched ? A->order_span : 0};
 }
 
 template <typename T>
@@ -152,7 +163,7 @@ __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restri
                                              F&& f) {
   if constexpr (FMT == MIFE_FMT_SELL || FMT == kFmtSell16) {
     const int wpb = blockDim.x >> 6, lane = threadIdx.x & 63;
-    for (int s = blockIdx.x * wpb + (threadIdx.x >> 6); s < A.n_slices; s += gridDim.x * wpb) {
+    auto one_slice = [&](int s) {
       T ax = (T)0;
       if (x) {
         if constexpr (FMT == kFmtSell16)
@@ -162,6 +173,18 @@ __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restri
       }
       const int row = s * kSlice + lane;
       if (row < A.n) f(row, ax);
+    };
+    if (A.order) {
+      // Scheduled: the dispatcher places block b on XCD b % 8, which walks its own
+      // range of the order (longest slices first); the grid is a multiple of 8.
+      const int xcd = blockIdx.x % kXcds, bpx = gridDim.x / kXcds;
+      const int32_t* ord = A.order + (size_t)xcd * A.span;
+      for (int k = (blockIdx.x / kXcds) * wpb + (threadIdx.x >> 6); k < A.span; k += bpx * wpb) {
+        const int s = ord[k];
+        if (s >= 0) one_slice(s);
+      }
+    } else {
+      for (int s = blockIdx.x * wpb + (threadIdx.x >> 6); s < A.n_slices; s += gridDim.x * wpb) one_slice(s);
     }
   } else {
     const int gpb = blockDim.x / L, sub = threadIdx.x % L;
@@ -183,7 +206,11 @@ inline int spmv_lanes(long long nnz, int n) {
   return 64;
 }
 
+// Grid of a row loop launched with 256 threads per block.  A scheduled SELL
+// matrix gets one slice per wavefront (blocks of 4), a multiple of 8 blocks.
 inline int row_grid(const MatArgs& A, int L) {
+  if (A.fmt != MIFE_FMT_CSR && A.order)
+    return kXcds * std::max(1, std::min(div_up(A.span, 4), kMaxSellGrid / kXcds));
   const int units = A.fmt != MIFE_FMT_CSR ? div_up(A.n_slices, 4) : div_up(A.n, 256 / L);
   return std::max(1, std::min(units, kMaxGrid));
 }

@@ -154,9 +154,12 @@ class SolverMatrix:
     stores it in slices of 64 rows, one row per wavefront lane, rows sorted by
     length inside windows of ``sigma`` rows so a slice is nearly rectangular.
     Either way the values are gathered straight from the equations-matrix CSR.
+    ``schedule`` gives SELL products a slice order: eight contiguous ranges of
+    slices (one per XCD), each walked longest slice first.
     """
 
-    def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 256, compress: bool = True):
+    def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 256, compress: bool = True,
+                 schedule: bool = True):
         dev = red.csr.device
         self.red = red
         self.n = red.n_free
@@ -205,8 +208,13 @@ class SolverMatrix:
             if ok.value:
                 self.col16, self.col_base = col16, cbase
                 colp = ptr(col16)
+        self.slice_order = None
+        span = C.c_int(0)
+        if schedule and n_slices > 0:
+            self.slice_order = torch.empty(_lib.lib().mife_sell_order_len(n_slices), dtype=torch.int32, device=dev)
+            call("mife_sell_order", n_slices, ptr(self.slice_ptr), ptr(self.slice_order), C.byref(span), _stream())
         self.desc = MatrixDesc(n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), colp, ptr(self.perm), n_slices,
-                               self.stored, ptr(self.col_base))
+                               self.stored, ptr(self.col_base), ptr(self.slice_order), span.value)
 
     def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
         """Solver-matrix values (this storage) from equations-CSR values (FP64 or FP32)."""

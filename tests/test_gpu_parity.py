@@ -217,6 +217,43 @@ def test_sell_storage_matches_csr_product(compress):
     assert _rel(xs.cpu().numpy(), xc.cpu().numpy()) < 1e-10
 
 
+@pytest.mark.parametrize("dims", [(5, 4, 3), (12, 11, 10)])
+def test_sell_schedule_covers_every_slice_and_keeps_products(dims):
+    """The SELL schedule lists every slice once, by decreasing length inside each of
+    its 8 contiguous ranges; products over the schedule equal the plain slice order
+    bit for bit (each row's sum is formed by one lane in the same order)."""
+    fem, _, _ = _fem()
+    pb, pm, pd, ob, om, od, kappa, fixed = _poisson_system(dims, 2, seed=9)
+    fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, kappa, fixed, np.zeros(pd.num_dofs))
+    Sp = fem.SolverMatrix(red, "sell", schedule=False)
+    Ss = fem.SolverMatrix(red, "sell")
+    ns = Ss.desc.n_slices
+    span = Ss.desc.order_span
+    rng = -(-ns // 8)                      # slices per range, ceil(ns / 8)
+    assert span == 4 * -(-rng // 4)
+    order = Ss.slice_order.cpu().numpy().reshape(8, span)
+    lens = np.diff(Ss.slice_ptr.cpu().numpy())
+    seen = []
+    for k in range(8):
+        ids = order[k][order[k] >= 0]
+        assert np.all(order[k][len(ids):] == -1)
+        assert np.array_equal(np.sort(ids), np.arange(min(k * rng, ns), min((k + 1) * rng, ns)))
+        L = lens[ids]
+        assert np.all(L[:-1] >= L[1:])
+        seen.append(ids)
+    assert np.array_equal(np.sort(np.concatenate(seen)), np.arange(ns))
+    for prec, dt in ((fem.F64, torch.float64), (fem.F32, torch.float32)):
+        vals = Ss.gather(v64, prec)
+        xv = torch.rand(red.n_free, dtype=dt, device="cuda")
+        yp = Sp.spmv(vals, xv, torch.empty_like(xv), prec)
+        ys = Ss.spmv(vals, xv, torch.empty_like(xv), prec)
+        assert torch.equal(yp, ys)
+    ws = fem.KrylovWorkspace(red.n_free, "cuda")
+    xs = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    info = fem.cg(Ss, Ss.gather(v64), b, xs, 1e-12, 10000, ws)
+    assert info.relres <= 1e-12
+
+
 def test_unit_cube_laplace_u_equals_x_and_reactions():
     fem, _, _ = _fem()
     pb, pm, pd, ob, om, od = _build((6, 5, 4), 1)
