@@ -143,6 +143,95 @@ __global__ __launch_bounds__(64 * WPB) void scalar_assemble_kernel(
   }
 }
 
+// ------------------------------------------------ triquadratic, on MFMA
+// K = B^T diag(W) B with B[(g,i)][a] = dphi_a/dx_i at Gauss point g (81 x 27,
+// padded to 84 x 32) runs as 16x16x4 MFMA tiles: tile (ti, tj) of K takes 21
+// steps over the padded (g,i) axis, A operand = B^T rows, B operand = W B.  K is
+// symmetric, so tiles (0,0), (0,1), (1,1) are formed and (1,0) is written as
+// the transpose of (0,1).  At step t lane l forms the two gradients
+// B[4t + l/16][16c + l%16], c = 0, 1, from the reference gradients (LDS) and
+// dxi/dx: each is its A-operand entry and, times W, its B-operand entry.
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator register i of lane l (column l % 16)
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * i + (lane >> 4); }
+};
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int i) { return 4 * (lane >> 4) + i; }
+};
+
+template <typename T, int WPB>
+__global__ __launch_bounds__(64 * WPB) void q2_assemble_mfma_kernel(
+    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
+    const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
+    const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
+    const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
+    bool atomic) {
+  constexpr int NN = 27, NG = 27, KD = 81, STEPS = 21;
+  using M = Mfma16<T>;
+  __shared__ double sdN[NG * NN * 3];
+  __shared__ GeomLDS<3, NN, NG> geo[WPB];
+  for (int t = threadIdx.x; t < NG * NN * 3; t += blockDim.x) sdN[t] = dN[t];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = blockIdx.x * WPB + wv;
+  const bool active = idx < elem_count;
+  long long e = 0;
+  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
+  GeomLDS<3, NN, NG>& G = geo[wv];
+  element_geometry<3, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
+  if (!active) return;
+  const double ke = kappa ? kappa[e] : 1.0;
+  const int kq = lane >> 4, col = lane & 15;
+  // lane's two operand positions: a = col and a = 16 + col (the latter is padding for col >= 11)
+  auto grad = [&](int k, int a) -> T {
+    if (k >= KD || a >= NN) return (T)0;
+    const int g = k / 3, i = k - 3 * (k / 3);
+    const double* d = &sdN[(g * NN + a) * 3];
+    return (T)(d[0] * G.ji[g][0][i] + d[1] * G.ji[g][1][i] + d[2] * G.ji[g][2][i]);
+  };
+  typename M::v4 k00 = {0, 0, 0, 0}, k01 = {0, 0, 0, 0}, k11 = {0, 0, 0, 0};
+#pragma unroll 3
+  for (int t = 0; t < STEPS; ++t) {
+    const int k = 4 * t + kq;
+    const T wk = k < KD ? (T)(G.rwg[k / 3] * ke) : (T)0;
+    const T a0 = grad(k, col), a1 = grad(k, 16 + col);
+    k00 = M::mac(a0, a0 * wk, k00);
+    k01 = M::mac(a0, a1 * wk, k01);
+    k11 = M::mac(a1, a1 * wk, k11);
+  }
+  const int32_t* pe = pos + e * (NN * NN);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = M::row(lane, i);
+    // tile (0,0): m = r, n = col
+    add_value(&vals[pe[r * NN + col]], k00[i], atomic);
+    // tile (0,1) and its transpose: m = r, n = 16 + col
+    if (16 + col < NN) {
+      add_value(&vals[pe[r * NN + 16 + col]], k01[i], atomic);
+      add_value(&vals[pe[(16 + col) * NN + r]], k01[i], atomic);
+    }
+    // tile (1,1): m = 16 + r, n = 16 + col
+    if (16 + r < NN && 16 + col < NN) add_value(&vals[pe[(16 + r) * NN + 16 + col]], k11[i], atomic);
+  }
+  if (source && svec && lane < NN) {
+    double s = 0.0;
+    for (int g = 0; g < NG; ++g) s += N[g * NN + lane] * G.rwg[g];
+    s *= source[e];
+    add_value(&svec[edofs[e * NN + lane]], s, atomic);
+  }
+}
+
 // ------------------------------------------------------- linear elasticity
 // Iron's per-block coefficient use (linear_elasticity_routines.F90:1255-1286):
 // diagonal block c uses JRWG*diagC[c][d] on dphi_d*dphi_d; off-diagonal block
@@ -231,16 +320,23 @@ static int launch_scalar(const mife_element_desc* d, int n_elem, const double* c
                          const int32_t* order, const int32_t* cptr, int ncol, T* vals, double* svec,
                          hipStream_t s) {
   constexpr int WPB = (sizeof(T) * NG * DIM * NN > 12000) ? 1 : 4;
+  constexpr bool kMfma = DIM == 3 && NN == 27 && NG == 27;
+  constexpr int WQ = 4;  // waves (elements) per block of the MFMA kernel
   const int tk = prof::begin(s);
+  auto launch = [&](const int32_t* ord, int begin, int count, bool atomic) {
+    if constexpr (kMfma)
+      q2_assemble_mfma_kernel<T, WQ><<<div_up(count, WQ), 64 * WQ, 0, s>>>(
+          coords, enodes, edofs, kappa, source, pos, ord, begin, count, d->dN, d->N, d->w, vals, svec, atomic);
+    else
+      scalar_assemble_kernel<DIM, NN, NG, T, WPB><<<div_up(count, WPB), 64 * WPB, 0, s>>>(
+          coords, enodes, edofs, kappa, source, pos, ord, begin, count, d->dN, d->N, d->w, vals, svec, atomic);
+  };
   if (!order) {
-    scalar_assemble_kernel<DIM, NN, NG, T, WPB><<<div_up(n_elem, WPB), 64 * WPB, 0, s>>>(
-        coords, enodes, edofs, kappa, source, pos, nullptr, 0, n_elem, d->dN, d->N, d->w, vals, svec, true);
+    launch(nullptr, 0, n_elem, true);
   } else {
     for (int c = 0; c < ncol; ++c) {
       const int cnt = cptr[c + 1] - cptr[c];
-      if (cnt <= 0) continue;
-      scalar_assemble_kernel<DIM, NN, NG, T, WPB><<<div_up(cnt, WPB), 64 * WPB, 0, s>>>(
-          coords, enodes, edofs, kappa, source, pos, order, cptr[c], cnt, d->dN, d->N, d->w, vals, svec, false);
+      if (cnt > 0) launch(order, cptr[c], cnt, false);
     }
   }
   prof::end(tk, s, sizeof(T) == 8 ? prof::kAssembleScalar : prof::kAssembleScalar32);
