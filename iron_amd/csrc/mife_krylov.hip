@@ -175,7 +175,11 @@ __global__ void fin_alpha_kernel(CgState* st, int nb) {
   }
 }
 
-template <typename T>
+// x += alpha p, r -= alpha q, z = D^-1 r and the r.z, r.r partials.  Each
+// thread takes R = 16 / sizeof(T) consecutive rows through 16-byte vector
+// accesses (R = 1 when a vector is not 16-byte aligned); rows past the last
+// full group are taken one per thread.
+template <typename T, int R>
 __global__ __launch_bounds__(256) void cg_update_kernel(int n, T* __restrict__ x, T* __restrict__ r,
                                                         T* __restrict__ z, const T* __restrict__ p,
                                                         const T* __restrict__ q, const T* __restrict__ dinv,
@@ -183,14 +187,40 @@ __global__ __launch_bounds__(256) void cg_update_kernel(int n, T* __restrict__ x
   if (st->done || st->pause) return;
   const T alpha = (T)st->alpha;
   double m[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    x[i] += alpha * p[i];
-    const T ri = r[i] - alpha * q[i];
-    const T zi = dinv[i] * ri;
-    r[i] = ri;
-    z[i] = zi;
+  auto row = [&](T& xi, T& ri, T& zi, T pi, T qi, T di) {
+    xi += alpha * pi;
+    ri = ri - alpha * qi;
+    zi = di * ri;
     m[0] += (double)ri * (double)zi;
     m[1] += (double)ri * (double)ri;
+  };
+  const int nv = n / R;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+  if constexpr (R > 1) {
+    typedef T VT __attribute__((ext_vector_type(R)));
+    for (int v = tid; v < nv; v += nth) {
+      VT xv = reinterpret_cast<VT*>(x)[v], rv = reinterpret_cast<VT*>(r)[v], zv;
+      const VT pv = reinterpret_cast<const VT*>(p)[v], qv = reinterpret_cast<const VT*>(q)[v];
+      const VT dv = reinterpret_cast<const VT*>(dinv)[v];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        T xi = xv[j], ri = rv[j], zi;
+        row(xi, ri, zi, pv[j], qv[j], dv[j]);
+        xv[j] = xi;
+        rv[j] = ri;
+        zv[j] = zi;
+      }
+      reinterpret_cast<VT*>(x)[v] = xv;
+      reinterpret_cast<VT*>(r)[v] = rv;
+      reinterpret_cast<VT*>(z)[v] = zv;
+    }
+  }
+  for (int i = (R > 1 ? nv * R : 0) + tid; i < n; i += nth) {
+    T xi = x[i], ri = r[i], zi;
+    row(xi, ri, zi, p[i], q[i], dinv[i]);
+    x[i] = xi;
+    r[i] = ri;
+    z[i] = zi;
   }
   block_partials<2>(m, st->part_b);
 }
@@ -574,6 +604,8 @@ struct Work {
 
 static int vec_grid(int n) { return std::max(1, std::min(div_up(n, 256), kMaxGrid)); }
 
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 struct Mat {
   MatArgs a;
   const int32_t* perm;
@@ -610,13 +642,18 @@ template <typename T>
 static void launch_cg_iter(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
                            hipStream_t s, int* tk) {
   const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
-  const int vg = vec_grid(M.a.n);
+  constexpr int R = 16 / sizeof(T);
+  const bool vec = aligned16(x) && aligned16(r) && aligned16(z) && aligned16(p) && aligned16(q) && aligned16(dinv);
+  const int vg = vec ? vec_grid(div_up(M.a.n, R)) : vec_grid(M.a.n);
   tk[0] = prof::begin(s);
   MIFE_DISPATCH_FMT(M.a, M.L, (cg_spmv_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, p, q, st)));
   prof::end(tk[0], s, sizeof(T) == 8 ? prof::kCgSpmv : prof::kCgSpmv32);
   fin_alpha_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
   tk[1] = prof::begin(s);
-  cg_update_kernel<T><<<vg, 256, 0, s>>>(M.a.n, x, r, z, p, q, dinv, st);
+  if (vec)
+    cg_update_kernel<T, R><<<vg, 256, 0, s>>>(M.a.n, x, r, z, p, q, dinv, st);
+  else
+    cg_update_kernel<T, 1><<<vg, 256, 0, s>>>(M.a.n, x, r, z, p, q, dinv, st);
   prof::end(tk[1], s, sizeof(T) == 8 ? prof::kCgUpdate : prof::kCgUpdate32);
   fin_beta_kernel<<<1, kFinThreads, 0, s>>>(st, vg);
 }
