@@ -11,7 +11,7 @@ every boundary face.  One step is one full pass of the path:
 The plain FP64 drop-in (FP64 assembly + FP64 Jacobi CG to 1e-8) is timed in the
 same run, and the C restatement of iron's CPU algorithm on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16|elastic]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16] [--legs main,mr,serve]
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="diffusion48", choices=["diffusion48", "laplace16", "elastic"])
+    ap.add_argument("--workload", default="diffusion48", choices=["diffusion48", "laplace16"])
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -48,7 +48,31 @@ def parse():
                          "writes 268,435,456 bytes each: the reference kernel for HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=128,
                     help="requests of the served-stream leg (the full stream has 1024); 0 skips it")
+    ap.add_argument("--legs", default="main,mr,serve",
+                    help="comma-separated legs to run: main (the workload above), mr, serve")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "cg_spmv_diffusion48_q2.md"
+WORKLOAD_TEXT = {
+    "diffusion48": "diffusion48: 48^3 Q2 hex, 3x3x3 Gauss, kappa~lognormal(0,1) seed {seed}, f=1, u=0 on all "
+                   "faces, FP32 inner CG + FP64 refinement to 1e-8",
+    "laplace16": "laplace16: 16^3 Q1 hex, 2x2x2 Gauss, kappa=1, u=0 at x=0 and u=1 at x=1, FP32 inner CG + "
+                 "FP64 refinement to 1e-10",
+}
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused FP32 SpMV on diffusion48, from the
+    calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE summary kept with the profiles
+    (line 'traffic_bytes_per_launch: N'), or None when that summary is absent."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def build_problem(args):
@@ -59,7 +83,7 @@ def build_problem(args):
     elif args.workload == "laplace16":
         ne, interp, extent, rtol = (16, 16, 16), basis.LINEAR_LAGRANGE, (1.0, 1.0, 1.0), 1e-10
     else:
-        raise SystemExit("elastic workload is run by tests/bench of the elasticity path")
+        raise SystemExit(f"unknown workload {args.workload}")
     b = basis.basis(3, interp)
     m = mesh.RegularMesh(ne, [b], extent)
     dm = mesh.DofMap(m, [0])
@@ -370,8 +394,8 @@ def serve_leg(n, seed=2024):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each box here has one GPU; "
-              "requests shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(served requests would shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, str(ROOT))
     import torch
@@ -384,6 +408,20 @@ def main():
             torch.mul(a, 1.0, out=bcopy)
         torch.cuda.synchronize()
         del a, bcopy
+    legs = set(args.legs.split(","))
+    out = main_leg(args) if "main" in legs else {"legs": sorted(legs)}
+    torch.cuda.empty_cache()
+    if "mr" in legs and not args.no_mr:
+        out["mooney_rivlin"] = mr_leg(2, 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 10.0))
+        torch.cuda.empty_cache()
+    if "serve" in legs and args.serve_requests > 0:
+        out["served_stream"] = serve_leg(args.serve_requests)
+    print(json.dumps(out))
+
+
+def main_leg(args):
+    import torch
+    from iron_amd import _lib
     P = build_problem(args)
     red, csr = P["red"], P["csr"]
     E = P["m"].num_elements
@@ -411,6 +449,8 @@ def main():
     bytes_spmv32 = 8.0 * nnz_f + 24.0 * n_f + 4.0
     avg_cg_ms = ms_cg / max(n_cg, 1)
     achieved = bytes_spmv32 / (avg_cg_ms * 1e-3) / 1e9 if n_cg else 0.0
+    # calibrated HBM bytes per launch (rocprofv3 counters), for this workload and storage only
+    traffic = measured_traffic() if (args.workload == "diffusion48" and args.format == "sell") else None
     n_a32, ms_a32 = prof.get("assemble_scalar_f32", (0, 0.0))
     n_a64, ms_a64 = prof.get("assemble_scalar", (0, 0.0))
     asm32_ms = ms_a32 / max(n_a32, 1)
@@ -423,8 +463,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
         "data": "synthetic (generated mesh, lognormal conductivity from a fixed seed)",
-        "config": {"workload": f"{args.workload}: 48^3 Q2 hex, 3x3x3 Gauss, kappa~lognormal(0,1) seed {args.seed}, "
-                               "f=1, u=0 on all faces, FP32 inner CG + FP64 refinement to 1e-8",
+        "config": {"workload": WORKLOAD_TEXT[args.workload].format(seed=args.seed),
                    "elements": E, "dofs": csr.n, "free_dofs": n_f, "solver_nnz": nnz_f, "eq_nnz": csr.nnz},
         "solves_per_s_mixed": args.steps / wall,
         "solves_per_s_fp64": len(lat64) / wall64,
@@ -438,7 +477,7 @@ def main():
         "relres_mixed": infos[-1].relres, "relres_fp64": infos64[-1].relres,
         "roofline": {"kernel": "cg_spmv<float> (fused SpMV + p/q update + p.q)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_spmv32, "avg_launch_ms": avg_cg_ms, "launches": n_cg},
         "kernel_ms": {k: v for k, v in prof.items() if v[0]},
         "kernel_ms_fp64": {k: v for k, v in prof64.items() if v[0]},
@@ -447,13 +486,7 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)
     del P
-    torch.cuda.empty_cache()
-    if not args.no_mr:
-        out["mooney_rivlin"] = mr_leg(2, 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 10.0))
-        torch.cuda.empty_cache()
-    if args.serve_requests > 0:
-        out["served_stream"] = serve_leg(args.serve_requests)
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":
