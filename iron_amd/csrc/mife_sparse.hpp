@@ -38,14 +38,19 @@ struct MatArgs {
   const int32_t* cbase;  // kFmtSell16: per-slice base column
   const int32_t* order;  // SELL schedule (8 ranges of `span` slices, -1 padded) or null
   int span;
+  int split;  // scheduled SELL: wavefronts per slice (1, 2 or 4)
 };
+
+// Wavefronts per slice of a scheduled SELL product: enough wavefronts to fill
+// the chip on small matrices (long rows, few slices), one per slice on large.
+inline int sell_split(int n_slices) { return n_slices < 4096 ? 4 : (n_slices < 65536 ? 2 : 1); }
 
 inline MatArgs mat_args(const mife_matrix* A) {
   const bool sell = A->format == MIFE_FMT_SELL;
   const bool c16 = sell && A->col_base != nullptr;
   const bool sched = sell && A->slice_order != nullptr && A->order_span > 0;
   return MatArgs{A->n, c16 ? kFmtSell16 : A->format, A->ptr, A->col, A->n_slices, c16 ? A->col_base : nullptr,
-                 sched ? A->slice_order : nullptr, sched ? A->order_span : 0};
+                 sched ? A->slice_order : nullptr, sched ? A->order_span : 0, sched ? sell_split(A->n_slices) : 1};
 }
 
 template <typename T>
@@ -157,6 +162,45 @@ __device__ __forceinline__ T sell16_row(const int32_t* __restrict__ sp, const ui
   return (a0 + a1) + (a2 + a3);
 }
 
+// Part h of W of a SELL row sum (int32 or 16-bit columns): the wavefront takes
+// chunks h, h + W, h + 2W, ... of eight entries; with W = 1 the order of the
+// additions is that of sell_row / sell16_row.
+template <typename T, bool C16>
+__device__ __forceinline__ T sell_row_part(const MatArgs& A, const T* __restrict__ vals, const T* __restrict__ x,
+                                           int s, int lane, int h, int W) {
+  constexpr int D = 8;
+  const int base = A.ptr[s];
+  const int len = (A.ptr[s + 1] - base) / kSlice;
+  const T* xs = C16 ? x + A.cbase[s] : x;
+  const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col) + base + lane;
+  const int32_t* c32 = A.col + base + lane;
+  const T* v = vals + base + lane;
+  auto col = [&](int j) -> int {
+    if constexpr (C16) return (int)stream_load(c16 + j * kSlice);
+    else return stream_load(c32 + j * kSlice);
+  };
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int j = h * D;
+  for (; j + D <= len; j += W * D) {
+    int ci[D];
+    T vi[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) ci[u] = col(j + u);
+#pragma unroll
+    for (int u = 0; u < D; ++u) vi[u] = stream_load(v + (j + u) * kSlice);
+    a0 += vi[0] * xs[ci[0]];
+    a1 += vi[1] * xs[ci[1]];
+    a2 += vi[2] * xs[ci[2]];
+    a3 += vi[3] * xs[ci[3]];
+    a0 += vi[4] * xs[ci[4]];
+    a1 += vi[5] * xs[ci[5]];
+    a2 += vi[6] * xs[ci[6]];
+    a3 += vi[7] * xs[ci[7]];
+  }
+  for (int k = j; k < len && k < j + D; ++k) a0 += stream_load(v + k * kSlice) * xs[col(k)];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // Calls f(row, (A x)_row) once per row (by one lane), over the whole grid.
 template <typename T, int FMT, int L, typename F>
 __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restrict__ vals, const T* __restrict__ x,
@@ -177,11 +221,34 @@ __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restri
     if (A.order) {
       // Scheduled: the dispatcher places block b on XCD b % 8, which walks its own
       // range of the order (longest slices first); the grid is a multiple of 8.
+      // W wavefronts share a slice (W = 1: one slice per wavefront); their partial
+      // row sums meet in LDS and the first of them adds them in a fixed order.
+      // Every wavefront of a block makes the same number of passes, so the
+      // barriers are uniform.
+      __shared__ T part[16][kSlice];
+      const int W = min(A.split, wpb), spb = wpb / W;
+      const int wave = threadIdx.x >> 6, g = wave / W, h = wave - g * W;
       const int xcd = blockIdx.x % kXcds, bpx = gridDim.x / kXcds;
       const int32_t* ord = A.order + (size_t)xcd * A.span;
-      for (int k = (blockIdx.x / kXcds) * wpb + (threadIdx.x >> 6); k < A.span; k += bpx * wpb) {
-        const int s = ord[k];
-        if (s >= 0) one_slice(s);
+      const int passes = (A.span + bpx * spb - 1) / (bpx * spb);
+      for (int ps = 0; ps < passes; ++ps) {
+        const int k = (ps * bpx + blockIdx.x / kXcds) * spb + g;
+        const int s = k < A.span ? ord[k] : -1;
+        if (W == 1) {
+          if (s >= 0) one_slice(s);
+          continue;
+        }
+        T ax = (T)0;
+        if (s >= 0 && x) ax = sell_row_part<T, FMT == kFmtSell16>(A, vals, x, s, lane, h, W);
+        part[wave][lane] = ax;
+        __syncthreads();
+        if (h == 0 && s >= 0) {
+          T t = part[wave][lane];
+          for (int q = 1; q < W; ++q) t += part[wave + q][lane];
+          const int row = s * kSlice + lane;
+          if (row < A.n) f(row, t);
+        }
+        __syncthreads();
       }
     } else {
       for (int s = blockIdx.x * wpb + (threadIdx.x >> 6); s < A.n_slices; s += gridDim.x * wpb) one_slice(s);
@@ -207,10 +274,10 @@ inline int spmv_lanes(long long nnz, int n) {
 }
 
 // Grid of a row loop launched with 256 threads per block.  A scheduled SELL
-// matrix gets one slice per wavefront (blocks of 4), a multiple of 8 blocks.
+// matrix gets 4 / split slices per block, a multiple of 8 blocks.
 inline int row_grid(const MatArgs& A, int L) {
   if (A.fmt != MIFE_FMT_CSR && A.order)
-    return kXcds * std::max(1, std::min(div_up(A.span, 4), kMaxSellGrid / kXcds));
+    return kXcds * std::max(1, std::min(div_up(A.span, 4 / A.split), kMaxSellGrid / kXcds));
   const int units = A.fmt != MIFE_FMT_CSR ? div_up(A.n_slices, 4) : div_up(A.n, 256 / L);
   return std::max(1, std::min(units, kMaxGrid));
 }

@@ -140,9 +140,9 @@ class MRNewton:
         self.sys = sys
         dev = sys.device
         self.red = fem.DirichletReduction(sys.csr, fixed_mask)
-        # ~200 entries per row and few rows: CSR with a wavefront per row group keeps
-        # every CU busy where 64-row slices would not
-        self.S = fem.SolverMatrix(self.red, fmt="csr")
+        # ~200 entries per row and few rows: scheduled SELL shares each 64-row slice
+        # between several wavefronts, which keeps every CU busy
+        self.S = fem.SolverMatrix(self.red)
         self.restart = int(restart)
         self.ws = fem.KrylovWorkspace(max(self.red.n_free, 1), dev, restart=self.restart)
         self.free = self.red.free_idx.long()

@@ -220,8 +220,8 @@ def test_sell_storage_matches_csr_product(compress):
 @pytest.mark.parametrize("dims", [(5, 4, 3), (12, 11, 10)])
 def test_sell_schedule_covers_every_slice_and_keeps_products(dims):
     """The SELL schedule lists every slice once, by decreasing length inside each of
-    its 8 contiguous ranges; products over the schedule equal the plain slice order
-    bit for bit (each row's sum is formed by one lane in the same order)."""
+    its 8 contiguous ranges; products over the schedule (where several wavefronts
+    share a slice on small matrices) agree with the plain slice order to rounding."""
     fem, _, _ = _fem()
     pb, pm, pd, ob, om, od, kappa, fixed = _poisson_system(dims, 2, seed=9)
     fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, kappa, fixed, np.zeros(pd.num_dofs))
@@ -247,7 +247,8 @@ def test_sell_schedule_covers_every_slice_and_keeps_products(dims):
         xv = torch.rand(red.n_free, dtype=dt, device="cuda")
         yp = Sp.spmv(vals, xv, torch.empty_like(xv), prec)
         ys = Ss.spmv(vals, xv, torch.empty_like(xv), prec)
-        assert torch.equal(yp, ys)
+        tol = 1e-14 if prec == fem.F64 else 1e-6
+        assert float((yp - ys).abs().max() / yp.abs().max()) <= tol
     ws = fem.KrylovWorkspace(red.n_free, "cuda")
     xs = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
     info = fem.cg(Ss, Ss.gather(v64), b, xs, 1e-12, 10000, ws)
