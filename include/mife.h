@@ -345,20 +345,37 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b, doubl
 
 /* -------------------------------------------- grouped solves (serving) */
 
-/* One independent system of a group: its CSR (local 0-based columns) lives in
- * shared pools at rp_off (n+1 row pointers), nz_off (columns and values), and
- * its vectors at row_off of the vector pools.  slot (0 <= slot < max_segments)
- * names the segment's persistent CG state. */
+/* One independent system of a group, stored in shared pools (mife_pools) with
+ * local 0-based columns.  CSR: n+1 row pointers at rp_off, columns and values at
+ * nz_off.  SELL-64 (as mife_matrix): n_slices+1 slice pointers at rp_off,
+ * values at nz_off, columns at nz_off of the int32 pool or, with cb_off >= 0,
+ * 16-bit offsets at nz_off of the 16-bit pool plus n_slices base columns at
+ * cb_off; the system's vectors are then in slice numbering.  Vectors live at
+ * row_off of the vector pools.  slot (0 <= slot < max_segments) names the
+ * segment's persistent CG state. */
 typedef struct mife_segment {
   int32_t n;
   int32_t maxit;
   int32_t slot;
-  int32_t pad;
+  int32_t format;   /* MIFE_FMT_CSR or MIFE_FMT_SELL */
   int64_t row_off;
-  int64_t rp_off;
+  int64_t rp_off;   /* CSR: row_ptr[n+1]; SELL: slice_ptr[n_slices+1] (entry offsets, multiples of 64) */
   int64_t nz_off;
   double rtol;
+  int32_t n_slices; /* SELL: ceil(n / 64) */
+  int32_t pad;
+  int64_t cb_off;   /* SELL: offset of the per-slice base columns (16-bit columns), or -1 (int32 columns) */
 } mife_segment;
+
+/* Shared storage of the grouped segments: row / slice pointers, int32 columns,
+ * 16-bit slice-relative columns with their per-slice base columns, values. */
+typedef struct mife_pools {
+  const int32_t* rp;
+  const int32_t* col;
+  const uint16_t* col16;
+  const int32_t* cbase;
+  const double* val;
+} mife_pools;
 
 /* Grouped Jacobi CG: every launch serves all segments of a table; each segment
  * has its own scalars, its own fixed-order reductions and its own stopping test
@@ -384,16 +401,13 @@ size_t mife_grouped_work_bytes(int max_segments, int max_blocks);
 int mife_grouped_blocks(int n_rows);
 int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segments, int max_blocks,
                        int* n_blocks_out, void* table_host, void* work, void* stream);
-int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_blocks,
-                         const int32_t* rp_pool, const int32_t* col_pool, const double* val_pool,
+int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
                          const double* b, double* x, double* vec5, int64_t pool_rows, void* work,
                          void* state, void* stream);
-int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_blocks,
-                            const int32_t* rp_pool, const int32_t* col_pool, const double* val_pool,
+int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
                             double* x, double* vec5, int64_t pool_rows, int iterations, void* work,
                             void* state, void* stream);
-int mife_grouped_cg_residual(int n_seg, int n_blocks, int max_segments, int max_blocks,
-                             const int32_t* rp_pool, const int32_t* col_pool, const double* val_pool,
+int mife_grouped_cg_residual(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
                              const double* b, const double* x, void* work, void* state, void* stream);
 int mife_grouped_cg_status(int max_segments, const void* state, int* iters, int* done, double* relres,
                            void* stream);

@@ -2,11 +2,12 @@
 // requests of a serving batch -- iterate together, four launches per
 // iteration for the whole group.
 //
-// Each system ("segment") keeps its CSR rows, columns and values in shared
-// pools at its own offsets, and its vectors in shared vector pools; column
-// indices are local to the segment.  A block table assigns every block of a
-// launch to one segment and a contiguous row range of it, so each block's
-// partial sums belong to exactly one segment; one-block-per-segment finishers
+// Each system ("segment") keeps its matrix in shared pools at its own offsets
+// -- CSR, or SELL-64 with int32 or 16-bit slice-relative columns -- and its
+// vectors in shared vector pools; column indices are local to the segment.  A
+// block table assigns every block of a launch to one segment and a contiguous
+// row range (CSR) or slice range (SELL, one slice per wavefront) of it, so each
+// block's partial sums belong to exactly one segment; one-block-per-segment finishers
 // reduce each segment's partials in a fixed order (deterministic, and
 // independent of which other segments share the launch) and update that
 // segment's CG scalars.  Converged segments stop at once: their blocks return
@@ -14,6 +15,7 @@
 // chunk of iterations can be captured in a HIP graph and replayed.
 #include "mife_common.hpp"
 #include "mife_profile.hpp"
+#include "mife_sparse.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -23,8 +25,9 @@ namespace mife {
 
 namespace grouped {
 
-constexpr int kL = 8;                  // lanes per row
+constexpr int kL = 8;                  // CSR: lanes per row
 constexpr int kRowsPerBlock = 256 / kL;
+constexpr int kSlicesPerBlock = 4;     // SELL: one slice per wavefront
 
 struct SegState {
   double rz, pq, alpha, beta, rr, bb, tol2, rr_true;
@@ -33,8 +36,8 @@ struct SegState {
 
 struct Block {
   int32_t seg;
-  int32_t row0;  // first row (local to the segment)
-  int32_t rows;
+  int32_t row0;  // first row (CSR) or first slice (SELL), local to the segment
+  int32_t rows;  // rows (CSR) or slices (SELL)
   int32_t part;  // index of this block's partial slot
 };
 
@@ -43,6 +46,8 @@ struct View {
   const Block* blocks;
   const int32_t* rp;
   const int32_t* col;
+  const uint16_t* col16;
+  const int32_t* cbase;
   const double* val;
   SegState* st;
   double* parts;  // 3 values per partial slot: parts[v * cap + slot]
@@ -62,6 +67,48 @@ __device__ __forceinline__ double row_dot(const View& V, const mife_segment& S, 
 #pragma unroll
   for (int off = kL / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, kL);
   return a;
+}
+
+// SELL: (A x) of lane `lane`'s row of slice s of the segment (complete sum).
+__device__ __forceinline__ double slice_dot(const View& V, const mife_segment& S, const double* __restrict__ x,
+                                            int s, int lane) {
+  const int32_t* sp = V.rp + S.rp_off;
+  const double* val = V.val + S.nz_off;
+  const double* xs = x + S.row_off;
+  if (S.cb_off >= 0) return sell16_row<double>(sp, V.col16 + S.nz_off, V.cbase + S.cb_off, val, xs, s, lane);
+  return sell_row<double>(sp, V.col + S.nz_off, val, xs, s, lane);
+}
+
+// Calls f(row, (A x)_row) for every row of block B (by one lane per row).
+template <typename F>
+__device__ __forceinline__ void block_rows(const View& V, const Block& B, const mife_segment& S,
+                                           const double* __restrict__ x, F&& f) {
+  if (S.format == MIFE_FMT_SELL) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave < B.rows) {
+      const int s = B.row0 + wave;
+      const double ax = slice_dot(V, S, x, s, lane);
+      const int row = s * kSlice + lane;
+      if (row < S.n) f(row, ax);
+    }
+    return;
+  }
+  const int lr = threadIdx.x / kL, sub = threadIdx.x % kL;
+  for (int row = B.row0 + lr; row < B.row0 + B.rows; row += kRowsPerBlock) {
+    const double ax = row_dot(V, S, x, row, sub);
+    if (sub == 0) f(row, ax);
+  }
+}
+
+// First row and row count of block B (vector kernels: one row per thread).
+__device__ __forceinline__ void block_row_range(const Block& B, const mife_segment& S, int& r0, int& r1) {
+  if (S.format == MIFE_FMT_SELL) {
+    r0 = B.row0 * kSlice;
+    r1 = min(S.n, (B.row0 + B.rows) * kSlice);
+  } else {
+    r0 = B.row0;
+    r1 = B.row0 + B.rows;
+  }
 }
 
 template <int NV>
@@ -86,15 +133,29 @@ using namespace grouped;
 __global__ __launch_bounds__(256) void gcg_diag_kernel(View V, double* __restrict__ dinv) {
   const Block B = V.blocks[blockIdx.x];
   const mife_segment S = V.segs[B.seg];
-  const int t = threadIdx.x;
-  if (t >= B.rows) return;
-  const int row = B.row0 + t;
-  const int32_t* rp = V.rp + S.rp_off;
-  const int32_t* col = V.col + S.nz_off;
-  const double* val = V.val + S.nz_off;
+  int r0, r1;
+  block_row_range(B, S, r0, r1);
+  const int row = r0 + (int)threadIdx.x;
+  if (row >= r1) return;
   double d = 0.0;
-  for (int k = rp[row]; k < rp[row + 1]; ++k)
-    if (col[k] == row) d = val[k];
+  if (S.format == MIFE_FMT_SELL) {
+    const int32_t* sp = V.rp + S.rp_off;
+    const int s = row / kSlice, lane = row - s * kSlice;
+    const double* val = V.val + S.nz_off;
+    for (int e = sp[s] + lane; e < sp[s + 1]; e += kSlice) {
+      const int c = S.cb_off >= 0 ? V.cbase[S.cb_off + s] + (int)V.col16[S.nz_off + e] : V.col[S.nz_off + e];
+      if (c == row && val[e] != 0.0) {
+        d = val[e];
+        break;
+      }
+    }
+  } else {
+    const int32_t* rp = V.rp + S.rp_off;
+    const int32_t* col = V.col + S.nz_off;
+    const double* val = V.val + S.nz_off;
+    for (int k = rp[row]; k < rp[row + 1]; ++k)
+      if (col[k] == row) d = val[k];
+  }
   dinv[S.row_off + row] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
@@ -105,23 +166,19 @@ __global__ __launch_bounds__(256) void gcg_init_kernel(View V, const double* __r
                                                        double* __restrict__ p, double* __restrict__ q) {
   const Block B = V.blocks[blockIdx.x];
   const mife_segment S = V.segs[B.seg];
-  const int lr = threadIdx.x / kL, sub = threadIdx.x % kL;
   double m[3] = {0.0, 0.0, 0.0};
-  for (int row = B.row0 + lr; row < B.row0 + B.rows; row += kRowsPerBlock) {
-    const double ax = row_dot(V, S, x, row, sub);
-    if (sub == 0) {
-      const long long g = S.row_off + row;
-      const double ri = b[g] - ax;
-      const double zi = dinv[g] * ri;
-      r[g] = ri;
-      z[g] = zi;
-      p[g] = 0.0;
-      q[g] = 0.0;
-      m[0] += ri * zi;
-      m[1] += ri * ri;
-      m[2] += b[g] * b[g];
-    }
-  }
+  block_rows(V, B, S, x, [&](int row, double ax) {
+    const long long g = S.row_off + row;
+    const double ri = b[g] - ax;
+    const double zi = dinv[g] * ri;
+    r[g] = ri;
+    z[g] = zi;
+    p[g] = 0.0;
+    q[g] = 0.0;
+    m[0] += ri * zi;
+    m[1] += ri * ri;
+    m[2] += b[g] * b[g];
+  });
   store_partials<3>(V, B, m);
 }
 
@@ -173,19 +230,15 @@ __global__ __launch_bounds__(256) void gcg_spmv_kernel(View V, const double* __r
   if (V.st[B.seg].done) return;
   const mife_segment S = V.segs[B.seg];
   const double beta = V.st[B.seg].beta;
-  const int lr = threadIdx.x / kL, sub = threadIdx.x % kL;
   double m[1] = {0.0};
-  for (int row = B.row0 + lr; row < B.row0 + B.rows; row += kRowsPerBlock) {
-    const double az = row_dot(V, S, z, row, sub);
-    if (sub == 0) {
-      const long long g = S.row_off + row;
-      const double pi = z[g] + beta * p[g];
-      const double qi = az + beta * q[g];
-      p[g] = pi;
-      q[g] = qi;
-      m[0] += pi * qi;
-    }
-  }
+  block_rows(V, B, S, z, [&](int row, double az) {
+    const long long g = S.row_off + row;
+    const double pi = z[g] + beta * p[g];
+    const double qi = az + beta * q[g];
+    p[g] = pi;
+    q[g] = qi;
+    m[0] += pi * qi;
+  });
   store_partials<1>(V, B, m);
 }
 
@@ -212,7 +265,9 @@ __global__ __launch_bounds__(256) void gcg_update_kernel(View V, double* __restr
   const mife_segment S = V.segs[B.seg];
   const double alpha = V.st[B.seg].alpha;
   double m[2] = {0.0, 0.0};
-  for (int row = B.row0 + threadIdx.x; row < B.row0 + B.rows; row += 256) {
+  int r0, r1;
+  block_row_range(B, S, r0, r1);
+  for (int row = r0 + threadIdx.x; row < r1; row += 256) {
     const long long g = S.row_off + row;
     x[g] += alpha * p[g];
     const double ri = r[g] - alpha * q[g];
@@ -246,15 +301,11 @@ __global__ __launch_bounds__(256) void gcg_resid_kernel(View V, const double* __
                                                         const double* __restrict__ x) {
   const Block B = V.blocks[blockIdx.x];
   const mife_segment S = V.segs[B.seg];
-  const int lr = threadIdx.x / kL, sub = threadIdx.x % kL;
   double m[1] = {0.0};
-  for (int row = B.row0 + lr; row < B.row0 + B.rows; row += kRowsPerBlock) {
-    const double ax = row_dot(V, S, x, row, sub);
-    if (sub == 0) {
-      const double ri = b[S.row_off + row] - ax;
-      m[0] += ri * ri;
-    }
-  }
+  block_rows(V, B, S, x, [&](int row, double ax) {
+    const double ri = b[S.row_off + row] - ax;
+    m[0] += ri * ri;
+  });
   store_partials<1>(V, B, m);
 }
 
@@ -336,17 +387,21 @@ int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segment
     const mife_segment& S = segs_host[k];
     MIFE_REQUIRE(S.n >= 0, MIFE_ERR_ARG, "segment %d has %d rows", k, S.n);
     MIFE_REQUIRE(S.slot >= 0 && S.slot < max_segments, MIFE_ERR_ARG, "segment slot %d out of range", S.slot);
+    MIFE_REQUIRE(S.format == MIFE_FMT_CSR || (S.format == MIFE_FMT_SELL && S.n_slices == div_up(S.n, kSlice)),
+                 MIFE_ERR_ARG, "segment %d: format %d / n_slices %d", k, S.format, S.n_slices);
     H.segs[S.slot] = S;
     H.seg_parts[k] = nb;
     H.seg_ids[k] = S.slot;
-    const int per = 4 * kRowsPerBlock;
+    const bool sell = S.format == MIFE_FMT_SELL;
+    const int per = sell ? kSlicesPerBlock : 4 * kRowsPerBlock;
+    const int units = sell ? S.n_slices : S.n;
     int r0 = 0;
     do {
       MIFE_REQUIRE(nb < max_blocks, MIFE_ERR_ARG, "block table full (%d blocks)", max_blocks);
-      H.blocks[nb] = Block{S.slot, r0, std::max(0, std::min(per, S.n - r0)), nb};
+      H.blocks[nb] = Block{S.slot, r0, std::max(0, std::min(per, units - r0)), nb};
       ++nb;
       r0 += per;
-    } while (r0 < S.n);
+    } while (r0 < units);
   }
   H.seg_parts[n_seg] = nb;
   hipStream_t st = (hipStream_t)stream;
@@ -362,20 +417,19 @@ int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segment
   return MIFE_OK;
 }
 
-static View make_view(const Layout& D, void* state, const int32_t* rp, const int32_t* col, const double* val,
-                      int cap) {
-  return View{D.segs, D.blocks, rp, col, val, (SegState*)state, D.parts, cap};
+static View make_view(const Layout& D, void* state, const mife_pools* P, int cap) {
+  return View{D.segs, D.blocks, P->rp, P->col, P->col16, P->cbase, P->val, (SegState*)state, D.parts, cap};
 }
 
-int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_blocks, const int32_t* rp_pool,
-                         const int32_t* col_pool, const double* val_pool, const double* b, double* x, double* vec5,
-                         int64_t pool_rows, void* work, void* state, void* stream) {
-  MIFE_REQUIRE(rp_pool && col_pool && val_pool && b && x && vec5 && work && state, MIFE_ERR_POINTER_IS_NULL,
+int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
+                         const double* b, double* x, double* vec5, int64_t pool_rows, void* work, void* state,
+                         void* stream) {
+  MIFE_REQUIRE(pools && pools->rp && pools->val && b && x && vec5 && work && state, MIFE_ERR_POINTER_IS_NULL,
                "null pointer argument");
   if (n_seg == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   Layout D = carve(work, max_segments, max_blocks);
-  View V = make_view(D, state, rp_pool, col_pool, val_pool, max_blocks);
+  View V = make_view(D, state, pools, max_blocks);
   double* dinv = vec5;
   double* r = vec5 + pool_rows;
   double* z = vec5 + 2 * pool_rows;
@@ -388,15 +442,15 @@ int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_bloc
   return MIFE_OK;
 }
 
-int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_blocks, const int32_t* rp_pool,
-                            const int32_t* col_pool, const double* val_pool, double* x, double* vec5,
-                            int64_t pool_rows, int iterations, void* work, void* state, void* stream) {
-  MIFE_REQUIRE(rp_pool && col_pool && val_pool && x && vec5 && work && state, MIFE_ERR_POINTER_IS_NULL,
+int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
+                            double* x, double* vec5, int64_t pool_rows, int iterations, void* work, void* state,
+                            void* stream) {
+  MIFE_REQUIRE(pools && pools->rp && pools->val && x && vec5 && work && state, MIFE_ERR_POINTER_IS_NULL,
                "null pointer argument");
   if (n_seg == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   Layout D = carve(work, max_segments, max_blocks);
-  View V = make_view(D, state, rp_pool, col_pool, val_pool, max_blocks);
+  View V = make_view(D, state, pools, max_blocks);
   double* dinv = vec5;
   double* r = vec5 + pool_rows;
   double* z = vec5 + 2 * pool_rows;
@@ -416,15 +470,14 @@ int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_b
   return MIFE_OK;
 }
 
-int mife_grouped_cg_residual(int n_seg, int n_blocks, int max_segments, int max_blocks, const int32_t* rp_pool,
-                             const int32_t* col_pool, const double* val_pool, const double* b, const double* x,
-                             void* work, void* state, void* stream) {
-  MIFE_REQUIRE(rp_pool && col_pool && val_pool && b && x && work && state, MIFE_ERR_POINTER_IS_NULL,
+int mife_grouped_cg_residual(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
+                             const double* b, const double* x, void* work, void* state, void* stream) {
+  MIFE_REQUIRE(pools && pools->rp && pools->val && b && x && work && state, MIFE_ERR_POINTER_IS_NULL,
                "null pointer argument");
   if (n_seg == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   Layout D = carve(work, max_segments, max_blocks);
-  View V = make_view(D, state, rp_pool, col_pool, val_pool, max_blocks);
+  View V = make_view(D, state, pools, max_blocks);
   gcg_resid_kernel<<<n_blocks, 256, 0, s>>>(V, b, x);
   gfin_resid_kernel<<<n_seg, 256, 0, s>>>(V, D.seg_parts, D.seg_ids);
   MIFE_CHECK_HIP(hipGetLastError());

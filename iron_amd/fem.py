@@ -155,11 +155,14 @@ class SolverMatrix:
     length inside windows of ``sigma`` rows so a slice is nearly rectangular.
     Either way the values are gathered straight from the equations-matrix CSR.
     ``schedule`` gives SELL products a slice order: eight contiguous ranges of
-    slices (one per XCD), each walked longest slice first.
+    slices (one per XCD), each walked longest slice first.  ``interleave =
+    (ncomp, n_per_comp)`` orders the rows of a vector field (iron's separated DOF
+    order: component-major) node by node before the length sort, so that the
+    columns of a slice stay within a narrow range (16-bit columns).
     """
 
     def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 256, compress: bool = True,
-                 schedule: bool = True):
+                 schedule: bool = True, interleave: tuple | None = None):
         dev = red.csr.device
         self.red = red
         self.n = red.n_free
@@ -175,6 +178,12 @@ class SolverMatrix:
         n = self.n
         lengths = np.diff(red.row_ptr.cpu().numpy().astype(np.int64))
         idx = np.arange(n)
+        if interleave is not None:
+            ncomp, nper = int(interleave[0]), int(interleave[1])
+            dof = red.free_idx.cpu().numpy().astype(np.int64)
+            key = (dof % nper) * ncomp + dof // nper
+            idx = np.empty(n, dtype=np.int64)
+            idx[np.argsort(key, kind="stable")] = np.arange(n)
         perm = np.lexsort((idx, -lengths, idx // sigma)).astype(np.int32)
         iperm = np.empty(n, dtype=np.int32)
         iperm[perm] = np.arange(n, dtype=np.int32)

@@ -2,8 +2,9 @@
 
 Requests (a diffusion or an elasticity box of its own size and coefficients)
 arrive over time.  Each admitted request is assembled and reduced on the GPU
-(the same kernels as a single solve), and its reduced system is copied into
-shared HBM pools: CSR row pointers, columns and values, and the CG vectors.
+(the same kernels as a single solve), and its reduced system is stored into
+shared HBM pools in SELL-64 form (slice pointers, 16-bit or int32 columns,
+values; vectors in slice numbering), next to the CG vectors.
 All admitted requests then iterate together in grouped Jacobi CG launches
 (mife_grouped_cg_*): one launch serves every unconverged system, each with its
 own scalars and stopping test.  Between chunks of iterations the server
@@ -29,15 +30,10 @@ import numpy as np
 import torch
 
 from . import _lib, fem
-from ._lib import call, ptr
+from ._lib import Pools, Segment, call, ptr
 from .basis import basis
 from .mesh import DofMap, RegularMesh, structured_colours
 
-
-class Segment(C.Structure):
-    """mife_segment"""
-    _fields_ = [("n", C.c_int32), ("maxit", C.c_int32), ("slot", C.c_int32), ("pad", C.c_int32),
-                ("row_off", C.c_int64), ("rp_off", C.c_int64), ("nz_off", C.c_int64), ("rtol", C.c_double)]
 
 
 @dataclass
@@ -100,6 +96,10 @@ class _Structure:
         self.red = fem.DirichletReduction(self.csr, fixed)
         self.rhs = None if rhs is None else torch.as_tensor(rhs, device=device)
         self.u0 = torch.as_tensor(np.where(fixed, ufix, 0.0), device=device)
+        # the solver matrix's storage (SELL-64; elasticity rows node by node so that
+        # the columns of a slice fit 16 bits)
+        self.S = fem.SolverMatrix(self.red, schedule=False, interleave=None if ncomp == 1 else (3, N))
+        self.perm = self.S.perm.long()
         if kind == "diffusion":
             self.asm = fem.ScalarAssembler(b, X, m.elem_nodes[0], self.csr)
         else:
@@ -159,11 +159,12 @@ class _FirstFit:
 class Server:
     """Grouped-CG serving of Request objects within an HBM budget (bytes)."""
 
-    BYTES_PER_ROW = 7 * 8 + 4      # b, x, D^-1, r, z, p, q (FP64) + row pointer
-    BYTES_PER_NZ = 8 + 4           # value + column
+    BYTES_PER_ROW = 7 * 8 + 4      # b, x, D^-1, r, z, p, q (FP64) + slice pointer / base column share
+    BYTES_PER_NZ = 8 + 4 + 2       # value + int32 column + 16-bit column (both pools exist)
 
     def __init__(self, hbm_budget: float = 32e9, max_segments: int = 256, nz_per_row: float = 60.0,
-                 chunk: int = 16, use_graph: bool = True, device=None, cache_shapes: int = 8):
+                 chunk: int = 16, use_graph: bool = True, device=None, cache_shapes: int = 8,
+                 admit_budget: float = 0.004):
         self.device = fem._dev(device)
         self.max_segments = int(max_segments)
         rows = int(hbm_budget / (self.BYTES_PER_ROW + nz_per_row * self.BYTES_PER_NZ))
@@ -176,7 +177,11 @@ class Server:
         self.vec5 = torch.zeros(5 * self.rows_cap, dtype=torch.float64, device=dev)
         self.rp = torch.zeros(self.rp_cap, dtype=torch.int32, device=dev)
         self.col = torch.zeros(max(self.nz_cap, 1), dtype=torch.int32, device=dev)
+        self.col16 = torch.zeros(max(self.nz_cap, 1), dtype=torch.int16, device=dev)
+        self.cb_cap = self.rows_cap // 64 + 2 * self.max_segments
+        self.cbase = torch.zeros(self.cb_cap, dtype=torch.int32, device=dev)
         self.val = torch.zeros(max(self.nz_cap, 1), dtype=torch.float64, device=dev)
+        self.pools = Pools(ptr(self.rp), ptr(self.col), ptr(self.col16), ptr(self.cbase), ptr(self.val))
         self.max_blocks = _lib.lib().mife_grouped_blocks(self.rows_cap) + 2 * self.max_segments
         L = _lib.lib()
         self.state = torch.zeros(int(L.mife_grouped_state_bytes(self.max_segments)), dtype=torch.uint8, device=dev)
@@ -189,13 +194,16 @@ class Server:
         self.alloc_rows = _FirstFit(self.rows_cap)
         self.alloc_rp = _FirstFit(self.rp_cap)
         self.alloc_nz = _FirstFit(self.nz_cap)
+        self.alloc_cb = _FirstFit(self.cb_cap)
         self.free_slots = list(range(self.max_segments))[::-1]
         self.chunk = int(chunk)
+        self.admit_budget = float(admit_budget)
         self.use_graph = bool(use_graph)
         self.cache = OrderedDict()
         self.cache_shapes = int(cache_shapes)
         self.graph = None
-        self.stats = dict(admitted=0, retired=0, chunks=0, graph_captures=0, graph_replays=0, table_builds=0)
+        self.stats = dict(admitted=0, retired=0, chunks=0, graph_captures=0, graph_replays=0, table_builds=0,
+                          iterations=0, spmv_bytes=0.0)
         # host-side wall time per phase (seconds): structure builds, admission
         # (assembly + reduction + pool copies), iteration chunks, status reads, retirement
         self.timing = dict(structure=0.0, admit=0.0, chunk=0.0, status=0.0, retire=0.0)
@@ -217,43 +225,52 @@ class Server:
 
     def fits(self, req: Request) -> bool:
         st = self._structure(req)
-        r = st.red
-        return (self._largest(self.alloc_rows) >= r.n_free and self._largest(self.alloc_nz) >= r.nnz
-                and self._largest(self.alloc_rp) >= r.n_free + 1 and bool(self.free_slots))
+        S = st.S
+        ns = S.desc.n_slices
+        return (self._largest(self.alloc_rows) >= S.n and self._largest(self.alloc_nz) >= S.stored
+                and self._largest(self.alloc_rp) >= ns + 1 and self._largest(self.alloc_cb) >= ns
+                and bool(self.free_slots))
 
     @staticmethod
     def _largest(a: _FirstFit) -> int:
         return max((n for _, n in a.free), default=0)
 
-    # -- admission: assemble, reduce, copy into the pools
+    # -- admission: assemble, reduce, store into the pools
     def admit(self, req: Request, now: float):
         st = self._structure(req)
-        red = st.red
-        nf, nnz = red.n_free, red.nnz
-        ro = self.alloc_rows.alloc(nf)
-        rpo = self.alloc_rp.alloc(nf + 1)
-        nzo = self.alloc_nz.alloc(nnz)
-        if ro is None or rpo is None or nzo is None:
-            for a, o, n in ((self.alloc_rows, ro, nf), (self.alloc_rp, rpo, nf + 1), (self.alloc_nz, nzo, nnz)):
+        red, S = st.red, st.S
+        nf, stored, ns = S.n, S.stored, S.desc.n_slices
+        c16 = S.col16 is not None
+        want = ((self.alloc_rows, nf), (self.alloc_rp, ns + 1), (self.alloc_nz, stored), (self.alloc_cb, ns))
+        offs = [a.alloc(n) for a, n in want]
+        if any(o is None for o in offs):
+            for (a, n), o in zip(want, offs):
                 if o is not None:
                     a.release(o, n)
             return None
+        ro, rpo, nzo, cbo = offs
         slot = self.free_slots.pop()
         coef = st.coefficients(req.seed, self.device)
         vals = st.csr.zeros()
         st.assemble(coef, vals)
         s = fem._stream()
-        # reduced values straight into the value pool, structure by copy
-        call("mife_gather_values", nnz, ptr(red.src_pos), ptr(vals), fem.F64, ptr(self.val) + 8 * nzo, fem.F64, s)
-        self.rp[rpo:rpo + nf + 1].copy_(red.row_ptr[: nf + 1])
-        self.col[nzo:nzo + nnz].copy_(red.col[:nnz])
+        # solver values straight from the equations CSR into the value pool
+        call("mife_gather_values", stored, ptr(S.src), ptr(vals), fem.F64, ptr(self.val) + 8 * nzo, fem.F64, s)
+        self.rp[rpo:rpo + ns + 1].copy_(S.slice_ptr[: ns + 1])
+        if c16:
+            self.col16[nzo:nzo + stored].copy_(S.col16[:stored])
+            self.cbase[cbo:cbo + ns].copy_(S.col_base[:ns])
+        else:
+            self.col[nzo:nzo + stored].copy_(S.col[:stored])
+        b = torch.empty(nf, dtype=torch.float64, device=self.device)
         call("mife_solver_rhs", nf, ptr(red.free_idx), ptr(st.csr.row_ptr), ptr(st.csr.col), ptr(vals),
-             ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(self.b) + 8 * ro, s)
+             ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(b), s)
+        self.b[ro:ro + nf].copy_(b[st.perm])          # slice numbering
         self.x[ro:ro + nf].zero_()
-        seg = Segment(nf, 200000, slot, 0, ro, rpo, nzo, req.rtol)
+        seg = Segment(nf, 200000, slot, _lib.MIFE_FMT_SELL, ro, rpo, nzo, req.rtol, ns, 0, cbo if c16 else -1)
         req.t_admit = now
         self.stats["admitted"] += 1
-        return dict(req=req, seg=seg, st=st, nf=nf, nnz=nnz)
+        return dict(req=req, seg=seg, st=st, nf=nf, nnz=red.nnz, stored=stored, ns=ns, cbo=cbo)
 
     def _build_table(self, entries, table):
         segs = (Segment * max(len(entries), 1))(*[e["seg"] for e in entries])
@@ -264,9 +281,9 @@ class Server:
         return nb.value
 
     def _iterate(self, n_seg, nb, iters):
-        call("mife_grouped_cg_iterate", n_seg, nb, self.max_segments, self.max_blocks, ptr(self.rp), ptr(self.col),
-             ptr(self.val), ptr(self.x), ptr(self.vec5), self.rows_cap, int(iters), ptr(self.table),
-             ptr(self.state), fem._stream())
+        call("mife_grouped_cg_iterate", n_seg, nb, self.max_segments, self.max_blocks, C.byref(self.pools),
+             ptr(self.x), ptr(self.vec5), self.rows_cap, int(iters), ptr(self.table), ptr(self.state),
+             fem._stream())
 
     def _status(self):
         it = np.zeros(self.max_segments, dtype=np.int32)
@@ -279,13 +296,16 @@ class Server:
     def _retire(self, e, now):
         req, seg, st = e["req"], e["seg"], e["st"]
         nf = e["nf"]
+        xf = torch.empty(nf, dtype=torch.float64, device=self.device)
+        xf[st.perm] = self.x[seg.row_off:seg.row_off + nf]   # back from slice numbering
         u = st.u0.clone()
-        st.red.field_update(self.x[seg.row_off:seg.row_off + nf], u)
+        st.red.field_update(xf, u)
         req.u = u.cpu().numpy()
         req.t_done = time.perf_counter() - now
         self.alloc_rows.release(seg.row_off, nf)
-        self.alloc_rp.release(seg.rp_off, nf + 1)
-        self.alloc_nz.release(seg.nz_off, e["nnz"])
+        self.alloc_rp.release(seg.rp_off, e["ns"] + 1)
+        self.alloc_nz.release(seg.nz_off, e["stored"])
+        self.alloc_cb.release(e["cbo"], e["ns"])
         self.free_slots.append(seg.slot)
         self.stats["retired"] += 1
 
@@ -322,13 +342,19 @@ class Server:
                 self.stats["chunks"] += 1
                 self.timing["chunk"] += time.perf_counter() - tc
                 launched = True
-            # admission of arrived requests (FIFO, stop at the first that does not fit)
+            # admission of arrived requests (FIFO, stop at the first that does not fit).
+            # The host work of an admission (structure, assembly, pool copies) is
+            # paced against the GPU: while a chunk runs, admissions stop once they
+            # have taken admit_budget seconds, and with nothing running the first
+            # admitted request starts iterating at once
             ta = time.perf_counter()
             with torch.cuda.stream(side):
                 new = []
                 now = time.perf_counter() - t0
                 while qi < len(queue) and queue[qi].arrival <= now and (
                         max_batch is None or len(active) + len(new) < max_batch):
+                    if new and (not (launched or pending) or time.perf_counter() - ta > self.admit_budget):
+                        break
                     e = self.admit(queue[qi], now) if self.fits(queue[qi]) else None
                     if e is None:
                         break
@@ -336,8 +362,8 @@ class Server:
                     qi += 1
                 if new:
                     nbi = self._build_table(new, self.init_table)
-                    call("mife_grouped_cg_init", len(new), nbi, self.max_segments, self.max_blocks, ptr(self.rp),
-                         ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
+                    call("mife_grouped_cg_init", len(new), nbi, self.max_segments, self.max_blocks,
+                         C.byref(self.pools), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
                          ptr(self.init_table), ptr(self.state), fem._stream())
                     pending += new
             self.timing["admit"] += time.perf_counter() - ta
@@ -348,12 +374,20 @@ class Server:
             ts = time.perf_counter()
             it, dn, rr = self._status()
             self.timing["status"] += time.perf_counter() - ts
+            for e in active:
+                # algorithmic bytes of the grouped products this chunk ran for the segment:
+                # 12 B per stored entry (FP64 value + column) and 44 B per row (row
+                # pointer, z, p, q read, p, q written)
+                ran = int(it[e["seg"].slot]) - e.get("it_seen", 0)
+                e["it_seen"] = int(it[e["seg"].slot])
+                self.stats["iterations"] += ran
+                self.stats["spmv_bytes"] += ran * (12.0 * e["nnz"] + 44.0 * e["nf"])
             finished = [e for e in active if dn[e["seg"].slot]]
             tr = time.perf_counter()
             if finished:
                 tbl = self._build_table(finished, self.resid_table)
-                call("mife_grouped_cg_residual", len(finished), tbl, self.max_segments, self.max_blocks, ptr(self.rp),
-                     ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.resid_table), ptr(self.state),
+                call("mife_grouped_cg_residual", len(finished), tbl, self.max_segments, self.max_blocks,
+                     C.byref(self.pools), ptr(self.b), ptr(self.x), ptr(self.resid_table), ptr(self.state),
                      fem._stream())
                 it, dn, rr = self._status()
                 redo = []
@@ -364,13 +398,14 @@ class Server:
                     if rr[slot] > e["req"].rtol and e.get("restarts", 0) < 4:
                         # the recurrence drifted below the true residual: restart CG from it
                         e["restarts"] = e.get("restarts", 0) + 1
+                        e["it_seen"] = 0
                         redo.append(e)
                     else:
                         self._retire(e, t0)
                 if redo:
                     nbr = self._build_table(redo, self.resid_table)
-                    call("mife_grouped_cg_init", len(redo), nbr, self.max_segments, self.max_blocks, ptr(self.rp),
-                         ptr(self.col), ptr(self.val), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
+                    call("mife_grouped_cg_init", len(redo), nbr, self.max_segments, self.max_blocks,
+                         C.byref(self.pools), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
                          ptr(self.resid_table), ptr(self.state), fem._stream())
                 redo_ids = {id(e) for e in redo}
                 retired = {id(e) for e in finished if id(e) not in redo_ids}
@@ -412,7 +447,7 @@ def solve_one_fp64(req: Request, structures: dict, device=None) -> Request:
     vals = st.csr.zeros()
     st.assemble(coef, vals)
     red = st.red
-    S = fem.SolverMatrix(red, fmt="csr")
+    S = fem.SolverMatrix(red, interleave=None if req.kind == "diffusion" else (3, st.mesh.num_nodes(0)))
     b = red.rhs(vals, st.u0, rhs=st.rhs)
     x = torch.zeros(red.n_free, dtype=torch.float64, device=dev)
     ws = fem.KrylovWorkspace(red.n_free, dev)
