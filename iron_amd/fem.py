@@ -176,30 +176,32 @@ class SolverMatrix:
         if fmt != "sell":
             raise ValueError(f"unknown solver matrix format {fmt!r}")
         n = self.n
-        lengths = np.diff(red.row_ptr.cpu().numpy().astype(np.int64))
-        idx = np.arange(n)
+        # slice order on the device: rows by window of sigma positions, then by
+        # decreasing length, then by position (one sort of a composite key)
+        lengths = (red.row_ptr[1:] - red.row_ptr[:-1]).to(torch.int64)
+        pos = torch.arange(n, dtype=torch.int64, device=dev)
         if interleave is not None:
             ncomp, nper = int(interleave[0]), int(interleave[1])
-            dof = red.free_idx.cpu().numpy().astype(np.int64)
-            key = (dof % nper) * ncomp + dof // nper
-            idx = np.empty(n, dtype=np.int64)
-            idx[np.argsort(key, kind="stable")] = np.arange(n)
-        perm = np.lexsort((idx, -lengths, idx // sigma)).astype(np.int32)
-        iperm = np.empty(n, dtype=np.int32)
-        iperm[perm] = np.arange(n, dtype=np.int32)
+            dof = red.free_idx.to(torch.int64)
+            order = torch.argsort((dof % nper) * ncomp + dof // nper, stable=True)
+            pos = torch.empty_like(pos)
+            pos[order] = torch.arange(n, dtype=torch.int64, device=dev)
+        L = int(lengths.max().item()) + 1 if n > 0 else 1
+        key = (pos // sigma) * (L * sigma) + (L - 1 - lengths) * sigma + pos % sigma
+        perm = torch.argsort(key)
+        self.perm = perm.to(torch.int32)
+        self.iperm = torch.empty_like(self.perm)
+        self.iperm[perm] = torch.arange(n, dtype=torch.int32, device=dev)
         n_slices = (n + 63) // 64
-        lp = np.zeros(n_slices * 64, dtype=np.int64)
+        lp = torch.zeros(n_slices * 64, dtype=torch.int64, device=dev)
         lp[:n] = lengths[perm]
-        slice_len = lp.reshape(n_slices, 64).max(axis=1)
-        slice_ptr = np.zeros(n_slices + 1, dtype=np.int64)
-        slice_ptr[1:] = np.cumsum(slice_len * 64)
-        if slice_ptr[-1] >= 2 ** 31:
+        slice_ptr = torch.zeros(n_slices + 1, dtype=torch.int64, device=dev)
+        slice_ptr[1:] = torch.cumsum(lp.view(n_slices, 64).amax(dim=1) * 64, 0)
+        self.stored = int(slice_ptr[-1].item())
+        if self.stored >= 2 ** 31:
             raise ValueError("SELL storage exceeds the int32 index range")
-        self.stored = int(slice_ptr[-1])
         self.padding = self.stored / max(red.nnz, 1) - 1.0
-        self.perm = torch.as_tensor(perm, device=dev)
-        self.iperm = torch.as_tensor(iperm, device=dev)
-        self.slice_ptr = torch.as_tensor(slice_ptr.astype(np.int32), device=dev)
+        self.slice_ptr = slice_ptr.to(torch.int32)
         self.col = torch.empty(max(self.stored, 1), dtype=torch.int32, device=dev)
         self.src = torch.empty(max(self.stored, 1), dtype=torch.int32, device=dev)
         call("mife_sell_build", n, ptr(red.row_ptr), ptr(red.col), ptr(red.src_pos), ptr(self.perm), ptr(self.iperm),
