@@ -15,7 +15,9 @@
  * work is ordered on the caller's hipStream_t (passed as void*; NULL = default
  * stream).  Index arrays are 0-based int32 (iron's CSR is 1-based; the
  * conversion is a constant -1).  Calls that read results back to the host
- * (iteration counts, norms) say so.
+ * (iteration counts, norms) say so.  The vector entry points (gather, solver
+ * RHS, field update, back-substitution) return MIFE_OK at once for a zero
+ * count, and their array arguments may then be NULL.
  */
 #ifndef MIFE_H
 #define MIFE_H

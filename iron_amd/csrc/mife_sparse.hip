@@ -217,10 +217,10 @@ int mife_sell_order(int n_slices, const int32_t* slice_ptr, int32_t* order, int*
 
 int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int src_prec, void* dst, int dst_prec,
                        void* stream) {
+  if (nnz == 0) return MIFE_OK;  // empty arrays may be passed as NULL
   MIFE_REQUIRE(src_pos && src && dst, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE((src_prec == MIFE_F64 || src_prec == MIFE_F32) && (dst_prec == MIFE_F64 || dst_prec == MIFE_F32),
                MIFE_ERR_ARG, "invalid precision");
-  if (nnz == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   const int grid = (int)std::min<long long>(div_up(nnz, 256), 8192);
   if (src_prec == MIFE_F64 && dst_prec == MIFE_F64)
@@ -238,9 +238,9 @@ int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int
 int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr, const int32_t* col,
                     const double* vals, const int32_t* new_idx, const double* u, const double* src,
                     const double* rhs, const uint8_t* rhs_fixed, double* b, void* stream) {
+  if (n_free == 0) return MIFE_OK;  // empty arrays may be passed as NULL
   MIFE_REQUIRE(free_idx && row_ptr && col && vals && new_idx && u && b, MIFE_ERR_POINTER_IS_NULL,
                "null pointer argument");
-  if (n_free == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   solver_rhs_kernel<<<div_up(n_free, 4), 256, 0, s>>>(n_free, free_idx, row_ptr, col, vals, new_idx, u, src, rhs,
                                                       rhs_fixed, b);
@@ -249,8 +249,8 @@ int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr,
 }
 
 int mife_field_update(int n_free, const int32_t* free_idx, const double* x, double* u, void* stream) {
+  if (n_free == 0) return MIFE_OK;  // empty arrays may be passed as NULL
   MIFE_REQUIRE(free_idx && x && u, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
-  if (n_free == 0) return MIFE_OK;
   field_update_kernel<<<div_up(n_free, 256), 256, 0, (hipStream_t)stream>>>(n_free, free_idx, x, u);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
@@ -258,8 +258,8 @@ int mife_field_update(int n_free, const int32_t* free_idx, const double* x, doub
 
 int mife_backsubstitute(int n, const int32_t* row_ptr, const int32_t* col, const double* vals, const double* u,
                         const double* src, const uint8_t* rhs_fixed, int iron_compat, double* R, void* stream) {
+  if (n == 0) return MIFE_OK;  // empty arrays may be passed as NULL
   MIFE_REQUIRE(row_ptr && col && vals && u && R, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
-  if (n == 0) return MIFE_OK;
   backsub_kernel<<<div_up(n, 4), 256, 0, (hipStream_t)stream>>>(n, row_ptr, col, vals, u, src, rhs_fixed,
                                                                   iron_compat, R);
   MIFE_CHECK_HIP(hipGetLastError());

@@ -176,6 +176,14 @@ class SolverMatrix:
         if fmt != "sell":
             raise ValueError(f"unknown solver matrix format {fmt!r}")
         n = self.n
+        if n == 0:  # every DOF fixed: an empty matrix (valid, non-null pointers)
+            z = torch.zeros(1, dtype=torch.int32, device=dev)
+            self.perm = self.iperm = torch.zeros(0, dtype=torch.int32, device=dev)
+            self.slice_ptr = self.col = self.src = z
+            self.col16 = self.col_base = self.slice_order = None
+            self.stored, self.padding = 0, 0.0
+            self.desc = MatrixDesc(0, _lib.MIFE_FMT_SELL, ptr(z), ptr(z), ptr(z), 0, 0, None, None, 0)
+            return
         # slice order on the device: rows by window of sigma positions, then by
         # decreasing length, then by position (one sort of a composite key)
         lengths = (red.row_ptr[1:] - red.row_ptr[:-1]).to(torch.int64)

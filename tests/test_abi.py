@@ -31,3 +31,29 @@ def test_error_text_roundtrip_without_gpu():
     from iron_amd import _lib
     assert _lib.lib().mife_version().decode().startswith("mi_fe")
     assert _lib.lib().mife_last_error(None, 0) == -1
+
+
+def test_argument_errors_without_gpu():
+    """Argument checks come before any device work and follow the C bindings'
+    convention: -1 for a null pointer, positive codes for invalid arguments, with
+    the reason in mife_last_error (bindings/generate_bindings/c.py:31-40)."""
+    from iron_amd import _lib
+    L = _lib.lib()
+    M = _lib.MatrixDesc
+    # null pointers
+    assert L.mife_spmv(None, None, None, None, 0, None) == -1
+    assert "null" in _lib.last_error()
+    assert L.mife_cg(None, None, None, None, 0, 1e-8, 10, None, None, None, None) == -1
+    assert L.mife_sell_order(4, None, None, None, None) == -1
+    assert L.mife_grouped_setup(None, 0, 1, 1, None, None, None, None) == -1
+    # invalid arguments on a well-formed (host-side) descriptor
+    d = M(10, 7, 1, 1, None, 0, 10, None, None, 0)
+    assert L.mife_spmv(ctypes.byref(d), 1, 1, 1, 0, None) == 2     # unknown format 7
+    assert "format" in _lib.last_error()
+    d = M(10, 0, 1, 1, None, 0, 10, None, None, 0)
+    assert L.mife_spmv(ctypes.byref(d), 1, 1, 1, 5, None) == 2     # unknown precision 5
+    assert "precision" in _lib.last_error()
+    assert L.mife_cg(ctypes.byref(d), 1, 1, 1, 0, -1.0, 10, 1, None, None, None) == 2   # negative rtol
+    # 8 ranges of ceil(ceil(n_slices / 8) / 4) * 4 entries
+    assert L.mife_sell_order_len(0) == 0 and L.mife_sell_order_len(1) == 32
+    assert L.mife_sell_order_len(13397) == 8 * 1676   # ceil(13397 / 8) = 1675 -> 1676
