@@ -1,7 +1,7 @@
 // Mooney-Rivlin finite elasticity: element residual, exact element Jacobian and
 // gravity vector, scattered into a global residual, CSR values and RHS vector.
 //
-// One 256-thread workgroup integrates one element in FP64:
+// One workgroup of kMrThreads threads integrates one element in FP64:
 //   1. gather the element's reference coordinates and current dependent values
 //      (deformed coordinates, pressures) into LDS;
 //   2. one thread per Gauss point forms dX/dxi, its inverse, |J| w, F = dz/dX,
@@ -57,6 +57,10 @@ struct GaussState {
 
 }  // namespace mr
 
+// Threads per element: 16 wavefronts, so that an element's latency-bound stages
+// (one colour holds few elements) have waves to switch between.
+constexpr int kMrThreads = 1024;
+
 template <int NN, int NP, int NG, int FORM>
 struct MrLDS {
   double X[NN][3];
@@ -97,7 +101,7 @@ __device__ __forceinline__ double tangent_entry(const mr::GaussState& g, int idx
 }
 
 template <int NN, int NP, int NG, int FORM>
-__global__ __launch_bounds__(256) void mr_element_kernel(
+__global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
     const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
     const double* __restrict__ state, const double* __restrict__ material, const double* __restrict__ dN,
     const double* __restrict__ Nu, const double* __restrict__ Np, const double* __restrict__ w, double bfx,
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(256) void mr_element_kernel(
     double* __restrict__ rhs, bool atomic) {
   constexpr int NE = 3 * NN + NP;
   constexpr int NENT = NE * NE;
-  constexpr int PER = (NENT + 255) / 256;
+  constexpr int PER = (NENT + kMrThreads - 1) / kMrThreads;
   __shared__ MrLDS<NN, NP, NG, FORM> S;
   const int t = threadIdx.x;
   const int idx = blockIdx.x;
@@ -235,12 +239,12 @@ __global__ __launch_bounds__(256) void mr_element_kernel(
   // 3. tangents and reference gradients
   const bool want_jac = jac != nullptr;
   if (want_jac) {
-    for (int k = t; k < NG * 81; k += 256) {
+    for (int k = t; k < NG * 81; k += kMrThreads) {
       const int g = k / 81;
       S.A[g][k - g * 81] = tangent_entry<FORM>(S.gs[g], k - g * 81);
     }
   }
-  for (int k = t; k < NG * NN; k += 256) {
+  for (int k = t; k < NG * NN; k += kMrThreads) {
     const int g = k / NN, a = k - g * NN;
     const double* xi = S.gs[g].xi;
 #pragma unroll
@@ -253,13 +257,13 @@ __global__ __launch_bounds__(256) void mr_element_kernel(
   }
   __syncthreads();
 
-  // 4. element Jacobian, entries r*NE+c owned by thread t: t, t+256, ...
+  // 4. element Jacobian, entries r*NE+c owned by thread t: t, t+kMrThreads, ...
   double acc[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) acc[k] = 0.0;
   if (want_jac) {
     for (int g = 0; g < NG; ++g) {
-      for (int k = t; k < NN * 27; k += 256) {
+      for (int k = t; k < NN * 27; k += kMrThreads) {
         const int a = k / 27, r = k - a * 27;  // r = (i*3+j)*3+L
         const int i = r / 9, jl = r - i * 9;
         const double* A = S.A[g];
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(256) void mr_element_kernel(
       const double wj = S.gs[g].wj;
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
-        const int ent = t + k * 256;
+        const int ent = t + k * kMrThreads;
         if (ent < NENT) {
           const int row = ent / NE, col = ent - (ent / NE) * NE;
           double v;
@@ -301,7 +305,7 @@ __global__ __launch_bounds__(256) void mr_element_kernel(
     const int32_t* pe = pos + e * (long long)NENT;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-      const int ent = t + k * 256;
+      const int ent = t + k * kMrThreads;
       if (ent < NENT) {
         double* dst = jac + pe[ent];
         if (atomic) atomicAdd(dst, acc[k]);
@@ -352,14 +356,14 @@ static int launch_mr(const mife_mr_desc* d, int n_elem, const double* coords, co
   const int want_rhs = (rhs != nullptr && body_force != nullptr) ? 1 : 0;
   const int tk = prof::begin(s);
   if (!order) {
-    mr_element_kernel<NN, NP, NG, FORM><<<n_elem, 256, 0, s>>>(coords, enodes, edofs, state, material, d->u->dN,
+    mr_element_kernel<NN, NP, NG, FORM><<<n_elem, kMrThreads, 0, s>>>(coords, enodes, edofs, state, material, d->u->dN,
                                                                d->u->N, d->Np, d->u->w, bx, by, bz, want_rhs, pos,
                                                                nullptr, 0, n_elem, residual, jac, rhs, true);
   } else {
     for (int c = 0; c < ncol; ++c) {
       const int cnt = cptr[c + 1] - cptr[c];
       if (cnt <= 0) continue;
-      mr_element_kernel<NN, NP, NG, FORM><<<cnt, 256, 0, s>>>(coords, enodes, edofs, state, material, d->u->dN,
+      mr_element_kernel<NN, NP, NG, FORM><<<cnt, kMrThreads, 0, s>>>(coords, enodes, edofs, state, material, d->u->dN,
                                                               d->u->N, d->Np, d->u->w, bx, by, bz, want_rhs, pos,
                                                               order, cptr[c], cnt, residual, jac, rhs, false);
     }
