@@ -55,6 +55,13 @@ struct GaussState {
   double wj;                             // w |det dX/dxi|
 };
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+// 16x16x4 FP64 MFMA: lane l gives A[l%16][l/16] and B[l/16][l%16]; accumulator
+// register r of lane l holds row 4r + l/16, column l%16.
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 }  // namespace mr
 
 // Threads per element: 16 wavefronts, so that an element's latency-bound stages
@@ -69,7 +76,6 @@ struct MrLDS {
   mr::GaussState gs[NG];
   double A[NG][81];
   double GX[NG][NN][3];
-  double T[NN][27];  // T[a][(i*3+j)*3+L] for the current Gauss point
 };
 
 // A[i,K,j,L] = dP_iK / dF_jL of one Gauss point; idx = ((i*3+K)*3+j)*3+L.
@@ -110,7 +116,6 @@ __global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
     double* __restrict__ rhs, bool atomic) {
   constexpr int NE = 3 * NN + NP;
   constexpr int NENT = NE * NE;
-  constexpr int PER = (NENT + kMrThreads - 1) / kMrThreads;
   __shared__ MrLDS<NN, NP, NG, FORM> S;
   const int t = threadIdx.x;
   const int idx = blockIdx.x;
@@ -257,60 +262,75 @@ __global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
   }
   __syncthreads();
 
-  // 4. element Jacobian, entries r*NE+c owned by thread t: t, t+kMrThreads, ...
-  double acc[PER];
-#pragma unroll
-  for (int k = 0; k < PER; ++k) acc[k] = 0.0;
+  // 4. element Jacobian.
+  // K_uu: K[(i,a)][(j,b)] = sum_g sum_L U[(i,j,a)][(g,L)] GX[g][b][L] with
+  // U[(i,j,a)][(g,L)] = w_g sum_K GX[g][a][K] A_g[(i,K)][(j,L)]: one GEMM of
+  // (9 NN) x (3 NG) by (3 NG) x NN on 16x16x4 FP64 MFMA tiles.  Wavefront w owns
+  // row tile w and every column tile; U is formed operand by operand from LDS.
   if (want_jac) {
-    for (int g = 0; g < NG; ++g) {
-      for (int k = t; k < NN * 27; k += kMrThreads) {
-        const int a = k / 27, r = k - a * 27;  // r = (i*3+j)*3+L
-        const int i = r / 9, jl = r - i * 9;
-        const double* A = S.A[g];
-        S.T[a][r] = S.GX[g][a][0] * A[(i * 3 + 0) * 9 + jl] + S.GX[g][a][1] * A[(i * 3 + 1) * 9 + jl] +
-                    S.GX[g][a][2] * A[(i * 3 + 2) * 9 + jl];
+    constexpr int MROWS = 9 * NN, KD = 3 * NG;
+    constexpr int MT = (MROWS + 15) / 16, NT = (NN + 15) / 16, STEPS = (KD + 3) / 4;
+    static_assert(MT <= kMrThreads / 64 && NT <= 2, "tile counts exceed the workgroup");
+    const int wv = t >> 6, lane = t & 63;
+    const int32_t* pe = pos + e * (long long)NENT;
+    if (wv < MT) {
+      const int mrow = 16 * wv + (lane & 15);
+      const bool rv = mrow < MROWS;
+      const int ia = rv ? mrow / (3 * NN) : 0, ja = rv ? (mrow / NN) % 3 : 0, aa = rv ? mrow % NN : 0;
+      mr::d4 k0 = {0.0, 0.0, 0.0, 0.0}, k1 = {0.0, 0.0, 0.0, 0.0};
+      for (int st = 0; st < STEPS; ++st) {
+        const int k = 4 * st + (lane >> 4);
+        const bool kv = k < KD;
+        const int g = kv ? k / 3 : 0, L = k - 3 * (k / 3);
+        double u = 0.0, b0 = 0.0, b1 = 0.0;
+        if (kv) {
+          const double* A = S.A[g];
+          if (rv)
+            u = S.gs[g].wj * (S.GX[g][aa][0] * A[(ia * 3 + 0) * 9 + ja * 3 + L] +
+                              S.GX[g][aa][1] * A[(ia * 3 + 1) * 9 + ja * 3 + L] +
+                              S.GX[g][aa][2] * A[(ia * 3 + 2) * 9 + ja * 3 + L]);
+          const int bb = lane & 15;
+          if (bb < NN) b0 = S.GX[g][bb][L];
+          if (NT > 1 && 16 + bb < NN) b1 = S.GX[g][16 + bb][L];
+        }
+        k0 = mr::mfma(u, b0, k0);
+        if (NT > 1) k1 = mr::mfma(u, b1, k1);
       }
-      __syncthreads();
-      const double wj = S.gs[g].wj;
 #pragma unroll
-      for (int k = 0; k < PER; ++k) {
-        const int ent = t + k * kMrThreads;
-        if (ent < NENT) {
-          const int row = ent / NE, col = ent - (ent / NE) * NE;
-          double v;
-          if (row < 3 * NN && col < 3 * NN) {
-            const int i = row / NN, a = row - i * NN, j = col / NN, b = col - j * NN;
-            const double* Tr = &S.T[a][(i * 3 + j) * 3];
-            v = Tr[0] * S.GX[g][b][0] + Tr[1] * S.GX[g][b][1] + Tr[2] * S.GX[g][b][2];
-          } else if (row < 3 * NN) {
-            const int i = row / NN, a = row - i * NN, q = col - 3 * NN;
-            const double* dp = S.gs[g].dPdp;
-            const double psi = (NP == 1) ? 1.0 : Np[g * NP + q];
-            v = (S.GX[g][a][0] * dp[i * 3 + 0] + S.GX[g][a][1] * dp[i * 3 + 1] + S.GX[g][a][2] * dp[i * 3 + 2]) *
-                psi;
-          } else if (col < 3 * NN) {
-            const int q = row - 3 * NN, j = col / NN, b = col - j * NN;
-            const double* jg = S.gs[g].JG;
-            const double psi = (NP == 1) ? 1.0 : Np[g * NP + q];
-            v = (S.GX[g][b][0] * jg[j * 3 + 0] + S.GX[g][b][1] * jg[j * 3 + 1] + S.GX[g][b][2] * jg[j * 3 + 2]) *
-                psi;
-          } else {
-            v = 0.0;
-          }
-          acc[k] += wj * v;
+      for (int r = 0; r < 4; ++r) {
+        const int orow = 16 * wv + 4 * r + (lane >> 4);  // accumulator row of register r
+        if (orow >= MROWS) continue;
+        const int i = orow / (3 * NN), j = (orow / NN) % 3, a = orow % NN;
+        const int b = lane & 15;
+        if (b < NN) {
+          double* dst = jac + pe[(i * NN + a) * NE + j * NN + b];
+          if (atomic) atomicAdd(dst, k0[r]);
+          else *dst += k0[r];
+        }
+        if (NT > 1 && 16 + b < NN) {
+          double* dst = jac + pe[(i * NN + a) * NE + j * NN + 16 + b];
+          if (atomic) atomicAdd(dst, k1[r]);
+          else *dst += k1[r];
         }
       }
-      __syncthreads();
     }
-    const int32_t* pe = pos + e * (long long)NENT;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int ent = t + k * kMrThreads;
-      if (ent < NENT) {
-        double* dst = jac + pe[ent];
-        if (atomic) atomicAdd(dst, acc[k]);
-        else *dst += acc[k];
+    // K_up = GX . dP/dp psi and K_pu = psi JG . GX (K_pp = 0): sums over the Gauss points
+    constexpr int NUP = 3 * NN * NP;
+    for (int ent = t; ent < 2 * NUP; ent += kMrThreads) {
+      const bool up = ent < NUP;
+      const int k = up ? ent : ent - NUP;
+      const int q = k % NP, r = k / NP, i = r / NN, a = r - (r / NN) * NN;
+      double v = 0.0;
+      for (int g = 0; g < NG; ++g) {
+        const double* m = up ? S.gs[g].dPdp : S.gs[g].JG;
+        const double psi = (NP == 1) ? 1.0 : Np[g * NP + q];
+        v += S.gs[g].wj * psi *
+             (S.GX[g][a][0] * m[i * 3 + 0] + S.GX[g][a][1] * m[i * 3 + 1] + S.GX[g][a][2] * m[i * 3 + 2]);
       }
+      const int row = up ? i * NN + a : 3 * NN + q, col = up ? 3 * NN + q : i * NN + a;
+      double* dst = jac + pe[row * NE + col];
+      if (atomic) atomicAdd(dst, v);
+      else *dst += v;
     }
   }
 
