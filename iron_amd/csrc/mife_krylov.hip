@@ -91,10 +91,22 @@ __global__ void diag_inv_kernel(MatArgs A, const T* __restrict__ vals, T* __rest
     const int s = i / kSlice, lane = i - s * kSlice;
     const int base = A.ptr[s], len = (A.ptr[s + 1] - base) / kSlice;
     const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col);
-    for (int j = 0; j < len; ++j) {
-      const int e = base + j * kSlice + lane;
-      const int c = A.fmt == kFmtSell16 ? A.cbase[s] + (int)c16[e] : A.col[e];
-      if (c == i && vals[e] != (T)0) { d = vals[e]; break; }
+    const int cb = A.fmt == kFmtSell16 ? A.cbase[s] : 0;
+    // the whole row, eight entries in flight (padding entries carry value 0);
+    // the first nonzero entry in column i is the diagonal
+    for (int j0 = 0; j0 < len; j0 += 8) {
+      int cj[8];
+      T vj[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int e = base + (j0 + u) * kSlice + lane;
+        const bool in = j0 + u < len;
+        cj[u] = in ? (A.fmt == kFmtSell16 ? cb + (int)c16[e] : A.col[e]) : -1;
+        vj[u] = in ? vals[e] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (d == (T)0 && cj[u] == i && vj[u] != (T)0) d = vj[u];
     }
   } else {
     int lo = A.ptr[i], hi = A.ptr[i + 1];
