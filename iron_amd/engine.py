@@ -165,7 +165,11 @@ def _linear_setup(es, rm: RegularMesh, dm: DofMap, fixed: np.ndarray):
     ed = dm.element_dofs()
     c.csr = fem.CSRStructure(ed, dm.num_dofs, dev, colours=structured_colours(rm))
     c.red = fem.DirichletReduction(c.csr, fixed)
-    c.S = fem.SolverMatrix(c.red)
+    # a vector field on one mesh component: rows node by node (16-bit SELL columns)
+    ncomp = len(dm.comp_mesh)
+    nper = rm.num_nodes(dm.comp_mesh[0])
+    inter = (ncomp, nper) if ncomp > 1 and dm.num_dofs == ncomp * nper else None
+    c.S = fem.SolverMatrix(c.red, interleave=inter)
     c.ws = fem.KrylovWorkspace(c.red.n_free, dev)
     es._device_cache = c
     return c

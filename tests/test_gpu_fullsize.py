@@ -105,7 +105,9 @@ def test_c3_elastic96_equilibrium():
     tip = np.nonzero(X[:, 0] == 3.0)[0]
     f = engine.neumann_integrate(m, dm, [b] * 3, {int(2 * N + i): -1.0 for i in tip})
     red = fem.DirichletReduction(csr, fixed)
-    S = fem.SolverMatrix(red)
+    # rows node by node: every slice's columns then fit 16 bits
+    S = fem.SolverMatrix(red, interleave=(3, N))
+    assert S.col16 is not None
     u = torch.zeros(csr.n, dtype=torch.float64, device="cuda")
     ft = torch.as_tensor(f, device="cuda")
     bvec = red.rhs(v, u, rhs=ft)
