@@ -157,11 +157,22 @@ def timed_steps(P, k, mixed, inner_rtol):
     return time.perf_counter() - t0, lat, infos
 
 
-def cpu_baseline(P, fp64_iters, budget_s):
-    """Times the C restatement of iron's algorithm (oracle/liboracle_fe.so) on one
-    core over a bounded sample and scales it to one full pass of the workload."""
+def host_threads():
+    """Host threads for the all-cores CPU baseline: OMP_NUM_THREADS when set (16 on a
+    one-GPU share of the box), else the CPUs this process may run on."""
+    try:
+        return max(1, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(P, fp64_iters, budget_s, threads=1):
+    """Times the C restatement of iron's algorithm (oracle/liboracle_fe.so) over a
+    bounded sample and scales it to one full pass of the workload: on one core (iron's
+    per-rank serial loop), or on `threads` OpenMP threads (elements shared out with
+    atomic adds, CG rows and dot products split) as a proxy for iron on that many
+    ranks."""
     import ctypes as C
-    import torch
     sys.path.insert(0, str(ROOT))
     so = ROOT / "oracle" / "liboracle_fe.so"
     if not so.exists():
@@ -183,22 +194,27 @@ def cpu_baseline(P, fp64_iters, budget_s):
     src = np.zeros(csr.n)
     P_ = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
     lib.oracle_assemble_scalar3.restype = C.c_int
+    lib.oracle_assemble_scalar3_par.restype = C.c_int
+    nthr = host_threads()
+
+    def assemble(e0, e1, nt):
+        args = (E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed), P_(kap), P_(f),
+                P_(rp), P_(col), P_(vals), P_(src), e0, e1)
+        return lib.oracle_assemble_scalar3(*args) if nt == 1 else lib.oracle_assemble_scalar3_par(*args, nt)
+
     # assembly sample: elements [0, Es) chosen to fit ~40% of the budget
     t = time.perf_counter()
-    probe = min(E, 512)
-    lib.oracle_assemble_scalar3(E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed),
-                                P_(kap), P_(f), P_(rp), P_(col), P_(vals), P_(src), 0, probe)
+    probe = min(E, 512 * threads)
+    assemble(0, probe, threads)
     per_elem = (time.perf_counter() - t) / probe
     Es = int(min(E, max(probe, 0.4 * budget_s / max(per_elem, 1e-9))))
     vals[:] = 0.0
     src[:] = 0.0
     t = time.perf_counter()
-    lib.oracle_assemble_scalar3(E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed),
-                                P_(kap), P_(f), P_(rp), P_(col), P_(vals), P_(src), 0, Es)
+    assemble(0, Es, threads)
     t_asm = (time.perf_counter() - t) * (E / Es)
-    if Es < E:  # complete the matrix (untimed) so the solve runs on the real system
-        lib.oracle_assemble_scalar3(E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed),
-                                    P_(kap), P_(f), P_(rp), P_(col), P_(vals), P_(src), Es, E)
+    if Es < E:  # complete the matrix (untimed, all host threads) so the solve runs on the real system
+        assemble(Es, E, nthr)
     new_idx = red.new_idx.cpu().numpy().astype(np.int32)
     u = P["u"].cpu().numpy()
     srp = np.zeros(red.n_free + 1, dtype=np.int32)
@@ -217,12 +233,12 @@ def cpu_baseline(P, fp64_iters, budget_s):
                               C.c_int, C.c_int, C.POINTER(C.c_double)]
     t = time.perf_counter()
     it_probe = 5
-    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, it_probe, 1, C.byref(rr))
+    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, it_probe, threads, C.byref(rr))
     per_it = (time.perf_counter() - t) / it_probe
     n_it = int(max(it_probe, min(fp64_iters, 0.5 * budget_s / max(per_it, 1e-9))))
     x[:] = 0.0
     t = time.perf_counter()
-    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, n_it, 1, C.byref(rr))
+    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, n_it, threads, C.byref(rr))
     per_it = (time.perf_counter() - t) / n_it
     t_solve = per_it * fp64_iters
     t_total = t_asm + t_red + t_solve
@@ -235,9 +251,10 @@ def cpu_baseline(P, fp64_iters, budget_s):
     except OSError:
         pass
     return {
-        "value": E / t_total, "unit": "elems/s", "cores": 1, "kind": "port",
-        "sample": (f"C restatement of iron's FP64 path on 1 core ({cpu}): Q2 element loop with bisection CSR add "
-                   f"over {Es} of {E} elements (x{E / Es:.2f}), K_ff/RHS reduction in full, {n_it} Jacobi-CG "
+        "value": E / t_total, "unit": "elems/s", "cores": threads, "kind": "port",
+        "sample": (f"C restatement of iron's FP64 path on {threads} thread(s) ({cpu}): element loop with bisection "
+                   f"CSR add"
+                   f" over {Es} of {E} elements (x{E / Es:.2f}), K_ff/RHS reduction in full (serial), {n_it} Jacobi-CG "
                    f"iterations scaled to the {fp64_iters} iterations the FP64 solve needs"),
         "assembly_s": t_asm, "reduce_s": t_red, "cg_iter_ms": per_it * 1e3, "solve_s": t_solve,
         "solves_per_s": 1.0 / t_total,
@@ -503,6 +520,9 @@ def main_leg(args):
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)
+        nthr = host_threads()
+        if nthr > 1:
+            out["cpu_baseline_all_cores"] = cpu_baseline(P, infos64[-1].iterations, 0.5 * args.cpu_seconds, nthr)
     del P
     return out
 

@@ -55,10 +55,47 @@ static int64_t location_find(const int32_t* rp, const int32_t* col, int row, int
   return -1;
 }
 
+/* Element matrix and source vector of element e (iron's metric path: GL = J^T J,
+ * GU = GL^-1, sqrt|det GL|, and the pgmsi . GU . pgnsi contraction of
+ * Laplace_equations_routines.F90:482-579).  Ke[nn][nn], se[nn], xe[nn][3] scratch. */
+static void element_scalar3(int e, int nn, int ng, const double* dN, const double* N, const double* w,
+                            const double* coords, const int32_t* enodes, const double* source, double* Ke,
+                            double* se, double* xe) {
+  memset(Ke, 0, sizeof(double) * nn * nn);
+  memset(se, 0, sizeof(double) * nn);
+  for (int a = 0; a < nn; ++a)
+    for (int d = 0; d < 3; ++d) xe[a * 3 + d] = coords[(int64_t)enodes[(int64_t)e * nn + a] * 3 + d];
+  for (int g = 0; g < ng; ++g) {
+    double J[3][3] = {{0}};
+    for (int a = 0; a < nn; ++a)
+      for (int j = 0; j < 3; ++j) {
+        const double dn = dN[((int64_t)g * nn + a) * 3 + j];
+        for (int d = 0; d < 3; ++d) J[d][j] += xe[a * 3 + d] * dn;
+      }
+    double GL[3][3], GU[3][3];
+    for (int mi = 0; mi < 3; ++mi)
+      for (int ni = 0; ni < 3; ++ni) GL[mi][ni] = J[0][mi] * J[0][ni] + J[1][mi] * J[1][ni] + J[2][mi] * J[2][ni];
+    const double detgl = inv3(GL, GU);
+    const double rwg = sqrt(fabs(detgl)) * w[g];
+    for (int ms = 0; ms < nn; ++ms) {
+      const double* pm = &dN[((int64_t)g * nn + ms) * 3];
+      for (int ns = 0; ns < nn; ++ns) {
+        const double* pn = &dN[((int64_t)g * nn + ns) * 3];
+        double sum = 0.0;
+        for (int mi = 0; mi < 3; ++mi)
+          for (int ni = 0; ni < 3; ++ni) sum += pm[mi] * pn[ni] * GU[mi][ni];
+        Ke[ms * nn + ns] += sum * rwg;
+      }
+      if (source) se[ms] += source[e] * N[(int64_t)g * nn + ms] * rwg;
+    }
+  }
+}
+
 /* Scalar 3-D assembly.  dN[ng][nn][3], N[ng][nn], w[ng]; coords[node][3];
  * enodes/edofs [E][nn]; kappa/source per element (NULL = 1 / none).
- * vals (CSR values) and src (vector) are accumulated into.  Returns -1 if an
- * entry is missing from the pattern. */
+ * vals (CSR values) and src (vector) are accumulated into, each entry found by
+ * iron's bisection (matrix_vector.F90:1765-1786).  Returns -1 if an entry is
+ * missing from the pattern. */
 int oracle_assemble_scalar3(int E, int nn, int ng, const double* dN, const double* N, const double* w,
                             const double* coords, const int32_t* enodes, const int32_t* edofs,
                             const double* kappa, const double* source, const int32_t* rp, const int32_t* col,
@@ -68,35 +105,8 @@ int oracle_assemble_scalar3(int E, int nn, int ng, const double* dN, const doubl
   double* xe = (double*)malloc(sizeof(double) * nn * 3);
   int bad = 0;
   for (int e = e_begin; e < e_end && e < E; ++e) {
-    memset(Ke, 0, sizeof(double) * nn * nn);
-    memset(se, 0, sizeof(double) * nn);
-    for (int a = 0; a < nn; ++a)
-      for (int d = 0; d < 3; ++d) xe[a * 3 + d] = coords[(int64_t)enodes[(int64_t)e * nn + a] * 3 + d];
+    element_scalar3(e, nn, ng, dN, N, w, coords, enodes, source, Ke, se, xe);
     const double ke = kappa ? kappa[e] : 1.0;
-    for (int g = 0; g < ng; ++g) {
-      double J[3][3] = {{0}};
-      for (int a = 0; a < nn; ++a)
-        for (int j = 0; j < 3; ++j) {
-          const double dn = dN[((int64_t)g * nn + a) * 3 + j];
-          for (int d = 0; d < 3; ++d) J[d][j] += xe[a * 3 + d] * dn;
-        }
-      double GL[3][3], GU[3][3];
-      for (int mi = 0; mi < 3; ++mi)
-        for (int ni = 0; ni < 3; ++ni) GL[mi][ni] = J[0][mi] * J[0][ni] + J[1][mi] * J[1][ni] + J[2][mi] * J[2][ni];
-      const double detgl = inv3(GL, GU);
-      const double rwg = sqrt(fabs(detgl)) * w[g];
-      for (int ms = 0; ms < nn; ++ms) {
-        const double* pm = &dN[((int64_t)g * nn + ms) * 3];
-        for (int ns = 0; ns < nn; ++ns) {
-          const double* pn = &dN[((int64_t)g * nn + ns) * 3];
-          double sum = 0.0;
-          for (int mi = 0; mi < 3; ++mi)
-            for (int ni = 0; ni < 3; ++ni) sum += pm[mi] * pn[ni] * GU[mi][ni];
-          Ke[ms * nn + ns] += sum * rwg;
-        }
-        if (source) se[ms] += source[e] * N[(int64_t)g * nn + ms] * rwg;
-      }
-    }
     for (int ms = 0; ms < nn; ++ms) {
       const int row = edofs[(int64_t)e * nn + ms];
       for (int ns = 0; ns < nn; ++ns) {
@@ -110,6 +120,55 @@ int oracle_assemble_scalar3(int E, int nn, int ng, const double* dN, const doubl
   free(Ke);
   free(se);
   free(xe);
+  return bad ? -1 : 0;
+}
+
+/* As oracle_assemble_scalar3 on `threads` OpenMP threads (elements shared out
+ * dynamically, CSR and vector adds atomic): the all-cores CPU proxy for iron on
+ * several ranks.  The sums equal the serial ones up to the order of additions. */
+int oracle_assemble_scalar3_par(int E, int nn, int ng, const double* dN, const double* N, const double* w,
+                                const double* coords, const int32_t* enodes, const int32_t* edofs,
+                                const double* kappa, const double* source, const int32_t* rp, const int32_t* col,
+                                double* vals, double* src, int e_begin, int e_end, int threads) {
+  const int e_stop = e_end < E ? e_end : E;
+  int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads) reduction(| : bad)
+#endif
+  {
+    double* Ke = (double*)malloc(sizeof(double) * nn * nn);
+    double* se = (double*)malloc(sizeof(double) * nn);
+    double* xe = (double*)malloc(sizeof(double) * nn * 3);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int e = e_begin; e < e_stop; ++e) {
+      element_scalar3(e, nn, ng, dN, N, w, coords, enodes, source, Ke, se, xe);
+      const double ke = kappa ? kappa[e] : 1.0;
+      for (int ms = 0; ms < nn; ++ms) {
+        const int row = edofs[(int64_t)e * nn + ms];
+        for (int ns = 0; ns < nn; ++ns) {
+          const int64_t p = location_find(rp, col, row, edofs[(int64_t)e * nn + ns]);
+          if (p < 0) { bad = 1; continue; }
+          const double v = ke * Ke[ms * nn + ns];
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+          vals[p] += v;
+        }
+        if (source && src) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+          src[row] += se[ms];
+        }
+      }
+    }
+    free(Ke);
+    free(se);
+    free(xe);
+  }
+  (void)threads;
   return bad ? -1 : 0;
 }
 
