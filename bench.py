@@ -158,8 +158,8 @@ def timed_steps(P, k, mixed, inner_rtol):
 
 
 def host_threads():
-    """Host threads for the all-cores CPU baseline: OMP_NUM_THREADS when set (16 on a
-    one-GPU share of the box), else the CPUs this process may run on."""
+    """Host threads for the all-cores CPU baseline: OMP_NUM_THREADS when set (the
+    host-core share given to this GPU), else the CPUs this process may run on."""
     try:
         return max(1, int(os.environ["OMP_NUM_THREADS"]))
     except (KeyError, ValueError):
