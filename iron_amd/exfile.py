@@ -1,5 +1,7 @@
 """Reader and writer for the CMISS exnode/exelem text formats that iron's
-Fields_NodesExport / Fields_ElementsExport write (field_IO_routines.F90).
+Fields_NodesExport / Fields_ElementsExport write (field_IO_routines.F90), and a
+reader for the analytic-analysis (.opanal) reports of
+analytic_analysis_routines.F90.
 
 Only what regular Lagrange meshes produce is handled: node-based components with
 derivatives and versions, and element "grid based" components (element-constant
@@ -140,3 +142,29 @@ def write_exnode(path, group: str, fields: list, values: dict):
             for v in comp:
                 lines.append(f"  {v: .16E}")
     Path(path).write_text("\n".join(lines) + "\n")
+
+
+def read_opanal(path) -> dict:
+    """Nodal blocks of an analytic-analysis report:
+    {variable label: {node: [numerical, analytic]}} (the "Nodal errors" table that
+    AnalyticAnalysis_Output writes per field variable, one row per node:
+    node, derivative, numerical, analytic, errors...)."""
+    out, var = {}, None
+    in_nodal = False
+    for line in Path(path).read_text().splitlines():
+        m = re.match(r"\s*Variable\s+\d+\s*:\s*(.+)$", line)
+        if m:
+            var = m.group(1).strip()
+            out[var] = {}
+            in_nodal = False
+            continue
+        if line.strip().startswith("Nodal errors"):
+            in_nodal = True
+            continue
+        if in_nodal and var is not None:
+            parts = line.split()
+            if len(parts) >= 4 and parts[0].isdigit():
+                out[var][int(parts[0])] = [float(parts[2]), float(parts[3])]
+            elif line.strip().startswith("RMS"):
+                in_nodal = False
+    return out
