@@ -314,6 +314,102 @@ __global__ __launch_bounds__(64 * WPB) void elastic_assemble_kernel(
   }
 }
 
+// Trilinear elasticity on MFMA: K[(i,a)][(j,b)] = sum_g sum_L U[(i,j,a)][(g,L)]
+// B[g][b][L], one (9 NN) x (3 NG) by (3 NG) x NN GEMM per element (one
+// wavefront per element, 16x16x4 tiles).  U carries iron's per-block
+// coefficients: U[(c,c,a)][(g,L)] = w diagC[c][L] B[g][a][L] on the diagonal
+// blocks and, for i != j, U[(i,j,a)][(g,L)] = w (Ca B[g][a][i] [L == j] +
+// Cb B[g][a][j] [L == i]) with (Ca, Cb) of the pair {i, j}
+// (linear_elasticity_routines.F90:1255-1286; the lower blocks are the transposes).
+template <typename T, int NN, int NG, int WPB>
+__global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
+    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const double* __restrict__ material,
+    const int32_t* __restrict__ pos, const int32_t* __restrict__ order, int elem_begin, int elem_count,
+    const double* __restrict__ dN, const double* __restrict__ w, T* __restrict__ vals, bool atomic) {
+  constexpr int ND = 3 * NN, MROWS = 9 * NN, KD = 3 * NG;
+  constexpr int MT = (MROWS + 15) / 16, STEPS = (KD + 3) / 4;
+  static_assert(NN <= 16, "one column tile");
+  using M = Mfma16<T>;
+  __shared__ GeomLDS<3, NN, NG> geo[WPB];
+  __shared__ T sB[WPB][NG][NN][3];
+  __shared__ T sDiag[WPB][NG][9];  // w * diagC[c][d]
+  __shared__ T sOff[WPB][NG][6];   // w * (Ca, Cb) of the pairs (0,1), (0,2), (1,2)
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int idx = blockIdx.x * WPB + wv;
+  const bool active = idx < elem_count;
+  long long e = 0;
+  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
+  GeomLDS<3, NN, NG>& G = geo[wv];
+  element_geometry<3, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
+  if (active) {
+    for (int t = lane; t < NG * NN; t += 64) {
+      const int g = t / NN, a = t - (t / NN) * NN;
+      // dphidx(a,mi) = sum_ni DXI_DX(mi,ni) dphi/dxi_ni   (linear_elasticity_routines.F90:1245-1252)
+#pragma unroll
+      for (int mi = 0; mi < 3; ++mi) {
+        double b = 0.0;
+#pragma unroll
+        for (int ni = 0; ni < 3; ++ni) b += G.ji[g][mi][ni] * dN[(g * NN + a) * 3 + ni];
+        sB[wv][g][a][mi] = (T)b;
+      }
+    }
+    if (lane < NG) {
+      const int g = lane;
+      const double* mat = material + e * 6;
+      const double E1 = mat[0], E2 = mat[1], E3 = mat[2], v13 = mat[3], v23 = mat[4], v12 = mat[5];
+      const double v31 = v13, v32 = v23, v21 = v12;
+      const double gama = 1.0 / (1.0 - v12 * v21 - v23 * v32 - v31 * v13 - 2.0 * v21 * v32 * v13);
+      const double C11 = E1 * (1.0 - v23 * v32) * gama, C22 = E2 * (1.0 - v13 * v31) * gama;
+      const double C33 = E3 * (1.0 - v12 * v21) * gama;
+      const double C12 = E1 * (v21 + v31 * v23) * gama, C13 = E1 * (v31 + v21 * v32) * gama;
+      const double C23 = E2 * (v32 + v12 * v31) * gama;
+      const double C44 = E2 / (2.0 * (1.0 + v23)), C55 = E1 / (2.0 * (1.0 + v13)), C66 = E3 / (2.0 * (1.0 + v12));
+      const double jr = G.rwg[g];
+      const double diag[9] = {C11, C66, C44, C66, C22, C55, C44, C55, C33};
+      for (int k = 0; k < 9; ++k) sDiag[wv][g][k] = (T)(jr * diag[k]);
+      const double off[6] = {C12, C66, C13, C44, C23, C55};
+      for (int k = 0; k < 6; ++k) sOff[wv][g][k] = (T)(jr * off[k]);
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int32_t* pe = pos + e * (ND * ND);
+  const int col = lane & 15;
+  for (int mt = 0; mt < MT; ++mt) {
+    const int mrow = 16 * mt + col;
+    const bool rv = mrow < MROWS;
+    const int i = rv ? mrow / (3 * NN) : 0, j = rv ? (mrow / NN) % 3 : 0, a = rv ? mrow % NN : 0;
+    const int blk = (i + j == 1) ? 0 : ((i + j == 2) ? 1 : 2);  // pairs (0,1), (0,2), (1,2)
+    typename M::v4 acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int st = 0; st < STEPS; ++st) {
+      const int k = 4 * st + (lane >> 4);
+      const bool kv = k < KD;
+      const int g = kv ? k / 3 : 0, L = k - 3 * (k / 3);
+      T u = 0, bop = 0;
+      if (kv) {
+        if (rv) {
+          const T* Bg = &sB[wv][g][a][0];
+          if (i == j)
+            u = sDiag[wv][g][i * 3 + L] * Bg[L];
+          else
+            u = (L == j ? sOff[wv][g][2 * blk] * Bg[i] : (T)0) + (L == i ? sOff[wv][g][2 * blk + 1] * Bg[j] : (T)0);
+        }
+        if (col < NN) bop = sB[wv][g][col][L];
+      }
+      acc = M::mac(u, bop, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int orow = 16 * mt + M::row(lane, r);
+      if (orow < MROWS && col < NN) {
+        const int oi = orow / (3 * NN), oj = (orow / NN) % 3, oa = orow % NN;
+        add_value(&vals[pe[(oi * NN + oa) * ND + oj * NN + col]], acc[r], atomic);
+      }
+    }
+  }
+}
+
 template <int DIM, int NN, int NG, typename T>
 static int launch_scalar(const mife_element_desc* d, int n_elem, const double* coords, const int32_t* enodes,
                          const int32_t* edofs, const double* kappa, const double* source, const int32_t* pos,
@@ -349,16 +445,23 @@ static int launch_elastic(const mife_element_desc* d, int n_elem, const double* 
                           const double* material, const int32_t* pos, const int32_t* order, const int32_t* cptr,
                           int ncol, T* vals, hipStream_t s) {
   constexpr int WPB = 2;
+  constexpr bool kMfma = NN <= 16;  // one MFMA column tile per element
+  constexpr int WQ = 4;             // elements (wavefronts) per block of the MFMA kernel
   const int tk = prof::begin(s);
+  auto launch = [&](const int32_t* ord, int begin, int count, bool atomic) {
+    if constexpr (kMfma)
+      elastic_assemble_mfma_kernel<T, NN, NG, WQ><<<div_up(count, WQ), 64 * WQ, 0, s>>>(
+          coords, enodes, material, pos, ord, begin, count, d->dN, d->w, vals, atomic);
+    else
+      elastic_assemble_kernel<NN, NG, T, WPB><<<div_up(count, WPB), 64 * WPB, 0, s>>>(
+          coords, enodes, material, pos, ord, begin, count, d->dN, d->w, vals, atomic);
+  };
   if (!order) {
-    elastic_assemble_kernel<NN, NG, T, WPB><<<div_up(n_elem, WPB), 64 * WPB, 0, s>>>(
-        coords, enodes, material, pos, nullptr, 0, n_elem, d->dN, d->w, vals, true);
+    launch(nullptr, 0, n_elem, true);
   } else {
     for (int c = 0; c < ncol; ++c) {
       const int cnt = cptr[c + 1] - cptr[c];
-      if (cnt <= 0) continue;
-      elastic_assemble_kernel<NN, NG, T, WPB><<<div_up(cnt, WPB), 64 * WPB, 0, s>>>(
-          coords, enodes, material, pos, order, cptr[c], cnt, d->dN, d->w, vals, false);
+      if (cnt > 0) launch(order, cptr[c], cnt, false);
     }
   }
   prof::end(tk, s, sizeof(T) == 8 ? prof::kAssembleElastic : prof::kAssembleElastic32);

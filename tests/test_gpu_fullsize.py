@@ -8,8 +8,9 @@ properties (the oracle is too slow at these sizes):
   assemblies agree, and the reactions balance the source (rows of K sum to zero,
   so sum of (K u - f) over all DOFs is -int f dV = -1).
 * 96x32x32 trilinear elasticity on [0,3]x[0,1]^2, lognormal Young's modulus, clamp at
-  x = 0, consistent traction t_z = -1 on x = 3: relative residual <= 1e-8 and the clamp
-  reactions balance the applied force in every direction.
+  x = 0, consistent traction t_z = -1 on x = 3: FP32 element stiffness (MFMA) in the
+  inner CG, FP64 refinement to a relative residual <= 1e-8, and the clamp reactions
+  balance the applied force in every direction.
 """
 import numpy as np
 import pytest
@@ -99,7 +100,11 @@ def test_c3_elastic96_equilibrium():
     mat = np.stack([Ee, Ee, Ee, np.full(E, 0.3), np.full(E, 0.3), np.full(E, 0.3)], axis=1)
     asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], csr)
     v = csr.zeros()
-    asm.assemble(v, torch.as_tensor(np.ascontiguousarray(mat), device="cuda"))
+    mt = torch.as_tensor(np.ascontiguousarray(mat), device="cuda")
+    asm.assemble(v, mt)
+    v32 = csr.zeros(fem.F32)
+    asm.assemble(v32, mt, prec=fem.F32)
+    assert float((v32.double() - v).abs().max()) <= 2e-6 * float(v.abs().max())
     clamp = X[:, 0] == 0.0
     fixed = np.concatenate([clamp, clamp, clamp])
     tip = np.nonzero(X[:, 0] == 3.0)[0]
@@ -112,7 +117,8 @@ def test_c3_elastic96_equilibrium():
     ft = torch.as_tensor(f, device="cuda")
     bvec = red.rhs(v, u, rhs=ft)
     x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
-    info = fem.cg(S, S.gather(v), bvec, x, 1e-8, 200000, fem.KrylovWorkspace(red.n_free, "cuda"))
+    info = fem.refine(S, S.gather(v), S.gather(v32, fem.F32), bvec, x, 1e-8, 200000,
+                      fem.KrylovWorkspace(red.n_free, "cuda"))
     assert info.relres <= 1e-8
     red.field_update(x, u)
     R = fem.backsubstitute(csr, v, u).cpu().numpy() - f
