@@ -354,7 +354,8 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b, doubl
  * 16-bit offsets at nz_off of the 16-bit pool plus n_slices base columns at
  * cb_off; the system's vectors are then in slice numbering.  Vectors live at
  * row_off of the vector pools.  slot (0 <= slot < max_segments) names the
- * segment's persistent CG state. */
+ * segment's persistent CG state.  prec: MIFE_F64 (FP64 CG) or MIFE_F32 (FP32 CG
+ * with FP64 reliable updates). */
 typedef struct mife_segment {
   int32_t n;
   int32_t maxit;
@@ -365,50 +366,63 @@ typedef struct mife_segment {
   int64_t nz_off;
   double rtol;
   int32_t n_slices; /* SELL: ceil(n / 64) */
-  int32_t pad;
+  int32_t prec;     /* MIFE_F64 or MIFE_F32 */
   int64_t cb_off;   /* SELL: offset of the per-slice base columns (16-bit columns), or -1 (int32 columns) */
 } mife_segment;
 
 /* Shared storage of the grouped segments: row / slice pointers, int32 columns,
- * 16-bit slice-relative columns with their per-slice base columns, values. */
+ * 16-bit slice-relative columns with their per-slice base columns, FP64 values
+ * and their FP32 copy at the same offsets (read by MIFE_F32 segments only). */
 typedef struct mife_pools {
   const int32_t* rp;
   const int32_t* col;
   const uint16_t* col16;
   const int32_t* cbase;
   const double* val;
+  const float* val32;
 } mife_pools;
 
 /* Grouped Jacobi CG: every launch serves all segments of a table; each segment
- * has its own scalars, its own fixed-order reductions and its own stopping test
- * ||r|| <= rtol ||b|| (recurrence residual), so a system's iterates do not
- * depend on which other systems share the launches.  No reference counterpart
- * (iron solves one system per KSPSolve, src/solver_routines.F90:11869); it is
- * the batching layer of the serving path.
+ * has its own scalars, its own fixed-order reductions and its own stopping
+ * test, so a system's iterates do not depend on which other systems share the
+ * launches.  No reference counterpart (iron solves one system per KSPSolve,
+ * src/solver_routines.F90:11869); it is the batching layer of the serving path.
+ * Per segment precision:
+ *   MIFE_F64: FP64 CG, done when the recurrence residual ||r|| <= rtol ||b||.
+ *   MIFE_F32: FP32 CG on the FP32 values with FP64 reliable updates (the
+ *     algorithm of mife_refine): every check_every iterations (and after the
+ *     last of a call) a segment whose FP32 r.r has fallen by inner_rtol^2 since
+ *     its last replacement (or below a quarter of its target, or at maxit) folds
+ *     x += x32 and replaces r by the FP64 residual b - A64 x; it is done when
+ *     that true residual meets rtol (or at maxit).
  *   mife_grouped_state_bytes / _work_bytes: device buffers for the per-slot
  *     state and for one table (max_blocks from the sum of mife_grouped_blocks(n)).
  *   mife_grouped_setup: builds the table of n_seg segments (host staging buffer
  *     of _work_bytes), synchronises the stream.
- *   mife_grouped_cg_init: Jacobi diagonal, r = b - A x, z, p = q = 0 and the
- *     state of the table's segments.
- *   mife_grouped_cg_iterate: `iterations` CG iterations of every unconverged
+ *   mife_grouped_cg_init: Jacobi diagonal, r = b - A x (FP64), z, p = q = 0 (and
+ *     x32 = 0) and the state of the table's segments.
+ *   mife_grouped_cg_iterate: `iterations` iterations of every unconverged
  *     segment; launches only (no allocation, memset or synchronisation), so the
  *     call can be captured in a HIP graph and replayed.
  *   mife_grouped_cg_residual: true ||b - A x|| of the table's segments.
  *   mife_grouped_cg_status: host copy of iterations, done flags and relative
- *     residuals of all max_segments slots (synchronises).
- * Vector pools: b, x [pool_rows] and vec5 [5][pool_rows] (D^-1, r, z, p, q). */
+ *     residuals of all max_segments slots (synchronises); the relative residual
+ *     is the last true one where one was computed, else the recurrence's.
+ * Vector pools: b, x [pool_rows] (FP64; x holds the solution), vec5
+ * [5][pool_rows] (FP64 D^-1, r, z, p, q of MIFE_F64 segments) and vec6
+ * [6][pool_rows] (FP32 D^-1, r, z, p, q, x32 of MIFE_F32 segments). */
 size_t mife_grouped_state_bytes(int max_segments);
 size_t mife_grouped_work_bytes(int max_segments, int max_blocks);
 int mife_grouped_blocks(int n_rows);
 int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segments, int max_blocks,
                        int* n_blocks_out, void* table_host, void* work, void* stream);
 int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
-                         const double* b, double* x, double* vec5, int64_t pool_rows, void* work,
-                         void* state, void* stream);
+                         const double* b, const double* x, double* vec5, float* vec6, int64_t pool_rows,
+                         void* work, void* state, void* stream);
 int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
-                            double* x, double* vec5, int64_t pool_rows, int iterations, void* work,
-                            void* state, void* stream);
+                            const double* b, double* x, double* vec5, float* vec6, int64_t pool_rows,
+                            int iterations, int check_every, double inner_rtol, void* work, void* state,
+                            void* stream);
 int mife_grouped_cg_residual(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
                              const double* b, const double* x, void* work, void* state, void* stream);
 int mife_grouped_cg_status(int max_segments, const void* state, int* iters, int* done, double* relres,

@@ -46,13 +46,13 @@ class Segment(C.Structure):
     """mife_segment: one system of a grouped solve."""
     _fields_ = [("n", C.c_int32), ("maxit", C.c_int32), ("slot", C.c_int32), ("format", C.c_int32),
                 ("row_off", C.c_int64), ("rp_off", C.c_int64), ("nz_off", C.c_int64), ("rtol", C.c_double),
-                ("n_slices", C.c_int32), ("pad", C.c_int32), ("cb_off", C.c_int64)]
+                ("n_slices", C.c_int32), ("prec", C.c_int32), ("cb_off", C.c_int64)]
 
 
 class Pools(C.Structure):
     """mife_pools: shared storage of grouped segments."""
     _fields_ = [("rp", C.c_void_p), ("col", C.c_void_p), ("col16", C.c_void_p), ("cbase", C.c_void_p),
-                ("val", C.c_void_p)]
+                ("val", C.c_void_p), ("val32", C.c_void_p)]
 
 
 class ElementDesc(C.Structure):
@@ -106,8 +106,9 @@ _SIG = {
     "mife_grouped_work_bytes": (C.c_size_t, [_I, _I]),
     "mife_grouped_blocks": (_I, [_I]),
     "mife_grouped_setup": (_I, [_P, _I, _I, _I, C.POINTER(_I), _P, _P, _P]),
-    "mife_grouped_cg_init": (_I, [_I, _I, _I, _I, C.POINTER(Pools), _P, _P, _P, _I64, _P, _P, _P]),
-    "mife_grouped_cg_iterate": (_I, [_I, _I, _I, _I, C.POINTER(Pools), _P, _P, _I64, _I, _P, _P, _P]),
+    "mife_grouped_cg_init": (_I, [_I, _I, _I, _I, C.POINTER(Pools), _P, _P, _P, _P, _I64, _P, _P, _P]),
+    "mife_grouped_cg_iterate": (_I, [_I, _I, _I, _I, C.POINTER(Pools), _P, _P, _P, _P, _I64, _I, _I, _D, _P, _P,
+                                     _P]),
     "mife_grouped_cg_residual": (_I, [_I, _I, _I, _I, C.POINTER(Pools), _P, _P, _P, _P, _P]),
     "mife_grouped_cg_status": (_I, [_I, _P, _P, _P, _P, _P]),
     "mife_profile_enable": (_I, [_I]),

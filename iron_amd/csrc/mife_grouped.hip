@@ -13,6 +13,19 @@
 // segment's CG scalars.  Converged segments stop at once: their blocks return
 // on entry.  Nothing here allocates, synchronises or issues memsets, so a
 // chunk of iterations can be captured in a HIP graph and replayed.
+//
+// Precision per segment (mife_segment.prec).  MIFE_F64 segments iterate in
+// FP64 throughout.  MIFE_F32 segments iterate in FP32 (FP32 values, vectors
+// and products, FP64 scalars) with reliable updates in FP64, as the
+// single-solve mife_refine: the FP64 solution x accumulates the FP32
+// correction x32, and whenever a segment's FP32 residual has fallen by
+// inner_rtol since its last replacement (tested every check_every iterations),
+// x += x32 and the FP32 residual is replaced by the true residual b - A64 x
+// computed with the FP64 values; p and q = A p are kept and
+// beta = (r.z)_replaced / (r.z)_before-the-step.  Such a segment is done when
+// its true FP64 residual meets its tolerance (or at its iteration limit); the
+// FP32 recurrence never stops it by itself.  Both kinds share every launch: a
+// block takes the branch of its segment's precision.
 #include "mife_common.hpp"
 #include "mife_profile.hpp"
 #include "mife_sparse.hpp"
@@ -31,8 +44,23 @@ constexpr int kSlicesPerBlock = 4;     // SELL: one slice per wavefront
 
 struct SegState {
   double rz, pq, alpha, beta, rr, bb, tol2, rr_true;
-  int iters, maxit, done, pad;
+  double rz_prev;  // r.z before the last step
+  double rr_rep;   // mixed precision: FP32 r.r right after the last replacement
+  int iters, maxit, done;
+  int pause;       // mixed precision: waiting for an FP64 replacement
+  int mixed;       // the segment iterates in FP32 with FP64 reliable updates
+  int pad;
 };
+
+// The vector pools: FP64 segments use the first set, FP32 segments the second
+// (both at the segment's row_off).
+struct Vecs {
+  double *dinv, *r, *z, *p, *q;
+  float *dinv32, *r32, *z32, *p32, *q32, *x32;
+};
+
+constexpr int kNumParts = 4;      // partial values per block slot
+constexpr int kReplaceGrid = 2048;  // grid of the (mostly idle) replacement passes
 
 struct Block {
   int32_t seg;
@@ -49,20 +77,23 @@ struct View {
   const uint16_t* col16;
   const int32_t* cbase;
   const double* val;
+  const float* val32;  // FP32 copy of the values (mixed precision), or null
   SegState* st;
-  double* parts;  // 3 values per partial slot: parts[v * cap + slot]
+  double* parts;  // kNumParts values per partial slot: parts[v * cap + slot]
   int cap;
 };
 
-// y_row = (A x)_row for the rows of this block's segment, 8 lanes per row.
-__device__ __forceinline__ double row_dot(const View& V, const mife_segment& S, const double* __restrict__ x,
-                                          int row, int sub) {
+// y_row = (A x)_row for the rows of this block's segment, 8 lanes per row;
+// vpool is the value pool of the precision T.
+template <typename T>
+__device__ __forceinline__ T row_dot(const View& V, const mife_segment& S, const T* __restrict__ vpool,
+                                     const T* __restrict__ x, int row, int sub) {
   const int32_t* rp = V.rp + S.rp_off;
   const int32_t* col = V.col + S.nz_off;
-  const double* val = V.val + S.nz_off;
-  const double* xs = x + S.row_off;
+  const T* val = vpool + S.nz_off;
+  const T* xs = x + S.row_off;
   const int k0 = rp[row], k1 = rp[row + 1];
-  double a = 0.0;
+  T a = 0;
   for (int k = k0 + sub; k < k1; k += kL) a += val[k] * xs[col[k]];
 #pragma unroll
   for (int off = kL / 2; off > 0; off >>= 1) a += __shfl_xor(a, off, kL);
@@ -70,24 +101,25 @@ __device__ __forceinline__ double row_dot(const View& V, const mife_segment& S, 
 }
 
 // SELL: (A x) of lane `lane`'s row of slice s of the segment (complete sum).
-__device__ __forceinline__ double slice_dot(const View& V, const mife_segment& S, const double* __restrict__ x,
-                                            int s, int lane) {
+template <typename T>
+__device__ __forceinline__ T slice_dot(const View& V, const mife_segment& S, const T* __restrict__ vpool,
+                                       const T* __restrict__ x, int s, int lane) {
   const int32_t* sp = V.rp + S.rp_off;
-  const double* val = V.val + S.nz_off;
-  const double* xs = x + S.row_off;
-  if (S.cb_off >= 0) return sell16_row<double>(sp, V.col16 + S.nz_off, V.cbase + S.cb_off, val, xs, s, lane);
-  return sell_row<double>(sp, V.col + S.nz_off, val, xs, s, lane);
+  const T* val = vpool + S.nz_off;
+  const T* xs = x + S.row_off;
+  if (S.cb_off >= 0) return sell16_row<T>(sp, V.col16 + S.nz_off, V.cbase + S.cb_off, val, xs, s, lane);
+  return sell_row<T>(sp, V.col + S.nz_off, val, xs, s, lane);
 }
 
 // Calls f(row, (A x)_row) for every row of block B (by one lane per row).
-template <typename F>
+template <typename T, typename F>
 __device__ __forceinline__ void block_rows(const View& V, const Block& B, const mife_segment& S,
-                                           const double* __restrict__ x, F&& f) {
+                                           const T* __restrict__ vpool, const T* __restrict__ x, F&& f) {
   if (S.format == MIFE_FMT_SELL) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (wave < B.rows) {
       const int s = B.row0 + wave;
-      const double ax = slice_dot(V, S, x, s, lane);
+      const T ax = slice_dot<T>(V, S, vpool, x, s, lane);
       const int row = s * kSlice + lane;
       if (row < S.n) f(row, ax);
     }
@@ -95,7 +127,7 @@ __device__ __forceinline__ void block_rows(const View& V, const Block& B, const 
   }
   const int lr = threadIdx.x / kL, sub = threadIdx.x % kL;
   for (int row = B.row0 + lr; row < B.row0 + B.rows; row += kRowsPerBlock) {
-    const double ax = row_dot(V, S, x, row, sub);
+    const T ax = row_dot<T>(V, S, vpool, x, row, sub);
     if (sub == 0) f(row, ax);
   }
 }
@@ -130,21 +162,21 @@ __device__ __forceinline__ void store_partials(const View& V, const Block& B, do
 
 using namespace grouped;
 
-__global__ __launch_bounds__(256) void gcg_diag_kernel(View V, double* __restrict__ dinv) {
-  const Block B = V.blocks[blockIdx.x];
-  const mife_segment S = V.segs[B.seg];
+template <typename T>
+__device__ __forceinline__ void diag_body(const View& V, const Block& B, const mife_segment& S,
+                                          const T* __restrict__ vpool, T* __restrict__ dinv) {
   int r0, r1;
   block_row_range(B, S, r0, r1);
   const int row = r0 + (int)threadIdx.x;
   if (row >= r1) return;
-  double d = 0.0;
+  T d = 0;
+  const T* val = vpool + S.nz_off;
   if (S.format == MIFE_FMT_SELL) {
     const int32_t* sp = V.rp + S.rp_off;
     const int s = row / kSlice, lane = row - s * kSlice;
-    const double* val = V.val + S.nz_off;
     for (int e = sp[s] + lane; e < sp[s + 1]; e += kSlice) {
       const int c = S.cb_off >= 0 ? V.cbase[S.cb_off + s] + (int)V.col16[S.nz_off + e] : V.col[S.nz_off + e];
-      if (c == row && val[e] != 0.0) {
+      if (c == row && val[e] != (T)0) {
         d = val[e];
         break;
       }
@@ -152,34 +184,61 @@ __global__ __launch_bounds__(256) void gcg_diag_kernel(View V, double* __restric
   } else {
     const int32_t* rp = V.rp + S.rp_off;
     const int32_t* col = V.col + S.nz_off;
-    const double* val = V.val + S.nz_off;
     for (int k = rp[row]; k < rp[row + 1]; ++k)
       if (col[k] == row) d = val[k];
   }
-  dinv[S.row_off + row] = d != 0.0 ? 1.0 / d : 1.0;
+  dinv[S.row_off + row] = d != (T)0 ? (T)1 / d : (T)1;
 }
 
-// r = b - A x, z = D^-1 r, p = q = 0; partials r.z, r.r, b.b
-__global__ __launch_bounds__(256) void gcg_init_kernel(View V, const double* __restrict__ b,
-                                                       const double* __restrict__ x, const double* __restrict__ dinv,
-                                                       double* __restrict__ r, double* __restrict__ z,
-                                                       double* __restrict__ p, double* __restrict__ q) {
+// Jacobi diagonal D^-1 of the segment's precision
+__global__ __launch_bounds__(256) void gcg_diag_kernel(View V, Vecs W) {
   const Block B = V.blocks[blockIdx.x];
   const mife_segment S = V.segs[B.seg];
-  double m[3] = {0.0, 0.0, 0.0};
-  block_rows(V, B, S, x, [&](int row, double ax) {
-    const long long g = S.row_off + row;
-    const double ri = b[g] - ax;
-    const double zi = dinv[g] * ri;
-    r[g] = ri;
-    z[g] = zi;
-    p[g] = 0.0;
-    q[g] = 0.0;
-    m[0] += ri * zi;
-    m[1] += ri * ri;
-    m[2] += b[g] * b[g];
-  });
-  store_partials<3>(V, B, m);
+  if (S.prec == MIFE_F32)
+    diag_body<float>(V, B, S, V.val32, W.dinv32);
+  else
+    diag_body<double>(V, B, S, V.val, W.dinv);
+}
+
+// FP64 segment: r = b - A x, z = D^-1 r, p = q = 0.
+// FP32 segment: r64 = b - A64 x, r32 = r64, z32 = D^-1 r32, p32 = q32 = x32 = 0.
+// Partials r.z, r.r, b.b and (FP32 segments) r64.r64.
+__global__ __launch_bounds__(256) void gcg_init_kernel(View V, Vecs W, const double* __restrict__ b,
+                                                       const double* __restrict__ x) {
+  const Block B = V.blocks[blockIdx.x];
+  const mife_segment S = V.segs[B.seg];
+  double m[kNumParts] = {0.0, 0.0, 0.0, 0.0};
+  if (S.prec == MIFE_F32) {
+    block_rows<double>(V, B, S, V.val, x, [&](int row, double ax) {
+      const long long g = S.row_off + row;
+      const double r64 = b[g] - ax;
+      const float ri = (float)r64;
+      const float zi = W.dinv32[g] * ri;
+      W.r32[g] = ri;
+      W.z32[g] = zi;
+      W.p32[g] = 0.0f;
+      W.q32[g] = 0.0f;
+      W.x32[g] = 0.0f;
+      m[0] += (double)ri * (double)zi;
+      m[1] += (double)ri * (double)ri;
+      m[2] += b[g] * b[g];
+      m[3] += r64 * r64;
+    });
+  } else {
+    block_rows<double>(V, B, S, V.val, x, [&](int row, double ax) {
+      const long long g = S.row_off + row;
+      const double ri = b[g] - ax;
+      const double zi = W.dinv[g] * ri;
+      W.r[g] = ri;
+      W.z[g] = zi;
+      W.p[g] = 0.0;
+      W.q[g] = 0.0;
+      m[0] += ri * zi;
+      m[1] += ri * ri;
+      m[2] += b[g] * b[g];
+    });
+  }
+  store_partials<kNumParts>(V, B, m);
 }
 
 // one block per segment: fixed-order sums of the segment's partial slots
@@ -204,48 +263,64 @@ __device__ __forceinline__ void seg_sums(const View& V, const int32_t* __restric
 __global__ __launch_bounds__(256) void gfin_init_kernel(View V, const int32_t* __restrict__ seg_parts,
                                                         const int32_t* __restrict__ seg_ids) {
   const int s = seg_ids[blockIdx.x];
-  double out[3];
-  seg_sums<3>(V, seg_parts, blockIdx.x, out);
+  double out[kNumParts];
+  seg_sums<kNumParts>(V, seg_parts, blockIdx.x, out);
   if (threadIdx.x == 0) {
     SegState& g = V.st[s];
     const mife_segment S = V.segs[s];
+    const bool mixed = S.prec == MIFE_F32;
     g.rz = out[0];
     g.rr = out[1];
     g.bb = out[2];
     g.beta = 0.0;
     g.alpha = 0.0;
     g.pq = 0.0;
+    g.rz_prev = 0.0;
+    g.rr_rep = out[1];
     g.iters = 0;
     g.maxit = S.maxit;
     g.tol2 = S.rtol * S.rtol;
-    g.rr_true = -1.0;
-    g.done = (out[1] <= g.tol2 * out[2] || S.maxit <= 0) ? 1 : 0;
+    g.rr_true = mixed ? out[3] : -1.0;
+    g.pause = 0;
+    g.mixed = mixed ? 1 : 0;
+    g.done = ((mixed ? out[3] : out[1]) <= g.tol2 * out[2] || S.maxit <= 0) ? 1 : 0;
   }
 }
 
-// p = z + beta p ; q = A z + beta q ; partial p.q
-__global__ __launch_bounds__(256) void gcg_spmv_kernel(View V, const double* __restrict__ z, double* __restrict__ p,
-                                                       double* __restrict__ q) {
-  const Block B = V.blocks[blockIdx.x];
-  if (V.st[B.seg].done) return;
-  const mife_segment S = V.segs[B.seg];
-  const double beta = V.st[B.seg].beta;
+__device__ __forceinline__ bool seg_idle(const SegState& g) { return g.done || g.pause; }
+
+template <typename T>
+__device__ __forceinline__ void spmv_body(const View& V, const Block& B, const mife_segment& S,
+                                          const T* __restrict__ vpool, const T* __restrict__ z,
+                                          T* __restrict__ p, T* __restrict__ q) {
+  const T beta = (T)V.st[B.seg].beta;
   double m[1] = {0.0};
-  block_rows(V, B, S, z, [&](int row, double az) {
+  block_rows<T>(V, B, S, vpool, z, [&](int row, T az) {
     const long long g = S.row_off + row;
-    const double pi = z[g] + beta * p[g];
-    const double qi = az + beta * q[g];
+    const T pi = z[g] + beta * p[g];
+    const T qi = az + beta * q[g];
     p[g] = pi;
     q[g] = qi;
-    m[0] += pi * qi;
+    m[0] += (double)pi * (double)qi;
   });
   store_partials<1>(V, B, m);
+}
+
+// p = z + beta p ; q = A z + beta q ; partial p.q (in the segment's precision)
+__global__ __launch_bounds__(256) void gcg_spmv_kernel(View V, Vecs W) {
+  const Block B = V.blocks[blockIdx.x];
+  if (seg_idle(V.st[B.seg])) return;
+  const mife_segment S = V.segs[B.seg];
+  if (S.prec == MIFE_F32)
+    spmv_body<float>(V, B, S, V.val32, W.z32, W.p32, W.q32);
+  else
+    spmv_body<double>(V, B, S, V.val, W.z, W.p, W.q);
 }
 
 __global__ __launch_bounds__(256) void gfin_alpha_kernel(View V, const int32_t* __restrict__ seg_parts,
                                                          const int32_t* __restrict__ seg_ids) {
   const int s = seg_ids[blockIdx.x];
-  if (V.st[s].done) return;
+  if (seg_idle(V.st[s])) return;
   double out[1];
   seg_sums<1>(V, seg_parts, blockIdx.x, out);
   if (threadIdx.x == 0) {
@@ -255,44 +330,128 @@ __global__ __launch_bounds__(256) void gfin_alpha_kernel(View V, const int32_t* 
   }
 }
 
-// x += alpha p ; r -= alpha q ; z = D^-1 r ; partials r.z, r.r
-__global__ __launch_bounds__(256) void gcg_update_kernel(View V, double* __restrict__ x, double* __restrict__ r,
-                                                         double* __restrict__ z, const double* __restrict__ p,
-                                                         const double* __restrict__ q,
-                                                         const double* __restrict__ dinv) {
-  const Block B = V.blocks[blockIdx.x];
-  if (V.st[B.seg].done) return;
-  const mife_segment S = V.segs[B.seg];
-  const double alpha = V.st[B.seg].alpha;
+template <typename T>
+__device__ __forceinline__ void update_body(const View& V, const Block& B, const mife_segment& S, T* __restrict__ x,
+                                            T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+                                            const T* __restrict__ q, const T* __restrict__ dinv) {
+  const T alpha = (T)V.st[B.seg].alpha;
   double m[2] = {0.0, 0.0};
   int r0, r1;
   block_row_range(B, S, r0, r1);
   for (int row = r0 + threadIdx.x; row < r1; row += 256) {
     const long long g = S.row_off + row;
     x[g] += alpha * p[g];
-    const double ri = r[g] - alpha * q[g];
-    const double zi = dinv[g] * ri;
+    const T ri = r[g] - alpha * q[g];
+    const T zi = dinv[g] * ri;
     r[g] = ri;
     z[g] = zi;
-    m[0] += ri * zi;
-    m[1] += ri * ri;
+    m[0] += (double)ri * (double)zi;
+    m[1] += (double)ri * (double)ri;
   }
   store_partials<2>(V, B, m);
 }
 
+// x += alpha p ; r -= alpha q ; z = D^-1 r ; partials r.z, r.r (x: the FP64
+// solution of FP64 segments; FP32 segments update their correction x32)
+__global__ __launch_bounds__(256) void gcg_update_kernel(View V, Vecs W, double* __restrict__ x) {
+  const Block B = V.blocks[blockIdx.x];
+  if (seg_idle(V.st[B.seg])) return;
+  const mife_segment S = V.segs[B.seg];
+  if (S.prec == MIFE_F32)
+    update_body<float>(V, B, S, W.x32, W.r32, W.z32, W.p32, W.q32, W.dinv32);
+  else
+    update_body<double>(V, B, S, x, W.r, W.z, W.p, W.q, W.dinv);
+}
+
+// FP64 segments: done when the recurrence residual meets the tolerance or at
+// maxit.  FP32 segments pause for a replacement at maxit and, on a CHECK
+// iteration, once r.r has fallen by delta2 since the last replacement or
+// below a quarter of the target.
+template <bool CHECK>
 __global__ __launch_bounds__(256) void gfin_beta_kernel(View V, const int32_t* __restrict__ seg_parts,
-                                                        const int32_t* __restrict__ seg_ids) {
+                                                        const int32_t* __restrict__ seg_ids, double delta2) {
   const int s = seg_ids[blockIdx.x];
-  if (V.st[s].done) return;
+  if (seg_idle(V.st[s])) return;
   double out[2];
   seg_sums<2>(V, seg_parts, blockIdx.x, out);
   if (threadIdx.x == 0) {
     SegState& g = V.st[s];
+    g.rz_prev = g.rz;
     g.beta = g.rz != 0.0 ? out[0] / g.rz : 0.0;
     g.rz = out[0];
     g.rr = out[1];
     g.iters += 1;
-    g.done = (out[1] <= g.tol2 * g.bb || g.iters >= g.maxit) ? 1 : 0;
+    if (!g.mixed) {
+      g.done = (out[1] <= g.tol2 * g.bb || g.iters >= g.maxit) ? 1 : 0;
+    } else if (g.iters >= g.maxit ||
+               (CHECK && (out[1] <= delta2 * g.rr_rep || out[1] <= 0.25 * g.tol2 * g.bb))) {
+      g.pause = 1;
+    }
+  }
+}
+
+// Reliable update, part 1 (paused segments): x += x32, x32 = 0.  Grid-stride
+// over the block table.
+__global__ __launch_bounds__(256) void gmx_fold_kernel(View V, Vecs W, int n_blocks, double* __restrict__ x) {
+  for (int bi = blockIdx.x; bi < n_blocks; bi += gridDim.x) {
+    const Block B = V.blocks[bi];
+    const SegState& g = V.st[B.seg];
+    if (!g.pause || g.done) continue;
+    const mife_segment S = V.segs[B.seg];
+    int r0, r1;
+    block_row_range(B, S, r0, r1);
+    for (int row = r0 + threadIdx.x; row < r1; row += 256) {
+      const long long i = S.row_off + row;
+      x[i] += (double)W.x32[i];
+      W.x32[i] = 0.0f;
+    }
+  }
+}
+
+// Reliable update, part 2 (paused segments): r64 = b - A64 x with the FP64
+// values, r32 = r64, z32 = D^-1 r32; partials r32.z32, r32.r32, r64.r64.
+// Grid-stride over the block table.
+__global__ __launch_bounds__(256) void gmx_replace_kernel(View V, Vecs W, int n_blocks, const double* __restrict__ b,
+                                                          const double* __restrict__ x) {
+  for (int bi = blockIdx.x; bi < n_blocks; bi += gridDim.x) {
+    const Block B = V.blocks[bi];
+    const SegState& g = V.st[B.seg];
+    if (!g.pause || g.done) continue;
+    const mife_segment S = V.segs[B.seg];
+    double m[3] = {0.0, 0.0, 0.0};
+    block_rows<double>(V, B, S, V.val, x, [&](int row, double ax) {
+      const long long i = S.row_off + row;
+      const double r64 = b[i] - ax;
+      const float ri = (float)r64;
+      const float zi = W.dinv32[i] * ri;
+      W.r32[i] = ri;
+      W.z32[i] = zi;
+      m[0] += (double)ri * (double)zi;
+      m[1] += (double)ri * (double)ri;
+      m[2] += r64 * r64;
+    });
+    store_partials<3>(V, B, m);
+    __syncthreads();  // the block's LDS partials are reused by its next table entry
+  }
+}
+
+// Reliable update, part 3 (one block per paused segment): beta from the
+// replaced r.z, the true residual and the stopping test; resumes the segment.
+__global__ __launch_bounds__(256) void gfin_replace_kernel(View V, const int32_t* __restrict__ seg_parts,
+                                                           const int32_t* __restrict__ seg_ids) {
+  const int s = seg_ids[blockIdx.x];
+  if (!V.st[s].pause || V.st[s].done) return;
+  double out[3];
+  seg_sums<3>(V, seg_parts, blockIdx.x, out);
+  if (threadIdx.x == 0) {
+    SegState& g = V.st[s];
+    g.beta = g.rz_prev != 0.0 ? out[0] / g.rz_prev : 0.0;
+    g.rz = out[0];
+    g.rr = out[1];
+    g.rr_rep = out[1];
+    g.rr_true = out[2];
+    g.pause = 0;
+    g.done = (out[2] <= g.tol2 * g.bb || g.iters >= g.maxit) ? 1 : 0;
   }
 }
 
@@ -302,7 +461,7 @@ __global__ __launch_bounds__(256) void gcg_resid_kernel(View V, const double* __
   const Block B = V.blocks[blockIdx.x];
   const mife_segment S = V.segs[B.seg];
   double m[1] = {0.0};
-  block_rows(V, B, S, x, [&](int row, double ax) {
+  block_rows<double>(V, B, S, V.val, x, [&](int row, double ax) {
     const double ri = b[S.row_off + row] - ax;
     m[0] += ri * ri;
   });
@@ -326,7 +485,7 @@ namespace {
 
 // Device layout of a table, carved from the caller's work buffer:
 //   mife_segment[max_seg] (by slot) | Block[max_blocks] | seg_parts[max_seg+1] |
-//   seg_ids[max_seg] | parts[3 * max_blocks]
+//   seg_ids[max_seg] | parts[kNumParts * max_blocks]
 // The per-slot CG scalars live in a separate state buffer, so tables can be
 // rebuilt (segments joining or leaving) while the other segments keep iterating.
 struct Layout {
@@ -367,7 +526,7 @@ size_t mife_grouped_work_bytes(int max_segments, int max_blocks) {
   b = align256(b + sizeof(Block) * max_blocks);
   b = align256(b + sizeof(int32_t) * (max_segments + 1));
   b = align256(b + sizeof(int32_t) * max_segments);
-  b += sizeof(double) * 3 * (size_t)max_blocks + 256;
+  b += sizeof(double) * kNumParts * (size_t)max_blocks + 256;
   return b;
 }
 
@@ -389,6 +548,7 @@ int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segment
     MIFE_REQUIRE(S.slot >= 0 && S.slot < max_segments, MIFE_ERR_ARG, "segment slot %d out of range", S.slot);
     MIFE_REQUIRE(S.format == MIFE_FMT_CSR || (S.format == MIFE_FMT_SELL && S.n_slices == div_up(S.n, kSlice)),
                  MIFE_ERR_ARG, "segment %d: format %d / n_slices %d", k, S.format, S.n_slices);
+    MIFE_REQUIRE(S.prec == MIFE_F64 || S.prec == MIFE_F32, MIFE_ERR_ARG, "segment %d: precision %d", k, S.prec);
     H.segs[S.slot] = S;
     H.seg_parts[k] = nb;
     H.seg_ids[k] = S.slot;
@@ -418,53 +578,64 @@ int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segment
 }
 
 static View make_view(const Layout& D, void* state, const mife_pools* P, int cap) {
-  return View{D.segs, D.blocks, P->rp, P->col, P->col16, P->cbase, P->val, (SegState*)state, D.parts, cap};
+  return View{D.segs, D.blocks, P->rp, P->col, P->col16, P->cbase, P->val, P->val32, (SegState*)state, D.parts, cap};
+}
+
+static Vecs make_vecs(double* vec5, float* vec6, int64_t rows) {
+  return Vecs{vec5, vec5 + rows, vec5 + 2 * rows, vec5 + 3 * rows, vec5 + 4 * rows,
+              vec6, vec6 + rows, vec6 + 2 * rows, vec6 + 3 * rows, vec6 + 4 * rows, vec6 + 5 * rows};
 }
 
 int mife_grouped_cg_init(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
-                         const double* b, double* x, double* vec5, int64_t pool_rows, void* work, void* state,
-                         void* stream) {
-  MIFE_REQUIRE(pools && pools->rp && pools->val && b && x && vec5 && work && state, MIFE_ERR_POINTER_IS_NULL,
-               "null pointer argument");
+                         const double* b, const double* x, double* vec5, float* vec6, int64_t pool_rows, void* work,
+                         void* state, void* stream) {
+  MIFE_REQUIRE(pools && pools->rp && pools->val && pools->val32 && b && x && vec5 && vec6 && work && state,
+               MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   if (n_seg == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   Layout D = carve(work, max_segments, max_blocks);
   View V = make_view(D, state, pools, max_blocks);
-  double* dinv = vec5;
-  double* r = vec5 + pool_rows;
-  double* z = vec5 + 2 * pool_rows;
-  double* p = vec5 + 3 * pool_rows;
-  double* q = vec5 + 4 * pool_rows;
-  gcg_diag_kernel<<<n_blocks, 256, 0, s>>>(V, dinv);
-  gcg_init_kernel<<<n_blocks, 256, 0, s>>>(V, b, x, dinv, r, z, p, q);
+  Vecs W = make_vecs(vec5, vec6, pool_rows);
+  gcg_diag_kernel<<<n_blocks, 256, 0, s>>>(V, W);
+  gcg_init_kernel<<<n_blocks, 256, 0, s>>>(V, W, b, x);
   gfin_init_kernel<<<n_seg, 256, 0, s>>>(V, D.seg_parts, D.seg_ids);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }
 
 int mife_grouped_cg_iterate(int n_seg, int n_blocks, int max_segments, int max_blocks, const mife_pools* pools,
-                            double* x, double* vec5, int64_t pool_rows, int iterations, void* work, void* state,
+                            const double* b, double* x, double* vec5, float* vec6, int64_t pool_rows,
+                            int iterations, int check_every, double inner_rtol, void* work, void* state,
                             void* stream) {
-  MIFE_REQUIRE(pools && pools->rp && pools->val && x && vec5 && work && state, MIFE_ERR_POINTER_IS_NULL,
-               "null pointer argument");
+  MIFE_REQUIRE(pools && pools->rp && pools->val && pools->val32 && b && x && vec5 && vec6 && work && state,
+               MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(check_every >= 1 && inner_rtol > 0.0 && inner_rtol < 1.0, MIFE_ERR_ARG,
+               "invalid check_every %d / inner_rtol %g", check_every, inner_rtol);
   if (n_seg == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   Layout D = carve(work, max_segments, max_blocks);
   View V = make_view(D, state, pools, max_blocks);
-  double* dinv = vec5;
-  double* r = vec5 + pool_rows;
-  double* z = vec5 + 2 * pool_rows;
-  double* p = vec5 + 3 * pool_rows;
-  double* q = vec5 + 4 * pool_rows;
+  Vecs W = make_vecs(vec5, vec6, pool_rows);
+  const double delta2 = inner_rtol * inner_rtol;
+  const int rg = std::min(n_blocks, kReplaceGrid);
   for (int it = 0; it < iterations; ++it) {
     const int tk = prof::begin(s);
-    gcg_spmv_kernel<<<n_blocks, 256, 0, s>>>(V, z, p, q);
+    gcg_spmv_kernel<<<n_blocks, 256, 0, s>>>(V, W);
     prof::end(tk, s, prof::kGroupedSpmv);
     gfin_alpha_kernel<<<n_seg, 256, 0, s>>>(V, D.seg_parts, D.seg_ids);
     const int tk2 = prof::begin(s);
-    gcg_update_kernel<<<n_blocks, 256, 0, s>>>(V, x, r, z, p, q, dinv);
+    gcg_update_kernel<<<n_blocks, 256, 0, s>>>(V, W, x);
     prof::end(tk2, s, prof::kGroupedUpdate);
-    gfin_beta_kernel<<<n_seg, 256, 0, s>>>(V, D.seg_parts, D.seg_ids);
+    if ((it + 1) % check_every == 0 || it + 1 == iterations) {
+      gfin_beta_kernel<true><<<n_seg, 256, 0, s>>>(V, D.seg_parts, D.seg_ids, delta2);
+      gmx_fold_kernel<<<rg, 256, 0, s>>>(V, W, n_blocks, x);
+      const int tk3 = prof::begin(s);
+      gmx_replace_kernel<<<rg, 256, 0, s>>>(V, W, n_blocks, b, x);
+      prof::end(tk3, s, prof::kGroupedReplace);
+      gfin_replace_kernel<<<n_seg, 256, 0, s>>>(V, D.seg_parts, D.seg_ids);
+    } else {
+      gfin_beta_kernel<false><<<n_seg, 256, 0, s>>>(V, D.seg_parts, D.seg_ids, delta2);
+    }
   }
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;

@@ -21,6 +21,7 @@ enum Kernel : int {
   kAssembleScalar32,
   kAssembleElastic32,
   kMinresSpmv,
+  kGroupedReplace,
   kNumKernels
 };
 

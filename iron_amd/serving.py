@@ -5,19 +5,27 @@ arrive over time.  Each admitted request is assembled and reduced on the GPU
 (the same kernels as a single solve), and its reduced system is stored into
 shared HBM pools in SELL-64 form (slice pointers, 16-bit or int32 columns,
 values; vectors in slice numbering), next to the CG vectors.
-All admitted requests then iterate together in grouped Jacobi CG launches
-(mife_grouped_cg_*): one launch serves every unconverged system, each with its
-own scalars and stopping test.  Between chunks of iterations the server
+All admitted requests then iterate together in grouped Jacobi CG launches: one
+launch serves every unconverged system, each with its own scalars and stopping
+test.  A request iterates either in FP64 or in FP32 (FP32 copy of the values,
+FP32 vectors) with FP64 reliable updates: its solution is accumulated in FP64
+and its residual is replaced by the true FP64 residual b - A x after every
+decade of FP32 reduction, so it stops on its FP64 residual (1e-8 relative).
+Both kinds share the same launches.  The default policy picks FP32 iterations
+for diffusion requests and FP64 for elasticity requests: on the lognormal
+elasticity boxes the FP32 operator's rounding (a 1e-4..1e-3 relative change of
+A^-1) makes each replacement restart part of the convergence, about twice the
+FP64 iteration count, which costs more than the FP32 bytes save.  Between chunks of iterations the server
 retires converged requests (their solution goes back to the host and their
 pool ranges are freed) and admits waiting ones, first come first served, as
 long as their rows and entries fit the pool budget (HBM-budget admission).  A
 chunk of iterations of an unchanged group is captured once in a HIP graph and
 replayed.
 
-No reference counterpart: iron solves one problem per Problem_Solve.  Each
-request's numbers are those of the single-solve path (tests compare every
-request against the FP64 oracle, and a request solved alone against the same
-request solved in a mix, bit for bit).
+No reference counterpart: iron solves one problem per Problem_Solve.  A
+request's iterates depend only on its own data, never on which other requests
+share the launches (tests compare every request against the FP64 oracle, and a
+request solved alone against the same request solved in a mix, bit for bit).
 """
 from __future__ import annotations
 
@@ -157,17 +165,32 @@ class _FirstFit:
 
 
 class Server:
-    """Grouped-CG serving of Request objects within an HBM budget (bytes)."""
+    """Grouped-CG serving of Request objects within an HBM budget (bytes).
 
-    BYTES_PER_ROW = 7 * 8 + 4      # b, x, D^-1, r, z, p, q (FP64) + slice pointer / base column share
-    BYTES_PER_NZ = 8 + 4 + 2       # value + int32 column + 16-bit column (both pools exist)
+    precision: "auto" (FP32 iterations with FP64 reliable updates for diffusion
+    requests, FP64 for elasticity), "mixed" (all FP32 + FP64 reliable updates) or
+    "fp64".  Reliable updates follow every ``inner_rtol`` reduction of a request's
+    FP32 residual, tested every ``check_every`` iterations."""
+
+    MIXED_KINDS = {"auto": ("diffusion",), "mixed": ("diffusion", "elasticity"), "fp64": ()}
 
     def __init__(self, hbm_budget: float = 32e9, max_segments: int = 256, nz_per_row: float = 60.0,
                  chunk: int = 16, use_graph: bool = True, device=None, cache_shapes: int = 8,
-                 admit_budget: float = 0.004):
+                 admit_budget: float = 0.004, precision: str = "auto", inner_rtol: float = 0.1,
+                 check_every: int = 4):
         self.device = fem._dev(device)
         self.max_segments = int(max_segments)
-        rows = int(hbm_budget / (self.BYTES_PER_ROW + nz_per_row * self.BYTES_PER_NZ))
+        if precision not in self.MIXED_KINDS:
+            raise ValueError(f"precision must be one of {sorted(self.MIXED_KINDS)}")
+        self.mixed_kinds = self.MIXED_KINDS[precision]
+        self.inner_rtol = float(inner_rtol)
+        self.check_every = int(check_every)
+        # pool bytes per row: b, x (FP64), D^-1, r, z, p, q (FP64), D^-1, r, z, p, q, x32 (FP32)
+        # and the slice pointer / base column share; per stored entry: FP64 value, FP32
+        # copy, int32 column, 16-bit column (all pools exist)
+        self.bytes_per_row = 2 * 8 + 5 * 8 + 6 * 4 + 4
+        self.bytes_per_nz = 8 + 4 + 4 + 2
+        rows = int(hbm_budget / (self.bytes_per_row + nz_per_row * self.bytes_per_nz))
         self.rows_cap = max(rows, 1024)
         self.nz_cap = int(self.rows_cap * nz_per_row)
         self.rp_cap = self.rows_cap + self.max_segments
@@ -175,13 +198,16 @@ class Server:
         self.b = torch.zeros(self.rows_cap, dtype=torch.float64, device=dev)
         self.x = torch.zeros(self.rows_cap, dtype=torch.float64, device=dev)
         self.vec5 = torch.zeros(5 * self.rows_cap, dtype=torch.float64, device=dev)
+        self.vec6 = torch.zeros(6 * self.rows_cap, dtype=torch.float32, device=dev)
         self.rp = torch.zeros(self.rp_cap, dtype=torch.int32, device=dev)
         self.col = torch.zeros(max(self.nz_cap, 1), dtype=torch.int32, device=dev)
         self.col16 = torch.zeros(max(self.nz_cap, 1), dtype=torch.int16, device=dev)
         self.cb_cap = self.rows_cap // 64 + 2 * self.max_segments
         self.cbase = torch.zeros(self.cb_cap, dtype=torch.int32, device=dev)
         self.val = torch.zeros(max(self.nz_cap, 1), dtype=torch.float64, device=dev)
-        self.pools = Pools(ptr(self.rp), ptr(self.col), ptr(self.col16), ptr(self.cbase), ptr(self.val))
+        self.val32 = torch.zeros(max(self.nz_cap, 1), dtype=torch.float32, device=dev)
+        self.pools = Pools(ptr(self.rp), ptr(self.col), ptr(self.col16), ptr(self.cbase), ptr(self.val),
+                           ptr(self.val32))
         self.max_blocks = _lib.lib().mife_grouped_blocks(self.rows_cap) + 2 * self.max_segments
         L = _lib.lib()
         self.state = torch.zeros(int(L.mife_grouped_state_bytes(self.max_segments)), dtype=torch.uint8, device=dev)
@@ -256,6 +282,9 @@ class Server:
         s = fem._stream()
         # solver values straight from the equations CSR into the value pool
         call("mife_gather_values", stored, ptr(S.src), ptr(vals), fem.F64, ptr(self.val) + 8 * nzo, fem.F64, s)
+        mixed = req.kind in self.mixed_kinds
+        if mixed:   # the FP32 operator is the FP64 one rounded
+            call("mife_gather_values", stored, ptr(S.src), ptr(vals), fem.F64, ptr(self.val32) + 4 * nzo, fem.F32, s)
         self.rp[rpo:rpo + ns + 1].copy_(S.slice_ptr[: ns + 1])
         if c16:
             self.col16[nzo:nzo + stored].copy_(S.col16[:stored])
@@ -267,10 +296,12 @@ class Server:
              ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(b), s)
         self.b[ro:ro + nf].copy_(b[st.perm])          # slice numbering
         self.x[ro:ro + nf].zero_()
-        seg = Segment(nf, 200000, slot, _lib.MIFE_FMT_SELL, ro, rpo, nzo, req.rtol, ns, 0, cbo if c16 else -1)
+        seg = Segment(nf, 200000, slot, _lib.MIFE_FMT_SELL, ro, rpo, nzo, req.rtol, ns,
+                      fem.F32 if mixed else fem.F64, cbo if c16 else -1)
         req.t_admit = now
         self.stats["admitted"] += 1
-        return dict(req=req, seg=seg, st=st, nf=nf, nnz=red.nnz, stored=stored, ns=ns, cbo=cbo)
+        return dict(req=req, seg=seg, st=st, nf=nf, nnz=red.nnz, stored=stored, ns=ns, cbo=cbo, cbo_used=c16,
+                    mixed=mixed)
 
     def _build_table(self, entries, table):
         segs = (Segment * max(len(entries), 1))(*[e["seg"] for e in entries])
@@ -282,7 +313,12 @@ class Server:
 
     def _iterate(self, n_seg, nb, iters):
         call("mife_grouped_cg_iterate", n_seg, nb, self.max_segments, self.max_blocks, C.byref(self.pools),
-             ptr(self.x), ptr(self.vec5), self.rows_cap, int(iters), ptr(self.table), ptr(self.state),
+             ptr(self.b), ptr(self.x), ptr(self.vec5), ptr(self.vec6), self.rows_cap, int(iters), self.check_every,
+             self.inner_rtol, ptr(self.table), ptr(self.state), fem._stream())
+
+    def _init(self, n_seg, nb, table):
+        call("mife_grouped_cg_init", n_seg, nb, self.max_segments, self.max_blocks, C.byref(self.pools),
+             ptr(self.b), ptr(self.x), ptr(self.vec5), ptr(self.vec6), self.rows_cap, ptr(table), ptr(self.state),
              fem._stream())
 
     def _status(self):
@@ -362,9 +398,7 @@ class Server:
                     qi += 1
                 if new:
                     nbi = self._build_table(new, self.init_table)
-                    call("mife_grouped_cg_init", len(new), nbi, self.max_segments, self.max_blocks,
-                         C.byref(self.pools), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
-                         ptr(self.init_table), ptr(self.state), fem._stream())
+                    self._init(len(new), nbi, self.init_table)
                     pending += new
             self.timing["admit"] += time.perf_counter() - ta
             if not launched:
@@ -376,12 +410,14 @@ class Server:
             self.timing["status"] += time.perf_counter() - ts
             for e in active:
                 # algorithmic bytes of the grouped products this chunk ran for the segment:
-                # 12 B per stored entry (FP64 value + column) and 44 B per row (row
-                # pointer, z, p, q read, p, q written)
+                # per nonzero its value and column (16-bit or int32), per row z (gathered
+                # once), p and q read and written
                 ran = int(it[e["seg"].slot]) - e.get("it_seen", 0)
                 e["it_seen"] = int(it[e["seg"].slot])
                 self.stats["iterations"] += ran
-                self.stats["spmv_bytes"] += ran * (12.0 * e["nnz"] + 44.0 * e["nf"])
+                cb = 2.0 if e["cbo_used"] else 4.0
+                vb = 4.0 if e["mixed"] else 8.0
+                self.stats["spmv_bytes"] += ran * ((vb + cb) * e["nnz"] + 5.0 * vb * e["nf"])
             finished = [e for e in active if dn[e["seg"].slot]]
             tr = time.perf_counter()
             if finished:
@@ -404,9 +440,7 @@ class Server:
                         self._retire(e, t0)
                 if redo:
                     nbr = self._build_table(redo, self.resid_table)
-                    call("mife_grouped_cg_init", len(redo), nbr, self.max_segments, self.max_blocks,
-                         C.byref(self.pools), ptr(self.b), ptr(self.x), ptr(self.vec5), self.rows_cap,
-                         ptr(self.resid_table), ptr(self.state), fem._stream())
+                    self._init(len(redo), nbr, self.resid_table)
                 redo_ids = {id(e) for e in redo}
                 retired = {id(e) for e in finished if id(e) not in redo_ids}
                 active = [e for e in active if id(e) not in retired]

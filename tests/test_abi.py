@@ -46,6 +46,19 @@ def test_argument_errors_without_gpu():
     assert L.mife_cg(None, None, None, None, 0, 1e-8, 10, None, None, None, None) == -1
     assert L.mife_sell_order(4, None, None, None, None) == -1
     assert L.mife_grouped_setup(None, 0, 1, 1, None, None, None, None) == -1
+    pools = _lib.Pools(1, 1, 1, 1, 1, None)   # the FP32 value pool is required
+    it = L.mife_grouped_cg_iterate
+    assert it(1, 1, 1, 1, ctypes.byref(pools), 1, 1, 1, 1, 4, 4, 4, 0.1, 1, 1, None) == -1
+    pools = _lib.Pools(1, 1, 1, 1, 1, 1)
+    assert it(1, 1, 1, 1, ctypes.byref(pools), 1, 1, 1, 1, 4, 4, 0, 0.1, 1, 1, None) == 2
+    assert "check_every" in _lib.last_error()
+    assert it(1, 1, 1, 1, ctypes.byref(pools), 1, 1, 1, 1, 4, 4, 4, 1.5, 1, 1, None) == 2
+    seg = _lib.Segment(10, 5, 0, _lib.MIFE_FMT_CSR, 0, 0, 0, 1e-8, 0, 7, -1)   # unknown precision 7
+    nb = ctypes.c_int(0)
+    stage = bytearray(int(L.mife_grouped_work_bytes(1, 4)))
+    buf = (ctypes.c_char * len(stage)).from_buffer(stage)
+    assert L.mife_grouped_setup(ctypes.byref(seg), 1, 1, 4, ctypes.byref(nb), buf, buf, None) == 2
+    assert "precision" in _lib.last_error()
     # invalid arguments on a well-formed (host-side) descriptor
     d = M(10, 7, 1, 1, None, 0, 10, None, None, 0)
     assert L.mife_spmv(ctypes.byref(d), 1, 1, 1, 0, None) == 2     # unknown format 7

@@ -1,7 +1,7 @@
-"""Serving path (grouped CG, HBM-budget admission, continuous batching, HIP-graph
-replay): every request against the FP64 oracle and the single-solve path, a request
-alone against the same request in a mix (bit for bit), and admission under a tight
-budget."""
+"""Serving path (grouped CG in FP64 and in FP32 with FP64 reliable updates,
+HBM-budget admission, continuous batching, HIP-graph replay): every request against
+the FP64 oracle and the single-solve path, a request alone against the same request
+in a mix (bit for bit), and admission under a tight budget."""
 import numpy as np
 import pytest
 
@@ -60,11 +60,11 @@ def _mix():
             serving.Request(4, "diffusion", (3, 3, 3), 15)]
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_grouped_requests_match_oracle_and_single_solve(use_graph):
+@pytest.mark.parametrize("use_graph,precision", [(False, "fp64"), (True, "fp64"), (False, "mixed"), (True, "auto")])
+def test_grouped_requests_match_oracle_and_single_solve(use_graph, precision):
     from iron_amd import serving
-    print(f"case serving mix graph={use_graph}", flush=True)
-    srv = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8, use_graph=use_graph)
+    print(f"case serving mix graph={use_graph} precision={precision}", flush=True)
+    srv = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8, use_graph=use_graph, precision=precision)
     reqs = srv.run(_mix())
     single = {}
     for r in reqs:
@@ -77,16 +77,20 @@ def test_grouped_requests_match_oracle_and_single_solve(use_graph):
         assert srv.stats["graph_replays"] > 0
 
 
-def test_alone_equals_in_mix_bitwise():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_alone_equals_in_mix_bitwise(precision):
+    """The target request (elasticity) iterates in FP64 or FP32; the mix around it
+    holds requests of both precisions (policy "auto") or all FP32."""
     from iron_amd import serving
-    print("case serving alone vs mix", flush=True)
+    print(f"case serving alone vs mix precision={precision}", flush=True)
     target = serving.Request(9, "elasticity", (5, 3, 4), 99)
-    alone = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8).run(
+    other = "auto" if precision == "fp64" else "mixed"
+    alone = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8, precision=precision).run(
         [serving.Request(9, "elasticity", (5, 3, 4), 99)])[0]
     mix = _mix() + [target]
     for k, r in enumerate(mix):
         r.arrival = 0.0 if k % 2 == 0 else 1e-3  # joins at different chunk boundaries
-    out = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8).run(mix)
+    out = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8, precision=other).run(mix)
     inmix = [r for r in out if r.rid == 9][0]
     assert inmix.iterations == alone.iterations
     assert np.array_equal(inmix.u, alone.u)
@@ -102,3 +106,27 @@ def test_budget_admission_queues_and_completes():
     for r in out:
         assert r.u is not None and r.relres <= 1e-8
     assert srv.stats["table_builds"] > 3
+
+
+def test_mixed_and_fp64_servers_agree_and_mixed_stops_on_the_fp64_residual():
+    """FP32 iterations with FP64 reliable updates reach the same 1e-8 FP64 residual
+    as FP64 iterations (the reported residual is the true FP64 one), on a mix whose
+    larger requests need several replacements."""
+    from iron_amd import serving
+    print("case serving mixed vs FP64", flush=True)
+    reqs = serving.make_stream(6, seed=31, lo=6, hi=14)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        srv = serving.Server(hbm_budget=4e8, max_segments=16, chunk=16, precision=prec)
+        out[prec] = {r.rid: r for r in srv.run([serving.Request(r.rid, r.kind, r.n_elem, r.seed) for r in reqs])}
+    for rid, r64 in out["fp64"].items():
+        r32 = out["mixed"][rid]
+        assert r32.relres <= 1e-8 and r64.relres <= 1e-8
+        ref = _oracle_solution(r64)
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(r32.u - ref).max() <= 1e-6 * scale, rid
+        assert np.abs(r32.u - r64.u).max() <= 1e-6 * scale, rid
+        print(f"request {rid} {r64.kind} {r64.n_elem}: iterations FP32+FP64 updates {r32.iterations}, "
+              f"FP64 {r64.iterations}", flush=True)
+        if r64.kind == "diffusion":   # FP32 iterations track the FP64 ones on diffusion
+            assert r32.iterations <= 1.25 * r64.iterations + 8, (rid, r32.iterations, r64.iterations)
