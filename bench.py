@@ -533,6 +533,7 @@ def main_leg(args):
         "kernel_ms": {k: v for k, v in prof.items() if v[0]},
         "kernel_ms_fp64": {k: v for k, v in prof64.items() if v[0]},
         "solver_format": args.format, "sell_padding": getattr(P["S"], "padding", 0.0),
+        "sell_stored": getattr(P["S"], "stored", 0),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)
