@@ -214,6 +214,45 @@ int mife_assemble_elastic(const mife_element_desc* desc, int n_elem,
                           const int32_t* colour_ptr_host, int n_colours, int prec,
                           void* vals, void* stream);
 
+/* Batched assembly: the CSR values of many independent trilinear meshes (the
+ * requests a server admits together) in one launch per colour and element
+ * kind.  Each job is one mesh with the arguments of mife_assemble_scalar
+ * (kind MIFE_JOB_SCALAR: coef = kappa or NULL, optional source) or
+ * mife_assemble_elastic (MIFE_JOB_ELASTIC: coef = material); its elements are
+ * taken in its colour order, so its values equal those of its own
+ * mife_assemble_* call bit for bit.  Jobs must not share output arrays.
+ * Trilinear elements with 2x2x2 Gauss points only (scalar_desc / elastic_desc:
+ * dim 3, 8 nodes, 8 points; either may be NULL when no job of its kind is
+ * given).  The host table (table_host, pageable or pinned) and the device
+ * table (work) hold mife_assemble_batch_work_bytes(n_jobs, max_colours) bytes;
+ * the call synchronises the stream once after copying the table.  No
+ * reference counterpart: iron assembles one equations set per call of
+ * EquationsSet_AssembleStaticLinearFEM (src/equations_set_routines.F90:1203-1364). */
+#define MIFE_JOB_SCALAR 0
+#define MIFE_JOB_ELASTIC 1
+
+typedef struct mife_assembly_job {
+  int32_t kind;
+  int32_t n_elem;
+  const double* coords;           /* [nodes][3] */
+  const int32_t* elem_nodes;      /* [n_elem][8] */
+  const int32_t* elem_dofs;       /* scalar with a source: [n_elem][8] */
+  const double* coef;             /* scalar: kappa [n_elem] or NULL; elastic: material [n_elem][6] */
+  const double* source;           /* scalar: f [n_elem] or NULL */
+  const int32_t* pos;             /* scatter map [n_elem][nd][nd] */
+  const int32_t* order;           /* element ids grouped by colour */
+  const int32_t* colour_ptr_host; /* [n_colours + 1], host memory */
+  int32_t n_colours;
+  int32_t pad;
+  void* vals;                     /* CSR values (FP64 or FP32 per prec), accumulated into */
+  double* src_vec;                /* scalar: FP64 source vector or NULL */
+} mife_assembly_job;
+
+size_t mife_assemble_batch_work_bytes(int n_jobs, int max_colours);
+int mife_assemble_batch(const mife_element_desc* scalar_desc, const mife_element_desc* elastic_desc,
+                        int n_jobs, const mife_assembly_job* jobs, int max_colours, int prec,
+                        void* table_host, void* work, void* stream);
+
 /* Mooney-Rivlin finite elasticity (3-D, FP64). */
 #define MIFE_MR_DEVIATORIC 0 /* EQUATIONS_SET_MOONEY_RIVLIN_SUBTYPE branch */
 #define MIFE_MR_CAUCHY 1     /* EQUATIONS_SET_NO_SUBTYPE general branch */

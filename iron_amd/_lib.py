@@ -60,6 +60,18 @@ class ElementDesc(C.Structure):
                 ("dN", C.c_void_p), ("N", C.c_void_p), ("w", C.c_void_p)]
 
 
+MIFE_JOB_SCALAR = 0
+MIFE_JOB_ELASTIC = 1
+
+
+class AssemblyJob(C.Structure):
+    """mife_assembly_job: one mesh of a batched assembly."""
+    _fields_ = [("kind", C.c_int32), ("n_elem", C.c_int32), ("coords", C.c_void_p), ("elem_nodes", C.c_void_p),
+                ("elem_dofs", C.c_void_p), ("coef", C.c_void_p), ("source", C.c_void_p), ("pos", C.c_void_p),
+                ("order", C.c_void_p), ("colour_ptr_host", C.c_void_p), ("n_colours", C.c_int32),
+                ("pad", C.c_int32), ("vals", C.c_void_p), ("src_vec", C.c_void_p)]
+
+
 class MrDesc(C.Structure):
     """mife_mr_desc: displacement basis, pressure DOFs per element, formulation."""
     _fields_ = [("u", C.POINTER(ElementDesc)), ("n_pressure", C.c_int), ("Np", C.c_void_p),
@@ -101,6 +113,9 @@ _SIG = {
                          C.POINTER(_D), _P]),
     "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_minres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_assemble_batch_work_bytes": (C.c_size_t, [_I, _I]),
+    "mife_assemble_batch": (_I, [C.POINTER(ElementDesc), C.POINTER(ElementDesc), _I, C.POINTER(AssemblyJob), _I, _I,
+                                 _P, _P, _P]),
     "mife_mr_assemble": (_I, [C.POINTER(MrDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "mife_grouped_state_bytes": (C.c_size_t, [_I]),
     "mife_grouped_work_bytes": (C.c_size_t, [_I, _I]),

@@ -10,6 +10,9 @@
 #include "mife_common.hpp"
 #include "mife_profile.hpp"
 
+#include <algorithm>
+#include <vector>
+
 namespace mife {
 
 struct Mat3 {
@@ -90,25 +93,33 @@ __device__ __forceinline__ void element_geometry(GeomLDS<DIM, NN, NG>& G, const 
 }
 
 // ------------------------------------------------------------ scalar field
+// Inputs of one scalar assembly (one mesh, one set of CSR values).
+template <typename T>
+struct ScalarJob {
+  const double* coords;
+  const int32_t* enodes;
+  const int32_t* edofs;
+  const double* kappa;
+  const double* source;
+  const int32_t* pos;
+  T* vals;
+  double* svec;
+};
+
+// One element per wavefront (element e when `active`); every wavefront of the
+// block calls it the same number of times (it holds block barriers).
 template <int DIM, int NN, int NG, typename T, int WPB>
-__global__ __launch_bounds__(64 * WPB) void scalar_assemble_kernel(
-    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
-    const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
-    const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
-    const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
-    bool atomic) {
+__device__ __forceinline__ void scalar_element(const ScalarJob<T>& J, long long e, bool active,
+                                               const double* __restrict__ dN, const double* __restrict__ N,
+                                               const double* __restrict__ w, bool atomic) {
   constexpr int KD = NG * DIM;
   __shared__ GeomLDS<DIM, NN, NG> geo[WPB];
   __shared__ T sB[WPB][KD * NN];
   __shared__ T sW[WPB][KD];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = blockIdx.x * WPB + wv;
-  const bool active = idx < elem_count;
-  long long e = 0;
-  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
   GeomLDS<DIM, NN, NG>& G = geo[wv];
-  element_geometry<DIM, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
-  const double ke = (active && kappa) ? kappa[e] : 1.0;
+  element_geometry<DIM, NN, NG>(G, J.coords, J.enodes, e, dN, w, lane, active);
+  const double ke = (active && J.kappa) ? J.kappa[e] : 1.0;
   if (active) {
     for (int t = lane; t < NG * NN; t += 64) {
       const int g = t / NN, a = t - (t / NN) * NN;
@@ -131,16 +142,34 @@ __global__ __launch_bounds__(64 * WPB) void scalar_assemble_kernel(
       T acc = 0;
 #pragma unroll 9
       for (int k = 0; k < KD; ++k) acc += W[k] * B[k * NN + m] * B[k * NN + n];
-      const int p = pos[e * (NN * NN) + t];
-      add_value(&vals[p], acc, atomic);
+      const int p = J.pos[e * (NN * NN) + t];
+      add_value(&J.vals[p], acc, atomic);
     }
-    if (source && svec && lane < NN) {
+    if (J.source && J.svec && lane < NN) {
+      // Rounded product and sum, unfused whatever the caller inlines around them
+      // (a kernel whose `atomic` is a constant would otherwise fuse them).
+#pragma clang fp contract(off)
       double s = 0.0;
-      for (int g = 0; g < NG; ++g) s += N[g * NN + lane] * G.rwg[g];
-      s *= source[e];
-      add_value(&svec[edofs[e * NN + lane]], s, atomic);
+      for (int g = 0; g < NG; ++g) s = fma(N[g * NN + lane], G.rwg[g], s);
+      s = s * J.source[e];
+      add_value(&J.svec[J.edofs[e * NN + lane]], s, atomic);
     }
   }
+}
+
+template <int DIM, int NN, int NG, typename T, int WPB>
+__global__ __launch_bounds__(64 * WPB) void scalar_assemble_kernel(
+    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
+    const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
+    const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
+    const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
+    bool atomic) {
+  const int idx = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const bool active = idx < elem_count;
+  long long e = 0;
+  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
+  scalar_element<DIM, NN, NG, T, WPB>(ScalarJob<T>{coords, enodes, edofs, kappa, source, pos, vals, svec}, e,
+                                      active, dN, N, w, atomic);
 }
 
 // ------------------------------------------------ triquadratic, on MFMA
@@ -321,11 +350,19 @@ __global__ __launch_bounds__(64 * WPB) void elastic_assemble_kernel(
 // blocks and, for i != j, U[(i,j,a)][(g,L)] = w (Ca B[g][a][i] [L == j] +
 // Cb B[g][a][j] [L == i]) with (Ca, Cb) of the pair {i, j}
 // (linear_elasticity_routines.F90:1255-1286; the lower blocks are the transposes).
+template <typename T>
+struct ElasticJob {
+  const double* coords;
+  const int32_t* enodes;
+  const double* material;
+  const int32_t* pos;
+  T* vals;
+};
+
 template <typename T, int NN, int NG, int WPB>
-__global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
-    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const double* __restrict__ material,
-    const int32_t* __restrict__ pos, const int32_t* __restrict__ order, int elem_begin, int elem_count,
-    const double* __restrict__ dN, const double* __restrict__ w, T* __restrict__ vals, bool atomic) {
+__device__ __forceinline__ void elastic_element(const ElasticJob<T>& J, long long e, bool active,
+                                                const double* __restrict__ dN, const double* __restrict__ w,
+                                                bool atomic) {
   constexpr int ND = 3 * NN, MROWS = 9 * NN, KD = 3 * NG;
   constexpr int MT = (MROWS + 15) / 16, STEPS = (KD + 3) / 4;
   static_assert(NN <= 16, "one column tile");
@@ -335,12 +372,8 @@ __global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
   __shared__ T sDiag[WPB][NG][9];  // w * diagC[c][d]
   __shared__ T sOff[WPB][NG][6];   // w * (Ca, Cb) of the pairs (0,1), (0,2), (1,2)
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = blockIdx.x * WPB + wv;
-  const bool active = idx < elem_count;
-  long long e = 0;
-  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
   GeomLDS<3, NN, NG>& G = geo[wv];
-  element_geometry<3, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
+  element_geometry<3, NN, NG>(G, J.coords, J.enodes, e, dN, w, lane, active);
   if (active) {
     for (int t = lane; t < NG * NN; t += 64) {
       const int g = t / NN, a = t - (t / NN) * NN;
@@ -355,7 +388,7 @@ __global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
     }
     if (lane < NG) {
       const int g = lane;
-      const double* mat = material + e * 6;
+      const double* mat = J.material + e * 6;
       const double E1 = mat[0], E2 = mat[1], E3 = mat[2], v13 = mat[3], v23 = mat[4], v12 = mat[5];
       const double v31 = v13, v32 = v23, v21 = v12;
       const double gama = 1.0 / (1.0 - v12 * v21 - v23 * v32 - v31 * v13 - 2.0 * v21 * v32 * v13);
@@ -372,8 +405,8 @@ __global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
     }
   }
   __syncthreads();
-  if (!active) return;
-  const int32_t* pe = pos + e * (ND * ND);
+  if (!active) return;  // no barrier follows
+  const int32_t* pe = J.pos + e * (ND * ND);
   const int col = lane & 15;
   for (int mt = 0; mt < MT; ++mt) {
     const int mrow = 16 * mt + col;
@@ -404,9 +437,105 @@ __global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
       const int orow = 16 * mt + M::row(lane, r);
       if (orow < MROWS && col < NN) {
         const int oi = orow / (3 * NN), oj = (orow / NN) % 3, oa = orow % NN;
-        add_value(&vals[pe[(oi * NN + oa) * ND + oj * NN + col]], acc[r], atomic);
+        add_value(&J.vals[pe[(oi * NN + oa) * ND + oj * NN + col]], acc[r], atomic);
       }
     }
+  }
+}
+
+template <typename T, int NN, int NG, int WPB>
+__global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
+    const double* __restrict__ coords, const int32_t* __restrict__ enodes, const double* __restrict__ material,
+    const int32_t* __restrict__ pos, const int32_t* __restrict__ order, int elem_begin, int elem_count,
+    const double* __restrict__ dN, const double* __restrict__ w, T* __restrict__ vals, bool atomic) {
+  const int idx = blockIdx.x * WPB + (threadIdx.x >> 6);
+  const bool active = idx < elem_count;
+  long long e = 0;
+  if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
+  elastic_element<T, NN, NG, WPB>(ElasticJob<T>{coords, enodes, material, pos, vals}, e, active, dN, w, atomic);
+}
+
+// ------------------------------------------- many meshes in one launch
+// A batch assembles the CSR values of many independent trilinear meshes (the
+// requests a server admits together): one launch per colour and element kind
+// covers every mesh of the batch.  A launch walks a list of segments -- the
+// elements of one colour of one mesh -- as one flattened index space; each
+// wavefront finds its segment by bisection and runs the same element code as
+// the one-mesh kernels, so a mesh's values are bit for bit those of its own
+// assembly.  Blocks loop over the index space (a resident grid).
+struct BatchJob {
+  const double* coords;
+  const int32_t* enodes;
+  const int32_t* edofs;
+  const double* coef;
+  const double* source;
+  const int32_t* pos;
+  const int32_t* order;
+  void* vals;
+  double* svec;
+};
+
+struct BatchSeg {
+  int32_t job;     // index into the job table
+  int32_t begin;   // first position of the colour in the job's element order
+  int32_t prefix;  // first flattened index of the segment
+  int32_t pad;
+};
+
+constexpr int kBatchGrid = 2048;
+
+__device__ __forceinline__ const BatchSeg& batch_seg(const BatchSeg* __restrict__ segs, int nseg, int idx) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].prefix <= idx)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return segs[lo];
+}
+
+template <typename T, int WPB>
+__global__ __launch_bounds__(64 * WPB) void scalar_batch_kernel(const BatchJob* __restrict__ jobs,
+                                                                const BatchSeg* __restrict__ segs, int nseg,
+                                                                int total, const double* __restrict__ dN,
+                                                                const double* __restrict__ N,
+                                                                const double* __restrict__ w) {
+  const int wv = threadIdx.x >> 6;
+  for (int base = blockIdx.x * WPB; base < total; base += gridDim.x * WPB) {
+    const int idx = base + wv;
+    const bool active = idx < total;
+    ScalarJob<T> J{};
+    long long e = 0;
+    if (active) {
+      const BatchSeg sg = batch_seg(segs, nseg, idx);
+      const BatchJob B = jobs[sg.job];
+      J = ScalarJob<T>{B.coords, B.enodes, B.edofs, B.coef, B.source, B.pos, (T*)B.vals, B.svec};
+      e = B.order[sg.begin + idx - sg.prefix];
+    }
+    scalar_element<3, 8, 8, T, WPB>(J, e, active, dN, N, w, false);
+  }
+}
+
+template <typename T, int WPB>
+__global__ __launch_bounds__(64 * WPB) void elastic_batch_kernel(const BatchJob* __restrict__ jobs,
+                                                                 const BatchSeg* __restrict__ segs, int nseg,
+                                                                 int total, const double* __restrict__ dN,
+                                                                 const double* __restrict__ w) {
+  const int wv = threadIdx.x >> 6;
+  for (int base = blockIdx.x * WPB; base < total; base += gridDim.x * WPB) {
+    const int idx = base + wv;
+    const bool active = idx < total;
+    ElasticJob<T> J{};
+    long long e = 0;
+    if (active) {
+      const BatchSeg sg = batch_seg(segs, nseg, idx);
+      const BatchJob B = jobs[sg.job];
+      J = ElasticJob<T>{B.coords, B.enodes, B.coef, B.pos, (T*)B.vals};
+      e = B.order[sg.begin + idx - sg.prefix];
+    }
+    elastic_element<T, 8, 8, WPB>(J, e, active, dN, w, false);
   }
 }
 
@@ -528,4 +657,117 @@ extern "C" int mife_assemble_elastic(const mife_element_desc* d, int n_elem, con
   }
   set_error("unsupported elastic element (nodes=%d, gauss=%d)", d->n_nodes, d->n_gauss);
   return MIFE_ERR_ARG;
+}
+
+namespace {
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" size_t mife_assemble_batch_work_bytes(int n_jobs, int max_colours) {
+  const size_t nj = (size_t)std::max(n_jobs, 1);
+  return align256(sizeof(BatchJob) * nj) + sizeof(BatchSeg) * nj * (size_t)std::max(max_colours, 1) + 256;
+}
+
+extern "C" int mife_assemble_batch(const mife_element_desc* scalar_desc, const mife_element_desc* elastic_desc,
+                                   int n_jobs, const mife_assembly_job* jobs, int max_colours, int prec,
+                                   void* table_host, void* work, void* stream) {
+  MIFE_REQUIRE(n_jobs >= 0 && max_colours >= 0, MIFE_ERR_ARG, "invalid job count %d / colour bound %d", n_jobs,
+               max_colours);
+  if (n_jobs == 0) return MIFE_OK;
+  MIFE_REQUIRE(jobs && table_host && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
+  bool any_scalar = false, any_elastic = false, any_source = false;
+  for (int k = 0; k < n_jobs; ++k) {
+    const mife_assembly_job& J = jobs[k];
+    MIFE_REQUIRE(J.kind == MIFE_JOB_SCALAR || J.kind == MIFE_JOB_ELASTIC, MIFE_ERR_ARG, "job %d: kind %d", k, J.kind);
+    MIFE_REQUIRE(J.n_elem >= 0 && J.n_colours >= 0 && J.n_colours <= max_colours, MIFE_ERR_ARG,
+                 "job %d: %d elements in %d colours (bound %d)", k, J.n_elem, J.n_colours, max_colours);
+    if (J.n_elem == 0) continue;
+    MIFE_REQUIRE(J.coords && J.elem_nodes && J.pos && J.order && J.colour_ptr_host && J.vals,
+                 MIFE_ERR_POINTER_IS_NULL, "job %d: null pointer", k);
+    MIFE_REQUIRE(J.colour_ptr_host[0] == 0 && J.colour_ptr_host[J.n_colours] == J.n_elem, MIFE_ERR_ARG,
+                 "job %d: colour pointers do not cover its %d elements", k, J.n_elem);
+    if (J.kind == MIFE_JOB_SCALAR) {
+      any_scalar = true;
+      if (J.source && J.src_vec) {
+        MIFE_REQUIRE(J.elem_dofs, MIFE_ERR_POINTER_IS_NULL, "job %d: source without element DOFs", k);
+        any_source = true;
+      }
+    } else {
+      any_elastic = true;
+      MIFE_REQUIRE(J.coef, MIFE_ERR_POINTER_IS_NULL, "job %d: elasticity without material", k);
+    }
+  }
+  if (any_scalar) {
+    MIFE_REQUIRE(scalar_desc && scalar_desc->dN && scalar_desc->w, MIFE_ERR_POINTER_IS_NULL, "scalar basis missing");
+    MIFE_REQUIRE(scalar_desc->dim == 3 && scalar_desc->n_nodes == 8 && scalar_desc->n_gauss == 8, MIFE_ERR_ARG,
+                 "batched scalar assembly takes trilinear elements with 2x2x2 Gauss points");
+    MIFE_REQUIRE(!any_source || scalar_desc->N, MIFE_ERR_POINTER_IS_NULL, "source given without basis values N");
+  }
+  if (any_elastic) {
+    MIFE_REQUIRE(elastic_desc && elastic_desc->dN && elastic_desc->w, MIFE_ERR_POINTER_IS_NULL,
+                 "elastic basis missing");
+    MIFE_REQUIRE(elastic_desc->dim == 3 && elastic_desc->n_nodes == 8 && elastic_desc->n_gauss == 8, MIFE_ERR_ARG,
+                 "batched elastic assembly takes trilinear elements with 2x2x2 Gauss points");
+  }
+  // host table: jobs | segments of launch 0 | segments of launch 1 | ...
+  char* H = (char*)table_host;
+  BatchJob* hj = (BatchJob*)H;
+  BatchSeg* hs = (BatchSeg*)(H + align256(sizeof(BatchJob) * n_jobs));
+  for (int k = 0; k < n_jobs; ++k) {
+    const mife_assembly_job& J = jobs[k];
+    hj[k] = BatchJob{J.coords, J.elem_nodes, J.elem_dofs, J.coef, J.source, J.pos, J.order, J.vals, J.src_vec};
+  }
+  struct Launch {
+    int kind, seg0, nseg, total;
+  };
+  std::vector<Launch> launches;
+  int ns = 0;
+  for (int c = 0; c < max_colours; ++c) {
+    for (int kind : {MIFE_JOB_SCALAR, MIFE_JOB_ELASTIC}) {
+      Launch L{kind, ns, 0, 0};
+      for (int k = 0; k < n_jobs; ++k) {
+        const mife_assembly_job& J = jobs[k];
+        if (J.kind != kind || J.n_elem == 0 || c >= J.n_colours) continue;
+        const int cnt = J.colour_ptr_host[c + 1] - J.colour_ptr_host[c];
+        if (cnt <= 0) continue;
+        MIFE_REQUIRE((long long)L.total + cnt < (1LL << 31), MIFE_ERR_ARG, "batch too large");
+        hs[ns++] = BatchSeg{k, J.colour_ptr_host[c], L.total, 0};
+        ++L.nseg;
+        L.total += cnt;
+      }
+      if (L.nseg > 0) launches.push_back(L);
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const size_t bytes = align256(sizeof(BatchJob) * n_jobs) + sizeof(BatchSeg) * (size_t)std::max(ns, 1);
+  MIFE_CHECK_HIP(hipMemcpyAsync(work, table_host, bytes, hipMemcpyHostToDevice, s));
+  MIFE_CHECK_HIP(hipStreamSynchronize(s));  // the host table may be reused at once
+  const BatchJob* dj = (const BatchJob*)work;
+  const BatchSeg* ds = (const BatchSeg*)((char*)work + align256(sizeof(BatchJob) * n_jobs));
+  constexpr int WPB = 4;
+  const int tk = prof::begin(s);
+  for (const Launch& L : launches) {
+    const int grid = std::max(1, std::min(div_up(L.total, WPB), kBatchGrid));
+    if (L.kind == MIFE_JOB_SCALAR) {
+      if (prec == MIFE_F64)
+        scalar_batch_kernel<double, WPB><<<grid, 64 * WPB, 0, s>>>(dj, ds + L.seg0, L.nseg, L.total,
+                                                                   scalar_desc->dN, scalar_desc->N, scalar_desc->w);
+      else
+        scalar_batch_kernel<float, WPB><<<grid, 64 * WPB, 0, s>>>(dj, ds + L.seg0, L.nseg, L.total,
+                                                                  scalar_desc->dN, scalar_desc->N, scalar_desc->w);
+    } else {
+      if (prec == MIFE_F64)
+        elastic_batch_kernel<double, WPB><<<grid, 64 * WPB, 0, s>>>(dj, ds + L.seg0, L.nseg, L.total,
+                                                                    elastic_desc->dN, elastic_desc->w);
+      else
+        elastic_batch_kernel<float, WPB><<<grid, 64 * WPB, 0, s>>>(dj, ds + L.seg0, L.nseg, L.total,
+                                                                   elastic_desc->dN, elastic_desc->w);
+    }
+  }
+  prof::end(tk, s, prof::kAssembleBatch);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
 }

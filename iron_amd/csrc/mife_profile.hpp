@@ -22,6 +22,7 @@ enum Kernel : int {
   kAssembleElastic32,
   kMinresSpmv,
   kGroupedReplace,
+  kAssembleBatch,
   kNumKernels
 };
 

@@ -229,7 +229,10 @@ class Server:
         self.cache_shapes = int(cache_shapes)
         self.graph = None
         self.stats = dict(admitted=0, retired=0, chunks=0, graph_captures=0, graph_replays=0, table_builds=0,
-                          iterations=0, spmv_bytes=0.0)
+                          iterations=0, spmv_bytes=0.0, assembly_batches=0)
+        self.q1 = fem.DeviceBasis(basis(3, 1, 2), dev)   # trilinear tables of the batched assembly
+        self.batch_dev = None
+        self.batch_host = None
         # host-side wall time per phase (seconds): structure builds, admission
         # (assembly + reduction + pool copies), iteration chunks, status reads, retirement
         self.timing = dict(structure=0.0, admit=0.0, chunk=0.0, status=0.0, retire=0.0)
@@ -263,10 +266,19 @@ class Server:
 
     # -- admission: assemble, reduce, store into the pools
     def admit(self, req: Request, now: float):
+        """Admits one request: pool ranges, assembly, reduction into the pools."""
+        e = self._prepare(req, now)
+        if e is not None:
+            self._assemble_batch([e])
+            self._finish(e)
+        return e
+
+    def _prepare(self, req: Request, now: float):
+        """Pool ranges, a slot and zeroed equations values for one request (None if
+        it does not fit)."""
         st = self._structure(req)
-        red, S = st.red, st.S
+        S = st.S
         nf, stored, ns = S.n, S.stored, S.desc.n_slices
-        c16 = S.col16 is not None
         want = ((self.alloc_rows, nf), (self.alloc_rp, ns + 1), (self.alloc_nz, stored), (self.alloc_cb, ns))
         offs = [a.alloc(n) for a, n in want]
         if any(o is None for o in offs):
@@ -277,13 +289,47 @@ class Server:
         ro, rpo, nzo, cbo = offs
         slot = self.free_slots.pop()
         coef = st.coefficients(req.seed, self.device)
-        vals = st.csr.zeros()
-        st.assemble(coef, vals)
+        req.t_admit = now
+        self.stats["admitted"] += 1
+        return dict(req=req, st=st, nf=nf, nnz=st.red.nnz, stored=stored, ns=ns, ro=ro, rpo=rpo, nzo=nzo, cbo=cbo,
+                    slot=slot, coef=coef, vals=st.csr.zeros(), mixed=req.kind in self.mixed_kinds)
+
+    def _assemble_batch(self, entries):
+        """FP64 equations values of every entry in one launch per colour and element
+        kind (mife_assemble_batch)."""
+        jobs = (_lib.AssemblyJob * len(entries))()
+        maxc = 1
+        for k, e in enumerate(entries):
+            st = e["st"]
+            cs, asm = st.csr, st.asm
+            kind = _lib.MIFE_JOB_SCALAR if st.kind == "diffusion" else _lib.MIFE_JOB_ELASTIC
+            jobs[k] = _lib.AssemblyJob(kind, asm.n_elem, ptr(asm.coords), ptr(asm.elem_nodes), ptr(cs.elem_dofs),
+                                       ptr(e["coef"]), None, ptr(cs.pos), ptr(cs.order), cs.colour_ptr.ctypes.data,
+                                       cs.n_colours, 0, ptr(e["vals"]), None)
+            maxc = max(maxc, cs.n_colours)
+        L = _lib.lib()
+        nb = int(L.mife_assemble_batch_work_bytes(len(entries), maxc))
+        if self.batch_dev is None or self.batch_dev.numel() < nb:
+            self.batch_dev = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            self.batch_host = np.empty(nb, dtype=np.uint8)
+        desc = C.byref(self.q1.desc)
+        call("mife_assemble_batch", desc, desc, len(entries), jobs, maxc, fem.F64, self.batch_host.ctypes.data,
+             ptr(self.batch_dev), fem._stream())
+        self.stats["assembly_batches"] += 1
+
+    def _finish(self, e):
+        """Solver values, slice structure and right-hand side of an assembled entry
+        into the pools; its segment."""
+        st, vals = e["st"], e.pop("vals")
+        e.pop("coef")
+        red, S = st.red, st.S
+        nf, stored, ns = e["nf"], e["stored"], e["ns"]
+        ro, rpo, nzo, cbo = e["ro"], e["rpo"], e["nzo"], e["cbo"]
+        c16 = S.col16 is not None
         s = fem._stream()
         # solver values straight from the equations CSR into the value pool
         call("mife_gather_values", stored, ptr(S.src), ptr(vals), fem.F64, ptr(self.val) + 8 * nzo, fem.F64, s)
-        mixed = req.kind in self.mixed_kinds
-        if mixed:   # the FP32 operator is the FP64 one rounded
+        if e["mixed"]:   # the FP32 operator is the FP64 one rounded
             call("mife_gather_values", stored, ptr(S.src), ptr(vals), fem.F64, ptr(self.val32) + 4 * nzo, fem.F32, s)
         self.rp[rpo:rpo + ns + 1].copy_(S.slice_ptr[: ns + 1])
         if c16:
@@ -296,12 +342,9 @@ class Server:
              ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(b), s)
         self.b[ro:ro + nf].copy_(b[st.perm])          # slice numbering
         self.x[ro:ro + nf].zero_()
-        seg = Segment(nf, 200000, slot, _lib.MIFE_FMT_SELL, ro, rpo, nzo, req.rtol, ns,
-                      fem.F32 if mixed else fem.F64, cbo if c16 else -1)
-        req.t_admit = now
-        self.stats["admitted"] += 1
-        return dict(req=req, seg=seg, st=st, nf=nf, nnz=red.nnz, stored=stored, ns=ns, cbo=cbo, cbo_used=c16,
-                    mixed=mixed)
+        e["seg"] = Segment(nf, 200000, e["slot"], _lib.MIFE_FMT_SELL, ro, rpo, nzo, e["req"].rtol, ns,
+                           fem.F32 if e["mixed"] else fem.F64, cbo if c16 else -1)
+        e["cbo_used"] = c16
 
     def _build_table(self, entries, table):
         segs = (Segment * max(len(entries), 1))(*[e["seg"] for e in entries])
@@ -391,12 +434,16 @@ class Server:
                         max_batch is None or len(active) + len(new) < max_batch):
                     if new and (not (launched or pending) or time.perf_counter() - ta > self.admit_budget):
                         break
-                    e = self.admit(queue[qi], now) if self.fits(queue[qi]) else None
+                    e = self._prepare(queue[qi], now) if self.fits(queue[qi]) else None
                     if e is None:
                         break
                     new.append(e)
                     qi += 1
                 if new:
+                    # the arrivals of this round: one batched assembly, then each into the pools
+                    self._assemble_batch(new)
+                    for e in new:
+                        self._finish(e)
                     nbi = self._build_table(new, self.init_table)
                     self._init(len(new), nbi, self.init_table)
                     pending += new

@@ -59,6 +59,19 @@ def test_argument_errors_without_gpu():
     buf = (ctypes.c_char * len(stage)).from_buffer(stage)
     assert L.mife_grouped_setup(ctypes.byref(seg), 1, 1, 4, ctypes.byref(nb), buf, buf, None) == 2
     assert "precision" in _lib.last_error()
+    # batched assembly: nothing to do, then an unknown job kind and a colour table that
+    # does not cover the elements (checked before any device work)
+    assert L.mife_assemble_batch(None, None, 0, None, 0, 0, None, None, None) == 0
+    job = (_lib.AssemblyJob * 1)()
+    job[0].kind, job[0].n_elem = 5, 0
+    host = bytearray(int(L.mife_assemble_batch_work_bytes(1, 8)))
+    hb = (ctypes.c_char * len(host)).from_buffer(host)
+    assert L.mife_assemble_batch(None, None, 1, job, 8, 0, hb, hb, None) == 2
+    assert "kind" in _lib.last_error()
+    cptr = (ctypes.c_int32 * 3)(0, 4, 9)
+    job[0] = _lib.AssemblyJob(0, 10, 1, 1, 1, None, None, 1, 1, ctypes.addressof(cptr), 2, 0, 1, None)
+    assert L.mife_assemble_batch(None, None, 1, job, 8, 0, hb, hb, None) == 2
+    assert "colour pointers" in _lib.last_error()
     # invalid arguments on a well-formed (host-side) descriptor
     d = M(10, 7, 1, 1, None, 0, 10, None, None, 0)
     assert L.mife_spmv(ctypes.byref(d), 1, 1, 1, 0, None) == 2     # unknown format 7
