@@ -11,7 +11,7 @@ every boundary face.  One step is one full pass of the path:
 The plain FP64 drop-in (FP64 assembly + FP64 Jacobi CG to 1e-8) is timed in the
 same run, and the C restatement of iron's CPU algorithm on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16] [--legs main,mr,serve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16] [--legs main,mr,elastic,laplace,serve]
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -48,8 +48,9 @@ def parse():
                          "writes 268,435,456 bytes each: the reference kernel for HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=128,
                     help="requests of the served-stream leg (the full stream has 1024); 0 skips it")
-    ap.add_argument("--legs", default="main,mr,serve",
-                    help="comma-separated legs to run: main (the workload above), mr, serve")
+    ap.add_argument("--legs", default="main,mr,elastic,laplace,serve",
+                    help="comma-separated legs to run: main (the workload above), mr, elastic (96x32x32 "
+                         "elasticity), laplace (16^3 Laplace), serve")
     return ap.parse_args()
 
 
@@ -353,6 +354,118 @@ def mr_leg(runs, cpu_seconds):
     return out
 
 
+def elastic_leg(runs=3):
+    """96x32x32 trilinear linear elasticity on [0,3]x[0,1]^2: lognormal Young's modulus
+    10 exp(0.3 N(0,1)) per element (seed 7), Poisson ratio 0.3, clamp at x = 0,
+    consistent traction t_z = -1 on x = 3.  One pass: FP64 and FP32 (MFMA) element
+    stiffness, solver values and RHS, FP32 Jacobi CG with FP64 refinement to a 1e-8
+    relative residual (rows node by node, 16-bit SELL columns), field update.  Also
+    the FP64 drop-in CG on the same system."""
+    import torch
+    from iron_amd import _lib, basis, engine, fem, mesh
+    ne = (96, 32, 32)
+    b = basis.basis(3, basis.LINEAR_LAGRANGE)
+    m = mesh.RegularMesh(ne, [b], (3.0, 1.0, 1.0))
+    dm = mesh.DofMap(m, [0, 0, 0])
+    dev = torch.device("cuda:0")
+    X = m.coords
+    E, N = m.num_elements, m.num_nodes(0)
+    g = torch.Generator().manual_seed(7)
+    Ee = 10.0 * torch.exp(0.3 * torch.randn(E, dtype=torch.float64, generator=g)).numpy()
+    mat = torch.as_tensor(np.ascontiguousarray(np.stack([Ee, Ee, Ee, np.full(E, 0.3), np.full(E, 0.3),
+                                                         np.full(E, 0.3)], axis=1)), device=dev)
+    csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, dev, colours=mesh.structured_colours(m))
+    asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], csr)
+    clamp = X[:, 0] == 0.0
+    red = fem.DirichletReduction(csr, np.concatenate([clamp, clamp, clamp]))
+    S = fem.SolverMatrix(red, interleave=(3, N))
+    tip = np.nonzero(X[:, 0] == 3.0)[0]
+    ft = torch.as_tensor(engine.neumann_integrate(m, dm, [b] * 3, {int(2 * N + i): -1.0 for i in tip}), device=dev)
+    v, v32 = csr.zeros(fem.F64), csr.zeros(fem.F32)
+    sv64 = torch.empty(max(S.stored, 1), dtype=torch.float64, device=dev)
+    sv32 = torch.empty(max(S.stored, 1), dtype=torch.float32, device=dev)
+    u = torch.zeros(csr.n, dtype=torch.float64, device=dev)
+    bvec = torch.empty(red.n_free, dtype=torch.float64, device=dev)
+    x = torch.zeros(red.n_free, dtype=torch.float64, device=dev)
+    ws = fem.KrylovWorkspace(red.n_free, dev)
+
+    def one(mixed=True):
+        v.zero_()
+        asm.assemble(v, mat)
+        S.gather(v, fem.F64, out=sv64)
+        u.zero_()
+        red.rhs(v, u, rhs=ft, out=bvec)
+        x.zero_()
+        if mixed:
+            v32.zero_()
+            asm.assemble(v32, mat, prec=fem.F32)
+            S.gather(v32, fem.F32, out=sv32)
+            info = fem.refine(S, sv64, sv32, bvec, x, 1e-8, 200000, ws)
+        else:
+            info = fem.cg(S, sv64, bvec, x, 1e-8, 200000, ws, prec=fem.F64)
+        red.field_update(x, u)
+        return info
+
+    res = {}
+    for mixed in (True, False):
+        one(mixed)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(runs):
+            info = one(mixed)
+        torch.cuda.synchronize()
+        res[mixed] = ((time.perf_counter() - t) / runs, info)
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    one(True)
+    torch.cuda.synchronize()
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    n_s, ms_s = prof.get("cg_spmv_f32", (0, 0.0))
+    per = ms_s / max(n_s, 1)
+    alg = 8.0 * red.nnz + 24.0 * red.n_free   # values + int32 columns, z, p/q read, p/q written
+    t_mx, info_mx = res[True]
+    t_64, info_64 = res[False]
+    a64 = prof.get("assemble_elastic", (0, 0.0))[1]
+    a32 = prof.get("assemble_elastic_f32", (0, 0.0))[1]
+    return {
+        "workload": "elasticity96: 96x32x32 Q1 hex on [0,3]x[0,1]^2, 2x2x2 Gauss, E = 10 exp(0.3 N(0,1)) per element "
+                    "(seed 7), nu = 0.3, clamp at x=0, traction t_z=-1 on x=3; FP32 element stiffness (MFMA) + FP32 "
+                    "Jacobi CG with FP64 refinement to 1e-8",
+        "elements": E, "free_dofs": red.n_free, "solver_nnz": red.nnz, "sell_padding": S.padding,
+        "ms_per_pass": t_mx * 1e3, "elems_per_s": E / t_mx, "iterations_inner": info_mx.iterations,
+        "refinement_steps": info_mx.outer, "relres": info_mx.relres,
+        "fp64_ms_per_pass": t_64 * 1e3, "fp64_iterations": info_64.iterations, "fp64_relres": info_64.relres,
+        "assembly_ms_f64": a64, "assembly_ms_f32": a32,
+        "assembly_elems_per_s_f32": E / (a32 * 1e-3) if a32 > 0 else None,
+        "roofline": {"kernel": "cg_spmv<float> (fused SpMV + p/q update + p.q, 16-bit columns)", "bound": "hbm",
+                     "achieved": alg / (per * 1e-3) / 1e9 if per > 0 else 0.0, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (alg / (per * 1e-3) / 1e9 / HBM_PEAK_GBS) if per > 0 else 0.0,
+                     "bytes_per_launch": alg, "avg_launch_ms": per, "launches": n_s},
+    }
+
+
+def laplace_leg(args, runs=5):
+    """16^3 trilinear Laplace, u = 0 at x = 0 and u = 1 at x = 1: the full pass (FP64
+    assembly, solver values, FP32 CG with FP64 refinement to 1e-10, field update), and
+    the FP64 drop-in CG."""
+    import argparse
+    import torch
+    a = argparse.Namespace(**vars(args))
+    a.workload = "laplace16"
+    P = build_problem(a)
+    out = {"workload": WORKLOAD_TEXT["laplace16"], "elements": P["m"].num_elements, "free_dofs": P["red"].n_free}
+    for mixed, key in ((True, "mixed"), (False, "fp64")):
+        timed_steps(P, 1, mixed, 0.1)
+        wall, lat, infos = timed_steps(P, runs, mixed, 0.1)
+        out[f"{key}_ms_per_pass"] = wall / runs * 1e3
+        out[f"{key}_elems_per_s"] = P["m"].num_elements * runs / wall
+        out[f"{key}_iterations"] = infos[-1].iterations
+        out[f"{key}_relres"] = infos[-1].relres
+    torch.cuda.synchronize()
+    return out
+
+
 def serve_leg(n, seed=2024):
     """Served stream: n requests, 70% diffusion / 30% elasticity trilinear boxes with
     N_x, N_y, N_z ~ U{8..64}, lognormal(0,1) coefficients (per-request seeds), BCs of
@@ -465,6 +578,10 @@ def main():
     if "mr" in legs and not args.no_mr:
         out["mooney_rivlin"] = mr_leg(2, 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 10.0))
         torch.cuda.empty_cache()
+    if "elastic" in legs:
+        out["elasticity96"] = elastic_leg()
+    if "laplace" in legs:
+        out["laplace16"] = laplace_leg(args)
     if "serve" in legs and args.serve_requests > 0:
         out["served_stream"] = serve_leg(args.serve_requests)
     print(json.dumps(out))

@@ -436,7 +436,8 @@ __global__ __launch_bounds__(256) void gmx_replace_kernel(View V, Vecs W, int n_
 }
 
 // Reliable update, part 3 (one block per paused segment): beta from the
-// replaced r.z, the true residual and the stopping test; resumes the segment.
+// replaced r.z (0 when the replaced r.r exceeds the recurrence's by
+// kRestartGrowth), the true residual and the stopping test; resumes the segment.
 __global__ __launch_bounds__(256) void gfin_replace_kernel(View V, const int32_t* __restrict__ seg_parts,
                                                            const int32_t* __restrict__ seg_ids) {
   const int s = seg_ids[blockIdx.x];
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(256) void gfin_replace_kernel(View V, const int32_t
   seg_sums<3>(V, seg_parts, blockIdx.x, out);
   if (threadIdx.x == 0) {
     SegState& g = V.st[s];
-    g.beta = g.rz_prev != 0.0 ? out[0] / g.rz_prev : 0.0;
+    g.beta = (g.rz_prev != 0.0 && out[1] <= kRestartGrowth * g.rr) ? out[0] / g.rz_prev : 0.0;
     g.rz = out[0];
     g.rr = out[1];
     g.rr_rep = out[1];

@@ -304,7 +304,8 @@ __global__ void fold_correction_kernel(int n, double* __restrict__ x64, float* _
 
 // Reliable update, part 2 (after residual64 into r64 and `ro`):
 // r32 = r64, z = D^-1 r32; the FP32 recurrence restarts from the true residual
-// while p and q = A p are kept, beta = (r.z)_new / (r.z)_before-the-step.
+// while p and q = A p are kept, beta = (r.z)_new / (r.z)_before-the-step (or 0
+// when the replaced r.r exceeds the recurrence's by kRestartGrowth).
 __global__ __launch_bounds__(256) void replace_residual_kernel(int n, const double* __restrict__ r64,
                                                                float* __restrict__ r, float* __restrict__ z,
                                                                const float* __restrict__ dinv, CgState* st) {
@@ -338,7 +339,7 @@ __global__ void fin_replace_kernel(CgState* st, const CgState* ro, int nb, int f
   double out[2];
   finish_sums<2>(st->part_b, nb, out);
   if (threadIdx.x == 0) {
-    st->beta = first ? 0.0 : out[0] / st->rz_prev;
+    st->beta = (first || out[1] > kRestartGrowth * st->rr) ? 0.0 : out[0] / st->rz_prev;
     st->rz = out[0];
     st->rr = out[1];
     st->maxrr = out[1];

@@ -26,6 +26,14 @@ constexpr int kXcds = 8;
 // Internal format tag of SELL with 16-bit slice-relative columns.
 constexpr int kFmtSell16 = 2;
 
+// Reliable updates: a replaced residual whose r.r exceeds the recurrence's by
+// more than this factor restarts the recurrence (beta = 0).  Once the FP32
+// recurrence has run far below the true residual (an FP32 solve that converged
+// early, as on an exactly representable solution), keeping the search
+// direction with beta = (r.z)_replaced / (r.z)_before-the-step lets the stale
+// direction swamp the new residual and the iteration diverges.
+constexpr double kRestartGrowth = 16.0;
+
 // Per-XCD range length of a SELL schedule: ceil(n_slices / 8) rounded up to 4.
 inline int sell_order_span(int n_slices) { return 4 * div_up(div_up(n_slices, kXcds), 4); }
 

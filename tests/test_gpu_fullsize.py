@@ -51,6 +51,32 @@ def test_c1_laplace16_u_equals_x():
     assert abs(R[X[:, 0] == 1.0].sum() - 1.0) < 1e-8
 
 
+def test_c1_laplace16_mixed_refinement_converges():
+    """FP32 CG with FP64 reliable updates to 1e-10 on the C1 system.  The exact
+    solution is linear, so the FP32 recurrence converges far below the true residual
+    within the first segment; the replacement then restarts the recurrence instead of
+    keeping a stale direction (which diverged)."""
+    print("case C1 16^3 Q1 Laplace, mixed refinement", flush=True)
+    fem, b, m, dm, csr = _linear(1, (16, 16, 16))
+    X = m.coords
+    fixed = (X[:, 0] == 0.0) | (X[:, 0] == 1.0)
+    asm = fem.ScalarAssembler(b, X, m.elem_nodes[0], csr)
+    v, v32 = csr.zeros(), csr.zeros(fem.F32)
+    asm.assemble(v)
+    asm.assemble(v32, prec=fem.F32)
+    red = fem.DirichletReduction(csr, fixed)
+    S = fem.SolverMatrix(red)
+    u = torch.as_tensor(np.where(X[:, 0] == 1.0, 1.0, 0.0), device="cuda")
+    bvec = red.rhs(v, u)
+    x = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    info = fem.refine(S, S.gather(v), S.gather(v32, fem.F32), bvec, x, 1e-10, 20000,
+                      fem.KrylovWorkspace(red.n_free, "cuda"))
+    red.field_update(x, u)
+    assert info.relres <= 1e-10
+    assert info.iterations <= 200, info.iterations
+    assert np.abs(u.cpu().numpy() - X[:, 0]).max() < 1e-8
+
+
 def test_c2_diffusion48_mixed_solve_balance():
     print("case C2 48^3 Q2 diffusion", flush=True)
     fem, b, m, dm, csr = _linear(2, (48, 48, 48))
