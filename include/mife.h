@@ -372,15 +372,17 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b, double
                const double* pc_inv, double rtol, int restart, int maxit, void* work,
                int* iters, double* relres, void* stream);
 
-/* Preconditioned MINRES (Paige-Saunders) in FP64 for symmetric, possibly
- * indefinite A, such as the Newton Jacobian of the Mooney-Rivlin deviatoric
- * formulation.  SPD diagonal preconditioner |1/a_ii|, or pc_inv[i] where nonzero
- * (as for mife_gmres, absolute value taken).  Stops when the preconditioned
- * residual norm has fallen by rtol or after maxit iterations; relres returns the
- * true ||b - A x|| / ||b|| at exit.  No reference counterpart (iron's Newton
- * solves go through KSP, src/solver_routines.F90:18674-18763).  Workspace:
- * mife_solver_work_bytes(n, 1). */
-int mife_minres(const mife_matrix* A, const double* vals, const double* b, double* x,
+/* Preconditioned MINRES (Paige-Saunders) for symmetric, possibly indefinite A,
+ * such as the Newton Jacobian of the Mooney-Rivlin deviatoric formulation.
+ * Vectors and scalars are FP64; vals are FP64 or FP32 (prec), FP32 values being
+ * read into FP64 arithmetic, which halves the value stream of every product.
+ * SPD diagonal preconditioner |1/a_ii|, or pc_inv[i] where nonzero (as for
+ * mife_gmres, absolute value taken).  Stops when the preconditioned residual
+ * norm has fallen by rtol or after maxit iterations; relres returns the true
+ * ||b - A x|| / ||b|| of the operator given, at exit.  No reference counterpart
+ * (iron's Newton solves go through KSP, src/solver_routines.F90:18674-18763).
+ * Workspace: mife_solver_work_bytes(n, 1). */
+int mife_minres(const mife_matrix* A, const void* vals, int prec, const double* b, double* x,
                 const double* pc_inv, double rtol, int maxit, void* work, int* iters,
                 double* relres, void* stream);
 

@@ -112,7 +112,7 @@ _SIG = {
     "mife_refine": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _D, _I, _P, C.POINTER(_I), C.POINTER(_I),
                          C.POINTER(_D), _P]),
     "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
-    "mife_minres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_minres": (_I, [C.POINTER(MatrixDesc), _P, _I, _P, _P, _P, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_assemble_batch_work_bytes": (C.c_size_t, [_I, _I]),
     "mife_assemble_batch": (_I, [C.POINTER(ElementDesc), C.POINTER(ElementDesc), _I, C.POINTER(AssemblyJob), _I, _I,
                                  _P, _P, _P]),

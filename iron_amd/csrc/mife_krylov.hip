@@ -82,8 +82,8 @@ __device__ __forceinline__ void finish_sums(const double* __restrict__ parts, in
   }
 }
 
-template <typename T>
-__global__ void diag_inv_kernel(MatArgs A, const T* __restrict__ vals, T* __restrict__ dinv) {
+template <typename T, typename TV = T>
+__global__ void diag_inv_kernel(MatArgs A, const TV* __restrict__ vals, T* __restrict__ dinv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   T d = 0;
@@ -102,7 +102,7 @@ __global__ void diag_inv_kernel(MatArgs A, const T* __restrict__ vals, T* __rest
         const int e = base + (j0 + u) * kSlice + lane;
         const bool in = j0 + u < len;
         cj[u] = in ? (A.fmt == kFmtSell16 ? cb + (int)c16[e] : A.col[e]) : -1;
-        vj[u] = in ? vals[e] : (T)0;
+        vj[u] = in ? (T)vals[e] : (T)0;
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
@@ -114,7 +114,7 @@ __global__ void diag_inv_kernel(MatArgs A, const T* __restrict__ vals, T* __rest
       const int mid = (lo + hi) >> 1;
       if (A.col[mid] < i) lo = mid + 1; else hi = mid;
     }
-    if (lo < A.ptr[i + 1] && A.col[lo] == i) d = vals[lo];
+    if (lo < A.ptr[i + 1] && A.col[lo] == i) d = (T)vals[lo];
   }
   dinv[i] = (d != (T)0) ? (T)1 / d : (T)1;
 }
@@ -252,13 +252,14 @@ __global__ void fin_beta_kernel(CgState* st, int nb) {
   }
 }
 
-// FP64 residual r = b - A x with ||r||^2 and ||b||^2.
-template <int FMT, int L>
-__global__ __launch_bounds__(256) void residual64_kernel(MatArgs A, const double* __restrict__ vals,
+// FP64 residual r = b - A x with ||r||^2 and ||b||^2 (values FP64, or FP32 read
+// into FP64 arithmetic).
+template <int FMT, int L, typename TV = double>
+__global__ __launch_bounds__(256) void residual64_kernel(MatArgs A, const TV* __restrict__ vals,
                                                          const double* __restrict__ b, const double* __restrict__ x,
                                                          double* __restrict__ r, CgState* st) {
   double m[2] = {0.0, 0.0};
-  for_each_row<double, FMT, L>(A, vals, x, [&](int row, double ax) {
+  for_each_row<double, FMT, L, TV>(A, vals, x, [&](int row, double ax) {
     const double ri = b[row] - ax;
     if (r) r[row] = ri;
     m[0] += ri * ri;
@@ -453,8 +454,8 @@ __device__ __forceinline__ double block_fixed_sum(const double* __restrict__ par
   return block_sum(acc, red);
 }
 
-template <int FMT, int L>
-__global__ __launch_bounds__(256) void minres_a_kernel(MatArgs A, const double* __restrict__ vals,
+template <int FMT, int L, typename TV>
+__global__ __launch_bounds__(256) void minres_a_kernel(MatArgs A, const TV* __restrict__ vals,
                                                        const double* __restrict__ z, double* __restrict__ v,
                                                        double* __restrict__ t, const double* __restrict__ r1,
                                                        double* __restrict__ w, double* __restrict__ w1,
@@ -502,7 +503,7 @@ __global__ __launch_bounds__(256) void minres_a_kernel(MatArgs A, const double* 
   const bool done = S.done != 0;
   const double ib = beta > 0.0 ? 1.0 / beta : 0.0, rb = S.oldb != 0.0 ? beta / S.oldb : 0.0;
   double m[1] = {0.0};
-  for_each_row<double, FMT, L>(A, vals, done ? nullptr : z, [&](int row, double az) {
+  for_each_row<double, FMT, L, TV>(A, vals, done ? nullptr : z, [&](int row, double az) {
     if (pend) {
       const double wo = w[row], w2o = w2[row];
       const double wn = (v[row] - oldeps * w2o - delta * wo) * denom;
@@ -553,15 +554,15 @@ __global__ __launch_bounds__(256) void minres_b_kernel(int n, double* __restrict
 }
 
 // r1 = r2 = b - A x, z = D^-1 r1, w = w2 = 0 ; partials r1.z and b.b ; scalars reset
-template <int FMT, int L>
-__global__ __launch_bounds__(256) void minres_init_kernel(MatArgs A, const double* __restrict__ vals,
+template <int FMT, int L, typename TV>
+__global__ __launch_bounds__(256) void minres_init_kernel(MatArgs A, const TV* __restrict__ vals,
                                                           const double* __restrict__ b, const double* __restrict__ x,
                                                           const double* __restrict__ dinv, double* __restrict__ r1,
                                                           double* __restrict__ r2, double* __restrict__ z,
                                                           double* __restrict__ w, double* __restrict__ w2,
                                                           MinresState* st, double tol, int maxit) {
   double m0 = 0.0, m1 = 0.0;
-  for_each_row<double, FMT, L>(A, vals, x, [&](int row, double ax) {
+  for_each_row<double, FMT, L, TV>(A, vals, x, [&](int row, double ax) {
     const double ri = b[row] - ax;
     const double zi = dinv[row] * ri;
     r1[row] = ri;
@@ -671,11 +672,12 @@ static void launch_cg_iter(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, 
   fin_beta_kernel<<<1, kFinThreads, 0, s>>>(st, vg);
 }
 
-static void launch_residual64(const Mat& M, const double* vals, const double* b, const double* x, double* r,
+template <typename TV = double>
+static void launch_residual64(const Mat& M, const TV* vals, const double* b, const double* x, double* r,
                               CgState* st, hipStream_t s) {
   const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   const int tk = prof::begin(s);
-  MIFE_DISPATCH_FMT(M.a, M.L, (residual64_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, r, st)));
+  MIFE_DISPATCH_FMT(M.a, M.L, (residual64_kernel<FMT_, L_, TV><<<grid, 256, 0, s>>>(M.a, vals, b, x, r, st)));
   prof::end(tk, s, prof::kResidual64);
   fin_residual_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
 }
@@ -919,10 +921,16 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
   return MIFE_OK;
 }
 
-int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, const double* pc_inv,
+}  // extern "C"
+
+namespace {
+
+// MINRES with FP64 vectors and scalars on FP64 values, or on FP32 values read
+// into FP64 arithmetic (TV = float: the matrix stream, nearly all the bytes of
+// an iteration, shrinks from 8 to 4 bytes per value).
+template <typename TV>
+int minres_impl(const mife_matrix* A, const TV* vals, const double* b_in, double* x_io, const double* pc_inv,
                 double rtol, int maxit, void* work, int* iters, double* relres, void* stream) {
-  MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
-  MIFE_REQUIRE(rtol > 0.0 && maxit >= 0, MIFE_ERR_ARG, "invalid MINRES arguments");
   Mat M;
   int rc = make_mat(A, &M);
   if (rc) return rc;
@@ -946,12 +954,12 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
   const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
-  diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
+  diag_inv_kernel<double, TV><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
   pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 1);
   fill_kernel<double><<<vg, 256, 0, s>>>(n, v, 0.0);
   fill_kernel<double><<<vg, 256, 0, s>>>(n, w1, 0.0);
-  MIFE_DISPATCH_FMT(M.a, M.L, (minres_init_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, b, x, dinv, r1, r2, z,
-                                                                                   wv, w2, st, rtol, maxit)));
+  MIFE_DISPATCH_FMT(M.a, M.L, (minres_init_kernel<FMT_, L_, TV><<<grid, 256, 0, s>>>(M.a, vals, b, x, dinv, r1, r2,
+                                                                                       z, wv, w2, st, rtol, maxit)));
   MIFE_CHECK_HIP(hipGetLastError());
   int done = 0, it = 0, k = 0;
   const int chunk = 32;
@@ -959,8 +967,8 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
     for (int c = 0; c < chunk; ++c, ++k) {
       const int nb = (k == 0) ? grid : vg;
       const int tk = prof::begin(s);
-      MIFE_DISPATCH_FMT(M.a, M.L, (minres_a_kernel<FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, v, t, r1, wv, w1, w2,
-                                                                                    x, st, k, nb)));
+      MIFE_DISPATCH_FMT(M.a, M.L, (minres_a_kernel<FMT_, L_, TV><<<grid, 256, 0, s>>>(M.a, vals, z, v, t, r1, wv, w1,
+                                                                                        w2, x, st, k, nb)));
       prof::end(tk, s, prof::kMinresSpmv);
       minres_b_kernel<<<vg, 256, 0, s>>>(n, t, r1, r2, z, dinv, st, k, grid);
     }
@@ -977,7 +985,7 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
   }
   // true residual at exit
   MIFE_CHECK_HIP(hipMemsetAsync(rs, 0, sizeof(CgState), s));
-  launch_residual64(M, vals, b, x, nullptr, rs, s);
+  launch_residual64<TV>(M, vals, b, x, nullptr, rs, s);
   HostState hs{};
   rc = read_state(rs, &hs, s);
   if (rc) return rc;
@@ -991,6 +999,20 @@ int mife_minres(const mife_matrix* A, const double* vals, const double* b_in, do
     return MIFE_ERR_NOT_CONVERGED;
   }
   return MIFE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mife_minres(const mife_matrix* A, const void* vals, int prec, const double* b_in, double* x_io,
+                const double* pc_inv, double rtol, int maxit, void* work, int* iters, double* relres, void* stream) {
+  MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(rtol > 0.0 && maxit >= 0, MIFE_ERR_ARG, "invalid MINRES arguments");
+  MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
+  if (prec == MIFE_F32)
+    return minres_impl<float>(A, (const float*)vals, b_in, x_io, pc_inv, rtol, maxit, work, iters, relres, stream);
+  return minres_impl<double>(A, (const double*)vals, b_in, x_io, pc_inv, rtol, maxit, work, iters, relres, stream);
 }
 
 int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, const double* pc_inv,

@@ -74,40 +74,40 @@ __device__ __forceinline__ T group_sum(T v) {
 }
 
 // CSR partial sum of `row` held by lane `sub` of its L-lane group (not reduced).
-template <typename T, int L>
+template <typename T, int L, typename TV = T>
 __device__ __forceinline__ T row_partial(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                         const T* __restrict__ vals, const T* __restrict__ x, int row, int sub) {
+                                         const TV* __restrict__ vals, const T* __restrict__ x, int row, int sub) {
   const int k0 = rp[row], k1 = rp[row + 1];
   T a0 = 0, a1 = 0;
   int k = k0 + sub;
   for (; k + 3 * L < k1; k += 4 * L) {
     const int c0 = stream_load(&col[k]), c1 = stream_load(&col[k + L]);
     const int c2 = stream_load(&col[k + 2 * L]), c3 = stream_load(&col[k + 3 * L]);
-    const T v0 = stream_load(&vals[k]), v1 = stream_load(&vals[k + L]);
-    const T v2 = stream_load(&vals[k + 2 * L]), v3 = stream_load(&vals[k + 3 * L]);
+    const T v0 = (T)stream_load(&vals[k]), v1 = (T)stream_load(&vals[k + L]);
+    const T v2 = (T)stream_load(&vals[k + 2 * L]), v3 = (T)stream_load(&vals[k + 3 * L]);
     a0 += v0 * x[c0];
     a1 += v1 * x[c1];
     a0 += v2 * x[c2];
     a1 += v3 * x[c3];
   }
-  for (; k < k1; k += L) a0 += stream_load(&vals[k]) * x[stream_load(&col[k])];
+  for (; k < k1; k += L) a0 += (T)stream_load(&vals[k]) * x[stream_load(&col[k])];
   return a0 + a1;
 }
 
 // SELL-64 row sum of slice `s`, lane `lane` (one row per lane, complete sum),
 // eight entries in flight per lane.
-template <typename T>
+template <typename T, typename TV = T>
 __device__ __forceinline__ T sell_row(const int32_t* __restrict__ sp, const int32_t* __restrict__ col,
-                                      const T* __restrict__ vals, const T* __restrict__ x, int s, int lane) {
+                                      const TV* __restrict__ vals, const T* __restrict__ x, int s, int lane) {
   const int base = sp[s];
   const int len = (sp[s + 1] - base) / kSlice;
   const int32_t* c = col + base + lane;
-  const T* v = vals + base + lane;
+  const TV* v = vals + base + lane;
   T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   int j = 0;
   for (; j + 7 < len; j += 8) {
     int ci[8];
-    T vi[8];
+    TV vi[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) ci[u] = stream_load(c + u * kSlice);
 #pragma unroll
@@ -124,7 +124,7 @@ __device__ __forceinline__ T sell_row(const int32_t* __restrict__ sp, const int3
     v += 8 * kSlice;
   }
   for (; j < len; ++j) {
-    a0 += stream_load(v) * x[stream_load(c)];
+    a0 += (T)stream_load(v) * x[stream_load(c)];
     c += kSlice;
     v += kSlice;
   }
@@ -133,20 +133,20 @@ __device__ __forceinline__ T sell_row(const int32_t* __restrict__ sp, const int3
 
 // As sell_row with 16-bit columns relative to the slice's base column; eight
 // entries in flight per lane (the 2-byte columns leave registers for it).
-template <typename T>
+template <typename T, typename TV = T>
 __device__ __forceinline__ T sell16_row(const int32_t* __restrict__ sp, const uint16_t* __restrict__ col,
-                                        const int32_t* __restrict__ cbase, const T* __restrict__ vals,
+                                        const int32_t* __restrict__ cbase, const TV* __restrict__ vals,
                                         const T* __restrict__ x, int s, int lane) {
   const int base = sp[s];
   const int len = (sp[s + 1] - base) / kSlice;
   const T* xs = x + cbase[s];
   const uint16_t* c = col + base + lane;
-  const T* v = vals + base + lane;
+  const TV* v = vals + base + lane;
   T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   int j = 0;
   for (; j + 7 < len; j += 8) {
     int ci[8];
-    T vi[8];
+    TV vi[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) ci[u] = stream_load(c + u * kSlice);
 #pragma unroll
@@ -163,7 +163,7 @@ __device__ __forceinline__ T sell16_row(const int32_t* __restrict__ sp, const ui
     v += 8 * kSlice;
   }
   for (; j < len; ++j) {
-    a0 += stream_load(v) * xs[(int)stream_load(c)];
+    a0 += (T)stream_load(v) * xs[(int)stream_load(c)];
     c += kSlice;
     v += kSlice;
   }
@@ -173,8 +173,8 @@ __device__ __forceinline__ T sell16_row(const int32_t* __restrict__ sp, const ui
 // Part h of W of a SELL row sum (int32 or 16-bit columns): the wavefront takes
 // chunks h, h + W, h + 2W, ... of eight entries; with W = 1 the order of the
 // additions is that of sell_row / sell16_row.
-template <typename T, bool C16>
-__device__ __forceinline__ T sell_row_part(const MatArgs& A, const T* __restrict__ vals, const T* __restrict__ x,
+template <typename T, bool C16, typename TV = T>
+__device__ __forceinline__ T sell_row_part(const MatArgs& A, const TV* __restrict__ vals, const T* __restrict__ x,
                                            int s, int lane, int h, int W) {
   constexpr int D = 8;
   const int base = A.ptr[s];
@@ -182,7 +182,7 @@ __device__ __forceinline__ T sell_row_part(const MatArgs& A, const T* __restrict
   const T* xs = C16 ? x + A.cbase[s] : x;
   const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col) + base + lane;
   const int32_t* c32 = A.col + base + lane;
-  const T* v = vals + base + lane;
+  const TV* v = vals + base + lane;
   auto col = [&](int j) -> int {
     if constexpr (C16) return (int)stream_load(c16 + j * kSlice);
     else return stream_load(c32 + j * kSlice);
@@ -191,7 +191,7 @@ __device__ __forceinline__ T sell_row_part(const MatArgs& A, const T* __restrict
   int j = h * D;
   for (; j + D <= len; j += W * D) {
     int ci[D];
-    T vi[D];
+    TV vi[D];
 #pragma unroll
     for (int u = 0; u < D; ++u) ci[u] = col(j + u);
 #pragma unroll
@@ -205,13 +205,13 @@ __device__ __forceinline__ T sell_row_part(const MatArgs& A, const T* __restrict
     a2 += vi[6] * xs[ci[6]];
     a3 += vi[7] * xs[ci[7]];
   }
-  for (int k = j; k < len && k < j + D; ++k) a0 += stream_load(v + k * kSlice) * xs[col(k)];
+  for (int k = j; k < len && k < j + D; ++k) a0 += (T)stream_load(v + k * kSlice) * xs[col(k)];
   return (a0 + a1) + (a2 + a3);
 }
 
 // Calls f(row, (A x)_row) once per row (by one lane), over the whole grid.
-template <typename T, int FMT, int L, typename F>
-__device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restrict__ vals, const T* __restrict__ x,
+template <typename T, int FMT, int L, typename TV = T, typename F>
+__device__ __forceinline__ void for_each_row(const MatArgs& A, const TV* __restrict__ vals, const T* __restrict__ x,
                                              F&& f) {
   if constexpr (FMT == MIFE_FMT_SELL || FMT == kFmtSell16) {
     const int wpb = blockDim.x >> 6, lane = threadIdx.x & 63;
@@ -219,9 +219,9 @@ __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restri
       T ax = (T)0;
       if (x) {
         if constexpr (FMT == kFmtSell16)
-          ax = sell16_row<T>(A.ptr, reinterpret_cast<const uint16_t*>(A.col), A.cbase, vals, x, s, lane);
+          ax = sell16_row<T, TV>(A.ptr, reinterpret_cast<const uint16_t*>(A.col), A.cbase, vals, x, s, lane);
         else
-          ax = sell_row<T>(A.ptr, A.col, vals, x, s, lane);
+          ax = sell_row<T, TV>(A.ptr, A.col, vals, x, s, lane);
       }
       const int row = s * kSlice + lane;
       if (row < A.n) f(row, ax);
@@ -247,7 +247,7 @@ __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restri
           continue;
         }
         T ax = (T)0;
-        if (s >= 0 && x) ax = sell_row_part<T, FMT == kFmtSell16>(A, vals, x, s, lane, h, W);
+        if (s >= 0 && x) ax = sell_row_part<T, FMT == kFmtSell16, TV>(A, vals, x, s, lane, h, W);
         part[wave][lane] = ax;
         __syncthreads();
         if (h == 0 && s >= 0) {
@@ -265,7 +265,7 @@ __device__ __forceinline__ void for_each_row(const MatArgs& A, const T* __restri
     const int gpb = blockDim.x / L, sub = threadIdx.x % L;
     for (int row = blockIdx.x * gpb + threadIdx.x / L; row < A.n; row += gridDim.x * gpb) {
       T ax = 0;
-      if (x) ax = group_sum<T, L>(row_partial<T, L>(A.ptr, A.col, vals, x, row, sub));
+      if (x) ax = group_sum<T, L>(row_partial<T, L, TV>(A.ptr, A.col, vals, x, row, sub));
       if (sub == 0) f(row, ax);
     }
   }

@@ -292,10 +292,12 @@ def gmres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, restart
 
 
 def minres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, pc_inv=None) -> SolveInfo:
-    """Preconditioned MINRES for a symmetric (indefinite) solver matrix; rtol applies to
-    the preconditioned residual norm, SolveInfo.relres is the true relative residual."""
+    """Preconditioned MINRES for a symmetric (indefinite) solver matrix, FP64 vectors;
+    vals FP64 or FP32 (read into FP64 arithmetic).  rtol applies to the preconditioned
+    residual norm, SolveInfo.relres is the true relative residual of that operator."""
     it, rr = C.c_int(0), C.c_double(0.0)
-    call("mife_minres", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), ptr(pc_inv), float(rtol), int(maxit),
+    prec = F32 if vals.dtype == torch.float32 else F64
+    call("mife_minres", C.byref(A.desc), ptr(vals), prec, ptr(b), ptr(x), ptr(pc_inv), float(rtol), int(maxit),
          ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
     return SolveInfo(it.value, rr.value)
 

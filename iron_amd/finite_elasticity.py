@@ -120,6 +120,9 @@ class NewtonOptions:
     linear_maxit: int = 100000
     linear: str = "auto"          # "minres", "gmres" or "auto" (by formulation)
     gmres_restart: int = 100
+    # values of the Newton matrix inside MINRES: "f32" (FP32 values, FP64 vectors: the
+    # Newton residual stays FP64, so only the linear solve's operator is rounded) or "f64"
+    minres_values: str = "f32"
 
 
 @dataclass
@@ -157,6 +160,7 @@ class MRNewton:
         self.f = torch.zeros(sys.ndofs, dtype=torch.float64, device=dev)
         self.J = sys.csr.zeros()
         self.Jff = None
+        self.Jff32 = None
 
     def _eval(self, x, body_force, jac: bool):
         self.R.zero_()
@@ -219,7 +223,6 @@ class MRNewton:
             if info.iterations >= o.maxit:
                 raise CMFEError(1, "Nonlinear line search solver did not converge. PETSc diverged maximum "
                                    "iterations.")
-            self.Jff = self.S.gather(self.J, fem.F64, out=self.Jff)
             dx = torch.zeros(self.red.n_free, dtype=torch.float64, device=x.device)
             b = -F
             if ew:
@@ -227,10 +230,16 @@ class MRNewton:
                 eta_prev, fn_prev = eta, fn
             else:
                 eta = float(o.linear_rtol)
-            if linear == "minres":
-                li = fem.minres(self.S, self.Jff, b, dx, eta, o.linear_maxit, self.ws, pc_inv=self.pc_pos)
+            if linear == "minres" and o.minres_values == "f32":
+                self.Jff32 = self.S.gather(self.J, fem.F32, out=self.Jff32)
+                vals = self.Jff32
             else:
-                li = fem.gmres(self.S, self.Jff, b, dx, eta, o.linear_maxit, self.ws,
+                self.Jff = self.S.gather(self.J, fem.F64, out=self.Jff)
+                vals = self.Jff
+            if linear == "minres":
+                li = fem.minres(self.S, vals, b, dx, eta, o.linear_maxit, self.ws, pc_inv=self.pc_pos)
+            else:
+                li = fem.gmres(self.S, vals, b, dx, eta, o.linear_maxit, self.ws,
                                restart=min(o.gmres_restart, self.restart), pc_inv=self.pc_neg)
             info.linear_iterations += li.iterations
             lam = 1.0

@@ -341,6 +341,16 @@ def test_minres_and_gmres_solve_saddle_point_jacobian(fmt):
         sol = dx.cpu().numpy()
         assert info.relres < 1e-9, (solver, info)
         assert np.abs(sol - ref).max() < 1e-7 * np.abs(ref).max(), solver
+    # MINRES on the FP32 values (FP64 vectors): it solves the rounded operator, which
+    # the dense reference forms by rounding the same entries to FP32
+    vals32 = S.gather(newton.J, fem.F32)
+    ref32 = np.linalg.solve(Kff.astype(np.float32).astype(np.float64), b)
+    dx = torch.zeros(len(fr), dtype=torch.float64, device=sysg.device)
+    info = fem.minres(S, vals32, bt, dx, 1e-12, 20000, newton.ws, pc_inv=newton.pc_pos)
+    sol = dx.cpu().numpy()
+    assert info.relres < 1e-9, info
+    assert np.abs(sol - ref32).max() < 1e-7 * np.abs(ref32).max()
+    assert np.abs(sol - ref).max() > 0.0
 
 
 def test_newton_eisenstat_walker_matches_fixed_tolerance():
