@@ -134,6 +134,14 @@ int mife_reduce_pattern(int n, const int32_t* row_ptr, const int32_t* col,
                         int32_t* s_row_ptr, int32_t* s_col, int32_t* src_pos,
                         int64_t* nnz_out, void* work, void* stream);
 
+/* Row permutation of a SELL-64 matrix: positions are cut into windows of
+ * sigma (1..1024) rows; inside a window the rows go by decreasing length, ties
+ * by position.  row_at[p] is the row at position p (NULL: row p); perm[k] is
+ * the row stored at slice position k.  No reference counterpart (the
+ * reference's solver matrix is a PETSc AIJ matrix). */
+int mife_sell_permutation(int n, const int32_t* row_ptr, const int32_t* row_at, int sigma, int32_t* perm,
+                          void* stream);
+
 /* SELL-64 layout of a CSR matrix: given the row permutation (perm, iperm) and
  * slice_ptr computed from the row lengths, writes sell_col (slice numbering)
  * and sell_src (position of each stored entry in the CSR values, -1 for

@@ -99,6 +99,7 @@ _SIG = {
     "mife_assemble_scalar": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
     "mife_assemble_elastic": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P]),
     "mife_spmv": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _P]),
+    "mife_sell_permutation": (_I, [_I, _P, _P, _I, _P, _P]),
     "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "mife_sell_compress": (_I, [_I, _P, _P, _P, _P, C.POINTER(_I), _P]),
     "mife_sell_order_len": (_I, [_I]),

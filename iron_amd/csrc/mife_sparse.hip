@@ -157,6 +157,54 @@ int mife_spmv(const mife_matrix* A, const void* vals, const void* x, void* y, in
   return MIFE_OK;
 }
 
+}  // extern "C"
+
+namespace mife {
+
+// Row order of a SELL-64 matrix, one window of `sigma` positions per block:
+// the window's rows by decreasing length, ties by position (the order a stable
+// sort of the window would give).  Each thread counts the rows its own row
+// follows, reading the window's lengths from LDS (every lane reads the same
+// entry at a time).
+__global__ __launch_bounds__(1024) void sell_perm_kernel(int n, const int32_t* __restrict__ row_ptr,
+                                                         const int32_t* __restrict__ row_at, int sigma,
+                                                         int32_t* __restrict__ perm) {
+  __shared__ int len_s[1024];
+  const long long w0 = (long long)blockIdx.x * sigma;
+  const int t = threadIdx.x;
+  const int cnt = (int)min((long long)sigma, (long long)n - w0);
+  int r = -1, len = -1;
+  if (t < cnt) {
+    r = row_at ? row_at[w0 + t] : (int)(w0 + t);
+    len = row_ptr[r + 1] - row_ptr[r];
+  }
+  if (t < sigma) len_s[t] = len;
+  __syncthreads();
+  if (t < cnt) {
+    int rank = 0;
+    for (int u = 0; u < cnt; ++u) {
+      const int lu = len_s[u];
+      rank += (lu > len || (lu == len && u < t)) ? 1 : 0;
+    }
+    perm[w0 + rank] = r;
+  }
+}
+
+}  // namespace mife
+
+extern "C" {
+
+int mife_sell_permutation(int n, const int32_t* row_ptr, const int32_t* row_at, int sigma, int32_t* perm,
+                          void* stream) {
+  MIFE_REQUIRE(n >= 0 && sigma >= 1 && sigma <= 1024, MIFE_ERR_ARG, "n %d / window %d (1..1024)", n, sigma);
+  if (n == 0) return MIFE_OK;
+  MIFE_REQUIRE(row_ptr && perm, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  const int threads = 64 * div_up(sigma, 64);
+  sell_perm_kernel<<<div_up(n, sigma), threads, 0, (hipStream_t)stream>>>(n, row_ptr, row_at, sigma, perm);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
 int mife_sell_build(int n, const int32_t* row_ptr, const int32_t* col, const int32_t* src_pos, const int32_t* perm,
                     const int32_t* iperm, int n_slices, const int32_t* slice_ptr, int32_t* sell_col,
                     int32_t* sell_src, void* stream) {

@@ -187,19 +187,23 @@ class SolverMatrix:
             self.desc = MatrixDesc(0, _lib.MIFE_FMT_SELL, ptr(z), ptr(z), ptr(z), 0, 0, None, None, 0)
             return
         # slice order on the device: rows by window of sigma positions, then by
-        # decreasing length, then by position (one sort of a composite key)
+        # decreasing length, then by position (mife_sell_permutation)
         lengths = (red.row_ptr[1:] - red.row_ptr[:-1]).to(torch.int64)
-        pos = torch.arange(n, dtype=torch.int64, device=dev)
+        row_at = None
         if interleave is not None:
+            # position of a row: rank of its (node, component) key among the free
+            # DOFs' keys (keys are distinct), by a prefix sum over all keys
             ncomp, nper = int(interleave[0]), int(interleave[1])
             dof = red.free_idx.to(torch.int64)
-            order = torch.argsort((dof % nper) * ncomp + dof // nper, stable=True)
-            pos = torch.empty_like(pos)
-            pos[order] = torch.arange(n, dtype=torch.int64, device=dev)
-        L = int(lengths.max().item()) + 1 if n > 0 else 1
-        key = (pos // sigma) * (L * sigma) + (L - 1 - lengths) * sigma + pos % sigma
-        perm = torch.argsort(key)
-        self.perm = perm.to(torch.int32)
+            key = (dof % nper) * ncomp + dof // nper
+            mark = torch.zeros(ncomp * nper, dtype=torch.int32, device=dev)
+            mark[key] = 1
+            pos = (torch.cumsum(mark, 0, dtype=torch.int32) - 1)[key]
+            row_at = torch.empty(n, dtype=torch.int32, device=dev)
+            row_at[pos.long()] = torch.arange(n, dtype=torch.int32, device=dev)
+        self.perm = torch.empty(n, dtype=torch.int32, device=dev)
+        call("mife_sell_permutation", n, ptr(red.row_ptr), ptr(row_at), int(sigma), ptr(self.perm), _stream())
+        perm = self.perm.long()
         self.iperm = torch.empty_like(self.perm)
         self.iperm[perm] = torch.arange(n, dtype=torch.int32, device=dev)
         n_slices = (n + 63) // 64

@@ -24,12 +24,22 @@ from .basis import LagrangeBasis
 def _lattice(dims):
     """[prod(dims), len(dims)] multi-indices, first index fastest (Fortran order)."""
     dims = [int(d) for d in dims]
-    cols, inner = [], 1
     total = int(np.prod(dims))
+    idx = np.arange(total, dtype=np.int64)
+    cols, inner = [], 1
     for d in dims:
-        cols.append(np.tile(np.repeat(np.arange(d, dtype=np.int64), inner), total // (inner * d)))
+        cols.append((idx // inner) % d)
         inner *= d
     return np.stack(cols, axis=1)
+
+
+def _outer_sum(terms):
+    """Sum over a Fortran-ordered grid: terms[k] are 1-D arrays along direction k
+    (first fastest); returns the flattened [prod(len)] array."""
+    out = terms[-1]
+    for t in terms[-2::-1]:
+        out = np.add.outer(out, t)
+    return out.reshape(-1)
 
 
 class RegularMesh:
@@ -49,21 +59,21 @@ class RegularMesh:
         b0 = self.bases[0]
         p0 = b0.nodes_xi - 1
         tot0 = [p0 * n + 1 for n in ne]
-        eidx = _lattice(ne)  # [E, d]
         strides0 = np.cumprod([1] + tot0[:-1])
-        self.user_nodes = []     # [b] -> [E, nn] user numbers (1-based)
+        self.user_nodes = []     # [b] -> [E, nn] user numbers (1-based; the first basis' only with a second basis)
         self.comp_user = []      # [b] -> sorted unique user numbers of the mesh component
         self.elem_nodes = []     # [b] -> [E, nn] local node numbers (0-based)
         for bi, b in enumerate(self.bases):
             if bi == 0:
-                base = (eidx * (p0 * strides0)[None, :]).sum(axis=1)          # [E] first node - 1
-                offs = (b.node_index * strides0[None, :]).sum(axis=1)          # [nn]
-                user = 1 + base[:, None] + offs[None, :]
+                # first node - 1 of every element: sum over directions of i_k p0 stride_k
+                base = _outer_sum([np.arange(ne[k], dtype=np.int32) * np.int32(p0 * strides0[k]) for k in range(d)])
+                offs = (b.node_index * strides0[None, :]).sum(axis=1).astype(np.int32)  # [nn]
                 # every lattice node of the first basis belongs to an element, so its
                 # sorted unique user numbers are 1..prod(tot0) and local = user - 1
-                self.user_nodes.append(user)
+                local = base[:, None] + offs[None, :]
+                self.user_nodes.append(local.astype(np.int64) + 1 if len(self.bases) > 1 else None)
                 self.comp_user.append(np.arange(1, int(np.prod(tot0)) + 1, dtype=np.int64))
-                self.elem_nodes.append((user - 1).astype(np.int32))
+                self.elem_nodes.append(local)
                 continue
             elif b.nodes_xi == b0.nodes_xi:
                 user = self.user_nodes[0].copy()
@@ -82,7 +92,9 @@ class RegularMesh:
         pos = _lattice(tot0)  # [N, d]: component-1 nodes are the whole lattice, in order
         coords = np.empty((len(self.comp_user[0]), d))
         for c in range(d):
-            coords[:, c] = self.origin[c] + pos[:, c].astype(np.float64) * delta[c]
+            # origin + i * delta, evaluated once per lattice line and then broadcast
+            line = self.origin[c] + np.arange(tot0[c], dtype=np.int64).astype(np.float64) * delta[c]
+            coords[:, c] = line[pos[:, c]]
         self.coords = coords
         self.node_grid_pos = pos
 
@@ -123,8 +135,14 @@ class DofMap:
 
     def element_dofs(self) -> np.ndarray:
         """[E, n_edof] component-major, then element node (int32)."""
-        cols = [self.offsets[c] + self.mesh.elem_nodes[mc] for c, mc in enumerate(self.comp_mesh)]
-        return np.ascontiguousarray(np.concatenate(cols, axis=1).astype(np.int32))
+        E = self.mesh.num_elements
+        widths = [self.mesh.elem_nodes[mc].shape[1] for mc in self.comp_mesh]
+        out = np.empty((E, sum(widths)), dtype=np.int32)
+        k = 0
+        for c, mc in enumerate(self.comp_mesh):
+            np.add(self.mesh.elem_nodes[mc], np.int32(self.offsets[c]), out=out[:, k:k + widths[c]])
+            k += widths[c]
+        return out
 
     def component_of(self, dof: np.ndarray) -> np.ndarray:
         return np.searchsorted(np.array(self.offsets[1:] + [self.num_dofs]), dof, side="right")
@@ -133,8 +151,4 @@ class DofMap:
 def structured_colours(mesh: RegularMesh):
     """Element colours of a regular grid: parity of the element index in each
     direction (2^dim colours; no two elements of a colour share a node)."""
-    eidx = np.array(np.unravel_index(np.arange(mesh.num_elements), mesh.n_elem, order="F")).T
-    c = np.zeros(mesh.num_elements, dtype=np.int32)
-    for k in range(mesh.dim):
-        c += (eidx[:, k] % 2).astype(np.int32) << k
-    return c
+    return _outer_sum([(np.arange(n, dtype=np.int32) % 2) << k for k, n in enumerate(mesh.n_elem)]).astype(np.int32)

@@ -308,3 +308,27 @@ def test_golden_analytic_laplace_through_gpu():
     vals = v64.cpu().numpy()[: len(col)]
     Rt = fem.backsubstitute(csr, v64, u, iron_compat=False).cpu().numpy()
     assert np.allclose(Rt, fo.backsubstitute(rp, col, vals, un), atol=1e-13)
+
+
+@pytest.mark.parametrize("sigma", [1, 64, 100, 1024])
+def test_sell_permutation_is_the_window_sort(sigma):
+    """mife_sell_permutation against a sort of the composite key (window, decreasing
+    length, position), with rows in their own order and at permuted positions."""
+    from iron_amd import _lib
+    print(f"case sell permutation window {sigma}", flush=True)
+    rng = np.random.default_rng(sigma)
+    n = 5000 + sigma // 3
+    lengths = rng.choice([27, 45, 75, 125, 3, 0], size=n)
+    row_ptr = torch.as_tensor(np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32), device="cuda")
+    for permuted in (False, True):
+        row_at = rng.permutation(n) if permuted else np.arange(n)
+        pos = np.empty(n, dtype=np.int64)
+        pos[row_at] = np.arange(n)
+        L = lengths.max() + 1
+        key = (pos // sigma) * (L * sigma) + (L - 1 - lengths) * sigma + pos % sigma
+        want = np.argsort(key, kind="stable")
+        perm = torch.empty(n, dtype=torch.int32, device="cuda")
+        ra = torch.as_tensor(row_at.astype(np.int32), device="cuda") if permuted else None
+        _lib.call("mife_sell_permutation", n, row_ptr.data_ptr(), None if ra is None else ra.data_ptr(), sigma,
+                  perm.data_ptr(), _lib.stream_ptr())
+        assert np.array_equal(perm.cpu().numpy(), want), permuted
