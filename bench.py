@@ -506,6 +506,8 @@ def serve_leg(n, seed=2024):
     wall_seq = time.perf_counter() - t
     service = np.array(service) * 1e3
     elems = sum(int(np.prod(r.n_elem)) for r in reqs)
+    stats_sat = dict(srv.stats)
+    del srv   # its pools, before the next server takes its own
     # the same stream through FP64 grouped iterations
     srv64 = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=True, precision="fp64")
     srv64.run(serving.make_stream(8, seed=seed + 1, lo=8, hi=16))
@@ -516,10 +518,11 @@ def serve_leg(n, seed=2024):
     it64 = sum(r.iterations for r in out64)
     del srv64
     # roofline of the grouped product: the same stream served once more without graph
-    # replay (so that every launch carries HIP events), algorithmic bytes counted per
-    # segment iteration by the server
+    # replay (so that every launch carries HIP events) and without the structure
+    # prefetch thread (whose kernels would share the GPU with the timed launches),
+    # algorithmic bytes counted per segment iteration by the server
     from iron_amd import _lib
-    srv_r = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=False)
+    srv_r = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=False, prefetch=0)
     srv_r.run(serving.make_stream(8, seed=seed + 1, lo=8, hi=16))
     srv_r.stats["spmv_bytes"] = 0.0
     _lib.profile_enable(True)
@@ -551,7 +554,7 @@ def serve_leg(n, seed=2024):
         "iterations_served_vs_all_fp64": [sum(r.iterations for r in out), it64],
         "replacement_ms": prof.get("grouped_replace", (0, 0.0))[1],
         "max_relres": max(r.relres for r in out + out_p),
-        "server_phase_s_at_capacity": timing_sat, "server_stats": dict(srv.stats), "roofline": roof,
+        "server_phase_s_at_capacity": timing_sat, "server_stats": stats_sat, "roofline": roof,
     }
 
 

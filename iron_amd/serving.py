@@ -32,6 +32,7 @@ from __future__ import annotations
 import ctypes as C
 import time
 from collections import OrderedDict
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -177,7 +178,7 @@ class Server:
     def __init__(self, hbm_budget: float = 32e9, max_segments: int = 256, nz_per_row: float = 60.0,
                  chunk: int = 16, use_graph: bool = True, device=None, cache_shapes: int = 8,
                  admit_budget: float = 0.004, precision: str = "auto", inner_rtol: float = 0.1,
-                 check_every: int = 4):
+                 check_every: int = 4, prefetch: int = 4):
         self.device = fem._dev(device)
         self.max_segments = int(max_segments)
         if precision not in self.MIXED_KINDS:
@@ -231,6 +232,12 @@ class Server:
         self.stats = dict(admitted=0, retired=0, chunks=0, graph_captures=0, graph_replays=0, table_builds=0,
                           iterations=0, spmv_bytes=0.0, assembly_batches=0)
         self.q1 = fem.DeviceBasis(basis(3, 1, 2), dev)   # trilinear tables of the batched assembly
+        # structures of the next `prefetch` queued requests are built by one worker
+        # thread on its own stream while the main thread waits for the GPU
+        self.prefetch = int(prefetch)
+        self._pool = ThreadPoolExecutor(max_workers=1) if self.prefetch > 0 else None
+        self._pf_stream = torch.cuda.Stream() if self.prefetch > 0 else None
+        self._pending = {}
         self.batch_dev = None
         self.batch_host = None
         # host-side wall time per phase (seconds): structure builds, admission
@@ -238,12 +245,30 @@ class Server:
         self.timing = dict(structure=0.0, admit=0.0, chunk=0.0, status=0.0, retire=0.0)
 
     # -- structures
+    def _build_structure(self, key):
+        with torch.cuda.stream(self._pf_stream):
+            st = _Structure(key[0], key[1], self.device)
+        self._pf_stream.synchronize()   # complete before another stream reads it
+        return st
+
+    def _prefetch(self, queue, qi, now):
+        """Queues structure builds for the next waiting requests that have arrived."""
+        if self._pool is None:
+            return
+        for r in queue[qi:qi + self.prefetch]:
+            if r.arrival > now:
+                break
+            key = (r.kind, tuple(r.n_elem))
+            if key not in self.cache and key not in self._pending:
+                self._pending[key] = self._pool.submit(self._build_structure, key)
+
     def _structure(self, req: Request) -> _Structure:
         key = (req.kind, tuple(req.n_elem))
         st = self.cache.get(key)
         if st is None:
             t = time.perf_counter()
-            st = _Structure(req.kind, req.n_elem, self.device)
+            fut = self._pending.pop(key, None)
+            st = fut.result() if fut is not None else _Structure(req.kind, req.n_elem, self.device)
             self.timing["structure"] += time.perf_counter() - t
             self.cache[key] = st
             while len(self.cache) > self.cache_shapes:
@@ -405,6 +430,7 @@ class Server:
         main = torch.cuda.current_stream()
         side = self.side if overlap else main
         while qi < len(queue) or active or pending:
+            self._prefetch(queue, qi, time.perf_counter() - t0)
             if pending:
                 main.wait_stream(side)
                 active += pending
@@ -493,10 +519,17 @@ class Server:
                 active = [e for e in active if id(e) not in retired]
                 changed = True
             self.timing["retire"] += time.perf_counter() - tr
+        for fut in self._pending.values():
+            fut.result()
+        self._pending.clear()
         return queue
 
     def _run_chunk(self, n_seg, nb):
-        if not self.use_graph:
+        # a capture needs the structure worker idle (its allocations may synchronise
+        # the device, which would invalidate the capture); only this thread submits
+        # builds, so none starts during the capture
+        worker_busy = any(not f.done() for f in self._pending.values())
+        if not self.use_graph or (self.graph is None and worker_busy):
             self._iterate(n_seg, nb, self.chunk)
             return
         if self.graph is None:
