@@ -404,7 +404,9 @@ int mife_minres(const mife_matrix* A, const void* vals, int prec, const double* 
  * cb_off; the system's vectors are then in slice numbering.  Vectors live at
  * row_off of the vector pools.  slot (0 <= slot < max_segments) names the
  * segment's persistent CG state.  prec: MIFE_F64 (FP64 CG) or MIFE_F32 (FP32 CG
- * with FP64 reliable updates). */
+ * with FP64 reliable updates).  The library sees the pools as bare pointers: the
+ * caller keeps every segment's ranges (rows, pointers, entries, base columns)
+ * inside its allocations. */
 typedef struct mife_segment {
   int32_t n;
   int32_t maxit;
