@@ -329,6 +329,17 @@ def mr_leg(runs, cpu_seconds):
                      "flops_per_element": mr_flops_per_element(27, 8, 27)},
         "kernel_ms": {k: v for k, v in prof.items() if v[0]},
     }
+    # the same load path with GMRES steps (restart 100, pressure-block preconditioner,
+    # FP64 values), each linear solve to 1e-8
+    opts_g = fe.NewtonOptions(atol=1e-12, rtol=1e-8, stol=0.0, linear_rtol=1e-8, linear="gmres")
+    x = torch.as_tensor(x0, device=sysg.device)
+    t = time.perf_counter()
+    infos_g = fe.load_increments(newton, x, 10, fixed, target, opts=opts_g)
+    torch.cuda.synchronize()
+    out["gmres_1e8"] = {"solve_s": time.perf_counter() - t, "newton_iterations": sum(i.iterations for i in infos_g),
+                        "gmres_iterations": sum(i.linear_iterations for i in infos_g),
+                        "max_node_error_vs_closed_form":
+                            float(np.abs(x.cpu().numpy()[:3 * nu].reshape(3, -1).T - xe).max())}
     if cpu_seconds > 0:
         sys.path.insert(0, str(ROOT))
         from oracle import mr_oracle as mo

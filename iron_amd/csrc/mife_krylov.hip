@@ -381,15 +381,71 @@ __global__ __launch_bounds__(256) void multidot_kernel(int n, int m, const doubl
   }
 }
 
+// h[j] = sum of the partials of V_j . w, one block per j (fixed order).
 __global__ void multidot_reduce(int m, int nblocks, const double* __restrict__ parts, double* __restrict__ h) {
   __shared__ double red[4];
-  for (int j = 0; j < m; ++j) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[(size_t)j * kDotBlocks + b];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) h[j] = s;
-    __syncthreads();
+  const int j = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[(size_t)j * kDotBlocks + b];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) h[j] = s;
+}
+
+// Device state of one GMRES restart cycle.
+struct GmresCycle {
+  double hk1;  // norm of the newest basis vector before scaling (0: breakdown)
+  int done;    // converged, broken down or out of iterations: the cycle stops
+  int k_used;  // Hessenberg columns formed
+};
+
+// Column k of the Hessenberg matrix from the two Gram-Schmidt passes and the new
+// vector's norm, the previous Givens rotations applied to it, rotation k formed
+// and g rotated (one thread; H is (m+1) x m, row-major).  Sets done when
+// |g[k+1]| <= tol_abs, on breakdown, or when this is the last iteration allowed.
+__global__ void gmres_givens_kernel(int k, int m, const double* __restrict__ h1, const double* __restrict__ h2,
+                                    const double* __restrict__ nrm, double* __restrict__ H, double* __restrict__ cs,
+                                    double* __restrict__ sn, double* __restrict__ g, GmresCycle* cy, double tol_abs,
+                                    int last) {
+  if (threadIdx.x != 0 || cy->done) return;
+  for (int j = 0; j <= k; ++j) H[j * m + k] = h1[j] + h2[j];
+  const double hk1 = sqrt(fmax(nrm[0], 0.0));
+  for (int j = 0; j < k; ++j) {
+    const double a = H[j * m + k], b = H[(j + 1) * m + k];
+    H[j * m + k] = cs[j] * a + sn[j] * b;
+    H[(j + 1) * m + k] = -sn[j] * a + cs[j] * b;
   }
+  const double hkk = H[k * m + k];
+  const double den = hypot(hkk, hk1);
+  const double c = den > 0.0 ? hkk / den : 1.0, sv = den > 0.0 ? hk1 / den : 0.0;
+  cs[k] = c;
+  sn[k] = sv;
+  H[k * m + k] = den;
+  g[k + 1] = -sv * g[k];
+  g[k] = c * g[k];
+  cy->hk1 = hk1;
+  cy->k_used = k + 1;
+  if (fabs(g[k + 1]) <= tol_abs || hk1 == 0.0 || last) cy->done = 1;
+}
+
+// Start of a restart cycle: V_0 = r / beta, g = (beta, 0, ...), state cleared.
+__global__ void gmres_cycle_init_kernel(int n, int m, const double* __restrict__ r, double* __restrict__ V0,
+                                        double beta, double* __restrict__ g, GmresCycle* cy) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int i = tid; i < n; i += gridDim.x * blockDim.x) V0[i] = r[i] / beta;
+  for (int j = tid; j <= m; j += gridDim.x * blockDim.x) g[j] = j == 0 ? beta : 0.0;
+  if (tid == 0) {
+    cy->hk1 = 0.0;
+    cy->done = 0;
+    cy->k_used = 0;
+  }
+}
+
+// V_{k+1} = w / hk1 while the cycle runs (nothing once it has stopped).
+__global__ void gmres_next_basis_kernel(int n, const double* __restrict__ w, double* __restrict__ v,
+                                        const GmresCycle* cy) {
+  if (cy->done) return;
+  const double a = cy->hk1 > 0.0 ? 1.0 / cy->hk1 : 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = a * w[i];
 }
 
 // w -= sum_j V_j h_j
@@ -809,6 +865,7 @@ size_t mife_solver_work_bytes(int n, int restart) {
   bytes += (4 * nn + 256) * 8;                   // FP32 vectors
   bytes += (8 * (size_t)kDotBlocks * (size_t)std::max(m, (size_t)2) + 256);  // GMRES multidot partials
   bytes += 8 * (3 * m + 8) + 768;                // GMRES coefficient vectors
+  bytes += 8 * (m + 1) * m + 8 * 3 * (m + 2) + 1024;  // GMRES Hessenberg matrix, rotations, cycle state
   bytes += 8 * 64 * 4 + 65536;
   return bytes;
 }
@@ -1039,6 +1096,11 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   double* hdev = w.take<double>(m + 2);
   double* ydev = w.take<double>(m + 2);
   double* ndev = w.take<double>(2);
+  double* Hdev = w.take<double>((size_t)(m + 1) * m);
+  double* csdev = w.take<double>(m + 2);
+  double* sndev = w.take<double>(m + 2);
+  double* gdev = w.take<double>(m + 2);
+  GmresCycle* cy = w.take<GmresCycle>(1);
   const int vg = vec_grid(n);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
@@ -1052,57 +1114,52 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   const double bn = sqrt(hs.bb);
   double beta = sqrt(hs.rr64);
   int it = 0;
-  std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), hcol(m + 2), hh(m + 1), h2(m + 1);
-  double hnorm = 0.0;
+  // Each restart cycle runs on the device in chunks of kChunk iterations: the
+  // Givens rotations and the stopping test are device kernels, and the host
+  // reads the cycle state once per chunk.  Iterations of a chunk after the stop
+  // compute unused columns (their rotation kernel returns at once).
+  constexpr int kChunk = 8;
+  std::vector<double> H((size_t)(m + 1) * m), g(m + 2);
   while (bn > 0 && beta / bn > rtol && it < maxit) {
-    scale_copy_kernel<<<vg, 256, 0, s>>>(n, r, V, 1.0 / beta);
-    std::fill(H.begin(), H.end(), 0.0);
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
-    int k_used = 0;
-    for (int k = 0; k < m; ++k) {
-      mul_kernel<<<vg, 256, 0, s>>>(n, dinv, V + k * ldv, t);
-      launch_spmv<double>(M.a, vals, t, wv, M.L, s);
-      // classical Gram-Schmidt, two passes
-      const dim3 mg(vg, div_up(k + 1, 32));
-      for (int pass = 0; pass < 2; ++pass) {
-        multidot_kernel<<<mg, 256, 0, s>>>(n, k + 1, V, ldv, wv, parts);
-        multidot_reduce<<<1, 256, 0, s>>>(k + 1, vg, parts, pass == 0 ? hdev : ydev);
-        orth_update_kernel<<<vg, 256, 0, s>>>(n, k + 1, V, ldv, pass == 0 ? hdev : ydev, wv);
+    gmres_cycle_init_kernel<<<vg, 256, 0, s>>>(n, m, r, V, beta, gdev, cy);
+    GmresCycle hc{};
+    int k = 0;
+    while (k < m) {
+      const int cnt = std::min(kChunk, std::min(m - k, maxit - it));
+      for (int c = 0; c < cnt; ++c) {
+        const int kk = k + c;
+        mul_kernel<<<vg, 256, 0, s>>>(n, dinv, V + kk * ldv, t);
+        launch_spmv<double>(M.a, vals, t, wv, M.L, s);
+        // classical Gram-Schmidt, two passes
+        const dim3 mg(vg, div_up(kk + 1, 32));
+        for (int pass = 0; pass < 2; ++pass) {
+          multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts);
+          multidot_reduce<<<kk + 1, 256, 0, s>>>(kk + 1, vg, parts, pass == 0 ? hdev : ydev);
+          orth_update_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, pass == 0 ? hdev : ydev, wv);
+        }
+        multidot_kernel<<<dim3(vg, 1), 256, 0, s>>>(n, 1, wv, ldv, wv, parts);
+        multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, ndev);
+        gmres_givens_kernel<<<1, 64, 0, s>>>(kk, m, hdev, ydev, ndev, Hdev, csdev, sndev, gdev, cy, rtol * bn,
+                                             (it + c + 1 >= maxit || kk + 1 == m) ? 1 : 0);
+        gmres_next_basis_kernel<<<vg, 256, 0, s>>>(n, wv, V + (kk + 1) * ldv, cy);
       }
-      multidot_kernel<<<dim3(vg, 1), 256, 0, s>>>(n, 1, wv, ldv, wv, parts);
-      multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, ndev);
       MIFE_CHECK_HIP(hipGetLastError());
-      MIFE_CHECK_HIP(hipMemcpyAsync(hh.data(), hdev, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-      MIFE_CHECK_HIP(hipMemcpyAsync(h2.data(), ydev, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
-      MIFE_CHECK_HIP(hipMemcpyAsync(&hnorm, ndev, sizeof(double), hipMemcpyDeviceToHost, s));
+      MIFE_CHECK_HIP(hipMemcpyAsync(&hc, cy, sizeof(GmresCycle), hipMemcpyDeviceToHost, s));
       MIFE_CHECK_HIP(hipStreamSynchronize(s));
-      for (int j = 0; j <= k; ++j) hcol[j] = hh[j] + h2[j];
-      const double hk1 = sqrt(std::max(hnorm, 0.0));
-      hcol[k + 1] = hk1;
-      if (hk1 > 0) scale_copy_kernel<<<vg, 256, 0, s>>>(n, wv, V + (k + 1) * ldv, 1.0 / hk1);
-      for (int j = 0; j < k; ++j) {
-        const double tmp = cs[j] * hcol[j] + sn[j] * hcol[j + 1];
-        hcol[j + 1] = -sn[j] * hcol[j] + cs[j] * hcol[j + 1];
-        hcol[j] = tmp;
-      }
-      const double den = std::hypot(hcol[k], hcol[k + 1]);
-      cs[k] = hcol[k] / den;
-      sn[k] = hcol[k + 1] / den;
-      hcol[k] = den;
-      hcol[k + 1] = 0.0;
-      g[k + 1] = -sn[k] * g[k];
-      g[k] = cs[k] * g[k];
-      for (int j = 0; j <= k; ++j) H[j * m + k] = hcol[j];
-      ++it;
-      k_used = k + 1;
-      if (std::fabs(g[k + 1]) / bn <= rtol || it >= maxit || hk1 == 0.0) break;
+      it += hc.k_used - k;
+      k = hc.k_used;
+      if (hc.done || cnt == 0) break;
     }
+    const int k_used = hc.k_used;
+    if (k_used == 0) break;
+    MIFE_CHECK_HIP(hipMemcpyAsync(H.data(), Hdev, sizeof(double) * (size_t)k_used * m, hipMemcpyDeviceToHost, s));
+    MIFE_CHECK_HIP(hipMemcpyAsync(g.data(), gdev, sizeof(double) * (k_used + 1), hipMemcpyDeviceToHost, s));
+    MIFE_CHECK_HIP(hipStreamSynchronize(s));
     std::vector<double> y(k_used);
     for (int i = k_used - 1; i >= 0; --i) {
       double sum = g[i];
-      for (int j = i + 1; j < k_used; ++j) sum -= H[i * m + j] * y[j];
-      y[i] = sum / H[i * m + i];
+      for (int j = i + 1; j < k_used; ++j) sum -= H[(size_t)i * m + j] * y[j];
+      y[i] = sum / H[(size_t)i * m + i];
     }
     MIFE_CHECK_HIP(hipMemcpyAsync(ydev, y.data(), sizeof(double) * k_used, hipMemcpyHostToDevice, s));
     gmres_update_kernel<<<vg, 256, 0, s>>>(n, k_used, V, ldv, ydev, dinv, x);
