@@ -400,38 +400,60 @@ struct GmresCycle {
 
 // Column k of the Hessenberg matrix from the two Gram-Schmidt passes and the new
 // vector's norm, the previous Givens rotations applied to it, rotation k formed
-// and g rotated (one thread; H is (m+1) x m, row-major).  Sets done when
-// |g[k+1]| <= tol_abs, on breakdown, or when this is the last iteration allowed.
-__global__ void gmres_givens_kernel(int k, int m, const double* __restrict__ h1, const double* __restrict__ h2,
-                                    const double* __restrict__ nrm, double* __restrict__ H, double* __restrict__ cs,
-                                    double* __restrict__ sn, double* __restrict__ g, GmresCycle* cy, double tol_abs,
-                                    int last) {
-  if (threadIdx.x != 0 || cy->done) return;
-  for (int j = 0; j <= k; ++j) H[j * m + k] = h1[j] + h2[j];
-  const double hk1 = sqrt(fmax(nrm[0], 0.0));
-  for (int j = 0; j < k; ++j) {
-    const double a = H[j * m + k], b = H[(j + 1) * m + k];
-    H[j * m + k] = cs[j] * a + sn[j] * b;
-    H[(j + 1) * m + k] = -sn[j] * a + cs[j] * b;
+// and g rotated; H is (m+1) x m, row-major.  The column and the rotations are
+// staged in LDS by the block, thread 0 runs the sequential rotations, and the
+// block writes the column back.  Sets done when |g[k+1]| <= tol_abs, on
+// breakdown, or when this is the last iteration allowed.
+__global__ __launch_bounds__(256) void gmres_givens_kernel(int k, int m, const double* __restrict__ h1,
+                                                           const double* __restrict__ h2,
+                                                           const double* __restrict__ nrm, double* __restrict__ H,
+                                                           double* __restrict__ cs, double* __restrict__ sn,
+                                                           double* __restrict__ g, GmresCycle* cy, double tol_abs,
+                                                           int last) {
+  __shared__ double col[kMaxRestart + 1];
+  __shared__ double c_s[kMaxRestart], s_s[kMaxRestart];
+  if (cy->done) return;
+  for (int j = threadIdx.x; j <= k; j += blockDim.x) col[j] = h1[j] + h2[j];
+  for (int j = threadIdx.x; j < k; j += blockDim.x) {
+    c_s[j] = cs[j];
+    s_s[j] = sn[j];
   }
-  const double hkk = H[k * m + k];
-  const double den = hypot(hkk, hk1);
-  const double c = den > 0.0 ? hkk / den : 1.0, sv = den > 0.0 ? hk1 / den : 0.0;
-  cs[k] = c;
-  sn[k] = sv;
-  H[k * m + k] = den;
-  g[k + 1] = -sv * g[k];
-  g[k] = c * g[k];
-  cy->hk1 = hk1;
-  cy->k_used = k + 1;
-  if (fabs(g[k + 1]) <= tol_abs || hk1 == 0.0 || last) cy->done = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double hk1 = sqrt(fmax(nrm[0], 0.0));
+    for (int j = 0; j < k; ++j) {
+      const double a = col[j], b = col[j + 1];
+      col[j] = c_s[j] * a + s_s[j] * b;
+      col[j + 1] = -s_s[j] * a + c_s[j] * b;
+    }
+    const double hkk = col[k];
+    const double den = hypot(hkk, hk1);
+    const double c = den > 0.0 ? hkk / den : 1.0, sv = den > 0.0 ? hk1 / den : 0.0;
+    cs[k] = c;
+    sn[k] = sv;
+    col[k] = den;
+    const double gk = g[k];
+    g[k + 1] = -sv * gk;
+    g[k] = c * gk;
+    cy->hk1 = hk1;
+    cy->k_used = k + 1;
+    if (fabs(sv * gk) <= tol_abs || hk1 == 0.0 || last) cy->done = 1;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j <= k; j += blockDim.x) H[j * m + k] = col[j];
 }
 
-// Start of a restart cycle: V_0 = r / beta, g = (beta, 0, ...), state cleared.
+// Start of a restart cycle: V_0 = r / beta and t = D^-1 V_0 (the first product's
+// input), g = (beta, 0, ...), state cleared.
 __global__ void gmres_cycle_init_kernel(int n, int m, const double* __restrict__ r, double* __restrict__ V0,
-                                        double beta, double* __restrict__ g, GmresCycle* cy) {
+                                        const double* __restrict__ dinv, double* __restrict__ t, double beta,
+                                        double* __restrict__ g, GmresCycle* cy) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int i = tid; i < n; i += gridDim.x * blockDim.x) V0[i] = r[i] / beta;
+  for (int i = tid; i < n; i += gridDim.x * blockDim.x) {
+    const double v = r[i] / beta;
+    V0[i] = v;
+    t[i] = dinv[i] * v;
+  }
   for (int j = tid; j <= m; j += gridDim.x * blockDim.x) g[j] = j == 0 ? beta : 0.0;
   if (tid == 0) {
     cy->hk1 = 0.0;
@@ -440,12 +462,35 @@ __global__ void gmres_cycle_init_kernel(int n, int m, const double* __restrict__
   }
 }
 
-// V_{k+1} = w / hk1 while the cycle runs (nothing once it has stopped).
+// V_{k+1} = w / hk1 and t = D^-1 V_{k+1} (the next product's input) while the
+// cycle runs (nothing once it has stopped).
 __global__ void gmres_next_basis_kernel(int n, const double* __restrict__ w, double* __restrict__ v,
+                                        const double* __restrict__ dinv, double* __restrict__ t,
                                         const GmresCycle* cy) {
   if (cy->done) return;
   const double a = cy->hk1 > 0.0 ? 1.0 / cy->hk1 : 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = a * w[i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double vi = a * w[i];
+    v[i] = vi;
+    t[i] = dinv[i] * vi;
+  }
+}
+
+// w -= sum_j V_j h_j, with the per-block partials of w.w (the second
+// Gram-Schmidt pass, which also yields the new vector's norm).
+__global__ __launch_bounds__(256) void orth_update_norm_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
+                                                               const double* __restrict__ h, double* __restrict__ w,
+                                                               double* __restrict__ parts) {
+  __shared__ double red[4];
+  double q = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = w[i];
+    for (int j = 0; j < m; ++j) s -= V[j * ldv + i] * h[j];
+    w[i] = s;
+    q += s * s;
+  }
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) parts[blockIdx.x] = q;
 }
 
 // w -= sum_j V_j h_j
@@ -460,10 +505,6 @@ __global__ void orth_update_kernel(int n, int m, const double* __restrict__ V, s
 
 __global__ void scale_copy_kernel(int n, const double* __restrict__ src, double* __restrict__ dst, double a) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = a * src[i];
-}
-
-__global__ void mul_kernel(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = a[i] * b[i];
 }
 
 // x += dinv * (V y)
@@ -1121,27 +1162,26 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   constexpr int kChunk = 8;
   std::vector<double> H((size_t)(m + 1) * m), g(m + 2);
   while (bn > 0 && beta / bn > rtol && it < maxit) {
-    gmres_cycle_init_kernel<<<vg, 256, 0, s>>>(n, m, r, V, beta, gdev, cy);
+    gmres_cycle_init_kernel<<<vg, 256, 0, s>>>(n, m, r, V, dinv, t, beta, gdev, cy);
     GmresCycle hc{};
     int k = 0;
     while (k < m) {
       const int cnt = std::min(kChunk, std::min(m - k, maxit - it));
       for (int c = 0; c < cnt; ++c) {
         const int kk = k + c;
-        mul_kernel<<<vg, 256, 0, s>>>(n, dinv, V + kk * ldv, t);
-        launch_spmv<double>(M.a, vals, t, wv, M.L, s);
-        // classical Gram-Schmidt, two passes
+        launch_spmv<double>(M.a, vals, t, wv, M.L, s);   // t = D^-1 V_kk (written with V_kk)
+        // classical Gram-Schmidt, two passes; the second also reduces ||w||^2
         const dim3 mg(vg, div_up(kk + 1, 32));
-        for (int pass = 0; pass < 2; ++pass) {
-          multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts);
-          multidot_reduce<<<kk + 1, 256, 0, s>>>(kk + 1, vg, parts, pass == 0 ? hdev : ydev);
-          orth_update_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, pass == 0 ? hdev : ydev, wv);
-        }
-        multidot_kernel<<<dim3(vg, 1), 256, 0, s>>>(n, 1, wv, ldv, wv, parts);
+        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts);
+        multidot_reduce<<<kk + 1, 256, 0, s>>>(kk + 1, vg, parts, hdev);
+        orth_update_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, hdev, wv);
+        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts);
+        multidot_reduce<<<kk + 1, 256, 0, s>>>(kk + 1, vg, parts, ydev);
+        orth_update_norm_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, ydev, wv, parts);
         multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, ndev);
-        gmres_givens_kernel<<<1, 64, 0, s>>>(kk, m, hdev, ydev, ndev, Hdev, csdev, sndev, gdev, cy, rtol * bn,
+        gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, ydev, ndev, Hdev, csdev, sndev, gdev, cy, rtol * bn,
                                              (it + c + 1 >= maxit || kk + 1 == m) ? 1 : 0);
-        gmres_next_basis_kernel<<<vg, 256, 0, s>>>(n, wv, V + (kk + 1) * ldv, cy);
+        gmres_next_basis_kernel<<<vg, 256, 0, s>>>(n, wv, V + (kk + 1) * ldv, dinv, t, cy);
       }
       MIFE_CHECK_HIP(hipGetLastError());
       MIFE_CHECK_HIP(hipMemcpyAsync(&hc, cy, sizeof(GmresCycle), hipMemcpyDeviceToHost, s));
