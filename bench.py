@@ -11,7 +11,7 @@ every boundary face.  One step is one full pass of the path:
 The plain FP64 drop-in (FP64 assembly + FP64 Jacobi CG to 1e-8) is timed in the
 same run, and the C restatement of iron's CPU algorithm on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16] [--legs main,mr,elastic,laplace,serve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16] [--legs main,mr,elastic,laplace,slab,serve]
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -48,9 +48,9 @@ def parse():
                          "writes 268,435,456 bytes each: the reference kernel for HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=128,
                     help="requests of the served-stream leg (the full stream has 1024); 0 skips it")
-    ap.add_argument("--legs", default="main,mr,elastic,laplace,serve",
+    ap.add_argument("--legs", default="main,mr,elastic,laplace,slab,serve",
                     help="comma-separated legs to run: main (the workload above), mr, elastic (96x32x32 "
-                         "elasticity), laplace (16^3 Laplace), serve")
+                         "elasticity), laplace (16^3 Laplace), slab (one eighth of the 48^3 case), serve")
     return ap.parse_args()
 
 
@@ -58,6 +58,9 @@ TRAFFIC_FILE = ROOT / "profiles" / "cg_spmv_diffusion48_q2.md"
 WORKLOAD_TEXT = {
     "diffusion48": "diffusion48: 48^3 Q2 hex, 3x3x3 Gauss, kappa~lognormal(0,1) seed {seed}, f=1, u=0 on all "
                    "faces, FP32 inner CG + FP64 refinement to 1e-8",
+    "diffusion48slab": "diffusion48 slab: 6x48x48 Q2 hex on [0,1/8]x[0,1]^2 (one of eight x-slabs of the 48^3 "
+                       "case), 3x3x3 Gauss, kappa~lognormal(0,1) seed {seed}, f=1, u=0 on all faces, FP32 inner CG "
+                       "+ FP64 refinement to 1e-8",
     "laplace16": "laplace16: 16^3 Q1 hex, 2x2x2 Gauss, kappa=1, u=0 at x=0 and u=1 at x=1, FP32 inner CG + "
                  "FP64 refinement to 1e-10",
 }
@@ -81,6 +84,8 @@ def build_problem(args):
     from iron_amd import basis, fem, mesh
     if args.workload == "diffusion48":
         ne, interp, extent, rtol = (48, 48, 48), basis.QUADRATIC_LAGRANGE, (1.0, 1.0, 1.0), 1e-8
+    elif args.workload == "diffusion48slab":
+        ne, interp, extent, rtol = (6, 48, 48), basis.QUADRATIC_LAGRANGE, (0.125, 1.0, 1.0), 1e-8
     elif args.workload == "laplace16":
         ne, interp, extent, rtol = (16, 16, 16), basis.LINEAR_LAGRANGE, (1.0, 1.0, 1.0), 1e-10
     else:
@@ -90,11 +95,12 @@ def build_problem(args):
     dm = mesh.DofMap(m, [0])
     dev = torch.device("cuda:0")
     E = m.num_elements
-    if args.workload == "diffusion48":
+    if args.workload in ("diffusion48", "diffusion48slab"):
         g = torch.Generator().manual_seed(args.seed)
         kappa = torch.randn(E, dtype=torch.float64, generator=g).exp()
         X = m.coords
-        fixed = np.any((np.abs(X) < 1e-12) | (np.abs(X - 1.0) < 1e-12), axis=1)
+        hi = np.array(extent)[None, :]
+        fixed = np.any((np.abs(X) < 1e-12) | (np.abs(X - hi) < 1e-12), axis=1)
         ufix = np.zeros(dm.num_dofs)
         source = torch.ones(E, dtype=torch.float64)
     else:
@@ -477,6 +483,42 @@ def laplace_leg(args, runs=5):
     return out
 
 
+def slab_leg(args, runs=3):
+    """The 48^3 triquadratic diffusion case cut into eight slabs along x: one slab's
+    kernels at their real per-GPU shape (6x48x48 elements; here with its own Dirichlet
+    faces, no halo).  Reports the assembly and fused FP32 SpMV rates at that size."""
+    import argparse
+    import torch
+    from iron_amd import _lib
+    a = argparse.Namespace(**vars(args))
+    a.workload = "diffusion48slab"
+    P = build_problem(a)
+    timed_steps(P, 1, True, args.inner_rtol)
+    _lib.profile_enable(True)
+    _lib.profile_reset()
+    wall, lat, infos = timed_steps(P, runs, True, args.inner_rtol)
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    red = P["red"]
+    n_s, ms_s = prof.get("cg_spmv_f32", (0, 0.0))
+    per = ms_s / max(n_s, 1)
+    alg = 8.0 * red.nnz + 24.0 * red.n_free
+    E = P["m"].num_elements
+    a64 = prof.get("assemble_scalar", (0, 0.0))
+    a32 = prof.get("assemble_scalar_f32", (0, 0.0))
+    torch.cuda.synchronize()
+    return {
+        "workload": WORKLOAD_TEXT["diffusion48slab"].format(seed=args.seed),
+        "elements": E, "free_dofs": red.n_free, "solver_nnz": red.nnz,
+        "ms_per_pass": wall / runs * 1e3, "iterations_inner": infos[-1].iterations,
+        "assembly_elems_per_s_f64": E / (a64[1] / max(a64[0], 1) * 1e-3) if a64[0] else None,
+        "assembly_elems_per_s_f32": E / (a32[1] / max(a32[0], 1) * 1e-3) if a32[0] else None,
+        "spmv_f32": {"avg_launch_ms": per, "bytes_per_launch": alg,
+                     "achieved_GBs": alg / (per * 1e-3) / 1e9 if per > 0 else 0.0,
+                     "frac": alg / (per * 1e-3) / 1e9 / HBM_PEAK_GBS if per > 0 else 0.0},
+    }
+
+
 def serve_leg(n, seed=2024):
     """Served stream: n requests, 70% diffusion / 30% elasticity trilinear boxes with
     N_x, N_y, N_z ~ U{8..64}, lognormal(0,1) coefficients (per-request seeds), BCs of
@@ -596,6 +638,8 @@ def main():
         out["elasticity96"] = elastic_leg()
     if "laplace" in legs:
         out["laplace16"] = laplace_leg(args)
+    if "slab" in legs:
+        out["diffusion48_slab"] = slab_leg(args)
     if "serve" in legs and args.serve_requests > 0:
         out["served_stream"] = serve_leg(args.serve_requests)
     print(json.dumps(out))
