@@ -95,6 +95,19 @@ class MRSystem:
              None if atomic else cs.colour_ptr.ctypes.data, cs.n_colours, ptr(residual), ptr(jac), ptr(rhs),
              fem._stream())
 
+    def assemble_colour(self, c: int, state, material, body_force=None, residual=None, rhs=None):
+        """As assemble, residual and gravity vector only, over the elements of colour c
+        (no two of which share a DOF)."""
+        cs = self.csr
+        b0, b1 = int(cs.colour_ptr[c]), int(cs.colour_ptr[c + 1])
+        cp = np.array([0, b1 - b0], dtype=np.int32)
+        bf = None
+        if body_force is not None:
+            bf = (C.c_double * 3)(*[float(v) for v in body_force])
+        call("mife_mr_assemble", C.byref(self.desc), self.rm.num_elements, ptr(self.coords), ptr(self.elem_nodes),
+             ptr(cs.elem_dofs), ptr(state), ptr(material), C.cast(bf, C.c_void_p) if bf is not None else None,
+             None, ptr(cs.order) + 4 * b0, cp.ctypes.data, 1, ptr(residual), None, ptr(rhs), fem._stream())
+
     def pressure_mass(self, mu_e: np.ndarray) -> np.ndarray:
         """Lumped sum_e (1/mu_e) int psi_q dV over the pressure DOFs ([ndofs], zero on
         displacement DOFs): the diagonal Schur-complement estimate."""
@@ -123,6 +136,10 @@ class NewtonOptions:
     # values of the Newton matrix inside MINRES: "f32" (FP32 values, FP64 vectors: the
     # Newton residual stays FP64, so only the linear solve's operator is rounded) or "f64"
     minres_values: str = "f32"
+    # Jacobian: "exact" (analytic consistent tangent) or "fd" (iron's one-sided element
+    # finite differences, step (1 + ||x||_2) fd_step)
+    jacobian: str = "exact"
+    fd_step: float = 1e-6
 
 
 @dataclass
@@ -162,14 +179,57 @@ class MRNewton:
         self.Jff = None
         self.Jff32 = None
 
-    def _eval(self, x, body_force, jac: bool):
+    def _eval(self, x, body_force, jac: bool, fd_step=None):
         self.R.zero_()
         self.f.zero_()
         if jac:
             self.J.zero_()
         self.sys.assemble(x, self.material, body_force=body_force, residual=self.R,
-                          jac=self.J if jac else None, rhs=self.f if body_force is not None else None)
+                          jac=self.J if jac and fd_step is None else None,
+                          rhs=self.f if body_force is not None else None)
+        if jac and fd_step is not None:
+            self._jacobian_fd(x, body_force, fd_step)
         return (self.R - self.f).index_select(0, self.free)
+
+    def _jacobian_fd(self, x, body_force, step):
+        """Element Jacobians by one-sided finite differences, as
+        EquationsSet_FiniteElementJacobianEvaluateFD (equations_set_routines.F90:2839-2962):
+        column a of element e is (R_e(x + delta e_a) - R_e(x)) / delta with
+        delta = (1 + ||x||_2) step, added into the equations CSR values.  Elements of one
+        colour share no DOF, so column a of every element of a colour comes from one
+        residual assembly of that colour with all their a-th DOFs perturbed."""
+        sysm = self.sys
+        cs = sysm.csr
+        E, nd = cs.n_elem, cs.n_edof
+        delta = (1.0 + float(torch.linalg.vector_norm(x).item())) * float(step)
+        ed = cs.elem_dofs.long()
+        pos = cs.pos.view(E, nd, nd)
+        dev = x.device
+        R0 = torch.zeros_like(self.R)
+        f0 = torch.zeros_like(self.R)
+        Ra = torch.zeros_like(self.R)
+        fa = torch.zeros_like(self.R)
+        bf = body_force
+        for c in range(cs.n_colours):
+            b0, b1 = int(cs.colour_ptr[c]), int(cs.colour_ptr[c + 1])
+            if b1 == b0:
+                continue
+            els = cs.order[b0:b1].long()
+            rows = ed[els]                                     # [ne_c, nd]
+            cols = pos[els].permute(2, 0, 1).reshape(nd, -1)  # [a] -> positions of column a, rows in order
+            R0.zero_()
+            f0.zero_()
+            sysm.assemble_colour(c, x, self.material, bf, residual=R0, rhs=f0 if bf is not None else None)
+            base = R0 - f0
+            dvec = torch.full((b1 - b0,), delta, dtype=torch.float64, device=dev)
+            for a in range(nd):
+                xp = x.clone()
+                xp.index_add_(0, rows[:, a], dvec)
+                Ra.zero_()
+                fa.zero_()
+                sysm.assemble_colour(c, xp, self.material, bf, residual=Ra, rhs=fa if bf is not None else None)
+                diff = ((Ra - fa) - base) / delta
+                self.J.index_add_(0, cols[a].long(), diff[rows].reshape(-1))
 
     @staticmethod
     def _ew_eta(fn, fn_prev, eta_prev, fn0, o):
@@ -205,7 +265,8 @@ class MRNewton:
         linear = o.linear
         if linear == "auto":
             linear = "minres" if self.sys.formulation == DEVIATORIC else "gmres"
-        F = self._eval(x, body_force, True)
+        fd = o.fd_step if o.jacobian == "fd" else None
+        F = self._eval(x, body_force, True, fd)
         fn = float(torch.linalg.vector_norm(F).item())
         info.fnorm0 = fn
         info.history.append(fn)
@@ -265,7 +326,7 @@ class MRNewton:
             snorm = lam * float(torch.linalg.vector_norm(dx).item())
             x.copy_(xt)
             info.iterations += 1
-            F = self._eval(x, body_force, True)
+            F = self._eval(x, body_force, True, fd)
             fn = float(torch.linalg.vector_norm(F).item())
             info.history.append(fn)
             xnorm = float(torch.linalg.vector_norm(x.index_select(0, self.free)).item())
@@ -384,6 +445,11 @@ def solve(problem, es, solver, bc):
             opts.linear_rtol = float(lin.linearIterativeRelativeTolerance)
             opts.linear_maxit = int(lin.linearIterativeMaximumIterations)
             opts.gmres_restart = int(lin.linearIterativeGMRESRestart)
+    if int(getattr(solver, "newtonJacobianCalculationType", iron.JacobianCalculationTypes.EQUATIONS)) == \
+            iron.JacobianCalculationTypes.FD:
+        opts.jacobian = "fd"
+        eq = es.equations
+        opts.fd_step = float(getattr(eq, "jacobianFiniteDifferenceStepSize", 1e-6)) if eq is not None else 1e-6
     x = torch.as_tensor(x_host, device=sys.device)
     n_inc = max(int(problem._control_loop.maximumIterations), 1)
     infos = load_increments(newton, x, n_inc, incremented, full, body_force, opts)

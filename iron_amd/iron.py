@@ -541,6 +541,14 @@ class Equations(_Obj):
         self.sparsityType = EquationsSparsityTypes.SPARSE
         self.outputType = EquationsOutputTypes.NONE
         self.lumpingType = 0
+        # finite-difference Jacobian step (equations_matrices_routines.F90:356)
+        self.jacobianFiniteDifferenceStepSize = 1e-6
+
+    def JacobianFiniteDifferenceStepSizeSet(self, jacobianNumber, stepSize):
+        """cmfe_Equations_JacobianFiniteDifferenceStepSizeSet (opencmiss_iron.F90:2325)."""
+        if float(stepSize) <= 0.0:
+            raise CMFEError(1, "the finite-difference step size must be positive")
+        self.jacobianFiniteDifferenceStepSize = float(stepSize)
 
 
 class EquationsSet(_Obj):

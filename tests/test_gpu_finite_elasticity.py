@@ -150,7 +150,7 @@ def test_newton_matches_oracle_random_bcs():
     assert np.abs(Rg - R_o).max() < 1e-7 * np.abs(R_o).max()
 
 
-def _cantilever_script(g):
+def _cantilever_script(g, jacobian=None):
     """bindings/python/tests/Cantilever.py / tests/FiniteElasticity/Cantilever.F90 through
     the iron-compatible API with the NO_SUBTYPE (general-branch) law."""
     from iron_amd import iron
@@ -232,7 +232,7 @@ def _cantilever_script(g):
     nls, ls = iron.Solver(), iron.Solver()
     problem.SolversCreateStart()
     problem.SolverGet([iron.ControlLoopIdentifiers.NODE], 1, nls)
-    nls.NewtonJacobianCalculationTypeSet(iron.JacobianCalculationTypes.EQUATIONS)
+    nls.NewtonJacobianCalculationTypeSet(jacobian if jacobian is not None else iron.JacobianCalculationTypes.EQUATIONS)
     nls.newtonAbsoluteTolerance = 1e-12
     nls.newtonRelativeTolerance = 1e-13
     nls.newtonSolutionTolerance = 0.0
@@ -257,12 +257,16 @@ def _cantilever_script(g):
     return dep, rm
 
 
-@pytest.mark.parametrize("name", ["mr_cantilever_python_1x1x1.json", "mr_cantilever_linear_3x2x2.json"])
-def test_cantilever_goldens_through_api(name):
+@pytest.mark.parametrize("name,fd", [("mr_cantilever_python_1x1x1.json", False),
+                                     ("mr_cantilever_linear_3x2x2.json", False),
+                                     ("mr_cantilever_linear_3x2x2.json", True)])
+def test_cantilever_goldens_through_api(name, fd):
+    """The reference's cantilever goldens through the API, with the analytic Jacobian
+    and with iron's default finite-difference Jacobian."""
     from iron_amd import iron
-    print(f"case {name}", flush=True)
+    print(f"case {name} fd={fd}", flush=True)
     g = json.loads((GOLDEN / name).read_text())
-    dep, rm = _cantilever_script(g)
+    dep, rm = _cantilever_script(g, iron.JacobianCalculationTypes.FD if fd else None)
     nodes = rm.comp_user[0]
     z = np.stack([dep._component(iron.FieldVariableTypes.U, c) for c in (1, 2, 3)], axis=1)
     r = np.stack([dep._component(iron.FieldVariableTypes.DELUDELN, c) for c in (1, 2, 3)], axis=1)
@@ -273,6 +277,39 @@ def test_cantilever_goldens_through_api(name):
     p = dep._component(iron.FieldVariableTypes.U, 4)
     gp = np.array([g["elements"][str(e + 1)]["pressure"] for e in range(rm.num_elements)])
     assert np.abs(p - 2.0 * gp).max() < 1e-9 * np.abs(gp).max()
+
+
+def test_fd_jacobian_matches_exact_and_newton_converges():
+    """iron's element finite-difference Jacobian (one-sided, step (1 + ||x||) 1e-6)
+    against the exact tangent at a deformed Q2/Q1 state, and a Newton solve with it
+    reaching the same state as with the exact tangent."""
+    from iron_amd import finite_elasticity as fe
+    print("case FD Jacobian", flush=True)
+    sysg, prob = _systems((2, 2, 2), 2, False, "deviatoric")
+    rng = np.random.default_rng(5)
+    x0 = prob.initial_state(0.5)
+    x0[:3 * prob.nu] += 0.02 * rng.standard_normal(3 * prob.nu)
+    fixed = np.zeros(prob.ndofs, bool)
+    fixed[:3 * prob.nu] = np.tile(prob.geo[:, 0] == 0.0, 3)
+    mat = np.tile([2.0, 6.0], (prob.mesh.num_elements, 1))
+    newton = fe.MRNewton(sysg, fixed, mat)
+    xs = torch.as_tensor(x0, device=sysg.device)
+    newton._eval(xs, None, True)
+    Jx = newton.J.clone()
+    newton._eval(xs, None, True, fd_step=1e-6)
+    Jf = newton.J.clone()
+    scale = float(Jx.abs().max())
+    assert float((Jf - Jx).abs().max()) <= 1e-4 * scale
+    assert float((Jf - Jx).abs().max()) > 0.0
+    fixed2, target = _uniaxial(sysg, prob, 1.1)
+    res = []
+    for jac in ("exact", "fd"):
+        x = torch.as_tensor(prob.initial_state(0.0), device=sysg.device)
+        opts = fe.NewtonOptions(atol=1e-12, rtol=1e-10, stol=0.0, linear_rtol=1e-10, jacobian=jac)
+        infos = fe.load_increments(fe.MRNewton(sysg, fixed2, mat), x, 2, fixed2, target, opts=opts)
+        res.append(x.cpu().numpy())
+        assert all(i.fnorm <= 1e-10 * i.fnorm0 or i.fnorm <= 1e-12 for i in infos), jac
+    assert np.abs(res[0] - res[1]).max() < 1e-8
 
 
 def test_c4_uniaxial_full_size():
