@@ -140,6 +140,39 @@ class NewtonOptions:
     # finite differences, step (1 + ||x||_2) fd_step)
     jacobian: str = "exact"
     fd_step: float = 1e-6
+    # line search on f = ||F||^2 / 2 along the Newton step (iron's
+    # SOLVER_NEWTON_LINESEARCH_*, solver_routines.F90:18826-18845): "basic" (full step),
+    # "linear" (one secant step to the critical point of f), "quadratic" or "cubic"
+    # (backtracking with quadratic / cubic models and sufficient decrease alpha)
+    linesearch: str = "cubic"
+    ls_alpha: float = 1e-4
+    ls_maxit: int = 40
+    ls_minlambda: float = 1e-12
+
+
+def backtrack_lambda(order: str, lam: float, f0: float, slope: float, f_lam: float,
+                     lam_prev: float | None = None, f_prev: float | None = None) -> float:
+    """Next step length of a backtracking line search on f(lambda) = ||F(x + lambda d)||^2 / 2
+    (Dennis & Schnabel 1983, Algorithm A6.3.1): f0 = f(0), slope = f'(0) < 0, f_lam =
+    f(lam) of the rejected step.  The first backtrack (or every one, order "quadratic")
+    minimises the quadratic through f0, slope and f_lam; later ones the cubic through
+    f0, slope, f(lam) and f(lam_prev).  The result is kept in [0.1 lam, 0.5 lam]."""
+    if order == "quadratic" or lam_prev is None:
+        den = 2.0 * (f_lam - f0 - slope * lam)
+        lt = -slope * lam * lam / den if den > 0.0 else 0.5 * lam
+    else:
+        t1 = f_lam - f0 - lam * slope
+        t2 = f_prev - f0 - lam_prev * slope
+        a = (t1 / (lam * lam) - t2 / (lam_prev * lam_prev)) / (lam - lam_prev)
+        b = (-lam_prev * t1 / (lam * lam) + lam * t2 / (lam_prev * lam_prev)) / (lam - lam_prev)
+        if a == 0.0:
+            lt = -slope / (2.0 * b) if b != 0.0 else 0.5 * lam
+        else:
+            d = max(b * b - 3.0 * a * slope, 0.0)
+            lt = (-b + d ** 0.5) / (3.0 * a)
+    if not np.isfinite(lt) or lt > 0.5 * lam:
+        lt = 0.5 * lam
+    return max(lt, 0.1 * lam)
 
 
 @dataclass
@@ -150,6 +183,7 @@ class NewtonInfo:
     fnorm: float = 0.0
     reason: str = ""
     history: list = field(default_factory=list)
+    steps: list = field(default_factory=list)     # line-search step length of each Newton step
 
 
 class MRNewton:
@@ -249,6 +283,47 @@ class MRNewton:
         target = max(o.atol, o.rtol * fn0)
         return max(eta, 0.5 * target / max(fn, 1e-300), 1e-14)
 
+    def _line_search(self, x, dx, F, fn, body_force, o):
+        """Step length along the Newton step dx from x (F = F(x), fn = ||F||); returns
+        (lambda, the new state, stalled)."""
+        xt = x.clone()
+
+        def trial(lam):
+            xt.copy_(x)
+            xt.index_add_(0, self.free, dx, alpha=lam)
+            Ft = self._eval(xt, body_force, False)
+            return Ft, float(torch.linalg.vector_norm(Ft).item())
+
+        if o.linesearch == "basic":
+            trial(1.0)
+            return 1.0, xt, False
+        if o.linesearch == "linear":
+            # one secant step on s(lambda) = F(x + lambda dx) . dx, s(0) = F . dx
+            s0 = float(torch.dot(F, dx).item())
+            F1, _ = trial(1.0)
+            s1 = float(torch.dot(F1, dx).item())
+            lam = s0 / (s0 - s1) if s0 != s1 else 1.0
+            if not np.isfinite(lam) or lam <= o.ls_minlambda:
+                lam = 1.0
+            _, ft = trial(lam)
+            return lam, xt, not np.isfinite(ft)
+        f0 = 0.5 * fn * fn
+        slope = -fn * fn          # f'(0) along an exact Newton step
+        lam, lam_prev, f_prev = 1.0, None, None
+        for _ in range(max(o.ls_maxit, 1)):
+            _, ft = trial(lam)
+            f_lam = 0.5 * ft * ft if np.isfinite(ft) else np.inf
+            if f_lam <= f0 + o.ls_alpha * lam * slope:
+                return lam, xt, False
+            if not np.isfinite(f_lam):
+                lam_new = 0.1 * lam
+            else:
+                lam_new = backtrack_lambda(o.linesearch, lam, f0, slope, f_lam, lam_prev, f_prev)
+            lam_prev, f_prev, lam = lam, f_lam, lam_new
+            if lam < o.ls_minlambda:
+                break
+        return lam, xt, True
+
     def internal_residual(self, x):
         self.R.zero_()
         self.sys.assemble(x, self.material, residual=self.R)
@@ -303,20 +378,7 @@ class MRNewton:
                 li = fem.gmres(self.S, vals, b, dx, eta, o.linear_maxit, self.ws,
                                restart=min(o.gmres_restart, self.restart), pc_inv=self.pc_neg)
             info.linear_iterations += li.iterations
-            lam = 1.0
-            xt = x.clone()
-            stalled = False
-            while True:
-                xt.copy_(x)
-                xt.index_add_(0, self.free, dx, alpha=lam)
-                Ft = self._eval(xt, body_force, False)
-                ft = float(torch.linalg.vector_norm(Ft).item())
-                if np.isfinite(ft) and ft <= (1.0 - 1e-4 * lam) * fn:
-                    break
-                lam *= 0.5
-                if lam < 1e-6:
-                    stalled = True
-                    break
+            lam, xt, stalled = self._line_search(x, dx, F, fn, body_force, o)
             if stalled:
                 # no decrease along an exact Newton direction: ||F|| sits at round-off
                 if fn <= 1e-10 * max(info.fnorm0, 1.0):
@@ -326,6 +388,7 @@ class MRNewton:
             snorm = lam * float(torch.linalg.vector_norm(dx).item())
             x.copy_(xt)
             info.iterations += 1
+            info.steps.append(lam)
             F = self._eval(x, body_force, True, fd)
             fn = float(torch.linalg.vector_norm(F).item())
             info.history.append(fn)
@@ -445,6 +508,11 @@ def solve(problem, es, solver, bc):
             opts.linear_rtol = float(lin.linearIterativeRelativeTolerance)
             opts.linear_maxit = int(lin.linearIterativeMaximumIterations)
             opts.gmres_restart = int(lin.linearIterativeGMRESRestart)
+    ls_type = int(getattr(solver, "newtonLineSearchType", iron.NewtonLineSearchTypes.CUBIC))
+    opts.linesearch = {iron.NewtonLineSearchTypes.NONORMS: "basic", iron.NewtonLineSearchTypes.LINEAR: "linear",
+                       iron.NewtonLineSearchTypes.QUADRATIC: "quadratic",
+                       iron.NewtonLineSearchTypes.CUBIC: "cubic"}[iron.NewtonLineSearchTypes(ls_type)]
+    opts.ls_alpha = float(getattr(solver, "newtonLineSearchAlpha", 1e-4))
     if int(getattr(solver, "newtonJacobianCalculationType", iron.JacobianCalculationTypes.EQUATIONS)) == \
             iron.JacobianCalculationTypes.FD:
         opts.jacobian = "fd"

@@ -190,6 +190,14 @@ class JacobianCalculationTypes(enum.IntEnum):
     EQUATIONS = 2
 
 
+class NewtonLineSearchTypes(enum.IntEnum):
+    """SOLVER_NEWTON_LINESEARCH_* (solver_routines.F90:231-234)."""
+    NONORMS = 1
+    LINEAR = 2
+    QUADRATIC = 3
+    CUBIC = 4
+
+
 class SolverLibraries(enum.IntEnum):
     CMISS = 1
     PETSC = 2
@@ -709,6 +717,8 @@ class Solver(_Obj):
         self.newtonAbsoluteTolerance = 1.0e-10
         self.newtonSolutionTolerance = 1.0e-5
         self.newtonJacobianCalculationType = JacobianCalculationTypes.FD
+        self.newtonLineSearchType = NewtonLineSearchTypes.CUBIC
+        self.newtonLineSearchAlpha = 1e-4
         self._problem = None
         self._linear = None
         self._solver_equations = None
@@ -727,6 +737,16 @@ class Solver(_Obj):
 
     def NewtonJacobianCalculationTypeSet(self, t):
         self.newtonJacobianCalculationType = JacobianCalculationTypes(int(t))
+
+    def NewtonLineSearchTypeSet(self, t):
+        """cmfe_Solver_NewtonLineSearchTypeSet: NONORMS, LINEAR, QUADRATIC or CUBIC."""
+        self.newtonLineSearchType = NewtonLineSearchTypes(int(t))
+
+    def NewtonLineSearchAlphaSet(self, alpha):
+        """cmfe_Solver_NewtonLineSearchAlphaSet: sufficient-decrease parameter."""
+        if float(alpha) <= 0.0:
+            raise CMFEError(1, "the line search alpha must be positive")
+        self.newtonLineSearchAlpha = float(alpha)
 
 
 class Problem(_Obj):

@@ -123,6 +123,29 @@ def test_uniaxial_stretch_gpu_newton(form):
     assert all(i.iterations <= 8 for i in infos)
 
 
+@pytest.mark.parametrize("ls", ["basic", "linear", "quadratic", "cubic"])
+def test_newton_line_searches_reach_the_closed_form(ls):
+    """Each of iron's line-search types (NONORMS, LINEAR, QUADRATIC, CUBIC) drives Newton
+    to the homogeneous-stretch closed form; the backtracking ones also in a single
+    increment to a 1.4 stretch.  (The backtracking step rule itself is pinned on the CPU,
+    tests/test_newton_linesearch.py.)"""
+    from iron_amd import finite_elasticity as fe
+    print(f"case line search {ls}", flush=True)
+    sysg, prob = _systems((2, 2, 2), 2, False, "deviatoric")
+    mat = np.tile([2.0, 6.0], (prob.mesh.num_elements, 1))
+    for lam, incs in ((1.2, 3), (1.4, 1)):
+        if ls in ("basic", "linear") and incs == 1:
+            continue
+        fixed, target = _uniaxial(sysg, prob, lam)
+        x = torch.as_tensor(prob.initial_state(0.0), device=sysg.device)
+        opts = fe.NewtonOptions(atol=1e-11, rtol=1e-12, stol=0.0, linear_rtol=1e-12, linesearch=ls)
+        infos = fe.load_increments(fe.MRNewton(sysg, fixed, mat), x, incs, fixed, target, opts=opts)
+        z, _ = prob.split(x.cpu().numpy())
+        exact = prob.geo * np.array([lam, lam ** -0.5, lam ** -0.5])
+        assert np.abs(z - exact).max() < 1e-9, (ls, lam)
+        assert len(infos[0].steps) == infos[0].iterations
+
+
 def test_newton_matches_oracle_random_bcs():
     """A non-homogeneous case: clamped x=0, prescribed shear+stretch at x=1, gravity."""
     from iron_amd import finite_elasticity as fe

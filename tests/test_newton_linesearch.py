@@ -1,0 +1,34 @@
+"""The backtracking step rule of the Newton line searches (CPU, no GPU): on merits
+that the models reproduce exactly, the quadratic and cubic fits return the exact
+minimiser, and every step stays in [0.1, 0.5] of the rejected one."""
+import numpy as np
+import pytest
+
+from iron_amd.finite_elasticity import backtrack_lambda
+
+
+def test_quadratic_model_is_exact_on_a_quadratic_merit():
+    f0, slope, c = 3.0, -2.0, 2.5          # f(l) = f0 + slope l + c l^2, minimum at l = 0.4
+    f = lambda l: f0 + slope * l + c * l * l
+    assert backtrack_lambda("quadratic", 1.0, f0, slope, f(1.0)) == pytest.approx(0.4, rel=1e-14)
+    # the cubic order takes the quadratic model at its first backtrack
+    assert backtrack_lambda("cubic", 1.0, f0, slope, f(1.0)) == pytest.approx(0.4, rel=1e-14)
+
+
+def test_cubic_model_is_exact_on_a_cubic_merit():
+    f0, slope, b2, a3 = 1.0, -1.0, 0.2, 10.0  # f(l) = f0 + slope l + b2 l^2 + a3 l^3
+    f = lambda l: f0 + slope * l + b2 * l * l + a3 * l ** 3
+    lam_prev, lam = 1.0, 0.5
+    lmin = (-b2 + np.sqrt(b2 * b2 - 3.0 * a3 * slope)) / (3.0 * a3)   # f'(lmin) = 0
+    assert 0.05 < lmin < 0.25
+    got = backtrack_lambda("cubic", lam, f0, slope, f(lam), lam_prev, f(lam_prev))
+    assert got == pytest.approx(lmin, rel=1e-12)
+
+
+def test_steps_are_safeguarded():
+    f0, slope = 1.0, -1.0
+    # a nearly flat rejection asks for a long step: capped at half
+    assert backtrack_lambda("quadratic", 1.0, f0, slope, f0 - 0.999) == pytest.approx(0.5)
+    # a huge rejection asks for a tiny step: floored at a tenth
+    assert backtrack_lambda("quadratic", 1.0, f0, slope, 1e6) == pytest.approx(0.1)
+    assert backtrack_lambda("cubic", 0.3, f0, slope, 1e6, 1.0, 2e6) == pytest.approx(0.03)
