@@ -145,6 +145,10 @@ class NewtonOptions:
     # "linear" (one secant step to the critical point of f), "quadratic" or "cubic"
     # (backtracking with quadratic / cubic models and sufficient decrease alpha)
     linesearch: str = "cubic"
+    # convergence test: "petsc" (atol / rtol on ||F||, stol on the step) or "energy"
+    # (iron's SOLVER_NEWTON_CONVERGENCE_ENERGY_NORM, problem_routines.F90:4232-4300:
+    # |step . F(new)| relative to its value after the first step, below atol)
+    convergence: str = "petsc"
     ls_alpha: float = 1e-4
     ls_maxit: int = 40
     ls_minlambda: float = 1e-12
@@ -184,6 +188,7 @@ class NewtonInfo:
     reason: str = ""
     history: list = field(default_factory=list)
     steps: list = field(default_factory=list)     # line-search step length of each Newton step
+    energy: list = field(default_factory=list)    # step . F(new) per step (energy-norm test)
 
 
 class MRNewton:
@@ -341,6 +346,10 @@ class MRNewton:
         if linear == "auto":
             linear = "minres" if self.sys.formulation == DEVIATORIC else "gmres"
         fd = o.fd_step if o.jacobian == "fd" else None
+        if o.convergence not in ("petsc", "energy"):
+            # SOLVER_NEWTON_CONVERGENCE_DIFFERENTIATED_RATIO is not implemented in iron either
+            raise CMFEError(1, f"convergence test {o.convergence!r} is not implemented")
+        energy_done, energy0 = False, 1.0
         F = self._eval(x, body_force, True, fd)
         fn = float(torch.linalg.vector_norm(F).item())
         info.fnorm0 = fn
@@ -350,10 +359,15 @@ class MRNewton:
         while True:
             if not np.isfinite(fn):
                 raise CMFEError(1, "Nonlinear line search solver did not converge. PETSc diverged F Norm NaN.")
-            if fn <= o.atol:
+            if o.convergence == "energy":
+                # F = 0 exactly: the step is zero, and so is its energy
+                if energy_done or fn == 0.0:
+                    info.reason = "converged energy norm"
+                    break
+            elif fn <= o.atol:
                 info.reason = "converged F norm absolute"
                 break
-            if info.iterations > 0 and fn <= o.rtol * info.fnorm0:
+            elif info.iterations > 0 and fn <= o.rtol * info.fnorm0:
                 info.reason = "converged F norm relative"
                 break
             if info.iterations >= o.maxit:
@@ -392,6 +406,17 @@ class MRNewton:
             F = self._eval(x, body_force, True, fd)
             fn = float(torch.linalg.vector_norm(F).item())
             info.history.append(fn)
+            if o.convergence == "energy":
+                # energy = step . F(new): relative to its first value, below atol; a first
+                # value at zero (5 eps, iron's ZERO_TOLERANCE) converges at once
+                energy = -lam * float(torch.dot(dx, F).item())
+                if info.iterations == 1:
+                    energy0 = energy
+                    energy_done = abs(energy) < 5.0 * np.finfo(np.float64).eps
+                else:
+                    energy_done = abs(energy / energy0) < o.atol
+                info.energy.append(energy)
+                continue
             xnorm = float(torch.linalg.vector_norm(x.index_select(0, self.free)).item())
             if snorm < o.stol * xnorm and fn > o.atol and fn > o.rtol * info.fnorm0:
                 info.reason = "converged S norm relative"
@@ -513,6 +538,11 @@ def solve(problem, es, solver, bc):
                        iron.NewtonLineSearchTypes.QUADRATIC: "quadratic",
                        iron.NewtonLineSearchTypes.CUBIC: "cubic"}[iron.NewtonLineSearchTypes(ls_type)]
     opts.ls_alpha = float(getattr(solver, "newtonLineSearchAlpha", 1e-4))
+    ct = int(getattr(solver, "newtonConvergenceTestType", iron.NewtonConvergenceTypes.PETSC_DEFAULT))
+    opts.convergence = {iron.NewtonConvergenceTypes.PETSC_DEFAULT: "petsc",
+                        iron.NewtonConvergenceTypes.ENERGY_NORM: "energy",
+                        iron.NewtonConvergenceTypes.DIFFERENTIATED_RATIO: "differentiated_ratio"}[
+        iron.NewtonConvergenceTypes(ct)]
     if int(getattr(solver, "newtonJacobianCalculationType", iron.JacobianCalculationTypes.EQUATIONS)) == \
             iron.JacobianCalculationTypes.FD:
         opts.jacobian = "fd"

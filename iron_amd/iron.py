@@ -190,6 +190,13 @@ class JacobianCalculationTypes(enum.IntEnum):
     EQUATIONS = 2
 
 
+class NewtonConvergenceTypes(enum.IntEnum):
+    """SOLVER_NEWTON_CONVERGENCE_* (solver_routines.F90:250-252)."""
+    PETSC_DEFAULT = 1
+    ENERGY_NORM = 2
+    DIFFERENTIATED_RATIO = 3
+
+
 class NewtonLineSearchTypes(enum.IntEnum):
     """SOLVER_NEWTON_LINESEARCH_* (solver_routines.F90:231-234)."""
     NONORMS = 1
@@ -718,6 +725,7 @@ class Solver(_Obj):
         self.newtonSolutionTolerance = 1.0e-5
         self.newtonJacobianCalculationType = JacobianCalculationTypes.FD
         self.newtonLineSearchType = NewtonLineSearchTypes.CUBIC
+        self.newtonConvergenceTestType = NewtonConvergenceTypes.PETSC_DEFAULT
         self.newtonLineSearchAlpha = 1e-4
         self._problem = None
         self._linear = None
@@ -737,6 +745,11 @@ class Solver(_Obj):
 
     def NewtonJacobianCalculationTypeSet(self, t):
         self.newtonJacobianCalculationType = JacobianCalculationTypes(int(t))
+
+    def NewtonConvergenceTestTypeSet(self, t):
+        """cmfe_Solver_NewtonConvergenceTestTypeSet (solver_routines.F90:18349): PETSC_DEFAULT
+        or ENERGY_NORM (DIFFERENTIATED_RATIO raises at solve time, as in iron)."""
+        self.newtonConvergenceTestType = NewtonConvergenceTypes(int(t))
 
     def NewtonLineSearchTypeSet(self, t):
         """cmfe_Solver_NewtonLineSearchTypeSet: NONORMS, LINEAR, QUADRATIC or CUBIC."""

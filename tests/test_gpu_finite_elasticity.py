@@ -146,6 +146,35 @@ def test_newton_line_searches_reach_the_closed_form(ls):
         assert len(infos[0].steps) == infos[0].iterations
 
 
+def test_newton_energy_norm_convergence_reaches_the_closed_form():
+    """iron's energy-norm convergence test (problem_routines.F90:4276-4297): each step's
+    energy lambda*y . F(new) relative to the first step's below atol ends Newton; with the
+    PETSc F-norm tests switched off it still reaches the homogeneous-stretch closed form,
+    the energies fall monotonically, and the differentiated-ratio type raises as in iron."""
+    from iron_amd import finite_elasticity as fe
+    from iron_amd.iron import CMFEError
+    print("case energy-norm convergence", flush=True)
+    sysg, prob = _systems((2, 2, 2), 2, False, "deviatoric")
+    mat = np.tile([2.0, 6.0], (prob.mesh.num_elements, 1))
+    lam = 1.2
+    fixed, target = _uniaxial(sysg, prob, lam)
+    x = torch.as_tensor(prob.initial_state(0.0), device=sysg.device)
+    opts = fe.NewtonOptions(atol=1e-14, rtol=0.0, stol=0.0, linear_rtol=1e-12, convergence="energy")
+    infos = fe.load_increments(fe.MRNewton(sysg, fixed, mat), x, 3, fixed, target, opts=opts)
+    z, _ = prob.split(x.cpu().numpy())
+    exact = prob.geo * np.array([lam, lam ** -0.5, lam ** -0.5])
+    assert np.abs(z - exact).max() < 1e-9
+    for info in infos:
+        assert info.reason == "converged energy norm"
+        assert len(info.energy) == info.iterations >= 2
+        e = np.abs(np.asarray(info.energy))
+        assert e[-1] < opts.atol * e[0] and np.all(np.diff(e) < 0.0)
+    x = torch.as_tensor(prob.initial_state(0.0), device=sysg.device)
+    bad = fe.NewtonOptions(convergence="differentiated_ratio")
+    with pytest.raises(CMFEError):
+        fe.load_increments(fe.MRNewton(sysg, fixed, mat), x, 1, fixed, target, opts=bad)
+
+
 def test_newton_matches_oracle_random_bcs():
     """A non-homogeneous case: clamped x=0, prescribed shear+stretch at x=1, gravity."""
     from iron_amd import finite_elasticity as fe

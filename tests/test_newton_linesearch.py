@@ -32,3 +32,15 @@ def test_steps_are_safeguarded():
     # a huge rejection asks for a tiny step: floored at a tenth
     assert backtrack_lambda("quadratic", 1.0, f0, slope, 1e6) == pytest.approx(0.1)
     assert backtrack_lambda("cubic", 0.3, f0, slope, 1e6, 1.0, 2e6) == pytest.approx(0.03)
+
+
+def test_convergence_test_types_map_through_the_api():
+    """cmfe_Solver_NewtonConvergenceTestTypeSet: iron's three types, PETSC_DEFAULT by
+    default (solver_routines.F90 SOLVER_NEWTON_CONVERGENCE_*)."""
+    from iron_amd import iron
+    s = iron.Solver()
+    assert s.newtonConvergenceTestType == iron.NewtonConvergenceTypes.PETSC_DEFAULT
+    s.NewtonConvergenceTestTypeSet(iron.NewtonConvergenceTypes.ENERGY_NORM)
+    assert s.newtonConvergenceTestType == 2
+    s.NewtonConvergenceTestTypeSet(3)
+    assert s.newtonConvergenceTestType == iron.NewtonConvergenceTypes.DIFFERENTIATED_RATIO
