@@ -331,6 +331,61 @@ int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr,
 int mife_field_update(int n_free, const int32_t* free_idx, const double* x,
                       double* u, void* stream);
 
+/* ------------------------------------- solver mapping with DOF constraints */
+
+/* A constrained DOF c is tied to one free DOF m, u_c = a_c u_m
+ * (BoundaryConditions_DofConstraintSet, src/boundary_condition_routines.F90:3204-3313;
+ * couplings of BoundaryConditions_DofConstraintsCreateFinish, :3320-3472).  The
+ * solver row / column of m stands for every equations row / column coupled to it,
+ * each scaled by its coupling coefficient (SOLVER_MAPPING_CALCULATE,
+ * src/solver_mapping_routines.F90:445-520, :1445-1511), so the solver matrix is
+ * P^T K P:
+ *   owner[g] [n]  solver DOF of equations DOF g, -1 when Dirichlet-fixed (device)
+ *   coef[g]  [n]  coupling coefficient of g (1 for an unconstrained free DOF)
+ * Work size for mife_couple_pattern; the same work buffer must be passed to both
+ * passes. */
+size_t mife_couple_pattern_work_bytes(int n, int64_t nnz, int n_s);
+
+/* Solver pattern of P^T K P (SOLVER_MATRIX_STRUCTURE_CALCULATE,
+ * src/solver_matrices_routines.F90:1612-2431): rows and columns are solver DOFs in
+ * ascending order, columns sorted and unique.  An entry's value is the sum of a run
+ * of equations entries (contrib_*), in ascending equations-CSR position.
+ * Pass 1 (s_col == NULL): s_row_ptr[n_s+1], grp_ptr[n_s+1], grp_rows[n - n_fixed]
+ * (equations rows of each solver row, ascending) and the host reads *nnz_s_out,
+ * *n_contrib_out.  Pass 2: s_col[nnz_s], contrib_ptr[nnz_s+1],
+ * contrib_pos[n_contrib] (equations-CSR positions), contrib_row[n_contrib]
+ * (their rows).  Rejects owner values outside [-1, n_s) and solver DOFs that no
+ * equations row maps to. */
+int mife_couple_pattern(int n, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
+                        const int32_t* owner, int n_s, int32_t* grp_ptr, int32_t* grp_rows,
+                        int32_t* s_row_ptr, int32_t* s_col, int32_t* contrib_ptr,
+                        int32_t* contrib_pos, int32_t* contrib_row, int64_t* nnz_s_out,
+                        int64_t* n_contrib_out, void* work, void* stream);
+
+/* Solver-matrix values dst[e] = sum_run (src[p] a_row(p)) a_col(p), accumulated in
+ * FP64 (SOLVER_MATRIX_EQUATIONS_MATRIX_ADD with coupling coefficients,
+ * src/solver_matrices_routines.F90:935-960).  src_prec / dst_prec: MIFE_F64 or
+ * MIFE_F32; col is the equations-CSR column array. */
+int mife_couple_values(int64_t nnz_s, const int32_t* contrib_ptr, const int32_t* contrib_pos,
+                       const int32_t* contrib_row, const int32_t* col, const double* coef,
+                       const void* src, int src_prec, void* dst, int dst_prec, void* stream);
+
+/* Solver RHS with coupled rows (SOLVER_MATRICES_STATIC_ASSEMBLE,
+ * src/solver_routines.F90:14640-14850, row coupling coefficients :14660-14680):
+ *   b[s] = sum_{r in group s} a_r (src[r] + rhs[r] - lift_r),
+ * lift_r as in mife_solver_rhs over the Dirichlet-fixed columns (owner < 0). */
+int mife_solver_rhs_coupled(int n_s, const int32_t* grp_ptr, const int32_t* grp_rows,
+                            const double* coef, const int32_t* row_ptr, const int32_t* col,
+                            const double* vals, const int32_t* owner, const double* u,
+                            const double* src, const double* rhs, const uint8_t* rhs_fixed,
+                            double* b, void* stream);
+
+/* u[g] = a_g x[owner[g]] for every DOF with owner[g] >= 0; fixed DOFs keep their
+ * values (SOLVER_VARIABLES_FIELD_UPDATE with coupling coefficients,
+ * src/solver_routines.F90:22814-23001, :22877). */
+int mife_field_update_coupled(int n, const int32_t* owner, const double* coef,
+                              const double* x, double* u, void* stream);
+
 /* Reactions / flux on the RHS variable, EQUATIONS_SET_BACKSUBSTITUTE CSR branch
  * (src/equations_set_routines.F90:1784-1830).  iron_compat=1 reproduces the
  * reference's u[k - rowstart] read (:1806); 0 computes the correct K u.

@@ -107,6 +107,12 @@ _SIG = {
     "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
     "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mife_field_update": (_I, [_I, _P, _P, _P, _P]),
+    "mife_couple_pattern_work_bytes": (C.c_size_t, [_I, _I64, _I]),
+    "mife_couple_pattern": (_I, [_I, _I64, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, C.POINTER(_I64),
+                                 C.POINTER(_I64), _P, _P]),
+    "mife_couple_values": (_I, [_I64, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P]),
+    "mife_solver_rhs_coupled": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "mife_field_update_coupled": (_I, [_I, _P, _P, _P, _P, _P]),
     "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "mife_solver_work_bytes": (C.c_size_t, [_I, _I]),
     "mife_cg": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),

@@ -149,14 +149,53 @@ def _boundary_conditions(bc, es, dm: DofMap, rm: RegularMesh):
     return fixed, u, neumann, increments, rhs_fixed, frozenset(integrated_only)
 
 
+def _dof_constraints(bc, es, dm: DofMap, rm: RegularMesh, fixed: np.ndarray):
+    """Solver DOF and coupling coefficient of every DOF under the constraints of
+    ConstrainNodeDofsEqual (DofConstraintSet / DofConstraintsCreateFinish,
+    boundary_condition_routines.F90:3204-3472; solver rows of SOLVER_MAPPING_CALCULATE,
+    solver_mapping_routines.F90:422-520).  Returns (owner, coef, constrained) or None."""
+    iron = _iron()
+    cons = getattr(bc, "_constraints", [])
+    if not cons:
+        return None
+    dep = es.dependentField
+    n = dm.num_dofs
+    master = np.full(n, -1, dtype=np.int64)
+    coef = np.ones(n, dtype=np.float64)
+    for (field, vt, comp, nodes, a) in cons:
+        if field is not dep and field.__dict__ is not dep.__dict__:
+            raise CMFEError(1, "DOF constraints must be set on the dependent field")
+        if vt != iron.FieldVariableTypes.U:
+            raise CMFEError(1, "DOF constraints are implemented on the U variable")
+        mc = dm.comp_mesh[comp - 1]
+        dofs = [int(dm.offsets[comp - 1] + rm.local_node(nd, mc)) for nd in nodes]
+        m = dofs[0]
+        for c in dofs[1:]:
+            if master[c] >= 0:
+                raise CMFEError(1, f"DOF {c + 1} is constrained twice")
+            master[c] = m
+            coef[c] = a
+    constrained = master >= 0
+    if np.any(fixed & constrained):
+        raise CMFEError(1, "a DOF that is part of a linear constraint also has a fixed boundary condition")
+    if np.any(fixed[master[constrained]]) or np.any(constrained[master[constrained]]):
+        raise CMFEError(1, "a DOF depended on by a linear constraint is not free in the boundary conditions")
+    solver_dof = ~fixed & ~constrained
+    rank = np.full(n, -1, dtype=np.int64)
+    rank[solver_dof] = np.arange(int(solver_dof.sum()))
+    owner = np.where(constrained, rank[np.maximum(master, 0)], rank)
+    return owner.astype(np.int32), coef, constrained
+
+
 class _LinearCache:
     pass
 
 
-def _linear_setup(es, rm: RegularMesh, dm: DofMap, fixed: np.ndarray):
+def _linear_setup(es, rm: RegularMesh, dm: DofMap, fixed: np.ndarray, couple=None):
     """Device structures for a linear equations set, cached while valid."""
     cache = getattr(es, "_device_cache", None)
-    key = (id(rm), dm.num_dofs, fixed.tobytes())
+    key = (id(rm), dm.num_dofs, fixed.tobytes(),
+           None if couple is None else (couple[0].tobytes(), couple[1].tobytes(), couple[2].tobytes()))
     if cache is not None and cache.key == key:
         return cache
     c = _LinearCache()
@@ -164,7 +203,12 @@ def _linear_setup(es, rm: RegularMesh, dm: DofMap, fixed: np.ndarray):
     dev = torch.device("cuda:0")
     ed = dm.element_dofs()
     c.csr = fem.CSRStructure(ed, dm.num_dofs, dev, colours=structured_colours(rm))
-    c.red = fem.DirichletReduction(c.csr, fixed)
+    if couple is None:
+        c.red = fem.DirichletReduction(c.csr, fixed)
+        c.lead = np.nonzero(~fixed)[0]
+    else:
+        c.red = fem.CoupledReduction(c.csr, *couple)
+        c.lead = c.red.lead
     # a vector field on one mesh component: rows node by node (16-bit SELL columns)
     ncomp = len(dm.comp_mesh)
     nper = rm.num_nodes(dm.comp_mesh[0])
@@ -198,7 +242,7 @@ def solve_linear(es, solver, bc):
     if mc != 0:
         raise CMFEError(1, "the dependent field must use the geometric mesh component")
     fixed, u_host, neumann, _, rhs_fixed, integrated_only = _boundary_conditions(bc, es, dm, rm)
-    c = _linear_setup(es, rm, dm, fixed)
+    c = _linear_setup(es, rm, dm, fixed, _dof_constraints(bc, es, dm, rm, fixed))
     dev = c.csr.device
     gvar = geo._var(iron.FieldVariableTypes.U)
     coords = np.stack([geo._component(iron.FieldVariableTypes.U, k + 1) for k in range(rm.dim)], axis=1)
@@ -258,7 +302,7 @@ def solve_linear(es, solver, bc):
         rtol, atol = 1e-14, 0.0  # no GPU direct factorisation: iterate to round-off
     bn = float(torch.linalg.vector_norm(bvec).item()) if c.red.n_free else 0.0
     rt = max(rtol, atol / bn) if bn > 0 else rtol
-    x = torch.as_tensor(u_host[~fixed] if c.red.n_free else np.zeros(0), device=dev).clone()
+    x = torch.as_tensor(u_host[c.lead] if c.red.n_free else np.zeros(0), device=dev).clone()
     s64 = c.S.gather(v64, fem.F64)
     ktype = int(solver.linearIterativeType)
     if c.red.n_free == 0:

@@ -147,6 +147,88 @@ class DirichletReduction:
         call("mife_field_update", self.n_free, ptr(self.free_idx), ptr(x), ptr(u), _stream())
 
 
+class CoupledReduction:
+    """Solver system with linear DOF constraints, P^T K P.
+
+    ``owner[g]`` is the solver DOF of equations DOF g (-1 when Dirichlet-fixed) and
+    ``coef[g]`` its coupling coefficient: a constrained DOF u_c = a u_m takes m's
+    solver DOF and coefficient a, every other free DOF coefficient 1
+    (SOLVER_MAPPING_CALCULATE with DOF couplings, solver_mapping_routines.F90:445-520,
+    :1445-1511).  Solver values are sums of coupled equations entries
+    (``mife_couple_values``), so ``src_pos`` indexes this object's own solver-CSR
+    values: a SELL solver matrix gathers from them.
+    """
+
+    coupled = True
+
+    def __init__(self, csr: CSRStructure, owner: np.ndarray, coef: np.ndarray, constrained: np.ndarray):
+        dev = csr.device
+        self.csr = csr
+        owner = np.asarray(owner, dtype=np.int32)
+        constrained = np.asarray(constrained, dtype=bool)
+        if owner.shape != (csr.n,) or np.shape(coef) != (csr.n,) or constrained.shape != (csr.n,):
+            raise ValueError("owner, coef and constrained need one entry per equations DOF")
+        self.n_free = int(owner.max()) + 1 if csr.n and owner.max() >= 0 else 0
+        self.fixed_mask = owner < 0
+        # the solver DOFs are the unconstrained free DOFs in ascending order
+        lead = np.nonzero((owner >= 0) & ~constrained)[0].astype(np.int32)
+        if len(lead) != self.n_free or not np.array_equal(owner[lead], np.arange(self.n_free)):
+            raise ValueError("owner must number the unconstrained free DOFs 0, 1, ... in ascending order")
+        self.lead = lead
+        self.owner = torch.as_tensor(owner, device=dev)
+        self.coef = torch.as_tensor(np.asarray(coef, dtype=np.float64), device=dev)
+        self.free_idx = torch.as_tensor(lead, device=dev)
+        n_s = self.n_free
+        self.row_ptr = torch.zeros(n_s + 1, dtype=torch.int32, device=dev)
+        self.grp_ptr = torch.zeros(n_s + 1, dtype=torch.int32, device=dev)
+        n_grp = int((owner >= 0).sum())
+        self.grp_rows = torch.empty(max(n_grp, 1), dtype=torch.int32, device=dev)
+        self.nnz, self.n_contrib = 0, 0
+        self.col = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.contrib_ptr = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.contrib_pos = self.contrib_row = torch.zeros(1, dtype=torch.int32, device=dev)
+        if n_s > 0 and csr.nnz > 0:
+            s = _stream()
+            wbytes = _lib.lib().mife_couple_pattern_work_bytes(csr.n, csr.nnz, n_s)
+            work = torch.empty(int(wbytes), dtype=torch.uint8, device=dev)
+            nnz_s, n_c = C.c_int64(0), C.c_int64(0)
+            call("mife_couple_pattern", csr.n, csr.nnz, ptr(csr.row_ptr), ptr(csr.col), ptr(self.owner), n_s,
+                 ptr(self.grp_ptr), ptr(self.grp_rows), ptr(self.row_ptr), None, None, None, None, C.byref(nnz_s),
+                 C.byref(n_c), ptr(work), s)
+            self.nnz, self.n_contrib = int(nnz_s.value), int(n_c.value)
+            self.col = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+            self.contrib_ptr = torch.empty(self.nnz + 1, dtype=torch.int32, device=dev)
+            self.contrib_pos = torch.empty(max(self.n_contrib, 1), dtype=torch.int32, device=dev)
+            self.contrib_row = torch.empty(max(self.n_contrib, 1), dtype=torch.int32, device=dev)
+            call("mife_couple_pattern", csr.n, csr.nnz, ptr(csr.row_ptr), ptr(csr.col), ptr(self.owner), n_s,
+                 ptr(self.grp_ptr), ptr(self.grp_rows), ptr(self.row_ptr), ptr(self.col), ptr(self.contrib_ptr),
+                 ptr(self.contrib_pos), ptr(self.contrib_row), None, None, ptr(work), s)
+            del work
+        self.src_pos = torch.arange(max(self.nnz, 1), dtype=torch.int32, device=dev)
+
+    def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Solver-CSR values of P^T K P (FP64 sums) from equations-CSR values."""
+        if out is None:
+            out = torch.empty(max(self.nnz, 1), dtype=torch.float64 if prec == F64 else torch.float32,
+                              device=vals.device)
+        src_prec = F64 if vals.dtype == torch.float64 else F32
+        call("mife_couple_values", self.nnz, ptr(self.contrib_ptr), ptr(self.contrib_pos), ptr(self.contrib_row),
+             ptr(self.csr.col), ptr(self.coef), ptr(vals), src_prec, ptr(out), prec, _stream())
+        return out
+
+    def rhs(self, vals64, u, src=None, rhs=None, rhs_fixed=None, out=None) -> torch.Tensor:
+        """b_s = sum over the rows coupled to s of a_r (src_r + rhs_r - lift_r)."""
+        if out is None:
+            out = torch.empty(max(self.n_free, 1), dtype=torch.float64, device=vals64.device)
+        call("mife_solver_rhs_coupled", self.n_free, ptr(self.grp_ptr), ptr(self.grp_rows), ptr(self.coef),
+             ptr(self.csr.row_ptr), ptr(self.csr.col), ptr(vals64), ptr(self.owner), ptr(u), ptr(src), ptr(rhs),
+             ptr(rhs_fixed), ptr(out), _stream())
+        return out
+
+    def field_update(self, x, u):
+        call("mife_field_update_coupled", self.csr.n, ptr(self.owner), ptr(self.coef), ptr(x), ptr(u), _stream())
+
+
 class SolverMatrix:
     """Storage of the reduced (free x free) solver matrix.
 
@@ -243,6 +325,11 @@ class SolverMatrix:
 
     def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
         """Solver-matrix values (this storage) from equations-CSR values (FP64 or FP32)."""
+        if getattr(self.red, "coupled", False):
+            # coupled rows and columns: solver-CSR sums first, then this storage
+            vals = self.red.gather(vals, F64 if vals.dtype == torch.float64 else F32)
+            if self.fmt == "csr" and out is None and (vals.dtype == torch.float64) == (prec == F64):
+                return vals
         if out is None:
             out = torch.empty(max(self.stored, 1), dtype=torch.float64 if prec == F64 else torch.float32,
                               device=vals.device)

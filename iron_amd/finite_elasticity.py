@@ -487,6 +487,8 @@ def solve(problem, es, solver, bc):
     fixed = np.zeros(n, dtype=bool)
     incremented = np.zeros(n, dtype=bool)
     full = np.zeros(n)
+    if getattr(bc, "_constraints", None):
+        raise CMFEError(1, "DOF constraints are implemented for linear equations sets only")
     for (fld, vt, node, comp, cond, value, add) in bc._entries:
         if vt != U:
             raise CMFEError(1, "finite elasticity supports boundary conditions on the U variable only")

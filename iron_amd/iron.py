@@ -135,6 +135,7 @@ class BoundaryConditionsTypes(enum.IntEnum):
     NEUMANN_INTEGRATED = 9
     FIXED_INCREMENTED = 13
     NEUMANN_INTEGRATED_ONLY = 20
+    LINEAR_CONSTRAINT = 21
 
 
 class LinearSolverTypes(enum.IntEnum):
@@ -696,6 +697,20 @@ class SolverEquations(_Obj):
 class BoundaryConditions(_Obj):
     def __init__(self):
         self._entries = []  # (field, vt, node, comp, condition, value, add)
+        self._constraints = []  # (field, vt, comp, nodes, coefficient)
+
+    def ConstrainNodeDofsEqual(self, field, fieldVariableType, versionNumber, derivativeNumber, component, nodes,
+                               coefficient):
+        """BoundaryConditions_ConstrainNodeDofsEqual / ConstrainDofsEqual
+        (boundary_condition_routines.F90:3100-3196): the DOF of ``component`` at each of
+        nodes[1:] becomes ``coefficient`` times the DOF at nodes[0], one solver DOF for
+        all of them (DofConstraintSet, :3204-3313)."""
+        nodes = tuple(int(n) for n in nodes)
+        if len(nodes) < 2:
+            raise CMFEError(1, "Cannot constrain zero or 1 DOF to be equal.")
+        if len(set(nodes)) != len(nodes):
+            raise CMFEError(1, "a node is duplicated in the DOFs constrained to be equal")
+        self._constraints.append((field, int(fieldVariableType), int(component), nodes, float(coefficient)))
 
     def SetNode(self, field, vt, version, derivative, node, comp, condition, value):
         self._entries.append((field, int(vt), int(node), int(comp), int(condition), float(value), False))

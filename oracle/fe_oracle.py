@@ -688,6 +688,100 @@ def field_update(n, free, x, fixed_mask, u_fixed):
     return u
 
 
+# ----------------------------------------------------------------------------
+# Linear DOF constraints: coupled rows and columns in the solver mapping
+# ----------------------------------------------------------------------------
+def constraint_couplings(fixed_mask, constraints):
+    """Solver DOF (owner) and coupling coefficient of every equations DOF.
+
+    constraints: (c, m, a) triples, 0-based, meaning u_c = a u_m.  Follows
+    BoundaryConditions_DofConstraintSet (boundary_condition_routines.F90:3204-3313:
+    one DOF per constraint, the DOF depended on must be free), the check and
+    coupling lists of BoundaryConditions_DofConstraintsCreateFinish (:3320-3472: a
+    constrained DOF later fixed is an error; m's list is [m (1), c (a), ...]) and the
+    row inclusion of SOLVER_MAPPING_CALCULATE (solver_mapping_routines.F90:422-520:
+    only FREE rows are solver rows, in ascending order; a constrained row enters
+    through the coupling list of its master).  A DOF constrained twice is rejected.
+    Returns (owner int64 [n], -1 where fixed; coef float64 [n]).
+    """
+    fixed = np.asarray(fixed_mask, dtype=bool)
+    n = len(fixed)
+    master = np.full(n, -1, dtype=np.int64)
+    coef = np.ones(n, dtype=np.float64)
+    for c, m, a in constraints:
+        c, m = int(c), int(m)
+        if c == m:
+            raise ValueError(f"DOF {c} is constrained to itself")
+        if master[c] >= 0:
+            raise ValueError(f"DOF {c} is constrained twice")
+        if fixed[m] or master[m] >= 0:
+            raise ValueError(f"DOF {m} is not free in the boundary conditions")
+        if fixed[c]:
+            raise ValueError(f"DOF {c} is part of a linear constraint but is fixed")
+        master[c] = m
+        coef[c] = float(a)
+    if np.any(master[master[master >= 0]] >= 0):
+        raise ValueError("a constraint depends on a constrained DOF")
+    solver_dof = (~fixed) & (master < 0)
+    rank = np.full(n, -1, dtype=np.int64)
+    rank[solver_dof] = np.arange(int(solver_dof.sum()))
+    owner = np.where(master >= 0, rank[np.maximum(master, 0)], rank)
+    owner[fixed] = -1
+    return owner, coef
+
+
+def reduce_coupled(row_ptr, col, vals, owner, coef, u_fixed, rhs_extra, rhs_fixed_mask=None):
+    """Solver system P^T K P and its RHS with coupled rows and columns.
+
+    Matrix: SOLVER_MATRIX_EQUATIONS_MATRIX_ADD (solver_matrices_routines.F90:935-960)
+    adds K_rc a_r a_c to solver entry (owner[r], owner[c]) for every equations entry,
+    row by row; the pattern of SOLVER_MATRIX_STRUCTURE_CALCULATE (:1612-2431) is the
+    sorted, unique set of those entries.  RHS: SOLVER_MATRICES_STATIC_ASSEMBLE
+    (solver_routines.F90:14640-14850) adds a_r (rhs_extra_r - lift_r) to solver row
+    owner[r], with the ZERO_TOLERANCE lifting of reduce_dirichlet over the fixed
+    columns.  Returns (s_row_ptr, s_col, s_vals, b).
+    """
+    owner = np.asarray(owner, dtype=np.int64)
+    coef = np.asarray(coef, dtype=np.float64)
+    n = len(owner)
+    n_s = int(owner.max()) + 1 if n and owner.max() >= 0 else 0
+    rows = np.repeat(np.arange(n), np.diff(row_ptr))
+    keep = (owner[rows] >= 0) & (owner[col] >= 0)
+    key = owner[rows[keep]] * n_s + owner[col[keep]]
+    contrib = (vals[keep] * coef[rows[keep]]) * coef[col[keep]]
+    order = np.argsort(key, kind="stable")
+    uk, first = np.unique(key[order], return_index=True)
+    s_vals = np.add.reduceat(contrib[order], first) if len(first) else np.zeros(0)
+    s_row = uk // max(n_s, 1)
+    s_col = uk % max(n_s, 1)
+    s_row_ptr = np.zeros(n_s + 1, dtype=np.int64)
+    np.add.at(s_row_ptr, s_row + 1, 1)
+    s_row_ptr = np.cumsum(s_row_ptr)
+    fixed = owner < 0
+    lift = (~fixed[rows]) & fixed[col]
+    if rhs_fixed_mask is not None:
+        lift &= ~np.asarray(rhs_fixed_mask, dtype=bool)[rows]
+    ud = np.asarray(u_fixed, dtype=np.float64)[col[lift]]
+    kv = vals[lift]
+    ok = (np.abs(ud) >= ZERO_TOLERANCE) & (np.abs(kv) >= ZERO_TOLERANCE)
+    r_eff = np.asarray(rhs_extra, dtype=np.float64).copy()
+    np.add.at(r_eff, rows[lift][ok], -1.0 * kv[ok] * ud[ok])
+    b = np.zeros(n_s)
+    nf = ~fixed
+    np.add.at(b, owner[nf], r_eff[nf] * coef[nf])
+    return s_row_ptr, s_col, s_vals, b
+
+
+def field_update_coupled(owner, coef, x, u_fixed):
+    """SOLVER_VARIABLES_FIELD_UPDATE with coupling coefficients (solver_routines.F90:
+    22814-23001, value = x * coefficient + 0 at :22877); fixed DOFs keep their values."""
+    owner = np.asarray(owner, dtype=np.int64)
+    u = np.where(owner < 0, np.asarray(u_fixed, dtype=np.float64), 0.0)
+    nf = owner >= 0
+    u[nf] = np.asarray(x, dtype=np.float64)[owner[nf]] * np.asarray(coef)[nf]
+    return u
+
+
 def backsubstitute(row_ptr, col, vals, u, source=None, rhs_fixed_mask=None, iron_compat=False):
     """EQUATIONS_SET_BACKSUBSTITUTE, CSR branch, equations_set_routines.F90:1784-1830.
 
