@@ -12,6 +12,8 @@ DOFs stay the same.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import torch
 
@@ -39,52 +41,109 @@ def boundary_faces(rm: RegularMesh):
     return out
 
 
-def neumann_integrate(rm: RegularMesh, dm: DofMap, comp_basis, point_values: dict, geom_comp=0,
-                      integrated_only=frozenset()):
-    """BoundaryConditions_NeumannIntegrate (boundary_condition_routines.F90:2654-2996),
-    3-D node-based case: every NEUMANN_POINT DOF o with value q_o is integrated over
-    every boundary face that contains its node, f_m += q_o * int_face phi_m phi_o dA,
-    for the face nodes m of the same component; a face holding a
-    NEUMANN_INTEGRATED_ONLY DOF of that component is skipped (:2860-2862).
-    ``point_values``: {dof: q}; ``integrated_only``: set of DOFs."""
-    if not point_values:
-        return np.zeros(dm.num_dofs)
-    f = np.zeros(dm.num_dofs)
+def neumann_matrix(rm: RegularMesh, dm: DofMap, comp_basis, point_dofs, geom_comp=0, integrated_only=frozenset()):
+    """Neumann integration matrix N of BoundaryConditions_NeumannMatricesInitialise
+    (boundary_condition_routines.F90:2227-2583): rows are the RHS variable's DOFs,
+    columns the NEUMANN_POINT DOFs in the order of ``point_dofs``, and
+    N[m, o] = sum over the boundary faces holding both nodes of int_face phi_m phi_o dA
+    (3-D node-based case of BoundaryConditions_NeumannIntegrate, :2654-2996, for the
+    face nodes m of o's component).  A face holding a NEUMANN_INTEGRATED_ONLY DOF of
+    that component is skipped (:2860-2862).  Geometry is interpolated with the basis of
+    mesh component ``geom_comp``.  Returns CSR (row_ptr [n+1], col, vals) with sorted,
+    unique columns per row; f = N q is the integrated set (FIELD_INTEGRATED_NEUMANN_SET)."""
+    n = dm.num_dofs
+    point_dofs = np.asarray(point_dofs, dtype=np.int64)
+    col_of = {int(d): j for j, d in enumerate(point_dofs)}
+    rows, cols, vals = [], [], []
     coords = rm.coords
-    by_comp = {}
-    for dof, q in point_values.items():
-        c = int(dm.component_of(np.array([dof]))[0])
-        by_comp.setdefault(c, {})[int(dof)] = float(q)
-    for c, vals in by_comp.items():
+    comps = sorted(set(int(c) for c in dm.component_of(point_dofs))) if len(point_dofs) else []
+    for c in comps:
         mc = dm.comp_mesh[c]
         b: LagrangeBasis = comp_basis[c]
         en = rm.elem_nodes[mc]
         geo_en = rm.elem_nodes[geom_comp]
+        gb = rm.bases[geom_comp]
         last = b.nodes_xi - 1
         for (e, k, side) in boundary_faces(rm):
-            fmask = b.node_index[:, k] == (0 if side == 0 else last)
-            fnodes = np.nonzero(fmask)[0]
+            fnodes = np.nonzero(b.node_index[:, k] == (0 if side == 0 else last))[0]
             dofs = dm.offsets[c] + en[e, fnodes]
-            qs = np.array([vals.get(int(d), None) for d in dofs], dtype=object)
-            has = np.array([q is not None for q in qs])
+            has = np.array([int(d) in col_of for d in dofs])
             if not has.any():
                 continue
             if integrated_only and any(int(d) in integrated_only for d in dofs):
                 continue
-            N, dN, wf = face_quadrature(b, k, side)
-            # geometry interpolated with the geometric basis of mesh component 1
-            gb = rm.bases[geom_comp]
-            Ng, dNg, _ = face_quadrature(gb, k, side)
+            N, _, wf = face_quadrature(b, k, side)
+            _, dNg, _ = face_quadrature(gb, k, side)
             xe = coords[geo_en[e]]
             tang = [j for j in range(rm.dim) if j != k]
             dx = np.einsum("gaj,ad->gdj", dNg[:, :, tang], xe)  # [ng, dim, 2]
             gl = np.einsum("gdi,gdj->gij", dx, dx)
             jac = np.sqrt(np.abs(gl[:, 0, 0] * gl[:, 1, 1] - gl[:, 0, 1] * gl[:, 1, 0]))
             Mf = np.einsum("ga,gb,g->ab", N[:, fnodes], N[:, fnodes], wf * jac)
-            qv = np.array([q if q is not None else 0.0 for q in qs], dtype=np.float64)
-            f_local = Mf[:, has] @ qv[has]
-            np.add.at(f, dofs, f_local)
+            src = np.nonzero(has)[0]
+            rr, cc = np.meshgrid(dofs, [col_of[int(dofs[j])] for j in src], indexing="ij")
+            rows.append(rr.ravel())
+            cols.append(cc.ravel())
+            vals.append(Mf[:, src].ravel())
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    if not rows:
+        return row_ptr, np.zeros(0, dtype=np.int64), np.zeros(0)
+    r = np.concatenate(rows).astype(np.int64)
+    cidx = np.concatenate(cols).astype(np.int64)
+    v = np.concatenate(vals)
+    key = r * max(len(point_dofs), 1) + cidx
+    order = np.argsort(key, kind="stable")
+    uk, first = np.unique(key[order], return_index=True)
+    sv = np.add.reduceat(v[order], first)
+    ur = uk // max(len(point_dofs), 1)
+    np.add.at(row_ptr, ur + 1, 1)
+    return np.cumsum(row_ptr), uk % max(len(point_dofs), 1), sv
+
+
+def neumann_integrate(rm: RegularMesh, dm: DofMap, comp_basis, point_values: dict, geom_comp=0,
+                      integrated_only=frozenset()):
+    """Integrated Neumann values f = N q on the host (see neumann_matrix);
+    ``point_values``: {dof: q}; ``integrated_only``: set of DOFs."""
+    if not point_values:
+        return np.zeros(dm.num_dofs)
+    pd = np.array(sorted(point_values), dtype=np.int64)
+    rp, col, v = neumann_matrix(rm, dm, comp_basis, pd, geom_comp, integrated_only)
+    q = np.array([point_values[int(d)] for d in pd])
+    f = np.zeros(dm.num_dofs)
+    rows = np.repeat(np.arange(dm.num_dofs), np.diff(rp))
+    np.add.at(f, rows, v * q[col])
     return f
+
+
+class NeumannIntegrator:
+    """The integration matrix on the device: NeumannIntegrate's f = N q
+    (boundary_condition_routines.F90:2654-2996) as one CSR product per solve, with
+    the point values q resident in HBM."""
+
+    def __init__(self, rm: RegularMesh, dm: DofMap, comp_basis, point_dofs, device, geom_comp=0,
+                 integrated_only=frozenset()):
+        from . import _lib
+        self.point_dofs = np.asarray(point_dofs, dtype=np.int64)
+        rp, col, v = neumann_matrix(rm, dm, comp_basis, self.point_dofs, geom_comp, integrated_only)
+        self.n = dm.num_dofs
+        self.nnz = len(col)
+        self.row_ptr = torch.as_tensor(rp.astype(np.int32), device=device)
+        self.col = torch.as_tensor(np.maximum(col, 0).astype(np.int32) if len(col) else np.zeros(1, np.int32),
+                                   device=device)
+        self.vals = torch.as_tensor(v if len(v) else np.zeros(1), dtype=torch.float64, device=device)
+        self.desc = _lib.MatrixDesc(self.n, _lib.MIFE_FMT_CSR, _lib.ptr(self.row_ptr), _lib.ptr(self.col), None, 0,
+                                    self.nnz, None)
+
+    def integrate(self, q: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """f [n] = N q for point values q [len(point_dofs)] (FP64, device)."""
+        from . import _lib
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.float64, device=q.device)
+        if len(self.point_dofs) == 0:
+            return out.zero_()
+        _lib.call("mife_spmv", C.byref(self.desc), _lib.ptr(self.vals), _lib.ptr(q), _lib.ptr(out), _lib.MIFE_F64,
+                  _lib.stream_ptr())
+        return out
 
 
 # ------------------------------------------------------------ helpers
@@ -286,11 +345,16 @@ def solve_linear(es, solver, bc):
         raise CMFEError(1, f"equations set kind {kind} is not implemented")
     # RHS: DELUDELN values of rows whose RHS DOF is fixed, plus the integrated
     # Neumann point values (b = f + N q, solver_routines.F90:14681-14696,14850)
-    rhs_host = np.where(rhs_fixed, dep._data(iron.FieldVariableTypes.DELUDELN), 0.0)
+    rhs = torch.as_tensor(np.where(rhs_fixed, dep._data(iron.FieldVariableTypes.DELUDELN), 0.0), device=dev)
     if neumann:
-        rhs_host = rhs_host + neumann_integrate(rm, dm, [b] * len(dm.comp_mesh), neumann,
-                                                integrated_only=integrated_only)
-    rhs = torch.as_tensor(rhs_host, device=dev)
+        # integration matrix built once per set of point DOFs, f = N q on the device
+        pd = np.array(sorted(neumann), dtype=np.int64)
+        nkey = (pd.tobytes(), tuple(sorted(integrated_only)))
+        if getattr(c, "neumann_key", None) != nkey:
+            c.neumann = NeumannIntegrator(rm, dm, [b] * len(dm.comp_mesh), pd, dev, integrated_only=integrated_only)
+            c.neumann_key = nkey
+        q = torch.as_tensor(np.array([neumann[int(d)] for d in pd], dtype=np.float64), device=dev)
+        rhs += c.neumann.integrate(q)
     rfix = torch.as_tensor(rhs_fixed.astype(np.uint8), device=dev) if rhs_fixed.any() else None
     u = torch.as_tensor(np.where(fixed, u_host, 0.0), device=dev)
     bvec = c.red.rhs(v64, u, src=src, rhs=rhs, rhs_fixed=rfix)
