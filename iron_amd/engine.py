@@ -167,7 +167,13 @@ def _element_values(field, comp: int, rm: RegularMesh, vt=None):
     raise CMFEError(1, "node-based material parameters are not supported by the element kernels")
 
 
-def _boundary_conditions(bc, es, dm: DofMap, rm: RegularMesh):
+def _is(a, b):
+    return a is b or a.__dict__ is b.__dict__
+
+
+def _boundary_conditions(bc, es, dm: DofMap, rm: RegularMesh, others=()):
+    """Conditions on this set's dependent field; entries on the dependent fields in
+    ``others`` (the other sets of the same solver equations) are skipped."""
     iron = _iron()
     dep = es.dependentField
     n = dm.num_dofs
@@ -178,7 +184,9 @@ def _boundary_conditions(bc, es, dm: DofMap, rm: RegularMesh):
     increments = {}
     integrated_only = set()
     for (field, vt, node, comp, cond, value, add) in bc._entries:
-        if field is not dep and field.__dict__ is not dep.__dict__:
+        if any(_is(field, o) for o in others):
+            continue
+        if not _is(field, dep):
             raise CMFEError(1, "boundary conditions must be set on the dependent field")
         mc = dm.comp_mesh[comp - 1]
         dof = int(dm.offsets[comp - 1] + rm.local_node(node, mc))
@@ -208,13 +216,13 @@ def _boundary_conditions(bc, es, dm: DofMap, rm: RegularMesh):
     return fixed, u, neumann, increments, rhs_fixed, frozenset(integrated_only)
 
 
-def _dof_constraints(bc, es, dm: DofMap, rm: RegularMesh, fixed: np.ndarray):
+def _dof_constraints(bc, es, dm: DofMap, rm: RegularMesh, fixed: np.ndarray, others=()):
     """Solver DOF and coupling coefficient of every DOF under the constraints of
     ConstrainNodeDofsEqual (DofConstraintSet / DofConstraintsCreateFinish,
     boundary_condition_routines.F90:3204-3472; solver rows of SOLVER_MAPPING_CALCULATE,
     solver_mapping_routines.F90:422-520).  Returns (owner, coef, constrained) or None."""
     iron = _iron()
-    cons = getattr(bc, "_constraints", [])
+    cons = [k for k in getattr(bc, "_constraints", []) if not any(_is(k[0], o) for o in others)]
     if not cons:
         return None
     dep = es.dependentField
@@ -222,7 +230,9 @@ def _dof_constraints(bc, es, dm: DofMap, rm: RegularMesh, fixed: np.ndarray):
     master = np.full(n, -1, dtype=np.int64)
     coef = np.ones(n, dtype=np.float64)
     for (field, vt, comp, nodes, a) in cons:
-        if field is not dep and field.__dict__ is not dep.__dict__:
+        if any(_is(field, o) for o in others):
+            continue
+        if not _is(field, dep):
             raise CMFEError(1, "DOF constraints must be set on the dependent field")
         if vt != iron.FieldVariableTypes.U:
             raise CMFEError(1, "DOF constraints are implemented on the U variable")
@@ -287,7 +297,8 @@ def solve(problem, es, solver, bc):
     return solve_linear(es, solver, bc)
 
 
-def solve_linear(es, solver, bc):
+def _prepare_linear(es, solver, bc, others=()):
+    """Structures, assembly and the RHS pieces of one linear equations set."""
     iron = _iron()
     geo = es.geometricField
     rm: RegularMesh = geo._mesh()
@@ -300,8 +311,8 @@ def solve_linear(es, solver, bc):
     b: LagrangeBasis = rm.bases[mc]
     if mc != 0:
         raise CMFEError(1, "the dependent field must use the geometric mesh component")
-    fixed, u_host, neumann, _, rhs_fixed, integrated_only = _boundary_conditions(bc, es, dm, rm)
-    c = _linear_setup(es, rm, dm, fixed, _dof_constraints(bc, es, dm, rm, fixed))
+    fixed, u_host, neumann, _, rhs_fixed, integrated_only = _boundary_conditions(bc, es, dm, rm, others)
+    c = _linear_setup(es, rm, dm, fixed, _dof_constraints(bc, es, dm, rm, fixed, others))
     dev = c.csr.device
     gvar = geo._var(iron.FieldVariableTypes.U)
     coords = np.stack([geo._component(iron.FieldVariableTypes.U, k + 1) for k in range(rm.dim)], axis=1)
@@ -357,36 +368,90 @@ def solve_linear(es, solver, bc):
         rhs += c.neumann.integrate(q)
     rfix = torch.as_tensor(rhs_fixed.astype(np.uint8), device=dev) if rhs_fixed.any() else None
     u = torch.as_tensor(np.where(fixed, u_host, 0.0), device=dev)
-    bvec = c.red.rhs(v64, u, src=src, rhs=rhs, rhs_fixed=rfix)
-    # KSP settings (solver_routines.F90:10921-11157)
+    x0 = u_host[c.lead] if c.red.n_free else np.zeros(0)
+    return dict(es=es, dep=dep, c=c, v64=v64, v32=v32, src=src, rhs=rhs, rfix=rfix, u=u, x0=x0, mixed=mixed)
+
+
+def _krylov(S, red, vals64, vals32, bvec, x, ws, solver, mixed):
+    """KSP settings (solver_routines.F90:10921-11157) and the solve."""
+    iron = _iron()
     rtol = float(solver.linearIterativeRelativeTolerance)
     atol = float(solver.linearIterativeAbsoluteTolerance)
     maxit = int(solver.linearIterativeMaximumIterations)
     if int(solver.linearType) == iron.LinearSolverTypes.DIRECT:
         rtol, atol = 1e-14, 0.0  # no GPU direct factorisation: iterate to round-off
-    bn = float(torch.linalg.vector_norm(bvec).item()) if c.red.n_free else 0.0
+    bn = float(torch.linalg.vector_norm(bvec).item()) if red.n_free else 0.0
     rt = max(rtol, atol / bn) if bn > 0 else rtol
-    x = torch.as_tensor(u_host[c.lead] if c.red.n_free else np.zeros(0), device=dev).clone()
-    s64 = c.S.gather(v64, fem.F64)
     ktype = int(solver.linearIterativeType)
-    if c.red.n_free == 0:
-        info = fem.SolveInfo(0, 0.0)
-    elif mixed and ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
-        s32 = c.S.gather(v32, fem.F32)
-        info = fem.refine(c.S, s64, s32, bvec, x, rt, maxit, c.ws)
-    elif ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
-        info = fem.cg(c.S, s64, bvec, x, rt, maxit, c.ws)
-    elif ktype == iron.IterativeLinearSolverTypes.GMRES or int(solver.linearType) == iron.LinearSolverTypes.DIRECT:
-        info = fem.gmres(c.S, s64, bvec, x, rt, maxit, c.ws, restart=int(solver.linearIterativeGMRESRestart))
-    else:
-        raise CMFEError(1, f"iterative solver type {ktype} is not implemented (CG and GMRES are)")
-    solver.last_info = info
-    c.red.field_update(x, u)
-    # back-substitution: iron's DELUDELN values and the correct reactions
-    R_iron = fem.backsubstitute(c.csr, v64, u, src=src, rhs_fixed=rfix, iron_compat=True)
+    if red.n_free == 0:
+        return fem.SolveInfo(0, 0.0)
+    s64 = S.gather(vals64, fem.F64)
+    if mixed and ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
+        s32 = S.gather(vals32, fem.F32)
+        return fem.refine(S, s64, s32, bvec, x, rt, maxit, ws)
+    if ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
+        return fem.cg(S, s64, bvec, x, rt, maxit, ws)
+    if ktype == iron.IterativeLinearSolverTypes.GMRES or int(solver.linearType) == iron.LinearSolverTypes.DIRECT:
+        return fem.gmres(S, s64, bvec, x, rt, maxit, ws, restart=int(solver.linearIterativeGMRESRestart))
+    raise CMFEError(1, f"iterative solver type {ktype} is not implemented (CG and GMRES are)")
+
+
+def _finish_linear(st, info):
+    """Back-substitution (iron's DELUDELN values and the correct reactions) and the
+    field writes of one set after its field update."""
+    iron = _iron()
+    c, v64, u, src = st["c"], st["v64"], st["u"], st["src"]
+    R_iron = fem.backsubstitute(c.csr, v64, u, src=src, rhs_fixed=st["rfix"], iron_compat=True)
     R = fem.backsubstitute(c.csr, v64, u, src=src, iron_compat=False)
+    dep, es = st["dep"], st["es"]
     dep._data(iron.FieldVariableTypes.U)[:] = u.cpu().numpy()
     dep._data(iron.FieldVariableTypes.DELUDELN)[:] = R_iron.cpu().numpy()
     es.reactions = R.cpu().numpy()
     es._last = dict(vals=v64, src=src, u=u, info=info)
+
+
+def solve_linear(es, solver, bc):
+    st = _prepare_linear(es, solver, bc)
+    c = st["c"]
+    bvec = c.red.rhs(st["v64"], st["u"], src=st["src"], rhs=st["rhs"], rhs_fixed=st["rfix"])
+    x = torch.as_tensor(st["x0"], device=c.csr.device).clone()
+    info = _krylov(c.S, c.red, st["v64"], st["v32"], bvec, x, c.ws, solver, st["mixed"])
+    solver.last_info = info
+    c.red.field_update(x, st["u"])
+    _finish_linear(st, info)
+    return info
+
+
+class _BlockCache:
+    pass
+
+
+def solve_linear_sets(sets, solver, bc):
+    """Several linear equations sets in one solver equations: one block-diagonal
+    solver system (rows and columns set by set, solver_mapping_routines.F90:791-830),
+    one Krylov solve, then each set's field update and back-substitution."""
+    sts = []
+    for es in sets:
+        others = tuple(o.dependentField for o in sets if o is not es)
+        sts.append(_prepare_linear(es, solver, bc, others))
+    mixed = sts[0]["mixed"]
+    key = tuple(id(st["c"].red) for st in sts)  # the block cache holds these reductions
+    holder = sets[0]
+    bc_cache = getattr(holder, "_block_cache", None)
+    if bc_cache is None or bc_cache.key != key:
+        bc_cache = _BlockCache()
+        bc_cache.key = key
+        bc_cache.red = fem.BlockReduction([st["c"].red for st in sts])
+        bc_cache.S = fem.SolverMatrix(bc_cache.red)
+        bc_cache.ws = fem.KrylovWorkspace(bc_cache.red.n_free, sts[0]["c"].csr.device)
+        holder._block_cache = bc_cache
+    red, S = bc_cache.red, bc_cache.S
+    bvec = red.rhs([(st["v64"], st["u"], st["src"], st["rhs"], st["rfix"]) for st in sts])
+    x = torch.as_tensor(np.concatenate([st["x0"] for st in sts]), device=sts[0]["c"].csr.device).clone()
+    v32 = [st["v32"] for st in sts] if mixed else None
+    info = _krylov(S, red, [st["v64"] for st in sts], v32, bvec, x, bc_cache.ws, solver, mixed)
+    solver.last_info = info
+    red.field_update(x, [st["u"] for st in sts])
+    for st in sts:
+        _finish_linear(st, info)
     return info

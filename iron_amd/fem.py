@@ -229,6 +229,61 @@ class CoupledReduction:
         call("mife_field_update_coupled", self.csr.n, ptr(self.owner), ptr(self.coef), ptr(x), ptr(u), _stream())
 
 
+class BlockReduction:
+    """Several equations sets in one solver system.
+
+    On one rank SOLVER_MAPPING_CALCULATE numbers the solver rows set by set, each
+    set's included rows in ascending order (solver_mapping_routines.F90:791-830), and
+    the columns likewise; with no interface conditions the solver matrix is block
+    diagonal.  ``parts`` are the per-set reductions (Dirichlet or coupled); values,
+    RHS and field updates are taken per part into slices of the joint vectors.
+    """
+
+    coupled = True
+
+    def __init__(self, parts):
+        self.parts = list(parts)
+        dev = self.parts[0].csr.device
+        self.row_off = np.cumsum([0] + [p.n_free for p in self.parts]).tolist()
+        self.nnz_off = np.cumsum([0] + [p.nnz for p in self.parts]).tolist()
+        self.n_free = self.row_off[-1]
+        self.nnz = self.nnz_off[-1]
+        rps, cols = [], []
+        for p, r0, z0 in zip(self.parts, self.row_off, self.nnz_off):
+            rps.append(p.row_ptr[:-1].to(torch.int64) + z0)
+            cols.append(p.col[: p.nnz].to(torch.int64) + r0)
+        rps.append(torch.tensor([self.nnz], dtype=torch.int64, device=dev))
+        self.row_ptr = torch.cat(rps).to(torch.int32)
+        self.col = torch.cat(cols).to(torch.int32) if self.nnz else torch.zeros(1, dtype=torch.int32, device=dev)
+        self.src_pos = torch.arange(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        self.free_idx = None
+
+    def gather(self, vals, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Joint solver-CSR values from the per-set equations values ``vals`` (a list)."""
+        if out is None:
+            out = torch.empty(max(self.nnz, 1), dtype=torch.float64 if prec == F64 else torch.float32,
+                              device=vals[0].device)
+        for p, v, z0 in zip(self.parts, vals, self.nnz_off):
+            if p.nnz:
+                p.gather(v, prec, out=out[z0:z0 + p.nnz])
+        return out
+
+    def rhs(self, parts_args, out=None) -> torch.Tensor:
+        """Joint RHS: ``parts_args`` holds (vals64, u, src, rhs, rhs_fixed) per set."""
+        dev = self.parts[0].csr.device
+        if out is None:
+            out = torch.empty(max(self.n_free, 1), dtype=torch.float64, device=dev)
+        for p, a, r0 in zip(self.parts, parts_args, self.row_off):
+            if p.n_free:
+                p.rhs(a[0], a[1], src=a[2], rhs=a[3], rhs_fixed=a[4], out=out[r0:r0 + p.n_free])
+        return out
+
+    def field_update(self, x, us):
+        for p, u, r0 in zip(self.parts, us, self.row_off):
+            if p.n_free:
+                p.field_update(x[r0:r0 + p.n_free], u)
+
+
 class SolverMatrix:
     """Storage of the reduced (free x free) solver matrix.
 
@@ -247,7 +302,7 @@ class SolverMatrix:
 
     def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 1024, compress: bool = True,
                  schedule: bool = True, interleave: tuple | None = None):
-        dev = red.csr.device
+        dev = red.row_ptr.device
         self.red = red
         self.n = red.n_free
         self.fmt = fmt
@@ -326,8 +381,9 @@ class SolverMatrix:
     def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
         """Solver-matrix values (this storage) from equations-CSR values (FP64 or FP32)."""
         if getattr(self.red, "coupled", False):
-            # coupled rows and columns: solver-CSR sums first, then this storage
-            vals = self.red.gather(vals, F64 if vals.dtype == torch.float64 else F32)
+            # coupled rows and columns, or several sets: solver-CSR values first, then this storage
+            v0 = vals[0] if isinstance(vals, (list, tuple)) else vals
+            vals = self.red.gather(vals, F64 if v0.dtype == torch.float64 else F32)
             if self.fmt == "csr" and out is None and (vals.dtype == torch.float64) == (prec == F64):
                 return vals
         if out is None:

@@ -820,6 +820,11 @@ class Problem(_Obj):
         se = self._solver_equations
         if se is None or not se._sets:
             raise CMFEError(1, "no equations set was added to the solver equations")
-        if len(se._sets) != 1:
-            raise CMFEError(1, "one equations set per solver equations is supported")
-        engine.solve(self, se._sets[0], self._solver, se._bc or BoundaryConditions())
+        bc = se._bc or BoundaryConditions()
+        if len(se._sets) > 1:
+            # several sets in one solver system (no interface conditions: block diagonal)
+            if any(es._kind == "mooney_rivlin" for es in se._sets):
+                raise CMFEError(1, "several equations sets per solver equations are implemented for linear sets")
+            engine.solve_linear_sets(se._sets, self._solver, bc)
+            return
+        engine.solve(self, se._sets[0], self._solver, bc)

@@ -196,3 +196,87 @@ def test_elasticity_end_plate_tie_mixed_precision_matches_the_oracle():
     assert np.max(np.abs(u - u_ref)) < 1e-8 * np.max(np.abs(u_ref))
     assert np.all(u[zf] == u[zf[0]]) and u[zf[0]] < 0
     assert solver.last_info.iterations > 0
+
+
+def test_two_equations_sets_in_one_solver_system_match_separate_oracle_solves():
+    """A Laplace set (with periodic ties) and a linear-elasticity set on two regions,
+    added to one solver equations: one block-diagonal solver system and one CG solve;
+    each set's field matches its own oracle solve."""
+    from iron_amd import iron
+    print("two equations sets in one solver system", flush=True)
+    ne1 = (4, 3, 2)
+    r1, d1, g1, m1 = _setup(iron, ne1, iron.BasisInterpolationSpecifications.QUADRATIC_LAGRANGE, (1.0, 1.0, 1.0))
+    es1, dep1 = _equations_set(iron, r1, g1, [iron.EquationsSetClasses.CLASSICAL_FIELD,
+                                              iron.EquationsSetTypes.LAPLACE_EQUATION,
+                                              iron.EquationsSetSubtypes.STANDARD_LAPLACE])
+    _equations(iron, es1)
+    ne2, ext2 = (4, 2, 2), (2.0, 1.0, 1.0)
+    rng = np.random.default_rng(4)
+    E = 10.0 * np.exp(0.3 * rng.standard_normal(int(np.prod(ne2))))
+    r2, d2, g2, m2 = _setup(iron, ne2, iron.BasisInterpolationSpecifications.LINEAR_LAGRANGE, ext2)
+    es2, dep2 = _equations_set(iron, r2, g2, [iron.EquationsSetClasses.ELASTICITY,
+                                              iron.EquationsSetTypes.LINEAR_ELASTICITY,
+                                              iron.EquationsSetSubtypes.THREE_DIMENSIONAL])
+    mat = iron.Field()
+    es2.MaterialsCreateStart(4, mat)
+    for c in (1, 2, 3):
+        mat.ComponentInterpolationSet(iron.FieldVariableTypes.U, c, iron.FieldInterpolationTypes.ELEMENT_BASED)
+    es2.MaterialsCreateFinish()
+    for c in (1, 2, 3):
+        off = mat._offset(iron.FieldVariableTypes.U, c)
+        mat.ParameterSetDataGet(iron.FieldVariableTypes.U, 1)[off:off + len(E)] = E
+    _equations(iron, es2)
+    problem, solver, se, bc = _problem(iron, es1, [iron.ProblemClasses.CLASSICAL_FIELD,
+                                                   iron.ProblemTypes.LAPLACE_EQUATION,
+                                                   iron.ProblemSubtypes.STANDARD_LAPLACE],
+                                       {"linearIterativeType": iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT,
+                                        "linearIterativeRelativeTolerance": 1.0e-12,
+                                        "linearIterativeAbsoluteTolerance": 0.0})
+    assert se.EquationsSetAdd(es2) == 2
+    X1, u1s = g1._mesh().coords, g1._mesh().comp_user[0]
+    X2, u2s = g2._mesh().coords, g2._mesh().comp_user[0]
+    for i in np.nonzero(np.abs(X1[:, 0]) < 1e-12)[0]:
+        bc.SetNode(dep1, iron.FieldVariableTypes.U, 1, 1, int(u1s[i]), 1, iron.BoundaryConditionsTypes.FIXED, 0.0)
+    for i in np.nonzero(np.abs(X1[:, 0] - 1) < 1e-12)[0]:
+        bc.SetNode(dep1, iron.FieldVariableTypes.U, 1, 1, int(u1s[i]), 1, iron.BoundaryConditionsTypes.FIXED, 1.0)
+    ties = _face_ties(X1, lambda i: 1.0)
+    for c, m, _ in ties:
+        bc.ConstrainNodeDofsEqual(dep1, iron.FieldVariableTypes.U, 1, 1, 1, [int(u1s[m]), int(u1s[c])], 1.0)
+    clamp = np.nonzero(np.abs(X2[:, 0]) < 1e-12)[0]
+    face = np.nonzero(np.abs(X2[:, 0] - ext2[0]) < 1e-12)[0]
+    for i in clamp:
+        for c in (1, 2, 3):
+            bc.SetNode(dep2, iron.FieldVariableTypes.U, 1, 1, int(u2s[i]), c, iron.BoundaryConditionsTypes.FIXED, 0.0)
+    for i in face:
+        bc.SetNode(dep2, iron.FieldVariableTypes.DELUDELN, 1, 1, int(u2s[i]), 3,
+                   iron.BoundaryConditionsTypes.NEUMANN_POINT, -1.0)
+    se.BoundaryConditionsCreateFinish()
+    problem.Solve()
+    u1 = dep1.ParameterSetDataGet(iron.FieldVariableTypes.U, 1).copy()
+    u2 = dep2.ParameterSetDataGet(iron.FieldVariableTypes.U, 1).copy()
+    blk = es1._block_cache.red
+    assert blk.n_free == es1._device_cache.red.n_free + es2._device_cache.red.n_free
+    # set 1: periodic Laplace keeps u = x (Q2 reproduces linear fields exactly)
+    assert np.max(np.abs(u1 - X1[:, 0])) < 1e-9
+    # set 2: its own oracle solve
+    ob = fo.Basis(3, 1, 2)
+    om = fo.regular_mesh(ne2, [ob], ext2)
+    od = fo.DofMap(om, [0, 0, 0])
+    rp, col = fo.csr_pattern(od)
+    xe = om.coords[om.elem_nodes[0] - 1]
+    nu = 0.3 * np.ones_like(E)
+    C = fo.linear_elasticity_tensor(E, E, E, nu, nu, nu)
+    vals = fo.assemble_csr(rp, col, od.element_dofs(), fo.linear_elasticity_element_matrices(xe, ob, C), od.num_dofs)
+    n, N1 = od.num_dofs, len(X2)
+    fixed = np.zeros(n, bool)
+    for c in range(3):
+        fixed[c * N1 + clamp] = True
+    pv = {int(2 * N1 + i): -1.0 for i in face}
+    rhs_fixed = np.zeros(n, bool)
+    rhs_fixed[list(pv)] = True
+    f = fo.neumann_integrate(om, od, pv)
+    free, srp, scol, sv, b = fo.reduce_dirichlet(rp, col, vals, fixed, np.zeros(n), f, rhs_fixed)
+    import scipy.sparse.linalg as spla
+    x = spla.spsolve(fo._csr_scipy(srp, scol, sv).tocsc(), b)
+    assert np.max(np.abs(u2[free] - x)) < 1e-8 * np.max(np.abs(x))
+    assert np.all(u2[fixed] == 0.0)
