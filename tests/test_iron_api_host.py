@@ -57,3 +57,59 @@ def test_field_parameter_access_and_mesh_numbering():
     # node 2 is the mid-edge node at x = 0.5
     assert geo.ParameterSetGetNodeDP(iron.FieldVariableTypes.U, 1, 1, 1, 2, 1) == 0.5
     assert geo.ParameterSetGetNodeDP(iron.FieldVariableTypes.U, 1, 1, 1, 45, 3) == 1.0
+
+
+def _laplace_set(ne=(3, 2, 2)):
+    cs = iron.CoordinateSystem(); cs.CreateStart(1); cs.CreateFinish()
+    region = iron.Region(); region.CreateStart(1, iron.WorldRegion); region.CreateFinish()
+    basis = iron.Basis(); basis.CreateStart(1)
+    basis.interpolationXi = [iron.BasisInterpolationSpecifications.LINEAR_LAGRANGE] * 3
+    basis.CreateFinish()
+    gm = iron.GeneratedMesh(); gm.CreateStart(1, region)
+    gm.basis = [basis]; gm.extent = [1.0, 1.0, 1.0]; gm.numberOfElements = list(ne)
+    mesh = iron.Mesh(); gm.CreateFinish(1, mesh)
+    dec = iron.Decomposition(); dec.CreateStart(1, mesh); dec.CreateFinish()
+    geo = iron.Field(); geo.CreateStart(1, region); geo.meshDecomposition = dec; geo.CreateFinish()
+    gm.GeometricParametersCalculate(geo)
+    es = iron.EquationsSet()
+    es.CreateStart(1, region, geo, [iron.EquationsSetClasses.CLASSICAL_FIELD, iron.EquationsSetTypes.LAPLACE_EQUATION,
+                                    iron.EquationsSetSubtypes.STANDARD_LAPLACE], 2, iron.Field())
+    es.CreateFinish()
+    dep = iron.Field()
+    es.DependentCreateStart(3, dep)
+    es.DependentCreateFinish()
+    return es, dep, geo
+
+
+def test_dof_constraint_mapping_and_the_reference_error_rules():
+    """ConstrainNodeDofsEqual -> solver DOF and coefficient per DOF, and iron's checks
+    (DofConstraintSet / DofConstraintsCreateFinish, boundary_condition_routines.F90:3204-3472)."""
+    from iron_amd import engine
+    es, dep, geo = _laplace_set()
+    rm = geo._mesh()
+    dm = engine._dof_map(dep, rm)
+    n = dm.num_dofs
+    U = iron.FieldVariableTypes.U
+    fixed = np.zeros(n, bool)
+    fixed[0] = True  # node 1
+    bc = iron.BoundaryConditions()
+    bc.ConstrainNodeDofsEqual(dep, U, 1, 1, 1, [2, 5, 9], 0.5)
+    owner, coef, constrained = engine._dof_constraints(bc, es, dm, rm, fixed)
+    assert constrained.sum() == 2 and constrained[4] and constrained[8]
+    assert owner[0] == -1 and owner[4] == owner[1] == owner[8] == 0 and owner[2] == 1
+    assert coef[4] == coef[8] == 0.5 and coef[1] == 1.0
+    assert owner.max() == n - 1 - 2 - 1
+    with pytest.raises(iron.CMFEError):  # the DOF depended on is fixed
+        b2 = iron.BoundaryConditions(); b2.ConstrainNodeDofsEqual(dep, U, 1, 1, 1, [1, 3], 1.0)
+        engine._dof_constraints(b2, es, dm, rm, fixed)
+    with pytest.raises(iron.CMFEError):  # a constrained DOF that is also fixed
+        b3 = iron.BoundaryConditions(); b3.ConstrainNodeDofsEqual(dep, U, 1, 1, 1, [3, 1], 1.0)
+        engine._dof_constraints(b3, es, dm, rm, fixed)
+    with pytest.raises(iron.CMFEError):  # constrained twice
+        b4 = iron.BoundaryConditions()
+        b4.ConstrainNodeDofsEqual(dep, U, 1, 1, 1, [2, 3], 1.0)
+        b4.ConstrainNodeDofsEqual(dep, U, 1, 1, 1, [4, 3], 1.0)
+        engine._dof_constraints(b4, es, dm, rm, fixed)
+    with pytest.raises(iron.CMFEError):  # fewer than two nodes
+        iron.BoundaryConditions().ConstrainNodeDofsEqual(dep, U, 1, 1, 1, [2], 1.0)
+    assert engine._dof_constraints(iron.BoundaryConditions(), es, dm, rm, fixed) is None
