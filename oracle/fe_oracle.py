@@ -19,6 +19,10 @@ Parity pinning (what checks each part; see tests/test_oracle_*.py):
   * assembly + Dirichlet lifting + solve + back-substitution -> the reference's
     golden AnalyticLaplaceLinearLagrange.opanal (solution and the iron-compat
     "del Phi/del n" values) and the closed form u = x of the unit-cube problem.
+  * constraint_couplings / reduce_coupled / field_update_coupled -> equal to
+    reduce_dirichlet without constraints, to a dense P^T K P, to the dense KKT
+    (Lagrange multiplier) solve of the constrained problem, and to the closed form
+    u = x with periodic faces; iron's constraint error rules.
   * Mooney-Rivlin residual/Jacobian -> zero residual at the reference state,
     central finite differences of the residual, and the homogeneous uniaxial
     stretch closed form.
