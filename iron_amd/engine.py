@@ -13,12 +13,13 @@ DOFs stay the same.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 
 import numpy as np
 import torch
 
 from . import fem
-from ._lib import CMFEError
+from ._lib import CMFEError, NotConvergedError
 from .basis import LagrangeBasis, face_quadrature
 from .mesh import DofMap, RegularMesh, structured_colours
 
@@ -385,6 +386,17 @@ def _krylov(S, red, vals64, vals32, bvec, x, ws, solver, mixed):
     ktype = int(solver.linearIterativeType)
     if red.n_free == 0:
         return fem.SolveInfo(0, 0.0)
+    try:
+        return _krylov_run(S, vals64, vals32, bvec, x, ws, solver, mixed, ktype, rt, maxit)
+    except NotConvergedError as e:
+        # iron flags a warning and keeps the iterate (solver_routines.F90:11875-11880)
+        warnings.warn(f"Linear iterative solver did not converge. PETSc diverged its. ({e})", RuntimeWarning,
+                      stacklevel=3)
+        return fem.SolveInfo(maxit, float("nan"))
+
+
+def _krylov_run(S, vals64, vals32, bvec, x, ws, solver, mixed, ktype, rt, maxit):
+    iron = _iron()
     s64 = S.gather(vals64, fem.F64)
     if mixed and ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
         s32 = S.gather(vals32, fem.F32)

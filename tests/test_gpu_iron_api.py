@@ -325,3 +325,39 @@ def test_linear_elasticity_cantilever_traction():
     assert np.all(u[fixed] == 0.0)
     # tip deflects in -z
     assert u[2 * N1 + face].mean() < 0
+
+
+def test_krylov_iteration_limit_warns_and_keeps_the_iterate():
+    """A KSP that stops at its iteration limit is a warning in iron, not an error
+    (PETSC_KSP_DIVERGED_ITS -> FLAG_WARNING, solver_routines.F90:11875-11880): the
+    field still takes the last iterate."""
+    from iron_amd import iron
+    print("KSP iteration limit through the API", flush=True)
+    ne = (6, 5, 4)
+    region, dec, geo, mesh = _setup(iron, ne, iron.BasisInterpolationSpecifications.LINEAR_LAGRANGE, (1.0, 1.0, 1.0))
+    es, dep = _equations_set(iron, region, geo, [iron.EquationsSetClasses.CLASSICAL_FIELD,
+                                                 iron.EquationsSetTypes.LAPLACE_EQUATION,
+                                                 iron.EquationsSetSubtypes.STANDARD_LAPLACE])
+    _equations(iron, es)
+    problem, solver, se, bc = _problem(iron, es, [iron.ProblemClasses.CLASSICAL_FIELD,
+                                                  iron.ProblemTypes.LAPLACE_EQUATION,
+                                                  iron.ProblemSubtypes.STANDARD_LAPLACE],
+                                       {"linearIterativeType": iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT,
+                                        "linearIterativeRelativeTolerance": 1.0e-12,
+                                        "linearIterativeAbsoluteTolerance": 0.0,
+                                        "linearIterativeMaximumIterations": 2})
+    rm = geo._mesh()
+    X = rm.coords
+    users = rm.comp_user[0]
+    for i in np.nonzero(np.abs(X[:, 0]) < 1e-12)[0]:
+        bc.SetNode(dep, iron.FieldVariableTypes.U, 1, 1, int(users[i]), 1, iron.BoundaryConditionsTypes.FIXED, 0.0)
+    for i in np.nonzero(np.abs(X[:, 0] - 1) < 1e-12)[0]:
+        bc.SetNode(dep, iron.FieldVariableTypes.U, 1, 1, int(users[i]), 1, iron.BoundaryConditionsTypes.FIXED, 1.0)
+    se.BoundaryConditionsCreateFinish()
+    with pytest.warns(RuntimeWarning, match="did not converge"):
+        problem.Solve()
+    u = dep.ParameterSetDataGet(iron.FieldVariableTypes.U, 1).copy()
+    interior = (X[:, 0] > 1e-12) & (X[:, 0] < 1 - 1e-12)
+    assert np.all(np.isfinite(u)) and np.any(u[interior] != 0.0)
+    assert np.max(np.abs(u[interior] - X[interior, 0])) > 1e-6  # two iterations are not the solution
+    assert solver.last_info.iterations == 2
