@@ -398,7 +398,11 @@ int mife_backsubstitute(int n, const int32_t* row_ptr, const int32_t* col,
 
 /* ----------------------------------------------------- Krylov solvers */
 
-/* Workspace bytes for mife_cg / mife_refine / mife_gmres on n rows. */
+/* Workspace bytes for mife_cg / mife_refine / mife_minres, and for mife_gmres
+ * with a restart length up to `restart`, on n rows.  Computed from the same
+ * layout the solvers take their buffers from; every solver is also given the
+ * size of its workspace (work_bytes) and returns MIFE_ERR_ARG, before any
+ * launch, when its layout does not fit. */
 size_t mife_solver_work_bytes(int n, int restart);
 
 /* Jacobi-preconditioned CG in the precision of `prec` (x, b and vals alike),
@@ -409,7 +413,8 @@ size_t mife_solver_work_bytes(int n, int restart);
  * host outputs (the call synchronises the stream).  Returns
  * MIFE_ERR_NOT_CONVERGED when maxit is reached. */
 int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int prec,
-            double rtol, int maxit, void* work, int* iters, double* relres, void* stream);
+            double rtol, int maxit, void* work, size_t work_bytes, int* iters, double* relres,
+            void* stream);
 
 /* Mixed-precision solve: Jacobi CG on the FP32 operator vals32 whose residual is
  * replaced by the FP64 residual r = b - A64 x each time the FP32 recurrence
@@ -422,7 +427,8 @@ int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int 
  * (Petsc_KSPSolve, src/solver_routines.F90:11869). */
 int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
                 const double* b, double* x, double rtol, double inner_rtol, int maxit,
-                void* work, int* iters, int* outer, double* relres, void* stream);
+                void* work, size_t work_bytes, int* iters, int* outer, double* relres,
+                void* stream);
 
 /* Right-preconditioned restarted GMRES(restart) in FP64 (KSPGMRES default
  * restart 30, src/solver_routines.F90:11320-11369), restart <= 256 (workspace
@@ -433,7 +439,7 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
  * displacement/pressure Jacobian (PCJACOBI would use 1 there). */
 int mife_gmres(const mife_matrix* A, const double* vals, const double* b, double* x,
                const double* pc_inv, double rtol, int restart, int maxit, void* work,
-               int* iters, double* relres, void* stream);
+               size_t work_bytes, int* iters, double* relres, void* stream);
 
 /* Preconditioned MINRES (Paige-Saunders) for symmetric, possibly indefinite A,
  * such as the Newton Jacobian of the Mooney-Rivlin deviatoric formulation.
@@ -446,8 +452,8 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b, double
  * (iron's Newton solves go through KSP, src/solver_routines.F90:18674-18763).
  * Workspace: mife_solver_work_bytes(n, 1). */
 int mife_minres(const mife_matrix* A, const void* vals, int prec, const double* b, double* x,
-                const double* pc_inv, double rtol, int maxit, void* work, int* iters,
-                double* relres, void* stream);
+                const double* pc_inv, double rtol, int maxit, void* work, size_t work_bytes,
+                int* iters, double* relres, void* stream);
 
 /* -------------------------------------------- grouped solves (serving) */
 

@@ -87,6 +87,7 @@ _P = C.c_void_p
 _I = C.c_int
 _I64 = C.c_int64
 _D = C.c_double
+_SZ = C.c_size_t
 _SIG = {
     "mife_version": (C.c_char_p, []),
     "mife_last_error": (_I, [C.c_char_p, _I]),
@@ -115,11 +116,13 @@ _SIG = {
     "mife_field_update_coupled": (_I, [_I, _P, _P, _P, _P, _P]),
     "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "mife_solver_work_bytes": (C.c_size_t, [_I, _I]),
-    "mife_cg": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
-    "mife_refine": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _D, _I, _P, C.POINTER(_I), C.POINTER(_I),
+    "mife_cg": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_refine": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _D, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_I),
                          C.POINTER(_D), _P]),
-    "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
-    "mife_minres": (_I, [C.POINTER(MatrixDesc), _P, _I, _P, _P, _P, _D, _I, _P, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_D),
+                        _P]),
+    "mife_minres": (_I, [C.POINTER(MatrixDesc), _P, _I, _P, _P, _P, _D, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_D),
+                         _P]),
     "mife_assemble_batch_work_bytes": (C.c_size_t, [_I, _I]),
     "mife_assemble_batch": (_I, [C.POINTER(ElementDesc), C.POINTER(ElementDesc), _I, C.POINTER(AssemblyJob), _I, _I,
                                  _P, _P, _P]),

@@ -29,6 +29,7 @@ constexpr int kRedBlocks = kMaxSellGrid;  // partial slots per reduced value (la
 constexpr int kDotBlocks = kMaxGrid;     // GMRES multidot partial slots (vector grid)
 constexpr int kMaxRestart = 256;
 constexpr int kFinThreads = 1024;  // one-block finishers: two partials per thread
+constexpr int kMaxIterChunk = 256;  // CG iterations launched between two host reads of the state
 
 struct CgState {
   double rz, pq, alpha, beta, rr, bb, tol2, scale;
@@ -700,18 +701,45 @@ __global__ void pc_override_kernel(int n, const int32_t* __restrict__ perm, cons
   }
 }
 
+// Zeroes a solver state (words of 8 bytes) with a kernel rather than a memset
+// node, so that the clear is an ordinary kernel on the caller's stream.
+__global__ void zero_words_kernel(uint64_t* __restrict__ p, size_t n_words) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_words; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
 // ------------------------------------------------------------- host side
+// Bump allocator over the caller's workspace.  A layout is taken in full before
+// any launch; with base == NULL it only counts, which is how
+// mife_solver_work_bytes sizes the workspace from the same layout code.
 struct Work {
   char* base;
   size_t off;
+  size_t cap;
   template <typename T>
   T* take(size_t count) {
-    off = (off + 255) & ~size_t(255);
-    T* p = reinterpret_cast<T*>(base + off);
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+    off = ((b + off + 255) & ~uintptr_t(255)) - b;
+    T* p = reinterpret_cast<T*>(b + off);
     off += count * sizeof(T);
     return p;
   }
+  bool fits() const { return off <= cap; }
 };
+
+static int check_fit(const Work& w, const char* who) {
+  MIFE_REQUIRE(w.fits(), MIFE_ERR_ARG, "%s needs a workspace of %zu bytes, %zu given (mife_solver_work_bytes)", who,
+               w.off, w.cap);
+  return MIFE_OK;
+}
+
+template <typename T>
+static int zero_state(T* st, hipStream_t s) {
+  static_assert(sizeof(T) % 8 == 0, "state size is a multiple of 8 bytes");
+  zero_words_kernel<<<64, 256, 0, s>>>(reinterpret_cast<uint64_t*>(st), sizeof(T) / 8);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
 
 static int vec_grid(int n) { return std::max(1, std::min(div_up(n, 256), kMaxGrid)); }
 
@@ -797,7 +825,8 @@ static int read_state(const CgState* st, HostState* h, hipStream_t s) {
 template <typename T>
 static int run_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
                      hipStream_t s, HostState* hs, int count) {
-  std::vector<int> tickets(2 * count, -1);
+  count = std::min(count, kMaxIterChunk);
+  int tickets[2 * kMaxIterChunk];
   const int it0 = hs->iters;
   const bool on = prof::enabled();
   for (int k = 0; k < count; ++k) {
@@ -844,21 +873,122 @@ static int run_to_convergence(const Mat& M, const T* vals, T* x, T* r, T* z, T* 
   return MIFE_OK;
 }
 
+// Workspace layouts.  Every buffer of a solve is taken here, before the first
+// launch; mife_solver_work_bytes runs the same functions on a counting Work.
 template <typename T>
-static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double rtol, int maxit, Work& w,
-                   hipStream_t s, int* iters, double* relres, bool* converged, const double* vals64_check) {
+struct CgBufs {
+  CgState *st, *st2;
+  T *b, *x, *r, *z, *p, *q, *dinv;
+};
+
+template <typename T>
+static void cg_layout(Work& w, int n, CgBufs<T>& B) {
+  B.st = w.take<CgState>(1);
+  B.st2 = w.take<CgState>(1);
+  B.b = w.take<T>(n);
+  B.x = w.take<T>(n);
+  B.r = w.take<T>(n);
+  B.z = w.take<T>(n);
+  B.p = w.take<T>(n);
+  B.q = w.take<T>(n);
+  B.dinv = w.take<T>(n);
+}
+
+struct RefineBufs {
+  CgState *st, *sto;
+  double *r64, *b, *x;
+  float *x32, *r, *z, *p, *q, *dinv;
+};
+
+static void refine_layout(Work& w, int n, RefineBufs& B) {
+  B.st = w.take<CgState>(1);
+  B.sto = w.take<CgState>(1);
+  B.r64 = w.take<double>(n);
+  B.b = w.take<double>(n);
+  B.x = w.take<double>(n);
+  B.x32 = w.take<float>(n);
+  B.r = w.take<float>(n);
+  B.z = w.take<float>(n);
+  B.p = w.take<float>(n);
+  B.q = w.take<float>(n);
+  B.dinv = w.take<float>(n);
+}
+
+struct MinresBufs {
+  MinresState* st;
+  CgState* rs;
+  double *dinv, *b, *x, *z, *v, *t, *r1, *r2, *wv, *w1, *w2;
+};
+
+static void minres_layout(Work& w, int n, MinresBufs& B) {
+  B.st = w.take<MinresState>(1);
+  B.rs = w.take<CgState>(1);
+  B.dinv = w.take<double>(n);
+  B.b = w.take<double>(n);
+  B.x = w.take<double>(n);
+  B.z = w.take<double>(n);
+  B.v = w.take<double>(n);
+  B.t = w.take<double>(n);
+  B.r1 = w.take<double>(n);
+  B.r2 = w.take<double>(n);
+  B.wv = w.take<double>(n);
+  B.w1 = w.take<double>(n);
+  B.w2 = w.take<double>(n);
+}
+
+struct GmresBufs {
+  CgState* st;
+  double *dinv, *r, *t, *wv, *b, *x, *V, *parts, *hdev, *ydev, *ndev, *Hdev, *csdev, *sndev, *gdev;
+  GmresCycle* cy;
+  size_t ldv;
+};
+
+static void gmres_layout(Work& w, int n, int m, GmresBufs& B) {
+  B.st = w.take<CgState>(1);
+  B.dinv = w.take<double>(n);
+  B.r = w.take<double>(n);
+  B.t = w.take<double>(n);
+  B.wv = w.take<double>(n);
+  B.b = w.take<double>(n);
+  B.x = w.take<double>(n);
+  B.ldv = ((size_t)n + 63) & ~size_t(63);
+  B.V = w.take<double>(B.ldv * (m + 1));
+  B.parts = w.take<double>((size_t)kDotBlocks * (m + 1));
+  B.hdev = w.take<double>(m + 2);
+  B.ydev = w.take<double>(m + 2);
+  B.ndev = w.take<double>(2);
+  B.Hdev = w.take<double>((size_t)(m + 1) * m);
+  B.csdev = w.take<double>(m + 2);
+  B.sndev = w.take<double>(m + 2);
+  B.gdev = w.take<double>(m + 2);
+  B.cy = w.take<GmresCycle>(1);
+}
+
+// Bytes a solve of n rows takes from a workspace at any base address (the
+// layout is 256-byte aligned from the base upwards, so 255 bytes of slack).
+template <typename F>
+static size_t layout_bytes(F&& f) {
+  Work w{nullptr, 0, 0};
+  f(w);
+  return w.off + 255;
+}
+
+template <typename T>
+static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double rtol, int maxit,
+                   const CgBufs<T>& B, hipStream_t s, int* iters, double* relres, bool* converged,
+                   const double* vals64_check) {
   const int n = M.a.n;
   const int vg = vec_grid(n);
-  CgState* st = w.take<CgState>(1);
-  T *b = w.take<T>(n), *x = w.take<T>(n), *r = w.take<T>(n), *z = w.take<T>(n), *p = w.take<T>(n),
-    *q = w.take<T>(n), *dinv = w.take<T>(n);
+  CgState* st = B.st;
+  T *b = B.b, *x = B.x, *r = B.r, *z = B.z, *p = B.p, *q = B.q, *dinv = B.dinv;
   permute_in_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
-  MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
+  int rc = zero_state(st, s);
+  if (rc) return rc;
   diag_inv_kernel<T><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
   launch_cg_init<T>(M, vals, b, x, dinv, r, z, p, q, st, rtol * rtol, maxit, s);
   HostState hs{};
-  int rc = read_state(st, &hs, s);
+  rc = read_state(st, &hs, s);
   if (rc) return rc;
   if (!hs.done) rc = run_to_convergence<T>(M, vals, x, r, z, p, q, dinv, st, s, &hs);
   if (rc) return rc;
@@ -866,9 +996,10 @@ static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double r
   if constexpr (sizeof(T) == 8) if (vals64_check) {
     // explicit FP64 residual at exit; when rounding has let the recurrence drift
     // below the true residual, restart CG from the true residual (a few times)
-    CgState* st2 = w.take<CgState>(1);
+    CgState* st2 = B.st2;
     for (int restart = 0;; ++restart) {
-      MIFE_CHECK_HIP(hipMemsetAsync(st2, 0, sizeof(CgState), s));
+      rc = zero_state(st2, s);
+      if (rc) return rc;
       launch_residual64(M, vals64_check, (const double*)b, (const double*)x, (double*)r, st2, s);
       HostState h2{};
       rc = read_state(st2, &h2, s);
@@ -891,28 +1022,10 @@ static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double r
   return MIFE_OK;
 }
 
-}  // namespace mife
-
-using namespace mife;
-
-extern "C" {
-
-size_t mife_solver_work_bytes(int n, int restart) {
-  const size_t nn = (size_t)std::max(n, 1);
-  const size_t m = (size_t)std::max(restart, 1) + 1;
-  size_t bytes = 0;
-  bytes += 3 * (sizeof(CgState) + 256);
-  bytes += (8 * nn + 256) * (10 + m);            // FP64 vectors incl. GMRES basis
-  bytes += (4 * nn + 256) * 8;                   // FP32 vectors
-  bytes += (8 * (size_t)kDotBlocks * (size_t)std::max(m, (size_t)2) + 256);  // GMRES multidot partials
-  bytes += 8 * (3 * m + 8) + 768;                // GMRES coefficient vectors
-  bytes += 8 * (m + 1) * m + 8 * 3 * (m + 2) + 1024;  // GMRES Hessenberg matrix, rotations, cycle state
-  bytes += 8 * 64 * 4 + 65536;
-  return bytes;
-}
-
-int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol, int maxit,
-            void* work, int* iters, double* relres, void* stream) {
+// mife_cg on a workspace of `cap` bytes at `work` (the caller's buffer, or the
+// library's own scratch array for the no-allocation entry point).
+static int cg_entry(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol,
+                    int maxit, void* work, size_t cap, int* iters, double* relres, void* stream) {
   MIFE_REQUIRE(vals && b && x && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
   MIFE_REQUIRE(maxit >= 0 && rtol >= 0.0, MIFE_ERR_ARG, "invalid CG arguments");
@@ -920,16 +1033,23 @@ int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int 
   int rc = make_mat(A, &M);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  Work w{(char*)work, 0};
+  Work w{(char*)work, 0, cap};
   int it = 0;
   double rel = 0.0;
   bool conv = false;
-  if (prec == MIFE_F64)
-    rc = cg_impl<double>(M, (const double*)vals, (const double*)b, (double*)x, rtol, maxit, w, s, &it, &rel, &conv,
+  if (prec == MIFE_F64) {
+    CgBufs<double> B;
+    cg_layout(w, M.a.n, B);
+    if ((rc = check_fit(w, "CG"))) return rc;
+    rc = cg_impl<double>(M, (const double*)vals, (const double*)b, (double*)x, rtol, maxit, B, s, &it, &rel, &conv,
                          (const double*)vals);
-  else
-    rc = cg_impl<float>(M, (const float*)vals, (const float*)b, (float*)x, rtol, maxit, w, s, &it, &rel, &conv,
+  } else {
+    CgBufs<float> B;
+    cg_layout(w, M.a.n, B);
+    if ((rc = check_fit(w, "CG"))) return rc;
+    rc = cg_impl<float>(M, (const float*)vals, (const float*)b, (float*)x, rtol, maxit, B, s, &it, &rel, &conv,
                         nullptr);
+  }
   if (rc) return rc;
   if (iters) *iters = it;
   if (relres) *relres = rel;
@@ -940,9 +1060,32 @@ int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int 
   return MIFE_OK;
 }
 
+}  // namespace mife
+
+using namespace mife;
+
+extern "C" {
+
+size_t mife_solver_work_bytes(int n, int restart) {
+  const int nn = std::max(n, 1);
+  const int m = std::min(std::max(restart, 1), kMaxRestart);
+  size_t bytes = 0;
+  bytes = std::max(bytes, layout_bytes([&](Work& w) { CgBufs<double> B; cg_layout(w, nn, B); }));
+  bytes = std::max(bytes, layout_bytes([&](Work& w) { CgBufs<float> B; cg_layout(w, nn, B); }));
+  bytes = std::max(bytes, layout_bytes([&](Work& w) { RefineBufs B; refine_layout(w, nn, B); }));
+  bytes = std::max(bytes, layout_bytes([&](Work& w) { MinresBufs B; minres_layout(w, nn, B); }));
+  bytes = std::max(bytes, layout_bytes([&](Work& w) { GmresBufs B; gmres_layout(w, nn, m, B); }));
+  return bytes;
+}
+
+int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol, int maxit,
+            void* work, size_t work_bytes, int* iters, double* relres, void* stream) {
+  return cg_entry(A, vals, b, x, prec, rtol, maxit, work, work_bytes, iters, relres, stream);
+}
+
 int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32, const double* b_in, double* x_io,
-                double rtol, double inner_rtol, int maxit, void* work, int* iters, int* outer, double* relres,
-                void* stream) {
+                double rtol, double inner_rtol, int maxit, void* work, size_t work_bytes, int* iters, int* outer,
+                double* relres, void* stream) {
   MIFE_REQUIRE(vals64 && vals32 && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0 && maxit > 0, MIFE_ERR_ARG, "invalid arguments");
   Mat M;
@@ -950,17 +1093,18 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const int n = M.a.n;
-  Work w{(char*)work, 0};
-  CgState* st = w.take<CgState>(1);
-  CgState* sto = w.take<CgState>(1);
-  double *r64 = w.take<double>(n), *b = w.take<double>(n), *x = w.take<double>(n);
-  float *x32 = w.take<float>(n), *r = w.take<float>(n), *z = w.take<float>(n), *p = w.take<float>(n),
-        *q = w.take<float>(n), *dinv = w.take<float>(n);
+  Work w{(char*)work, 0, work_bytes};
+  RefineBufs B;
+  refine_layout(w, n, B);
+  if ((rc = check_fit(w, "mixed-precision CG"))) return rc;
+  CgState *st = B.st, *sto = B.sto;
+  double *r64 = B.r64, *b = B.b, *x = B.x;
+  float *x32 = B.x32, *r = B.r, *z = B.z, *p = B.p, *q = B.q, *dinv = B.dinv;
   const int vg = vec_grid(n);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
-  MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
-  MIFE_CHECK_HIP(hipMemsetAsync(sto, 0, sizeof(CgState), s));
+  if ((rc = zero_state(st, s))) return rc;
+  if ((rc = zero_state(sto, s))) return rc;
   diag_inv_kernel<float><<<div_up(n, 256), 256, 0, s>>>(M.a, vals32, dinv);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, x32, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, p, 0.0f);
@@ -996,7 +1140,7 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
     // a quarter of the target when that is nearer
     if (nrep > 1) {
       const double target = std::max(delta2 * hs.rr, 0.25 * tol2 * hs.bb);
-      seg = predict_iters(hs.rr, target, log_rate, 4, 256);
+      seg = predict_iters(hs.rr, target, log_rate, 4, kMaxIterChunk);
     }
     seg = std::min(seg, maxit - hs.iters);
     const int it0 = hs.iters;
@@ -1028,26 +1172,20 @@ namespace {
 // an iteration, shrinks from 8 to 4 bytes per value).
 template <typename TV>
 int minres_impl(const mife_matrix* A, const TV* vals, const double* b_in, double* x_io, const double* pc_inv,
-                double rtol, int maxit, void* work, int* iters, double* relres, void* stream) {
+                double rtol, int maxit, void* work, size_t work_bytes, int* iters, double* relres, void* stream) {
   Mat M;
   int rc = make_mat(A, &M);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const int n = M.a.n;
-  Work w{(char*)work, 0};
-  MinresState* st = w.take<MinresState>(1);
-  CgState* rs = w.take<CgState>(1);
-  double* dinv = w.take<double>(n);
-  double* b = w.take<double>(n);
-  double* x = w.take<double>(n);
-  double* z = w.take<double>(n);
-  double* v = w.take<double>(n);
-  double* t = w.take<double>(n);
-  double* r1 = w.take<double>(n);
-  double* r2 = w.take<double>(n);
-  double* wv = w.take<double>(n);
-  double* w1 = w.take<double>(n);
-  double* w2 = w.take<double>(n);
+  Work w{(char*)work, 0, work_bytes};
+  MinresBufs B;
+  minres_layout(w, n, B);
+  if ((rc = check_fit(w, "MINRES"))) return rc;
+  MinresState* st = B.st;
+  CgState* rs = B.rs;
+  double *dinv = B.dinv, *b = B.b, *x = B.x, *z = B.z, *v = B.v, *t = B.t, *r1 = B.r1, *r2 = B.r2, *wv = B.wv,
+         *w1 = B.w1, *w2 = B.w2;
   const int vg = vec_grid(n);
   const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
@@ -1082,7 +1220,7 @@ int minres_impl(const mife_matrix* A, const TV* vals, const double* b_in, double
     }
   }
   // true residual at exit
-  MIFE_CHECK_HIP(hipMemsetAsync(rs, 0, sizeof(CgState), s));
+  if ((rc = zero_state(rs, s))) return rc;
   launch_residual64<TV>(M, vals, b, x, nullptr, rs, s);
   HostState hs{};
   rc = read_state(rs, &hs, s);
@@ -1104,17 +1242,21 @@ int minres_impl(const mife_matrix* A, const TV* vals, const double* b_in, double
 extern "C" {
 
 int mife_minres(const mife_matrix* A, const void* vals, int prec, const double* b_in, double* x_io,
-                const double* pc_inv, double rtol, int maxit, void* work, int* iters, double* relres, void* stream) {
+                const double* pc_inv, double rtol, int maxit, void* work, size_t work_bytes, int* iters,
+                double* relres, void* stream) {
   MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(rtol > 0.0 && maxit >= 0, MIFE_ERR_ARG, "invalid MINRES arguments");
   MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
   if (prec == MIFE_F32)
-    return minres_impl<float>(A, (const float*)vals, b_in, x_io, pc_inv, rtol, maxit, work, iters, relres, stream);
-  return minres_impl<double>(A, (const double*)vals, b_in, x_io, pc_inv, rtol, maxit, work, iters, relres, stream);
+    return minres_impl<float>(A, (const float*)vals, b_in, x_io, pc_inv, rtol, maxit, work, work_bytes, iters,
+                              relres, stream);
+  return minres_impl<double>(A, (const double*)vals, b_in, x_io, pc_inv, rtol, maxit, work, work_bytes, iters,
+                             relres, stream);
 }
 
 int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, double* x_io, const double* pc_inv,
-               double rtol, int restart, int maxit, void* work, int* iters, double* relres, void* stream) {
+               double rtol, int restart, int maxit, void* work, size_t work_bytes, int* iters, double* relres,
+               void* stream) {
   MIFE_REQUIRE(vals && b_in && x_io && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(restart >= 1 && restart <= kMaxRestart && rtol > 0.0, MIFE_ERR_ARG, "invalid GMRES arguments");
   Mat M;
@@ -1123,31 +1265,22 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   hipStream_t s = (hipStream_t)stream;
   const int n = M.a.n;
   const int m = restart;
-  Work w{(char*)work, 0};
-  CgState* st = w.take<CgState>(1);
-  double* dinv = w.take<double>(n);
-  double* r = w.take<double>(n);
-  double* t = w.take<double>(n);
-  double* wv = w.take<double>(n);
-  double* b = w.take<double>(n);
-  double* x = w.take<double>(n);
-  const size_t ldv = ((size_t)n + 63) & ~size_t(63);
-  double* V = w.take<double>(ldv * (m + 1));
-  double* parts = w.take<double>((size_t)kDotBlocks * (m + 1));
-  double* hdev = w.take<double>(m + 2);
-  double* ydev = w.take<double>(m + 2);
-  double* ndev = w.take<double>(2);
-  double* Hdev = w.take<double>((size_t)(m + 1) * m);
-  double* csdev = w.take<double>(m + 2);
-  double* sndev = w.take<double>(m + 2);
-  double* gdev = w.take<double>(m + 2);
-  GmresCycle* cy = w.take<GmresCycle>(1);
+  Work w{(char*)work, 0, work_bytes};
+  GmresBufs B;
+  gmres_layout(w, n, m, B);
+  if ((rc = check_fit(w, "GMRES"))) return rc;
+  CgState* st = B.st;
+  double *dinv = B.dinv, *r = B.r, *t = B.t, *wv = B.wv, *b = B.b, *x = B.x, *V = B.V, *parts = B.parts;
+  double *hdev = B.hdev, *ydev = B.ydev, *ndev = B.ndev, *Hdev = B.Hdev, *csdev = B.csdev, *sndev = B.sndev,
+         *gdev = B.gdev;
+  GmresCycle* cy = B.cy;
+  const size_t ldv = B.ldv;
   const int vg = vec_grid(n);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
   if (pc_inv) pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 0);
-  MIFE_CHECK_HIP(hipMemsetAsync(st, 0, sizeof(CgState), s));
+  if ((rc = zero_state(st, s))) return rc;
   launch_residual64(M, vals, b, x, r, st, s);
   HostState hs{};
   rc = read_state(st, &hs, s);

@@ -42,7 +42,8 @@ def boundary_faces(rm: RegularMesh):
     return out
 
 
-def neumann_matrix(rm: RegularMesh, dm: DofMap, comp_basis, point_dofs, geom_comp=0, integrated_only=frozenset()):
+def neumann_matrix(rm: RegularMesh, dm: DofMap, comp_basis, point_dofs, geom_comp=0, integrated_only=frozenset(),
+                   coords=None):
     """Neumann integration matrix N of BoundaryConditions_NeumannMatricesInitialise
     (boundary_condition_routines.F90:2227-2583): rows are the RHS variable's DOFs,
     columns the NEUMANN_POINT DOFs in the order of ``point_dofs``, and
@@ -50,13 +51,15 @@ def neumann_matrix(rm: RegularMesh, dm: DofMap, comp_basis, point_dofs, geom_com
     (3-D node-based case of BoundaryConditions_NeumannIntegrate, :2654-2996, for the
     face nodes m of o's component).  A face holding a NEUMANN_INTEGRATED_ONLY DOF of
     that component is skipped (:2860-2862).  Geometry is interpolated with the basis of
-    mesh component ``geom_comp``.  Returns CSR (row_ptr [n+1], col, vals) with sorted,
-    unique columns per row; f = N q is the integrated set (FIELD_INTEGRATED_NEUMANN_SET)."""
+    mesh component ``geom_comp`` from the node coordinates ``coords`` ([nodes of that
+    component, dim]; the geometric field's values, default the generated mesh's).
+    Returns CSR (row_ptr [n+1], col, vals) with sorted, unique columns per row; f = N q
+    is the integrated set (FIELD_INTEGRATED_NEUMANN_SET)."""
     n = dm.num_dofs
     point_dofs = np.asarray(point_dofs, dtype=np.int64)
     col_of = {int(d): j for j, d in enumerate(point_dofs)}
     rows, cols, vals = [], [], []
-    coords = rm.coords
+    coords = rm.coords if coords is None else np.asarray(coords, dtype=np.float64)
     comps = sorted(set(int(c) for c in dm.component_of(point_dofs))) if len(point_dofs) else []
     for c in comps:
         mc = dm.comp_mesh[c]
@@ -102,13 +105,13 @@ def neumann_matrix(rm: RegularMesh, dm: DofMap, comp_basis, point_dofs, geom_com
 
 
 def neumann_integrate(rm: RegularMesh, dm: DofMap, comp_basis, point_values: dict, geom_comp=0,
-                      integrated_only=frozenset()):
+                      integrated_only=frozenset(), coords=None):
     """Integrated Neumann values f = N q on the host (see neumann_matrix);
     ``point_values``: {dof: q}; ``integrated_only``: set of DOFs."""
     if not point_values:
         return np.zeros(dm.num_dofs)
     pd = np.array(sorted(point_values), dtype=np.int64)
-    rp, col, v = neumann_matrix(rm, dm, comp_basis, pd, geom_comp, integrated_only)
+    rp, col, v = neumann_matrix(rm, dm, comp_basis, pd, geom_comp, integrated_only, coords)
     q = np.array([point_values[int(d)] for d in pd])
     f = np.zeros(dm.num_dofs)
     rows = np.repeat(np.arange(dm.num_dofs), np.diff(rp))
@@ -122,10 +125,10 @@ class NeumannIntegrator:
     the point values q resident in HBM."""
 
     def __init__(self, rm: RegularMesh, dm: DofMap, comp_basis, point_dofs, device, geom_comp=0,
-                 integrated_only=frozenset()):
+                 integrated_only=frozenset(), coords=None):
         from . import _lib
         self.point_dofs = np.asarray(point_dofs, dtype=np.int64)
-        rp, col, v = neumann_matrix(rm, dm, comp_basis, self.point_dofs, geom_comp, integrated_only)
+        rp, col, v = neumann_matrix(rm, dm, comp_basis, self.point_dofs, geom_comp, integrated_only, coords)
         self.n = dm.num_dofs
         self.nnz = len(col)
         self.row_ptr = torch.as_tensor(rp.astype(np.int32), device=device)
@@ -148,6 +151,23 @@ class NeumannIntegrator:
 
 
 # ------------------------------------------------------------ helpers
+def geometry_coords(geo, rm: RegularMesh, mesh_comp: int) -> np.ndarray:
+    """Node coordinates [nodes, dim] of the geometric field, whose components must be
+    node based on mesh component ``mesh_comp`` (0-based): the values the element
+    kernels interpolate geometry from (iron interpolates the geometric field,
+    FIELD_INTERPOLATE_XI on the GEOMETRIC_FIELD's parameters, field_routines.F90:8042-8260)."""
+    iron = _iron()
+    var = geo._var(iron.FieldVariableTypes.U)
+    if var.ncomp < rm.dim:
+        raise CMFEError(1, f"the geometric field needs {rm.dim} components")
+    for k in range(rm.dim):
+        if var.mesh_comp[k] - 1 != mesh_comp or var.interp[k] != iron.FieldInterpolationTypes.NODE_BASED:
+            raise CMFEError(1, f"geometric field component {k + 1} must be node based on mesh component "
+                               f"{mesh_comp + 1}")
+    return np.ascontiguousarray(np.stack([geo._component(iron.FieldVariableTypes.U, k + 1) for k in range(rm.dim)],
+                                         axis=1), dtype=np.float64)
+
+
 def _dof_map(field, rm: RegularMesh):
     iron = _iron()
     v = field._var(iron.FieldVariableTypes.U)
@@ -315,8 +335,7 @@ def _prepare_linear(es, solver, bc, others=()):
     fixed, u_host, neumann, _, rhs_fixed, integrated_only = _boundary_conditions(bc, es, dm, rm, others)
     c = _linear_setup(es, rm, dm, fixed, _dof_constraints(bc, es, dm, rm, fixed, others))
     dev = c.csr.device
-    gvar = geo._var(iron.FieldVariableTypes.U)
-    coords = np.stack([geo._component(iron.FieldVariableTypes.U, k + 1) for k in range(rm.dim)], axis=1)
+    coords = geometry_coords(geo, rm, 0)
     kind = es._kind
     mixed = str(getattr(solver, "linearIterativePrecision", "fp64")).lower() == "mixed"
     v64 = c.csr.zeros(fem.F64)
@@ -361,9 +380,10 @@ def _prepare_linear(es, solver, bc, others=()):
     if neumann:
         # integration matrix built once per set of point DOFs, f = N q on the device
         pd = np.array(sorted(neumann), dtype=np.int64)
-        nkey = (pd.tobytes(), tuple(sorted(integrated_only)))
+        nkey = (pd.tobytes(), tuple(sorted(integrated_only)), coords.tobytes())
         if getattr(c, "neumann_key", None) != nkey:
-            c.neumann = NeumannIntegrator(rm, dm, [b] * len(dm.comp_mesh), pd, dev, integrated_only=integrated_only)
+            c.neumann = NeumannIntegrator(rm, dm, [b] * len(dm.comp_mesh), pd, dev, integrated_only=integrated_only,
+                                          coords=coords)
             c.neumann_key = nkey
         q = torch.as_tensor(np.array([neumann[int(d)] for d in pd], dtype=np.float64), device=dev)
         rhs += c.neumann.integrate(q)
@@ -389,10 +409,14 @@ def _krylov(S, red, vals64, vals32, bvec, x, ws, solver, mixed):
     try:
         return _krylov_run(S, vals64, vals32, bvec, x, ws, solver, mixed, ktype, rt, maxit)
     except NotConvergedError as e:
-        # iron flags a warning and keeps the iterate (solver_routines.F90:11875-11880)
-        warnings.warn(f"Linear iterative solver did not converge. PETSc diverged its. ({e})", RuntimeWarning,
-                      stacklevel=3)
-        return fem.SolveInfo(maxit, float("nan"))
+        # iron flags a warning and keeps the iterate (solver_routines.F90:11875-11880);
+        # the library's message names the actual stop (iteration limit, or a true
+        # residual above the tolerance after the recurrence's restarts)
+        info = getattr(e, "info", None) or fem.SolveInfo(maxit, float("nan"))
+        what = "direct (iterated to round-off)" if int(solver.linearType) == iron.LinearSolverTypes.DIRECT \
+            else "iterative"
+        warnings.warn(f"Linear {what} solver did not converge: {e}", RuntimeWarning, stacklevel=3)
+        return info
 
 
 def _krylov_run(S, vals64, vals32, bvec, x, ws, solver, mixed, ktype, rt, maxit):

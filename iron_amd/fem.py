@@ -407,35 +407,59 @@ class SolveInfo:
 
 
 class KrylovWorkspace:
+    """Device workspace of the Krylov solvers on n rows; GMRES restarts up to
+    ``restart`` fit (``ensure`` grows it for a longer restart)."""
+
     def __init__(self, n: int, device, restart: int = 30):
         self.n = n
-        self.restart = restart
-        self.buf = torch.empty(int(_lib.lib().mife_solver_work_bytes(n, restart)), dtype=torch.uint8, device=device)
+        self.device = device
+        self.restart = 0
+        self.buf = None
+        self.ensure(restart)
+
+    def ensure(self, restart: int):
+        restart = max(int(restart), 1)
+        if self.buf is None or restart > self.restart:
+            self.restart = restart
+            self.nbytes = int(_lib.lib().mife_solver_work_bytes(self.n, restart))
+            self.buf = None
+            self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=self.device)
+        return self
+
+
+def _call_solver(name, *args, it, rr, outer=None):
+    try:
+        call(name, *args)
+    except _lib.NotConvergedError as e:
+        e.info = SolveInfo(it.value, rr.value, outer.value if outer is not None else 0)
+        raise
+    return SolveInfo(it.value, rr.value, outer.value if outer is not None else 0)
 
 
 def cg(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, prec=F64) -> SolveInfo:
-    """Jacobi CG on A (b, x in the original free-DOF numbering)."""
+    """Jacobi CG on A (b, x in the original free-DOF numbering).  On reaching maxit
+    raises NotConvergedError with ``.info`` (iterations, relative residual)."""
     it, rr = C.c_int(0), C.c_double(0.0)
-    call("mife_cg", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), prec, float(rtol), int(maxit), ptr(ws.buf),
-         C.byref(it), C.byref(rr), _stream())
-    return SolveInfo(it.value, rr.value)
+    return _call_solver("mife_cg", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), prec, float(rtol), int(maxit),
+                        ptr(ws.buf), ws.nbytes, C.byref(it), C.byref(rr), _stream(), it=it, rr=rr)
 
 
 def refine(A: SolverMatrix, vals64, vals32, b, x, rtol, maxit, ws: KrylovWorkspace, inner_rtol=0.1) -> SolveInfo:
     """FP32 Jacobi CG with FP64 residual replacement every ``inner_rtol`` reduction."""
     it, outer, rr = C.c_int(0), C.c_int(0), C.c_double(0.0)
-    call("mife_refine", C.byref(A.desc), ptr(vals64), ptr(vals32), ptr(b), ptr(x), float(rtol), float(inner_rtol),
-         int(maxit), ptr(ws.buf), C.byref(it), C.byref(outer), C.byref(rr), _stream())
-    return SolveInfo(it.value, rr.value, outer.value)
+    return _call_solver("mife_refine", C.byref(A.desc), ptr(vals64), ptr(vals32), ptr(b), ptr(x), float(rtol),
+                        float(inner_rtol), int(maxit), ptr(ws.buf), ws.nbytes, C.byref(it), C.byref(outer),
+                        C.byref(rr), _stream(), it=it, rr=rr, outer=outer)
 
 
 def gmres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, restart=30, pc_inv=None) -> SolveInfo:
     """Restarted GMRES; pc_inv (FP64, solver-row order) overrides the Jacobi entry
-    of rows where it is nonzero."""
+    of rows where it is nonzero.  The workspace grows to fit ``restart``."""
+    ws.ensure(restart)
     it, rr = C.c_int(0), C.c_double(0.0)
-    call("mife_gmres", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), ptr(pc_inv), float(rtol), int(restart),
-         int(maxit), ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
-    return SolveInfo(it.value, rr.value)
+    return _call_solver("mife_gmres", C.byref(A.desc), ptr(vals), ptr(b), ptr(x), ptr(pc_inv), float(rtol),
+                        int(restart), int(maxit), ptr(ws.buf), ws.nbytes, C.byref(it), C.byref(rr), _stream(),
+                        it=it, rr=rr)
 
 
 def minres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, pc_inv=None) -> SolveInfo:
@@ -444,9 +468,8 @@ def minres(A: SolverMatrix, vals, b, x, rtol, maxit, ws: KrylovWorkspace, pc_inv
     residual norm, SolveInfo.relres is the true relative residual of that operator."""
     it, rr = C.c_int(0), C.c_double(0.0)
     prec = F32 if vals.dtype == torch.float32 else F64
-    call("mife_minres", C.byref(A.desc), ptr(vals), prec, ptr(b), ptr(x), ptr(pc_inv), float(rtol), int(maxit),
-         ptr(ws.buf), C.byref(it), C.byref(rr), _stream())
-    return SolveInfo(it.value, rr.value)
+    return _call_solver("mife_minres", C.byref(A.desc), ptr(vals), prec, ptr(b), ptr(x), ptr(pc_inv), float(rtol),
+                        int(maxit), ptr(ws.buf), ws.nbytes, C.byref(it), C.byref(rr), _stream(), it=it, rr=rr)
 
 
 def backsubstitute(csr: CSRStructure, vals64, u, src=None, rhs_fixed=None, iron_compat=False, out=None):

@@ -42,9 +42,12 @@ class MRSystem:
     Displacement (and geometry) on mesh component 0 (``rm.bases[0]``); pressure
     either node based on mesh component ``p_mesh_comp`` or element based
     (``p_mesh_comp=None``).  DOF layout (separated order): x over the
-    displacement nodes, then y, then z, then the pressure DOFs."""
+    displacement nodes, then y, then z, then the pressure DOFs.  ``coords`` are the
+    reference node coordinates on mesh component 0 (the geometric field's values;
+    default the generated mesh's)."""
 
-    def __init__(self, rm: RegularMesh, p_mesh_comp: int | None, formulation: int = DEVIATORIC, device=None):
+    def __init__(self, rm: RegularMesh, p_mesh_comp: int | None, formulation: int = DEVIATORIC, device=None,
+                 coords: np.ndarray | None = None):
         if rm.dim != 3:
             raise CMFEError(1, "finite elasticity is implemented for 3-D meshes")
         dev = fem._dev(device)
@@ -78,7 +81,10 @@ class MRSystem:
         self.Np = None
         if self.bp is not None:
             self.Np = torch.as_tensor(np.ascontiguousarray(self.bp.N), dtype=torch.float64, device=dev)
-        self.coords = torch.as_tensor(np.ascontiguousarray(rm.coords, dtype=np.float64), device=dev)
+        self.X = np.ascontiguousarray(rm.coords if coords is None else coords, dtype=np.float64)
+        if self.X.shape != (self.nu, 3):
+            raise CMFEError(1, "reference coordinates need one row of 3 values per displacement node")
+        self.coords = torch.as_tensor(self.X, device=dev)
         self.elem_nodes = torch.as_tensor(np.ascontiguousarray(en_u, dtype=np.int32), device=dev)
         self.desc = MrDesc(C.pointer(self.db.desc), self.n_pressure, ptr(self.Np), self.formulation)
 
@@ -112,7 +118,7 @@ class MRSystem:
         """Lumped sum_e (1/mu_e) int psi_q dV over the pressure DOFs ([ndofs], zero on
         displacement DOFs): the diagonal Schur-complement estimate."""
         bu = self.bu
-        X = self.rm.coords[self.rm.elem_nodes[0]]                  # [E, nn, 3]
+        X = self.X[self.rm.elem_nodes[0]]                          # [E, nn, 3]
         J = np.einsum("ead,gaj->egdj", X, bu.dN)
         wj = np.abs(np.linalg.det(J)) * bu.weights[None, :]         # [E, ng]
         psi = np.ones((bu.num_gauss, 1)) if self.bp is None else self.bp.N
@@ -390,7 +396,7 @@ class MRNewton:
                 li = fem.minres(self.S, vals, b, dx, eta, o.linear_maxit, self.ws, pc_inv=self.pc_pos)
             else:
                 li = fem.gmres(self.S, vals, b, dx, eta, o.linear_maxit, self.ws,
-                               restart=min(o.gmres_restart, self.restart), pc_inv=self.pc_neg)
+                               restart=o.gmres_restart, pc_inv=self.pc_neg)
             info.linear_iterations += li.iterations
             lam, xt, stalled = self._line_search(x, dx, F, fn, body_force, o)
             if stalled:
@@ -472,10 +478,14 @@ def solve(problem, es, solver, bc):
         raise CMFEError(1, "the pressure must be element based or node based")
     sub = es.specification[2]
     form = DEVIATORIC if sub == iron.EquationsSetSubtypes.MOONEY_RIVLIN else CAUCHY
+    # conditions on any other field are an error, as on the linear path
+    if any(not engine._is(entry[0], dep) for entry in bc._entries):
+        raise CMFEError(1, "boundary conditions must be set on the dependent field")
+    coords = engine.geometry_coords(es.geometricField, rm, 0)
     cache = getattr(es, "_mr_cache", None)
-    key = (id(rm), p_mc, form)
+    key = (id(rm), p_mc, form, coords.tobytes())
     if cache is None or cache[0] != key:
-        sys = MRSystem(rm, p_mc, form)
+        sys = MRSystem(rm, p_mc, form, coords=coords)
         cache = (key, sys, {})
         es._mr_cache = cache
     sys = cache[1]

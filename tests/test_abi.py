@@ -43,7 +43,7 @@ def test_argument_errors_without_gpu():
     # null pointers
     assert L.mife_spmv(None, None, None, None, 0, None) == -1
     assert "null" in _lib.last_error()
-    assert L.mife_cg(None, None, None, None, 0, 1e-8, 10, None, None, None, None) == -1
+    assert L.mife_cg(None, None, None, None, 0, 1e-8, 10, None, 0, None, None, None) == -1
     assert L.mife_sell_order(4, None, None, None, None) == -1
     assert L.mife_grouped_setup(None, 0, 1, 1, None, None, None, None) == -1
     pools = _lib.Pools(1, 1, 1, 1, 1, None)   # the FP32 value pool is required
@@ -79,7 +79,29 @@ def test_argument_errors_without_gpu():
     d = M(10, 0, 1, 1, None, 0, 10, None, None, 0)
     assert L.mife_spmv(ctypes.byref(d), 1, 1, 1, 5, None) == 2     # unknown precision 5
     assert "precision" in _lib.last_error()
-    assert L.mife_cg(ctypes.byref(d), 1, 1, 1, 0, -1.0, 10, 1, None, None, None) == 2   # negative rtol
+    assert L.mife_cg(ctypes.byref(d), 1, 1, 1, 0, -1.0, 10, 1, 1 << 30, None, None, None) == 2   # negative rtol
+    # a workspace smaller than the solver's layout is refused before any launch,
+    # for every solver and every GMRES restart up to 256
+    n = 1000
+    d = M(n, 0, 1, 1, None, 0, 10, None, None, 0)
+    w = L.mife_solver_work_bytes(n, 30)
+    assert L.mife_cg(ctypes.byref(d), 1, 1, 1, 0, 1e-8, 10, 1, 1024, None, None, None) == 2
+    assert "workspace" in _lib.last_error()
+    assert L.mife_refine(ctypes.byref(d), 1, 1, 1, 1, 1e-8, 0.1, 10, 1, 1024, None, None, None, None) == 2
+    assert "workspace" in _lib.last_error()
+    assert L.mife_minres(ctypes.byref(d), 1, 0, 1, 1, None, 1e-8, 10, 1, 1024, None, None, None) == 2
+    assert "workspace" in _lib.last_error()
+    for R in (31, 38, 39, 64, 100, 255, 256):
+        assert L.mife_gmres(ctypes.byref(d), 1, 1, 1, None, 1e-8, R, 10, 1, w, None, None, None) == 2
+        assert "GMRES needs a workspace" in _lib.last_error()
+    for R in (30, 100, 256):
+        wr = L.mife_solver_work_bytes(n, R)
+        assert wr > L.mife_solver_work_bytes(n, R - 1)    # the GMRES basis sets the size here
+        # one byte short of the layout at an aligned base is refused (the size is the
+        # layout's plus 255 bytes of alignment slack)
+        assert L.mife_gmres(ctypes.byref(d), 1, 1, 1, None, 1e-8, R, 10, 256, wr - 256, None, None, None) == 2
+        assert "GMRES needs a workspace" in _lib.last_error()
+    assert L.mife_gmres(ctypes.byref(d), 1, 1, 1, None, 1e-8, 257, 10, 1, 1 << 40, None, None, None) == 2
     # 8 ranges of ceil(ceil(n_slices / 8) / 4) * 4 entries
     assert L.mife_sell_order_len(0) == 0 and L.mife_sell_order_len(1) == 32
     assert L.mife_sell_order_len(13397) == 8 * 1676   # ceil(13397 / 8) = 1675 -> 1676

@@ -113,3 +113,54 @@ def test_dof_constraint_mapping_and_the_reference_error_rules():
     with pytest.raises(iron.CMFEError):  # fewer than two nodes
         iron.BoundaryConditions().ConstrainNodeDofsEqual(dep, U, 1, 1, 1, [2], 1.0)
     assert engine._dof_constraints(iron.BoundaryConditions(), es, dm, rm, fixed) is None
+
+
+def _mr_set():
+    cs = iron.CoordinateSystem(); cs.CreateStart(1); cs.CreateFinish()
+    region = iron.Region(); region.CreateStart(1, iron.WorldRegion); region.CreateFinish()
+    basis = iron.Basis(); basis.CreateStart(1)
+    basis.interpolationXi = [iron.BasisInterpolationSpecifications.LINEAR_LAGRANGE] * 3
+    basis.CreateFinish()
+    gm = iron.GeneratedMesh(); gm.CreateStart(1, region)
+    gm.basis = [basis]; gm.extent = [1.0, 1.0, 1.0]; gm.numberOfElements = [2, 1, 1]
+    mesh = iron.Mesh(); gm.CreateFinish(1, mesh)
+    dec = iron.Decomposition(); dec.CreateStart(1, mesh); dec.CreateFinish()
+    geo = iron.Field(); geo.CreateStart(1, region); geo.meshDecomposition = dec; geo.CreateFinish()
+    gm.GeometricParametersCalculate(geo)
+    es = iron.EquationsSet()
+    es.CreateStart(1, region, geo, [iron.EquationsSetClasses.ELASTICITY, iron.EquationsSetTypes.FINITE_ELASTICITY,
+                                    iron.EquationsSetSubtypes.MOONEY_RIVLIN], 2, iron.Field())
+    es.CreateFinish()
+    dep = iron.Field()
+    es.DependentCreateStart(3, dep)
+    dep.ComponentInterpolationSet(iron.FieldVariableTypes.U, 4, iron.FieldInterpolationTypes.ELEMENT_BASED)
+    es.DependentCreateFinish()
+    return es, dep, geo
+
+
+def test_finite_elasticity_rejects_conditions_on_another_field():
+    """A condition set on the geometric field is an error (as on the linear path),
+    raised before any device work."""
+    from iron_amd import finite_elasticity
+    es, dep, geo = _mr_set()
+    bc = iron.BoundaryConditions()
+    bc.SetNode(geo, iron.FieldVariableTypes.U, 1, 1, 1, 1, iron.BoundaryConditionsTypes.FIXED, 0.0)
+    with pytest.raises(iron.CMFEError, match="dependent field"):
+        finite_elasticity.solve(None, es, None, bc)
+
+
+def test_geometry_is_read_from_the_geometric_field():
+    """Element kernels, Neumann integration and finite elasticity take node
+    coordinates from the geometric field's values: moved nodes are seen, and a
+    geometric field not node based on the first mesh component is refused."""
+    from iron_amd import engine
+    es, dep, geo = _laplace_set()
+    rm = geo._mesh()
+    X = engine.geometry_coords(geo, rm, 0)
+    assert np.array_equal(X, rm.coords)
+    geo.ParameterSetUpdateNodeDP(iron.FieldVariableTypes.U, 1, 1, 1, 6, 2, 0.4)
+    X2 = engine.geometry_coords(geo, rm, 0)
+    k = int(rm.local_node(6, 0))
+    assert X2[k, 1] == 0.4 and np.array_equal(np.delete(X2, k, 0), np.delete(rm.coords, k, 0))
+    with pytest.raises(iron.CMFEError):
+        engine.geometry_coords(geo, rm, 1)

@@ -143,3 +143,35 @@ def test_deviatoric_differs_from_goldens():
     nodes = prob.mesh.comp_nodes[0]
     gz = np.array([g["nodes"][str(n)]["dependent"] for n in nodes])
     assert np.abs(z - gz).max() > 0.1
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("order,elem_p,geometry", [(1, True, "distorted"), (2, False, "affine")])
+def test_homogeneous_state_in_equilibrium_on_non_box_geometry(form, order, elem_p, geometry):
+    """On reference geometry that is not a box (interior nodes moved at random on a
+    trilinear mesh; a general affine map of a triquadratic one, so that the Gauss rule
+    stays exact), any homogeneous isochoric deformation x = F X with constant p has a
+    constant stress: the residual vanishes on every displacement DOF off the boundary
+    and on every pressure DOF (J = 1), whatever p is."""
+    rng = np.random.default_rng(11)
+    ne, extent = (3, 2, 2), (1.5, 1.0, 1.2)
+    prob = _problem(ne, order, elem_p, form, extent=extent)
+    X0 = prob.geo.copy()
+    on_b = np.any((np.abs(X0) < 1e-12) | (np.abs(X0 - np.asarray(extent)) < 1e-12), axis=1)
+    if geometry == "distorted":
+        h = np.asarray(extent) / np.asarray(ne)
+        prob.geo = X0 + np.where(on_b[:, None], 0.0, 0.2 * h * rng.uniform(-1, 1, X0.shape))
+        assert np.abs(prob.geo - X0).max() > 0.05
+    else:
+        prob.geo = X0 @ np.array([[1.0, 0.3, -0.2], [0.1, 0.9, 0.25], [-0.15, 0.2, 1.1]]).T
+    F = np.array([[1.1, 0.2, 0.0], [-0.1, 0.95, 0.15], [0.05, 0.0, 1.0]])
+    F /= np.cbrt(np.linalg.det(F))
+    x = prob.initial_state(-1.7)
+    z = prob.geo @ F.T
+    x[:3 * prob.nu] = np.concatenate([z[:, c] for c in range(3)])
+    R, _ = prob.residual(x)
+    Ru = R[:3 * prob.nu].reshape(3, -1)
+    scale = np.abs(R).max()
+    assert scale > 1e-3                                  # boundary rows carry the traction
+    assert np.abs(Ru[:, ~on_b]).max() < 1e-11 * scale
+    assert np.abs(R[3 * prob.nu:]).max() < 1e-11 * scale
