@@ -117,6 +117,7 @@ _SIG = {
     "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "mife_solver_work_bytes": (C.c_size_t, [_I, _I]),
     "mife_cg": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_cg_noalloc": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_refine": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _D, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_I),
                          C.POINTER(_D), _P]),
     "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_D),
@@ -148,6 +149,11 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is not built; run `python -m iron_amd.build` (hipcc, gfx950)")
+        # torch brings its own HIP runtime with the same soname (libamdhip64.so.7): load
+        # it first, so that the library binds to that one runtime.  Loaded the other way
+        # round the process holds two runtimes, and device memory from one is foreign to
+        # kernels launched through the other.
+        import torch  # noqa: F401
         h = C.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIG.items():
             f = getattr(h, name)

@@ -105,3 +105,14 @@ def test_argument_errors_without_gpu():
     # 8 ranges of ceil(ceil(n_slices / 8) / 4) * 4 entries
     assert L.mife_sell_order_len(0) == 0 and L.mife_sell_order_len(1) == 32
     assert L.mife_sell_order_len(13397) == 8 * 1676   # ceil(13397 / 8) = 1675 -> 1676
+
+
+def test_one_hip_runtime_in_the_process():
+    """torch bundles a HIP runtime with the soname the library links against; the
+    binding loads torch's first so that the process holds one runtime (two runtimes
+    would make torch's device memory foreign to the library's kernels)."""
+    from iron_amd import _lib
+    _lib.lib()
+    with open("/proc/self/maps") as f:
+        paths = {line.split()[-1] for line in f if "libamdhip64" in line}
+    assert len(paths) == 1, paths
