@@ -20,7 +20,7 @@ def test_header_declares_entry_points():
 
 def test_library_exports_every_declared_symbol():
     from iron_amd import _lib
-    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    lib = _lib.lib()
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
     # the Python binding covers the same set
