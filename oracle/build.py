@@ -1,4 +1,4 @@
-"""Compile the C restatement (fe_oracle.c) into oracle/liboracle_fe.so.
+"""Compile the C restatements (fe_oracle.c, mr_oracle.c) into oracle/liboracle_fe.so.
 
 Test infrastructure: used only by tests/ and bench.py's cpu_baseline leg.
 Built for x86-64-v3 so the library runs on any AVX2 host it is shipped to.
@@ -13,7 +13,7 @@ SO = HERE / "liboracle_fe.so"
 
 def build():
     cmd = ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-shared", "-fPIC", str(HERE / "fe_oracle.c"),
-           "-o", str(SO), "-lm"]
+           str(HERE / "mr_oracle.c"), "-o", str(SO), "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(r.stderr)

@@ -271,3 +271,141 @@ int oracle_cg(int n, const int32_t* rp, const int32_t* col, const double* vals, 
   free(r); free(z); free(p); free(q); free(dinv);
   return it;
 }
+
+/* Element matrix of 3-D linear elasticity, element e (LINEAR_ELASTICITY_FINITE_ELEMENT_CALCULATE,
+ * linear_elasticity_routines.F90:1109-1403): DPHIDX(ns,mi) = sum_ni DXI_DX(mi,ni) dphi_ns/dxi_ni
+ * (:1245-1252, as written), JRWG times the tensor of LINEAR_ELASTICITY_TENSOR (:1410-1451) on the
+ * diagonal blocks [C11,C66,C44 | C66,C22,C55 | C44,C55,C33] and the off-diagonal pairs
+ * (1,2): [C12,C66], (1,3): [C13,C44], (2,3): [C23,C55], upper triangle mirrored (:1373-1380).
+ * mat[e] = (E1, E2, E3, v13, v23, v12); Ke[3nn][3nn] component-major. */
+static void element_elastic3(int e, int nn, int ng, const double* dN, const double* w, const double* coords,
+                             const int32_t* enodes, const double* mat, double* Ke, double* xe, double* dphidx) {
+  const int n = 3 * nn;
+  memset(Ke, 0, sizeof(double) * n * n);
+  for (int a = 0; a < nn; ++a)
+    for (int d = 0; d < 3; ++d) xe[a * 3 + d] = coords[(int64_t)enodes[(int64_t)e * nn + a] * 3 + d];
+  const double* m = mat + (int64_t)e * 6;
+  const double E1 = m[0], E2 = m[1], E3 = m[2], v13 = m[3], v23 = m[4], v12 = m[5];
+  const double v31 = v13, v32 = v23, v21 = v12;
+  const double gama = 1.0 / (1.0 - v12 * v21 - v23 * v32 - v31 * v13 - 2.0 * v21 * v32 * v13);
+  const double C11 = E1 * (1.0 - v23 * v32) * gama, C22 = E2 * (1.0 - v13 * v31) * gama;
+  const double C33 = E3 * (1.0 - v12 * v21) * gama, C12 = E1 * (v21 + v31 * v23) * gama;
+  const double C13 = E1 * (v31 + v21 * v32) * gama, C23 = E2 * (v32 + v12 * v31) * gama;
+  const double C44 = E2 / (2.0 * (1.0 + v23)), C55 = E1 / (2.0 * (1.0 + v13)), C66 = E3 / (2.0 * (1.0 + v12));
+  for (int g = 0; g < ng; ++g) {
+    double J[3][3] = {{0}};
+    for (int a = 0; a < nn; ++a)
+      for (int j = 0; j < 3; ++j) {
+        const double dn = dN[((int64_t)g * nn + a) * 3 + j];
+        for (int d = 0; d < 3; ++d) J[d][j] += xe[a * 3 + d] * dn;
+      }
+    double GL[3][3], GU[3][3], JI[3][3];
+    for (int mi = 0; mi < 3; ++mi)
+      for (int ni = 0; ni < 3; ++ni) GL[mi][ni] = J[0][mi] * J[0][ni] + J[1][mi] * J[1][ni] + J[2][mi] * J[2][ni];
+    const double jrwg = sqrt(fabs(inv3(GL, GU))) * w[g];
+    inv3(J, JI); /* DXI_DX(i,j) = dxi_i/dx_j */
+    for (int s = 0; s < nn; ++s)
+      for (int mi = 0; mi < 3; ++mi) {
+        double v = 0.0;
+        for (int ni = 0; ni < 3; ++ni) v += JI[mi][ni] * dN[((int64_t)g * nn + s) * 3 + ni];
+        dphidx[s * 3 + mi] = v;
+      }
+    const double dg[3][3] = {{C11, C66, C44}, {C66, C22, C55}, {C44, C55, C33}};
+    for (int xi = 0; xi < 3; ++xi)
+      for (int s = 0; s < nn; ++s)
+        for (int t = 0; t < nn; ++t) {
+          double v = 0.0;
+          for (int d = 0; d < 3; ++d) v += dphidx[s * 3 + d] * dphidx[t * 3 + d] * dg[xi][d];
+          Ke[(xi * nn + s) * n + xi * nn + t] += v * jrwg;
+        }
+    const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    const double ca[3] = {C12, C13, C23}, cb[3] = {C66, C44, C55};
+    for (int o = 0; o < 3; ++o) {
+      const int r = pr[o][0], c = pr[o][1];
+      for (int s = 0; s < nn; ++s)
+        for (int t = 0; t < nn; ++t) {
+          const double v = (dphidx[s * 3 + r] * dphidx[t * 3 + c] * ca[o] +
+                            dphidx[s * 3 + c] * dphidx[t * 3 + r] * cb[o]) * jrwg;
+          Ke[(r * nn + s) * n + c * nn + t] += v;
+          Ke[(c * nn + t) * n + r * nn + s] += v;
+        }
+    }
+  }
+}
+
+/* Linear-elasticity assembly (3 components on one basis): element loop in element order,
+ * each entry located by bisection.  edofs [E][3nn]. */
+int oracle_assemble_elastic3(int E, int nn, int ng, const double* dN, const double* w, const double* coords,
+                             const int32_t* enodes, const int32_t* edofs, const double* mat, const int32_t* rp,
+                             const int32_t* col, double* vals, int e_begin, int e_end) {
+  const int n = 3 * nn;
+  double* Ke = (double*)malloc(sizeof(double) * n * n);
+  double* xe = (double*)malloc(sizeof(double) * nn * 3);
+  double* dphidx = (double*)malloc(sizeof(double) * nn * 3);
+  int bad = 0;
+  for (int e = e_begin; e < e_end && e < E; ++e) {
+    element_elastic3(e, nn, ng, dN, w, coords, enodes, mat, Ke, xe, dphidx);
+    for (int ms = 0; ms < n; ++ms) {
+      const int row = edofs[(int64_t)e * n + ms];
+      for (int ns = 0; ns < n; ++ns) {
+        const int64_t p = location_find(rp, col, row, edofs[(int64_t)e * n + ns]);
+        if (p < 0) { bad = 1; continue; }
+        vals[p] += Ke[ms * n + ns];
+      }
+    }
+  }
+  free(Ke); free(xe); free(dphidx);
+  return bad ? -1 : 0;
+}
+
+/* As oracle_assemble_elastic3 on `threads` OpenMP threads (atomic adds). */
+int oracle_assemble_elastic3_par(int E, int nn, int ng, const double* dN, const double* w, const double* coords,
+                                 const int32_t* enodes, const int32_t* edofs, const double* mat, const int32_t* rp,
+                                 const int32_t* col, double* vals, int e_begin, int e_end, int threads) {
+  const int n = 3 * nn;
+  const int e_stop = e_end < E ? e_end : E;
+  int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads) reduction(| : bad)
+#endif
+  {
+    double* Ke = (double*)malloc(sizeof(double) * n * n);
+    double* xe = (double*)malloc(sizeof(double) * nn * 3);
+    double* dphidx = (double*)malloc(sizeof(double) * nn * 3);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int e = e_begin; e < e_stop; ++e) {
+      element_elastic3(e, nn, ng, dN, w, coords, enodes, mat, Ke, xe, dphidx);
+      for (int ms = 0; ms < n; ++ms) {
+        const int row = edofs[(int64_t)e * n + ms];
+        for (int ns = 0; ns < n; ++ns) {
+          const int64_t p = location_find(rp, col, row, edofs[(int64_t)e * n + ns]);
+          if (p < 0) { bad = 1; continue; }
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+          vals[p] += Ke[ms * n + ns];
+        }
+      }
+    }
+    free(Ke); free(xe); free(dphidx);
+  }
+  (void)threads;
+  return bad ? -1 : 0;
+}
+
+/* EQUATIONS_SET_BACKSUBSTITUTE, CSR branch (equations_set_routines.F90:1640-1912), the
+ * correct form: R_i = sum_j K_ij u_j - src_i over every equations row. */
+void oracle_backsubstitute(int n, const int32_t* rp, const int32_t* col, const double* vals, const double* u,
+                           const double* src, double* R, int threads) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
+#endif
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; ++k) s += vals[k] * u[col[k]];
+    R[i] = s - (src ? src[i] : 0.0);
+  }
+  (void)threads;
+}
