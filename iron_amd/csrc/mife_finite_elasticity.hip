@@ -388,7 +388,9 @@ static int launch_mr(const mife_mr_desc* d, int n_elem, const double* coords, co
                                                               order, cptr[c], cnt, residual, jac, rhs, false);
     }
   }
-  prof::end(tk, s, prof::kFiniteElasticity);
+  // residual + Jacobian launches and residual-only launches (line searches,
+  // increment starts) are timed under separate keys
+  prof::end(tk, s, jac ? prof::kFiniteElasticity : prof::kFiniteElasticityResidual);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }

@@ -13,7 +13,8 @@ static const char* kNames[kNumKernels] = {"cg_spmv",          "cg_update",      
                                           "gather_values",    "finite_elasticity", "grouped_spmv",
                                           "grouped_update",   "cg_spmv_f32",       "cg_update_f32",
                                           "assemble_scalar_f32", "assemble_elastic_f32",
-                                          "minres_spmv",      "grouped_replace",   "assemble_batch"};
+                                          "minres_spmv",      "grouped_replace",   "assemble_batch",
+                                          "finite_elasticity_residual"};
 
 struct Interval {
   hipEvent_t a, b;

@@ -23,6 +23,7 @@ enum Kernel : int {
   kMinresSpmv,
   kGroupedReplace,
   kAssembleBatch,
+  kFiniteElasticityResidual,
   kNumKernels
 };
 
