@@ -318,8 +318,9 @@ def solve(problem, es, solver, bc):
     return solve_linear(es, solver, bc)
 
 
-def _prepare_linear(es, solver, bc, others=()):
-    """Structures, assembly and the RHS pieces of one linear equations set."""
+def _prepare_linear(es, solver, bc, others=(), force_prec=None):
+    """Structures, assembly and the RHS pieces of one linear equations set
+    (``force_prec`` overrides the solver's precision setting)."""
     iron = _iron()
     geo = es.geometricField
     rm: RegularMesh = geo._mesh()
@@ -337,7 +338,13 @@ def _prepare_linear(es, solver, bc, others=()):
     dev = c.csr.device
     coords = geometry_coords(geo, rm, 0)
     kind = es._kind
-    mixed = str(getattr(solver, "linearIterativePrecision", "fp64")).lower() == "mixed"
+    prec = force_prec or str(getattr(solver, "linearIterativePrecision", "fp64")).lower()
+    if prec not in ("fp64", "mixed", "auto"):
+        raise CMFEError(1, f"linearIterativePrecision must be 'fp64', 'mixed' or 'auto', not {prec!r}")
+    if prec == "auto":
+        prec = fem.auto_precision(len(dm.comp_mesh), c.S.stored)
+    mixed = prec == "mixed"
+    solver.linearIterativePrecisionUsed = prec
     v64 = c.csr.zeros(fem.F64)
     v32 = c.csr.zeros(fem.F32) if mixed else None
     src = None
@@ -467,9 +474,11 @@ def solve_linear_sets(sets, solver, bc):
     solver system (rows and columns set by set, solver_mapping_routines.F90:791-830),
     one Krylov solve, then each set's field update and back-substitution."""
     sts = []
+    # one precision for the joint system: "auto" keeps several sets in FP64
+    force = "fp64" if str(getattr(solver, "linearIterativePrecision", "fp64")).lower() == "auto" else None
     for es in sets:
         others = tuple(o.dependentField for o in sets if o is not es)
-        sts.append(_prepare_linear(es, solver, bc, others))
+        sts.append(_prepare_linear(es, solver, bc, others, force_prec=force))
     mixed = sts[0]["mixed"]
     key = tuple(id(st["c"].red) for st in sts)  # the block cache holds these reductions
     holder = sets[0]

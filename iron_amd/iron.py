@@ -13,8 +13,10 @@ property ``xxx`` (python.py:142-169 generates both forms).  Errors raise
 CMFEError, as the bindings' wrapper does (_utils.in.py:66-77).
 
 Extensions beyond iron (no counterpart in the reference):
-  * ``Solver.linearIterativePrecision`` = "fp64" (default) or "mixed"
-    (FP32 inner CG with FP64 residual replacement);
+  * ``Solver.linearIterativePrecision`` = "fp64" (default), "mixed" (FP32 inner
+    CG with FP64 residual replacement) or "auto" (mixed for a scalar field whose
+    solver matrix holds at least fem.MIXED_MIN_STORED entries, else FP64; the
+    choice is left in ``Solver.linearIterativePrecisionUsed``);
   * a source field on a GENERALISED_LAPLACE equations set
     (``EquationsSet.SourceCreateStart``), which combines the element
     conductivity of the generalised Laplace subtype with a Poisson source
