@@ -7,18 +7,23 @@ per-element conductivity lognormal(0,1) from a fixed seed, source f = 1, u = 0 o
 every boundary face.  One step is one full pass of the path:
   FP32 (served inner operator) and FP64 (outer operator) element integration with
   coloured scatter into the CSR -> K_ff gather + RHS -> FP32 Jacobi CG inside FP64
-  iterative refinement to ||b - A x|| <= 1e-8 ||b|| -> field update.
+  iterative refinement to ||b - A x|| <= 1e-8 ||b|| -> field update ->
+  back-substitution (reactions K u - f on every equations row).
 The plain FP64 drop-in (FP64 assembly + FP64 Jacobi CG to 1e-8) is timed in the
 same run, and the C restatement of iron's CPU algorithm on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload diffusion48|laplace16] [--legs main,mr,elastic,laplace,slab,serve]
+Further legs: the 16^3 trilinear Laplace, the 96x32x32 trilinear elasticity
+cantilever, the 12^3 Q2/Q1 Mooney-Rivlin stretch, one x-slab of the 48^3 case
+and the served stream of 1024 requests, each with its CPU baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs main,mr,elastic,laplace,slab,serve]
 Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
-import math
 import os
 import sys
 import time
@@ -28,6 +33,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
@@ -37,7 +43,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="diffusion48", choices=["diffusion48", "laplace16"])
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0, help="budget of the main CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--format", default="sell", choices=["sell", "csr"], help="solver-matrix storage")
     ap.add_argument("--inner-rtol", type=float, default=0.1,
@@ -46,8 +52,11 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run three streaming passes y = 1.0 * x over 256 MiB FP32 arrays (reads and "
                          "writes 268,435,456 bytes each: the reference kernel for HBM byte counters)")
-    ap.add_argument("--serve-requests", type=int, default=128,
-                    help="requests of the served-stream leg (the full stream has 1024); 0 skips it")
+    ap.add_argument("--serve-requests", type=int, default=1024,
+                    help="requests of the served-stream leg (0 skips it)")
+    ap.add_argument("--serve-policies", default="auto,mixed,fp64",
+                    help="precision policies of the served stream, each measured at capacity and at Poisson "
+                         "arrivals")
     ap.add_argument("--legs", default="main,mr,elastic,laplace,slab,serve",
                     help="comma-separated legs to run: main (the workload above), mr, elastic (96x32x32 "
                          "elasticity), laplace (16^3 Laplace), slab (one eighth of the 48^3 case), serve")
@@ -57,12 +66,12 @@ def parse():
 TRAFFIC_FILE = ROOT / "profiles" / "cg_spmv_diffusion48_q2.md"
 WORKLOAD_TEXT = {
     "diffusion48": "diffusion48: 48^3 Q2 hex, 3x3x3 Gauss, kappa~lognormal(0,1) seed {seed}, f=1, u=0 on all "
-                   "faces, FP32 inner CG + FP64 refinement to 1e-8",
+                   "faces, FP32 inner CG + FP64 refinement to 1e-8, back-substitution",
     "diffusion48slab": "diffusion48 slab: 6x48x48 Q2 hex on [0,1/8]x[0,1]^2 (one of eight x-slabs of the 48^3 "
                        "case), 3x3x3 Gauss, kappa~lognormal(0,1) seed {seed}, f=1, u=0 on all faces, FP32 inner CG "
                        "+ FP64 refinement to 1e-8",
-    "laplace16": "laplace16: 16^3 Q1 hex, 2x2x2 Gauss, kappa=1, u=0 at x=0 and u=1 at x=1, FP32 inner CG + "
-                 "FP64 refinement to 1e-10",
+    "laplace16": "laplace16: 16^3 Q1 hex, 2x2x2 Gauss, kappa=1, u=0 at x=0 and u=1 at x=1, CG + Jacobi to 1e-10, "
+                 "back-substitution",
 }
 
 
@@ -110,7 +119,7 @@ def build_problem(args):
         ufix = np.where(np.abs(X[:, 0] - 1.0) < 1e-12, 1.0, 0.0)
         source = None
     csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, dev, colours=mesh.structured_colours(m))
-    asm = fem.ScalarAssembler(b, m.coords, m.elem_nodes[0], csr)
+    asm = fem.ScalarAssembler(b, m.coords, m.elem_nodes[0], csr, grid=ne)
     red = fem.DirichletReduction(csr, fixed)
     S = fem.SolverMatrix(red, args.format)
     P = dict(m=m, dm=dm, basis=b, csr=csr, asm=asm, red=red, S=S, rtol=rtol, fixed=fixed, ufix=ufix,
@@ -120,6 +129,8 @@ def build_problem(args):
     P["v32"] = csr.zeros(fem.F32)
     P["src"] = torch.zeros(dm.num_dofs, dtype=torch.float64, device=dev)
     P["u"] = torch.as_tensor(np.where(fixed, ufix, 0.0), device=dev)
+    P["u0"] = P["u"].clone()
+    P["R"] = torch.empty(dm.num_dofs, dtype=torch.float64, device=dev)
     P["sv64"] = torch.empty(max(S.stored, 1), dtype=torch.float64, device=dev)
     P["sv32"] = torch.empty(max(S.stored, 1), dtype=torch.float32, device=dev)
     P["b"] = torch.empty(red.n_free, dtype=torch.float64, device=dev)
@@ -128,6 +139,8 @@ def build_problem(args):
 
 
 def step(P, mixed: bool, inner_rtol: float):
+    """One pass: assembly, solver values and RHS, the solve, field update and the
+    back-substitution of the reactions."""
     from iron_amd import fem
     v64, v32, src = P["v64"], P["v32"], P["src"]
     v64.zero_()
@@ -136,6 +149,7 @@ def step(P, mixed: bool, inner_rtol: float):
                       prec=fem.F64)
     red, S = P["red"], P["S"]
     S.gather(v64, fem.F64, out=P["sv64"])
+    P["u"].copy_(P["u0"])
     red.rhs(v64, P["u"], src=src if P["source"] is not None else None, out=P["b"])
     x = P["x"]
     x.zero_()
@@ -147,6 +161,7 @@ def step(P, mixed: bool, inner_rtol: float):
     else:
         info = fem.cg(S, P["sv64"], P["b"], x, P["rtol"], 200000, P["ws"], prec=fem.F64)
     red.field_update(x, P["u"])
+    fem.backsubstitute(P["csr"], v64, P["u"], src=src if P["source"] is not None else None, out=P["R"])
     return info
 
 
@@ -164,6 +179,7 @@ def timed_steps(P, k, mixed, inner_rtol):
     return time.perf_counter() - t0, lat, infos
 
 
+# ------------------------------------------------------------------ CPU baselines
 def host_threads():
     """Host threads for the all-cores CPU baseline: OMP_NUM_THREADS when set (the
     host-core share given to this GPU), else the CPUs this process may run on."""
@@ -173,18 +189,97 @@ def host_threads():
         return len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(P, fp64_iters, budget_s, threads=1):
-    """Times the C restatement of iron's algorithm (oracle/liboracle_fe.so) over a
-    bounded sample and scales it to one full pass of the workload: on one core (iron's
-    per-rank serial loop), or on `threads` OpenMP threads (elements shared out with
-    atomic adds, CG rows and dot products split) as a proxy for iron on that many
-    ranks."""
-    import ctypes as C
-    sys.path.insert(0, str(ROOT))
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _P(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def oracle_lib():
+    """The C restatement of iron's CPU path (oracle/liboracle_fe.so), or None."""
     so = ROOT / "oracle" / "liboracle_fe.so"
     if not so.exists():
         return None
     lib = C.CDLL(str(so))
+    for name in ("oracle_assemble_scalar3", "oracle_assemble_scalar3_par", "oracle_assemble_elastic3",
+                 "oracle_assemble_elastic3_par", "oracle_mr_assemble"):
+        getattr(lib, name).restype = C.c_int
+    lib.oracle_reduce.restype = C.c_int64
+    lib.oracle_cg.restype = C.c_int
+    lib.oracle_cg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                              C.c_int, C.c_int, C.POINTER(C.c_double)]
+    lib.oracle_gmres.restype = C.c_int
+    lib.oracle_gmres.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    return lib
+
+
+def _sampled_assembly(assemble, E, budget_s, threads):
+    """Times assemble(e0, e1, nt) over elements [0, Es), Es sized to the budget;
+    completes the matrix untimed; returns the full-assembly estimate and Es."""
+    nthr = host_threads()
+    t = time.perf_counter()
+    probe = min(E, 256 * threads)
+    assemble(0, probe, threads)
+    per_elem = (time.perf_counter() - t) / probe
+    Es = int(min(E, max(probe, budget_s / max(per_elem, 1e-9))))
+    return per_elem, Es, nthr
+
+
+def _cg_sample(lib, n, srp, scol, sv, bb, iters_needed, budget_s, threads):
+    """Per-iteration time of the C Jacobi CG on the real system, over a bounded
+    number of iterations (at most iters_needed)."""
+    x = np.zeros(n)
+    rr = C.c_double(0.0)
+    t = time.perf_counter()
+    it_probe = 5
+    lib.oracle_cg(n, _P(srp), _P(scol), _P(sv), _P(bb), _P(x), 0.0, it_probe, threads, C.byref(rr))
+    per_it = (time.perf_counter() - t) / it_probe
+    n_it = int(max(it_probe, min(iters_needed, budget_s / max(per_it, 1e-9))))
+    x[:] = 0.0
+    t = time.perf_counter()
+    lib.oracle_cg(n, _P(srp), _P(scol), _P(sv), _P(bb), _P(x), 0.0, n_it, threads, C.byref(rr))
+    return (time.perf_counter() - t) / n_it, n_it
+
+
+def _reduce_and_backsub(lib, csr_rp, csr_col, vals, new_idx, u, src, n_free, threads):
+    """The C reduction (K_ff + lifted RHS, serial as iron's per-rank loop) and the
+    back-substitution over every equations row: (srp, scol, sv, b, t_reduce, t_backsub)."""
+    n = len(csr_rp) - 1
+    srp = np.zeros(n_free + 1, dtype=np.int32)
+    nnz = int(csr_rp[-1])
+    scol = np.zeros(max(nnz, 1), dtype=np.int32)
+    sv = np.zeros(max(nnz, 1))
+    bb = np.zeros(n_free)
+    t = time.perf_counter()
+    lib.oracle_reduce(n, _P(csr_rp), _P(csr_col), _P(vals), _P(new_idx), _P(u), _P(src), _P(srp), _P(scol), _P(sv),
+                      _P(bb))
+    t_red = time.perf_counter() - t
+    R = np.zeros(n)
+    t = time.perf_counter()
+    lib.oracle_backsubstitute(n, _P(csr_rp), _P(csr_col), _P(vals), _P(u), _P(src), _P(R), threads)
+    t_bs = time.perf_counter() - t
+    return srp, scol, sv, bb, t_red, t_bs
+
+
+def cpu_baseline_scalar(P, fp64_iters, budget_s, threads=1):
+    """The C restatement of iron's FP64 path for a scalar problem, timed over a
+    bounded sample and scaled to one full pass: element loop with bisection CSR add
+    over a sample of elements (scaled to all), the K_ff/RHS reduction and the
+    back-substitution in full, and a bounded number of Jacobi-CG iterations scaled to
+    the iterations the FP64 solve needs.  threads > 1: elements shared out with
+    atomic adds, CG rows and dot products split (a proxy for iron on that many ranks)."""
+    lib = oracle_lib()
+    if lib is None:
+        return None
     m, b, csr, red = P["m"], P["basis"], P["csr"], P["red"]
     E = m.num_elements
     rp = csr.row_ptr.cpu().numpy().astype(np.int32)
@@ -199,75 +294,80 @@ def cpu_baseline(P, fp64_iters, budget_s, threads=1):
     f = np.ones(E) if P["source"] is not None else None
     vals = np.zeros(csr.nnz)
     src = np.zeros(csr.n)
-    P_ = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
-    lib.oracle_assemble_scalar3.restype = C.c_int
-    lib.oracle_assemble_scalar3_par.restype = C.c_int
-    nthr = host_threads()
 
     def assemble(e0, e1, nt):
-        args = (E, b.num_nodes, b.num_gauss, P_(dN), P_(N), P_(w), P_(coords), P_(en), P_(ed), P_(kap), P_(f),
-                P_(rp), P_(col), P_(vals), P_(src), e0, e1)
+        args = (E, b.num_nodes, b.num_gauss, _P(dN), _P(N), _P(w), _P(coords), _P(en), _P(ed), _P(kap), _P(f),
+                _P(rp), _P(col), _P(vals), _P(src), e0, e1)
         return lib.oracle_assemble_scalar3(*args) if nt == 1 else lib.oracle_assemble_scalar3_par(*args, nt)
 
-    # assembly sample: elements [0, Es) chosen to fit ~40% of the budget
-    t = time.perf_counter()
-    probe = min(E, 512 * threads)
-    assemble(0, probe, threads)
-    per_elem = (time.perf_counter() - t) / probe
-    Es = int(min(E, max(probe, 0.4 * budget_s / max(per_elem, 1e-9))))
+    per_elem, Es, nthr = _sampled_assembly(assemble, E, 0.4 * budget_s, threads)
     vals[:] = 0.0
     src[:] = 0.0
     t = time.perf_counter()
     assemble(0, Es, threads)
     t_asm = (time.perf_counter() - t) * (E / Es)
-    if Es < E:  # complete the matrix (untimed, all host threads) so the solve runs on the real system
+    if Es < E:   # complete the matrix (untimed, all host threads): the solve runs on the real system
         assemble(Es, E, nthr)
     new_idx = red.new_idx.cpu().numpy().astype(np.int32)
-    u = P["u"].cpu().numpy()
-    srp = np.zeros(red.n_free + 1, dtype=np.int32)
-    scol = np.zeros(max(red.nnz, 1), dtype=np.int32)
-    sv = np.zeros(max(red.nnz, 1))
-    bb = np.zeros(red.n_free)
-    lib.oracle_reduce.restype = C.c_int64
-    t = time.perf_counter()
-    lib.oracle_reduce(csr.n, P_(rp), P_(col), P_(vals), P_(new_idx), P_(u), P_(src if f is not None else None),
-                      P_(srp), P_(scol), P_(sv), P_(bb))
-    t_red = time.perf_counter() - t
-    x = np.zeros(red.n_free)
-    rr = C.c_double(0.0)
-    lib.oracle_cg.restype = C.c_int
-    lib.oracle_cg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
-                              C.c_int, C.c_int, C.POINTER(C.c_double)]
-    t = time.perf_counter()
-    it_probe = 5
-    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, it_probe, threads, C.byref(rr))
-    per_it = (time.perf_counter() - t) / it_probe
-    n_it = int(max(it_probe, min(fp64_iters, 0.5 * budget_s / max(per_it, 1e-9))))
-    x[:] = 0.0
-    t = time.perf_counter()
-    lib.oracle_cg(red.n_free, P_(srp), P_(scol), P_(sv), P_(bb), P_(x), 0.0, n_it, threads, C.byref(rr))
-    per_it = (time.perf_counter() - t) / n_it
+    u = P["u0"].cpu().numpy()
+    srp, scol, sv, bb, t_red, t_bs = _reduce_and_backsub(lib, rp, col, vals, new_idx, u, src if f is not None else None,
+                                                         red.n_free, threads)
+    per_it, n_it = _cg_sample(lib, red.n_free, srp, scol, sv, bb, fp64_iters, 0.5 * budget_s, threads)
     t_solve = per_it * fp64_iters
-    t_total = t_asm + t_red + t_solve
-    cpu = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    t_total = t_asm + t_red + t_solve + t_bs
     return {
         "value": E / t_total, "unit": "elems/s", "cores": threads, "kind": "port",
-        "sample": (f"C restatement of iron's FP64 path on {threads} thread(s) ({cpu}): element loop with bisection "
-                   f"CSR add"
-                   f" over {Es} of {E} elements (x{E / Es:.2f}), K_ff/RHS reduction in full (serial), {n_it} Jacobi-CG "
-                   f"iterations scaled to the {fp64_iters} iterations the FP64 solve needs"),
-        "assembly_s": t_asm, "reduce_s": t_red, "cg_iter_ms": per_it * 1e3, "solve_s": t_solve,
-        "solves_per_s": 1.0 / t_total,
+        "sample": (f"C restatement of iron's FP64 path on {threads} thread(s) ({cpu_model()}): element loop with "
+                   f"bisection CSR add over {Es} of {E} elements (x{E / Es:.2f}), K_ff/RHS reduction (serial) and "
+                   f"back-substitution in full, {n_it} Jacobi-CG iterations scaled to the {fp64_iters} iterations "
+                   f"the FP64 solve needs"),
+        "assembly_s": t_asm, "reduce_s": t_red, "backsub_s": t_bs, "cg_iter_ms": per_it * 1e3, "solve_s": t_solve,
+        "pass_s": t_total, "solves_per_s": 1.0 / t_total,
     }
 
 
+def cpu_baseline_elastic(E_count, b, coords, en, ed, mat, csr_rp, csr_col, new_idx, n_free, rhs, fp64_iters,
+                         budget_s, threads=1):
+    """As cpu_baseline_scalar for 3-D linear elasticity (iron's element loop with its
+    DXI_DX index order, bisection add), the reduction with the traction RHS, the
+    back-substitution and Jacobi CG."""
+    lib = oracle_lib()
+    if lib is None:
+        return None
+    dN = np.ascontiguousarray(b.dN, dtype=np.float64)
+    w = np.ascontiguousarray(b.weights, dtype=np.float64)
+    vals = np.zeros(int(csr_rp[-1]))
+
+    def assemble(e0, e1, nt):
+        args = (E_count, b.num_nodes, b.num_gauss, _P(dN), _P(w), _P(coords), _P(en), _P(ed), _P(mat), _P(csr_rp),
+                _P(csr_col), _P(vals), e0, e1)
+        return lib.oracle_assemble_elastic3(*args) if nt == 1 else lib.oracle_assemble_elastic3_par(*args, nt)
+
+    per_elem, Es, nthr = _sampled_assembly(assemble, E_count, 0.4 * budget_s, threads)
+    vals[:] = 0.0
+    t = time.perf_counter()
+    assemble(0, Es, threads)
+    t_asm = (time.perf_counter() - t) * (E_count / Es)
+    if Es < E_count:
+        assemble(Es, E_count, nthr)
+    u = np.zeros(len(csr_rp) - 1)
+    srp, scol, sv, bb, t_red, t_bs = _reduce_and_backsub(lib, csr_rp, csr_col, vals, new_idx, u, rhs, n_free,
+                                                         threads)
+    per_it, n_it = _cg_sample(lib, n_free, srp, scol, sv, bb, fp64_iters, 0.5 * budget_s, threads)
+    t_solve = per_it * fp64_iters
+    t_total = t_asm + t_red + t_solve + t_bs
+    return {
+        "value": E_count / t_total, "unit": "elems/s", "cores": threads, "kind": "port",
+        "sample": (f"C restatement of iron's FP64 linear-elasticity path on {threads} thread(s) ({cpu_model()}): "
+                   f"element loop with bisection CSR add over {Es} of {E_count} elements (x{E_count / Es:.2f}), "
+                   f"reduction (serial) and back-substitution in full, {n_it} Jacobi-CG iterations scaled to the "
+                   f"{fp64_iters} iterations the FP64 solve needs"),
+        "assembly_s": t_asm, "reduce_s": t_red, "backsub_s": t_bs, "cg_iter_ms": per_it * 1e3, "solve_s": t_solve,
+        "pass_s": t_total, "solves_per_s": 1.0 / t_total,
+    }
+
+
+# ------------------------------------------------------------------ Mooney-Rivlin
 def mr_flops_per_element(nn, npd, ng):
     """Algorithmic FP64 flops of one Mooney-Rivlin element (residual + exact Jacobian),
     counted from the kernel's loops: per Gauss point 72 nn^2 (displacement block:
@@ -277,10 +377,87 @@ def mr_flops_per_element(nn, npd, ng):
     return ng * (72 * nn * nn + 54 * nn * npd + 216 * nn + 4200)
 
 
+def cpu_baseline_mr(sysg, newton, x_final, mat, jac_assemblies, res_assemblies, gmres_iters, restart, budget_s):
+    """The C restatement of the Newton path on one core: the Mooney-Rivlin element
+    loop (residual + exact Jacobian, bisection add) over a sample of elements scaled
+    to the whole mesh, the residual-only loop likewise, and GMRES(restart) with the
+    pressure-block diagonal preconditioner on the real Newton matrix at the final
+    state over a bounded number of iterations; scaled to the Jacobian and residual
+    assemblies and the GMRES iterations the GPU run needed."""
+    lib = oracle_lib()
+    if lib is None:
+        return None
+    rm = sysg.rm
+    bu, bp = sysg.bu, sysg.bp
+    E, nn = rm.num_elements, bu.num_nodes
+    dN = np.ascontiguousarray(bu.dN, dtype=np.float64)
+    w = np.ascontiguousarray(bu.weights, dtype=np.float64)
+    psi = np.ones((bu.num_gauss, 1)) if bp is None else np.ascontiguousarray(bp.N, dtype=np.float64)
+    npd = psi.shape[1]
+    coords = np.ascontiguousarray(sysg.X)
+    state = np.ascontiguousarray(x_final.cpu().numpy())
+    en = np.ascontiguousarray(rm.elem_nodes[0], dtype=np.int32)
+    ed = np.ascontiguousarray(sysg.elem_dofs, dtype=np.int32)
+    matc = np.ascontiguousarray(mat, dtype=np.float64)
+    rp, col = sysg.csr.to_host()
+    rp, col = rp.astype(np.int32), col.astype(np.int32)
+    vals = np.zeros(len(col))
+    R = np.zeros(sysg.ndofs)
+
+    def assemble(e0, e1, with_jac):
+        return lib.oracle_mr_assemble(E, nn, npd, bu.num_gauss, sysg.nu, _P(dN), _P(psi), _P(w), _P(coords),
+                                      _P(state), _P(en), _P(ed), _P(matc), _P(rp), _P(col),
+                                      _P(vals) if with_jac else None, _P(R), e0, e1)
+
+    t = time.perf_counter()
+    assemble(0, 8, True)
+    per = (time.perf_counter() - t) / 8
+    Es = int(min(E, max(8, 0.3 * budget_s / max(per, 1e-9))))
+    vals[:] = 0.0
+    R[:] = 0.0
+    t = time.perf_counter()
+    assemble(0, Es, True)
+    t_jac = (time.perf_counter() - t) * E / Es
+    t = time.perf_counter()
+    assemble(0, Es, False)
+    t_res = (time.perf_counter() - t) * E / Es
+    vals[:] = 0.0
+    R[:] = 0.0
+    assemble(0, E, True)
+    new_idx = newton.red.new_idx.cpu().numpy().astype(np.int32)
+    n_free = newton.red.n_free
+    srp, scol, sv, bb, _, _ = _reduce_and_backsub(lib, rp, col, vals, new_idx, np.zeros(sysg.ndofs), None, n_free, 1)
+    bb = -R[newton.red.free_idx.cpu().numpy()]
+    pc = np.ascontiguousarray(newton.pc_neg.cpu().numpy())
+    xg = np.zeros(n_free)
+    rr = C.c_double(0.0)
+    t = time.perf_counter()
+    lib.oracle_gmres(n_free, _P(srp), _P(scol), _P(sv), _P(pc), _P(bb), _P(xg), 0.0, restart, 8, 1, C.byref(rr))
+    per_it = (time.perf_counter() - t) / 8
+    n_it = int(max(restart, min(gmres_iters, 0.5 * budget_s / max(per_it, 1e-9))))
+    n_it = (n_it // restart) * restart if n_it >= restart else n_it
+    xg[:] = 0.0
+    t = time.perf_counter()
+    lib.oracle_gmres(n_free, _P(srp), _P(scol), _P(sv), _P(pc), _P(bb), _P(xg), 0.0, restart, n_it, 1, C.byref(rr))
+    per_it = (time.perf_counter() - t) / n_it
+    total = jac_assemblies * t_jac + res_assemblies * t_res + gmres_iters * per_it
+    return {
+        "value": 1.0 / total, "unit": "solves/s (10 load increments)", "cores": 1, "kind": "port",
+        "sample": (f"C restatement on 1 thread ({cpu_model()}): Mooney-Rivlin residual + exact Jacobian element "
+                   f"loop with bisection add over {Es} of {E} elements (x{E / Es:.2f}), the residual-only loop "
+                   f"likewise, and {n_it} GMRES({restart}) iterations on the final Newton matrix; scaled to the "
+                   f"{jac_assemblies} Jacobian and {res_assemblies} residual assemblies and {gmres_iters} GMRES "
+                   f"iterations of the GPU run"),
+        "jacobian_assembly_s": t_jac, "residual_assembly_s": t_res, "gmres_iter_ms": per_it * 1e3,
+        "solve_s": total, "assembly_elems_per_s": E / t_jac,
+    }
+
+
 def mr_leg(runs, cpu_seconds):
     """12^3 triquadratic/trilinear Mooney-Rivlin cube (c1=2, c2=6), rollers on x=0, y=0,
     z=0, x=1 face stretched to 1.2 in 10 load increments, Newton to a relative residual
-    of 1e-8 per increment (preconditioned MINRES steps, Eisenstat-Walker tolerances)."""
+    of 1e-8 per increment: the default preconditioned-MINRES steps with Eisenstat-Walker
+    tolerances, and GMRES(100) steps each to 1e-8 as the configuration states it."""
     import torch
     from iron_amd import _lib, basis, finite_elasticity as fe, mesh
     rm = mesh.RegularMesh((12, 12, 12), [basis.basis(3, 2, 3), basis.basis(3, 1, 3)], (1.0, 1.0, 1.0))
@@ -295,30 +472,37 @@ def mr_leg(runs, cpu_seconds):
     target[:nu][X[:, 0] == 1.0] = 1.2
     mat = np.tile([2.0, 6.0], (rm.num_elements, 1))
     newton = fe.MRNewton(sysg, fixed, mat)
-    opts = fe.NewtonOptions(atol=1e-12, rtol=1e-8, stol=0.0, linear_rtol="ew")
+    xe = X * np.array([1.2, 1.2 ** -0.5, 1.2 ** -0.5])
+    E = rm.num_elements
+    fl = mr_flops_per_element(27, 8, 27) * E
 
-    def run():
+    def run(opts):
         x = torch.as_tensor(x0, device=sysg.device)
         infos = fe.load_increments(newton, x, 10, fixed, target, opts=opts)
         torch.cuda.synchronize()
         return x, infos
 
-    run()
-    _lib.profile_enable(True)
-    _lib.profile_reset()
-    times = []
-    for _ in range(runs):
-        t = time.perf_counter()
-        x, infos = run()
-        times.append(time.perf_counter() - t)
-    prof = _lib.profile_read()
-    _lib.profile_enable(False)
-    xe = X * np.array([1.2, 1.2 ** -0.5, 1.2 ** -0.5])
+    def profiled(opts, reps):
+        run(opts)
+        _lib.profile_enable(True)
+        _lib.profile_reset()
+        times = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            x, infos = run(opts)
+            times.append(time.perf_counter() - t)
+        prof = _lib.profile_read()
+        _lib.profile_enable(False)
+        return x, infos, times, prof
+
+    opts = fe.NewtonOptions(atol=1e-12, rtol=1e-8, stol=0.0, linear_rtol="ew")
+    x, infos, times, prof = profiled(opts, runs)
     err = float(np.abs(x.cpu().numpy()[:3 * nu].reshape(3, -1).T - xe).max())
-    n_mr, ms_mr = prof.get("finite_elasticity", (0, 0.0))
-    avg_mr = ms_mr / max(n_mr, 1)
-    E = rm.num_elements
-    fl = mr_flops_per_element(27, 8, 27) * E
+    n_j, ms_j = prof.get("finite_elasticity", (0, 0.0))
+    n_r, ms_r = prof.get("finite_elasticity_residual", (0, 0.0))
+    avg_j = ms_j / max(n_j, 1)
+    avg_r = ms_r / max(n_r, 1)
+    ach = fl / (avg_j * 1e-3) / 1e12 if avg_j > 0 else None
     out = {
         "workload": "mr12: 12^3 Q2/Q1 Mooney-Rivlin (deviatoric, c1=2, c2=6), rollers + x-stretch to 1.2 in 10 "
                     "increments, Newton to 1e-8 per increment, MINRES steps with Eisenstat-Walker tolerances",
@@ -327,57 +511,43 @@ def mr_leg(runs, cpu_seconds):
         "newton_iterations": sum(i.iterations for i in infos),
         "minres_iterations": sum(i.linear_iterations for i in infos),
         "max_node_error_vs_closed_form": err,
-        "assembly_ms": avg_mr, "assemblies": n_mr,
-        "assembly_elems_per_s": E / (avg_mr * 1e-3) if avg_mr > 0 else None,
-        "roofline": {"kernel": "mr_element_kernel<27,8,27> (residual + exact Jacobian, FP64)", "bound": "fp64",
-                     "achieved": fl / (avg_mr * 1e-3) / 1e12 if avg_mr > 0 else None, "peak": 78.6,
-                     "unit": "TFLOP/s", "frac": (fl / (avg_mr * 1e-3) / 1e12) / 78.6 if avg_mr > 0 else None,
-                     "flops_per_element": mr_flops_per_element(27, 8, 27)},
+        "jacobian_assembly_ms": avg_j, "jacobian_assemblies": n_j // max(runs, 1),
+        "residual_assembly_ms": avg_r, "residual_assemblies": n_r // max(runs, 1),
+        "assembly_elems_per_s": E / (avg_j * 1e-3) if avg_j > 0 else None,
+        "roofline": {"kernel": "mr_element_kernel<27,8,27> (residual + exact Jacobian, FP64), residual+Jacobian "
+                               "launches only", "bound": "fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS if ach else None,
+                     "flops_per_element": mr_flops_per_element(27, 8, 27), "avg_launch_ms": avg_j,
+                     "launches": n_j},
         "kernel_ms": {k: v for k, v in prof.items() if v[0]},
     }
-    # the same load path with GMRES steps (restart 100, pressure-block preconditioner,
-    # FP64 values), each linear solve to 1e-8
-    opts_g = fe.NewtonOptions(atol=1e-12, rtol=1e-8, stol=0.0, linear_rtol=1e-8, linear="gmres")
-    x = torch.as_tensor(x0, device=sysg.device)
-    t = time.perf_counter()
-    infos_g = fe.load_increments(newton, x, 10, fixed, target, opts=opts_g)
-    torch.cuda.synchronize()
-    out["gmres_1e8"] = {"solve_s": time.perf_counter() - t, "newton_iterations": sum(i.iterations for i in infos_g),
-                        "gmres_iterations": sum(i.linear_iterations for i in infos_g),
+    # the configuration's solver: GMRES(100), pressure-block preconditioner, each Newton
+    # step to 1e-8
+    opts_g = fe.NewtonOptions(atol=1e-12, rtol=1e-8, stol=0.0, linear_rtol=1e-8, linear="gmres", gmres_restart=100)
+    xg, infos_g, times_g, prof_g = profiled(opts_g, 1)
+    gi = sum(i.linear_iterations for i in infos_g)
+    out["gmres_1e8"] = {"solve_s": times_g[0], "newton_iterations": sum(i.iterations for i in infos_g),
+                        "gmres_iterations": gi, "restart": 100,
+                        "us_per_gmres_iteration_incl_newton": times_g[0] / max(gi, 1) * 1e6,
                         "max_node_error_vs_closed_form":
-                            float(np.abs(x.cpu().numpy()[:3 * nu].reshape(3, -1).T - xe).max())}
+                            float(np.abs(xg.cpu().numpy()[:3 * nu].reshape(3, -1).T - xe).max()),
+                        "kernel_ms": {k: v for k, v in prof_g.items() if v[0]}}
     if cpu_seconds > 0:
-        sys.path.insert(0, str(ROOT))
-        from oracle import mr_oracle as mo
-        from oracle.fe_oracle import Basis, regular_mesh
-        om = regular_mesh((12, 12, 12), [Basis(3, 2, 3), Basis(3, 1, 3)], (1.0, 1.0, 1.0))
-        prob = mo.MRProblem(om, False, np.full(E, 2.0), np.full(E, 6.0))
-        xs = prob.initial_state(0.0)
-        geo_e, dep_e, pe = prob.element_arrays(xs)
-        ns = 16
-        t = time.perf_counter()
-        k = 0
-        while time.perf_counter() - t < 0.5 * cpu_seconds and k + ns <= E:
-            mo.mr_element_residual(geo_e[k:k + ns], dep_e[k:k + ns], pe[k:k + ns], prob.bu, prob.bp,
-                                   prob.c1[k:k + ns], prob.c2[k:k + ns])
-            mo.mr_element_jacobian(geo_e[k:k + ns], dep_e[k:k + ns], pe[k:k + ns], prob.bu, prob.bp,
-                                   prob.c1[k:k + ns], prob.c2[k:k + ns])
-            k += ns
-        per = (time.perf_counter() - t) / max(k, 1)
-        out["cpu_baseline"] = {"value": 1.0 / per, "unit": "elems/s (residual + Jacobian)", "cores": 1,
-                               "kind": "port",
-                               "sample": f"NumPy FP64 oracle element residual + exact Jacobian on {k} of {E} "
-                                         f"elements (vectorised over {ns} elements per call)"}
+        out["cpu_baseline"] = cpu_baseline_mr(sysg, newton, xg, mat, prof_g.get("finite_elasticity", (0, 0))[0],
+                                              prof_g.get("finite_elasticity_residual", (0, 0))[0], gi, 100,
+                                              cpu_seconds)
     return out
 
 
-def elastic_leg(runs=3):
+# ------------------------------------------------------------------ elasticity
+def elastic_leg(runs=3, cpu_seconds=10.0):
     """96x32x32 trilinear linear elasticity on [0,3]x[0,1]^2: lognormal Young's modulus
     10 exp(0.3 N(0,1)) per element (seed 7), Poisson ratio 0.3, clamp at x = 0,
-    consistent traction t_z = -1 on x = 3.  One pass: FP64 and FP32 (MFMA) element
-    stiffness, solver values and RHS, FP32 Jacobi CG with FP64 refinement to a 1e-8
-    relative residual (rows node by node, 16-bit SELL columns), field update.  Also
-    the FP64 drop-in CG on the same system."""
+    uniform traction t_z = -1 on the x = 3 face (NEUMANN_POINT -1 on its nodes, the side
+    faces next to it left out by NEUMANN_INTEGRATED_ONLY DOFs one element in, so the
+    nodal forces sum to -1 x the face area).  One pass: FP64 (and in the mixed pass FP32
+    MFMA) element stiffness, solver values and RHS, Jacobi CG to a 1e-8 relative
+    residual (rows node by node, 16-bit SELL columns), field update, back-substitution."""
     import torch
     from iron_amd import _lib, basis, engine, fem, mesh
     ne = (96, 32, 32)
@@ -389,19 +559,24 @@ def elastic_leg(runs=3):
     E, N = m.num_elements, m.num_nodes(0)
     g = torch.Generator().manual_seed(7)
     Ee = 10.0 * torch.exp(0.3 * torch.randn(E, dtype=torch.float64, generator=g)).numpy()
-    mat = torch.as_tensor(np.ascontiguousarray(np.stack([Ee, Ee, Ee, np.full(E, 0.3), np.full(E, 0.3),
-                                                         np.full(E, 0.3)], axis=1)), device=dev)
+    mat_h = np.ascontiguousarray(np.stack([Ee, Ee, Ee, np.full(E, 0.3), np.full(E, 0.3), np.full(E, 0.3)], axis=1))
+    mat = torch.as_tensor(mat_h, device=dev)
     csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, dev, colours=mesh.structured_colours(m))
-    asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], csr)
+    asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], csr, grid=ne)
     clamp = X[:, 0] == 0.0
     red = fem.DirichletReduction(csr, np.concatenate([clamp, clamp, clamp]))
     S = fem.SolverMatrix(red, interleave=(3, N))
     tip = np.nonzero(X[:, 0] == 3.0)[0]
-    ft = torch.as_tensor(engine.neumann_integrate(m, dm, [b] * 3, {int(2 * N + i): -1.0 for i in tip}), device=dev)
+    hx = 3.0 / ne[0]
+    onb = (X[:, 1] == 0.0) | (X[:, 1] == 1.0) | (X[:, 2] == 0.0) | (X[:, 2] == 1.0)
+    ring = frozenset(int(2 * N + i) for i in np.nonzero(np.isclose(X[:, 0], 3.0 - hx) & onb)[0])
+    ft_h = engine.neumann_integrate(m, dm, [b] * 3, {int(2 * N + i): -1.0 for i in tip}, integrated_only=ring)
+    ft = torch.as_tensor(ft_h, device=dev)
     v, v32 = csr.zeros(fem.F64), csr.zeros(fem.F32)
     sv64 = torch.empty(max(S.stored, 1), dtype=torch.float64, device=dev)
     sv32 = torch.empty(max(S.stored, 1), dtype=torch.float32, device=dev)
     u = torch.zeros(csr.n, dtype=torch.float64, device=dev)
+    R = torch.empty(csr.n, dtype=torch.float64, device=dev)
     bvec = torch.empty(red.n_free, dtype=torch.float64, device=dev)
     x = torch.zeros(red.n_free, dtype=torch.float64, device=dev)
     ws = fem.KrylovWorkspace(red.n_free, dev)
@@ -421,6 +596,7 @@ def elastic_leg(runs=3):
         else:
             info = fem.cg(S, sv64, bvec, x, 1e-8, 200000, ws, prec=fem.F64)
         red.field_update(x, u)
+        fem.backsubstitute(csr, v, u, out=R)
         return info
 
     res = {}
@@ -445,11 +621,15 @@ def elastic_leg(runs=3):
     t_64, info_64 = res[False]
     a64 = prof.get("assemble_elastic", (0, 0.0))[1]
     a32 = prof.get("assemble_elastic_f32", (0, 0.0))[1]
-    return {
+    pick = fem.auto_precision(3, S.stored)
+    t_pick = t_mx if pick == "mixed" else t_64
+    out = {
         "workload": "elasticity96: 96x32x32 Q1 hex on [0,3]x[0,1]^2, 2x2x2 Gauss, E = 10 exp(0.3 N(0,1)) per element "
-                    "(seed 7), nu = 0.3, clamp at x=0, traction t_z=-1 on x=3; FP32 element stiffness (MFMA) + FP32 "
-                    "Jacobi CG with FP64 refinement to 1e-8",
+                    "(seed 7), nu = 0.3, clamp at x=0, uniform traction t_z=-1 on x=3 (nodal forces sum to -1), "
+                    "Jacobi CG to 1e-8, back-substitution",
         "elements": E, "free_dofs": red.n_free, "solver_nnz": red.nnz, "sell_padding": S.padding,
+        "applied_force_z": float(ft_h.sum()),
+        "api_precision": pick, "api_ms_per_pass": t_pick * 1e3, "api_elems_per_s": E / t_pick,
         "ms_per_pass": t_mx * 1e3, "elems_per_s": E / t_mx, "iterations_inner": info_mx.iterations,
         "refinement_steps": info_mx.outer, "relres": info_mx.relres,
         "fp64_ms_per_pass": t_64 * 1e3, "fp64_iterations": info_64.iterations, "fp64_relres": info_64.relres,
@@ -460,14 +640,29 @@ def elastic_leg(runs=3):
                      "frac": (alg / (per * 1e-3) / 1e9 / HBM_PEAK_GBS) if per > 0 else 0.0,
                      "bytes_per_launch": alg, "avg_launch_ms": per, "launches": n_s},
     }
+    if cpu_seconds > 0:
+        rp = csr.row_ptr.cpu().numpy().astype(np.int32)
+        col = csr.col[: csr.nnz].cpu().numpy().astype(np.int32)
+        ed = np.ascontiguousarray(dm.element_dofs(), dtype=np.int32)
+        en = np.ascontiguousarray(m.elem_nodes[0], dtype=np.int32)
+        new_idx = red.new_idx.cpu().numpy().astype(np.int32)
+        args = (E, b, np.ascontiguousarray(X, dtype=np.float64), en, ed, mat_h, rp, col, new_idx, red.n_free, ft_h,
+                info_64.iterations)
+        out["cpu_baseline"] = cpu_baseline_elastic(*args, cpu_seconds, 1)
+        nthr = host_threads()
+        if nthr > 1:
+            out["cpu_baseline_all_cores"] = cpu_baseline_elastic(*args, 0.5 * cpu_seconds, nthr)
+    return out
 
 
-def laplace_leg(args, runs=5):
+# ------------------------------------------------------------------ Laplace
+def laplace_leg(args, runs=5, cpu_seconds=4.0):
     """16^3 trilinear Laplace, u = 0 at x = 0 and u = 1 at x = 1: the full pass (FP64
-    assembly, solver values, FP32 CG with FP64 refinement to 1e-10, field update), and
-    the FP64 drop-in CG."""
+    assembly, solver values, CG to 1e-10 in FP64 or FP32 CG with FP64 refinement, field
+    update, back-substitution) in both precisions, and the one the API picks."""
     import argparse
     import torch
+    from iron_amd import fem
     a = argparse.Namespace(**vars(args))
     a.workload = "laplace16"
     P = build_problem(a)
@@ -479,7 +674,15 @@ def laplace_leg(args, runs=5):
         out[f"{key}_elems_per_s"] = P["m"].num_elements * runs / wall
         out[f"{key}_iterations"] = infos[-1].iterations
         out[f"{key}_relres"] = infos[-1].relres
+    pick = fem.auto_precision(1, P["S"].stored)
+    out["api_precision"] = pick
+    out["api_ms_per_pass"] = out[f"{pick}_ms_per_pass"]
     torch.cuda.synchronize()
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_baseline_scalar(P, out["fp64_iterations"], cpu_seconds, 1)
+        nthr = host_threads()
+        if nthr > 1:
+            out["cpu_baseline_all_cores"] = cpu_baseline_scalar(P, out["fp64_iterations"], 0.5 * cpu_seconds, nthr)
     return out
 
 
@@ -519,37 +722,52 @@ def slab_leg(args, runs=3):
     }
 
 
-def serve_leg(n, seed=2024):
+# ------------------------------------------------------------------ served stream
+def serve_leg(n, policies, seed=2024):
     """Served stream: n requests, 70% diffusion / 30% elasticity trilinear boxes with
     N_x, N_y, N_z ~ U{8..64}, lognormal(0,1) coefficients (per-request seeds), BCs of
-    the 16^3 Laplace and 96x32x32 elasticity cases, solved to a 1e-8 FP64 residual.
-    Measured: the grouped server's capacity (all requests at t=0; FP32 iterations
-    with FP64 reliable updates, and FP64 iterations for comparison), then latency at
-    Poisson arrivals of 0.7x that capacity (arrival -> solution on the host), and the
-    one-at-a-time FP64 path on the same requests."""
+    the 16^3 Laplace and 96x32x32 elasticity cases, each solved to a 1e-8 FP64 residual.
+    For each precision policy: the grouped server's capacity (all requests at t=0) and
+    latency (arrival -> solution on the host) at Poisson arrivals of 0.7x that capacity;
+    then the one-at-a-time FP64 path on the same requests, and the grouped product's
+    roofline on the first 256 requests."""
     import torch
-    from iron_amd import serving
-    srv = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=True)
-    warm = serving.make_stream(8, seed=seed + 1, lo=8, hi=16)
-    srv.run(warm)
-    reqs = serving.make_stream(n, seed=seed)
-    t = time.perf_counter()
-    out = srv.run(reqs)
-    torch.cuda.synchronize()
-    wall_sat = time.perf_counter() - t
-    cap = n / wall_sat
-    timing_sat = dict(srv.timing)
-    for k in srv.timing:
-        srv.timing[k] = 0.0
-    reqs_p = serving.make_stream(n, seed=seed, rate=0.7 * cap)
-    t = time.perf_counter()
-    out_p = srv.run(reqs_p)
-    torch.cuda.synchronize()
-    wall_p = time.perf_counter() - t
-    lat = np.array([r.latency for r in out_p]) * 1e3
+    from iron_amd import _lib, serving
+    pol_out = {}
+    for pol in policies:
+        srv = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=True, precision=pol)
+        srv.run(serving.make_stream(8, seed=seed + 1, lo=8, hi=16))
+        for k in srv.timing:
+            srv.timing[k] = 0.0
+        reqs = serving.make_stream(n, seed=seed)
+        t = time.perf_counter()
+        out = srv.run(reqs)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        cap = n / wall
+        timing = dict(srv.timing)
+        for k in srv.timing:
+            srv.timing[k] = 0.0
+        reqs_p = serving.make_stream(n, seed=seed, rate=0.7 * cap)
+        t = time.perf_counter()
+        out_p = srv.run(reqs_p)
+        torch.cuda.synchronize()
+        wall_p = time.perf_counter() - t
+        lat = np.array([r.latency for r in out_p]) * 1e3
+        host = timing["admit"] + timing["structure"] + timing["retire"] + timing["chunk"]
+        pol_out[pol] = {
+            "capacity_solves_per_s": cap, "elems_per_s_at_capacity": sum(int(np.prod(r.n_elem)) for r in out) / wall,
+            "poisson_rate_per_s": 0.7 * cap, "poisson_solves_per_s": n / wall_p,
+            "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
+            "iterations": sum(r.iterations for r in out), "max_relres": max(r.relres for r in out + out_p),
+            "server_phase_s_at_capacity": timing, "host_share_of_wall": host / wall,
+            "server_stats": {k: v for k, v in srv.stats.items() if k != "spmv_bytes"},
+        }
+        del srv
+        torch.cuda.empty_cache()
     structures = {}
-    seq = serving.make_stream(n, seed=seed)
     service = []
+    seq = serving.make_stream(n, seed=seed)
     t = time.perf_counter()
     for r in seq:
         t1 = time.perf_counter()
@@ -558,96 +776,52 @@ def serve_leg(n, seed=2024):
     torch.cuda.synchronize()
     wall_seq = time.perf_counter() - t
     service = np.array(service) * 1e3
-    elems = sum(int(np.prod(r.n_elem)) for r in reqs)
-    stats_sat = dict(srv.stats)
-    del srv   # its pools, before the next server takes its own
-    # the same stream through FP64 grouped iterations
-    srv64 = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=True, precision="fp64")
-    srv64.run(serving.make_stream(8, seed=seed + 1, lo=8, hi=16))
-    t = time.perf_counter()
-    out64 = srv64.run(serving.make_stream(n, seed=seed))
-    torch.cuda.synchronize()
-    cap64 = n / (time.perf_counter() - t)
-    it64 = sum(r.iterations for r in out64)
-    del srv64
-    # roofline of the grouped product: the same stream served once more without graph
-    # replay (so that every launch carries HIP events) and without the structure
-    # prefetch thread (whose kernels would share the GPU with the timed launches),
-    # algorithmic bytes counted per segment iteration by the server
-    from iron_amd import _lib
+    # roofline of the grouped product: the first 256 requests served without graph
+    # replay (every launch carries HIP events) and without the structure prefetch thread
+    # (whose kernels would share the GPU with the timed launches), algorithmic bytes
+    # counted per segment iteration by the server
+    nr = min(n, 256)
     srv_r = serving.Server(hbm_budget=60e9, max_segments=256, chunk=16, use_graph=False, prefetch=0)
     srv_r.run(serving.make_stream(8, seed=seed + 1, lo=8, hi=16))
     srv_r.stats["spmv_bytes"] = 0.0
     _lib.profile_enable(True)
     _lib.profile_reset()
-    srv_r.run(serving.make_stream(n, seed=seed))
+    srv_r.run(serving.make_stream(nr, seed=seed))
     torch.cuda.synchronize()
     prof = _lib.profile_read()
     _lib.profile_enable(False)
     n_g, ms_g = prof.get("grouped_spmv", (0, 0.0))
     ach = srv_r.stats["spmv_bytes"] / (ms_g * 1e-3) / 1e9 if ms_g > 0 else 0.0
-    roof = {"kernel": "gcg_spmv (grouped SpMV over the admitted requests + p/q update + p.q; FP32 for diffusion "
-                      "requests, FP64 for elasticity)",
-            "bound": "hbm",
-            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "bytes_total": srv_r.stats["spmv_bytes"], "launches": n_g, "avg_launch_ms": ms_g / max(n_g, 1)}
-    return {
+    del srv_r
+    seq_rate = n / wall_seq
+    main = pol_out.get("auto") or next(iter(pol_out.values()))
+    res = {
         "workload": f"served stream: {n} requests, 70% diffusion / 30% elasticity Q1 boxes N~U{{8..64}}^3, "
-                    "lognormal(0,1) coefficients, Jacobi CG to a 1e-8 FP64 residual (diffusion: FP32 iterations with "
-                    "FP64 reliable updates; elasticity: FP64), grouped CG + HBM-pool admission",
-        "requests": n, "elements_total": elems,
-        "capacity_solves_per_s": cap, "elems_per_s_at_capacity": elems / wall_sat,
-        "poisson_rate_per_s": 0.7 * cap, "poisson_solves_per_s": n / wall_p,
-        "p50_latency_ms": float(np.percentile(lat, 50)), "p99_latency_ms": float(np.percentile(lat, 99)),
-        "one_at_a_time_fp64_solves_per_s": n / wall_seq,
+                    "lognormal(0,1) coefficients, Jacobi CG to a 1e-8 FP64 residual, grouped CG + HBM-pool "
+                    "admission; policies: auto (diffusion FP32 iterations with FP64 reliable updates, elasticity "
+                    "FP64), mixed (every request FP32 + FP64 reliable updates), fp64",
+        "requests": n, "elements_total": sum(int(np.prod(r.n_elem)) for r in seq),
+        "capacity_solves_per_s": main["capacity_solves_per_s"],
+        "p50_latency_ms": main["p50_latency_ms"], "p99_latency_ms": main["p99_latency_ms"],
+        "policies": pol_out,
+        "one_at_a_time_fp64_solves_per_s": seq_rate,
         "one_at_a_time_p50_service_ms": float(np.percentile(service, 50)),
         "one_at_a_time_p99_service_ms": float(np.percentile(service, 99)),
-        "speedup_vs_one_at_a_time": (n / wall_sat) / (n / wall_seq),
-        "grouped_all_fp64_capacity_solves_per_s": cap64,
-        "iterations_served_vs_all_fp64": [sum(r.iterations for r in out), it64],
-        "replacement_ms": prof.get("grouped_replace", (0, 0.0))[1],
-        "max_relres": max(r.relres for r in out + out_p),
-        "server_phase_s_at_capacity": timing_sat, "server_stats": stats_sat, "roofline": roof,
+        "speedup_vs_one_at_a_time": {p: v["capacity_solves_per_s"] / seq_rate for p, v in pol_out.items()},
+        "roofline": {"kernel": "gcg_spmv (grouped SpMV over the admitted requests + p/q update + p.q), first "
+                               f"{nr} requests, policy auto", "bound": "hbm",
+                     "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "launches": n_g, "avg_launch_ms": ms_g / max(n_g, 1)},
     }
+    if "mixed" in pol_out and "fp64" in pol_out:
+        res["mixed_vs_fp64_capacity"] = pol_out["mixed"]["capacity_solves_per_s"] / \
+            pol_out["fp64"]["capacity_solves_per_s"]
+    return res
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on one GPU "
-              "(served requests would shard across GPUs without communication)", file=sys.stderr)
-        sys.exit(2)
-    sys.path.insert(0, str(ROOT))
-    import torch
-    from iron_amd import _lib, fem
-    torch.cuda.init()
-    if args.calibrate_copy:
-        a = torch.ones(1 << 26, dtype=torch.float32, device="cuda")
-        bcopy = torch.empty_like(a)
-        for _ in range(3):
-            torch.mul(a, 1.0, out=bcopy)
-        torch.cuda.synchronize()
-        del a, bcopy
-    legs = set(args.legs.split(","))
-    out = main_leg(args) if "main" in legs else {"legs": sorted(legs)}
-    torch.cuda.empty_cache()
-    if "mr" in legs and not args.no_mr:
-        out["mooney_rivlin"] = mr_leg(2, 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 10.0))
-        torch.cuda.empty_cache()
-    if "elastic" in legs:
-        out["elasticity96"] = elastic_leg()
-    if "laplace" in legs:
-        out["laplace16"] = laplace_leg(args)
-    if "slab" in legs:
-        out["diffusion48_slab"] = slab_leg(args)
-    if "serve" in legs and args.serve_requests > 0:
-        out["served_stream"] = serve_leg(args.serve_requests)
-    print(json.dumps(out))
-
-
+# ------------------------------------------------------------------ main
 def main_leg(args):
-    import torch
-    from iron_amd import _lib
+    from iron_amd import _lib, fem
     P = build_problem(args)
     red, csr = P["red"], P["csr"]
     E = P["m"].num_elements
@@ -681,10 +855,11 @@ def main_leg(args):
     n_a64, ms_a64 = prof.get("assemble_scalar", (0, 0.0))
     asm32_ms = ms_a32 / max(n_a32, 1)
     asm64_ms = ms_a64 / max(n_a64, 1)
+    pick = fem.auto_precision(1, P["S"].stored)
     out = {
         "metric": "elems/s assembled; solves/s mixed vs FP64 at 1e-8 rel resid; p50/p99",
         "value": E * args.steps / wall,
-        "unit": "elems/s (assembled and solved to 1e-8, full pass)",
+        "unit": "elems/s (assembled, solved to 1e-8 and back-substituted, full pass)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 inner / f64 outer",
@@ -693,6 +868,7 @@ def main_leg(args):
                    "elements": E, "dofs": csr.n, "free_dofs": n_f, "solver_nnz": nnz_f, "eq_nnz": csr.nnz},
         "solves_per_s_mixed": args.steps / wall,
         "solves_per_s_fp64": len(lat64) / wall64,
+        "api_precision": pick,
         "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
         "p50_ms_fp64": float(np.percentile(np.array(lat64) * 1e3, 50)),
         "assembly_elems_per_s_f32": E / (asm32_ms * 1e-3) if asm32_ms > 0 else None,
@@ -711,12 +887,48 @@ def main_leg(args):
         "sell_stored": getattr(P["S"], "stored", 0),
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(P, infos64[-1].iterations, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline_scalar(P, infos64[-1].iterations, args.cpu_seconds)
         nthr = host_threads()
         if nthr > 1:
-            out["cpu_baseline_all_cores"] = cpu_baseline(P, infos64[-1].iterations, 0.5 * args.cpu_seconds, nthr)
+            out["cpu_baseline_all_cores"] = cpu_baseline_scalar(P, infos64[-1].iterations, 0.5 * args.cpu_seconds,
+                                                                nthr)
     del P
     return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(served requests would shard across GPUs without communication)", file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, str(ROOT))
+    import torch
+    torch.cuda.init()
+    if args.calibrate_copy:
+        a = torch.ones(1 << 26, dtype=torch.float32, device="cuda")
+        bcopy = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=bcopy)
+        torch.cuda.synchronize()
+        del a, bcopy
+    legs = set(args.legs.split(","))
+    cpu = 0.0 if args.no_cpu_baseline else args.cpu_seconds
+    out = main_leg(args) if "main" in legs else {"legs": sorted(legs)}
+    torch.cuda.empty_cache()
+    if "mr" in legs and not args.no_mr:
+        out["mooney_rivlin"] = mr_leg(2, min(cpu, 8.0))
+        torch.cuda.empty_cache()
+    if "elastic" in legs:
+        out["elasticity96"] = elastic_leg(cpu_seconds=min(cpu, 10.0))
+        torch.cuda.empty_cache()
+    if "laplace" in legs:
+        out["laplace16"] = laplace_leg(args, cpu_seconds=min(cpu, 4.0))
+    if "slab" in legs:
+        out["diffusion48_slab"] = slab_leg(args)
+    if "serve" in legs and args.serve_requests > 0:
+        out["served_stream"] = serve_leg(args.serve_requests, args.serve_policies.split(","))
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -222,6 +222,30 @@ int mife_assemble_elastic(const mife_element_desc* desc, int n_elem,
                           const int32_t* colour_ptr_host, int n_colours, int prec,
                           void* vals, void* stream);
 
+/* Structured assembly on a generated regular 3-D mesh: n_elem[3] elements per
+ * direction, elements and nodes numbered x fastest (iron's generated mesh,
+ * src/generated_mesh_routines.F90:1568-1612), one component (a scalar field:
+ * Laplace / generalised Laplace / Poisson, coef = per-element conductivity or
+ * NULL, source = per-element f or NULL) or three (linear elasticity, coef =
+ * material [E][6], no source) in separated DOF order, trilinear (8 nodes, 2x2x2
+ * Gauss) or, for one component, triquadratic (27 nodes, 3x3x3 Gauss) elements.
+ * Replaces the element loop with EquationsMatrices_ElementAdd
+ * (src/equations_matrices_routines.F90:1140-1265, MATRIX_VALUES_ADD_DP2
+ * src/matrix_vector.F90:3119): the element kernels store every element matrix in
+ * `stage` (one launch), then each CSR row (row_ptr over n_rows = n_comp x nodes
+ * rows, iron's sorted pattern from mife_csr_pattern) is written whole: entry by
+ * entry the element entries that meet there, summed in the element-colour order
+ * of the coloured scatter, so vals (and src_vec) equal bit for bit what
+ * mife_assemble_scalar / mife_assemble_elastic with the structured colouring add
+ * into zeroed arrays.  vals and src_vec are overwritten, not added to; no
+ * position map is read.  stage: mife_assemble_grid_stage_bytes(E, nodes per
+ * element, n_comp, prec) bytes of device memory (the FP64 size serves both). */
+size_t mife_assemble_grid_stage_bytes(int n_elem, int n_nodes, int n_comp, int prec);
+int mife_assemble_grid(const mife_element_desc* desc, const int32_t* n_elem, int n_comp,
+                       const double* coords, const int32_t* elem_nodes, const double* coef,
+                       const double* source, const int32_t* row_ptr, int64_t n_rows, int prec,
+                       void* vals, double* src_vec, void* stage, size_t stage_bytes, void* stream);
+
 /* Batched assembly: the CSR values of many independent trilinear meshes (the
  * requests a server admits together) in one launch per colour and element
  * kind.  Each job is one mesh with the arguments of mife_assemble_scalar

@@ -99,6 +99,9 @@ _SIG = {
     "mife_reduce_pattern": (_I, [_I, _P, _P, _P, _I, _P, _P, _P, C.POINTER(_I64), _P, _P]),
     "mife_assemble_scalar": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P]),
     "mife_assemble_elastic": (_I, [C.POINTER(ElementDesc), _I, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P]),
+    "mife_assemble_grid_stage_bytes": (C.c_size_t, [_I, _I, _I, _I]),
+    "mife_assemble_grid": (_I, [C.POINTER(ElementDesc), _P, _I, _P, _P, _P, _P, _P, _I64, _I, _P, _P, _P, _SZ,
+                                _P]),
     "mife_spmv": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _P]),
     "mife_sell_permutation": (_I, [_I, _P, _P, _I, _P, _P]),
     "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),

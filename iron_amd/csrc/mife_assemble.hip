@@ -101,13 +101,16 @@ struct ScalarJob {
   const int32_t* edofs;
   const double* kappa;
   const double* source;
-  const int32_t* pos;
+  const int32_t* pos;   // NULL: stage the element matrix at vals[e][NN][NN]
   T* vals;
   double* svec;
+  double* sstage;       // non-NULL: stage the element source vector at sstage[e][NN]
 };
 
 // One element per wavefront (element e when `active`); every wavefront of the
-// block calls it the same number of times (it holds block barriers).
+// block calls it the same number of times (it holds block barriers).  Without a
+// position map the element matrix is stored (not added) in element order, for the
+// row-gather pass of a structured assembly.
 template <int DIM, int NN, int NG, typename T, int WPB>
 __device__ __forceinline__ void scalar_element(const ScalarJob<T>& J, long long e, bool active,
                                                const double* __restrict__ dN, const double* __restrict__ N,
@@ -142,17 +145,22 @@ __device__ __forceinline__ void scalar_element(const ScalarJob<T>& J, long long 
       T acc = 0;
 #pragma unroll 9
       for (int k = 0; k < KD; ++k) acc += W[k] * B[k * NN + m] * B[k * NN + n];
-      const int p = J.pos[e * (NN * NN) + t];
-      add_value(&J.vals[p], acc, atomic);
+      if (J.pos)
+        add_value(&J.vals[J.pos[e * (NN * NN) + t]], acc, atomic);
+      else
+        J.vals[e * (NN * NN) + t] = acc;
     }
-    if (J.source && J.svec && lane < NN) {
+    if (J.source && (J.svec || J.sstage) && lane < NN) {
       // Rounded product and sum, unfused whatever the caller inlines around them
       // (a kernel whose `atomic` is a constant would otherwise fuse them).
 #pragma clang fp contract(off)
       double s = 0.0;
       for (int g = 0; g < NG; ++g) s = fma(N[g * NN + lane], G.rwg[g], s);
       s = s * J.source[e];
-      add_value(&J.svec[J.edofs[e * NN + lane]], s, atomic);
+      if (J.sstage)
+        J.sstage[e * NN + lane] = s;
+      else
+        add_value(&J.svec[J.edofs[e * NN + lane]], s, atomic);
     }
   }
 }
@@ -163,13 +171,13 @@ __global__ __launch_bounds__(64 * WPB) void scalar_assemble_kernel(
     const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
     const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
-    bool atomic) {
+    bool atomic, double* __restrict__ sstage = nullptr) {
   const int idx = blockIdx.x * WPB + (threadIdx.x >> 6);
   const bool active = idx < elem_count;
   long long e = 0;
   if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
-  scalar_element<DIM, NN, NG, T, WPB>(ScalarJob<T>{coords, enodes, edofs, kappa, source, pos, vals, svec}, e,
-                                      active, dN, N, w, atomic);
+  scalar_element<DIM, NN, NG, T, WPB>(ScalarJob<T>{coords, enodes, edofs, kappa, source, pos, vals, svec, sstage},
+                                      e, active, dN, N, w, atomic);
 }
 
 // ------------------------------------------------ triquadratic, on MFMA
@@ -206,7 +214,7 @@ __global__ __launch_bounds__(64 * WPB) void q2_assemble_mfma_kernel(
     const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
     const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
-    bool atomic) {
+    bool atomic, double* __restrict__ sstage = nullptr) {
   constexpr int NN = 27, NG = 27, KD = 81, STEPS = 21;
   using M = Mfma16<T>;
   __shared__ double sdN[NG * NN * 3];
@@ -239,25 +247,43 @@ __global__ __launch_bounds__(64 * WPB) void q2_assemble_mfma_kernel(
     k01 = M::mac(a0, a1 * wk, k01);
     k11 = M::mac(a1, a1 * wk, k11);
   }
-  const int32_t* pe = pos + e * (NN * NN);
+  if (pos) {
+    const int32_t* pe = pos + e * (NN * NN);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = M::row(lane, i);
-    // tile (0,0): m = r, n = col
-    add_value(&vals[pe[r * NN + col]], k00[i], atomic);
-    // tile (0,1) and its transpose: m = r, n = 16 + col
-    if (16 + col < NN) {
-      add_value(&vals[pe[r * NN + 16 + col]], k01[i], atomic);
-      add_value(&vals[pe[(16 + col) * NN + r]], k01[i], atomic);
+    for (int i = 0; i < 4; ++i) {
+      const int r = M::row(lane, i);
+      // tile (0,0): m = r, n = col
+      add_value(&vals[pe[r * NN + col]], k00[i], atomic);
+      // tile (0,1) and its transpose: m = r, n = 16 + col
+      if (16 + col < NN) {
+        add_value(&vals[pe[r * NN + 16 + col]], k01[i], atomic);
+        add_value(&vals[pe[(16 + col) * NN + r]], k01[i], atomic);
+      }
+      // tile (1,1): m = 16 + r, n = 16 + col
+      if (16 + r < NN && 16 + col < NN) add_value(&vals[pe[(16 + r) * NN + 16 + col]], k11[i], atomic);
     }
-    // tile (1,1): m = 16 + r, n = 16 + col
-    if (16 + r < NN && 16 + col < NN) add_value(&vals[pe[(16 + r) * NN + 16 + col]], k11[i], atomic);
+  } else {
+    // the element matrix, stored in element order for the row-gather pass
+    T* ke_out = vals + e * (NN * NN);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = M::row(lane, i);
+      ke_out[r * NN + col] = k00[i];
+      if (16 + col < NN) {
+        ke_out[r * NN + 16 + col] = k01[i];
+        ke_out[(16 + col) * NN + r] = k01[i];
+      }
+      if (16 + r < NN && 16 + col < NN) ke_out[(16 + r) * NN + 16 + col] = k11[i];
+    }
   }
-  if (source && svec && lane < NN) {
+  if (source && (svec || sstage) && lane < NN) {
     double s = 0.0;
     for (int g = 0; g < NG; ++g) s += N[g * NN + lane] * G.rwg[g];
     s *= source[e];
-    add_value(&svec[edofs[e * NN + lane]], s, atomic);
+    if (sstage)
+      sstage[e * NN + lane] = s;
+    else
+      add_value(&svec[edofs[e * NN + lane]], s, atomic);
   }
 }
 
@@ -406,7 +432,7 @@ __device__ __forceinline__ void elastic_element(const ElasticJob<T>& J, long lon
   }
   __syncthreads();
   if (!active) return;  // no barrier follows
-  const int32_t* pe = J.pos + e * (ND * ND);
+  const int32_t* pe = J.pos ? J.pos + e * (ND * ND) : nullptr;
   const int col = lane & 15;
   for (int mt = 0; mt < MT; ++mt) {
     const int mrow = 16 * mt + col;
@@ -437,7 +463,11 @@ __device__ __forceinline__ void elastic_element(const ElasticJob<T>& J, long lon
       const int orow = 16 * mt + M::row(lane, r);
       if (orow < MROWS && col < NN) {
         const int oi = orow / (3 * NN), oj = (orow / NN) % 3, oa = orow % NN;
-        add_value(&J.vals[pe[(oi * NN + oa) * ND + oj * NN + col]], acc[r], atomic);
+        const int t = (oi * NN + oa) * ND + oj * NN + col;
+        if (pe)
+          add_value(&J.vals[pe[t]], acc[r], atomic);
+        else
+          J.vals[e * (ND * ND) + t] = acc[r];   // element matrix in element order
       }
     }
   }
@@ -511,7 +541,7 @@ __global__ __launch_bounds__(64 * WPB) void scalar_batch_kernel(const BatchJob* 
     if (active) {
       const BatchSeg sg = batch_seg(segs, nseg, idx);
       const BatchJob B = jobs[sg.job];
-      J = ScalarJob<T>{B.coords, B.enodes, B.edofs, B.coef, B.source, B.pos, (T*)B.vals, B.svec};
+      J = ScalarJob<T>{B.coords, B.enodes, B.edofs, B.coef, B.source, B.pos, (T*)B.vals, B.svec, nullptr};
       e = B.order[sg.begin + idx - sg.prefix];
     }
     scalar_element<3, 8, 8, T, WPB>(J, e, active, dN, N, w, false);
@@ -598,9 +628,200 @@ static int launch_elastic(const mife_element_desc* d, int n_elem, const double* 
   return MIFE_OK;
 }
 
+// --------------------------------------------- structured (grid) assembly
+// On a generated regular mesh (elements and nodes numbered x fastest, as iron's
+// generated mesh numbers them; one or three components in separated DOF order)
+// every CSR row is the tensor product of 1-D neighbour ranges, so a row and the
+// elements that touch it follow from the node's grid index.  The assembly then
+// runs as two passes: the element kernels store every element matrix in element
+// order (one launch, no colouring), and this kernel writes each CSR row whole --
+// entry by entry the sum of the element entries that meet there, in the order of
+// the element colours (parity of the element index per direction), which is the
+// order in which the coloured scatter adds them, so the values are bit for bit
+// those of the coloured scatter.  No position map is read and no value is read
+// back: each row is one contiguous store.
+//
+// 1-D element range of node i (order P, n elements): elements ex with
+// P ex <= i <= P ex + P, clipped to [0, n - 1].
+__device__ __forceinline__ void elem_range(int i, int P, int n, int& lo, int& hi) {
+  lo = i > 0 ? (i - 1) / P : 0;
+  hi = min(i / P, n - 1);
+}
+
+// Element of [lo, hi] (one or two consecutive elements) whose index parity is
+// `bit`, or -1.
+__device__ __forceinline__ int elem_of_parity(int lo, int hi, int bit) {
+  if ((lo & 1) == bit) return lo;
+  if (hi != lo && (hi & 1) == bit) return hi;
+  return -1;
+}
+
+// One wavefront per CSR row (WPB rows per block).  stage: element matrices
+// [E][NC NN][NC NN]; sstage (NC == 1, may be NULL): element source vectors [E][NN].
+// Entries past a row's stored length are not written.
+template <typename T, int P, int NC, int WPB>
+__global__ __launch_bounds__(64 * WPB) void grid_gather_kernel(int nx, int ny, int nz,
+                                                               const int32_t* __restrict__ row_ptr,
+                                                               const T* __restrict__ stage, T* __restrict__ vals,
+                                                               const double* __restrict__ sstage,
+                                                               double* __restrict__ svec) {
+  constexpr int NB = P + 1, NN = NB * NB * NB, ND = NC * NN;
+  const int Nx = P * nx + 1, Ny = P * ny + 1, Nz = P * nz + 1;
+  const long long Nn = (long long)Nx * Ny * Nz;
+  const long long row = (long long)blockIdx.x * WPB + (threadIdx.x >> 6);
+  if (row >= NC * Nn) return;
+  const int lane = threadIdx.x & 63;
+  const int c = (int)(row / Nn);
+  const long long node = row - c * Nn;
+  const int ix = (int)(node % Nx), iy = (int)((node / Nx) % Ny), iz = (int)(node / ((long long)Nx * Ny));
+  int lx, hx, ly, hy, lz, hz;
+  elem_range(ix, P, nx, lx, hx);
+  elem_range(iy, P, ny, ly, hy);
+  elem_range(iz, P, nz, lz, hz);
+  const int jx0 = P * lx, jy0 = P * ly, jz0 = P * lz;
+  const int cx = P * hx + P - jx0 + 1, cy = P * hy + P - jy0 + 1, cz = P * hz + P - jz0 + 1;
+  const int cnt = cx * cy * cz;
+  const int base = row_ptr[row];
+  const int len = row_ptr[row + 1] - base;
+  for (int p = lane; p < NC * cnt && p < len; p += 64) {
+    const int c2 = p / cnt, q = p - c2 * cnt;
+    const int jx = jx0 + q % cx, jy = jy0 + (q / cx) % cy, jz = jz0 + q / (cx * cy);
+    int ljx, hjx, ljy, hjy, ljz, hjz;
+    elem_range(jx, P, nx, ljx, hjx);
+    elem_range(jy, P, ny, ljy, hjy);
+    elem_range(jz, P, nz, ljz, hjz);
+    // elements holding both nodes, per direction
+    const int ax = max(lx, ljx), bx = min(hx, hjx), ay = max(ly, ljy), by = min(hy, hjy);
+    const int az = max(lz, ljz), bz = min(hz, hjz);
+    T sum = (T)0;
+#pragma unroll
+    for (int col = 0; col < 8; ++col) {
+      const int ex = elem_of_parity(ax, bx, col & 1), ey = elem_of_parity(ay, by, (col >> 1) & 1),
+                ez = elem_of_parity(az, bz, col >> 2);
+      if (ex < 0 || ey < 0 || ez < 0) continue;
+      const long long e = ex + (long long)nx * (ey + (long long)ny * ez);
+      const int a = (ix - P * ex) + NB * ((iy - P * ey) + NB * (iz - P * ez));
+      const int b = (jx - P * ex) + NB * ((jy - P * ey) + NB * (jz - P * ez));
+      sum += stage[e * (ND * ND) + (c * NN + a) * ND + c2 * NN + b];
+    }
+    vals[base + p] = sum;
+  }
+  if (NC == 1 && sstage && lane == 0) {
+    double sv = 0.0;
+#pragma unroll
+    for (int col = 0; col < 8; ++col) {
+      const int ex = elem_of_parity(lx, hx, col & 1), ey = elem_of_parity(ly, hy, (col >> 1) & 1),
+                ez = elem_of_parity(lz, hz, col >> 2);
+      if (ex < 0 || ey < 0 || ez < 0) continue;
+      const long long e = ex + (long long)nx * (ey + (long long)ny * ez);
+      const int a = (ix - P * ex) + NB * ((iy - P * ey) + NB * (iz - P * ez));
+      sv += sstage[e * NN + a];
+    }
+    svec[row] = sv;
+  }
+}
+
+template <typename T>
+static size_t grid_stage_bytes(long long E, int nn, int nc) {
+  const size_t nd = (size_t)nc * nn;
+  return ((size_t)E * nd * nd * sizeof(T) + 255) / 256 * 256 + (nc == 1 ? (size_t)E * nn * sizeof(double) : 0);
+}
+
+template <typename T, int P, int NC>
+static int launch_grid(const mife_element_desc* d, const int* ne, const double* coords, const int32_t* enodes,
+                       const double* coef, const double* source, const int32_t* row_ptr, long long n_rows, T* vals,
+                       double* svec, void* stage, hipStream_t s) {
+  constexpr int NB = P + 1, NN = NB * NB * NB, ND = NC * NN;
+  const long long E = (long long)ne[0] * ne[1] * ne[2];
+  T* kst = (T*)stage;
+  double* sst = (NC == 1 && source && svec)
+      ? (double*)((char*)stage + ((size_t)E * ND * ND * sizeof(T) + 255) / 256 * 256)
+      : nullptr;
+  const int tk = prof::begin(s);
+  constexpr int WQ = 4;
+  const int grid_e = div_up(E, WQ);
+  if constexpr (NC == 1) {
+    if constexpr (NN == 27) {
+      q2_assemble_mfma_kernel<T, WQ><<<grid_e, 64 * WQ, 0, s>>>(coords, enodes, nullptr, coef, source, nullptr,
+                                                                nullptr, 0, (int)E, d->dN, d->N, d->w, kst,
+                                                                nullptr, false, sst);
+    } else {
+      scalar_assemble_kernel<3, NN, NN, T, 4><<<grid_e, 64 * 4, 0, s>>>(coords, enodes, nullptr, coef, source,
+                                                                        nullptr, nullptr, 0, (int)E, d->dN, d->N,
+                                                                        d->w, kst, nullptr, false, sst);
+    }
+  } else {
+    elastic_assemble_mfma_kernel<T, NN, NN, WQ><<<grid_e, 64 * WQ, 0, s>>>(coords, enodes, coef, nullptr, nullptr,
+                                                                           0, (int)E, d->dN, d->w, kst, false);
+  }
+  constexpr int WR = 4;
+  grid_gather_kernel<T, P, NC, WR><<<div_up(n_rows, WR), 64 * WR, 0, s>>>(ne[0], ne[1], ne[2], row_ptr, kst, vals,
+                                                                          sst, sst ? svec : nullptr);
+  const int key = NC == 1 ? (sizeof(T) == 8 ? prof::kAssembleScalar : prof::kAssembleScalar32)
+                          : (sizeof(T) == 8 ? prof::kAssembleElastic : prof::kAssembleElastic32);
+  prof::end(tk, s, key);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
 }  // namespace mife
 
 using namespace mife;
+
+extern "C" size_t mife_assemble_grid_stage_bytes(int n_elem, int n_nodes, int n_comp, int prec) {
+  if (n_elem <= 0 || n_nodes <= 0 || (n_comp != 1 && n_comp != 3)) return 0;
+  return prec == MIFE_F32 ? grid_stage_bytes<float>(n_elem, n_nodes, n_comp)
+                          : grid_stage_bytes<double>(n_elem, n_nodes, n_comp);
+}
+
+extern "C" int mife_assemble_grid(const mife_element_desc* d, const int32_t* n_elem, int n_comp,
+                                  const double* coords, const int32_t* elem_nodes, const double* coef,
+                                  const double* source, const int32_t* row_ptr, int64_t n_rows, int prec, void* vals,
+                                  double* src_vec, void* stage, size_t stage_bytes, void* stream) {
+  MIFE_REQUIRE(d && n_elem && coords && elem_nodes && row_ptr && vals && stage && d->dN && d->w,
+               MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(d->dim == 3, MIFE_ERR_ARG, "the grid assembly is 3-D");
+  MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
+  MIFE_REQUIRE(n_comp == 1 || n_comp == 3, MIFE_ERR_ARG, "%d components (1: scalar field, 3: elasticity)", n_comp);
+  MIFE_REQUIRE(n_comp == 1 || coef, MIFE_ERR_POINTER_IS_NULL, "elasticity without material");
+  MIFE_REQUIRE(n_comp == 1 || !source, MIFE_ERR_ARG, "a source vector is a scalar-field term");
+  MIFE_REQUIRE(!source || !src_vec || d->N, MIFE_ERR_POINTER_IS_NULL, "source given without basis values N");
+  MIFE_REQUIRE(n_elem[0] > 0 && n_elem[1] > 0 && n_elem[2] > 0, MIFE_ERR_ARG, "empty grid");
+  int P = 0;
+  if (d->n_nodes == 8 && d->n_gauss == 8) P = 1;
+  if (d->n_nodes == 27 && d->n_gauss == 27 && n_comp == 1) P = 2;
+  MIFE_REQUIRE(P > 0, MIFE_ERR_ARG, "grid assembly element: %d nodes, %d Gauss points, %d components", d->n_nodes,
+               d->n_gauss, n_comp);
+  const long long E = (long long)n_elem[0] * n_elem[1] * n_elem[2];
+  const long long Nn = (long long)(P * n_elem[0] + 1) * (P * n_elem[1] + 1) * (P * n_elem[2] + 1);
+  MIFE_REQUIRE(E < (1LL << 31) && n_rows == n_comp * Nn, MIFE_ERR_ARG,
+               "%lld rows given, the grid has %lld nodes x %d components", (long long)n_rows, Nn, n_comp);
+  const size_t need = prec == MIFE_F32 ? grid_stage_bytes<float>(E, d->n_nodes, n_comp)
+                                       : grid_stage_bytes<double>(E, d->n_nodes, n_comp);
+  MIFE_REQUIRE(stage_bytes >= need, MIFE_ERR_ARG, "stage of %zu bytes, %zu needed (mife_assemble_grid_stage_bytes)",
+               stage_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  const int ne[3] = {n_elem[0], n_elem[1], n_elem[2]};
+  if (n_comp == 3) {
+    if (prec == MIFE_F64)
+      return launch_grid<double, 1, 3>(d, ne, coords, elem_nodes, coef, nullptr, row_ptr, n_rows, (double*)vals,
+                                       nullptr, stage, s);
+    return launch_grid<float, 1, 3>(d, ne, coords, elem_nodes, coef, nullptr, row_ptr, n_rows, (float*)vals, nullptr,
+                                    stage, s);
+  }
+  if (P == 2) {
+    if (prec == MIFE_F64)
+      return launch_grid<double, 2, 1>(d, ne, coords, elem_nodes, coef, source, row_ptr, n_rows, (double*)vals,
+                                       src_vec, stage, s);
+    return launch_grid<float, 2, 1>(d, ne, coords, elem_nodes, coef, source, row_ptr, n_rows, (float*)vals, src_vec,
+                                    stage, s);
+  }
+  if (prec == MIFE_F64)
+    return launch_grid<double, 1, 1>(d, ne, coords, elem_nodes, coef, source, row_ptr, n_rows, (double*)vals,
+                                     src_vec, stage, s);
+  return launch_grid<float, 1, 1>(d, ne, coords, elem_nodes, coef, source, row_ptr, n_rows, (float*)vals, src_vec,
+                                  stage, s);
+}
 
 extern "C" int mife_assemble_scalar(const mife_element_desc* d, int n_elem, const double* coords,
                                     const int32_t* elem_nodes, const int32_t* elem_dofs, const double* kappa,

@@ -348,8 +348,10 @@ def _prepare_linear(es, solver, bc, others=(), force_prec=None):
     v64 = c.csr.zeros(fem.F64)
     v32 = c.csr.zeros(fem.F32) if mixed else None
     src = None
+    # generated regular meshes in 3-D take the structured assembly (no position map)
+    grid = rm.n_elem if rm.dim == 3 else None
     if kind in ("laplace", "generalised_laplace", "poisson"):
-        asm = fem.ScalarAssembler(b, coords, rm.elem_nodes[mc], c.csr)
+        asm = fem.ScalarAssembler(b, coords, rm.elem_nodes[mc], c.csr, grid=grid)
         kappa = None
         source = None
         if kind == "generalised_laplace":
@@ -372,7 +374,7 @@ def _prepare_linear(es, solver, bc, others=(), force_prec=None):
     elif kind == "linear_elasticity":
         if rm.dim != 3:
             raise CMFEError(1, "linear elasticity is implemented for the 3-D subtype")
-        asm = fem.ElasticAssembler(b, coords, rm.elem_nodes[mc], c.csr)
+        asm = fem.ElasticAssembler(b, coords, rm.elem_nodes[mc], c.csr, grid=grid)
         mf = es.materialsField
         mat = np.stack([_element_values(mf, j, rm) for j in range(1, 7)], axis=1)
         mt = torch.as_tensor(np.ascontiguousarray(mat), device=dev)

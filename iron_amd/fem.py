@@ -74,9 +74,7 @@ class CSRStructure:
         call("mife_csr_pattern", ptr(self.elem_dofs), self.n_elem, self.n_edof, self.n, ptr(self.row_ptr),
              ptr(self.col), None, ptr(work), s)
         del work
-        self.pos = torch.empty(self.n_elem * self.n_edof * self.n_edof, dtype=torch.int32, device=dev)
-        call("mife_scatter_map", ptr(self.elem_dofs), self.n_elem, self.n_edof, ptr(self.row_ptr), ptr(self.col),
-             ptr(self.pos), s)
+        self._pos = None
         if colours is None:
             col_h = np.empty(self.n_elem, dtype=np.int32)
             nc = C.c_int(0)
@@ -90,6 +88,16 @@ class CSRStructure:
         self.colour_ptr[1:] = np.cumsum(counts)
         self.n_colours = len(counts)
         self.order = torch.as_tensor(order, device=dev)
+
+    @property
+    def pos(self) -> torch.Tensor:
+        """Per-element CSR position map [E][n_edof][n_edof] (built on first use: the
+        structured grid assembly does without it)."""
+        if self._pos is None:
+            self._pos = torch.empty(self.n_elem * self.n_edof * self.n_edof, dtype=torch.int32, device=self.device)
+            call("mife_scatter_map", ptr(self.elem_dofs), self.n_elem, self.n_edof, ptr(self.row_ptr), ptr(self.col),
+                 ptr(self._pos), _stream())
+        return self._pos
 
     def zeros(self, prec=F64):
         return torch.zeros(max(self.nnz, 1), dtype=torch.float64 if prec == F64 else torch.float32, device=self.device)
@@ -497,19 +505,54 @@ def backsubstitute(csr: CSRStructure, vals64, u, src=None, rhs_fixed=None, iron_
     return out
 
 
-class ScalarAssembler:
-    """Laplace / generalised Laplace / constant-source Poisson on one basis."""
+class _GridStage:
+    """Structured assembly on a generated regular grid (mife_assemble_grid): element
+    matrices staged in one launch, then every CSR row written whole.  Holds the stage
+    buffer (sized for FP64, shared by both precisions)."""
 
-    def __init__(self, basis: LagrangeBasis, coords: np.ndarray, elem_nodes: np.ndarray, csr: CSRStructure):
+    def __init__(self, grid, basis: LagrangeBasis, n_comp: int, csr: CSRStructure):
+        self.grid = np.ascontiguousarray(grid, dtype=np.int32)
+        self.n_comp = n_comp
+        self.csr = csr
+        E = int(np.prod(self.grid))
+        self.nbytes = int(_lib.lib().mife_assemble_grid_stage_bytes(E, basis.num_nodes, n_comp, F64))
+        self.buf = None
+
+    def run(self, desc, coords, enodes, coef, source, prec, vals, src_vec):
+        if self.buf is None:
+            self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=self.csr.device)
+        call("mife_assemble_grid", C.byref(desc), self.grid.ctypes.data, self.n_comp, ptr(coords), ptr(enodes),
+             ptr(coef), ptr(source), ptr(self.csr.row_ptr), self.csr.n, prec, ptr(vals), ptr(src_vec),
+             ptr(self.buf), self.nbytes, _stream())
+
+
+def _grid_ok(grid, basis: LagrangeBasis) -> bool:
+    return grid is not None and basis.num_xi == 3 and (basis.num_nodes, basis.num_gauss) in ((8, 8), (27, 27))
+
+
+class ScalarAssembler:
+    """Laplace / generalised Laplace / constant-source Poisson on one basis.
+
+    ``grid`` (elements per direction of a generated regular mesh, numbered as iron's
+    generator numbers them) selects the structured assembly, which writes the CSR
+    values and the source vector whole (bit for bit the coloured scatter's sums);
+    otherwise the element kernels add into ``vals`` through the position map."""
+
+    def __init__(self, basis: LagrangeBasis, coords: np.ndarray, elem_nodes: np.ndarray, csr: CSRStructure,
+                 grid=None):
         dev = csr.device
         self.csr = csr
         self.db = DeviceBasis(basis, dev)
         self.coords = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64), device=dev)
         self.elem_nodes = torch.as_tensor(np.ascontiguousarray(elem_nodes, dtype=np.int32), device=dev)
         self.n_elem = elem_nodes.shape[0]
+        self.grid = _GridStage(grid, basis, 1, csr) if _grid_ok(grid, basis) else None
 
     def assemble(self, vals, kappa=None, source=None, src_vec=None, prec=F64, atomic=False):
         cs = self.csr
+        if self.grid is not None and not atomic:
+            self.grid.run(self.db.desc, self.coords, self.elem_nodes, kappa, source, prec, vals, src_vec)
+            return vals
         call("mife_assemble_scalar", C.byref(self.db.desc), self.n_elem, ptr(self.coords), ptr(self.elem_nodes),
              ptr(cs.elem_dofs), ptr(kappa), ptr(source), ptr(cs.pos), None if atomic else ptr(cs.order),
              None if atomic else cs.colour_ptr.ctypes.data, cs.n_colours, prec, ptr(vals), ptr(src_vec), _stream())
@@ -517,18 +560,25 @@ class ScalarAssembler:
 
 
 class ElasticAssembler:
-    """3-D linear elasticity on one basis (all three components on it)."""
+    """3-D linear elasticity on one basis (all three components on it); ``grid`` as
+    for ScalarAssembler."""
 
-    def __init__(self, basis: LagrangeBasis, coords: np.ndarray, elem_nodes: np.ndarray, csr: CSRStructure):
+    def __init__(self, basis: LagrangeBasis, coords: np.ndarray, elem_nodes: np.ndarray, csr: CSRStructure,
+                 grid=None):
         dev = csr.device
         self.csr = csr
         self.db = DeviceBasis(basis, dev)
         self.coords = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64), device=dev)
         self.elem_nodes = torch.as_tensor(np.ascontiguousarray(elem_nodes, dtype=np.int32), device=dev)
         self.n_elem = elem_nodes.shape[0]
+        ok = _grid_ok(grid, basis) and basis.num_nodes == 8
+        self.grid = _GridStage(grid, basis, 3, csr) if ok else None
 
     def assemble(self, vals, material, prec=F64, atomic=False):
         cs = self.csr
+        if self.grid is not None and not atomic:
+            self.grid.run(self.db.desc, self.coords, self.elem_nodes, material, None, prec, vals, None)
+            return vals
         call("mife_assemble_elastic", C.byref(self.db.desc), self.n_elem, ptr(self.coords), ptr(self.elem_nodes),
              ptr(cs.elem_dofs), ptr(material), ptr(cs.pos), None if atomic else ptr(cs.order),
              None if atomic else cs.colour_ptr.ctypes.data, cs.n_colours, prec, ptr(vals), _stream())

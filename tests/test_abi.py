@@ -102,6 +102,20 @@ def test_argument_errors_without_gpu():
         assert L.mife_gmres(ctypes.byref(d), 1, 1, 1, None, 1e-8, R, 10, 256, wr - 256, None, None, None) == 2
         assert "GMRES needs a workspace" in _lib.last_error()
     assert L.mife_gmres(ctypes.byref(d), 1, 1, 1, None, 1e-8, 257, 10, 1, 1 << 40, None, None, None) == 2
+    # structured assembly: row count and stage size are checked on the host
+    desc = _lib.ElementDesc(3, 27, 27, 1, 1, 1)
+    ne = (ctypes.c_int32 * 3)(4, 3, 2)
+    rows = 9 * 7 * 5
+    need = L.mife_assemble_grid_stage_bytes(24, 27, 1, 0)
+    assert need >= 24 * 27 * 27 * 8
+    assert L.mife_assemble_grid(ctypes.byref(desc), ne, 1, 1, 1, None, None, 1, rows + 1, 0, 1, None, 1, need,
+                                None) == 2
+    assert "rows" in _lib.last_error()
+    assert L.mife_assemble_grid(ctypes.byref(desc), ne, 1, 1, 1, None, None, 1, rows, 0, 1, None, 1, need - 1,
+                                None) == 2
+    assert "stage" in _lib.last_error()
+    assert L.mife_assemble_grid(ctypes.byref(desc), ne, 3, 1, 1, 1, None, 1, 3 * rows, 0, 1, None, 1, 1 << 40,
+                                None) == 2   # no triquadratic elasticity
     # 8 ranges of ceil(ceil(n_slices / 8) / 4) * 4 entries
     assert L.mife_sell_order_len(0) == 0 and L.mife_sell_order_len(1) == 32
     assert L.mife_sell_order_len(13397) == 8 * 1676   # ceil(13397 / 8) = 1675 -> 1676
