@@ -440,6 +440,20 @@ int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int 
             double rtol, int maxit, void* work, size_t work_bytes, int* iters, double* relres,
             void* stream);
 
+/* mife_cg with no allocation and no workspace argument: the solve's buffers
+ * (two solver states of 524,544 bytes and seven vectors of n values, each
+ * 256-byte aligned) live in a 64 MiB array that the library holds on each GPU it
+ * runs on until the process ends, so at most 1,179,616 rows in FP64 or
+ * 2,359,232 in FP32; larger systems return MIFE_ERR_ARG before any device work.
+ * The call makes no hipMalloc / hipFree or other allocating call.  All calls on one
+ * GPU share that array and it keeps the last call's contents (every buffer is
+ * written before it is read): calls that overlap on one GPU, from several host
+ * threads or streams, race and must be serialised by the caller.  Results are those
+ * of mife_cg.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc). */
+int mife_cg_noalloc(const mife_matrix* A, const void* vals, const void* b, void* x, int prec,
+                    double rtol, int maxit, int* iters, double* relres, void* stream);
+
 /* Mixed-precision solve: Jacobi CG on the FP32 operator vals32 whose residual is
  * replaced by the FP64 residual r = b - A64 x each time the FP32 recurrence
  * residual has fallen by the factor inner_rtol since the last replacement

@@ -120,6 +120,7 @@ _SIG = {
     "mife_backsubstitute": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _P, _P]),
     "mife_solver_work_bytes": (C.c_size_t, [_I, _I]),
     "mife_cg": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_D), _P]),
+    "mife_cg_noalloc": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _D, _I, C.POINTER(_I), C.POINTER(_D), _P]),
     "mife_refine": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _D, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_I),
                          C.POINTER(_D), _P]),
     "mife_gmres": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _P, _D, _I, _I, _P, _SZ, C.POINTER(_I), C.POINTER(_D),

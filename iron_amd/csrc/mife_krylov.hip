@@ -21,6 +21,7 @@ extern "C" int mife_profile_enable(int on);
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <mutex>
 #include <vector>
 
 namespace mife {
@@ -1060,11 +1061,56 @@ static int cg_entry(const mife_matrix* A, const void* vals, const void* b, void*
   return MIFE_OK;
 }
 
+// ------------------------------------------ CG without any allocation
+// The workspace of mife_cg_noalloc: a file-scope device array, one copy per GPU
+// for the life of the process.  Host code never uses the name `scratch` as an
+// address; it takes the per-device address from hipGetSymbolAddress once, with
+// that device current, and passes that pointer to the kernels.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int scratch_of_current_device(void** out) {
+  int dev = -1;
+  MIFE_CHECK_HIP(hipGetDevice(&dev));
+  MIFE_REQUIRE(dev >= 0 && dev < kMaxDevices, MIFE_ERR_ARG, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MIFE_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MIFE_REQUIRE(p != nullptr, MIFE_ERR_HIP, "no device address for the scratch array on device %d", dev);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MIFE_OK;
+}
+
 }  // namespace mife
 
 using namespace mife;
 
 extern "C" {
+
+int mife_cg_noalloc(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol,
+                    int maxit, int* iters, double* relres, void* stream) {
+  MIFE_REQUIRE(A && vals && b && x, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
+  MIFE_REQUIRE(A->n > 0, MIFE_ERR_ARG, "empty matrix");
+  // the layout at any 256-byte aligned base, checked before the device is touched
+  const size_t need = prec == MIFE_F64
+      ? layout_bytes([&](Work& w) { CgBufs<double> B; cg_layout(w, A->n, B); }) - 255
+      : layout_bytes([&](Work& w) { CgBufs<float> B; cg_layout(w, A->n, B); }) - 255;
+  MIFE_REQUIRE(need <= kScratchBytes, MIFE_ERR_ARG,
+               "mife_cg_noalloc: %d rows need %zu bytes of scratch, more than its %zu bytes", A->n, need,
+               kScratchBytes);
+  void* base = nullptr;
+  int rc = scratch_of_current_device(&base);
+  if (rc) return rc;
+  return cg_entry(A, vals, b, x, prec, rtol, maxit, base, kScratchBytes, iters, relres, stream);
+}
 
 size_t mife_solver_work_bytes(int n, int restart) {
   const int nn = std::max(n, 1);
