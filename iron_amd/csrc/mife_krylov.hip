@@ -358,39 +358,88 @@ __global__ void fill_kernel(int n, T* __restrict__ v, T val) {
 }
 
 // ------------------------------------------------------------------ GMRES
-// Partials of V_j . w for j < m over a fixed grid; reduced by multidot_reduce.
+// Partials of V_j . w for j < m over a fixed grid; reduced by the Gram-Schmidt update.
 // blockIdx.y selects a chunk of 32 basis vectors (m <= 32 * gridDim.y).
 __global__ __launch_bounds__(256) void multidot_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
-                                                       const double* __restrict__ w, double* __restrict__ parts) {
+                                                       const double* __restrict__ w, double* __restrict__ parts,
+                                                       int pstride) {
   __shared__ double red[32][4];
   const int j0 = blockIdx.y * 32;
   const int mc = min(32, m - j0);
+  // fully unrolled with predicates, so the 32 accumulators stay in registers
   double acc[32];
+#pragma unroll
   for (int j = 0; j < 32; ++j) acc[j] = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double wi = w[i];
-    for (int j = 0; j < mc; ++j) acc[j] += V[(j0 + j) * ldv + i] * wi;
+    const double* Vi = V + (size_t)j0 * ldv + i;
+#pragma unroll
+    for (int j = 0; j < 32; ++j)
+      if (j < mc) acc[j] += Vi[(size_t)j * ldv] * wi;
   }
-  for (int j = 0; j < mc; ++j) {
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
     const double s = wave_sum(acc[j]);
-    if ((threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
+    if (j < mc && (threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
   }
   __syncthreads();
   if (threadIdx.x < mc) {
     double s = 0.0;
     for (int wv = 0; wv < 4; ++wv) s += red[threadIdx.x][wv];
-    parts[(size_t)(j0 + threadIdx.x) * kDotBlocks + blockIdx.x] = s;
+    parts[(size_t)(j0 + threadIdx.x) * pstride + blockIdx.x] = s;
   }
 }
 
-// h[j] = sum of the partials of V_j . w, one block per j (fixed order).
-__global__ void multidot_reduce(int m, int nblocks, const double* __restrict__ parts, double* __restrict__ h) {
+// Row blocks of the Gram-Schmidt dot products: few enough that every block of the
+// following update can reduce all partials of all m coefficients itself.
+constexpr int kGsBlocks = 32;
+
+// h[j] = sum over the kGsBlocks partials of coefficient j (fixed order), for j < m,
+// into LDS hs; block 0 also stores h to h_out.
+__device__ __forceinline__ void gs_coefficients(int m, const double* __restrict__ parts, double* hs,
+                                                double* __restrict__ h_out) {
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const double* pj = parts + (size_t)j * kGsBlocks;
+    double s = 0.0;
+#pragma unroll 8
+    for (int b = 0; b < kGsBlocks; ++b) s += pj[b];
+    hs[j] = s;
+    if (blockIdx.x == 0) h_out[j] = s;
+  }
+  __syncthreads();
+}
+
+// w -= sum_j V_j h_j with h reduced from the dot partials by every block.
+__global__ __launch_bounds__(256) void gs_update_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
+                                                        const double* __restrict__ parts, double* __restrict__ h_out,
+                                                        double* __restrict__ w) {
+  __shared__ double hs[kMaxRestart + 1];
+  gs_coefficients(m, parts, hs, h_out);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = w[i];
+    for (int j = 0; j < m; ++j) s -= V[j * ldv + i] * hs[j];
+    w[i] = s;
+  }
+}
+
+// As gs_update_kernel, and the per-block partials of the updated w.w (the new
+// vector's norm) into nparts[blockIdx.x].
+__global__ __launch_bounds__(256) void gs_update_norm_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
+                                                             const double* __restrict__ parts,
+                                                             double* __restrict__ h_out, double* __restrict__ w,
+                                                             double* __restrict__ nparts) {
+  __shared__ double hs[kMaxRestart + 1];
   __shared__ double red[4];
-  const int j = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += parts[(size_t)j * kDotBlocks + b];
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) h[j] = s;
+  gs_coefficients(m, parts, hs, h_out);
+  double q = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = w[i];
+    for (int j = 0; j < m; ++j) s -= V[j * ldv + i] * hs[j];
+    w[i] = s;
+    q += s * s;
+  }
+  q = block_sum(q, red);
+  if (threadIdx.x == 0) nparts[blockIdx.x] = q;
 }
 
 // Device state of one GMRES restart cycle.
@@ -408,13 +457,18 @@ struct GmresCycle {
 // breakdown, or when this is the last iteration allowed.
 __global__ __launch_bounds__(256) void gmres_givens_kernel(int k, int m, const double* __restrict__ h1,
                                                            const double* __restrict__ h2,
-                                                           const double* __restrict__ nrm, double* __restrict__ H,
-                                                           double* __restrict__ cs, double* __restrict__ sn,
-                                                           double* __restrict__ g, GmresCycle* cy, double tol_abs,
-                                                           int last) {
+                                                           const double* __restrict__ nparts, int nb,
+                                                           double* __restrict__ H, double* __restrict__ cs,
+                                                           double* __restrict__ sn, double* __restrict__ g,
+                                                           GmresCycle* cy, double tol_abs, int last) {
   __shared__ double col[kMaxRestart + 1];
   __shared__ double c_s[kMaxRestart], s_s[kMaxRestart];
+  __shared__ double red[4];
   if (cy->done) return;
+  // ||w||^2 from the update's per-block partials (fixed order)
+  double q = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) q += nparts[b];
+  const double nrm2 = block_sum(q, red);
   for (int j = threadIdx.x; j <= k; j += blockDim.x) col[j] = h1[j] + h2[j];
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
     c_s[j] = cs[j];
@@ -422,7 +476,7 @@ __global__ __launch_bounds__(256) void gmres_givens_kernel(int k, int m, const d
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double hk1 = sqrt(fmax(nrm[0], 0.0));
+    const double hk1 = sqrt(fmax(nrm2, 0.0));
     for (int j = 0; j < k; ++j) {
       const double a = col[j], b = col[j + 1];
       col[j] = c_s[j] * a + s_s[j] * b;
@@ -476,37 +530,6 @@ __global__ void gmres_next_basis_kernel(int n, const double* __restrict__ w, dou
     v[i] = vi;
     t[i] = dinv[i] * vi;
   }
-}
-
-// w -= sum_j V_j h_j, with the per-block partials of w.w (the second
-// Gram-Schmidt pass, which also yields the new vector's norm).
-__global__ __launch_bounds__(256) void orth_update_norm_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
-                                                               const double* __restrict__ h, double* __restrict__ w,
-                                                               double* __restrict__ parts) {
-  __shared__ double red[4];
-  double q = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = w[i];
-    for (int j = 0; j < m; ++j) s -= V[j * ldv + i] * h[j];
-    w[i] = s;
-    q += s * s;
-  }
-  q = block_sum(q, red);
-  if (threadIdx.x == 0) parts[blockIdx.x] = q;
-}
-
-// w -= sum_j V_j h_j
-__global__ void orth_update_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
-                                   const double* __restrict__ h, double* __restrict__ w) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = w[i];
-    for (int j = 0; j < m; ++j) s -= V[j * ldv + i] * h[j];
-    w[i] = s;
-  }
-}
-
-__global__ void scale_copy_kernel(int n, const double* __restrict__ src, double* __restrict__ dst, double a) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = a * src[i];
 }
 
 // x += dinv * (V y)
@@ -957,7 +980,7 @@ static void gmres_layout(Work& w, int n, int m, GmresBufs& B) {
   B.parts = w.take<double>((size_t)kDotBlocks * (m + 1));
   B.hdev = w.take<double>(m + 2);
   B.ydev = w.take<double>(m + 2);
-  B.ndev = w.take<double>(2);
+  B.ndev = w.take<double>(kMaxGrid);
   B.Hdev = w.take<double>((size_t)(m + 1) * m);
   B.csdev = w.take<double>(m + 2);
   B.sndev = w.take<double>(m + 2);
@@ -1349,16 +1372,16 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
       for (int c = 0; c < cnt; ++c) {
         const int kk = k + c;
         launch_spmv<double>(M.a, vals, t, wv, M.L, s);   // t = D^-1 V_kk (written with V_kk)
-        // classical Gram-Schmidt, two passes; the second also reduces ||w||^2
-        const dim3 mg(vg, div_up(kk + 1, 32));
-        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts);
-        multidot_reduce<<<kk + 1, 256, 0, s>>>(kk + 1, vg, parts, hdev);
-        orth_update_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, hdev, wv);
-        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts);
-        multidot_reduce<<<kk + 1, 256, 0, s>>>(kk + 1, vg, parts, ydev);
-        orth_update_norm_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, ydev, wv, parts);
-        multidot_reduce<<<1, 256, 0, s>>>(1, vg, parts, ndev);
-        gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, ydev, ndev, Hdev, csdev, sndev, gdev, cy, rtol * bn,
+        // classical Gram-Schmidt, two passes: dot partials over kGsBlocks row
+        // blocks, then an update whose every block reduces the coefficients
+        // itself; the second update also yields the norm partials, which the
+        // rotation kernel reduces
+        const dim3 mg(kGsBlocks, div_up(kk + 1, 32));
+        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts, kGsBlocks);
+        gs_update_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, parts, hdev, wv);
+        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts, kGsBlocks);
+        gs_update_norm_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, parts, ydev, wv, ndev);
+        gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, ydev, ndev, vg, Hdev, csdev, sndev, gdev, cy, rtol * bn,
                                              (it + c + 1 >= maxit || kk + 1 == m) ? 1 : 0);
         gmres_next_basis_kernel<<<vg, 256, 0, s>>>(n, wv, V + (kk + 1) * ldv, dinv, t, cy);
       }

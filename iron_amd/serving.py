@@ -66,6 +66,14 @@ class Request:
 
 
 # ------------------------------------------------------------------ problems
+def lognormal_field(n: int, seed: int, device) -> torch.Tensor:
+    """exp(N(0,1)) per element, drawn on the device from the request's seed (the
+    device's Philox generator: no host draw or upload per request)."""
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    return torch.randn(n, dtype=torch.float64, generator=g, device=device).exp_()
+
+
 class _Structure:
     """Device structures of one request shape (cached by (kind, n_elem))."""
 
@@ -102,6 +110,7 @@ class _Structure:
         self.fixed = fixed
         self.ufix = ufix
         self.csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, device, colours=structured_colours(m))
+        self.csr.pos   # the batched assembly scatters through it: build it with the structure
         self.red = fem.DirichletReduction(self.csr, fixed)
         self.rhs = None if rhs is None else torch.as_tensor(rhs, device=device)
         self.u0 = torch.as_tensor(np.where(fixed, ufix, 0.0), device=device)
@@ -116,15 +125,13 @@ class _Structure:
 
     def coefficients(self, seed: int, device):
         """Lognormal(0,1) element coefficients from the request's seed: conductivity, or
-        Young's modulus 10 exp(N(0,1)) with Poisson ratio 0.3."""
-        E = self.mesh.num_elements
-        g = torch.Generator().manual_seed(int(seed))
-        z = torch.randn(E, dtype=torch.float64, generator=g).exp()
+        Young's modulus 10 exp(N(0,1)) with Poisson ratio 0.3 (material [E][6])."""
+        z = lognormal_field(self.mesh.num_elements, seed, device)
         if self.kind == "diffusion":
-            return z.to(device)
-        Ey = 10.0 * z.numpy()
-        mat = np.stack([Ey, Ey, Ey, np.full(E, 0.3), np.full(E, 0.3), np.full(E, 0.3)], axis=1)
-        return torch.as_tensor(np.ascontiguousarray(mat), device=device)
+            return z
+        Ey = 10.0 * z
+        nu = torch.full_like(Ey, 0.3)
+        return torch.stack([Ey, Ey, Ey, nu, nu, nu], dim=1).contiguous()
 
     def assemble(self, coef, vals):
         vals.zero_()
