@@ -233,8 +233,11 @@ int mife_assemble_elastic(const mife_element_desc* desc, int n_elem,
  * (src/equations_matrices_routines.F90:1140-1265, MATRIX_VALUES_ADD_DP2
  * src/matrix_vector.F90:3119): the element kernels store every element matrix in
  * `stage` (one launch), then each CSR row (row_ptr over n_rows = n_comp x nodes
- * rows, iron's sorted pattern from mife_csr_pattern) is written whole: entry by
- * entry the element entries that meet there, summed in the element-colour order
+ * rows, iron's sorted pattern from mife_csr_pattern on the generated mesh's
+ * element DOFs -- the full grid stencil; the caller guarantees it, as the row
+ * structure is derived from the grid indices) is written whole, one x-line of
+ * rows per wavefront: entry by entry the element entries that meet there, summed
+ * in the element-colour order
  * of the coloured scatter, so vals (and src_vec) equal bit for bit what
  * mife_assemble_scalar / mife_assemble_elastic with the structured colouring add
  * into zeroed arrays.  vals and src_vec are overwritten, not added to; no

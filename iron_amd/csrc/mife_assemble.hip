@@ -51,6 +51,50 @@ __device__ __forceinline__ void add_value(T* p, T v, bool atomic) {
   else *p += v;
 }
 
+// Stage layout of a structured (grid) assembly.  With row m = (c, a), column
+// n = (c2, b), node a = (ax, ay, az) on the element's NB^3 lattice and element
+// e = ex + nx (ey + ny ez), entry (m, n) of element e sits at
+//   [ez][ey][c][az][ay][c2][bz][by][ex][ax][bx],
+// i.e. stripe U = (ez ny + ey) SU + u(c, az, ay, c2, bz, by), SU = NC^2 NB^4 stripes
+// per element row, each stripe nx NB^2 values long.  The entries that one x-line
+// of CSR rows needs for one y/z neighbour block are then one contiguous stripe.
+// The element kernels write the layout block-wise: each wavefront puts its
+// element's entries into LDS as [u][wave][NB^2], and after a block barrier the
+// block stores stripe by stripe, so the entries of the block's consecutive
+// elements along x go out as one contiguous run per stripe.  Blocks are dealt to
+// the eight XCDs in contiguous ranges (block b runs on XCD b % 8), so blocks that
+// share a cache line share an L2.
+template <int NN, int NC>
+struct GridStripe {
+  static constexpr int NB = NN == 8 ? 2 : 3, B2 = NB * NB, SU = NC * NC * B2 * B2;
+  // LDS slot of entry (m, n) of wavefront w's element
+  static __device__ __forceinline__ int slot(int m, int n, int w, int wpb) {
+    const int c = m / NN, a = m - c * NN, c2 = n / NN, b = n - c2 * NN;
+    const int u = ((((c * NB + a / B2) * NB + (a / NB) % NB) * NC + c2) * NB + b / B2) * NB + (b / NB) % NB;
+    return (u * wpb + w) * B2 + (a % NB) * NB + b % NB;
+  }
+  // stage offset of element e's stripe-0 run
+  static __device__ __forceinline__ long long base(long long e, int nx, int ny) {
+    const long long ex = e % nx, eyz = e / nx;
+    return (eyz * SU * nx + ex) * B2;
+  }
+  // logical block: contiguous ranges per XCD in a grid stage (gridDim.x a multiple of 8)
+  static __device__ __forceinline__ int block(int nx) {
+    if (nx <= 0) return blockIdx.x;
+    return (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  }
+  // block-wide store of the LDS slots K (sbase[w] < 0: wavefront w holds no element)
+  template <typename T, int WPB>
+  static __device__ __forceinline__ void store(const T* K, const long long* sbase, int nx, T* stage) {
+    const long long stride = (long long)nx * B2;
+    for (int f = threadIdx.x; f < SU * WPB * B2; f += 64 * WPB) {
+      const int u = f / (WPB * B2), r = f - u * (WPB * B2), w = r / B2, j = r - w * B2;
+      const long long b = sbase[w];
+      if (b >= 0) stage[b + u * stride + j] = K[f];
+    }
+  }
+};
+
 // Per-Gauss-point geometry of one element held in LDS by its wavefront.
 template <int DIM, int NN, int NG>
 struct GeomLDS {
@@ -105,17 +149,22 @@ struct ScalarJob {
   T* vals;
   double* svec;
   double* sstage;       // non-NULL: stage the element source vector at sstage[e][NN]
+  int gnx, gny;         // GRID: elements in x and y (GridStripe layout)
 };
 
 // One element per wavefront (element e when `active`); every wavefront of the
-// block calls it the same number of times (it holds block barriers).  Without a
-// position map the element matrix is stored (not added) in element order, for the
+// block calls it the same number of times (it holds block barriers).  GRID: the
+// element matrix is stored (not added) in the grid stage (GridStripe) for the
 // row-gather pass of a structured assembly.
-template <int DIM, int NN, int NG, typename T, int WPB>
+template <int DIM, int NN, int NG, typename T, int WPB, bool GRID = false>
 __device__ __forceinline__ void scalar_element(const ScalarJob<T>& J, long long e, bool active,
                                                const double* __restrict__ dN, const double* __restrict__ N,
                                                const double* __restrict__ w, bool atomic) {
   constexpr int KD = NG * DIM;
+  static_assert(!GRID || DIM == 3, "the grid assembly is 3-D");
+  using GS = GridStripe<(DIM == 3 ? NN : 8), 1>;
+  __shared__ T sK[GRID ? WPB * NN * NN : 1];
+  __shared__ long long sbase[WPB];
   __shared__ GeomLDS<DIM, NN, NG> geo[WPB];
   __shared__ T sB[WPB][KD * NN];
   __shared__ T sW[WPB][KD];
@@ -145,10 +194,10 @@ __device__ __forceinline__ void scalar_element(const ScalarJob<T>& J, long long 
       T acc = 0;
 #pragma unroll 9
       for (int k = 0; k < KD; ++k) acc += W[k] * B[k * NN + m] * B[k * NN + n];
-      if (J.pos)
-        add_value(&J.vals[J.pos[e * (NN * NN) + t]], acc, atomic);
+      if constexpr (GRID)
+        sK[GS::slot(m, n, wv, WPB)] = acc;
       else
-        J.vals[e * (NN * NN) + t] = acc;
+        add_value(&J.vals[J.pos[e * (NN * NN) + t]], acc, atomic);
     }
     if (J.source && (J.svec || J.sstage) && lane < NN) {
       // Rounded product and sum, unfused whatever the caller inlines around them
@@ -163,21 +212,28 @@ __device__ __forceinline__ void scalar_element(const ScalarJob<T>& J, long long 
         add_value(&J.svec[J.edofs[e * NN + lane]], s, atomic);
     }
   }
+  if constexpr (GRID) {
+    if (lane == 0) sbase[wv] = active ? GS::base(e, J.gnx, J.gny) : -1;
+    __syncthreads();
+    GS::template store<T, WPB>(sK, sbase, J.gnx, J.vals);
+  }
 }
 
-template <int DIM, int NN, int NG, typename T, int WPB>
+template <int DIM, int NN, int NG, typename T, int WPB, bool GRID = false>
 __global__ __launch_bounds__(64 * WPB) void scalar_assemble_kernel(
     const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
     const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
     const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
-    bool atomic, double* __restrict__ sstage = nullptr) {
-  const int idx = blockIdx.x * WPB + (threadIdx.x >> 6);
+    bool atomic, double* __restrict__ sstage = nullptr, int gnx = 0, int gny = 0) {
+  const int blk = GRID ? GridStripe<NN, 1>::block(gnx) : (int)blockIdx.x;
+  const int idx = blk * WPB + (threadIdx.x >> 6);
   const bool active = idx < elem_count;
   long long e = 0;
   if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
-  scalar_element<DIM, NN, NG, T, WPB>(ScalarJob<T>{coords, enodes, edofs, kappa, source, pos, vals, svec, sstage},
-                                      e, active, dN, N, w, atomic);
+  scalar_element<DIM, NN, NG, T, WPB, GRID>(
+      ScalarJob<T>{coords, enodes, edofs, kappa, source, pos, vals, svec, sstage, gnx, gny}, e, active, dN, N, w,
+      atomic);
 }
 
 // ------------------------------------------------ triquadratic, on MFMA
@@ -214,38 +270,58 @@ __global__ __launch_bounds__(64 * WPB) void q2_assemble_mfma_kernel(
     const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
     const double* __restrict__ N, const double* __restrict__ w, T* __restrict__ vals, double* __restrict__ svec,
-    bool atomic, double* __restrict__ sstage = nullptr) {
+    bool atomic, double* __restrict__ sstage = nullptr, int gnx = 0, int gny = 0) {
   constexpr int NN = 27, NG = 27, KD = 81, STEPS = 21;
   using M = Mfma16<T>;
-  __shared__ double sdN[NG * NN * 3];
-  __shared__ GeomLDS<3, NN, NG> geo[WPB];
+  using GS = GridStripe<NN, 1>;
+  // the element matrices of a grid stage reuse the LDS of the reference
+  // gradients and geometry once every wavefront is past its products
+  struct Tables {
+    double dN[NG * NN * 3];
+    GeomLDS<3, NN, NG> geo[WPB];
+  };
+  union Shared {
+    Tables t;
+    T K[WPB * NN * NN];
+  };
+  static_assert(sizeof(T) * WPB * NN * NN <= sizeof(Tables), "element matrices exceed the tables' LDS");
+  __shared__ Shared sh;
+  __shared__ long long sbase[WPB];
+  double* sdN = sh.t.dN;
   for (int t = threadIdx.x; t < NG * NN * 3; t += blockDim.x) sdN[t] = dN[t];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int idx = blockIdx.x * WPB + wv;
+  const int idx = GS::block(gnx) * WPB + wv;
   const bool active = idx < elem_count;
   long long e = 0;
   if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
-  GeomLDS<3, NN, NG>& G = geo[wv];
+  GeomLDS<3, NN, NG>& G = sh.t.geo[wv];
   element_geometry<3, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
-  if (!active) return;
-  const double ke = kappa ? kappa[e] : 1.0;
+  if (pos && !active) return;  // the scatter holds no further barrier
   const int kq = lane >> 4, col = lane & 15;
-  // lane's two operand positions: a = col and a = 16 + col (the latter is padding for col >= 11)
-  auto grad = [&](int k, int a) -> T {
-    if (k >= KD || a >= NN) return (T)0;
-    const int g = k / 3, i = k - 3 * (k / 3);
-    const double* d = &sdN[(g * NN + a) * 3];
-    return (T)(d[0] * G.ji[g][0][i] + d[1] * G.ji[g][1][i] + d[2] * G.ji[g][2][i]);
-  };
   typename M::v4 k00 = {0, 0, 0, 0}, k01 = {0, 0, 0, 0}, k11 = {0, 0, 0, 0};
+  double s = 0.0;
+  if (active) {
+    const double ke = kappa ? kappa[e] : 1.0;
+    // lane's two operand positions: a = col and a = 16 + col (the latter is padding for col >= 11)
+    auto grad = [&](int k, int a) -> T {
+      if (k >= KD || a >= NN) return (T)0;
+      const int g = k / 3, i = k - 3 * (k / 3);
+      const double* d = &sdN[(g * NN + a) * 3];
+      return (T)(d[0] * G.ji[g][0][i] + d[1] * G.ji[g][1][i] + d[2] * G.ji[g][2][i]);
+    };
 #pragma unroll 3
-  for (int t = 0; t < STEPS; ++t) {
-    const int k = 4 * t + kq;
-    const T wk = k < KD ? (T)(G.rwg[k / 3] * ke) : (T)0;
-    const T a0 = grad(k, col), a1 = grad(k, 16 + col);
-    k00 = M::mac(a0, a0 * wk, k00);
-    k01 = M::mac(a0, a1 * wk, k01);
-    k11 = M::mac(a1, a1 * wk, k11);
+    for (int t = 0; t < STEPS; ++t) {
+      const int k = 4 * t + kq;
+      const T wk = k < KD ? (T)(G.rwg[k / 3] * ke) : (T)0;
+      const T a0 = grad(k, col), a1 = grad(k, 16 + col);
+      k00 = M::mac(a0, a0 * wk, k00);
+      k01 = M::mac(a0, a1 * wk, k01);
+      k11 = M::mac(a1, a1 * wk, k11);
+    }
+    if (source && (svec || sstage) && lane < NN) {
+      for (int g = 0; g < NG; ++g) s += N[g * NN + lane] * G.rwg[g];
+      s *= source[e];
+    }
   }
   if (pos) {
     const int32_t* pe = pos + e * (NN * NN);
@@ -262,29 +338,28 @@ __global__ __launch_bounds__(64 * WPB) void q2_assemble_mfma_kernel(
       // tile (1,1): m = 16 + r, n = 16 + col
       if (16 + r < NN && 16 + col < NN) add_value(&vals[pe[(16 + r) * NN + 16 + col]], k11[i], atomic);
     }
-  } else {
-    // the element matrix, stored in element order for the row-gather pass
-    T* ke_out = vals + e * (NN * NN);
+    if (source && svec && lane < NN) add_value(&svec[edofs[e * NN + lane]], s, atomic);
+    return;
+  }
+  // grid stage: the element source vector in element order, the matrix in stripes
+  if (active && source && sstage && lane < NN) sstage[e * NN + lane] = s;
+  __syncthreads();  // every wavefront is done with the tables
+  if (active) {
+    T* K = sh.K;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = M::row(lane, i);
-      ke_out[r * NN + col] = k00[i];
+      K[GS::slot(r, col, wv, WPB)] = k00[i];
       if (16 + col < NN) {
-        ke_out[r * NN + 16 + col] = k01[i];
-        ke_out[(16 + col) * NN + r] = k01[i];
+        K[GS::slot(r, 16 + col, wv, WPB)] = k01[i];
+        K[GS::slot(16 + col, r, wv, WPB)] = k01[i];
       }
-      if (16 + r < NN && 16 + col < NN) ke_out[(16 + r) * NN + 16 + col] = k11[i];
+      if (16 + r < NN && 16 + col < NN) K[GS::slot(16 + r, 16 + col, wv, WPB)] = k11[i];
     }
   }
-  if (source && (svec || sstage) && lane < NN) {
-    double s = 0.0;
-    for (int g = 0; g < NG; ++g) s += N[g * NN + lane] * G.rwg[g];
-    s *= source[e];
-    if (sstage)
-      sstage[e * NN + lane] = s;
-    else
-      add_value(&svec[edofs[e * NN + lane]], s, atomic);
-  }
+  if (lane == 0) sbase[wv] = active ? GS::base(e, gnx, gny) : -1;
+  __syncthreads();
+  GS::template store<T, WPB>(sh.K, sbase, gnx, vals);
 }
 
 // ------------------------------------------------------- linear elasticity
@@ -383,9 +458,12 @@ struct ElasticJob {
   const double* material;
   const int32_t* pos;
   T* vals;
+  int gnx, gny;  // GRID: elements in x and y (GridStripe layout)
 };
 
-template <typename T, int NN, int NG, int WPB>
+// GRID: every wavefront of the block must call it (it ends with block barriers);
+// the element matrix goes to the grid stage (GridStripe) instead of the CSR.
+template <typename T, int NN, int NG, int WPB, bool GRID = false>
 __device__ __forceinline__ void elastic_element(const ElasticJob<T>& J, long long e, bool active,
                                                 const double* __restrict__ dN, const double* __restrict__ w,
                                                 bool atomic) {
@@ -393,6 +471,9 @@ __device__ __forceinline__ void elastic_element(const ElasticJob<T>& J, long lon
   constexpr int MT = (MROWS + 15) / 16, STEPS = (KD + 3) / 4;
   static_assert(NN <= 16, "one column tile");
   using M = Mfma16<T>;
+  using GS = GridStripe<NN, 3>;
+  __shared__ T sK[GRID ? WPB * ND * ND : 1];
+  __shared__ long long sbase[WPB];
   __shared__ GeomLDS<3, NN, NG> geo[WPB];
   __shared__ T sB[WPB][NG][NN][3];
   __shared__ T sDiag[WPB][NG][9];  // w * diagC[c][d]
@@ -431,10 +512,10 @@ __device__ __forceinline__ void elastic_element(const ElasticJob<T>& J, long lon
     }
   }
   __syncthreads();
-  if (!active) return;  // no barrier follows
-  const int32_t* pe = J.pos ? J.pos + e * (ND * ND) : nullptr;
+  if (!GRID && !active) return;  // the scatter holds no further barrier
+  const int32_t* pe = GRID ? nullptr : J.pos + e * (ND * ND);
   const int col = lane & 15;
-  for (int mt = 0; mt < MT; ++mt) {
+  for (int mt = 0; active && mt < MT; ++mt) {
     const int mrow = 16 * mt + col;
     const bool rv = mrow < MROWS;
     const int i = rv ? mrow / (3 * NN) : 0, j = rv ? (mrow / NN) % 3 : 0, a = rv ? mrow % NN : 0;
@@ -463,26 +544,34 @@ __device__ __forceinline__ void elastic_element(const ElasticJob<T>& J, long lon
       const int orow = 16 * mt + M::row(lane, r);
       if (orow < MROWS && col < NN) {
         const int oi = orow / (3 * NN), oj = (orow / NN) % 3, oa = orow % NN;
-        const int t = (oi * NN + oa) * ND + oj * NN + col;
-        if (pe)
-          add_value(&J.vals[pe[t]], acc[r], atomic);
+        if constexpr (GRID)
+          sK[GS::slot(oi * NN + oa, oj * NN + col, wv, WPB)] = acc[r];
         else
-          J.vals[e * (ND * ND) + t] = acc[r];   // element matrix in element order
+          add_value(&J.vals[pe[(oi * NN + oa) * ND + oj * NN + col]], acc[r], atomic);
       }
     }
   }
+  if constexpr (GRID) {
+    if (lane == 0) sbase[wv] = active ? GS::base(e, J.gnx, J.gny) : -1;
+    __syncthreads();
+    GS::template store<T, WPB>(sK, sbase, J.gnx, J.vals);
+  }
 }
 
-template <typename T, int NN, int NG, int WPB>
+// GRID: a structured assembly's stage (elements in order, gnx x gny per z-layer).
+template <typename T, int NN, int NG, int WPB, bool GRID = false>
 __global__ __launch_bounds__(64 * WPB) void elastic_assemble_mfma_kernel(
     const double* __restrict__ coords, const int32_t* __restrict__ enodes, const double* __restrict__ material,
     const int32_t* __restrict__ pos, const int32_t* __restrict__ order, int elem_begin, int elem_count,
-    const double* __restrict__ dN, const double* __restrict__ w, T* __restrict__ vals, bool atomic) {
-  const int idx = blockIdx.x * WPB + (threadIdx.x >> 6);
+    const double* __restrict__ dN, const double* __restrict__ w, T* __restrict__ vals, bool atomic, int gnx = 0,
+    int gny = 0) {
+  const int blk = GRID ? GridStripe<NN, 3>::block(gnx) : (int)blockIdx.x;
+  const int idx = blk * WPB + (threadIdx.x >> 6);
   const bool active = idx < elem_count;
   long long e = 0;
   if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
-  elastic_element<T, NN, NG, WPB>(ElasticJob<T>{coords, enodes, material, pos, vals}, e, active, dN, w, atomic);
+  elastic_element<T, NN, NG, WPB, GRID>(ElasticJob<T>{coords, enodes, material, pos, vals, gnx, gny}, e, active,
+                                        dN, w, atomic);
 }
 
 // ------------------------------------------- many meshes in one launch
@@ -541,7 +630,7 @@ __global__ __launch_bounds__(64 * WPB) void scalar_batch_kernel(const BatchJob* 
     if (active) {
       const BatchSeg sg = batch_seg(segs, nseg, idx);
       const BatchJob B = jobs[sg.job];
-      J = ScalarJob<T>{B.coords, B.enodes, B.edofs, B.coef, B.source, B.pos, (T*)B.vals, B.svec, nullptr};
+      J = ScalarJob<T>{B.coords, B.enodes, B.edofs, B.coef, B.source, B.pos, (T*)B.vals, B.svec, nullptr, 0, 0};
       e = B.order[sg.begin + idx - sg.prefix];
     }
     scalar_element<3, 8, 8, T, WPB>(J, e, active, dN, N, w, false);
@@ -562,7 +651,7 @@ __global__ __launch_bounds__(64 * WPB) void elastic_batch_kernel(const BatchJob*
     if (active) {
       const BatchSeg sg = batch_seg(segs, nseg, idx);
       const BatchJob B = jobs[sg.job];
-      J = ElasticJob<T>{B.coords, B.enodes, B.coef, B.pos, (T*)B.vals};
+      J = ElasticJob<T>{B.coords, B.enodes, B.coef, B.pos, (T*)B.vals, 0, 0};
       e = B.order[sg.begin + idx - sg.prefix];
     }
     elastic_element<T, 8, 8, WPB>(J, e, active, dN, w, false);
@@ -656,57 +745,181 @@ __device__ __forceinline__ int elem_of_parity(int lo, int hi, int bit) {
   return -1;
 }
 
-// One wavefront per CSR row (WPB rows per block).  stage: element matrices
-// [E][NC NN][NC NN]; sstage (NC == 1, may be NULL): element source vectors [E][NN].
-// Entries past a row's stored length are not written.
-template <typename T, int P, int NC, int WPB>
-__global__ __launch_bounds__(64 * WPB) void grid_gather_kernel(int nx, int ny, int nz,
-                                                               const int32_t* __restrict__ row_ptr,
-                                                               const T* __restrict__ stage, T* __restrict__ vals,
-                                                               const double* __restrict__ sstage,
-                                                               double* __restrict__ svec) {
-  constexpr int NB = P + 1, NN = NB * NB * NB, ND = NC * NN;
+// Neighbour pairs of an x-line (order P, nx elements): pair t couples node ix to
+// node jx, pairs ordered by ix, then jx.  Nodes repeat with period P: a vertex
+// node couples to 2P + 1 nodes, each of the P - 1 nodes inside an element to
+// P + 1, so a period holds P^2 + 2P pairs; the first vertex lacks its P left
+// neighbours, which the shift by P accounts for.  An x-line has
+// nx (P^2 + 2P) + 1 pairs.
+__device__ __forceinline__ void x_pair(int t, int P, int& ix, int& jx) {
+  const int per = P * P + 2 * P;
+  const int tp = t + P;
+  const int e = tp / per, r = tp - e * per;
+  if (r < 2 * P + 1) {
+    ix = P * e;
+    jx = ix - P + r;
+  } else {
+    const int r2 = r - (2 * P + 1), k = r2 / (P + 1);
+    ix = P * e + 1 + k;
+    jx = P * e + (r2 - k * (P + 1));
+  }
+}
+
+// First pair of node ix of an x-line with nx elements; ix = P nx + 1 (one past
+// the last node, which lacks its P right neighbours) gives the pair count.
+__device__ __forceinline__ int x_pair_start(int ix, int P, int nx) {
+  if (ix > P * nx) return nx * (P * P + 2 * P) + 1;
+  const int e = ix / P, k = ix - e * P;
+  const int tp = e * (P * P + 2 * P) + (k ? 2 * P + 1 + (k - 1) * (P + 1) : 0);
+  return max(tp - P, 0);
+}
+
+// One wavefront (one block) per (component c, x-line (iy, iz), run of R
+// consecutive nodes ix, R = P (64 / (P^2 + 2P)) so that the run's pairs fit the
+// 64 lanes).  Lane l takes one pair (ix, jx): CSR row (c, ix, iy, iz) and its
+// x-neighbour offset qx.  The wavefront walks the y/z neighbour blocks
+// (c2, qz, qy) of the x-line, which are the same for every lane, so the y/z
+// index arithmetic is wave-uniform.  In the GridStripe layout the entries of
+// block (c2, qz, qy) for the element slots (sy, sz) are stripe U, nx NB^2
+// values, of which lane l reads X = ex NB^2 + ax NB + bx for its element slots
+// sx.  Per (c2, qz) the lanes first issue the loads of all qy, then sum each
+// entry's (up to eight) element values in colour order -- parity of the element
+// index per direction, z slowest, the order of the coloured scatter -- and put
+// the sums in LDS, where each row's (c2, qz) piece, cy cx entries, is
+// contiguous as in the CSR row; the pieces are then stored row by row.  Units
+// are dealt to the eight XCDs in contiguous ranges (block b runs on XCD b % 8),
+// so the x-lines that share stripes share an L2.
+// stage: element matrices in the GridStripe layout; sstage (NC == 1, may be
+// NULL): element source vectors [E][NN].  Every row must hold the full grid
+// stencil (the host checks the element DOFs); entries beyond a row's stored
+// length are not written.
+template <typename T, typename I, int P, int NC>
+__global__ __launch_bounds__(64) void grid_gather_kernel(int nx, int ny, int nz, int runs, int n_units,
+                                                         const int32_t* __restrict__ row_ptr,
+                                                         const T* __restrict__ stage, T* __restrict__ vals,
+                                                         const double* __restrict__ sstage,
+                                                         double* __restrict__ svec) {
+  constexpr int NB = P + 1, NN = NB * NB * NB, PER = P * P + 2 * P, R = P * (64 / PER), Q = 2 * P + 1;
+  constexpr int SU = NC * NC * NB * NB * NB * NB;
+  __shared__ T piece[Q * 64];
   const int Nx = P * nx + 1, Ny = P * ny + 1, Nz = P * nz + 1;
-  const long long Nn = (long long)Nx * Ny * Nz;
-  const long long row = (long long)blockIdx.x * WPB + (threadIdx.x >> 6);
-  if (row >= NC * Nn) return;
-  const int lane = threadIdx.x & 63;
-  const int c = (int)(row / Nn);
-  const long long node = row - c * Nn;
-  const int ix = (int)(node % Nx), iy = (int)((node / Nx) % Ny), iz = (int)(node / ((long long)Nx * Ny));
-  int lx, hx, ly, hy, lz, hz;
-  elem_range(ix, P, nx, lx, hx);
+  const int unit = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (unit >= n_units) return;  // whole block: no barrier is reached
+  const int lane = threadIdx.x;
+  const int run = unit % runs, line = unit / runs;
+  const int iy = line % Ny, iz = (line / Ny) % Nz, c = line / (Ny * Nz);
+  const int r0 = run * R, r1 = min(r0 + R, Nx);
+  const long long rowline = ((long long)c * Nz + iz) * ((long long)Ny * Nx) + (long long)iy * Nx;
+  const int t0 = x_pair_start(r0, P, nx), npairs = x_pair_start(r1, P, nx) - t0;
+  // wave-uniform y/z ranges
+  int ly, hy, lz, hz;
   elem_range(iy, P, ny, ly, hy);
   elem_range(iz, P, nz, lz, hz);
-  const int jx0 = P * lx, jy0 = P * ly, jz0 = P * lz;
-  const int cx = P * hx + P - jx0 + 1, cy = P * hy + P - jy0 + 1, cz = P * hz + P - jz0 + 1;
-  const int cnt = cx * cy * cz;
-  const int base = row_ptr[row];
-  const int len = row_ptr[row + 1] - base;
-  for (int p = lane; p < NC * cnt && p < len; p += 64) {
-    const int c2 = p / cnt, q = p - c2 * cnt;
-    const int jx = jx0 + q % cx, jy = jy0 + (q / cx) % cy, jz = jz0 + q / (cx * cy);
-    int ljx, hjx, ljy, hjy, ljz, hjz;
-    elem_range(jx, P, nx, ljx, hjx);
-    elem_range(jy, P, ny, ljy, hjy);
-    elem_range(jz, P, nz, ljz, hjz);
-    // elements holding both nodes, per direction
-    const int ax = max(lx, ljx), bx = min(hx, hjx), ay = max(ly, ljy), by = min(hy, hjy);
-    const int az = max(lz, ljz), bz = min(hz, hjz);
-    T sum = (T)0;
+  const int cy = P * (hy - ly) + P + 1, cz = P * (hz - lz) + P + 1;
+  // lane's pair: row ix, neighbour jx, x element slots
+  const bool act = lane < npairs;
+  int ix = r0, jx = r0;
+  if (act) x_pair(t0 + lane, P, ix, jx);
+  int lx, hx, ljx, hjx;
+  elem_range(ix, P, nx, lx, hx);
+  elem_range(jx, P, nx, ljx, hjx);
+  const int cx = P * (hx - lx) + P + 1, qx = jx - P * lx;
+  const int ax = max(lx, ljx), bx = min(hx, hjx);
+  int X[2];
+  bool xv[2];
 #pragma unroll
-    for (int col = 0; col < 8; ++col) {
-      const int ex = elem_of_parity(ax, bx, col & 1), ey = elem_of_parity(ay, by, (col >> 1) & 1),
-                ez = elem_of_parity(az, bz, col >> 2);
-      if (ex < 0 || ey < 0 || ez < 0) continue;
-      const long long e = ex + (long long)nx * (ey + (long long)ny * ez);
-      const int a = (ix - P * ex) + NB * ((iy - P * ey) + NB * (iz - P * ez));
-      const int b = (jx - P * ex) + NB * ((jy - P * ey) + NB * (jz - P * ez));
-      sum += stage[e * (ND * ND) + (c * NN + a) * ND + c2 * NN + b];
-    }
-    vals[base + p] = sum;
+  for (int s = 0; s < 2; ++s) {
+    const int ex = elem_of_parity(ax, bx, s);
+    xv[s] = act && ex >= 0;
+    X[s] = xv[s] ? (ex * NB + (ix - P * ex)) * NB + (jx - P * ex) : 0;
   }
-  if (NC == 1 && sstage && lane == 0) {
+  // LDS place of the lane's entry qy of a piece: rows' pieces back to back
+  const int wslot = cy * (x_pair_start(ix, P, nx) - t0) + qx;
+  // store plan: LDS slot k = lane + 64 i holds entry k of the pieces, i.e. of
+  // row fix = row of pair t0 + k / cy, at (c2 cz + qz) cy cx(fix) + local
+  int fpos[Q], fcx[Q], flen[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int k = lane + 64 * i;
+    fpos[i] = -1;
+    fcx[i] = 0;
+    flen[i] = 0;
+    if (k < cy * npairs) {
+      int fix, fjx;
+      x_pair(t0 + k / cy, P, fix, fjx);
+      int flx, fhx;
+      elem_range(fix, P, nx, flx, fhx);
+      const int rp = row_ptr[rowline + fix];
+      fpos[i] = rp + (k - cy * (x_pair_start(fix, P, nx) - t0));
+      fcx[i] = P * (fhx - flx) + P + 1;
+      flen[i] = row_ptr[rowline + fix + 1];
+    }
+  }
+  const I stripe = (I)nx * (NB * NB);
+  for (int c2 = 0; c2 < NC; ++c2) {
+    for (int qz = 0; qz < cz; ++qz) {
+      const int jz = P * lz + qz;
+      int ljz, hjz;
+      elem_range(jz, P, nz, ljz, hjz);
+      const int az = max(lz, ljz), bz = min(hz, hjz);
+      int ez[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) ez[s] = elem_of_parity(az, bz, s);
+      // loads of every qy first (no use before all are issued)
+      T v[Q][2][2][2];
+      bool yv[Q][2];
+#pragma unroll
+      for (int qy = 0; qy < Q; ++qy) {
+        const int jy = P * ly + qy;
+        int ljy, hjy;
+        elem_range(jy, P, ny, ljy, hjy);
+        const int ay = max(ly, ljy), by = min(hy, hjy);
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy) {
+          const int ey = qy < cy ? elem_of_parity(ay, by, sy) : -1;
+          yv[qy][sy] = ey >= 0;
+#pragma unroll
+          for (int sz = 0; sz < 2; ++sz) {
+            if (ey < 0 || ez[sz] < 0) continue;
+            const I U = ((I)ez[sz] * ny + ey) * SU +
+                        (((((c * NB + (iz - P * ez[sz])) * NB + (iy - P * ey)) * NC + c2) * NB + (jz - P * ez[sz])) *
+                             NB +
+                         (jy - P * ey));
+            const T* st = stage + U * stripe;
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+              if (xv[sx]) v[qy][sz][sy][sx] = st[X[sx]];
+          }
+        }
+      }
+#pragma unroll
+      for (int qy = 0; qy < Q; ++qy) {
+        if (qy >= cy) continue;
+        T sum = (T)0;
+#pragma unroll
+        for (int sz = 0; sz < 2; ++sz) {
+          if (ez[sz] < 0) continue;
+#pragma unroll
+          for (int sy = 0; sy < 2; ++sy) {
+            if (!yv[qy][sy]) continue;
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+              if (xv[sx]) sum += v[qy][sz][sy][sx];
+          }
+        }
+        if (act) piece[wslot + qy * cx] = sum;
+      }
+      __syncthreads();
+      const int blk = (c2 * cz + qz) * cy;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const int at = fpos[i] + blk * fcx[i];
+        if (fpos[i] >= 0 && at < flen[i]) vals[at] = piece[lane + 64 * i];
+      }
+      __syncthreads();
+    }
+  }
+  if (NC == 1 && sstage && act && jx == ix) {
     double sv = 0.0;
 #pragma unroll
     for (int col = 0; col < 8; ++col) {
@@ -717,7 +930,7 @@ __global__ __launch_bounds__(64 * WPB) void grid_gather_kernel(int nx, int ny, i
       const int a = (ix - P * ex) + NB * ((iy - P * ey) + NB * (iz - P * ez));
       sv += sstage[e * NN + a];
     }
-    svec[row] = sv;
+    svec[rowline + ix] = sv;
   }
 }
 
@@ -739,24 +952,34 @@ static int launch_grid(const mife_element_desc* d, const int* ne, const double* 
       : nullptr;
   const int tk = prof::begin(s);
   constexpr int WQ = 4;
-  const int grid_e = div_up(E, WQ);
+  const int grid_e = div_up(div_up(E, WQ), 8) * 8;  // a multiple of the XCD count
   if constexpr (NC == 1) {
     if constexpr (NN == 27) {
       q2_assemble_mfma_kernel<T, WQ><<<grid_e, 64 * WQ, 0, s>>>(coords, enodes, nullptr, coef, source, nullptr,
                                                                 nullptr, 0, (int)E, d->dN, d->N, d->w, kst,
-                                                                nullptr, false, sst);
+                                                                nullptr, false, sst, ne[0], ne[1]);
     } else {
-      scalar_assemble_kernel<3, NN, NN, T, 4><<<grid_e, 64 * 4, 0, s>>>(coords, enodes, nullptr, coef, source,
+      scalar_assemble_kernel<3, NN, NN, T, 4, true><<<grid_e, 64 * 4, 0, s>>>(coords, enodes, nullptr, coef, source,
                                                                         nullptr, nullptr, 0, (int)E, d->dN, d->N,
-                                                                        d->w, kst, nullptr, false, sst);
+                                                                        d->w, kst, nullptr, false, sst, ne[0],
+                                                                        ne[1]);
     }
   } else {
-    elastic_assemble_mfma_kernel<T, NN, NN, WQ><<<grid_e, 64 * WQ, 0, s>>>(coords, enodes, coef, nullptr, nullptr,
-                                                                           0, (int)E, d->dN, d->w, kst, false);
+    elastic_assemble_mfma_kernel<T, NN, NN, WQ, true><<<grid_e, 64 * WQ, 0, s>>>(coords, enodes, coef, nullptr,
+                                                                                 nullptr,
+                                                                           0, (int)E, d->dN, d->w, kst, false, ne[0],
+                                                                           ne[1]);
   }
-  constexpr int WR = 4;
-  grid_gather_kernel<T, P, NC, WR><<<div_up(n_rows, WR), 64 * WR, 0, s>>>(ne[0], ne[1], ne[2], row_ptr, kst, vals,
-                                                                          sst, sst ? svec : nullptr);
+  constexpr int R = P * (64 / (P * P + 2 * P));
+  const int runs = div_up(P * ne[0] + 1, R);
+  const long long units = (long long)NC * (P * ne[1] + 1) * (P * ne[2] + 1) * runs;
+  const int blocks = div_up(units, 8) * 8;  // a multiple of the XCD count
+  if ((size_t)E * ND * ND < (size_t(1) << 31))
+    grid_gather_kernel<T, int, P, NC><<<blocks, 64, 0, s>>>(ne[0], ne[1], ne[2], runs, (int)units, row_ptr, kst, vals,
+                                                           sst, sst ? svec : nullptr);
+  else
+    grid_gather_kernel<T, long long, P, NC><<<blocks, 64, 0, s>>>(ne[0], ne[1], ne[2], runs, (int)units, row_ptr, kst,
+                                                                 vals, sst, sst ? svec : nullptr);
   const int key = NC == 1 ? (sizeof(T) == 8 ? prof::kAssembleScalar : prof::kAssembleScalar32)
                           : (sizeof(T) == 8 ? prof::kAssembleElastic : prof::kAssembleElastic32);
   prof::end(tk, s, key);
@@ -794,7 +1017,7 @@ extern "C" int mife_assemble_grid(const mife_element_desc* d, const int32_t* n_e
                d->n_gauss, n_comp);
   const long long E = (long long)n_elem[0] * n_elem[1] * n_elem[2];
   const long long Nn = (long long)(P * n_elem[0] + 1) * (P * n_elem[1] + 1) * (P * n_elem[2] + 1);
-  MIFE_REQUIRE(E < (1LL << 31) && n_rows == n_comp * Nn, MIFE_ERR_ARG,
+  MIFE_REQUIRE(E < (1LL << 31) && n_rows < (1LL << 31) && n_rows == n_comp * Nn, MIFE_ERR_ARG,
                "%lld rows given, the grid has %lld nodes x %d components", (long long)n_rows, Nn, n_comp);
   const size_t need = prec == MIFE_F32 ? grid_stage_bytes<float>(E, d->n_nodes, n_comp)
                                        : grid_stage_bytes<double>(E, d->n_nodes, n_comp);

@@ -505,6 +505,20 @@ def backsubstitute(csr: CSRStructure, vals64, u, src=None, rhs_fixed=None, iron_
     return out
 
 
+def grid_element_dofs(grid, P: int, n_comp: int) -> np.ndarray:
+    """[E][n_comp (P+1)^3] DOFs of a generated regular grid of order P: elements and
+    nodes numbered x fastest, local nodes xi1 fastest, components in separated blocks
+    of one DOF per node (generated_mesh_routines.F90:1568-1612)."""
+    nx, ny, nz = (int(g) for g in grid)
+    NB = P + 1
+    Nx, Ny, Nz = P * nx + 1, P * ny + 1, P * nz + 1
+    ez, ey, ex = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    az, ay, ax = np.meshgrid(np.arange(NB), np.arange(NB), np.arange(NB), indexing="ij")
+    node = ((P * ex.reshape(-1, 1) + ax.reshape(1, -1)) + Nx * ((P * ey.reshape(-1, 1) + ay.reshape(1, -1))
+                                                                + Ny * (P * ez.reshape(-1, 1) + az.reshape(1, -1))))
+    return np.concatenate([c * Nx * Ny * Nz + node for c in range(n_comp)], axis=1).astype(np.int32)
+
+
 class _GridStage:
     """Structured assembly on a generated regular grid (mife_assemble_grid): element
     matrices staged in one launch, then every CSR row written whole.  Holds the stage
@@ -515,6 +529,11 @@ class _GridStage:
         self.n_comp = n_comp
         self.csr = csr
         E = int(np.prod(self.grid))
+        # the kernels derive every row and element from grid indices: the element DOFs
+        # must be the generated mesh's (nodes x fastest, components separated)
+        want = grid_element_dofs(self.grid, 1 if basis.num_nodes == 8 else 2, n_comp)
+        if csr.n != int(want.max()) + 1 or not np.array_equal(csr.elem_dofs.cpu().numpy(), want):
+            raise ValueError(f"the element DOFs are not those of a generated {tuple(self.grid)} grid")
         self.nbytes = int(_lib.lib().mife_assemble_grid_stage_bytes(E, basis.num_nodes, n_comp, F64))
         self.buf = None
 
