@@ -114,10 +114,15 @@ __device__ __forceinline__ void element_geometry(GeomLDS<DIM, NN, NG>& G, const 
     for (int d = 0; d < DIM; ++d) G.x[lane * DIM + d] = coords[node * DIM + d];
   }
   __syncthreads();
-  if (active && lane < NG) {
-    const int g = lane;
-    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int a = 0; a < NN; ++a) {
+  // dx/dxi at Gauss point g = lane % 32: the node sum split between the two
+  // half-waves (nodes below and from NN / 2), the halves added across lanes
+  // l and l + 32 (NG <= 32), else one lane per Gauss point
+  constexpr bool SPLIT = NG <= 32;
+  const int g = SPLIT ? (lane & 31) : lane;
+  double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  if (active && g < NG) {
+    const int a0 = SPLIT && lane >= 32 ? NN / 2 : 0, a1 = SPLIT && lane < 32 ? NN / 2 : NN;
+    for (int a = a0; a < a1; ++a) {
 #pragma unroll
       for (int j = 0; j < DIM; ++j) {
         const double dn = dN[(g * NN + a) * DIM + j];
@@ -125,6 +130,14 @@ __device__ __forceinline__ void element_geometry(GeomLDS<DIM, NN, NG>& G, const 
         for (int d = 0; d < DIM; ++d) J[d][j] += G.x[a * DIM + d] * dn;
       }
     }
+  }
+  if constexpr (SPLIT) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) J[d][j] += __shfl_xor(J[d][j], 32);
+  }
+  if (active && lane < NG) {
     double Ji[3][3];
     const double det = invert<DIM>(J, Ji);
 #pragma unroll
@@ -265,7 +278,7 @@ struct Mfma16<float> {
 };
 
 template <typename T, int WPB>
-__global__ __launch_bounds__(64 * WPB) void q2_assemble_mfma_kernel(
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void q2_assemble_mfma_kernel(
     const double* __restrict__ coords, const int32_t* __restrict__ enodes, const int32_t* __restrict__ edofs,
     const double* __restrict__ kappa, const double* __restrict__ source, const int32_t* __restrict__ pos,
     const int32_t* __restrict__ order, int elem_begin, int elem_count, const double* __restrict__ dN,
@@ -295,7 +308,7 @@ __global__ __launch_bounds__(64 * WPB) void q2_assemble_mfma_kernel(
   long long e = 0;
   if (active) e = order ? order[elem_begin + idx] : (long long)elem_begin + idx;
   GeomLDS<3, NN, NG>& G = sh.t.geo[wv];
-  element_geometry<3, NN, NG>(G, coords, enodes, e, dN, w, lane, active);
+  element_geometry<3, NN, NG>(G, coords, enodes, e, sdN, w, lane, active);
   if (pos && !active) return;  // the scatter holds no further barrier
   const int kq = lane >> 4, col = lane & 15;
   typename M::v4 k00 = {0, 0, 0, 0}, k01 = {0, 0, 0, 0}, k11 = {0, 0, 0, 0};
