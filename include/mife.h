@@ -249,6 +249,62 @@ int mife_assemble_grid(const mife_element_desc* desc, const int32_t* n_elem, int
                        const double* source, const int32_t* row_ptr, int64_t n_rows, int prec,
                        void* vals, double* src_vec, void* stage, size_t stage_bytes, void* stream);
 
+/* Device structures of a generated box (a served request's mesh): node
+ * coordinates, element connectivity, the equations CSR, the Dirichlet
+ * reduction and the SELL-64 storage of the reduced matrix, written from closed
+ * forms of the grid indices.  Replaces, for a box, the generated regular mesh
+ * and DOF numbering (src/generated_mesh_routines.F90:1568-1612, :3217-3384;
+ * src/field_routines.F90:10056), EquationsMatrix_StructureCalculate
+ * (src/equations_matrices_routines.F90:4316-4602) and the solver mapping and
+ * matrix structure (src/solver_mapping_routines.F90:422-547, :1445-1511;
+ * src/solver_matrices_routines.F90:1612-2431).  Numbering as the generic path:
+ * nodes and elements x fastest, DOFs in separated order (c N + node), CSR rows
+ * sorted; solver rows and columns are the free DOFs ascending.  Dirichlet
+ * planes x = 0 (fix_lo, value u_lo) and x = x_max (fix_hi, value u_hi), every
+ * component; the box is [0, extent].
+ * mife_box_sizes fills the counts on the host (no device work); mife_box_build
+ * writes the arrays on `stream` without allocating or synchronising:
+ *   coords [n_nodes][3] (may be NULL), elem_nodes [n_elem][nodes_per_elem],
+ *   row_ptr [n_dofs + 1], col [nnz], free_idx [n_free], new_idx [n_dofs] (-1:
+ *   fixed), u0 [n_dofs] (Dirichlet values, 0 on free DOFs), slice_ptr
+ *   [n_slices + 1], sell_src [stored] (equations-CSR position, -1: padding) and
+ *   either sell_col16 [stored] + col_base [n_slices] (col16 = 1) or sell_col
+ *   [stored].  Every slice has width `width` (the longest reduced row); padding
+ *   entries hold the row's own column and no source. */
+typedef struct {
+  int32_t n_elem[3];
+  int32_t order;   /* 1 trilinear, 2 triquadratic */
+  int32_t n_comp;  /* 1 (scalar field) or 3 (displacement) */
+  int32_t fix_lo, fix_hi;
+  int32_t pad;
+  double extent[3];
+  double u_lo, u_hi;
+} mife_box_desc;
+
+typedef struct {
+  int64_t n_nodes, n_elem, nodes_per_elem, n_dofs, nnz;
+  int64_t n_free, red_nnz, n_slices, width, stored;
+  int32_t col16, pad;
+} mife_box_counts;
+
+typedef struct {
+  double* coords;
+  int32_t* elem_nodes;
+  int32_t* row_ptr;
+  int32_t* col;
+  int32_t* free_idx;
+  int32_t* new_idx;
+  double* u0;
+  int32_t* slice_ptr;
+  int32_t* sell_col;
+  uint16_t* sell_col16;
+  int32_t* col_base;
+  int32_t* sell_src;
+} mife_box_arrays;
+
+int mife_box_sizes(const mife_box_desc* desc, mife_box_counts* out);
+int mife_box_build(const mife_box_desc* desc, const mife_box_arrays* arrays, void* stream);
+
 /* Batched assembly: the CSR values of many independent trilinear meshes (the
  * requests a server admits together) in one launch per colour and element
  * kind.  Each job is one mesh with the arguments of mife_assemble_scalar

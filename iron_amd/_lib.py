@@ -60,6 +60,26 @@ class ElementDesc(C.Structure):
                 ("dN", C.c_void_p), ("N", C.c_void_p), ("w", C.c_void_p)]
 
 
+class BoxDesc(C.Structure):
+    """mife_box_desc: a generated box, its order, components and Dirichlet x-planes."""
+    _fields_ = [("n_elem", C.c_int32 * 3), ("order", C.c_int32), ("n_comp", C.c_int32), ("fix_lo", C.c_int32),
+                ("fix_hi", C.c_int32), ("pad", C.c_int32), ("extent", C.c_double * 3), ("u_lo", C.c_double),
+                ("u_hi", C.c_double)]
+
+
+class BoxCounts(C.Structure):
+    """mife_box_counts: sizes of a box's structures (closed forms, host)."""
+    _fields_ = [("n_nodes", C.c_int64), ("n_elem", C.c_int64), ("nodes_per_elem", C.c_int64), ("n_dofs", C.c_int64),
+                ("nnz", C.c_int64), ("n_free", C.c_int64), ("red_nnz", C.c_int64), ("n_slices", C.c_int64),
+                ("width", C.c_int64), ("stored", C.c_int64), ("col16", C.c_int32), ("pad", C.c_int32)]
+
+
+class BoxArrays(C.Structure):
+    """mife_box_arrays: device arrays mife_box_build writes."""
+    _fields_ = [(name, C.c_void_p) for name in ("coords", "elem_nodes", "row_ptr", "col", "free_idx", "new_idx", "u0",
+                                                 "slice_ptr", "sell_col", "sell_col16", "col_base", "sell_src")]
+
+
 MIFE_JOB_SCALAR = 0
 MIFE_JOB_ELASTIC = 1
 
@@ -102,6 +122,8 @@ _SIG = {
     "mife_assemble_grid_stage_bytes": (C.c_size_t, [_I, _I, _I, _I]),
     "mife_assemble_grid": (_I, [C.POINTER(ElementDesc), _P, _I, _P, _P, _P, _P, _P, _I64, _I, _P, _P, _P, _SZ,
                                 _P]),
+    "mife_box_sizes": (_I, [C.POINTER(BoxDesc), C.POINTER(BoxCounts)]),
+    "mife_box_build": (_I, [C.POINTER(BoxDesc), C.POINTER(BoxArrays), _P]),
     "mife_spmv": (_I, [C.POINTER(MatrixDesc), _P, _P, _P, _I, _P]),
     "mife_sell_permutation": (_I, [_I, _P, _P, _I, _P, _P]),
     "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),

@@ -8,6 +8,7 @@
 // are launched one after another, so every CSR entry is updated by at most one
 // wavefront per launch and the summation order is fixed (deterministic).
 #include "mife_common.hpp"
+#include "mife_grid.hpp"
 #include "mife_profile.hpp"
 
 #include <algorithm>
@@ -735,58 +736,14 @@ static int launch_elastic(const mife_element_desc* d, int n_elem, const double* 
 // generated mesh numbers them; one or three components in separated DOF order)
 // every CSR row is the tensor product of 1-D neighbour ranges, so a row and the
 // elements that touch it follow from the node's grid index.  The assembly then
-// runs as two passes: the element kernels store every element matrix in element
-// order (one launch, no colouring), and this kernel writes each CSR row whole --
+// runs as two passes: the element kernels store every element matrix in the
+// GridStripe layout (one launch, no colouring), and this kernel writes each CSR row whole --
 // entry by entry the sum of the element entries that meet there, in the order of
 // the element colours (parity of the element index per direction), which is the
 // order in which the coloured scatter adds them, so the values are bit for bit
 // those of the coloured scatter.  No position map is read and no value is read
-// back: each row is one contiguous store.
+// back.  The 1-D neighbour arithmetic is in mife_grid.hpp.
 //
-// 1-D element range of node i (order P, n elements): elements ex with
-// P ex <= i <= P ex + P, clipped to [0, n - 1].
-__device__ __forceinline__ void elem_range(int i, int P, int n, int& lo, int& hi) {
-  lo = i > 0 ? (i - 1) / P : 0;
-  hi = min(i / P, n - 1);
-}
-
-// Element of [lo, hi] (one or two consecutive elements) whose index parity is
-// `bit`, or -1.
-__device__ __forceinline__ int elem_of_parity(int lo, int hi, int bit) {
-  if ((lo & 1) == bit) return lo;
-  if (hi != lo && (hi & 1) == bit) return hi;
-  return -1;
-}
-
-// Neighbour pairs of an x-line (order P, nx elements): pair t couples node ix to
-// node jx, pairs ordered by ix, then jx.  Nodes repeat with period P: a vertex
-// node couples to 2P + 1 nodes, each of the P - 1 nodes inside an element to
-// P + 1, so a period holds P^2 + 2P pairs; the first vertex lacks its P left
-// neighbours, which the shift by P accounts for.  An x-line has
-// nx (P^2 + 2P) + 1 pairs.
-__device__ __forceinline__ void x_pair(int t, int P, int& ix, int& jx) {
-  const int per = P * P + 2 * P;
-  const int tp = t + P;
-  const int e = tp / per, r = tp - e * per;
-  if (r < 2 * P + 1) {
-    ix = P * e;
-    jx = ix - P + r;
-  } else {
-    const int r2 = r - (2 * P + 1), k = r2 / (P + 1);
-    ix = P * e + 1 + k;
-    jx = P * e + (r2 - k * (P + 1));
-  }
-}
-
-// First pair of node ix of an x-line with nx elements; ix = P nx + 1 (one past
-// the last node, which lacks its P right neighbours) gives the pair count.
-__device__ __forceinline__ int x_pair_start(int ix, int P, int nx) {
-  if (ix > P * nx) return nx * (P * P + 2 * P) + 1;
-  const int e = ix / P, k = ix - e * P;
-  const int tp = e * (P * P + 2 * P) + (k ? 2 * P + 1 + (k - 1) * (P + 1) : 0);
-  return max(tp - P, 0);
-}
-
 // One wavefront (one block) per (component c, x-line (iy, iz), run of R
 // consecutive nodes ix, R = P (64 / (P^2 + 2P)) so that the run's pairs fit the
 // 64 lanes).  Lane l takes one pair (ix, jx): CSR row (c, ix, iy, iz) and its

@@ -41,7 +41,6 @@ import torch
 from . import _lib, fem
 from ._lib import Pools, Segment, call, ptr
 from .basis import basis
-from .mesh import DofMap, RegularMesh, structured_colours
 
 
 
@@ -74,71 +73,143 @@ def lognormal_field(n: int, seed: int, device) -> torch.Tensor:
     return torch.randn(n, dtype=torch.float64, generator=g, device=device).exp_()
 
 
+_Q1 = {}
+
+
+def _q1(device) -> fem.DeviceBasis:
+    """Trilinear basis tables on `device` (shared by every box)."""
+    key = str(device)
+    if key not in _Q1:
+        _Q1[key] = fem.DeviceBasis(basis(3, 1, 2), device)
+    return _Q1[key]
+
+
+class _BoxSolverMatrix:
+    """The reduced matrix's SELL-64 storage of a box (solver order, uniform slice
+    width): what the grouped server copies into its pools, and a solver matrix of
+    its own for the single-solve path (identity permutation)."""
+
+    def __init__(self, cnt: _lib.BoxCounts, device):
+        self.n = int(cnt.n_free)
+        self.stored = int(cnt.stored)
+        ns = int(cnt.n_slices)
+        self.slice_ptr = torch.empty(ns + 1, dtype=torch.int32, device=device)
+        self.src = torch.empty(self.stored, dtype=torch.int32, device=device)
+        if cnt.col16:
+            self.col16 = torch.empty(self.stored, dtype=torch.int16, device=device)
+            self.col_base = torch.empty(ns, dtype=torch.int32, device=device)
+            self.col = None
+        else:
+            self.col = torch.empty(self.stored, dtype=torch.int32, device=device)
+            self.col16 = self.col_base = None
+        self.perm = torch.arange(self.n, dtype=torch.int32, device=device)
+        colp = ptr(self.col16) if self.col16 is not None else ptr(self.col)
+        self.desc = _lib.MatrixDesc(self.n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), colp, ptr(self.perm), ns,
+                                    self.stored, ptr(self.col_base), None, 0)
+
+    def gather(self, vals, prec=fem.F64, out=None):
+        if out is None:
+            out = torch.empty(self.stored, dtype=torch.float64 if prec == fem.F64 else torch.float32,
+                              device=vals.device)
+        call("mife_gather_values", self.stored, ptr(self.src), ptr(vals), fem.F64, ptr(out), prec, fem._stream())
+        return out
+
+
+class _BoxReduction:
+    """Dirichlet reduction of a box: free DOFs ascending (the builder's new_idx /
+    free_idx), the right-hand side b = f - K_fd u_d and the field update."""
+
+    def __init__(self, st: "_Structure", cnt: _lib.BoxCounts, device):
+        self.st = st
+        self.n_free = int(cnt.n_free)
+        self.nnz = int(cnt.red_nnz)
+        self.free_idx = torch.empty(self.n_free, dtype=torch.int32, device=device)
+        self.new_idx = torch.empty(int(cnt.n_dofs), dtype=torch.int32, device=device)
+
+    def rhs(self, vals, u, rhs=None, out=None):
+        st = self.st
+        if out is None:
+            out = torch.empty(self.n_free, dtype=torch.float64, device=vals.device)
+        call("mife_solver_rhs", self.n_free, ptr(self.free_idx), ptr(st.row_ptr), ptr(st.col), ptr(vals),
+             ptr(self.new_idx), ptr(u), None, ptr(rhs), None, ptr(out), fem._stream())
+        return out
+
+    def field_update(self, x, u):
+        call("mife_field_update", self.n_free, ptr(self.free_idx), ptr(x), ptr(u), fem._stream())
+
+
 class _Structure:
-    """Device structures of one request shape (cached by (kind, n_elem))."""
+    """Device structures of one request shape (cached by (kind, n_elem)), built by
+    mife_box_build from closed forms of the box: trilinear elements, the
+    equations CSR, the Dirichlet reduction (diffusion: u = 0 on x = 0 and u = 1 on
+    x = 1; elasticity: clamped at x = 0) and the reduced matrix's SELL-64
+    storage.  The sizes come from the box alone, so nothing waits for the device."""
 
     def __init__(self, kind: str, n_elem, device):
         self.kind = kind
-        b = basis(3, 1, 2)
-        self.basis = b
-        extent = (1.0, 1.0, 1.0) if kind == "diffusion" else (3.0, 1.0, 1.0)
-        m = RegularMesh(n_elem, [b], extent)
-        self.mesh = m
+        self.n_elem = tuple(int(n) for n in n_elem)
+        self.device = device
         ncomp = 1 if kind == "diffusion" else 3
-        dm = DofMap(m, [0] * ncomp)
-        self.dm = dm
-        X = m.coords
-        N = m.num_nodes(0)
-        if kind == "diffusion":
-            fixed = (X[:, 0] == 0.0) | (X[:, 0] == 1.0)
-            ufix = np.where(X[:, 0] == 1.0, 1.0, 0.0)
-            rhs = None
-        else:
-            clamp = X[:, 0] == 0.0
-            fixed = np.concatenate([clamp, clamp, clamp])
-            ufix = np.zeros(3 * N)
+        self.ncomp = ncomp
+        extent = (1.0, 1.0, 1.0) if kind == "diffusion" else (3.0, 1.0, 1.0)
+        desc = _lib.BoxDesc(self.n_elem, 1, ncomp, 1, 1 if kind == "diffusion" else 0, 0, extent, 0.0,
+                            1.0 if kind == "diffusion" else 0.0)
+        cnt = _lib.BoxCounts()
+        call("mife_box_sizes", C.byref(desc), C.byref(cnt))
+        self.num_elements = int(cnt.n_elem)
+        self.num_nodes = int(cnt.n_nodes)
+        self.n = int(cnt.n_dofs)
+        self.nnz = int(cnt.nnz)
+        dev = device
+        self.coords = torch.empty((self.num_nodes, 3), dtype=torch.float64, device=dev)
+        self.elem_nodes = torch.empty((self.num_elements, 8), dtype=torch.int32, device=dev)
+        self.row_ptr = torch.empty(self.n + 1, dtype=torch.int32, device=dev)
+        self.col = torch.empty(self.nnz, dtype=torch.int32, device=dev)
+        self.u0 = torch.empty(self.n, dtype=torch.float64, device=dev)
+        self.red = _BoxReduction(self, cnt, dev)
+        self.S = _BoxSolverMatrix(cnt, dev)
+        S = self.S
+        arrays = _lib.BoxArrays(ptr(self.coords), ptr(self.elem_nodes), ptr(self.row_ptr), ptr(self.col),
+                                ptr(self.red.free_idx), ptr(self.red.new_idx), ptr(self.u0), ptr(S.slice_ptr),
+                                ptr(S.col), ptr(S.col16), ptr(S.col_base), ptr(S.src))
+        call("mife_box_build", C.byref(desc), C.byref(arrays), fem._stream())
+        self.rhs = None
+        if kind == "elasticity":
             # consistent nodal forces of t_z = -1 on the x = 3 face: int phi dA on the
             # bilinear face grid is h_y h_z (1/2 at face edges, 1 inside) per direction
-            ny, nz = n_elem[1], n_elem[2]
-            hy, hz = 1.0 / ny, 1.0 / nz
-            cy = np.where((np.arange(ny + 1) == 0) | (np.arange(ny + 1) == ny), 0.5, 1.0)
-            cz = np.where((np.arange(nz + 1) == 0) | (np.arange(nz + 1) == nz), 0.5, 1.0)
-            rhs = np.zeros(3 * N)
-            tip = np.nonzero(X[:, 0] == 3.0)[0]
-            gp = m.node_grid_pos[tip]
-            rhs[2 * N + tip] = -1.0 * hy * hz * cy[gp[:, 1]] * cz[gp[:, 2]]
-        self.fixed = fixed
-        self.ufix = ufix
-        self.csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, device, colours=structured_colours(m))
-        self.csr.pos   # the batched assembly scatters through it: build it with the structure
-        self.red = fem.DirichletReduction(self.csr, fixed)
-        self.rhs = None if rhs is None else torch.as_tensor(rhs, device=device)
-        self.u0 = torch.as_tensor(np.where(fixed, ufix, 0.0), device=device)
-        # the solver matrix's storage (SELL-64; elasticity rows node by node so that
-        # the columns of a slice fit 16 bits)
-        self.S = fem.SolverMatrix(self.red, schedule=False, interleave=None if ncomp == 1 else (3, N))
-        self.perm = self.S.perm.long()
-        if kind == "diffusion":
-            self.asm = fem.ScalarAssembler(b, X, m.elem_nodes[0], self.csr)
-        else:
-            self.asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], self.csr)
+            ny, nz = self.n_elem[1], self.n_elem[2]
+            Nx, Ny = self.n_elem[0] + 1, ny + 1
+            iy = torch.arange(ny + 1, device=dev)
+            iz = torch.arange(nz + 1, device=dev)
+            cy = torch.where((iy == 0) | (iy == ny), 0.5, 1.0).to(torch.float64)
+            cz = torch.where((iz == 0) | (iz == nz), 0.5, 1.0).to(torch.float64)
+            w = (-1.0 * (1.0 / ny) * (1.0 / nz)) * (cz[:, None] * cy[None, :])
+            tip = (Nx - 1) + Nx * (iy[None, :] + Ny * iz[:, None])
+            self.rhs = torch.zeros(self.n, dtype=torch.float64, device=dev)
+            self.rhs[2 * self.num_nodes + tip.reshape(-1)] = w.reshape(-1)
+        self.stage_bytes = int(_lib.lib().mife_assemble_grid_stage_bytes(self.num_elements, 8, ncomp, fem.F64))
+        self._grid = np.array(self.n_elem, dtype=np.int32)
+
+    def zeros(self):
+        return torch.zeros(self.nnz, dtype=torch.float64, device=self.device)
 
     def coefficients(self, seed: int, device):
         """Lognormal(0,1) element coefficients from the request's seed: conductivity, or
         Young's modulus 10 exp(N(0,1)) with Poisson ratio 0.3 (material [E][6])."""
-        z = lognormal_field(self.mesh.num_elements, seed, device)
+        z = lognormal_field(self.num_elements, seed, device)
         if self.kind == "diffusion":
             return z
         Ey = 10.0 * z
         nu = torch.full_like(Ey, 0.3)
         return torch.stack([Ey, Ey, Ey, nu, nu, nu], dim=1).contiguous()
 
-    def assemble(self, coef, vals):
-        vals.zero_()
-        if self.kind == "diffusion":
-            self.asm.assemble(vals, kappa=coef)
-        else:
-            self.asm.assemble(vals, coef)
+    def assemble(self, coef, vals, stage):
+        """FP64 equations values (overwritten) by the structured assembly; `stage`
+        holds at least stage_bytes bytes."""
+        q1 = _q1(self.device)
+        call("mife_assemble_grid", C.byref(q1.desc), self._grid.ctypes.data, self.ncomp, ptr(self.coords),
+             ptr(self.elem_nodes), ptr(coef), None, ptr(self.row_ptr), self.n, fem.F64, ptr(vals), None,
+             ptr(stage), stage.numel(), fem._stream())
         return vals
 
 
@@ -238,15 +309,13 @@ class Server:
         self.graph = None
         self.stats = dict(admitted=0, retired=0, chunks=0, graph_captures=0, graph_replays=0, table_builds=0,
                           iterations=0, spmv_bytes=0.0, assembly_batches=0)
-        self.q1 = fem.DeviceBasis(basis(3, 1, 2), dev)   # trilinear tables of the batched assembly
         # structures of the next `prefetch` queued requests are built by one worker
         # thread on its own stream while the main thread waits for the GPU
         self.prefetch = int(prefetch)
         self._pool = ThreadPoolExecutor(max_workers=1) if self.prefetch > 0 else None
         self._pf_stream = torch.cuda.Stream() if self.prefetch > 0 else None
         self._pending = {}
-        self.batch_dev = None
-        self.batch_host = None
+        self.stage = None   # element-matrix stage of the structured assembly (largest box so far)
         # host-side wall time per phase (seconds): structure builds, admission
         # (assembly + reduction + pool copies), iteration chunks, status reads, retirement
         self.timing = dict(structure=0.0, admit=0.0, chunk=0.0, status=0.0, retire=0.0)
@@ -324,29 +393,18 @@ class Server:
         req.t_admit = now
         self.stats["admitted"] += 1
         return dict(req=req, st=st, nf=nf, nnz=st.red.nnz, stored=stored, ns=ns, ro=ro, rpo=rpo, nzo=nzo, cbo=cbo,
-                    slot=slot, coef=coef, vals=st.csr.zeros(), mixed=req.kind in self.mixed_kinds)
+                    slot=slot, coef=coef, vals=torch.empty(st.nnz, dtype=torch.float64, device=self.device),
+                    mixed=req.kind in self.mixed_kinds)
 
     def _assemble_batch(self, entries):
-        """FP64 equations values of every entry in one launch per colour and element
-        kind (mife_assemble_batch)."""
-        jobs = (_lib.AssemblyJob * len(entries))()
-        maxc = 1
-        for k, e in enumerate(entries):
-            st = e["st"]
-            cs, asm = st.csr, st.asm
-            kind = _lib.MIFE_JOB_SCALAR if st.kind == "diffusion" else _lib.MIFE_JOB_ELASTIC
-            jobs[k] = _lib.AssemblyJob(kind, asm.n_elem, ptr(asm.coords), ptr(asm.elem_nodes), ptr(cs.elem_dofs),
-                                       ptr(e["coef"]), None, ptr(cs.pos), ptr(cs.order), cs.colour_ptr.ctypes.data,
-                                       cs.n_colours, 0, ptr(e["vals"]), None)
-            maxc = max(maxc, cs.n_colours)
-        L = _lib.lib()
-        nb = int(L.mife_assemble_batch_work_bytes(len(entries), maxc))
-        if self.batch_dev is None or self.batch_dev.numel() < nb:
-            self.batch_dev = torch.empty(nb, dtype=torch.uint8, device=self.device)
-            self.batch_host = np.empty(nb, dtype=np.uint8)
-        desc = C.byref(self.q1.desc)
-        call("mife_assemble_batch", desc, desc, len(entries), jobs, maxc, fem.F64, self.batch_host.ctypes.data,
-             ptr(self.batch_dev), fem._stream())
+        """FP64 equations values of every entry: the structured assembly of each box
+        (element matrices staged, then every CSR row written whole), one stage
+        buffer shared by the admissions, which run in order on one stream."""
+        need = max(e["st"].stage_bytes for e in entries)
+        if self.stage is None or self.stage.numel() < need:
+            self.stage = torch.empty(need, dtype=torch.uint8, device=self.device)
+        for e in entries:
+            e["st"].assemble(e["coef"], e["vals"], self.stage)
         self.stats["assembly_batches"] += 1
 
     def _finish(self, e):
@@ -369,10 +427,8 @@ class Server:
             self.cbase[cbo:cbo + ns].copy_(S.col_base[:ns])
         else:
             self.col[nzo:nzo + stored].copy_(S.col[:stored])
-        b = torch.empty(nf, dtype=torch.float64, device=self.device)
-        call("mife_solver_rhs", nf, ptr(red.free_idx), ptr(st.csr.row_ptr), ptr(st.csr.col), ptr(vals),
-             ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(b), s)
-        self.b[ro:ro + nf].copy_(b[st.perm])          # slice numbering
+        call("mife_solver_rhs", nf, ptr(red.free_idx), ptr(st.row_ptr), ptr(st.col), ptr(vals),
+             ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(self.b) + 8 * ro, s)
         self.x[ro:ro + nf].zero_()
         e["seg"] = Segment(nf, 200000, e["slot"], _lib.MIFE_FMT_SELL, ro, rpo, nzo, e["req"].rtol, ns,
                            fem.F32 if e["mixed"] else fem.F64, cbo if c16 else -1)
@@ -407,10 +463,8 @@ class Server:
     def _retire(self, e, now):
         req, seg, st = e["req"], e["seg"], e["st"]
         nf = e["nf"]
-        xf = torch.empty(nf, dtype=torch.float64, device=self.device)
-        xf[st.perm] = self.x[seg.row_off:seg.row_off + nf]   # back from slice numbering
         u = st.u0.clone()
-        st.red.field_update(xf, u)
+        st.red.field_update(self.x[seg.row_off:seg.row_off + nf], u)
         req.u = u.cpu().numpy()
         req.t_done = time.perf_counter() - now
         self.alloc_rows.release(seg.row_off, nf)
@@ -556,7 +610,7 @@ class Server:
 
 def solve_one_fp64(req: Request, structures: dict, device=None) -> Request:
     """The one-at-a-time FP64 path for a request: assemble, reduce, Jacobi CG, field
-    update (the single-solve engine)."""
+    update (the single-solve engine on the request's own structures)."""
     dev = fem._dev(device)
     key = (req.kind, tuple(req.n_elem))
     st = structures.get(key)
@@ -565,10 +619,9 @@ def solve_one_fp64(req: Request, structures: dict, device=None) -> Request:
         structures.clear()
         structures[key] = st
     coef = st.coefficients(req.seed, dev)
-    vals = st.csr.zeros()
-    st.assemble(coef, vals)
-    red = st.red
-    S = fem.SolverMatrix(red, interleave=None if req.kind == "diffusion" else (3, st.mesh.num_nodes(0)))
+    vals = torch.empty(st.nnz, dtype=torch.float64, device=dev)
+    st.assemble(coef, vals, torch.empty(st.stage_bytes, dtype=torch.uint8, device=dev))
+    red, S = st.red, st.S
     b = red.rhs(vals, st.u0, rhs=st.rhs)
     x = torch.zeros(red.n_free, dtype=torch.float64, device=dev)
     ws = fem.KrylovWorkspace(red.n_free, dev)

@@ -11,13 +11,39 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 
+class _Mesh:
+    """A generated trilinear box with the generic structures: CSR, position map,
+    structured colouring and the per-mesh assembler."""
+
+    def __init__(self, kind, ne):
+        from iron_amd import fem
+        from iron_amd.basis import basis
+        from iron_amd.mesh import DofMap, RegularMesh, structured_colours
+        self.kind = kind
+        b = basis(3, 1, 2)
+        m = RegularMesh(ne, [b], (1.0, 1.0, 1.0) if kind == "diffusion" else (3.0, 1.0, 1.0))
+        self.mesh = m
+        dm = DofMap(m, [0] * (1 if kind == "diffusion" else 3))
+        self.csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, "cuda", colours=structured_colours(m))
+        A = fem.ScalarAssembler if kind == "diffusion" else fem.ElasticAssembler
+        self.asm = A(b, m.coords, m.elem_nodes[0], self.csr)
+
+    def coefficients(self, seed, dev):
+        from iron_amd import serving
+        z = serving.lognormal_field(self.mesh.num_elements, seed, dev)
+        if self.kind == "diffusion":
+            return z
+        nu = torch.full_like(z, 0.3)
+        return torch.stack([10 * z, 10 * z, 10 * z, nu, nu, nu], dim=1).contiguous()
+
+
 def test_batch_equals_per_mesh_assembly():
-    from iron_amd import _lib, fem, serving
+    from iron_amd import _lib, fem
     from iron_amd.basis import basis
     print("case batched assembly of 6 meshes", flush=True)
     shapes = [("diffusion", (4, 3, 5)), ("elasticity", (6, 2, 3)), ("diffusion", (7, 7, 6)),
               ("elasticity", (3, 3, 3)), ("diffusion", (1, 1, 1)), ("elasticity", (1, 2, 1))]
-    sts = [serving._Structure(k, ne, "cuda") for k, ne in shapes]
+    sts = [_Mesh(k, ne) for k, ne in shapes]
     q1 = fem.DeviceBasis(basis(3, 1, 2), "cuda")
     L = _lib.lib()
     for prec in (fem.F64, fem.F32):
