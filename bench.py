@@ -36,6 +36,11 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+def progress(msg):
+    """One progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,7 +59,7 @@ def parse():
                          "writes 268,435,456 bytes each: the reference kernel for HBM byte counters)")
     ap.add_argument("--serve-requests", type=int, default=1024,
                     help="requests of the served-stream leg (0 skips it)")
-    ap.add_argument("--serve-policies", default="auto,mixed,fp64",
+    ap.add_argument("--serve-policies", default="auto,split,mixed,fp64",
                     help="precision policies of the served stream, each measured at capacity and at Poisson "
                          "arrivals")
     ap.add_argument("--legs", default="main,mr,elastic,laplace,slab,serve",
@@ -758,6 +763,7 @@ def serve_leg(n, policies, seed=2024):
         wall_p = time.perf_counter() - t
         lat = np.array([r.latency for r in out_p]) * 1e3
         host = timing["admit"] + timing["structure"] + timing["retire"] + timing["chunk"]
+        progress(f"served stream, policy {pol}: {cap:.1f} solves/s at capacity")
         pol_out[pol] = {
             "capacity_solves_per_s": cap, "elems_per_s_at_capacity": sum(int(np.prod(r.n_elem)) for r in out) / wall,
             "poisson_rate_per_s": 0.7 * cap, "poisson_solves_per_s": n / wall_p,
@@ -772,12 +778,15 @@ def serve_leg(n, policies, seed=2024):
     service = []
     seq = serving.make_stream(n, seed=seed)
     t = time.perf_counter()
-    for r in seq:
+    for i, r in enumerate(seq):
         t1 = time.perf_counter()
         serving.solve_one_fp64(r, structures)
         service.append(time.perf_counter() - t1)
+        if (i + 1) % 256 == 0:
+            progress(f"served stream one at a time: {i + 1} of {n}")
     torch.cuda.synchronize()
     wall_seq = time.perf_counter() - t
+    progress(f"served stream one at a time: {n / wall_seq:.1f} solves/s")
     service = np.array(service) * 1e3
     # roofline of the grouped product: the first 256 requests served without graph
     # replay (every launch carries HIP events) and without the structure prefetch thread
@@ -801,8 +810,9 @@ def serve_leg(n, policies, seed=2024):
     res = {
         "workload": f"served stream: {n} requests, 70% diffusion / 30% elasticity Q1 boxes N~U{{8..64}}^3, "
                     "lognormal(0,1) coefficients, Jacobi CG to a 1e-8 FP64 residual, grouped CG + HBM-pool "
-                    "admission; policies: auto (diffusion FP32 iterations with FP64 reliable updates, elasticity "
-                    "FP64), mixed (every request FP32 + FP64 reliable updates), fp64",
+                    "admission on device-built box structures; policies: auto (the single-solve rule per request: "
+                    "FP64 on boxes this size), split (diffusion FP32 iterations with FP64 reliable updates, "
+                    "elasticity FP64), mixed (every request FP32 + FP64 reliable updates), fp64",
         "requests": n, "elements_total": sum(int(np.prod(r.n_elem)) for r in seq),
         "capacity_solves_per_s": main["capacity_solves_per_s"],
         "p50_latency_ms": main["p50_latency_ms"], "p99_latency_ms": main["p99_latency_ms"],
@@ -917,20 +927,27 @@ def main():
         del a, bcopy
     legs = set(args.legs.split(","))
     cpu = 0.0 if args.no_cpu_baseline else args.cpu_seconds
+    progress(f"legs {sorted(legs)}")
     out = main_leg(args) if "main" in legs else {"legs": sorted(legs)}
+    progress("main leg done")
     torch.cuda.empty_cache()
     if "mr" in legs and not args.no_mr:
         out["mooney_rivlin"] = mr_leg(2, min(cpu, 8.0))
+        progress("Mooney-Rivlin leg done")
         torch.cuda.empty_cache()
     if "elastic" in legs:
         out["elasticity96"] = elastic_leg(cpu_seconds=min(cpu, 10.0))
+        progress("elasticity leg done")
         torch.cuda.empty_cache()
     if "laplace" in legs:
         out["laplace16"] = laplace_leg(args, cpu_seconds=min(cpu, 4.0))
+        progress("Laplace leg done")
     if "slab" in legs:
         out["diffusion48_slab"] = slab_leg(args)
+        progress("slab leg done")
     if "serve" in legs and args.serve_requests > 0:
         out["served_stream"] = serve_leg(args.serve_requests, args.serve_policies.split(","))
+        progress("served-stream leg done")
     print(json.dumps(out))
 
 

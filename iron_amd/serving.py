@@ -11,11 +11,16 @@ test.  A request iterates either in FP64 or in FP32 (FP32 copy of the values,
 FP32 vectors) with FP64 reliable updates: its solution is accumulated in FP64
 and its residual is replaced by the true FP64 residual b - A x after every
 decade of FP32 reduction, so it stops on its FP64 residual (1e-8 relative).
-Both kinds share the same launches.  The default policy picks FP32 iterations
-for diffusion requests and FP64 for elasticity requests: on the lognormal
+Both kinds share the same launches.  The default policy ("auto") applies the
+single-solve rule per request, which keeps these boxes (at most a few million
+stored entries) in FP64: there a request is bound by launches and the Infinity
+Cache, not by HBM bytes, so FP32 values save little, and on the lognormal
 elasticity boxes the FP32 operator's rounding (a 1e-4..1e-3 relative change of
-A^-1) makes each replacement restart part of the convergence, about twice the
-FP64 iteration count, which costs more than the FP32 bytes save.  Between chunks of iterations the server
+A^-1) makes each replacement restart part of the convergence -- about 2.8x the
+FP64 iterations over a 1024-request stream, a net loss ("mixed" runs it; "split"
+runs diffusion in FP32 and elasticity in FP64).  Requests are stored on
+device-built box structures (mife_box_build) and assembled by the structured
+grid path.  Between chunks of iterations the server
 retires converged requests (their solution goes back to the host and their
 pool ranges are freed) and admits waiting ones, first come first served, as
 long as their rows and entries fit the pool budget (HBM-budget admission).  A
@@ -246,12 +251,16 @@ class _FirstFit:
 class Server:
     """Grouped-CG serving of Request objects within an HBM budget (bytes).
 
-    precision: "auto" (FP32 iterations with FP64 reliable updates for diffusion
-    requests, FP64 for elasticity), "mixed" (all FP32 + FP64 reliable updates) or
-    "fp64".  Reliable updates follow every ``inner_rtol`` reduction of a request's
-    FP32 residual, tested every ``check_every`` iterations."""
+    precision: "auto" (per request, the single-solve rule fem.auto_precision: FP32
+    iterations with FP64 reliable updates for a scalar field whose solver matrix
+    holds at least fem.MIXED_MIN_STORED entries, else FP64 -- on boxes of up to 64^3
+    trilinear elements every request iterates in FP64), "split" (FP32 + reliable
+    updates for diffusion requests, FP64 for elasticity), "mixed" (every request
+    FP32 + reliable updates) or "fp64".  Reliable updates follow every
+    ``inner_rtol`` reduction of a request's FP32 residual, tested every
+    ``check_every`` iterations."""
 
-    MIXED_KINDS = {"auto": ("diffusion",), "mixed": ("diffusion", "elasticity"), "fp64": ()}
+    MIXED_KINDS = {"auto": None, "split": ("diffusion",), "mixed": ("diffusion", "elasticity"), "fp64": ()}
 
     def __init__(self, hbm_budget: float = 32e9, max_segments: int = 256, nz_per_row: float = 60.0,
                  chunk: int = 16, use_graph: bool = True, device=None, cache_shapes: int = 8,
@@ -353,6 +362,11 @@ class Server:
             self.cache.move_to_end(key)
         return st
 
+    def _mixed(self, st: "_Structure") -> bool:
+        if self.mixed_kinds is None:
+            return fem.auto_precision(st.ncomp, st.S.stored) == "mixed"
+        return st.kind in self.mixed_kinds
+
     def fits(self, req: Request) -> bool:
         st = self._structure(req)
         S = st.S
@@ -394,7 +408,7 @@ class Server:
         self.stats["admitted"] += 1
         return dict(req=req, st=st, nf=nf, nnz=st.red.nnz, stored=stored, ns=ns, ro=ro, rpo=rpo, nzo=nzo, cbo=cbo,
                     slot=slot, coef=coef, vals=torch.empty(st.nnz, dtype=torch.float64, device=self.device),
-                    mixed=req.kind in self.mixed_kinds)
+                    mixed=self._mixed(st))
 
     def _assemble_batch(self, entries):
         """FP64 equations values of every entry: the structured assembly of each box

@@ -60,7 +60,8 @@ def _mix():
             serving.Request(4, "diffusion", (3, 3, 3), 15)]
 
 
-@pytest.mark.parametrize("use_graph,precision", [(False, "fp64"), (True, "fp64"), (False, "mixed"), (True, "auto")])
+@pytest.mark.parametrize("use_graph,precision", [(False, "fp64"), (True, "fp64"), (False, "mixed"), (True, "auto"),
+                                                (True, "split")])
 def test_grouped_requests_match_oracle_and_single_solve(use_graph, precision):
     from iron_amd import serving
     print(f"case serving mix graph={use_graph} precision={precision}", flush=True)
@@ -80,11 +81,11 @@ def test_grouped_requests_match_oracle_and_single_solve(use_graph, precision):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_alone_equals_in_mix_bitwise(precision):
     """The target request (elasticity) iterates in FP64 or FP32; the mix around it
-    holds requests of both precisions (policy "auto") or all FP32."""
+    holds requests of both precisions (policy "split") or all FP32."""
     from iron_amd import serving
     print(f"case serving alone vs mix precision={precision}", flush=True)
     target = serving.Request(9, "elasticity", (5, 3, 4), 99)
-    other = "auto" if precision == "fp64" else "mixed"
+    other = "split" if precision == "fp64" else "mixed"
     alone = serving.Server(hbm_budget=2e8, max_segments=16, chunk=8, precision=precision).run(
         [serving.Request(9, "elasticity", (5, 3, 4), 99)])[0]
     mix = _mix() + [target]
@@ -130,3 +131,66 @@ def test_mixed_and_fp64_servers_agree_and_mixed_stops_on_the_fp64_residual():
               f"FP64 {r64.iterations}", flush=True)
         if r64.kind == "diffusion":   # FP32 iterations track the FP64 ones on diffusion
             assert r32.iterations <= 1.25 * r64.iterations + 8, (rid, r32.iterations, r64.iterations)
+
+
+@pytest.mark.parametrize("kind,ne", [("diffusion", (64, 64, 64)), ("elasticity", (64, 48, 40)),
+                                     ("diffusion", (8, 64, 13))])
+def test_served_solution_at_full_size_has_the_residual_it_reports(kind, ne):
+    """Requests at the stream's largest sizes (beyond the dense oracle): the served
+    solution u, put through an independent FP64 path -- the generic structures
+    (mesh, DOF map, CSR pattern, position-map assembly, Dirichlet reduction) and
+    the CSR SpMV -- has the relative residual ||b_f - K_ff u_f|| / ||b_f|| the
+    server reports (<= 1e-8), its fixed DOFs hold the Dirichlet values, and the
+    elasticity reactions balance the applied load."""
+    from iron_amd import fem, serving
+    from iron_amd.basis import basis
+    from iron_amd.mesh import DofMap, RegularMesh
+    print(f"case served full size {kind} {ne}", flush=True)
+    req = serving.Request(0, kind, ne, 77)
+    out = serving.Server(hbm_budget=8e9, max_segments=4, chunk=16).run([req])[0]
+    assert out.relres <= 1e-8
+    b = basis(3, 1, 2)
+    extent = (1.0, 1.0, 1.0) if kind == "diffusion" else (3.0, 1.0, 1.0)
+    m = RegularMesh(ne, [b], extent)
+    nc = 1 if kind == "diffusion" else 3
+    dm = DofMap(m, [0] * nc)
+    csr = fem.CSRStructure(dm.element_dofs(), dm.num_dofs, "cuda")
+    st = serving._Structure(kind, ne, "cuda")
+    coef = st.coefficients(req.seed, "cuda")
+    A = fem.ScalarAssembler if kind == "diffusion" else fem.ElasticAssembler
+    vals = csr.zeros()
+    if kind == "diffusion":
+        A(b, m.coords, m.elem_nodes[0], csr).assemble(vals, kappa=coef)
+    else:
+        A(b, m.coords, m.elem_nodes[0], csr).assemble(vals, coef)
+    X = m.coords
+    N = len(X)
+    if kind == "diffusion":
+        fixed = (X[:, 0] == 0.0) | (X[:, 0] == 1.0)
+        ufix = np.where(X[:, 0] == 1.0, 1.0, 0.0)
+    else:
+        fixed = np.tile(X[:, 0] == 0.0, 3)
+        ufix = np.zeros(3 * N)
+    assert np.array_equal(out.u[fixed], ufix[fixed])
+    rp, col = csr.to_host()
+    u = torch.as_tensor(out.u, device="cuda")
+    Ku = torch.zeros(csr.n, dtype=torch.float64, device="cuda")
+    rows = torch.repeat_interleave(torch.arange(csr.n, device="cuda"), torch.as_tensor(np.diff(rp), device="cuda"))
+    Ku.index_add_(0, rows, vals[: csr.nnz] * u[torch.as_tensor(col, device="cuda")])
+    f = st.rhs if st.rhs is not None else torch.zeros(csr.n, dtype=torch.float64, device="cuda")
+    free = torch.as_tensor(~fixed, device="cuda")
+    # b_f = f_f - K_fd u_d, K_ff u_f = K u - K_fd u_d on the free rows
+    ud = torch.as_tensor(np.where(fixed, ufix, 0.0), device="cuda")
+    Kud = torch.zeros(csr.n, dtype=torch.float64, device="cuda")
+    Kud.index_add_(0, rows, vals[: csr.nnz] * ud[torch.as_tensor(col, device="cuda")])
+    bf = (f - Kud)[free]
+    rf = (f - Ku)[free]
+    rel = float(torch.linalg.vector_norm(rf) / torch.linalg.vector_norm(bf))
+    print(f"served {kind} {ne}: reported {out.relres:.3e}, independent {rel:.3e}, {out.iterations} its", flush=True)
+    assert rel <= 1.05e-8
+    assert abs(rel - out.relres) <= 0.05 * out.relres + 1e-12
+    if kind == "elasticity":
+        # reactions at the clamp balance the applied z load (-1)
+        R = (Ku - f)[torch.as_tensor(fixed, device="cuda")]
+        Rz = float(R[2 * int(fixed.sum()) // 3:].sum())
+        assert abs(Rz - 1.0) <= 1e-6
