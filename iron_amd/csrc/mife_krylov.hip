@@ -27,7 +27,6 @@ extern "C" int mife_profile_enable(int on);
 namespace mife {
 
 constexpr int kRedBlocks = kMaxSellGrid;  // partial slots per reduced value (largest row grid)
-constexpr int kDotBlocks = kMaxGrid;     // GMRES multidot partial slots (vector grid)
 constexpr int kMaxRestart = 256;
 constexpr int kFinThreads = 1024;  // one-block finishers: two partials per thread
 constexpr int kMaxIterChunk = 256;  // CG iterations launched between two host reads of the state
@@ -358,88 +357,81 @@ __global__ void fill_kernel(int n, T* __restrict__ v, T val) {
 }
 
 // ------------------------------------------------------------------ GMRES
-// Partials of V_j . w for j < m over a fixed grid; reduced by the Gram-Schmidt update.
-// blockIdx.y selects a chunk of 32 basis vectors (m <= 32 * gridDim.y).
-__global__ __launch_bounds__(256) void multidot_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
-                                                       const double* __restrict__ w, double* __restrict__ parts,
-                                                       int pstride) {
-  __shared__ double red[32][4];
-  const int j0 = blockIdx.y * 32;
-  const int mc = min(32, m - j0);
-  // fully unrolled with predicates, so the 32 accumulators stay in registers
-  double acc[32];
-#pragma unroll
-  for (int j = 0; j < 32; ++j) acc[j] = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double wi = w[i];
-    const double* Vi = V + (size_t)j0 * ldv + i;
-#pragma unroll
-    for (int j = 0; j < 32; ++j)
-      if (j < mc) acc[j] += Vi[(size_t)j * ldv] * wi;
+// Classical Gram-Schmidt of the new vector w against the basis V_0..V_{m-1}
+// (columns of V, stride ldv), two passes per iteration.  On the GMRES systems
+// here n is a few 10^4 rows and m up to 257, so each pass is a matrix-vector
+// product of a few 10 MB that must spread over the whole chip: the dots take one
+// 1024-thread block per basis vector (fixed-order sums, no partials), and the
+// update takes one block per 64 rows with its 8 wavefronts splitting the basis
+// vectors, their partial sums combined in wavefront order.
+constexpr int kDotThreads = 1024;
+constexpr int kAxpyWaves = 8;
+
+// h[j] = V_j . w for j < gridDim.x.
+__global__ __launch_bounds__(kDotThreads) void gs_dots_kernel(int n, const double* __restrict__ V, size_t ldv,
+                                                              const double* __restrict__ w, double* __restrict__ h) {
+  __shared__ double red[kDotThreads / 64];
+  const double* Vj = V + (size_t)blockIdx.x * ldv;
+  double s0 = 0.0, s1 = 0.0;
+  int i = threadIdx.x;
+  for (; i + kDotThreads < n; i += 2 * kDotThreads) {
+    s0 += Vj[i] * w[i];
+    s1 += Vj[i + kDotThreads] * w[i + kDotThreads];
   }
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const double s = wave_sum(acc[j]);
-    if (j < mc && (threadIdx.x & 63) == 0) red[j][threadIdx.x >> 6] = s;
-  }
+  if (i < n) s0 += Vj[i] * w[i];
+  const double s = wave_sum(s0 + s1);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x < mc) {
-    double s = 0.0;
-    for (int wv = 0; wv < 4; ++wv) s += red[threadIdx.x][wv];
-    parts[(size_t)(j0 + threadIdx.x) * pstride + blockIdx.x] = s;
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kDotThreads / 64; ++k) t += red[k];
+    h[blockIdx.x] = t;
   }
 }
 
-// Row blocks of the Gram-Schmidt dot products: few enough that every block of the
-// following update can reduce all partials of all m coefficients itself.
-constexpr int kGsBlocks = 32;
-
-// h[j] = sum over the kGsBlocks partials of coefficient j (fixed order), for j < m,
-// into LDS hs; block 0 also stores h to h_out.
-__device__ __forceinline__ void gs_coefficients(int m, const double* __restrict__ parts, double* hs,
-                                                double* __restrict__ h_out) {
-  for (int j = threadIdx.x; j < m; j += blockDim.x) {
-    const double* pj = parts + (size_t)j * kGsBlocks;
-    double s = 0.0;
-#pragma unroll 8
-    for (int b = 0; b < kGsBlocks; ++b) s += pj[b];
-    hs[j] = s;
-    if (blockIdx.x == 0) h_out[j] = s;
-  }
+// w -= sum_{j < m} V_j h_j; NORM: also the block partials of the updated w.w
+// into nparts[blockIdx.x] (blocks loop over 64-row tiles).
+template <bool NORM>
+__global__ __launch_bounds__(64 * kAxpyWaves) void gs_axpy_kernel(int n, int m, const double* __restrict__ V,
+                                                                  size_t ldv, const double* __restrict__ h,
+                                                                  double* __restrict__ w,
+                                                                  double* __restrict__ nparts) {
+  __shared__ double hs[kMaxRestart + 1];
+  __shared__ double part[kAxpyWaves][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = h[j];
   __syncthreads();
-}
-
-// w -= sum_j V_j h_j with h reduced from the dot partials by every block.
-__global__ __launch_bounds__(256) void gs_update_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
-                                                        const double* __restrict__ parts, double* __restrict__ h_out,
-                                                        double* __restrict__ w) {
-  __shared__ double hs[kMaxRestart + 1];
-  gs_coefficients(m, parts, hs, h_out);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = w[i];
-    for (int j = 0; j < m; ++j) s -= V[j * ldv + i] * hs[j];
-    w[i] = s;
+  // wavefront q takes basis vectors [j0, j1)
+  const int per = (m + kAxpyWaves - 1) / kAxpyWaves;
+  const int j0 = min(m, q * per), j1 = min(m, j0 + per);
+  double nrm = 0.0;
+  for (int r0 = blockIdx.x * 64; r0 < n; r0 += gridDim.x * 64) {
+    const int row = r0 + lane;
+    double a0 = 0.0, a1 = 0.0;
+    if (row < n) {
+      int j = j0;
+      for (; j + 1 < j1; j += 2) {
+        a0 += V[(size_t)j * ldv + row] * hs[j];
+        a1 += V[(size_t)(j + 1) * ldv + row] * hs[j + 1];
+      }
+      if (j < j1) a0 += V[(size_t)j * ldv + row] * hs[j];
+    }
+    part[q][lane] = a0 + a1;
+    __syncthreads();
+    if (q == 0 && row < n) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < kAxpyWaves; ++k) acc += part[k][lane];
+      const double wi = w[row] - acc;
+      w[row] = wi;
+      nrm += wi * wi;
+    }
+    __syncthreads();
   }
-}
-
-// As gs_update_kernel, and the per-block partials of the updated w.w (the new
-// vector's norm) into nparts[blockIdx.x].
-__global__ __launch_bounds__(256) void gs_update_norm_kernel(int n, int m, const double* __restrict__ V, size_t ldv,
-                                                             const double* __restrict__ parts,
-                                                             double* __restrict__ h_out, double* __restrict__ w,
-                                                             double* __restrict__ nparts) {
-  __shared__ double hs[kMaxRestart + 1];
-  __shared__ double red[4];
-  gs_coefficients(m, parts, hs, h_out);
-  double q = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double s = w[i];
-    for (int j = 0; j < m; ++j) s -= V[j * ldv + i] * hs[j];
-    w[i] = s;
-    q += s * s;
+  if (NORM && q == 0) {
+    const double t = wave_sum(nrm);
+    if (lane == 0) nparts[blockIdx.x] = t;
   }
-  q = block_sum(q, red);
-  if (threadIdx.x == 0) nparts[blockIdx.x] = q;
 }
 
 // Device state of one GMRES restart cycle.
@@ -962,7 +954,7 @@ static void minres_layout(Work& w, int n, MinresBufs& B) {
 
 struct GmresBufs {
   CgState* st;
-  double *dinv, *r, *t, *wv, *b, *x, *V, *parts, *hdev, *ydev, *ndev, *Hdev, *csdev, *sndev, *gdev;
+  double *dinv, *r, *t, *wv, *b, *x, *V, *hdev, *ydev, *ndev, *Hdev, *csdev, *sndev, *gdev;
   GmresCycle* cy;
   size_t ldv;
 };
@@ -977,7 +969,6 @@ static void gmres_layout(Work& w, int n, int m, GmresBufs& B) {
   B.x = w.take<double>(n);
   B.ldv = ((size_t)n + 63) & ~size_t(63);
   B.V = w.take<double>(B.ldv * (m + 1));
-  B.parts = w.take<double>((size_t)kDotBlocks * (m + 1));
   B.hdev = w.take<double>(m + 2);
   B.ydev = w.take<double>(m + 2);
   B.ndev = w.take<double>(kMaxGrid);
@@ -1339,12 +1330,13 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   gmres_layout(w, n, m, B);
   if ((rc = check_fit(w, "GMRES"))) return rc;
   CgState* st = B.st;
-  double *dinv = B.dinv, *r = B.r, *t = B.t, *wv = B.wv, *b = B.b, *x = B.x, *V = B.V, *parts = B.parts;
+  double *dinv = B.dinv, *r = B.r, *t = B.t, *wv = B.wv, *b = B.b, *x = B.x, *V = B.V;
   double *hdev = B.hdev, *ydev = B.ydev, *ndev = B.ndev, *Hdev = B.Hdev, *csdev = B.csdev, *sndev = B.sndev,
          *gdev = B.gdev;
   GmresCycle* cy = B.cy;
   const size_t ldv = B.ldv;
   const int vg = vec_grid(n);
+  const int ag = std::min(div_up(n, 64), kMaxGrid);  // Gram-Schmidt update: 64-row tiles
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
@@ -1372,16 +1364,14 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
       for (int c = 0; c < cnt; ++c) {
         const int kk = k + c;
         launch_spmv<double>(M.a, vals, t, wv, M.L, s);   // t = D^-1 V_kk (written with V_kk)
-        // classical Gram-Schmidt, two passes: dot partials over kGsBlocks row
-        // blocks, then an update whose every block reduces the coefficients
-        // itself; the second update also yields the norm partials, which the
-        // rotation kernel reduces
-        const dim3 mg(kGsBlocks, div_up(kk + 1, 32));
-        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts, kGsBlocks);
-        gs_update_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, parts, hdev, wv);
-        multidot_kernel<<<mg, 256, 0, s>>>(n, kk + 1, V, ldv, wv, parts, kGsBlocks);
-        gs_update_norm_kernel<<<vg, 256, 0, s>>>(n, kk + 1, V, ldv, parts, ydev, wv, ndev);
-        gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, ydev, ndev, vg, Hdev, csdev, sndev, gdev, cy, rtol * bn,
+        // classical Gram-Schmidt, two passes (coefficients h, then the
+        // correction y); the second update also yields the norm partials, which
+        // the rotation kernel reduces
+        gs_dots_kernel<<<kk + 1, kDotThreads, 0, s>>>(n, V, ldv, wv, hdev);
+        gs_axpy_kernel<false><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, hdev, wv, nullptr);
+        gs_dots_kernel<<<kk + 1, kDotThreads, 0, s>>>(n, V, ldv, wv, ydev);
+        gs_axpy_kernel<true><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, ydev, wv, ndev);
+        gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, ydev, ndev, ag, Hdev, csdev, sndev, gdev, cy, rtol * bn,
                                              (it + c + 1 >= maxit || kk + 1 == m) ? 1 : 0);
         gmres_next_basis_kernel<<<vg, 256, 0, s>>>(n, wv, V + (kk + 1) * ldv, dinv, t, cy);
       }

@@ -81,8 +81,9 @@ def test_argument_errors_without_gpu():
     assert "precision" in _lib.last_error()
     assert L.mife_cg(ctypes.byref(d), 1, 1, 1, 0, -1.0, 10, 1, 1 << 30, None, None, None) == 2   # negative rtol
     # a workspace smaller than the solver's layout is refused before any launch,
-    # for every solver and every GMRES restart up to 256
-    n = 1000
+    # for every solver and every GMRES restart up to 256 (n large enough that the
+    # GMRES basis, not the reduction state, sets the size)
+    n = 20000
     d = M(n, 0, 1, 1, None, 0, 10, None, None, 0)
     w = L.mife_solver_work_bytes(n, 30)
     assert L.mife_cg(ctypes.byref(d), 1, 1, 1, 0, 1e-8, 10, 1, 1024, None, None, None) == 2
