@@ -360,46 +360,53 @@ __global__ void fill_kernel(int n, T* __restrict__ v, T val) {
 // Classical Gram-Schmidt of the new vector w against the basis V_0..V_{m-1}
 // (columns of V, stride ldv), two passes per iteration.  On the GMRES systems
 // here n is a few 10^4 rows and m up to 257, so each pass is a matrix-vector
-// product of a few 10 MB that must spread over the whole chip: the dots take one
-// 1024-thread block per basis vector (fixed-order sums, no partials), and the
-// update takes one block per 64 rows with its 8 wavefronts splitting the basis
-// vectors, their partial sums combined in wavefront order.
-constexpr int kDotThreads = 1024;
+// product of a few 10 MB that must spread over the whole chip: the dots take a
+// block per (basis vector, eighth of the rows), whose partials every consumer
+// sums in chunk order, and the update takes one block per 64 rows with its 8
+// wavefronts splitting the basis vectors, their partial sums combined in
+// wavefront order.
+constexpr int kDotChunks = 8;
 constexpr int kAxpyWaves = 8;
 
-// h[j] = V_j . w for j < gridDim.x.
-__global__ __launch_bounds__(kDotThreads) void gs_dots_kernel(int n, const double* __restrict__ V, size_t ldv,
-                                                              const double* __restrict__ w, double* __restrict__ h) {
-  __shared__ double red[kDotThreads / 64];
-  const double* Vj = V + (size_t)blockIdx.x * ldv;
+// parts[j kDotChunks + c] = V_j . w over row chunk c; block (c, j).
+__global__ __launch_bounds__(256) void gs_dots_kernel(int n, const double* __restrict__ V, size_t ldv,
+                                                      const double* __restrict__ w, double* __restrict__ parts) {
+  __shared__ double red[4];
+  const int c = blockIdx.x, j = blockIdx.y;
+  const int per = (n + kDotChunks - 1) / kDotChunks;
+  const int i1 = min(n, (c + 1) * per);
+  const double* Vj = V + (size_t)j * ldv;
   double s0 = 0.0, s1 = 0.0;
-  int i = threadIdx.x;
-  for (; i + kDotThreads < n; i += 2 * kDotThreads) {
+  int i = c * per + threadIdx.x;
+  for (; i + 256 < i1; i += 512) {
     s0 += Vj[i] * w[i];
-    s1 += Vj[i + kDotThreads] * w[i + kDotThreads];
+    s1 += Vj[i + 256] * w[i + 256];
   }
-  if (i < n) s0 += Vj[i] * w[i];
-  const double s = wave_sum(s0 + s1);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int k = 0; k < kDotThreads / 64; ++k) t += red[k];
-    h[blockIdx.x] = t;
-  }
+  if (i < i1) s0 += Vj[i] * w[i];
+  const double t = block_sum(s0 + s1, red);
+  if (threadIdx.x == 0) parts[j * kDotChunks + c] = t;
 }
 
-// w -= sum_{j < m} V_j h_j; NORM: also the block partials of the updated w.w
-// into nparts[blockIdx.x] (blocks loop over 64-row tiles).
+// Coefficient j from its chunk partials (fixed order).
+__device__ __forceinline__ double dot_coef(const double* __restrict__ parts, int j) {
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < kDotChunks; ++c) s += parts[j * kDotChunks + c];
+  return s;
+}
+
+// w -= sum_{j < m} V_j h_j with h from the dot partials hp; NORM: also the block
+// partials of the updated w.w into nparts[blockIdx.x] (blocks loop over 64-row
+// tiles).
 template <bool NORM>
 __global__ __launch_bounds__(64 * kAxpyWaves) void gs_axpy_kernel(int n, int m, const double* __restrict__ V,
-                                                                  size_t ldv, const double* __restrict__ h,
+                                                                  size_t ldv, const double* __restrict__ hp,
                                                                   double* __restrict__ w,
                                                                   double* __restrict__ nparts) {
   __shared__ double hs[kMaxRestart + 1];
   __shared__ double part[kAxpyWaves][64];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = h[j];
+  for (int j = threadIdx.x; j < m; j += blockDim.x) hs[j] = dot_coef(hp, j);
   __syncthreads();
   // wavefront q takes basis vectors [j0, j1)
   const int per = (m + kAxpyWaves - 1) / kAxpyWaves;
@@ -441,7 +448,8 @@ struct GmresCycle {
   int k_used;  // Hessenberg columns formed
 };
 
-// Column k of the Hessenberg matrix from the two Gram-Schmidt passes and the new
+// Column k of the Hessenberg matrix from the two Gram-Schmidt passes (their dot
+// partials h1, h2) and the new
 // vector's norm, the previous Givens rotations applied to it, rotation k formed
 // and g rotated; H is (m+1) x m, row-major.  The column and the rotations are
 // staged in LDS by the block, thread 0 runs the sequential rotations, and the
@@ -461,7 +469,7 @@ __global__ __launch_bounds__(256) void gmres_givens_kernel(int k, int m, const d
   double q = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) q += nparts[b];
   const double nrm2 = block_sum(q, red);
-  for (int j = threadIdx.x; j <= k; j += blockDim.x) col[j] = h1[j] + h2[j];
+  for (int j = threadIdx.x; j <= k; j += blockDim.x) col[j] = dot_coef(h1, j) + dot_coef(h2, j);
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
     c_s[j] = cs[j];
     s_s[j] = sn[j];
@@ -969,8 +977,8 @@ static void gmres_layout(Work& w, int n, int m, GmresBufs& B) {
   B.x = w.take<double>(n);
   B.ldv = ((size_t)n + 63) & ~size_t(63);
   B.V = w.take<double>(B.ldv * (m + 1));
-  B.hdev = w.take<double>(m + 2);
-  B.ydev = w.take<double>(m + 2);
+  B.hdev = w.take<double>((size_t)(m + 2) * kDotChunks);   // dot partials (first pass)
+  B.ydev = w.take<double>((size_t)(m + 2) * kDotChunks);   // dot partials (second pass); y of the update
   B.ndev = w.take<double>(kMaxGrid);
   B.Hdev = w.take<double>((size_t)(m + 1) * m);
   B.csdev = w.take<double>(m + 2);
@@ -1367,9 +1375,9 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
         // classical Gram-Schmidt, two passes (coefficients h, then the
         // correction y); the second update also yields the norm partials, which
         // the rotation kernel reduces
-        gs_dots_kernel<<<kk + 1, kDotThreads, 0, s>>>(n, V, ldv, wv, hdev);
+        gs_dots_kernel<<<dim3(kDotChunks, kk + 1), 256, 0, s>>>(n, V, ldv, wv, hdev);
         gs_axpy_kernel<false><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, hdev, wv, nullptr);
-        gs_dots_kernel<<<kk + 1, kDotThreads, 0, s>>>(n, V, ldv, wv, ydev);
+        gs_dots_kernel<<<dim3(kDotChunks, kk + 1), 256, 0, s>>>(n, V, ldv, wv, ydev);
         gs_axpy_kernel<true><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, ydev, wv, ndev);
         gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, ydev, ndev, ag, Hdev, csdev, sndev, gdev, cy, rtol * bn,
                                              (it + c + 1 >= maxit || kk + 1 == m) ? 1 : 0);
