@@ -39,8 +39,16 @@ Parity pinning (tests/test_mr_oracle.py):
     with the pressure doubled: the files were written by a version whose second Piola
     tensor carried 2 p C^-1.  The "deviatoric" branch that the current source runs for the
     MOONEY_RIVLIN subtype gives a different solution (tip moves by 0.34) and is pinned by
-    the first three checks only.  output_shear is the NEARLY_INCOMPRESSIBLE subtype and
-    output_cantilever/cubic_2x1x1 is cubic Hermite; neither is restated.
+    the first three checks only.  output_cantilever/cubic_2x1x1 is cubic Hermite and is
+    not restated.
+  * the NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN subtype (formulation
+    "nearly_incompressible", no pressure field): the compressible branch of the residual,
+    finite_elasticity_routines.F90:3293-3360, with the second Piola tensor of
+    FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR :5308-5320 pushed forward.  Its Jacobian here is
+    a central difference of the element residual (iron's own for this subtype is the
+    equations' finite-difference Jacobian).  Pinned by R = 0 in the reference state, by
+    the homogeneous volumetric stretch closed form and by the golden
+    tests/FiniteElasticity/output_shear/SimpleShear.part0 (deformed nodes and DELUDELN).
 """
 from __future__ import annotations
 
@@ -105,6 +113,27 @@ def gauss_cauchy_tensor(F, p, c1, c2):
     return np.einsum("...ik,...kl,...jl->...ij", F, S, F) / J[..., None, None]
 
 
+def gauss_nearly_incompressible_cauchy(F, c1, c2, kappa):
+    """FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR, NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN case,
+    finite_elasticity_routines.F90:5308-5320, pushed forward as for the other laws:
+    W^v(J) = kappa (J - 1)^2 / 2, WV' = kappa (J - 1),
+    S = 2 J^(-2/3) [(c1 + c2 I1) I - c2 C - (c1 I1 + 2 c2 I2 - 1.5 WV' J^(5/3)) / 3 C^-1],
+    sigma = F S F^T / J, with C = F^T F, I1 = tr C, I2 = (I1^2 - tr C^2) / 2, J = det F.
+    Returns the 3x3 Cauchy tensor."""
+    eye = np.eye(3)
+    azl = np.einsum("...ki,...kj->...ij", F, F)
+    azu = np.linalg.inv(azl)
+    J = np.linalg.det(F)
+    i1 = azl[..., 0, 0] + azl[..., 1, 1] + azl[..., 2, 2]
+    sq = np.einsum("...ij,...jk->...ik", azl, azl)
+    i2 = 0.5 * (i1 ** 2 - sq[..., 0, 0] - sq[..., 1, 1] - sq[..., 2, 2])
+    wv = kappa * (J - 1.0)
+    coef = (c1 * i1 + 2.0 * c2 * i2 - 1.5 * wv * J ** (5.0 / 3.0)) / 3.0
+    S = 2.0 * (J ** (-2.0 / 3.0))[..., None, None] * ((c1 + c2 * i1)[..., None, None] * eye
+                                                     - c2[..., None, None] * azl - coef[..., None, None] * azu)
+    return np.einsum("...ik,...kl,...jl->...ij", F, S, F) / J[..., None, None]
+
+
 # ----------------------------------------------------------------------------
 # Element residual, gravity vector, exact Jacobian
 # ----------------------------------------------------------------------------
@@ -115,7 +144,7 @@ def _pressure_at_gauss(pe, bp: Basis | None):
 
 
 def mr_element_residual(geo_e, dep_e, pe, bu: Basis, bp: Basis | None, c1, c2,
-                        formulation="deviatoric"):
+                        formulation="deviatoric", kappa=None):
     """FiniteElasticity_FiniteElementResidualEvaluate, finite_elasticity_routines.F90.
 
     formulation "deviatoric": the MOONEY_RIVLIN subtype branch :1669-1756 (isochoric
@@ -123,6 +152,9 @@ def mr_element_residual(geo_e, dep_e, pe, bu: Basis, bp: Basis | None, c1, c2,
     formulation "cauchy": the general branch taken by EQUATIONS_SET_NO_SUBTYPE,
     :1824-2000, with the Mooney-Rivlin law of FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR
     (:6036-6068): S = 2[c1 I + c2 (I1 I - C)] + p C^-1.
+    formulation "nearly_incompressible": the compressible branch :3293-3360 with
+    gauss_nearly_incompressible_cauchy (kappa [E]); no pressure DOFs (pe, bp unused),
+    the result is [E, 3*nn].
 
     geo_e, dep_e: [E, nn, 3] geometric and deformed nodal coordinates; pe: [E, npn]
     nodal pressures (bp = pressure basis) or [E] element pressures (bp = None);
@@ -138,9 +170,14 @@ def mr_element_residual(geo_e, dep_e, pe, bu: Basis, bp: Basis | None, c1, c2,
     # dZ/dNu = dZ/dXi . dXi/dNu with the material frame = the global frame (:5171-5175)
     F = np.einsum("egij,egjk->egik", dz, dxi_dx)
     jznu = jz / jx
-    p = _pressure_at_gauss(pe, bp) * np.ones_like(jz)
     cc1 = np.broadcast_to(c1[:, None], jz.shape)
     cc2 = np.broadcast_to(c2[:, None], jz.shape)
+    if formulation == "nearly_incompressible":
+        # sum_g w Jxxi Jznu sigma_ck dphi_a/dz_k (:3345-3351)
+        T = gauss_nearly_incompressible_cauchy(F, cc1, cc2, np.broadcast_to(kappa[:, None], jz.shape))
+        T = T * (jz * w)[..., None, None]
+        return np.einsum("egkn,egkm->emn", dphidz, T).reshape(E, 3 * nn)
+    p = _pressure_at_gauss(pe, bp) * np.ones_like(jz)
     if formulation == "deviatoric":
         sig = gauss_stress_tensor(F, jznu, p, cc1, cc2)
         T = sig[..., TENSOR_TO_VOIGT3]
@@ -304,16 +341,20 @@ class MRProblem:
     gravity: tuple | None = None
     gravity_jacobian: str = "geometric"
     formulation: str = "deviatoric"
+    kappa: np.ndarray | None = None   # nearly_incompressible: bulk penalty per element
 
     def __post_init__(self):
         m = self.mesh
         self.bu = m.bases[0]
-        self.bp = None if self.p_element_based else m.bases[1]
+        self.bp = None if (self.p_element_based or self.p_element_based is None) else m.bases[1]
         self.nu = len(m.comp_nodes[0])
         lut = np.full(int(m.comp_nodes[0].max()) + 1, -1, dtype=np.int64)
         lut[m.comp_nodes[0]] = np.arange(self.nu)
         self.en_u = lut[m.elem_nodes[0]]
-        if self.p_element_based:
+        if self.p_element_based is None:   # no pressure field (nearly incompressible)
+            self.npd = 0
+            self.en_p = np.zeros((m.num_elements, 0), dtype=np.int64)
+        elif self.p_element_based:
             self.npd = m.num_elements
             self.en_p = np.arange(m.num_elements)[:, None]
         else:
@@ -336,14 +377,18 @@ class MRProblem:
         z, p = self.split(x)
         geo_e = self.geo[self.en_u]
         dep_e = z[self.en_u]
-        pe = p[self.en_p[:, 0]] if self.p_element_based else p[self.en_p]
+        if self.npd == 0:
+            pe = None
+        else:
+            pe = p[self.en_p[:, 0]] if self.p_element_based else p[self.en_p]
         return geo_e, dep_e, pe
 
     def residual(self, x, load=1.0):
         """(R_int, F_ext) as global vectors: R_int the assembled nonlinear residual, F_ext
         the assembled gravity vector scaled by the current load fraction."""
         geo_e, dep_e, pe = self.element_arrays(x)
-        re = mr_element_residual(geo_e, dep_e, pe, self.bu, self.bp, self.c1, self.c2, self.formulation)
+        re = mr_element_residual(geo_e, dep_e, pe, self.bu, self.bp, self.c1, self.c2, self.formulation,
+                                 self.kappa)
         R = np.zeros(self.ndofs)
         np.add.at(R, self.edofs, re)
         f = np.zeros(self.ndofs)
@@ -353,9 +398,29 @@ class MRProblem:
             np.add.at(f, self.edofs[:, :fe.shape[1]], fe)
         return R, f
 
-    def jacobian(self, x):
+    def element_jacobian_fd(self, x, h=1e-7):
+        """Central difference of the element residual in each element DOF (all elements
+        at once): [E, n, n]."""
         geo_e, dep_e, pe = self.element_arrays(x)
-        ke = mr_element_jacobian(geo_e, dep_e, pe, self.bu, self.bp, self.c1, self.c2, self.formulation)
+        E, nn, _ = dep_e.shape
+        n = 3 * nn
+        K = np.zeros((E, n, n))
+        for k in range(n):
+            c, a = divmod(k, nn)
+            dp, dm = dep_e.copy(), dep_e.copy()
+            dp[:, a, c] += h
+            dm[:, a, c] -= h
+            rp = mr_element_residual(geo_e, dp, pe, self.bu, self.bp, self.c1, self.c2, self.formulation, self.kappa)
+            rm = mr_element_residual(geo_e, dm, pe, self.bu, self.bp, self.c1, self.c2, self.formulation, self.kappa)
+            K[:, :, k] = (rp - rm) / (2.0 * h)
+        return K
+
+    def jacobian(self, x):
+        if self.formulation == "nearly_incompressible":
+            ke = self.element_jacobian_fd(x)
+        else:
+            geo_e, dep_e, pe = self.element_arrays(x)
+            ke = mr_element_jacobian(geo_e, dep_e, pe, self.bu, self.bp, self.c1, self.c2, self.formulation)
         n = self.edofs.shape[1]
         rows = np.repeat(self.edofs, n, axis=1).ravel()
         cols = np.tile(self.edofs, (1, n)).ravel()

@@ -64,6 +64,28 @@ def main(ref_root: str):
         (HERE / name).write_text(json.dumps(data, indent=1))
         print("wrote", name)
 
+    # Nearly incompressible Mooney-Rivlin simple shear (no pressure field): nodal
+    # deformed coordinates and DELUDELN values, element connectivity.
+    stem = "tests/FiniteElasticity/output_shear/SimpleShear.part0"
+    nodes = read_exnode(ref / (stem + ".exnode"))
+    elems = read_exelem(ref / (stem + ".exelem"))
+    data = {
+        "source": stem + ".exnode/.exelem",
+        "setup": {"comment": "tests/FiniteElasticity/SimpleShear.F90: 2x2x2 triquadratic Lagrange, 3x3x3 Gauss, "
+                             "extent 1x1x1, EQUATIONS_SET_NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE (3 "
+                             "components), materials c1=0.5, c2=0, kappa=1e4, top (z=1) nodes fixed at "
+                             "(x+0.1, y, z), bottom (z=0) nodes fixed at (x, y, z), one load step, Newton with "
+                             "the equations-calculated (finite-difference) Jacobian and a direct linear solve",
+                  "n_elem": [2, 2, 2], "extent": [1.0, 1.0, 1.0], "c1": 0.5, "c2": 0.0, "kappa": 10000.0,
+                  "shear": 0.1},
+        "nodes": {n: {"dependent": [rec["Dependent"][c][0] for c in range(3)],
+                      "deludeln": [rec["del U/del n"][c][0] for c in range(3)]}
+                  for n, rec in nodes.items()},
+        "elements": {e: {"nodes": rec["nodes"]} for e, rec in elems.items()},
+    }
+    (HERE / "mr_simple_shear_2x2x2.json").write_text(json.dumps(data, indent=1))
+    print("wrote mr_simple_shear_2x2x2.json")
+
 
 if __name__ == "__main__":
     main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

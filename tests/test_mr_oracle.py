@@ -175,3 +175,55 @@ def test_homogeneous_state_in_equilibrium_on_non_box_geometry(form, order, elem_
     assert scale > 1e-3                                  # boundary rows carry the traction
     assert np.abs(Ru[:, ~on_b]).max() < 1e-11 * scale
     assert np.abs(R[3 * prob.nu:]).max() < 1e-11 * scale
+
+
+def _simple_shear():
+    g = json.loads((GOLDEN / "mr_simple_shear_2x2x2.json").read_text())
+    s = g["setup"]
+    bu = Basis(3, 2, 3)
+    m = regular_mesh(tuple(s["n_elem"]), [bu], tuple(s["extent"]))
+    E = m.num_elements
+    prob = mo.MRProblem(m, None, np.full(E, s["c1"]), np.full(E, s["c2"]), formulation="nearly_incompressible",
+                        kappa=np.full(E, s["kappa"]))
+    z = prob.geo[:, 2]
+    top, bottom = z == s["extent"][2], z == 0.0
+    fixed = np.tile(top | bottom, 3)
+    x0 = prob.initial_state()
+    value = x0.copy()
+    value[:prob.nu][top] += s["shear"] * s["extent"][0]
+    return g, prob, fixed, x0, value
+
+
+def test_nearly_incompressible_reference_state_and_volumetric_stretch():
+    """R = 0 in the reference state (S = 2[(c1 + c2 I1) I - c2 C - (c1 I1 + 2 c2 I2)/3 C^-1]
+    vanishes at C = I); under a homogeneous volumetric stretch z = s X the Cauchy stress is
+    the pure pressure sigma = kappa (J - 1) I (the isochoric part vanishes at C = s^2 I),
+    so the element residual is the traction of that pressure."""
+    bu = Basis(3, 2, 3)
+    m = regular_mesh((1, 1, 1), [bu], (1.0, 1.0, 1.0))
+    prob = mo.MRProblem(m, None, np.array([0.7]), np.array([0.3]), formulation="nearly_incompressible",
+                        kappa=np.array([50.0]))
+    x = prob.initial_state()
+    R, _ = prob.residual(x)
+    assert np.abs(R).max() < 1e-13
+    s = 1.01
+    F = np.eye(3) * s
+    sig = mo.gauss_nearly_incompressible_cauchy(F[None], np.array([0.7]), np.array([0.3]), np.array([50.0]))[0]
+    assert np.allclose(sig, 50.0 * (s ** 3 - 1.0) * np.eye(3), rtol=1e-12, atol=1e-12)
+
+
+def test_simple_shear_golden():
+    """tests/FiniteElasticity/output_shear/SimpleShear.part0: the nearly incompressible
+    Mooney-Rivlin box sheared by 0.1 at its top face; deformed nodes and DELUDELN (the
+    reactions at the fixed faces, ~1e-9 elsewhere) against the reference's files."""
+    g, prob, fixed, x0, value = _simple_shear()
+    x, R = mo.solve(prob, x0, fixed, fixed_value=value)
+    z, _ = prob.split(x)
+    nodes = prob.mesh.comp_nodes[0]
+    gz = np.array([g["nodes"][str(n)]["dependent"] for n in nodes])
+    gr = np.array([g["nodes"][str(n)]["deludeln"] for n in nodes])
+    print("max node difference", np.abs(z - gz).max(), "max DELUDELN difference", np.abs(R.reshape(3, -1).T - gr).max())
+    assert np.abs(z - gz).max() < 1e-7
+    assert np.abs(R.reshape(3, -1).T - gr).max() < 1e-5 * np.abs(gr).max()
+    for e in range(prob.mesh.num_elements):
+        assert list(prob.mesh.elem_nodes[0][e]) == g["elements"][str(e + 1)]["nodes"]
