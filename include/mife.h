@@ -347,14 +347,18 @@ int mife_assemble_batch(const mife_element_desc* scalar_desc, const mife_element
 /* Mooney-Rivlin finite elasticity (3-D, FP64). */
 #define MIFE_MR_DEVIATORIC 0 /* EQUATIONS_SET_MOONEY_RIVLIN_SUBTYPE branch */
 #define MIFE_MR_CAUCHY 1     /* EQUATIONS_SET_NO_SUBTYPE general branch */
+#define MIFE_MR_NEARLY_INCOMPRESSIBLE 2 /* EQUATIONS_SET_NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE:
+                                           bulk penalty, no pressure DOFs */
 
 typedef struct mife_mr_desc {
   const mife_element_desc* u; /* displacement (= geometric) basis: dim 3, dN, N, w */
-  int n_pressure;             /* pressure DOFs per element: 1 (element based) or the
-                                 pressure basis' node count (node based) */
+  int n_pressure;             /* pressure DOFs per element: 1 (element based), the
+                                 pressure basis' node count (node based), or 0
+                                 (MIFE_MR_NEARLY_INCOMPRESSIBLE, which has no pressure) */
   const double* Np;           /* [n_gauss][n_pressure] pressure basis at the displacement
                                  Gauss points; NULL when n_pressure == 1 */
-  int formulation;            /* MIFE_MR_DEVIATORIC or MIFE_MR_CAUCHY */
+  int formulation;            /* MIFE_MR_DEVIATORIC, MIFE_MR_CAUCHY or
+                                 MIFE_MR_NEARLY_INCOMPRESSIBLE */
 } mife_mr_desc;
 
 /* Residual, exact Jacobian and gravity vector of the Mooney-Rivlin equations,
@@ -362,7 +366,8 @@ typedef struct mife_mr_desc {
  * FiniteElasticity_FiniteElementResidualEvaluate (MOONEY_RIVLIN subtype
  * src/finite_elasticity_routines.F90:1669-1760 with
  * FINITE_ELASTICITY_GAUSS_STRESS_TENSOR :6985-7058, or the general branch
- * :1824-2000 with FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR :6036-6068), the gravity
+ * :1824-2000 with FINITE_ELASTICITY_GAUSS_CAUCHY_TENSOR :6036-6068, or the
+ * NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN branch :3293-3360 with its law :5308-5320), the gravity
  * term :3370-3430, FiniteElasticity_FiniteElementJacobianEvaluate (:879-1296; here
  * the exact derivative of the residual, which iron's FD path
  * src/equations_set_routines.F90:2839-2962 approximates) and EquationsMatrices_ElementAdd.
@@ -371,7 +376,8 @@ typedef struct mife_mr_desc {
  *   elem_dofs  [n_elem][3*n_nodes + n_pressure] global DOFs: x over the element
  *              nodes, then y, z, then the pressure DOFs
  *   state      [n_dofs] current dependent values (deformed coordinates, pressures)
- *   material   [n_elem][2] c1, c2
+ *   material   [n_elem][2] c1, c2; [n_elem][3] c1, c2, kappa (bulk penalty) for
+ *              MIFE_MR_NEARLY_INCOMPRESSIBLE
  *   body_force NULL or host [3] = density * gravity * load fraction
  *   pos        [n_elem][ne][ne] CSR positions (ne = 3*n_nodes + n_pressure), needed
  *              with jac_vals

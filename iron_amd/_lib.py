@@ -100,6 +100,7 @@ class MrDesc(C.Structure):
 
 MIFE_MR_DEVIATORIC = 0
 MIFE_MR_CAUCHY = 1
+MIFE_MR_NEARLY_INCOMPRESSIBLE = 2
 
 _lib = None
 

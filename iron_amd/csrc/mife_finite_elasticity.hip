@@ -14,7 +14,8 @@
 // The residual is written in the reference configuration,
 //   R_(i,a) = sum_g w |J_x| dphi_a/dX_K P_iK,   R_q = sum_g w |J_x| (det F - 1) psi_q,
 // which equals the Cauchy-stress form dphi/dz . sigma |J_z| w of
-// finite_elasticity_routines.F90:1669-1756 (P = det F sigma F^-T).
+// finite_elasticity_routines.F90:1669-1756 (P = det F sigma F^-T).  The nearly
+// incompressible law (:3293-3360, :5308-5320) has no pressure DOFs and no R_q.
 #include "mife_common.hpp"
 #include "mife_profile.hpp"
 
@@ -50,7 +51,8 @@ struct GaussState {
   double P[9];                           // first Piola stress
   double dPdp[9];                        // dP/dp
   double JG[9];                          // det F * F^-T = d(det F)/dF
-  double s[8];                           // I1, I2, J, a, b, p, c1, c2
+  double s[8];                           // I1, I2, J, a, b, p, c1, c2; nearly incompressible:
+                                         // I1, I2, J, a, coef, dV, c1, c2
   double xi[9];                          // dxi_m/dX_K as xi[m*3+K]
   double wj;                             // w |det dX/dxi|
 };
@@ -72,7 +74,7 @@ template <int NN, int NP, int NG, int FORM>
 struct MrLDS {
   double X[NN][3];
   double Z[NN][3];
-  double p[NP];
+  double p[NP > 0 ? NP : 1];
   mr::GaussState gs[NG];
   double A[NG][81];
   double GX[NG][NN][3];
@@ -98,6 +100,18 @@ __device__ __forceinline__ double tangent_entry(const mr::GaussState& g, int idx
     const double ib2 = -(4.0 / 3.0) * b * GjL * (I2F[i * 3 + K] - (4.0 / 3.0) * I2 * GiK) +
                        b * (i2ff - (4.0 / 3.0) * GiK * I2F[j * 3 + L] + (4.0 / 3.0) * I2 * GiL * GjK);
     return c1 * ib1 + c2 * ib2 + p * J * (GjL * GiK - GiL * GjK);
+  } else if constexpr (FORM == MIFE_MR_NEARLY_INCOMPRESSIBLE) {
+    // P = F S with S = 2a T, T = (c1 + c2 I1) I - c2 C - coef C^-1, a = J^(-2/3),
+    // coef = (c1 I1 + 2 c2 I2 - 1.5 kappa (J - 1) J^(5/3)) / 3:
+    // dP_iK/dF_jL = dij S_LK - 2/3 G_jL P_iK + 2a [F dT/dF_jL]_iK, using dI1 = 2F,
+    // dI2 = 2(I1 F - F C), dJ = J G, dC^-1_MK = -(C^-1_ML G_jK + G_jM C^-1_LK), F C^-1 = G.
+    const double I1 = g.s[0], a = g.s[3], coef = g.s[4], dV = g.s[5], c1 = g.s[6];
+    const double FjL = F[j * 3 + L];
+    const double FC = F[j * 3 + 0] * g.C[0 * 3 + L] + F[j * 3 + 1] * g.C[1 * 3 + L] + F[j * 3 + 2] * g.C[2 * 3 + L];
+    const double dcoef = (2.0 * c1 * FjL + 4.0 * c2 * (I1 * FjL - FC) - dV * G[j * 3 + L]) * (1.0 / 3.0);
+    const double FdT = 2.0 * c2 * FjL * F[i * 3 + K] - c2 * (F[i * 3 + L] * F[j * 3 + K] + g.B[i * 3 + j] * dKL) -
+                       dcoef * G[i * 3 + K] + coef * (G[i * 3 + L] * G[j * 3 + K] + dij * g.Ci[L * 3 + K]);
+    return dij * g.X[L * 3 + K] - (2.0 / 3.0) * G[j * 3 + L] * g.P[i * 3 + K] + 2.0 * a * FdT;
   } else {
     const double* S = g.X;
     return dij * S[L * 3 + K] +
@@ -166,7 +180,8 @@ __global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
 #pragma unroll
       for (int q = 0; q < NP; ++q) pg += Np[g * NP + q] * S.p[q];
     }
-    const double c1 = material[e * 2 + 0], c2 = material[e * 2 + 1];
+    constexpr int MS = FORM == MIFE_MR_NEARLY_INCOMPRESSIBLE ? 3 : 2;  // material values per element
+    const double c1 = material[e * MS + 0], c2 = material[e * MS + 1];
     mr::GaussState& gs = S.gs[g];
     double C[3][3], B[3][3];
 #pragma unroll
@@ -210,6 +225,34 @@ __global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
         }
       gs.s[3] = a;
       gs.s[4] = b;
+      gs.s[5] = pg;
+    } else if constexpr (FORM == MIFE_MR_NEARLY_INCOMPRESSIBLE) {
+      // S = 2 J^(-2/3) [(c1 + c2 I1) I - c2 C - (c1 I1 + 2 c2 I2 - 1.5 WV' J^(5/3)) / 3 C^-1],
+      // WV' = kappa (J - 1)
+      const double kappa = material[e * MS + 2];
+      const double a = pow(J, -2.0 / 3.0), J23 = 1.0 / a, J53 = J * J23;
+      const double coef = (c1 * I1 + 2.0 * c2 * I2 - 1.5 * kappa * (J - 1.0) * J53) * (1.0 / 3.0);
+      double Ci[3][3];
+      mr::inv3(C, Ci);
+      double Sm[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          Sm[i][j] = 2.0 * a * ((i == j ? c1 + c2 * I1 : 0.0) - c2 * C[i][j] - coef * Ci[i][j]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int K = 0; K < 3; ++K) {
+          gs.X[i * 3 + K] = Sm[i][K];
+          gs.Ci[i * 3 + K] = Ci[i][K];
+          gs.P[i * 3 + K] = F[i][0] * Sm[0][K] + F[i][1] * Sm[1][K] + F[i][2] * Sm[2][K];
+          gs.dPdp[i * 3 + K] = 0.0;
+        }
+      gs.s[3] = a;
+      gs.s[4] = coef;
+      // d(1.5 kappa (J - 1) J^(5/3)) / dF_jL = dV G_jL
+      gs.s[5] = 1.5 * kappa * J * (J53 + (5.0 / 3.0) * (J - 1.0) * J23);
     } else {
       double Ci[3][3];
       mr::inv3(C, Ci);
@@ -230,11 +273,11 @@ __global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
         }
       gs.s[3] = 0.0;
       gs.s[4] = 0.0;
+      gs.s[5] = pg;
     }
     gs.s[0] = I1;
     gs.s[1] = I2;
     gs.s[2] = J;
-    gs.s[5] = pg;
     gs.s[6] = c1;
     gs.s[7] = c2;
     gs.wj = fabs(detx) * w[g];
@@ -316,6 +359,7 @@ __global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
     }
     // K_up = GX . dP/dp psi and K_pu = psi JG . GX (K_pp = 0): sums over the Gauss points
     constexpr int NUP = 3 * NN * NP;
+    if constexpr (NP > 0)
     for (int ent = t; ent < 2 * NUP; ent += kMrThreads) {
       const bool up = ent < NUP;
       const int k = up ? ent : ent - NUP;
@@ -344,7 +388,7 @@ __global__ __launch_bounds__(kMrThreads) void mr_element_kernel(
         r += S.gs[g].wj *
              (S.GX[g][a][0] * P[i * 3 + 0] + S.GX[g][a][1] * P[i * 3 + 1] + S.GX[g][a][2] * P[i * 3 + 2]);
       }
-    } else {
+    } else if constexpr (NP > 0) {
       const int q = t - 3 * NN;
       for (int g = 0; g < NG; ++g) {
         const double psi = (NP == 1) ? 1.0 : Np[g * NP + q];
@@ -409,11 +453,15 @@ extern "C" int mife_mr_assemble(const mife_mr_desc* d, int n_elem, const double*
   MIFE_REQUIRE(d->u->dN && d->u->w, MIFE_ERR_POINTER_IS_NULL, "displacement basis without dN or w");
   MIFE_REQUIRE(!jac_vals || pos, MIFE_ERR_POINTER_IS_NULL, "Jacobian requested without a position map");
   MIFE_REQUIRE(!rhs || !body_force || d->u->N, MIFE_ERR_POINTER_IS_NULL, "gravity requested without basis values N");
-  MIFE_REQUIRE(d->n_pressure == 1 || d->Np, MIFE_ERR_POINTER_IS_NULL, "node-based pressure without its basis values");
+  MIFE_REQUIRE(d->n_pressure <= 1 || d->Np, MIFE_ERR_POINTER_IS_NULL, "node-based pressure without its basis values");
   MIFE_REQUIRE(!order || colour_ptr_host, MIFE_ERR_POINTER_IS_NULL, "colour order without colour_ptr");
   MIFE_REQUIRE(d->u->dim == 3, MIFE_ERR_ARG, "finite elasticity kernel is 3-D only");
-  MIFE_REQUIRE(d->formulation == MIFE_MR_DEVIATORIC || d->formulation == MIFE_MR_CAUCHY, MIFE_ERR_ARG,
-               "invalid formulation %d", d->formulation);
+  MIFE_REQUIRE(d->formulation == MIFE_MR_DEVIATORIC || d->formulation == MIFE_MR_CAUCHY ||
+                   d->formulation == MIFE_MR_NEARLY_INCOMPRESSIBLE,
+               MIFE_ERR_ARG, "invalid formulation %d", d->formulation);
+  MIFE_REQUIRE((d->formulation == MIFE_MR_NEARLY_INCOMPRESSIBLE) == (d->n_pressure == 0), MIFE_ERR_ARG,
+               "the nearly incompressible law has no pressure DOFs, the others need them (n_pressure %d)",
+               d->n_pressure);
   if (n_elem == 0 || (!residual && !jac_vals && !rhs)) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
   const int key = d->u->n_nodes * 10000 + d->n_pressure * 100 + d->u->n_gauss;
@@ -426,7 +474,16 @@ extern "C" int mife_mr_assemble(const mife_mr_desc* d, int n_elem, const double*
     return launch_mr<NN, NP, NG, MIFE_MR_CAUCHY>(d, n_elem, coords, elem_nodes, elem_dofs, state, material,   \
                                                  body_force, jac_vals ? pos : nullptr, order, colour_ptr_host, \
                                                  n_colours, residual, jac_vals, rhs, s);
+#define MR_NI_CASE(NN, NG)                                                                                 \
+  case NN * 10000 + NG:                                                                                     \
+    return launch_mr<NN, 0, NG, MIFE_MR_NEARLY_INCOMPRESSIBLE>(d, n_elem, coords, elem_nodes, elem_dofs,    \
+                                                               state, material, body_force,                 \
+                                                               jac_vals ? pos : nullptr, order,             \
+                                                               colour_ptr_host, n_colours, residual,        \
+                                                               jac_vals, rhs, s);
   switch (key) {
+    MR_NI_CASE(8, 8)
+    MR_NI_CASE(27, 27)
     MR_CASE(8, 1, 8)
     MR_CASE(8, 8, 8)
     MR_CASE(27, 1, 27)
@@ -437,4 +494,5 @@ extern "C" int mife_mr_assemble(const mife_mr_desc* d, int n_elem, const double*
       return MIFE_ERR_ARG;
   }
 #undef MR_CASE
+#undef MR_NI_CASE
 }

@@ -34,6 +34,7 @@ from .mesh import RegularMesh, structured_colours
 
 DEVIATORIC = _lib.MIFE_MR_DEVIATORIC
 CAUCHY = _lib.MIFE_MR_CAUCHY
+NEARLY_INCOMPRESSIBLE = _lib.MIFE_MR_NEARLY_INCOMPRESSIBLE
 
 
 class MRSystem:
@@ -41,7 +42,8 @@ class MRSystem:
 
     Displacement (and geometry) on mesh component 0 (``rm.bases[0]``); pressure
     either node based on mesh component ``p_mesh_comp`` or element based
-    (``p_mesh_comp=None``).  DOF layout (separated order): x over the
+    (``p_mesh_comp=None``), and absent for the NEARLY_INCOMPRESSIBLE formulation
+    (``p_mesh_comp`` ignored).  DOF layout (separated order): x over the
     displacement nodes, then y, then z, then the pressure DOFs.  ``coords`` are the
     reference node coordinates on mesh component 0 (the geometric field's values;
     default the generated mesh's)."""
@@ -59,7 +61,12 @@ class MRSystem:
         self.nu = rm.num_nodes(0)
         en_u = rm.elem_nodes[0]
         E = rm.num_elements
-        if p_mesh_comp is None:
+        if self.formulation == NEARLY_INCOMPRESSIBLE:
+            self.bp = None
+            self.npd = 0
+            en_p = np.zeros((E, 0), dtype=np.int32)
+            self.n_pressure = 0
+        elif p_mesh_comp is None:
             self.bp = None
             self.npd = E
             en_p = np.arange(E, dtype=np.int32)[:, None]
@@ -117,6 +124,8 @@ class MRSystem:
     def pressure_mass(self, mu_e: np.ndarray) -> np.ndarray:
         """Lumped sum_e (1/mu_e) int psi_q dV over the pressure DOFs ([ndofs], zero on
         displacement DOFs): the diagonal Schur-complement estimate."""
+        if self.npd == 0:
+            return np.zeros(self.ndofs)
         bu = self.bu
         X = self.X[self.rm.elem_nodes[0]]                          # [E, nn, 3]
         J = np.einsum("ead,gaj->egdj", X, bu.dN)
@@ -211,7 +220,9 @@ class MRNewton:
         self.restart = int(restart)
         self.ws = fem.KrylovWorkspace(max(self.red.n_free, 1), dev, restart=self.restart)
         self.free = self.red.free_idx.long()
-        mat = np.ascontiguousarray(material, dtype=np.float64).reshape(-1, 2)
+        # c1, c2 per element (c1, c2, kappa for the nearly incompressible law)
+        mat = np.ascontiguousarray(material, dtype=np.float64).reshape(
+            -1, 3 if sys.formulation == NEARLY_INCOMPRESSIBLE else 2)
         self.material = torch.as_tensor(mat, device=dev)
         mu = 2.0 * (mat[:, 0] + mat[:, 1])
         pm = sys.pressure_mass(np.where(mu > 0, mu, 1.0))[~np.asarray(fixed_mask, dtype=bool)]
@@ -350,6 +361,9 @@ class MRNewton:
             return info
         linear = o.linear
         if linear == "auto":
+            # the deviatoric tangent is symmetric; the general branch's is not, nor is the
+            # nearly incompressible one when c2 != 0 (its c2 term carries J^(-2/3) where a
+            # strain energy would give J^(-4/3))
             linear = "minres" if self.sys.formulation == DEVIATORIC else "gmres"
         fd = o.fd_step if o.jacobian == "fd" else None
         if o.convergence not in ("petsc", "energy"):
@@ -458,26 +472,35 @@ def load_increments(newton: MRNewton, x, n_increments: int, incremented: np.ndar
 # ------------------------------------------------------- iron API solve
 def solve(problem, es, solver, bc):
     """Problem.Solve() for a FINITE_ELASTICITY equations set (MOONEY_RIVLIN subtype,
-    or NO_SUBTYPE for the general branch)."""
+    NO_SUBTYPE for the general branch, or NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN with a
+    3-component dependent field and materials c1, c2, kappa)."""
     from . import engine
     from . import iron
     rm: RegularMesh = es.geometricField._mesh()
     dep = es.dependentField
     U = iron.FieldVariableTypes.U
     var = dep._var(U)
-    if var.ncomp != 4:
+    sub = es.specification[2]
+    nearly = sub == iron.EquationsSetSubtypes.NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN
+    if nearly and var.ncomp != 3:
+        raise CMFEError(1, "the nearly incompressible dependent field has 3 displacement components")
+    if not nearly and var.ncomp != 4:
         raise CMFEError(1, "the finite-elasticity dependent field needs 3 displacement components and a pressure")
     if any(mc != 1 for mc in var.mesh_comp[:3]) or any(
             it != iron.FieldInterpolationTypes.NODE_BASED for it in var.interp[:3]):
         raise CMFEError(1, "displacement components must be node based on mesh component 1")
-    if var.interp[3] == iron.FieldInterpolationTypes.ELEMENT_BASED:
+    if nearly:
+        p_mc = None
+    elif var.interp[3] == iron.FieldInterpolationTypes.ELEMENT_BASED:
         p_mc = None
     elif var.interp[3] == iron.FieldInterpolationTypes.NODE_BASED:
         p_mc = var.mesh_comp[3] - 1
     else:
         raise CMFEError(1, "the pressure must be element based or node based")
-    sub = es.specification[2]
-    form = DEVIATORIC if sub == iron.EquationsSetSubtypes.MOONEY_RIVLIN else CAUCHY
+    if nearly:
+        form = NEARLY_INCOMPRESSIBLE
+    else:
+        form = DEVIATORIC if sub == iron.EquationsSetSubtypes.MOONEY_RIVLIN else CAUCHY
     # conditions on any other field are an error, as on the linear path
     if any(not engine._is(entry[0], dep) for entry in bc._entries):
         raise CMFEError(1, "boundary conditions must be set on the dependent field")
@@ -514,9 +537,9 @@ def solve(problem, es, solver, bc):
             full[dof] = (dep._data(U)[dof] + value) if add else value
         else:
             raise CMFEError(1, f"boundary condition type {cond} is not supported for finite elasticity")
-    # materials: c1, c2 per element; density (V) and gravity (source) -> body force
+    # materials: c1, c2 (and kappa) per element; density (V) and gravity (source) -> body force
     mf = es.materialsField
-    mat = np.stack([engine._element_values(mf, 1, rm), engine._element_values(mf, 2, rm)], axis=1)
+    mat = np.stack([engine._element_values(mf, c, rm) for c in ((1, 2, 3) if nearly else (1, 2))], axis=1)
     body_force = None
     if es.sourceField is not None and mf is not None and iron.FieldVariableTypes.V in mf._vars:
         rho = engine._element_values(mf, 1, rm, vt=iron.FieldVariableTypes.V)

@@ -49,6 +49,16 @@ class GeneratedMeshTypes(enum.IntEnum):
     REGULAR = 1
 
 
+class GeneratedMeshSurfaceTypes(enum.IntEnum):
+    # generated_mesh_routines.F90:103-108
+    REGULAR_LEFT = 8
+    REGULAR_RIGHT = 9
+    REGULAR_TOP = 10
+    REGULAR_BOTTOM = 11
+    REGULAR_FRONT = 12
+    REGULAR_BACK = 13
+
+
 class DecompositionTypes(enum.IntEnum):
     ALL = 1
     CALCULATED = 2
@@ -109,6 +119,7 @@ class EquationsSetSubtypes(enum.IntEnum):
     CONSTANT_SOURCE_POISSON = 1
     THREE_DIMENSIONAL = 4
     MOONEY_RIVLIN = 1
+    NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN = 22
 
 
 class ProblemClasses(enum.IntEnum):
@@ -372,6 +383,39 @@ class GeneratedMesh(_Obj):
         if self._region is not None:
             self._region._nodes = mesh._regular.num_nodes(0)
 
+    def SurfaceGet(self, *args):
+        """SurfaceGet(surfaceType) or SurfaceGet(meshComponent, surfaceType) ->
+        (surfaceNodes, normalXi): the user numbers of the mesh component's nodes on a
+        face of the regular mesh and the signed xi direction of its outward normal
+        (GENERATED_MESH_REGULAR_SURFACE_GET, generated_mesh_routines.F90:3943-4084).
+        Nodes run with xi1 fastest, then xi2, then xi3 over the face, which on the
+        generated lattice is ascending user number.  Directions beyond the mesh's
+        dimension hold one layer of nodes, as in the reference."""
+        if len(args) == 1:
+            mc, st = 1, int(args[0])
+        elif len(args) == 2:
+            mc, st = int(args[0]), int(args[1])
+        else:
+            raise CMFEError(1, "SurfaceGet takes a surface type and optionally a mesh component first")
+        rm = self._mesh._regular
+        if not 1 <= mc <= len(rm.bases):
+            raise CMFEError(1, f"mesh component {mc} does not exist")
+        faces = {GeneratedMeshSurfaceTypes.REGULAR_LEFT: (0, False, -1),
+                 GeneratedMeshSurfaceTypes.REGULAR_RIGHT: (0, True, 1),
+                 GeneratedMeshSurfaceTypes.REGULAR_FRONT: (1, False, -2),
+                 GeneratedMeshSurfaceTypes.REGULAR_BACK: (1, True, 2),
+                 GeneratedMeshSurfaceTypes.REGULAR_BOTTOM: (2, False, -3),
+                 GeneratedMeshSurfaceTypes.REGULAR_TOP: (2, True, 3)}
+        if st not in faces:
+            raise CMFEError(1, f"The specified surface type of {st} is invalid for a regular mesh.")
+        axis, high, normal = faces[GeneratedMeshSurfaceTypes(st)]
+        users = rm.comp_user[mc - 1]
+        if axis >= rm.dim:
+            return users.astype(np.int32).copy(), normal
+        pos = rm.node_grid_pos[users - 1, axis]
+        edge = rm.node_grid_pos[:, axis].max() if high else 0
+        return users[pos == edge].astype(np.int32), normal
+
     def GeometricParametersCalculate(self, field):
         rm = self._mesh._regular
         data = field._data(FieldVariableTypes.U)
@@ -594,7 +638,7 @@ class EquationsSet(_Obj):
               or (cls == EquationsSetClasses.ELASTICITY and typ == EquationsSetTypes.LINEAR_ELASTICITY
                   and sub == 4)
               or (cls == EquationsSetClasses.ELASTICITY and typ == EquationsSetTypes.FINITE_ELASTICITY
-                  and sub in (0, 1)))
+                  and sub in (0, 1, 22)))
         if not ok:
             raise CMFEError(1, f"equations set specification {self.specification} is not implemented")
 
@@ -607,6 +651,11 @@ class EquationsSet(_Obj):
         if cls == EquationsSetClasses.CLASSICAL_FIELD:
             return {(1, 1): "laplace", (1, 2): "generalised_laplace", (2, 1): "poisson"}[(typ, sub)]
         return {1: "linear_elasticity", 2: "mooney_rivlin"}[typ]
+
+    @property
+    def _nearly_incompressible(self):
+        # no pressure component; materials c1, c2 and the bulk penalty kappa
+        return self.specification[2] == EquationsSetSubtypes.NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN
 
     def _geom_decomp(self):
         return self.geometricField.meshDecomposition
@@ -624,7 +673,7 @@ class EquationsSet(_Obj):
     def DependentCreateStart(self, userNumber, field):
         kind = self._kind
         ncomp = {"laplace": 1, "generalised_laplace": 1, "poisson": 1, "linear_elasticity": 3,
-                 "mooney_rivlin": 4}[kind]
+                 "mooney_rivlin": 3 if self._nearly_incompressible else 4}[kind]
         self.dependentField = self._new_field(field, ncomp, variables=(FieldVariableTypes.U,
                                                                        FieldVariableTypes.DELUDELN))
         field.type = FieldTypes.GENERAL
@@ -635,7 +684,7 @@ class EquationsSet(_Obj):
     def MaterialsCreateStart(self, userNumber, field):
         kind = self._kind
         ncomp = {"laplace": 0, "generalised_laplace": 6, "poisson": 1, "linear_elasticity": 6,
-                 "mooney_rivlin": 2}[kind]
+                 "mooney_rivlin": 3 if self._nearly_incompressible else 2}[kind]
         variables = (FieldVariableTypes.U, FieldVariableTypes.V) if kind == "mooney_rivlin" else \
             (FieldVariableTypes.U,)
         self.materialsField = self._new_field(field, max(ncomp, 1), FieldInterpolationTypes.CONSTANT, variables)

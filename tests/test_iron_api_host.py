@@ -164,3 +164,68 @@ def test_geometry_is_read_from_the_geometric_field():
     assert X2[k, 1] == 0.4 and np.array_equal(np.delete(X2, k, 0), np.delete(rm.coords, k, 0))
     with pytest.raises(iron.CMFEError):
         engine.geometry_coords(geo, rm, 1)
+
+
+def _q2_mesh_with_linear_component(ne=(2, 3, 2), dims=3):
+    cs = iron.CoordinateSystem(); cs.CreateStart(1); cs.CreateFinish()
+    region = iron.Region(); region.CreateStart(1, iron.WorldRegion); region.CreateFinish()
+    bases = []
+    for k, interp in enumerate((iron.BasisInterpolationSpecifications.QUADRATIC_LAGRANGE,
+                                iron.BasisInterpolationSpecifications.LINEAR_LAGRANGE)):
+        b = iron.Basis(); b.CreateStart(k + 1); b.numberOfXi = dims
+        b.interpolationXi = [interp] * dims
+        b.CreateFinish()
+        bases.append(b)
+    gm = iron.GeneratedMesh(); gm.CreateStart(1, region)
+    gm.basis = bases; gm.extent = [2.0, 1.0, 1.0][:dims]; gm.numberOfElements = list(ne)[:dims]
+    mesh = iron.Mesh(); gm.CreateFinish(1, mesh)
+    return gm, mesh._regular
+
+
+def test_regular_surface_nodes_and_normals():
+    """GENERATED_MESH_REGULAR_SURFACE_GET (generated_mesh_routines.F90:3943-4084): the
+    face's nodes in the reference's loop order (LEFT/RIGHT: k outer, j inner; TOP/BOTTOM:
+    j outer, i inner; FRONT/BACK: k outer, i inner), rebuilt here from node coordinates,
+    for the quadratic component and the linear (corner-node) component."""
+    gm, rm = _q2_mesh_with_linear_component()
+    S = iron.GeneratedMeshSurfaceTypes
+    X = rm.coords
+    user0 = np.arange(1, len(X) + 1)
+    expect = {S.REGULAR_LEFT: (0, 0.0, (1, 2), -1), S.REGULAR_RIGHT: (0, 2.0, (1, 2), 1),
+              S.REGULAR_FRONT: (1, 0.0, (0, 2), -2), S.REGULAR_BACK: (1, 1.0, (0, 2), 2),
+              S.REGULAR_BOTTOM: (2, 0.0, (0, 1), -3), S.REGULAR_TOP: (2, 1.0, (0, 1), 3)}
+    corner = np.zeros(len(X), bool)
+    corner[rm.comp_user[1] - 1] = True
+    for st, (axis, val, (inner, outer), normal) in expect.items():
+        for mc, sel in ((1, np.ones(len(X), bool)), (2, corner)):
+            on = sel & (X[:, axis] == val)
+            order = np.lexsort((X[on, inner], X[on, outer]))
+            nodes, nx = gm.SurfaceGet(mc, st) if mc == 2 else gm.SurfaceGet(st)
+            assert nx == normal
+            assert np.array_equal(nodes, user0[on][order]), (st, mc)
+    n_top, _ = gm.SurfaceGet(S.REGULAR_TOP)
+    assert len(n_top) == 5 * 7
+    with pytest.raises(iron.CMFEError):
+        gm.SurfaceGet(7)
+    # a 2-D mesh has one layer in xi3: TOP and BOTTOM are every node of the component
+    gm2, rm2 = _q2_mesh_with_linear_component((2, 2), dims=2)
+    assert np.array_equal(gm2.SurfaceGet(S.REGULAR_TOP)[0], rm2.comp_user[0])
+    assert np.array_equal(gm2.SurfaceGet(2, S.REGULAR_BOTTOM)[0], rm2.comp_user[1])
+
+
+def test_nearly_incompressible_subtype_fields():
+    """EQUATIONS_SET_NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN_SUBTYPE (equations_set_constants.F90:146):
+    a 3-component dependent field (no pressure) and materials c1, c2, kappa."""
+    assert iron.EquationsSetSubtypes.NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN == 22
+    gm, rm = _q2_mesh_with_linear_component((1, 1, 1))
+    es = iron.EquationsSet()
+    es.CreateStart(1, None, None, [iron.EquationsSetClasses.ELASTICITY, iron.EquationsSetTypes.FINITE_ELASTICITY,
+                                   iron.EquationsSetSubtypes.NEARLY_INCOMPRESSIBLE_MOONEY_RIVLIN], 2, None)
+    dec = iron.Decomposition(); dec.CreateStart(1, gm._mesh); dec.CreateFinish()
+    geo = iron.Field(); geo.meshDecomposition = dec
+    es.geometricField = geo
+    dep, mat = iron.Field(), iron.Field()
+    es.DependentCreateStart(3, dep)
+    es.MaterialsCreateStart(4, mat)
+    assert dep._var(iron.FieldVariableTypes.U).ncomp == 3
+    assert mat._var(iron.FieldVariableTypes.U).ncomp == 3
