@@ -404,6 +404,14 @@ int mife_spmv(const mife_matrix* A, const void* vals, const void* x, void* y, in
 int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int src_prec,
                        void* dst, int dst_prec, void* stream);
 
+/* The same copy into both precisions at once, for the mixed-precision solve:
+ * dst64[k] = src[src_pos[k]] and dst32[k] = (float)dst64[k] (0 where src_pos[k] < 0).
+ * The FP32 inner iterations then run on the FP64 matrix rounded to FP32, read
+ * from the FP64 values in the same pass (SOLVER_MATRIX_EQUATIONS_MATRIX_ADD,
+ * src/solver_matrices_routines.F90:806-1023, once per precision). */
+int mife_gather_values2(int64_t nnz, const int32_t* src_pos, const double* src, double* dst64,
+                        float* dst32, void* stream);
+
 /* Solver RHS over the free rows (SOLVER_MATRICES_STATIC_ASSEMBLE RHS part,
  * src/solver_routines.F90:14640-14850):
  *   b[f] = src[i] + rhs[i] - lift_i,  i = free_idx[f],

@@ -132,6 +132,7 @@ _SIG = {
     "mife_sell_order_len": (_I, [_I]),
     "mife_sell_order": (_I, [_I, _P, _P, C.POINTER(_I), _P]),
     "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
+    "mife_gather_values2": (_I, [_I64, _P, _P, _P, _P, _P]),
     "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mife_field_update": (_I, [_I, _P, _P, _P, _P]),
     "mife_couple_pattern_work_bytes": (C.c_size_t, [_I, _I64, _I]),

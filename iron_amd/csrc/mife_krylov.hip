@@ -33,7 +33,10 @@ constexpr int kMaxIterChunk = 256;  // CG iterations launched between two host r
 
 struct CgState {
   double rz, pq, alpha, beta, rr, bb, tol2, scale;
-  double rz_prev, maxrr, delta2, rr64;
+  double rz_prev, maxrr, rr_seg, rr64;  // rr_seg: recurrence r.r when a segment ended
+  // Segmented (mixed-precision) solves: a segment pauses once r.r <= seg_rr, set at
+  // each replacement to seg_drop x the new r.r (seg_drop = 0: no pausing).
+  double seg_drop, seg_rr;
   int iters, done, maxit, pause;
   unsigned int cnt_a, cnt_b, cnt_c, pad2;
   // per-block partials of up to 4 reduced values each (value v of block b at v*kRedBlocks+b)
@@ -250,6 +253,7 @@ __global__ void fin_beta_kernel(CgState* st, int nb) {
     st->iters += 1;
     st->maxrr = fmax(st->maxrr, out[1]);
     st->done = (out[1] <= st->tol2 * st->bb || st->iters >= st->maxit) ? 1 : 0;
+    st->pause = (st->seg_rr > 0.0 && out[1] <= st->seg_rr) ? 1 : 0;
   }
 }
 
@@ -337,16 +341,23 @@ __global__ __launch_bounds__(256) void restart_residual64_kernel(int n, const do
   block_partials<2>(m, st->part_b);
 }
 
+// Also keeps the recurrence's r.r from before the replacement (rr_seg) and the
+// FP64 residual (rr64), so one read of st gives the host the whole cycle.
 __global__ void fin_replace_kernel(CgState* st, const CgState* ro, int nb, int first, double tol2) {
   double out[2];
   finish_sums<2>(st->part_b, nb, out);
   if (threadIdx.x == 0) {
+    st->rr_seg = st->rr;
+    st->rr64 = ro->rr64;
     st->beta = (first || out[1] > kRestartGrowth * st->rr) ? 0.0 : out[0] / st->rz_prev;
     st->rz = out[0];
     st->rr = out[1];
     st->maxrr = out[1];
     st->bb = ro->bb;
     st->pause = 0;
+    // the next segment ends when the recurrence has fallen by seg_drop, or at a
+    // quarter of the target r.r when that is nearer
+    st->seg_rr = st->seg_drop > 0.0 ? fmax(st->seg_drop * out[1], 0.25 * tol2 * ro->bb) : 0.0;
     st->done = (ro->rr64 <= tol2 * ro->bb) ? 1 : 0;
   }
 }
@@ -833,7 +844,8 @@ static void launch_residual64(const Mat& M, const TV* vals, const double* b, con
 
 struct HostState {
   double rz, pq, alpha, beta, rr, bb, tol2, scale;
-  double rz_prev, maxrr, delta2, rr64;
+  double rz_prev, maxrr, rr_seg, rr64;
+  double seg_drop, seg_rr;
   int iters, done, maxit, pause;
 };
 
@@ -843,15 +855,13 @@ static int read_state(const CgState* st, HostState* h, hipStream_t s) {
   return MIFE_OK;
 }
 
-// Launches exactly `count` CG iterations (iterations after a device-side stop
-// return at once).  With profiling on, one iteration in 8 carries event pairs,
-// which keeps the timing overhead off the solve.
+// Launches exactly `count` (<= kMaxIterChunk) CG iterations after iteration it0
+// (iterations after a device-side stop return at once), no synchronisation.
+// With profiling on, one iteration in 8 carries event pairs, which keeps the
+// timing overhead off the solve; their tickets go to tickets[0 .. 2 count).
 template <typename T>
-static int run_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
-                     hipStream_t s, HostState* hs, int count) {
-  count = std::min(count, kMaxIterChunk);
-  int tickets[2 * kMaxIterChunk];
-  const int it0 = hs->iters;
+static int launch_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
+                        hipStream_t s, int it0, int count, int* tickets) {
   const bool on = prof::enabled();
   for (int k = 0; k < count; ++k) {
     const bool sample = on && ((it0 + k) % 8) == 0;
@@ -860,13 +870,31 @@ static int run_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, 
     if (on && !sample) mife_profile_enable(1);
   }
   MIFE_CHECK_HIP(hipGetLastError());
-  int rc = read_state(st, hs, s);
-  if (rc) return rc;
+  return MIFE_OK;
+}
+
+// Drops the profile tickets of launched iterations the device skipped (at or
+// after the iteration count `done_at` it reports).
+static void discard_skipped(const int* tickets, int it0, int count, int done_at) {
   for (int k = 0; k < count; ++k)
-    if (it0 + k >= hs->iters) {
+    if (it0 + k >= done_at) {
       prof::discard(tickets[2 * k]);
       prof::discard(tickets[2 * k + 1]);
     }
+}
+
+// Launches `count` CG iterations and reads the state.
+template <typename T>
+static int run_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
+                     hipStream_t s, HostState* hs, int count) {
+  count = std::min(count, kMaxIterChunk);
+  int tickets[2 * kMaxIterChunk];
+  const int it0 = hs->iters;
+  int rc = launch_iters<T>(M, vals, x, r, z, p, q, dinv, st, s, it0, count, tickets);
+  if (rc) return rc;
+  rc = read_state(st, hs, s);
+  if (rc) return rc;
+  discard_skipped(tickets, it0, count, hs->iters);
   return MIFE_OK;
 }
 
@@ -1177,19 +1205,25 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
   fill_kernel<float><<<vg, 256, 0, s>>>(n, x32, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, p, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, q, 0.0f);
-  // the inner FP32 recurrence never stops on its own (tol2 = 0 on the device):
-  // the host sizes every segment and the FP64 residual decides convergence
+  // the inner FP32 recurrence never converges on its own (tol2 = 0 on the
+  // device): a segment pauses once its r.r has fallen by delta2 (or reached a
+  // quarter of the target), and the FP64 residual decides convergence
+  const double delta2 = inner_rtol * inner_rtol;  // FP32 reduction per segment
   HostState init{};
   init.tol2 = 0.0;
   init.maxit = maxit;
+  init.seg_drop = delta2;
   const double tol2 = rtol * rtol;
   MIFE_CHECK_HIP(hipMemcpyAsync(st, &init, sizeof(HostState), hipMemcpyHostToDevice, s));
-  const double delta2 = inner_rtol * inner_rtol;  // FP32 reduction per segment
-  HostState hs{}, ho{};
+  HostState hs{};
   int nrep = 0;
   bool first = true;
   double log_rate = 0.0;
-  int seg = 16;
+  int seg = 48, it0 = 0, launched = 0;
+  double rr0 = 0.0;
+  int tickets[2 * kMaxIterChunk];
+  // One host read per cycle: the FP32 segment and the FP64 replacement that
+  // follows it are launched back to back.
   while (true) {
     // FP64 replacement: x64 += x32, r64 = b - A64 x64, r32 = r64, z = D^-1 r32
     if (!first) fold_correction_kernel<<<vg, 256, 0, s>>>(n, x, x32);
@@ -1199,28 +1233,31 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
     MIFE_CHECK_HIP(hipGetLastError());
     rc = read_state(st, &hs, s);
     if (rc) return rc;
-    rc = read_state(sto, &ho, s);
-    if (rc) return rc;
+    if (!first) {
+      discard_skipped(tickets, it0, launched, hs.iters);
+      const int ran = hs.iters - it0;
+      if (ran > 0 && rr0 > 0.0 && hs.rr_seg > 0.0 && hs.rr_seg < rr0) log_rate = std::log(hs.rr_seg / rr0) / ran;
+    }
     first = false;
     ++nrep;
     if (hs.done || hs.iters >= maxit) break;
-    // the next segment: until the FP32 residual has fallen by inner_rtol, or to
-    // a quarter of the target when that is nearer
+    // the next segment: the device pauses it at its target (seg_rr); the host
+    // launches the iterations the observed rate predicts, plus a margin, and the
+    // ones past the pause return at once
     if (nrep > 1) {
-      const double target = std::max(delta2 * hs.rr, 0.25 * tol2 * hs.bb);
-      seg = predict_iters(hs.rr, target, log_rate, 4, kMaxIterChunk);
+      seg = predict_iters(hs.rr, hs.seg_rr, log_rate, 4, kMaxIterChunk);
+      seg += seg / 4 + 4;
     }
-    seg = std::min(seg, maxit - hs.iters);
-    const int it0 = hs.iters;
-    const double rr0 = hs.rr;
-    rc = run_iters<float>(M, vals32, x32, r, z, p, q, dinv, st, s, &hs, seg);
+    seg = std::min(std::min(seg, maxit - hs.iters), kMaxIterChunk);
+    it0 = hs.iters;
+    rr0 = hs.rr;
+    launched = seg;
+    rc = launch_iters<float>(M, vals32, x32, r, z, p, q, dinv, st, s, it0, seg, tickets);
     if (rc) return rc;
-    const int ran = hs.iters - it0;
-    if (ran > 0 && rr0 > 0.0 && hs.rr > 0.0 && hs.rr < rr0) log_rate = std::log(hs.rr / rr0) / ran;
   }
   permute_out_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x, x_io);
   MIFE_CHECK_HIP(hipStreamSynchronize(s));
-  const double rel = ho.bb > 0 ? sqrt(ho.rr64 / ho.bb) : 0.0;
+  const double rel = hs.bb > 0 ? sqrt(hs.rr64 / hs.bb) : 0.0;
   if (iters) *iters = hs.iters;
   if (outer) *outer = nrep;
   if (relres) *relres = rel;

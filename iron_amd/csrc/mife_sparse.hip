@@ -17,6 +17,17 @@ __global__ void gather_kernel(long long nnz, const int32_t* __restrict__ src_pos
   }
 }
 
+// Both precisions in one pass: dst64[k] = src[src_pos[k]], dst32[k] = (float)dst64[k].
+__global__ void gather2_kernel(long long nnz, const int32_t* __restrict__ src_pos, const double* __restrict__ src,
+                               double* __restrict__ dst64, float* __restrict__ dst32) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x) {
+    const int p = src_pos[k];
+    const double v = p >= 0 ? src[p] : 0.0;
+    dst64[k] = v;
+    dst32[k] = (float)v;
+  }
+}
+
 // One wavefront per slice, one row per lane: entry j of slice row k.
 __global__ void sell_build_kernel(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                                   const int32_t* __restrict__ src_pos, const int32_t* __restrict__ perm,
@@ -279,6 +290,18 @@ int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int
     gather_kernel<float, float><<<grid, 256, 0, s>>>(nnz, src_pos, (const float*)src, (float*)dst);
   else
     gather_kernel<float, double><<<grid, 256, 0, s>>>(nnz, src_pos, (const float*)src, (double*)dst);
+  MIFE_CHECK_HIP(hipGetLastError());
+  return MIFE_OK;
+}
+
+int mife_gather_values2(int64_t nnz, const int32_t* src_pos, const double* src, double* dst64, float* dst32,
+                        void* stream) {
+  if (nnz == 0) return MIFE_OK;
+  MIFE_REQUIRE(src_pos && src && dst64 && dst32, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
+  MIFE_REQUIRE(nnz > 0, MIFE_ERR_ARG, "negative entry count");
+  hipStream_t s = (hipStream_t)stream;
+  const int grid = (int)std::min<long long>(div_up(nnz, 256), 8192);
+  gather2_kernel<<<grid, 256, 0, s>>>(nnz, src_pos, src, dst64, dst32);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }

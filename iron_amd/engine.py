@@ -345,8 +345,9 @@ def _prepare_linear(es, solver, bc, others=(), force_prec=None):
         prec = fem.auto_precision(len(dm.comp_mesh), c.S.stored)
     mixed = prec == "mixed"
     solver.linearIterativePrecisionUsed = prec
+    # the mixed solve's FP32 values are the FP64 values rounded, taken in the
+    # same gather as the FP64 solver values: one assembly, in FP64
     v64 = c.csr.zeros(fem.F64)
-    v32 = c.csr.zeros(fem.F32) if mixed else None
     src = None
     # generated regular meshes in 3-D take the structured assembly (no position map)
     grid = rm.n_elem if rm.dim == 3 else None
@@ -369,8 +370,6 @@ def _prepare_linear(es, solver, bc, others=(), force_prec=None):
         if source is not None:
             src = torch.zeros(dm.num_dofs, dtype=torch.float64, device=dev)
         asm.assemble(v64, kappa=kappa, source=source, src_vec=src, prec=fem.F64)
-        if mixed:
-            asm.assemble(v32, kappa=kappa, prec=fem.F32)
     elif kind == "linear_elasticity":
         if rm.dim != 3:
             raise CMFEError(1, "linear elasticity is implemented for the 3-D subtype")
@@ -379,8 +378,6 @@ def _prepare_linear(es, solver, bc, others=(), force_prec=None):
         mat = np.stack([_element_values(mf, j, rm) for j in range(1, 7)], axis=1)
         mt = torch.as_tensor(np.ascontiguousarray(mat), device=dev)
         asm.assemble(v64, mt, prec=fem.F64)
-        if mixed:
-            asm.assemble(v32, mt, prec=fem.F32)
     else:
         raise CMFEError(1, f"equations set kind {kind} is not implemented")
     # RHS: DELUDELN values of rows whose RHS DOF is fixed, plus the integrated
@@ -399,10 +396,10 @@ def _prepare_linear(es, solver, bc, others=(), force_prec=None):
     rfix = torch.as_tensor(rhs_fixed.astype(np.uint8), device=dev) if rhs_fixed.any() else None
     u = torch.as_tensor(np.where(fixed, u_host, 0.0), device=dev)
     x0 = u_host[c.lead] if c.red.n_free else np.zeros(0)
-    return dict(es=es, dep=dep, c=c, v64=v64, v32=v32, src=src, rhs=rhs, rfix=rfix, u=u, x0=x0, mixed=mixed)
+    return dict(es=es, dep=dep, c=c, v64=v64, src=src, rhs=rhs, rfix=rfix, u=u, x0=x0, mixed=mixed)
 
 
-def _krylov(S, red, vals64, vals32, bvec, x, ws, solver, mixed):
+def _krylov(S, red, vals64, bvec, x, ws, solver, mixed):
     """KSP settings (solver_routines.F90:10921-11157) and the solve."""
     iron = _iron()
     rtol = float(solver.linearIterativeRelativeTolerance)
@@ -416,7 +413,7 @@ def _krylov(S, red, vals64, vals32, bvec, x, ws, solver, mixed):
     if red.n_free == 0:
         return fem.SolveInfo(0, 0.0)
     try:
-        return _krylov_run(S, vals64, vals32, bvec, x, ws, solver, mixed, ktype, rt, maxit)
+        return _krylov_run(S, vals64, bvec, x, ws, solver, mixed, ktype, rt, maxit)
     except NotConvergedError as e:
         # iron flags a warning and keeps the iterate (solver_routines.F90:11875-11880);
         # the library's message names the actual stop (iteration limit, or a true
@@ -428,12 +425,12 @@ def _krylov(S, red, vals64, vals32, bvec, x, ws, solver, mixed):
         return info
 
 
-def _krylov_run(S, vals64, vals32, bvec, x, ws, solver, mixed, ktype, rt, maxit):
+def _krylov_run(S, vals64, bvec, x, ws, solver, mixed, ktype, rt, maxit):
     iron = _iron()
-    s64 = S.gather(vals64, fem.F64)
     if mixed and ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
-        s32 = S.gather(vals32, fem.F32)
+        s64, s32 = S.gather2(vals64)
         return fem.refine(S, s64, s32, bvec, x, rt, maxit, ws)
+    s64 = S.gather(vals64, fem.F64)
     if ktype == iron.IterativeLinearSolverTypes.CONJUGATE_GRADIENT:
         return fem.cg(S, s64, bvec, x, rt, maxit, ws)
     if ktype == iron.IterativeLinearSolverTypes.GMRES or int(solver.linearType) == iron.LinearSolverTypes.DIRECT:
@@ -460,7 +457,7 @@ def solve_linear(es, solver, bc):
     c = st["c"]
     bvec = c.red.rhs(st["v64"], st["u"], src=st["src"], rhs=st["rhs"], rhs_fixed=st["rfix"])
     x = torch.as_tensor(st["x0"], device=c.csr.device).clone()
-    info = _krylov(c.S, c.red, st["v64"], st["v32"], bvec, x, c.ws, solver, st["mixed"])
+    info = _krylov(c.S, c.red, st["v64"], bvec, x, c.ws, solver, st["mixed"])
     solver.last_info = info
     c.red.field_update(x, st["u"])
     _finish_linear(st, info)
@@ -495,8 +492,7 @@ def solve_linear_sets(sets, solver, bc):
     red, S = bc_cache.red, bc_cache.S
     bvec = red.rhs([(st["v64"], st["u"], st["src"], st["rhs"], st["rfix"]) for st in sts])
     x = torch.as_tensor(np.concatenate([st["x0"] for st in sts]), device=sts[0]["c"].csr.device).clone()
-    v32 = [st["v32"] for st in sts] if mixed else None
-    info = _krylov(S, red, [st["v64"] for st in sts], v32, bvec, x, bc_cache.ws, solver, mixed)
+    info = _krylov(S, red, [st["v64"] for st in sts], bvec, x, bc_cache.ws, solver, mixed)
     solver.last_info = info
     red.field_update(x, [st["u"] for st in sts])
     for st in sts:

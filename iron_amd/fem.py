@@ -401,6 +401,20 @@ class SolverMatrix:
         call("mife_gather_values", self.stored, ptr(self.src), ptr(vals), src_prec, ptr(out), prec, _stream())
         return out
 
+    def gather2(self, vals64: torch.Tensor, out64: torch.Tensor | None = None,
+                out32: torch.Tensor | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+        """FP64 solver-matrix values and the same values rounded to FP32, from FP64
+        equations-CSR values in one pass (the mixed-precision solve's two copies)."""
+        if getattr(self.red, "coupled", False):
+            vals64 = self.red.gather(vals64, F64)
+        n = max(self.stored, 1)
+        if out64 is None:
+            out64 = torch.empty(n, dtype=torch.float64, device=vals64.device)
+        if out32 is None:
+            out32 = torch.empty(n, dtype=torch.float32, device=vals64.device)
+        call("mife_gather_values2", self.stored, ptr(self.src), ptr(vals64), ptr(out64), ptr(out32), _stream())
+        return out64, out32
+
     def spmv(self, vals, x, y, prec=F64):
         """y = A x in this storage's row numbering (SELL: slice numbering)."""
         call("mife_spmv", C.byref(self.desc), ptr(vals), ptr(x), ptr(y), prec, _stream())
@@ -415,13 +429,13 @@ MIXED_MIN_STORED = 16_000_000
 
 def auto_precision(n_components: int, stored: int) -> str:
     """The precision "auto" picks for one solve: "mixed" (FP32 Jacobi CG with FP64
-    residual replacement) for a scalar field whose solver matrix holds at least
-    MIXED_MIN_STORED stored entries, else "fp64".  Measured on MI355X: 48^3
-    triquadratic diffusion (52.7 M entries) runs a mixed pass in 38 ms against 52 ms in
-    FP64; 16^3 trilinear Laplace (0.1 M) 1.6 ms against 0.5 ms; 96x32x32 trilinear
-    elasticity (24 M, three components) 86 ms against 82 ms, the FP32 operator's
-    rounding costing vector fields 1.6-2.3x the FP64 iterations."""
-    return "mixed" if n_components == 1 and stored >= MIXED_MIN_STORED else "fp64"
+    residual replacement) when the solver matrix holds at least MIXED_MIN_STORED
+    stored entries, else "fp64" (``n_components`` is kept for callers that scope the
+    rule).  Measured on MI355X, full passes: 48^3 triquadratic diffusion (54 M
+    entries) mixed 37 ms against 52 ms in FP64; 96x32x32 trilinear elasticity (26 M,
+    three components) 70 ms against 82 ms; 16^3 trilinear Laplace (0.1 M) 2.0 ms
+    against 0.5 ms, where launches rather than bytes bound an iteration."""
+    return "mixed" if stored >= MIXED_MIN_STORED else "fp64"
 
 
 @dataclass

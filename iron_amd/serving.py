@@ -251,8 +251,8 @@ class _FirstFit:
 class Server:
     """Grouped-CG serving of Request objects within an HBM budget (bytes).
 
-    precision: "auto" (per request, the single-solve rule fem.auto_precision: FP32
-    iterations with FP64 reliable updates for a scalar field whose solver matrix
+    precision: "auto" (per request, the single-solve rule fem.auto_precision for scalar
+    fields: FP32 iterations with FP64 reliable updates when the solver matrix
     holds at least fem.MIXED_MIN_STORED entries, else FP64 -- on boxes of up to 64^3
     trilinear elements every request iterates in FP64), "split" (FP32 + reliable
     updates for diffusion requests, FP64 for elasticity), "mixed" (every request
@@ -364,7 +364,9 @@ class Server:
 
     def _mixed(self, st: "_Structure") -> bool:
         if self.mixed_kinds is None:
-            return fem.auto_precision(st.ncomp, st.S.stored) == "mixed"
+            # scalar fields only: the grouped solver's FP32 elasticity iterations are
+            # not measured to pay
+            return st.ncomp == 1 and fem.auto_precision(st.ncomp, st.S.stored) == "mixed"
         return st.kind in self.mixed_kinds
 
     def fits(self, req: Request) -> bool:

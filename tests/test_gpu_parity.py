@@ -200,6 +200,12 @@ def test_sell_storage_matches_csr_product(compress):
         assert np.array_equal(full, Ss.col[: Ss.stored].cpu().numpy())
     vc = Sc.gather(v64, fem.F64)
     vs = Ss.gather(v64, fem.F64)
+    # both precisions in one gather: the FP64 copy and its FP32 rounding, bit for bit
+    for S_ in (Sc, Ss):
+        d64, d32 = S_.gather2(v64)
+        assert torch.equal(d64, S_.gather(v64, fem.F64))
+        assert torch.equal(d32, S_.gather(v64, fem.F32))
+        assert torch.equal(d32, d64.float())
     xv = torch.rand(red.n_free, dtype=torch.float64, device="cuda")
     yc = Sc.spmv(vc, xv, torch.empty_like(xv))
     perm = Ss.perm.long()
