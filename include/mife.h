@@ -57,10 +57,13 @@ extern "C" {
  * NULL, col points to uint16_t offsets and the column of an entry of slice s is
  * col_base[s] + offset (mife_sell_compress); CSR ignores col_base.
  * SELL schedule: when slice_order is not NULL it holds 8 ranges of order_span
- * entries (mife_sell_order): range k lists the slices of the k-th eighth of the
- * matrix, longest first, padded with -1.  SELL products then give range k to
- * the blocks that the dispatcher places on XCD k and start the longest slices
- * first; NULL keeps the plain slice order.  CSR ignores both fields. */
+ * slots (mife_sell_order): range k lists the slices of the k-th eighth of the
+ * matrix, longest first (16-byte aligned).  A slot is four int32: the slice, its first entry
+ * (ptr[slice]), its entries per row and its base column (col_base[slice], 0
+ * without compression); padding slots are {-1, 0, 0, 0}.  SELL products then
+ * give range k to the blocks that the dispatcher places on XCD k, start the
+ * longest slices first and read a slice's bounds in one load; NULL keeps the
+ * plain slice order.  CSR ignores both fields. */
 typedef struct mife_matrix {
   int n;
   int format;
@@ -160,17 +163,20 @@ int mife_sell_build(int n, const int32_t* row_ptr, const int32_t* col, const int
 int mife_sell_compress(int n_slices, const int32_t* slice_ptr, const int32_t* sell_col,
                        int32_t* col_base, uint16_t* col16, int* ok, void* stream);
 
-/* Length (entries) of the slice_order array of a SELL matrix with n_slices
- * slices: 8 * order_span, order_span = 4 * ceil(ceil(n_slices / 8) / 4). */
+/* Length (int32 entries) of the slice_order array of a SELL matrix with
+ * n_slices slices: 4 * 8 * order_span, order_span = 4 * ceil(ceil(n_slices / 8) / 4). */
 int mife_sell_order_len(int n_slices);
 
 /* SELL schedule (see mife_matrix): splits the slices into 8 contiguous ranges
- * and lists each range by decreasing slice length (ties by slice index),
- * padding with -1.  order has mife_sell_order_len(n_slices) entries (device);
- * *order_span (host) receives the per-range length.  The ranking runs on the
- * host from a copy of slice_ptr; synchronises the stream.  No reference
- * counterpart (PETSc's MatMult walks CSR rows in order). */
-int mife_sell_order(int n_slices, const int32_t* slice_ptr, int32_t* order, int* order_span, void* stream);
+ * and lists each range by decreasing slice length (ties by slice index) as
+ * {slice, first entry, entries per row, base column} slots, padding with
+ * {-1, 0, 0, 0}.  col_base: the compressed storage's per-slice base columns,
+ * or NULL (base 0).  order has mife_sell_order_len(n_slices) int32 entries
+ * (device); *order_span (host) receives the per-range slot count.  The ranking
+ * runs on the host from copies of slice_ptr and col_base; synchronises the
+ * stream.  No reference counterpart (PETSc's MatMult walks CSR rows in order). */
+int mife_sell_order(int n_slices, const int32_t* slice_ptr, const int32_t* col_base, int32_t* order,
+                    int* order_span, void* stream);
 
 /* ------------------------------------------------------------- assembly */
 

@@ -130,7 +130,7 @@ _SIG = {
     "mife_sell_build": (_I, [_I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     "mife_sell_compress": (_I, [_I, _P, _P, _P, _P, C.POINTER(_I), _P]),
     "mife_sell_order_len": (_I, [_I]),
-    "mife_sell_order": (_I, [_I, _P, _P, C.POINTER(_I), _P]),
+    "mife_sell_order": (_I, [_I, _P, _P, _P, C.POINTER(_I), _P]),
     "mife_gather_values": (_I, [_I64, _P, _P, _I, _P, _I, _P]),
     "mife_gather_values2": (_I, [_I64, _P, _P, _P, _P, _P]),
     "mife_solver_rhs": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

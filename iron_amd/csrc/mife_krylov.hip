@@ -5,13 +5,16 @@
 // CG runs two fused kernels per iteration and keeps every scalar on the
 // device.  With q = A p maintained by the recurrence q = A z + beta q, the
 // search direction update folds into the SpMV kernel:
-//   K_A: p = z + beta p ; q = A z + beta q ; partial (p.q)   -> alpha = rz / pq
-//   K_B: x += alpha p ; r -= alpha q ; z = D^-1 r ; partial (r.z), (r.r) -> beta
-// Each bandwidth kernel leaves one partial sum per block; a one-block
-// finisher kernel reduces them in a fixed order (deterministic) and updates the
-// device-side state, so no device-scope fence runs inside the bandwidth
-// kernels and the host only looks at the state every few iterations.  Kernels
-// return at once once the state says stop.
+//   K_A: p = z + beta p ; q = A z + beta q ; partial (p.q)
+//   K_B: alpha = rz / pq ; x += alpha p ; r -= alpha q ; z = D^-1 r ;
+//        partials (r.z), (r.r)                                -> beta (finisher)
+// Each bandwidth kernel leaves one partial sum per block.  K_B's blocks (few and
+// large, one round on the chip) each sum K_A's partials themselves in one fixed
+// order, so alpha needs no launch of its own; a one-block finisher reduces K_B's
+// partials (deterministic) and updates the device-side state for the next K_A.
+// No device-scope fence runs inside the bandwidth kernels, the host only looks
+// at the state every few iterations, and kernels return at once once the state
+// says stop.
 #include "mife_common.hpp"
 #include "mife_profile.hpp"
 #include "mife_sparse.hpp"
@@ -29,6 +32,9 @@ namespace mife {
 constexpr int kRedBlocks = kMaxSellGrid;  // partial slots per reduced value (largest row grid)
 constexpr int kMaxRestart = 256;
 constexpr int kFinThreads = 1024;  // one-block finishers: two partials per thread
+// CG update blocks: 1024 threads, so that the blocks are few (one round) and each
+// sums K_A's partials once
+constexpr int kUpdateThreads = 1024;
 constexpr int kMaxIterChunk = 256;  // CG iterations launched between two host reads of the state
 
 struct CgState {
@@ -49,7 +55,7 @@ struct CgState {
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&mine)[NV], double* parts) {
   static_assert(NV <= 4, "at most 4 reduced values per kernel");
-  __shared__ double red[NV][8];
+  __shared__ double red[NV][16];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     double s = wave_sum(mine[v]);
@@ -66,7 +72,8 @@ __device__ __forceinline__ void block_partials(double (&mine)[NV], double* parts
   }
 }
 
-// In a one-block finisher: fixed-order sums of a producer's nb partials.
+// Fixed-order sums of a producer's nb partials, the result in every thread (a
+// one-block finisher, or every block of a consumer kernel).
 template <int NV>
 __device__ __forceinline__ void finish_sums(const double* __restrict__ parts, int nb, double (&out)[NV]) {
   __shared__ double red[NV][16];
@@ -181,27 +188,27 @@ __global__ __launch_bounds__(256) void cg_spmv_kernel(MatArgs A, const T* __rest
   block_partials<1>(m, st->part_a);
 }
 
-__global__ void fin_alpha_kernel(CgState* st, int nb) {
-  if (st->done || st->pause) return;
-  double out[1];
-  finish_sums<1>(st->part_a, nb, out);
-  if (threadIdx.x == 0) {
-    st->pq = out[0];
-    st->alpha = st->rz / out[0];
-  }
-}
-
-// x += alpha p, r -= alpha q, z = D^-1 r and the r.z, r.r partials.  Each
-// thread takes R = 16 / sizeof(T) consecutive rows through 16-byte vector
-// accesses (R = 1 when a vector is not 16-byte aligned); rows past the last
-// full group are taken one per thread.
+// alpha = rz / (p.q) from K_A's nb_a partials (summed in every block), then
+// x += alpha p, r -= alpha q, z = D^-1 r and the r.z, r.r partials.  Block 0
+// records pq and alpha, which no block of this kernel reads.  Each thread takes
+// R = 16 / sizeof(T) consecutive rows through 16-byte vector accesses (R = 1
+// when a vector is not 16-byte aligned); rows past the last full group are
+// taken one per thread.
 template <typename T, int R>
-__global__ __launch_bounds__(256) void cg_update_kernel(int n, T* __restrict__ x, T* __restrict__ r,
-                                                        T* __restrict__ z, const T* __restrict__ p,
-                                                        const T* __restrict__ q, const T* __restrict__ dinv,
-                                                        CgState* st) {
+__global__ __launch_bounds__(kUpdateThreads) void cg_update_kernel(int n, int nb_a, T* __restrict__ x,
+                                                                   T* __restrict__ r, T* __restrict__ z,
+                                                                   const T* __restrict__ p,
+                                                                   const T* __restrict__ q,
+                                                                   const T* __restrict__ dinv, CgState* st) {
   if (st->done || st->pause) return;
-  const T alpha = (T)st->alpha;
+  double pq[1];
+  finish_sums<1>(st->part_a, nb_a, pq);
+  const double alpha64 = st->rz / pq[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pq = pq[0];
+    st->alpha = alpha64;
+  }
+  const T alpha = (T)alpha64;
   double m[2] = {0.0, 0.0};
   auto row = [&](T& xi, T& ri, T& zi, T pi, T qi, T di) {
     xi += alpha * pi;
@@ -792,6 +799,8 @@ static int make_mat(const mife_matrix* A, Mat* m) {
   MIFE_REQUIRE(A->n > 0, MIFE_ERR_ARG, "empty matrix");
   MIFE_REQUIRE(A->format == MIFE_FMT_CSR || (A->perm && A->n_slices == div_up(A->n, kSlice)), MIFE_ERR_ARG,
                "SELL matrix needs perm and n_slices = ceil(n/64)");
+  MIFE_REQUIRE(!A->slice_order || ((uintptr_t)A->slice_order & 15) == 0, MIFE_ERR_ARG,
+               "slice_order must be 16-byte aligned");
   m->a = mat_args(A);
   m->perm = A->format == MIFE_FMT_SELL ? A->perm : nullptr;
   m->L = spmv_lanes(A->nnz, A->n);
@@ -809,27 +818,27 @@ static void launch_cg_init(const Mat& M, const T* vals, const T* b, const T* x, 
   prof::end(tk, s, prof::kCgInit);
 }
 
-// Launches one CG iteration (SpMV kernel + finisher, update kernel + finisher);
-// with profiling on, the two bandwidth kernels' event tickets are returned
-// through tk[0..1] (else -1).
+// Launches one CG iteration (SpMV kernel, update kernel, finisher); with
+// profiling on, the two bandwidth kernels' event tickets are returned through
+// tk[0..1] (else -1).
 template <typename T>
 static void launch_cg_iter(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
                            hipStream_t s, int* tk) {
   const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   constexpr int R = 16 / sizeof(T);
   const bool vec = aligned16(x) && aligned16(r) && aligned16(z) && aligned16(p) && aligned16(q) && aligned16(dinv);
-  const int vg = vec ? vec_grid(div_up(M.a.n, R)) : vec_grid(M.a.n);
+  const int units = vec ? div_up(M.a.n, R) : M.a.n;
+  const int ug = std::max(1, std::min(div_up(units, kUpdateThreads), kMaxGrid));
   tk[0] = prof::begin(s);
   MIFE_DISPATCH_FMT(M.a, M.L, (cg_spmv_kernel<T, FMT_, L_><<<grid, 256, 0, s>>>(M.a, vals, z, p, q, st)));
   prof::end(tk[0], s, sizeof(T) == 8 ? prof::kCgSpmv : prof::kCgSpmv32);
-  fin_alpha_kernel<<<1, kFinThreads, 0, s>>>(st, grid);
   tk[1] = prof::begin(s);
   if (vec)
-    cg_update_kernel<T, R><<<vg, 256, 0, s>>>(M.a.n, x, r, z, p, q, dinv, st);
+    cg_update_kernel<T, R><<<ug, kUpdateThreads, 0, s>>>(M.a.n, grid, x, r, z, p, q, dinv, st);
   else
-    cg_update_kernel<T, 1><<<vg, 256, 0, s>>>(M.a.n, x, r, z, p, q, dinv, st);
+    cg_update_kernel<T, 1><<<ug, kUpdateThreads, 0, s>>>(M.a.n, grid, x, r, z, p, q, dinv, st);
   prof::end(tk[1], s, sizeof(T) == 8 ? prof::kCgUpdate : prof::kCgUpdate32);
-  fin_beta_kernel<<<1, kFinThreads, 0, s>>>(st, vg);
+  fin_beta_kernel<<<1, kFinThreads, 0, s>>>(st, ug);
 }
 
 template <typename TV = double>

@@ -157,6 +157,8 @@ int mife_spmv(const mife_matrix* A, const void* vals, const void* x, void* y, in
   MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
   MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL, MIFE_ERR_ARG, "invalid matrix format");
   if (A->n == 0) return MIFE_OK;
+  MIFE_REQUIRE(!A->slice_order || ((uintptr_t)A->slice_order & 15) == 0, MIFE_ERR_ARG,
+               "slice_order must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   const MatArgs M = mat_args(A);
   const int L = spmv_lanes(A->nnz, A->n);
@@ -249,24 +251,40 @@ int mife_sell_compress(int n_slices, const int32_t* slice_ptr, const int32_t* se
   return MIFE_OK;
 }
 
-int mife_sell_order_len(int n_slices) { return n_slices > 0 ? 8 * sell_order_span(n_slices) : 0; }
+int mife_sell_order_len(int n_slices) { return n_slices > 0 ? 4 * 8 * sell_order_span(n_slices) : 0; }
 
-int mife_sell_order(int n_slices, const int32_t* slice_ptr, int32_t* order, int* order_span, void* stream) {
+int mife_sell_order(int n_slices, const int32_t* slice_ptr, const int32_t* col_base, int32_t* order,
+                    int* order_span, void* stream) {
   MIFE_REQUIRE(slice_ptr && order && order_span, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(n_slices >= 0, MIFE_ERR_ARG, "negative slice count");
   *order_span = 0;
   if (n_slices == 0) return MIFE_OK;
   hipStream_t s = (hipStream_t)stream;
-  std::vector<int32_t> sp(n_slices + 1);
+  std::vector<int32_t> sp(n_slices + 1), cb(n_slices, 0);
   MIFE_CHECK_HIP(hipMemcpyAsync(sp.data(), slice_ptr, sizeof(int32_t) * sp.size(), hipMemcpyDeviceToHost, s));
+  if (col_base)
+    MIFE_CHECK_HIP(hipMemcpyAsync(cb.data(), col_base, sizeof(int32_t) * cb.size(), hipMemcpyDeviceToHost, s));
   MIFE_CHECK_HIP(hipStreamSynchronize(s));
   const int range = div_up(n_slices, 8), span = sell_order_span(n_slices);
-  std::vector<int32_t> h((size_t)8 * span, -1);
+  std::vector<int32_t> ids((size_t)span);
+  std::vector<int32_t> h((size_t)4 * 8 * span, 0);
   for (int k = 0; k < 8; ++k) {
     const int s0 = std::min(k * range, n_slices), s1 = std::min(s0 + range, n_slices);
-    int32_t* out = h.data() + (size_t)k * span;
-    for (int i = s0; i < s1; ++i) out[i - s0] = i;
-    std::stable_sort(out, out + (s1 - s0), [&](int32_t a, int32_t b) { return sp[a + 1] - sp[a] > sp[b + 1] - sp[b]; });
+    for (int i = s0; i < s1; ++i) ids[i - s0] = i;
+    std::stable_sort(ids.begin(), ids.begin() + (s1 - s0),
+                     [&](int32_t a, int32_t b) { return sp[a + 1] - sp[a] > sp[b + 1] - sp[b]; });
+    int32_t* out = h.data() + (size_t)4 * k * span;
+    for (int j = 0; j < span; ++j) {
+      if (j < s1 - s0) {
+        const int32_t sl = ids[j];
+        out[4 * j + 0] = sl;
+        out[4 * j + 1] = sp[sl];
+        out[4 * j + 2] = (sp[sl + 1] - sp[sl]) / kSlice;
+        out[4 * j + 3] = cb[sl];
+      } else {
+        out[4 * j + 0] = -1;
+      }
+    }
   }
   MIFE_CHECK_HIP(hipMemcpyAsync(order, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, s));
   MIFE_CHECK_HIP(hipStreamSynchronize(s));

@@ -170,16 +170,15 @@ __device__ __forceinline__ T sell16_row(const int32_t* __restrict__ sp, const ui
   return (a0 + a1) + (a2 + a3);
 }
 
-// Part h of W of a SELL row sum (int32 or 16-bit columns): the wavefront takes
-// chunks h, h + W, h + 2W, ... of eight entries; with W = 1 the order of the
-// additions is that of sell_row / sell16_row.
+// Part h of W of a SELL row sum (int32 or 16-bit columns) of the slice whose
+// entries start at `base`, `len` per row, base column cb (16-bit columns): the
+// wavefront takes chunks h, h + W, h + 2W, ... of eight entries; with W = 1 the
+// order of the additions is that of sell_row / sell16_row.
 template <typename T, bool C16, typename TV = T>
 __device__ __forceinline__ T sell_row_part(const MatArgs& A, const TV* __restrict__ vals, const T* __restrict__ x,
-                                           int s, int lane, int h, int W) {
+                                           int base, int len, int cb, int lane, int h, int W) {
   constexpr int D = 8;
-  const int base = A.ptr[s];
-  const int len = (A.ptr[s + 1] - base) / kSlice;
-  const T* xs = C16 ? x + A.cbase[s] : x;
+  const T* xs = C16 ? x + cb : x;
   const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col) + base + lane;
   const int32_t* c32 = A.col + base + lane;
   const TV* v = vals + base + lane;
@@ -229,25 +228,34 @@ __device__ __forceinline__ void for_each_row(const MatArgs& A, const TV* __restr
     if (A.order) {
       // Scheduled: the dispatcher places block b on XCD b % 8, which walks its own
       // range of the order (longest slices first); the grid is a multiple of 8.
-      // W wavefronts share a slice (W = 1: one slice per wavefront); their partial
-      // row sums meet in LDS and the first of them adds them in a fixed order.
-      // Every wavefront of a block makes the same number of passes, so the
-      // barriers are uniform.
+      // A slot gives the slice, its first entry, its length and its base column in
+      // one 16-byte load.  W wavefronts share a slice (W = 1: one slice per
+      // wavefront); their partial row sums meet in LDS and the first of them adds
+      // them in a fixed order.  Every wavefront of a block makes the same number of
+      // passes, so the barriers are uniform.
       __shared__ T part[16][kSlice];
       const int W = min(A.split, wpb), spb = wpb / W;
       const int wave = threadIdx.x >> 6, g = wave / W, h = wave - g * W;
       const int xcd = blockIdx.x % kXcds, bpx = gridDim.x / kXcds;
-      const int32_t* ord = A.order + (size_t)xcd * A.span;
+      const int4* ord = reinterpret_cast<const int4*>(A.order) + (size_t)xcd * A.span;
       const int passes = (A.span + bpx * spb - 1) / (bpx * spb);
       for (int ps = 0; ps < passes; ++ps) {
         const int k = (ps * bpx + blockIdx.x / kXcds) * spb + g;
-        const int s = k < A.span ? ord[k] : -1;
+        int4 slot = make_int4(-1, 0, 0, 0);
+        if (k < A.span) slot = ord[k];
+        const int s = slot.x;
         if (W == 1) {
-          if (s >= 0) one_slice(s);
+          if (s >= 0) {
+            T ax = (T)0;
+            if (x) ax = sell_row_part<T, FMT == kFmtSell16, TV>(A, vals, x, slot.y, slot.z, slot.w, lane, 0, 1);
+            const int row = s * kSlice + lane;
+            if (row < A.n) f(row, ax);
+          }
           continue;
         }
         T ax = (T)0;
-        if (s >= 0 && x) ax = sell_row_part<T, FMT == kFmtSell16, TV>(A, vals, x, s, lane, h, W);
+        if (s >= 0 && x)
+          ax = sell_row_part<T, FMT == kFmtSell16, TV>(A, vals, x, slot.y, slot.z, slot.w, lane, h, W);
         part[wave][lane] = ax;
         __syncthreads();
         if (h == 0 && s >= 0) {

@@ -382,7 +382,8 @@ class SolverMatrix:
         span = C.c_int(0)
         if schedule and n_slices > 0:
             self.slice_order = torch.empty(_lib.lib().mife_sell_order_len(n_slices), dtype=torch.int32, device=dev)
-            call("mife_sell_order", n_slices, ptr(self.slice_ptr), ptr(self.slice_order), C.byref(span), _stream())
+            call("mife_sell_order", n_slices, ptr(self.slice_ptr), ptr(self.col_base), ptr(self.slice_order),
+                 C.byref(span), _stream())
         self.desc = MatrixDesc(n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), colp, ptr(self.perm), n_slices,
                                self.stored, ptr(self.col_base), ptr(self.slice_order), span.value)
 

@@ -44,7 +44,7 @@ def test_argument_errors_without_gpu():
     assert L.mife_spmv(None, None, None, None, 0, None) == -1
     assert "null" in _lib.last_error()
     assert L.mife_cg(None, None, None, None, 0, 1e-8, 10, None, 0, None, None, None) == -1
-    assert L.mife_sell_order(4, None, None, None, None) == -1
+    assert L.mife_sell_order(4, None, None, None, None, None) == -1
     assert L.mife_grouped_setup(None, 0, 1, 1, None, None, None, None) == -1
     pools = _lib.Pools(1, 1, 1, 1, 1, None)   # the FP32 value pool is required
     it = L.mife_grouped_cg_iterate
@@ -118,8 +118,9 @@ def test_argument_errors_without_gpu():
     assert L.mife_assemble_grid(ctypes.byref(desc), ne, 3, 1, 1, 1, None, 1, 3 * rows, 0, 1, None, 1, 1 << 40,
                                 None) == 2   # no triquadratic elasticity
     # 8 ranges of ceil(ceil(n_slices / 8) / 4) * 4 entries
-    assert L.mife_sell_order_len(0) == 0 and L.mife_sell_order_len(1) == 32
-    assert L.mife_sell_order_len(13397) == 8 * 1676   # ceil(13397 / 8) = 1675 -> 1676
+    # four int32 per slot, eight ranges of span slots
+    assert L.mife_sell_order_len(0) == 0 and L.mife_sell_order_len(1) == 4 * 32
+    assert L.mife_sell_order_len(13397) == 4 * 8 * 1676   # ceil(13397 / 8) = 1675 -> 1676
 
 
 def test_one_hip_runtime_in_the_process():

@@ -226,8 +226,9 @@ def test_sell_storage_matches_csr_product(compress):
 @pytest.mark.parametrize("dims", [(5, 4, 3), (12, 11, 10)])
 def test_sell_schedule_covers_every_slice_and_keeps_products(dims):
     """The SELL schedule lists every slice once, by decreasing length inside each of
-    its 8 contiguous ranges; products over the schedule (where several wavefronts
-    share a slice on small matrices) agree with the plain slice order to rounding."""
+    its 8 contiguous ranges, each slot carrying the slice's first entry, entries per
+    row and base column; products over the schedule (where several wavefronts share
+    a slice on small matrices) agree with the plain slice order to rounding."""
     fem, _, _ = _fem()
     pb, pm, pd, ob, om, od, kappa, fixed = _poisson_system(dims, 2, seed=9)
     fem, csr, red, v64, v32, src, u, b = _gpu_linear(pb, pm, pd, kappa, fixed, np.zeros(pd.num_dofs))
@@ -237,12 +238,19 @@ def test_sell_schedule_covers_every_slice_and_keeps_products(dims):
     span = Ss.desc.order_span
     rng = -(-ns // 8)                      # slices per range, ceil(ns / 8)
     assert span == 4 * -(-rng // 4)
-    order = Ss.slice_order.cpu().numpy().reshape(8, span)
-    lens = np.diff(Ss.slice_ptr.cpu().numpy())
+    slots = Ss.slice_order.cpu().numpy().reshape(8, span, 4)
+    order = slots[:, :, 0]
+    sp = Ss.slice_ptr.cpu().numpy()
+    lens = np.diff(sp)
+    cbase = Ss.col_base.cpu().numpy() if Ss.col_base is not None else np.zeros(ns, dtype=np.int32)
     seen = []
     for k in range(8):
         ids = order[k][order[k] >= 0]
         assert np.all(order[k][len(ids):] == -1)
+        assert np.all(slots[k, len(ids):, 1:] == 0)
+        assert np.array_equal(slots[k, :len(ids), 1], sp[ids])
+        assert np.array_equal(slots[k, :len(ids), 2], lens[ids] // 64)
+        assert np.array_equal(slots[k, :len(ids), 3], cbase[ids])
         assert np.array_equal(np.sort(ids), np.arange(min(k * rng, ns), min((k + 1) * rng, ns)))
         L = lens[ids]
         assert np.all(L[:-1] >= L[1:])
