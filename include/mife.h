@@ -669,6 +669,18 @@ int mife_profile_enable(int on);
 int mife_profile_reset(void);
 int mife_profile_read(int n_max, char* names, int* counts, double* total_ms, int* n_out);
 
+/* ---------------------------------------------------------- HIP graphs */
+
+/* CG solves (mife_cg, mife_refine) on systems of at most 131,072 rows replay
+ * their iterations from a cached HIP graph of 8 iterations (captured on first
+ * use for each set of kernel arguments) when profiling is off; on by default.
+ * The kernels and their order are those of the host launches, so results are
+ * bit for bit the same.  No reference counterpart (PETSc's KSPSolve launches
+ * nothing). */
+int mife_graphs_enable(int on);
+/* Number of instantiated CG chunk graphs held by the library (at most 16). */
+int mife_graphs_cached(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -165,6 +165,8 @@ _SIG = {
     "mife_grouped_cg_residual": (_I, [_I, _I, _I, _I, C.POINTER(Pools), _P, _P, _P, _P, _P]),
     "mife_grouped_cg_status": (_I, [_I, _P, _P, _P, _P, _P]),
     "mife_profile_enable": (_I, [_I]),
+    "mife_graphs_enable": (_I, [_I]),
+    "mife_graphs_cached": (_I, []),
     "mife_profile_reset": (_I, []),
     "mife_profile_read": (_I, [_I, C.c_char_p, _P, _P, C.POINTER(_I)]),
 }
@@ -226,6 +228,11 @@ def stream_ptr(stream=None) -> int:
 
 def call(name: str, *args):
     check(getattr(lib(), name)(*args))
+
+
+def graphs_enable(on: bool = True):
+    """HIP-graph replay of CG iteration chunks on small systems (default on)."""
+    call("mife_graphs_enable", 1 if on else 0)
 
 
 def profile_enable(on: bool = True):

@@ -24,6 +24,7 @@ extern "C" int mife_profile_enable(int on);
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -103,21 +104,18 @@ __global__ void diag_inv_kernel(MatArgs A, const TV* __restrict__ vals, T* __res
     const int base = A.ptr[s], len = (A.ptr[s + 1] - base) / kSlice;
     const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col);
     const int cb = A.fmt == kFmtSell16 ? A.cbase[s] : 0;
-    // the whole row, eight entries in flight (padding entries carry value 0);
-    // the first nonzero entry in column i is the diagonal
+    // the row's columns, eight in flight; a value is read only where the column
+    // is i (padding entries carry value 0): the first nonzero one is the diagonal
     for (int j0 = 0; j0 < len; j0 += 8) {
       int cj[8];
-      T vj[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int e = base + (j0 + u) * kSlice + lane;
-        const bool in = j0 + u < len;
-        cj[u] = in ? (A.fmt == kFmtSell16 ? cb + (int)c16[e] : A.col[e]) : -1;
-        vj[u] = in ? (T)vals[e] : (T)0;
+        cj[u] = j0 + u < len ? (A.fmt == kFmtSell16 ? cb + (int)c16[e] : A.col[e]) : -1;
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (d == (T)0 && cj[u] == i && vj[u] != (T)0) d = vj[u];
+        if (d == (T)0 && cj[u] == i) d = (T)vals[base + (j0 + u) * kSlice + lane];
     }
   } else {
     int lo = A.ptr[i], hi = A.ptr[i + 1];
@@ -864,14 +862,104 @@ static int read_state(const CgState* st, HostState* h, hipStream_t s) {
   return MIFE_OK;
 }
 
+// ------------------------------------------------- CG chunks as HIP graphs
+// On small systems an iteration's three kernels take a few microseconds each,
+// and launching them from the host costs more than running them.  There a
+// chunk of kGraphIters iterations is captured once into a HIP graph (on a
+// private stream: the caller's may be the null stream, which cannot be
+// captured) and replayed with one launch; the instantiated graphs are cached by
+// their kernel arguments.  Only kernels are captured (no memset or copy nodes).
+constexpr int kGraphIters = 8;
+constexpr int kGraphMaxRows = 131072;  // launch-bound below this many rows
+constexpr int kGraphCache = 16;
+
+static bool g_graphs_on = true;
+
+struct GraphKey {
+  int dev, tsize, n, fmt, n_slices, span, split, L;
+  const void* ptrs[13];
+  bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
+};
+
+struct GraphEntry {
+  GraphKey key;
+  hipGraphExec_t exec;
+  unsigned long long used;
+};
+
+static std::mutex g_graph_mu;
+static std::vector<GraphEntry> g_graphs;
+static unsigned long long g_graph_clock = 0;
+static hipStream_t g_capture_stream[64];
+
+template <typename T>
+static int cg_chunk_graph(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
+                          hipGraphExec_t* out) {
+  GraphKey key;
+  std::memset(&key, 0, sizeof(key));
+  MIFE_CHECK_HIP(hipGetDevice(&key.dev));
+  MIFE_REQUIRE(key.dev >= 0 && key.dev < 64, MIFE_ERR_ARG, "device %d beyond the 64 supported", key.dev);
+  key.tsize = (int)sizeof(T);
+  key.n = M.a.n;
+  key.fmt = M.a.fmt;
+  key.n_slices = M.a.n_slices;
+  key.span = M.a.span;
+  key.split = M.a.split;
+  key.L = M.L;
+  const void* ps[13] = {M.a.ptr, M.a.col, M.a.cbase, M.a.order, vals, x, r, z, p, q, dinv, st, nullptr};
+  std::memcpy(key.ptrs, ps, sizeof(ps));
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  for (auto& e : g_graphs)
+    if (e.key == key) {
+      e.used = ++g_graph_clock;
+      *out = e.exec;
+      return MIFE_OK;
+    }
+  hipStream_t& cs = g_capture_stream[key.dev];
+  if (!cs) MIFE_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  MIFE_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  int tk[2];
+  for (int k = 0; k < kGraphIters; ++k) launch_cg_iter<T>(M, vals, x, r, z, p, q, dinv, st, cs, tk);
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(cs, &graph);
+  MIFE_REQUIRE(ce == hipSuccess && graph, MIFE_ERR_HIP, "CG chunk capture failed: %s", hipGetErrorString(ce));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  MIFE_REQUIRE(ie == hipSuccess, MIFE_ERR_HIP, "CG chunk graph instantiation failed: %s", hipGetErrorString(ie));
+  if ((int)g_graphs.size() >= kGraphCache) {
+    auto old = std::min_element(g_graphs.begin(), g_graphs.end(),
+                                [](const GraphEntry& a, const GraphEntry& b) { return a.used < b.used; });
+    hipGraphExecDestroy(old->exec);
+    g_graphs.erase(old);
+  }
+  g_graphs.push_back(GraphEntry{key, exec, ++g_graph_clock});
+  *out = exec;
+  return MIFE_OK;
+}
+
 // Launches exactly `count` (<= kMaxIterChunk) CG iterations after iteration it0
 // (iterations after a device-side stop return at once), no synchronisation.
-// With profiling on, one iteration in 8 carries event pairs, which keeps the
-// timing overhead off the solve; their tickets go to tickets[0 .. 2 count).
+// Small systems with profiling off replay the cached graph of kGraphIters
+// iterations count / kGraphIters times and launch the rest one by one (the
+// same kernels, in the same order).  With profiling on, one iteration in 8
+// carries event pairs, which keeps the timing overhead off the solve; their
+// tickets go to tickets[0 .. 2 count).
 template <typename T>
 static int launch_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
                         hipStream_t s, int it0, int count, int* tickets) {
   const bool on = prof::enabled();
+  if (!on && g_graphs_on && M.a.n <= kGraphMaxRows) {
+    hipGraphExec_t exec = nullptr;
+    int rc = cg_chunk_graph<T>(M, vals, x, r, z, p, q, dinv, st, &exec);
+    if (rc) return rc;
+    for (int c = 0; c < count / kGraphIters; ++c) MIFE_CHECK_HIP(hipGraphLaunch(exec, s));
+    for (int k = count / kGraphIters * kGraphIters; k < count; ++k)
+      launch_cg_iter<T>(M, vals, x, r, z, p, q, dinv, st, s, &tickets[2 * k]);
+    for (int k = 0; k < 2 * count; ++k) tickets[k] = -1;
+    MIFE_CHECK_HIP(hipGetLastError());
+    return MIFE_OK;
+  }
   for (int k = 0; k < count; ++k) {
     const bool sample = on && ((it0 + k) % 8) == 0;
     if (on && !sample) mife_profile_enable(0);
@@ -1152,6 +1240,16 @@ static int scratch_of_current_device(void** out) {
 using namespace mife;
 
 extern "C" {
+
+int mife_graphs_enable(int on) {
+  g_graphs_on = on != 0;
+  return MIFE_OK;
+}
+
+int mife_graphs_cached(void) {
+  std::lock_guard<std::mutex> lock(g_graph_mu);
+  return (int)g_graphs.size();
+}
 
 int mife_cg_noalloc(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol,
                     int maxit, int* iters, double* relres, void* stream) {

@@ -133,6 +133,16 @@ class DirichletReduction:
         self.src_pos = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
         call("mife_reduce_pattern", csr.n, ptr(csr.row_ptr), ptr(csr.col), ptr(self.new_idx), self.n_free,
              ptr(self.row_ptr), ptr(self.col), ptr(self.src_pos), None, ptr(work), s)
+        # rows with no fixed column have no lifting term: flagged once here, the RHS
+        # kernel skips their rows (the same sums, without reading them)
+        self.no_lift = torch.ones(csr.n, dtype=torch.uint8, device=dev)
+        if csr.nnz and self.n_free < csr.n:
+            fixed_col = torch.index_select(self.new_idx, 0, csr.col[: csr.nnz]) < 0
+            cum = torch.cumsum(fixed_col, 0, dtype=torch.int32)
+            cum = torch.cat([torch.zeros(1, dtype=torch.int32, device=dev), cum])
+            rp = csr.row_ptr.long()
+            self.no_lift = ((cum[rp[1:]] - cum[rp[:-1]]) == 0).to(torch.uint8)
+            del fixed_col, cum
 
     def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
         """K_ff values from equations-CSR values (FP64 or FP32) in precision ``prec``."""
@@ -147,8 +157,9 @@ class DirichletReduction:
         """b_f = src + rhs - K_fd u_d (no lifting on rows flagged in ``rhs_fixed``, uint8)."""
         if out is None:
             out = torch.empty(max(self.n_free, 1), dtype=torch.float64, device=vals64.device)
+        skip = self.no_lift if rhs_fixed is None else (self.no_lift | rhs_fixed)
         call("mife_solver_rhs", self.n_free, ptr(self.free_idx), ptr(self.csr.row_ptr), ptr(self.csr.col),
-             ptr(vals64), ptr(self.new_idx), ptr(u), ptr(src), ptr(rhs), ptr(rhs_fixed), ptr(out), _stream())
+             ptr(vals64), ptr(self.new_idx), ptr(u), ptr(src), ptr(rhs), ptr(skip), ptr(out), _stream())
         return out
 
     def field_update(self, x, u):
