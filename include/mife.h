@@ -396,6 +396,50 @@ int mife_mr_assemble(const mife_mr_desc* desc, int n_elem, const double* coords,
                      int n_colours, double* residual, double* jac_vals, double* rhs,
                      void* stream);
 
+/* Staged scatter of the Mooney-Rivlin element pass.  One launch over all
+ * elements writes each element's entries into its slots of a stage buffer, and
+ * a fixed-order reduction adds them into the global arrays: for every CSR entry
+ * k the slots e*ne*ne + a*ne + b that add into it, jac_list[jac_ptr[k] ..
+ * jac_ptr[k+1]) in increasing order (element order), and for every DOF the slots
+ * e*ne + a, dof_list[dof_ptr[d] .. dof_ptr[d+1]).  No reference counterpart
+ * (iron adds element matrices in element order, EquationsMatrices_ElementAdd). */
+typedef struct mife_mr_plan {
+  int64_t nnz;              /* CSR entries (jac_ptr has nnz + 1 entries) */
+  int n_dofs;               /* DOFs (dof_ptr has n_dofs + 1 entries) */
+  int pad;
+  const int32_t* jac_ptr;
+  const int32_t* jac_list;  /* [n_elem * ne * ne] */
+  const int32_t* dof_ptr;
+  const int32_t* dof_list;  /* [n_elem * ne] */
+} mife_mr_plan;
+
+/* Workspace bytes of mife_mr_plan_build. */
+size_t mife_mr_plan_work_bytes(int n_elem, int ne, int64_t nnz, int n_dofs);
+
+/* Builds a plan from the scatter map pos [n_elem][ne][ne] (CSR positions) and
+ * elem_dofs [n_elem][ne]: counts, an exclusive scan, an atomic fill and a sort
+ * of each segment.  Synchronises the stream; fails on a position or DOF out of
+ * range. */
+int mife_mr_plan_build(int n_elem, int ne, const int32_t* pos, int64_t nnz, const int32_t* elem_dofs,
+                       int n_dofs, int32_t* jac_ptr, int32_t* jac_list, int32_t* dof_ptr,
+                       int32_t* dof_list, void* work, size_t work_bytes, void* stream);
+
+/* Bytes of the stage buffer of mife_mr_assemble_staged: n_elem (ne^2 + 2 ne)
+ * doubles, ne = 3 n_nodes + n_pressure.  Zero-fill it once before first use
+ * (the pressure-pressure and gravity-pressure slots are never written). */
+size_t mife_mr_stage_bytes(const mife_mr_desc* desc, int n_elem);
+
+/* As mife_mr_assemble, with the element pass run once over all elements into
+ * `stage` and the plan's fixed-order reductions (+=) into residual, jac_vals
+ * and rhs.  Replaces the same reference routines as mife_mr_assemble; the sums
+ * into each entry run in element order, so results do not depend on the launch
+ * schedule. */
+int mife_mr_assemble_staged(const mife_mr_desc* desc, int n_elem, const double* coords,
+                            const int32_t* elem_nodes, const int32_t* elem_dofs,
+                            const double* state, const double* material,
+                            const double* body_force, const mife_mr_plan* plan, double* stage,
+                            double* residual, double* jac_vals, double* rhs, void* stream);
+
 /* --------------------------------------------------- sparse operations */
 
 /* y = A x (prec selects FP64 or FP32 values and vectors).  For SELL, x and y
@@ -675,8 +719,9 @@ int mife_profile_read(int n_max, char* names, int* counts, double* total_ms, int
  * their iterations from a cached HIP graph of 8 iterations (captured on first
  * use for each set of kernel arguments) when profiling is off; on by default.
  * The kernels and their order are those of the host launches, so results are
- * bit for bit the same.  No reference counterpart (PETSc's KSPSolve launches
- * nothing). */
+ * bit for bit the same.  mife_cg_noalloc never takes this path (capture and
+ * instantiation may allocate).  No reference counterpart (PETSc's KSPSolve
+ * launches nothing). */
 int mife_graphs_enable(int on);
 /* Number of instantiated CG chunk graphs held by the library (at most 16). */
 int mife_graphs_cached(void);

@@ -98,6 +98,12 @@ class MrDesc(C.Structure):
                 ("formulation", C.c_int)]
 
 
+class MrPlan(C.Structure):
+    """mife_mr_plan: per CSR entry and per DOF, the element stage slots that add into it."""
+    _fields_ = [("nnz", C.c_int64), ("n_dofs", C.c_int), ("pad", C.c_int), ("jac_ptr", C.c_void_p),
+                ("jac_list", C.c_void_p), ("dof_ptr", C.c_void_p), ("dof_list", C.c_void_p)]
+
+
 MIFE_MR_DEVIATORIC = 0
 MIFE_MR_CAUCHY = 1
 MIFE_MR_NEARLY_INCOMPRESSIBLE = 2
@@ -155,6 +161,11 @@ _SIG = {
     "mife_assemble_batch": (_I, [C.POINTER(ElementDesc), C.POINTER(ElementDesc), _I, C.POINTER(AssemblyJob), _I, _I,
                                  _P, _P, _P]),
     "mife_mr_assemble": (_I, [C.POINTER(MrDesc), _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "mife_mr_plan_work_bytes": (C.c_size_t, [_I, _I, _I64, _I]),
+    "mife_mr_plan_build": (_I, [_I, _I, _P, _I64, _P, _I, _P, _P, _P, _P, _P, C.c_size_t, _P]),
+    "mife_mr_stage_bytes": (C.c_size_t, [C.POINTER(MrDesc), _I]),
+    "mife_mr_assemble_staged": (_I, [C.POINTER(MrDesc), _I, _P, _P, _P, _P, _P, _P, C.POINTER(MrPlan), _P, _P,
+                                     _P, _P, _P]),
     "mife_grouped_state_bytes": (C.c_size_t, [_I]),
     "mife_grouped_work_bytes": (C.c_size_t, [_I, _I]),
     "mife_grouped_blocks": (_I, [_I]),

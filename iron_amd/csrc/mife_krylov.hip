@@ -789,6 +789,7 @@ struct Mat {
   MatArgs a;
   const int32_t* perm;
   int L;
+  bool graphs = true;  // CG chunks may replay cached HIP graphs (not without allocation)
 };
 
 static int make_mat(const mife_matrix* A, Mat* m) {
@@ -949,7 +950,7 @@ template <typename T>
 static int launch_iters(const Mat& M, const T* vals, T* x, T* r, T* z, T* p, T* q, const T* dinv, CgState* st,
                         hipStream_t s, int it0, int count, int* tickets) {
   const bool on = prof::enabled();
-  if (!on && g_graphs_on && M.a.n <= kGraphMaxRows) {
+  if (!on && g_graphs_on && M.graphs && M.a.n <= kGraphMaxRows) {
     hipGraphExec_t exec = nullptr;
     int rc = cg_chunk_graph<T>(M, vals, x, r, z, p, q, dinv, st, &exec);
     if (rc) return rc;
@@ -1172,13 +1173,14 @@ static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double r
 
 // mife_cg on a workspace of `cap` bytes at `work` (the caller's buffer, or the
 // library's own scratch array for the no-allocation entry point).
-static int cg_entry(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol,
+static int cg_entry(bool graphs, const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol,
                     int maxit, void* work, size_t cap, int* iters, double* relres, void* stream) {
   MIFE_REQUIRE(vals && b && x && work, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
   MIFE_REQUIRE(maxit >= 0 && rtol >= 0.0, MIFE_ERR_ARG, "invalid CG arguments");
   Mat M;
   int rc = make_mat(A, &M);
+  M.graphs = graphs;  // graph instantiation may allocate
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   Work w{(char*)work, 0, cap};
@@ -1266,7 +1268,7 @@ int mife_cg_noalloc(const mife_matrix* A, const void* vals, const void* b, void*
   void* base = nullptr;
   int rc = scratch_of_current_device(&base);
   if (rc) return rc;
-  return cg_entry(A, vals, b, x, prec, rtol, maxit, base, kScratchBytes, iters, relres, stream);
+  return cg_entry(false, A, vals, b, x, prec, rtol, maxit, base, kScratchBytes, iters, relres, stream);
 }
 
 size_t mife_solver_work_bytes(int n, int restart) {
@@ -1283,7 +1285,7 @@ size_t mife_solver_work_bytes(int n, int restart) {
 
 int mife_cg(const mife_matrix* A, const void* vals, const void* b, void* x, int prec, double rtol, int maxit,
             void* work, size_t work_bytes, int* iters, double* relres, void* stream) {
-  return cg_entry(A, vals, b, x, prec, rtol, maxit, work, work_bytes, iters, relres, stream);
+  return cg_entry(true, A, vals, b, x, prec, rtol, maxit, work, work_bytes, iters, relres, stream);
 }
 
 int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32, const double* b_in, double* x_io,

@@ -28,7 +28,7 @@ import numpy as np
 import torch
 
 from . import _lib, fem
-from ._lib import CMFEError, MrDesc, call, ptr
+from ._lib import CMFEError, MrDesc, MrPlan, call, ptr
 from .basis import LagrangeBasis
 from .mesh import RegularMesh, structured_colours
 
@@ -94,14 +94,40 @@ class MRSystem:
         self.coords = torch.as_tensor(self.X, device=dev)
         self.elem_nodes = torch.as_tensor(np.ascontiguousarray(en_u, dtype=np.int32), device=dev)
         self.desc = MrDesc(C.pointer(self.db.desc), self.n_pressure, ptr(self.Np), self.formulation)
+        # staged scatter: one element launch into a stage buffer, then fixed-order
+        # sums by a plan built once from the scatter map
+        cs = self.csr
+        ne = cs.n_edof
+        E = rm.num_elements
+        self._plan_arrays = [torch.empty(cs.nnz + 1, dtype=torch.int32, device=dev),
+                             torch.empty(max(E * ne * ne, 1), dtype=torch.int32, device=dev),
+                             torch.empty(self.ndofs + 1, dtype=torch.int32, device=dev),
+                             torch.empty(max(E * ne, 1), dtype=torch.int32, device=dev)]
+        wb = int(_lib.lib().mife_mr_plan_work_bytes(E, ne, cs.nnz, self.ndofs))
+        work = torch.empty(wb, dtype=torch.uint8, device=dev)
+        call("mife_mr_plan_build", E, ne, ptr(cs.pos), cs.nnz, ptr(cs.elem_dofs), self.ndofs,
+             *[ptr(a) for a in self._plan_arrays], ptr(work), wb, fem._stream())
+        del work
+        self.plan = MrPlan(cs.nnz, self.ndofs, 0, *[ptr(a) for a in self._plan_arrays])
+        sb = int(_lib.lib().mife_mr_stage_bytes(C.byref(self.desc), E))
+        self.stage = torch.zeros(max(sb // 8, 1), dtype=torch.float64, device=dev)
 
-    def assemble(self, state, material, body_force=None, residual=None, jac=None, rhs=None, atomic=False):
+    def assemble(self, state, material, body_force=None, residual=None, jac=None, rhs=None, atomic=False,
+                 staged=True):
         """Accumulate the residual, the Jacobian (equations-CSR values) and the gravity
-        vector at ``state`` into the given (zeroed) device arrays."""
+        vector at ``state`` into the given (zeroed) device arrays.  Staged (default):
+        one element launch and sums in element order; else one launch per colour
+        (``atomic``: one launch with atomic adds)."""
         cs = self.csr
         bf = None
         if body_force is not None:
             bf = (C.c_double * 3)(*[float(v) for v in body_force])
+        if staged and not atomic:
+            call("mife_mr_assemble_staged", C.byref(self.desc), self.rm.num_elements, ptr(self.coords),
+                 ptr(self.elem_nodes), ptr(cs.elem_dofs), ptr(state), ptr(material),
+                 C.cast(bf, C.c_void_p) if bf is not None else None, C.byref(self.plan), ptr(self.stage),
+                 ptr(residual), ptr(jac), ptr(rhs), fem._stream())
+            return
         call("mife_mr_assemble", C.byref(self.desc), self.rm.num_elements, ptr(self.coords), ptr(self.elem_nodes),
              ptr(cs.elem_dofs), ptr(state), ptr(material), C.cast(bf, C.c_void_p) if bf is not None else None,
              ptr(cs.pos) if jac is not None else None, None if atomic else ptr(cs.order),

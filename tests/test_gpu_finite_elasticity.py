@@ -74,13 +74,19 @@ def test_residual_jacobian_gravity_match_oracle(form, order, elem_p):
     assert np.abs(f.cpu().numpy() - f_o).max() <= 1e-12 * np.abs(f_o).max()
     Kg = _csr_dense(sysg, J)
     assert np.abs(Kg - K_o).max() <= 1e-12 * np.abs(K_o).max()
-    # atomic scatter agrees with the coloured one
-    R2 = torch.zeros_like(R)
-    J2 = sysg.csr.zeros()
-    sysg.assemble(xs, mat, residual=R2, jac=J2, atomic=True)
+    # the atomic scatter and the coloured launches agree with the staged sums; the
+    # staged sums (element order) repeat bit for bit
+    for kw in ({"atomic": True}, {"staged": False}):
+        R2 = torch.zeros_like(R)
+        J2 = sysg.csr.zeros()
+        sysg.assemble(xs, mat, residual=R2, jac=J2, **kw)
+        torch.cuda.synchronize()
+        assert torch.allclose(R2, R, rtol=0, atol=1e-12 * float(R.abs().max()))
+        assert torch.allclose(J2, J, rtol=0, atol=1e-12 * float(J.abs().max()))
+    R3, J3, f3 = torch.zeros_like(R), sysg.csr.zeros(), torch.zeros_like(R)
+    sysg.assemble(xs, mat, body_force=[2.0 * 0.3, 2.0 * -0.2, 2.0 * -9.8], residual=R3, jac=J3, rhs=f3)
     torch.cuda.synchronize()
-    assert torch.allclose(R2, R, rtol=0, atol=1e-12 * float(R.abs().max()))
-    assert torch.allclose(J2, J, rtol=0, atol=1e-12 * float(J.abs().max()))
+    assert torch.equal(R3, R) and torch.equal(J3, J) and torch.equal(f3, f)
 
 
 def _uniaxial(sysg, prob, lam):
