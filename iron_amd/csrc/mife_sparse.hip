@@ -8,19 +8,31 @@
 
 namespace mife {
 
+// Each block copies one contiguous chunk of kGatherChunk entries.  In SELL order
+// entries k and k + 64 are consecutive entries of the same 64 CSR rows, so the
+// CSR lines a chunk reads are reused by the same block (one CU, one L2); a grid
+// stride over the whole array spreads those reuses over the XCDs and fetches each
+// line several times.
+constexpr int kGatherChunk = 4096;
+
 template <typename S, typename D>
-__global__ void gather_kernel(long long nnz, const int32_t* __restrict__ src_pos, const S* __restrict__ src,
-                              D* __restrict__ dst) {
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(256) void gather_kernel(long long nnz, const int32_t* __restrict__ src_pos,
+                                                     const S* __restrict__ src, D* __restrict__ dst) {
+  const long long k0 = (long long)blockIdx.x * kGatherChunk;
+  const long long k1 = k0 + kGatherChunk < nnz ? k0 + kGatherChunk : nnz;
+  for (long long k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
     const int p = src_pos[k];
     dst[k] = p >= 0 ? (D)src[p] : (D)0;
   }
 }
 
 // Both precisions in one pass: dst64[k] = src[src_pos[k]], dst32[k] = (float)dst64[k].
-__global__ void gather2_kernel(long long nnz, const int32_t* __restrict__ src_pos, const double* __restrict__ src,
-                               double* __restrict__ dst64, float* __restrict__ dst32) {
-  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (long long)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(256) void gather2_kernel(long long nnz, const int32_t* __restrict__ src_pos,
+                                                      const double* __restrict__ src, double* __restrict__ dst64,
+                                                      float* __restrict__ dst32) {
+  const long long k0 = (long long)blockIdx.x * kGatherChunk;
+  const long long k1 = k0 + kGatherChunk < nnz ? k0 + kGatherChunk : nnz;
+  for (long long k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
     const int p = src_pos[k];
     const double v = p >= 0 ? src[p] : 0.0;
     dst64[k] = v;
@@ -299,7 +311,8 @@ int mife_gather_values(int64_t nnz, const int32_t* src_pos, const void* src, int
   MIFE_REQUIRE((src_prec == MIFE_F64 || src_prec == MIFE_F32) && (dst_prec == MIFE_F64 || dst_prec == MIFE_F32),
                MIFE_ERR_ARG, "invalid precision");
   hipStream_t s = (hipStream_t)stream;
-  const int grid = (int)std::min<long long>(div_up(nnz, 256), 8192);
+  MIFE_REQUIRE(div_up(nnz, (long long)kGatherChunk) <= INT32_MAX, MIFE_ERR_ARG, "too many entries");
+  const int grid = (int)div_up(nnz, (long long)kGatherChunk);
   if (src_prec == MIFE_F64 && dst_prec == MIFE_F64)
     gather_kernel<double, double><<<grid, 256, 0, s>>>(nnz, src_pos, (const double*)src, (double*)dst);
   else if (src_prec == MIFE_F64)
@@ -318,7 +331,8 @@ int mife_gather_values2(int64_t nnz, const int32_t* src_pos, const double* src, 
   MIFE_REQUIRE(src_pos && src && dst64 && dst32, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(nnz > 0, MIFE_ERR_ARG, "negative entry count");
   hipStream_t s = (hipStream_t)stream;
-  const int grid = (int)std::min<long long>(div_up(nnz, 256), 8192);
+  MIFE_REQUIRE(div_up(nnz, (long long)kGatherChunk) <= INT32_MAX, MIFE_ERR_ARG, "too many entries");
+  const int grid = (int)div_up(nnz, (long long)kGatherChunk);
   gather2_kernel<<<grid, 256, 0, s>>>(nnz, src_pos, src, dst64, dst32);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
