@@ -73,6 +73,10 @@ __global__ void sell_build_kernel(int n, const int32_t* __restrict__ rp, const i
 
 // One wavefront per free row: b[f] = src + rhs - sum over fixed columns of K_ij u_j,
 // skipping |u_j| < ZT or |K_ij| < ZT as SOLVER_MATRICES_STATIC_ASSEMBLE does.
+// 64 free rows per wavefront, one per lane: each lane forms src + rhs of its row;
+// then the wavefront walks the rows that take a lifting term (a ballot of the
+// lanes), all 64 lanes over each such row's entries.  The sums are those of one
+// wavefront per row, in the same order.
 __global__ void solver_rhs_kernel(int n_free, const int32_t* __restrict__ free_idx,
                                   const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
                                   const double* __restrict__ vals, const int32_t* __restrict__ new_idx,
@@ -80,26 +84,34 @@ __global__ void solver_rhs_kernel(int n_free, const int32_t* __restrict__ free_i
                                   const double* __restrict__ rhs, const uint8_t* __restrict__ rhs_fixed,
                                   double* __restrict__ b) {
   const double ZT = 5.0 * 2.220446049250313e-16;
-  const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (f >= n_free) return;
-  const int i = free_idx[f];
-  double lift = 0.0;
-  const bool lifted = !(rhs_fixed && rhs_fixed[i]);
-  for (int k = row_ptr[i] + lane; lifted && k < row_ptr[i + 1]; k += 64) {
-    const int j = col[k];
-    if (new_idx[j] < 0) {
-      const double uj = u[j], kv = vals[k];
-      if (fabs(uj) >= ZT && fabs(kv) >= ZT) lift += -1.0 * kv * uj;
-    }
-  }
-  lift = wave_sum(lift);
-  if (lane == 0) {
-    double v = 0.0;
+  const int f = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64 + lane;
+  int i = -1;
+  double v = 0.0, mine = 0.0;
+  bool need = false;
+  if (f < n_free) {
+    i = free_idx[f];
     if (src) v += 1.0 * src[i];
     if (rhs) v += rhs[i];
-    b[f] = v + lift;
+    need = !(rhs_fixed && rhs_fixed[i]);
   }
+  unsigned long long mask = __ballot(need);
+  while (mask) {
+    const int l = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const int il = __shfl(i, l, 64);
+    double lift = 0.0;
+    for (int k = row_ptr[il] + lane; k < row_ptr[il + 1]; k += 64) {
+      const int j = col[k];
+      if (new_idx[j] < 0) {
+        const double uj = u[j], kv = vals[k];
+        if (fabs(uj) >= ZT && fabs(kv) >= ZT) lift += -1.0 * kv * uj;
+      }
+    }
+    lift = wave_sum(lift);
+    if (lane == l) mine = lift;
+  }
+  if (f < n_free) b[f] = v + mine;
 }
 
 __global__ void field_update_kernel(int n_free, const int32_t* __restrict__ free_idx, const double* __restrict__ x,
@@ -345,8 +357,8 @@ int mife_solver_rhs(int n_free, const int32_t* free_idx, const int32_t* row_ptr,
   MIFE_REQUIRE(free_idx && row_ptr && col && vals && new_idx && u && b, MIFE_ERR_POINTER_IS_NULL,
                "null pointer argument");
   hipStream_t s = (hipStream_t)stream;
-  solver_rhs_kernel<<<div_up(n_free, 4), 256, 0, s>>>(n_free, free_idx, row_ptr, col, vals, new_idx, u, src, rhs,
-                                                      rhs_fixed, b);
+  solver_rhs_kernel<<<div_up(n_free, 256), 256, 0, s>>>(n_free, free_idx, row_ptr, col, vals, new_idx, u, src, rhs,
+                                                        rhs_fixed, b);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }
