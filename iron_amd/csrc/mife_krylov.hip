@@ -94,38 +94,68 @@ __device__ __forceinline__ void finish_sums(const double* __restrict__ parts, in
   }
 }
 
-template <typename T, typename TV = T>
-__global__ void diag_inv_kernel(MatArgs A, const TV* __restrict__ vals, T* __restrict__ dinv) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
+// Jacobi diagonal: dinv[i] = 1 / a_ii, or 1 where the diagonal is absent or zero.
+// SELL storage (FMT): one block of kDiagWaves wavefronts per slice.  Wavefront g
+// scans chunks g, g + kDiagWaves, ... (eight column positions each, the eight
+// loads issued together) of the slice's 64 rows and keeps the value at column
+// i; a row's columns are distinct and padding entries carry value 0, so at most
+// one nonzero candidate exists and the first nonzero one in wavefront order is
+// it.  CSR: bisection in the sorted row, one thread per row.
+constexpr int kDiagWaves = 16;
+
+template <typename T, typename TV, int FMT>
+__global__ __launch_bounds__(64 * kDiagWaves) void diag_inv_sell_kernel(MatArgs A, const TV* __restrict__ vals,
+                                                                      T* __restrict__ dinv) {
+  __shared__ T cand[kDiagWaves][kSlice];
+  const int s = blockIdx.x, lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int i = s * kSlice + lane;
   T d = 0;
-  if (A.fmt == MIFE_FMT_SELL || A.fmt == kFmtSell16) {
-    const int s = i / kSlice, lane = i - s * kSlice;
+  if (i < A.n) {
     const int base = A.ptr[s], len = (A.ptr[s + 1] - base) / kSlice;
     const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col);
-    const int cb = A.fmt == kFmtSell16 ? A.cbase[s] : 0;
-    // the row's columns, eight in flight; a value is read only where the column
-    // is i (padding entries carry value 0): the first nonzero one is the diagonal
-    for (int j0 = 0; j0 < len; j0 += 8) {
+    const int cb = FMT == kFmtSell16 ? A.cbase[s] : 0;
+    for (int j0 = 8 * g; j0 < len; j0 += 8 * kDiagWaves) {
       int cj[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int e = base + (j0 + u) * kSlice + lane;
-        cj[u] = j0 + u < len ? (A.fmt == kFmtSell16 ? cb + (int)c16[e] : A.col[e]) : -1;
+        const int e = base + min(j0 + u, len - 1) * kSlice + lane;  // in the slice: j0 < len
+        cj[u] = FMT == kFmtSell16 ? cb + (int)c16[e] : A.col[e];
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (d == (T)0 && cj[u] == i) d = (T)vals[base + (j0 + u) * kSlice + lane];
+        if (d == (T)0 && j0 + u < len && cj[u] == i) d = (T)vals[base + (j0 + u) * kSlice + lane];
     }
-  } else {
-    int lo = A.ptr[i], hi = A.ptr[i + 1];
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (A.col[mid] < i) lo = mid + 1; else hi = mid;
-    }
-    if (lo < A.ptr[i + 1] && A.col[lo] == i) d = (T)vals[lo];
   }
+  cand[g][lane] = d;
+  __syncthreads();
+  if (g == 0 && i < A.n) {
+    for (int h = 1; h < kDiagWaves && d == (T)0; ++h) d = cand[h][lane];
+    dinv[i] = (d != (T)0) ? (T)1 / d : (T)1;
+  }
+}
+
+template <typename T, typename TV = T>
+__global__ void diag_inv_csr_kernel(MatArgs A, const TV* __restrict__ vals, T* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  T d = 0;
+  int lo = A.ptr[i], hi = A.ptr[i + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (A.col[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  if (lo < A.ptr[i + 1] && A.col[lo] == i) d = (T)vals[lo];
   dinv[i] = (d != (T)0) ? (T)1 / d : (T)1;
+}
+
+template <typename T, typename TV = T>
+static void launch_diag_inv(const MatArgs& A, const TV* vals, T* dinv, hipStream_t s) {
+  if (A.fmt == kFmtSell16)
+    diag_inv_sell_kernel<T, TV, kFmtSell16><<<A.n_slices, 64 * kDiagWaves, 0, s>>>(A, vals, dinv);
+  else if (A.fmt == MIFE_FMT_SELL)
+    diag_inv_sell_kernel<T, TV, MIFE_FMT_SELL><<<A.n_slices, 64 * kDiagWaves, 0, s>>>(A, vals, dinv);
+  else
+    diag_inv_csr_kernel<T, TV><<<div_up(A.n, 256), 256, 0, s>>>(A, vals, dinv);
 }
 
 // r = b - A x (x == NULL: x = 0), z = D^-1 r, p = q = 0; reduces rz, rr, bb and
@@ -1134,7 +1164,7 @@ static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double r
   permute_in_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   int rc = zero_state(st, s);
   if (rc) return rc;
-  diag_inv_kernel<T><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
+  launch_diag_inv<T>(M.a, vals, dinv, s);
   launch_cg_init<T>(M, vals, b, x, dinv, r, z, p, q, st, rtol * rtol, maxit, s);
   HostState hs{};
   rc = read_state(st, &hs, s);
@@ -1310,7 +1340,7 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   if ((rc = zero_state(st, s))) return rc;
   if ((rc = zero_state(sto, s))) return rc;
-  diag_inv_kernel<float><<<div_up(n, 256), 256, 0, s>>>(M.a, vals32, dinv);
+  launch_diag_inv<float>(M.a, vals32, dinv, s);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, x32, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, p, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, q, 0.0f);
@@ -1404,7 +1434,7 @@ int minres_impl(const mife_matrix* A, const TV* vals, const double* b_in, double
   const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
-  diag_inv_kernel<double, TV><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
+  launch_diag_inv<double, TV>(M.a, vals, dinv, s);
   pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 1);
   fill_kernel<double><<<vg, 256, 0, s>>>(n, v, 0.0);
   fill_kernel<double><<<vg, 256, 0, s>>>(n, w1, 0.0);
@@ -1493,7 +1523,7 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   const int ag = std::min(div_up(n, 64), kMaxGrid);  // Gram-Schmidt update: 64-row tiles
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
-  diag_inv_kernel<double><<<div_up(n, 256), 256, 0, s>>>(M.a, vals, dinv);
+  launch_diag_inv<double>(M.a, vals, dinv, s);
   if (pc_inv) pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 0);
   if ((rc = zero_state(st, s))) return rc;
   launch_residual64(M, vals, b, x, r, st, s);
