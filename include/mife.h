@@ -71,7 +71,7 @@ typedef struct mife_matrix {
   const int32_t* col;
   const int32_t* perm;
   int n_slices;
-  int64_t nnz;
+  int64_t nnz; /* stored entries (SELL: ptr[n_slices], padding included); sizes launches only */
   const int32_t* col_base;
   const int32_t* slice_order;
   int order_span;

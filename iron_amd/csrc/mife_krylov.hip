@@ -95,26 +95,28 @@ __device__ __forceinline__ void finish_sums(const double* __restrict__ parts, in
 }
 
 // Jacobi diagonal: dinv[i] = 1 / a_ii, or 1 where the diagonal is absent or zero.
-// SELL storage (FMT): one block of kDiagWaves wavefronts per slice.  Wavefront g
-// scans chunks g, g + kDiagWaves, ... (eight column positions each, the eight
-// loads issued together) of the slice's 64 rows and keeps the value at column
-// i; a row's columns are distinct and padding entries carry value 0, so at most
-// one nonzero candidate exists and the first nonzero one in wavefront order is
-// it.  CSR: bisection in the sorted row, one thread per row.
-constexpr int kDiagWaves = 16;
+// SELL storage (FMT): W wavefronts per slice, max(4 / W, 1) slices per block.
+// Wavefront g of a slice scans chunks g, g + W, ... (eight column positions each,
+// the eight loads issued together) of the slice's 64 rows and keeps the value at
+// column i; a row's columns are distinct and padding entries carry value 0, so
+// at most one nonzero candidate exists and the first nonzero one in wavefront
+// order is it.  CSR: bisection in the sorted row, one thread per row.
+constexpr int kDiagMaxWaves = 16;
 
-template <typename T, typename TV, int FMT>
-__global__ __launch_bounds__(64 * kDiagWaves) void diag_inv_sell_kernel(MatArgs A, const TV* __restrict__ vals,
-                                                                      T* __restrict__ dinv) {
-  __shared__ T cand[kDiagWaves][kSlice];
-  const int s = blockIdx.x, lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+template <typename T, typename TV, int FMT, int W>
+__global__ __launch_bounds__(64 * (W < 4 ? 4 : W)) void diag_inv_sell_kernel(MatArgs A, const TV* __restrict__ vals,
+                                                                            T* __restrict__ dinv) {
+  constexpr int WPB = W < 4 ? 4 : W, SPB = WPB / W;
+  __shared__ T cand[WPB][kSlice];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = wave % W;
+  const int s = blockIdx.x * SPB + wave / W;  // slices per block: SPB, the grid covers them
   const int i = s * kSlice + lane;
   T d = 0;
-  if (i < A.n) {
+  if (s < A.n_slices && i < A.n) {
     const int base = A.ptr[s], len = (A.ptr[s + 1] - base) / kSlice;
     const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col);
     const int cb = FMT == kFmtSell16 ? A.cbase[s] : 0;
-    for (int j0 = 8 * g; j0 < len; j0 += 8 * kDiagWaves) {
+    for (int j0 = 8 * g; j0 < len; j0 += 8 * W) {
       int cj[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -126,10 +128,10 @@ __global__ __launch_bounds__(64 * kDiagWaves) void diag_inv_sell_kernel(MatArgs 
         if (d == (T)0 && j0 + u < len && cj[u] == i) d = (T)vals[base + (j0 + u) * kSlice + lane];
     }
   }
-  cand[g][lane] = d;
+  cand[wave][lane] = d;
   __syncthreads();
-  if (g == 0 && i < A.n) {
-    for (int h = 1; h < kDiagWaves && d == (T)0; ++h) d = cand[h][lane];
+  if (g == 0 && s < A.n_slices && i < A.n) {
+    for (int h = 1; h < W && d == (T)0; ++h) d = cand[wave + h][lane];
     dinv[i] = (d != (T)0) ? (T)1 / d : (T)1;
   }
 }
@@ -148,12 +150,31 @@ __global__ void diag_inv_csr_kernel(MatArgs A, const TV* __restrict__ vals, T* _
   dinv[i] = (d != (T)0) ? (T)1 / d : (T)1;
 }
 
+// Wavefronts per slice from the mean stored row length (`stored` entries over the
+// slices): about two chunks of eight per wavefront, at most kDiagMaxWaves.
+template <typename T, typename TV, int FMT>
+static void launch_diag_inv_sell(const MatArgs& A, long long stored, const TV* vals, T* dinv, hipStream_t s) {
+  const long long chunks = stored / (8LL * kSlice * std::max(A.n_slices, 1));
+  const int w = chunks >= 24 ? 16 : chunks >= 12 ? 8 : chunks >= 6 ? 4 : chunks >= 3 ? 2 : 1;
+  auto go = [&](auto wc) {
+    constexpr int W = decltype(wc)::value, SPB = (W < 4 ? 4 : W) / W;
+    diag_inv_sell_kernel<T, TV, FMT, W><<<div_up(A.n_slices, SPB), 64 * (W < 4 ? 4 : W), 0, s>>>(A, vals, dinv);
+  };
+  switch (w) {
+    case 16: go(std::integral_constant<int, 16>{}); break;
+    case 8: go(std::integral_constant<int, 8>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 2: go(std::integral_constant<int, 2>{}); break;
+    default: go(std::integral_constant<int, 1>{}); break;
+  }
+}
+
 template <typename T, typename TV = T>
-static void launch_diag_inv(const MatArgs& A, const TV* vals, T* dinv, hipStream_t s) {
+static void launch_diag_inv(const MatArgs& A, long long stored, const TV* vals, T* dinv, hipStream_t s) {
   if (A.fmt == kFmtSell16)
-    diag_inv_sell_kernel<T, TV, kFmtSell16><<<A.n_slices, 64 * kDiagWaves, 0, s>>>(A, vals, dinv);
+    launch_diag_inv_sell<T, TV, kFmtSell16>(A, stored, vals, dinv, s);
   else if (A.fmt == MIFE_FMT_SELL)
-    diag_inv_sell_kernel<T, TV, MIFE_FMT_SELL><<<A.n_slices, 64 * kDiagWaves, 0, s>>>(A, vals, dinv);
+    launch_diag_inv_sell<T, TV, MIFE_FMT_SELL>(A, stored, vals, dinv, s);
   else
     diag_inv_csr_kernel<T, TV><<<div_up(A.n, 256), 256, 0, s>>>(A, vals, dinv);
 }
@@ -819,6 +840,7 @@ struct Mat {
   MatArgs a;
   const int32_t* perm;
   int L;
+  long long stored;    // stored entries (SELL: including padding)
   bool graphs = true;  // CG chunks may replay cached HIP graphs (not without allocation)
 };
 
@@ -833,6 +855,7 @@ static int make_mat(const mife_matrix* A, Mat* m) {
   m->a = mat_args(A);
   m->perm = A->format == MIFE_FMT_SELL ? A->perm : nullptr;
   m->L = spmv_lanes(A->nnz, A->n);
+  m->stored = A->nnz;
   return MIFE_OK;
 }
 
@@ -1164,7 +1187,7 @@ static int cg_impl(const Mat& M, const T* vals, const T* b_in, T* x_io, double r
   permute_in_kernel<T><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   int rc = zero_state(st, s);
   if (rc) return rc;
-  launch_diag_inv<T>(M.a, vals, dinv, s);
+  launch_diag_inv<T>(M.a, M.stored, vals, dinv, s);
   launch_cg_init<T>(M, vals, b, x, dinv, r, z, p, q, st, rtol * rtol, maxit, s);
   HostState hs{};
   rc = read_state(st, &hs, s);
@@ -1340,7 +1363,7 @@ int mife_refine(const mife_matrix* A, const double* vals64, const float* vals32,
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
   if ((rc = zero_state(st, s))) return rc;
   if ((rc = zero_state(sto, s))) return rc;
-  launch_diag_inv<float>(M.a, vals32, dinv, s);
+  launch_diag_inv<float>(M.a, M.stored, vals32, dinv, s);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, x32, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, p, 0.0f);
   fill_kernel<float><<<vg, 256, 0, s>>>(n, q, 0.0f);
@@ -1434,7 +1457,7 @@ int minres_impl(const mife_matrix* A, const TV* vals, const double* b_in, double
   const int grid = row_grid(M.a, M.a.fmt != MIFE_FMT_CSR ? 64 : M.L);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
-  launch_diag_inv<double, TV>(M.a, vals, dinv, s);
+  launch_diag_inv<double, TV>(M.a, M.stored, vals, dinv, s);
   pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 1);
   fill_kernel<double><<<vg, 256, 0, s>>>(n, v, 0.0);
   fill_kernel<double><<<vg, 256, 0, s>>>(n, w1, 0.0);
@@ -1523,7 +1546,7 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   const int ag = std::min(div_up(n, 64), kMaxGrid);  // Gram-Schmidt update: 64-row tiles
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, b_in, b);
   permute_in_kernel<double><<<vg, 256, 0, s>>>(n, M.perm, x_io, x);
-  launch_diag_inv<double>(M.a, vals, dinv, s);
+  launch_diag_inv<double>(M.a, M.stored, vals, dinv, s);
   if (pc_inv) pc_override_kernel<<<vg, 256, 0, s>>>(n, M.perm, pc_inv, dinv, 0);
   if ((rc = zero_state(st, s))) return rc;
   launch_residual64(M, vals, b, x, r, st, s);

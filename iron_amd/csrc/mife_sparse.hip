@@ -120,26 +120,37 @@ __global__ void field_update_kernel(int n_free, const int32_t* __restrict__ free
   if (f < n_free) u[free_idx[f]] = x[f];
 }
 
-__global__ void backsub_kernel(int n, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                               const double* __restrict__ vals, const double* __restrict__ u,
-                               const double* __restrict__ src, const uint8_t* __restrict__ rhs_fixed,
-                               int iron_compat, double* __restrict__ R) {
-  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (i >= n) return;
-  double acc = 0.0;
-  const bool free_row = !(rhs_fixed && rhs_fixed[i]);
-  if (free_row) {
+// R_i = sum_k K_ik u_j - s_i for the rows whose RHS DOF is free (R_i = -s_i for
+// the others).  kBacksubLanes lanes share a row, each summing a strided part of
+// it four entries per trip (the SpMV's row_partial); the parts meet by shuffles
+// inside the lane group.  Rows are dealt over a resident grid.
+constexpr int kBacksubLanes = 16;
+
+__global__ __launch_bounds__(256) void backsub_kernel(int n, const int32_t* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const double* __restrict__ vals, const double* __restrict__ u,
+                                                      const double* __restrict__ src,
+                                                      const uint8_t* __restrict__ rhs_fixed, int iron_compat,
+                                                      double* __restrict__ R) {
+  constexpr int L = kBacksubLanes;
+  const int gpb = blockDim.x / L, sub = threadIdx.x % L;
+  for (int i = blockIdx.x * gpb + threadIdx.x / L; i < n; i += gridDim.x * gpb) {
+    // the row bounds, the fixed flag and the source are independent loads; a
+    // fixed row becomes an empty one
     const int k0 = row_ptr[i];
-    for (int k = k0 + lane; k < row_ptr[i + 1]; k += 64) {
+    int k1 = row_ptr[i + 1];
+    if (rhs_fixed && rhs_fixed[i]) k1 = k0;
+    double acc = 0.0;
+    if (iron_compat) {
       // equations_set_routines.F90:1806 reads the dependent value at the position
       // within the row; the correct product reads it at the column.
-      const int j = iron_compat ? (k - k0) : col[k];
-      acc += vals[k] * u[j];
+      for (int k = k0 + sub; k < k1; k += L) acc += vals[k] * u[k - k0];
+    } else {
+      acc = row_range_partial<double, L>(k0, k1, col, vals, u, sub);
     }
+    acc = group_sum<double, L>(acc);
+    if (sub == 0) R[i] = acc - (src ? src[i] : 0.0);
   }
-  acc = wave_sum(acc);
-  if (lane == 0) R[i] = acc - (src ? src[i] : 0.0);
 }
 
 // One wavefront per slice: base = min column of the slice, 16-bit offsets, and
@@ -375,8 +386,8 @@ int mife_backsubstitute(int n, const int32_t* row_ptr, const int32_t* col, const
                         const double* src, const uint8_t* rhs_fixed, int iron_compat, double* R, void* stream) {
   if (n == 0) return MIFE_OK;  // empty arrays may be passed as NULL
   MIFE_REQUIRE(row_ptr && col && vals && u && R, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
-  backsub_kernel<<<div_up(n, 4), 256, 0, (hipStream_t)stream>>>(n, row_ptr, col, vals, u, src, rhs_fixed,
-                                                                  iron_compat, R);
+  const int grid = std::max(1, std::min(div_up(n, 256 / kBacksubLanes), kMaxGrid));
+  backsub_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(n, row_ptr, col, vals, u, src, rhs_fixed, iron_compat, R);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }

@@ -73,11 +73,11 @@ __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
-// CSR partial sum of `row` held by lane `sub` of its L-lane group (not reduced).
+// CSR partial sum of entries [k0, k1) held by lane `sub` of its L-lane group
+// (not reduced), four entries per trip.
 template <typename T, int L, typename TV = T>
-__device__ __forceinline__ T row_partial(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                         const TV* __restrict__ vals, const T* __restrict__ x, int row, int sub) {
-  const int k0 = rp[row], k1 = rp[row + 1];
+__device__ __forceinline__ T row_range_partial(int k0, int k1, const int32_t* __restrict__ col,
+                                               const TV* __restrict__ vals, const T* __restrict__ x, int sub) {
   T a0 = 0, a1 = 0;
   int k = k0 + sub;
   for (; k + 3 * L < k1; k += 4 * L) {
@@ -92,6 +92,13 @@ __device__ __forceinline__ T row_partial(const int32_t* __restrict__ rp, const i
   }
   for (; k < k1; k += L) a0 += (T)stream_load(&vals[k]) * x[stream_load(&col[k])];
   return a0 + a1;
+}
+
+// CSR partial sum of `row` held by lane `sub` of its L-lane group (not reduced).
+template <typename T, int L, typename TV = T>
+__device__ __forceinline__ T row_partial(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                         const TV* __restrict__ vals, const T* __restrict__ x, int row, int sub) {
+  return row_range_partial<T, L, TV>(rp[row], rp[row + 1], col, vals, x, sub);
 }
 
 // SELL-64 row sum of slice `s`, lane `lane` (one row per lane, complete sum),
