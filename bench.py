@@ -574,7 +574,7 @@ def elastic_leg(runs=3, cpu_seconds=10.0):
     faces next to it left out by NEUMANN_INTEGRATED_ONLY DOFs one element in, so the
     nodal forces sum to -1 x the face area).  One pass: FP64 (and in the mixed pass FP32
     MFMA) element stiffness, solver values and RHS, Jacobi CG to a 1e-8 relative
-    residual (rows node by node, 16-bit SELL columns), field update, back-substitution."""
+    residual (SELL3: 3x3 node blocks, 16-bit node columns), field update, back-substitution."""
     import torch
     from iron_amd import _lib, basis, engine, fem, mesh
     ne = (96, 32, 32)
@@ -592,7 +592,7 @@ def elastic_leg(runs=3, cpu_seconds=10.0):
     asm = fem.ElasticAssembler(b, X, m.elem_nodes[0], csr, grid=ne)
     clamp = X[:, 0] == 0.0
     red = fem.DirichletReduction(csr, np.concatenate([clamp, clamp, clamp]))
-    S = fem.SolverMatrix(red, interleave=(3, N))
+    S = fem.SolverMatrix(red, interleave=(3, N), block=True)
     tip = np.nonzero(X[:, 0] == 3.0)[0]
     hx = 3.0 / ne[0]
     onb = (X[:, 1] == 0.0) | (X[:, 1] == 1.0) | (X[:, 2] == 0.0) | (X[:, 2] == 1.0)
@@ -644,7 +644,14 @@ def elastic_leg(runs=3, cpu_seconds=10.0):
     _lib.profile_enable(False)
     n_s, ms_s = prof.get("cg_spmv_f32", (0, 0.0))
     per = ms_s / max(n_s, 1)
-    alg = 8.0 * red.nnz + 24.0 * red.n_free   # values + int32 columns, z, p/q read, p/q written
+    if S.block == 3:
+        # SELL3: an FP32 value per nonzero and one 16-bit node column per 3x3 block
+        alg = 4.0 * red.nnz + 2.0 * red.nnz / 9.0 + 24.0 * red.n_free
+        kern = ("cg_spmv<float> (fused SpMV + p/q update + p.q) on SELL3 3x3 node blocks; bytes: 4 per nonzero, "
+                "2 per block (16-bit node column), 24 per row")
+    else:
+        alg = 8.0 * red.nnz + 24.0 * red.n_free   # values + int32 columns, z, p/q read, p/q written
+        kern = "cg_spmv<float> (fused SpMV + p/q update + p.q, 16-bit columns)"
     t_mx, info_mx = res[True]
     t_64, info_64 = res[False]
     a64 = prof.get("assemble_elastic", (0, 0.0))[1]
@@ -663,7 +670,8 @@ def elastic_leg(runs=3, cpu_seconds=10.0):
         "fp64_ms_per_pass": t_64 * 1e3, "fp64_iterations": info_64.iterations, "fp64_relres": info_64.relres,
         "assembly_ms_f64": a64, "assembly_ms_f32": a32,
         "assembly_elems_per_s_f32": E / (a32 * 1e-3) if a32 > 0 else None,
-        "roofline": {"kernel": "cg_spmv<float> (fused SpMV + p/q update + p.q, 16-bit columns)", "bound": "hbm",
+        "solver_format": S.fmt,
+        "roofline": {"kernel": kern, "bound": "hbm",
                      "achieved": alg / (per * 1e-3) / 1e9 if per > 0 else 0.0, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (alg / (per * 1e-3) / 1e9 / HBM_PEAK_GBS) if per > 0 else 0.0,
                      "bytes_per_launch": alg, "avg_launch_ms": per, "launches": n_s},

@@ -45,6 +45,7 @@ extern "C" {
 /* Solver-matrix storage formats. */
 #define MIFE_FMT_CSR 0
 #define MIFE_FMT_SELL 1
+#define MIFE_FMT_SELL3 3
 
 /* A square solver matrix (values passed separately, FP64 or FP32).
  * CSR: ptr = row_ptr[n+1], col = column indices, perm = NULL.
@@ -63,7 +64,17 @@ extern "C" {
  * without compression); padding slots are {-1, 0, 0, 0}.  SELL products then
  * give range k to the blocks that the dispatcher places on XCD k, start the
  * longest slices first and read a slice's bounds in one load; NULL keeps the
- * plain slice order.  CSR ignores both fields. */
+ * plain slice order.  CSR ignores both fields.
+ * SELL3 (three unknowns per node, the three rows of a node alike): the SELL
+ * layout over node rows with 3x3 blocks as entries.  n = 3 x node rows; vector
+ * position 3k + c (node row k in slice order, component c) is original row
+ * perm[3k + c]; ptr = slice_ptr[n_slices+1] in blocks (multiples
+ * of 64), n_slices = ceil(n / 192); block j of slice node row k of slice s
+ * (b = ptr[s] + 64 j + (k - 64 s)) has node column col_base[s] + col[b]
+ * (uint16_t, col_base required) and value t = 3 r + c (row r, column
+ * component c) at values[9 (ptr[s] + 64 j) + 64 t + (k - 64 s)]; padding
+ * blocks hold zeros.  nnz = stored values (9 per block).  slice_order as
+ * SELL, with the base and length in blocks. */
 typedef struct mife_matrix {
   int n;
   int format;

@@ -32,7 +32,7 @@ class NotConvergedError(CMFEError):
 
 
 class MatrixDesc(C.Structure):
-    """mife_matrix: a square solver matrix in CSR or SELL-64 storage."""
+    """mife_matrix: a square solver matrix in CSR, SELL-64 or node-block SELL3 storage."""
     _fields_ = [("n", C.c_int), ("format", C.c_int), ("ptr", C.c_void_p), ("col", C.c_void_p),
                 ("perm", C.c_void_p), ("n_slices", C.c_int), ("nnz", C.c_int64), ("col_base", C.c_void_p),
                 ("slice_order", C.c_void_p), ("order_span", C.c_int)]
@@ -40,6 +40,7 @@ class MatrixDesc(C.Structure):
 
 MIFE_FMT_CSR = 0
 MIFE_FMT_SELL = 1
+MIFE_FMT_SELL3 = 3
 
 
 class Segment(C.Structure):

@@ -136,6 +136,34 @@ __global__ __launch_bounds__(64 * (W < 4 ? 4 : W)) void diag_inv_sell_kernel(Mat
   }
 }
 
+// SELL3: one thread per node row k (lane of a slice) finds its block at node
+// column k (eight columns in flight; the real block comes before the padding,
+// whose column is also k) and takes its values 0, 4 and 8.
+template <typename T, typename TV>
+__global__ __launch_bounds__(256) void diag_inv_sell3_kernel(MatArgs A, const TV* __restrict__ vals,
+                                                             T* __restrict__ dinv) {
+  const int lane = threadIdx.x & 63, s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= A.n_slices) return;
+  const int k = s * kSlice + lane;
+  if (3 * k >= A.n) return;
+  const int base = A.ptr[s], len = (A.ptr[s + 1] - base) / kSlice, cb = A.cbase[s];
+  const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col);
+  int jd = -1;
+  for (int j0 = 0; j0 < len && jd < 0; j0 += 8) {
+    int cj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cj[u] = cb + (int)c16[base + min(j0 + u, len - 1) * kSlice + lane];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (jd < 0 && j0 + u < len && cj[u] == k) jd = j0 + u;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T d = jd >= 0 ? (T)vals[(size_t)9 * (base + jd * kSlice) + 4 * c * kSlice + lane] : (T)0;
+    dinv[3 * k + c] = (d != (T)0) ? (T)1 / d : (T)1;
+  }
+}
+
 template <typename T, typename TV = T>
 __global__ void diag_inv_csr_kernel(MatArgs A, const TV* __restrict__ vals, T* __restrict__ dinv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,7 +199,9 @@ static void launch_diag_inv_sell(const MatArgs& A, long long stored, const TV* v
 
 template <typename T, typename TV = T>
 static void launch_diag_inv(const MatArgs& A, long long stored, const TV* vals, T* dinv, hipStream_t s) {
-  if (A.fmt == kFmtSell16)
+  if (A.fmt == kFmtSell3)
+    diag_inv_sell3_kernel<T, TV><<<div_up(A.n_slices, 4), 256, 0, s>>>(A, vals, dinv);
+  else if (A.fmt == kFmtSell16)
     launch_diag_inv_sell<T, TV, kFmtSell16>(A, stored, vals, dinv, s);
   else if (A.fmt == MIFE_FMT_SELL)
     launch_diag_inv_sell<T, TV, MIFE_FMT_SELL>(A, stored, vals, dinv, s);
@@ -846,14 +876,17 @@ struct Mat {
 
 static int make_mat(const mife_matrix* A, Mat* m) {
   MIFE_REQUIRE(A && A->ptr && A->col, MIFE_ERR_POINTER_IS_NULL, "null matrix");
-  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL, MIFE_ERR_ARG, "invalid matrix format");
+  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL || A->format == MIFE_FMT_SELL3, MIFE_ERR_ARG,
+               "invalid matrix format");
   MIFE_REQUIRE(A->n > 0, MIFE_ERR_ARG, "empty matrix");
-  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || (A->perm && A->n_slices == div_up(A->n, kSlice)), MIFE_ERR_ARG,
+  MIFE_REQUIRE(A->format != MIFE_FMT_SELL || (A->perm && A->n_slices == div_up(A->n, kSlice)), MIFE_ERR_ARG,
                "SELL matrix needs perm and n_slices = ceil(n/64)");
+  MIFE_REQUIRE(A->format != MIFE_FMT_SELL3 || (A->perm && sell3_shape_ok(A)), MIFE_ERR_ARG,
+               "SELL3 matrix needs perm, n = 3 x node rows, n_slices = ceil(n/192) and col_base");
   MIFE_REQUIRE(!A->slice_order || ((uintptr_t)A->slice_order & 15) == 0, MIFE_ERR_ARG,
                "slice_order must be 16-byte aligned");
   m->a = mat_args(A);
-  m->perm = A->format == MIFE_FMT_SELL ? A->perm : nullptr;
+  m->perm = A->format != MIFE_FMT_CSR ? A->perm : nullptr;
   m->L = spmv_lanes(A->nnz, A->n);
   m->stored = A->nnz;
   return MIFE_OK;

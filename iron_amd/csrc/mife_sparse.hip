@@ -190,8 +190,11 @@ extern "C" {
 int mife_spmv(const mife_matrix* A, const void* vals, const void* x, void* y, int prec, void* stream) {
   MIFE_REQUIRE(A && A->ptr && A->col && vals && x && y, MIFE_ERR_POINTER_IS_NULL, "null pointer argument");
   MIFE_REQUIRE(prec == MIFE_F64 || prec == MIFE_F32, MIFE_ERR_ARG, "invalid precision %d", prec);
-  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL, MIFE_ERR_ARG, "invalid matrix format");
+  MIFE_REQUIRE(A->format == MIFE_FMT_CSR || A->format == MIFE_FMT_SELL || A->format == MIFE_FMT_SELL3, MIFE_ERR_ARG,
+               "invalid matrix format");
   if (A->n == 0) return MIFE_OK;
+  MIFE_REQUIRE(A->format != MIFE_FMT_SELL3 || sell3_shape_ok(A), MIFE_ERR_ARG,
+               "SELL3 matrix needs n = 3 x node rows, n_slices = ceil(n/192) and col_base");
   MIFE_REQUIRE(!A->slice_order || ((uintptr_t)A->slice_order & 15) == 0, MIFE_ERR_ARG,
                "slice_order must be 16-byte aligned");
   hipStream_t s = (hipStream_t)stream;

@@ -25,6 +25,8 @@ constexpr int kXcds = 8;
 
 // Internal format tag of SELL with 16-bit slice-relative columns.
 constexpr int kFmtSell16 = 2;
+// Node-block SELL (3x3 blocks, 16-bit slice-relative node columns): the public tag.
+constexpr int kFmtSell3 = MIFE_FMT_SELL3;
 
 // Reliable updates: a replaced residual whose r.r exceeds the recurrence's by
 // more than this factor restarts the recurrence (beta = 0).  Once the FP32
@@ -39,26 +41,34 @@ inline int sell_order_span(int n_slices) { return 4 * div_up(div_up(n_slices, kX
 
 struct MatArgs {
   int n;
-  int fmt;  // MIFE_FMT_CSR / MIFE_FMT_SELL / kFmtSell16
+  int fmt;  // MIFE_FMT_CSR / MIFE_FMT_SELL / kFmtSell16 / kFmtSell3 (n: scalar rows)
   const int32_t* ptr;
-  const int32_t* col;  // kFmtSell16: uint16 offsets
+  const int32_t* col;  // kFmtSell16, kFmtSell3: uint16 offsets
   int n_slices;
-  const int32_t* cbase;  // kFmtSell16: per-slice base column
+  const int32_t* cbase;  // kFmtSell16, kFmtSell3: per-slice base column (kFmtSell3: node)
   const int32_t* order;  // SELL schedule (8 ranges of `span` slices, -1 padded) or null
   int span;
   int split;  // scheduled SELL: wavefronts per slice (1, 2 or 4)
 };
+
+// Shape of a SELL3 descriptor: three rows per node row, ceil(node rows / 64)
+// slices, 16-bit node columns.
+inline bool sell3_shape_ok(const mife_matrix* A) {
+  return A->n % 3 == 0 && A->n_slices == div_up(A->n / 3, kSlice) && A->col_base != nullptr;
+}
 
 // Wavefronts per slice of a scheduled SELL product: enough wavefronts to fill
 // the chip on small matrices (long rows, few slices), one per slice on large.
 inline int sell_split(int n_slices) { return n_slices < 4096 ? 4 : (n_slices < 65536 ? 2 : 1); }
 
 inline MatArgs mat_args(const mife_matrix* A) {
-  const bool sell = A->format == MIFE_FMT_SELL;
+  const bool sell3 = A->format == MIFE_FMT_SELL3;
+  const bool sell = A->format == MIFE_FMT_SELL || sell3;
   const bool c16 = sell && A->col_base != nullptr;
   const bool sched = sell && A->slice_order != nullptr && A->order_span > 0;
-  return MatArgs{A->n, c16 ? kFmtSell16 : A->format, A->ptr, A->col, A->n_slices, c16 ? A->col_base : nullptr,
-                 sched ? A->slice_order : nullptr, sched ? A->order_span : 0, sched ? sell_split(A->n_slices) : 1};
+  return MatArgs{A->n, sell3 ? kFmtSell3 : (c16 ? kFmtSell16 : A->format), A->ptr, A->col, A->n_slices,
+                 c16 ? A->col_base : nullptr, sched ? A->slice_order : nullptr, sched ? A->order_span : 0,
+                 sched ? sell_split(A->n_slices) : 1};
 }
 
 template <typename T>
@@ -215,11 +225,115 @@ __device__ __forceinline__ T sell_row_part(const MatArgs& A, const TV* __restric
   return (a0 + a1) + (a2 + a3);
 }
 
+// Part h of W of the three row sums of a SELL3 node row (lane `lane` of the
+// slice whose blocks start at `base`, `len` per node row, base node cb): the
+// wavefront takes chunks h, h + W, ... of four blocks.  A block adds to its
+// rows the dot of its row with the three x values of its node column; blocks
+// alternate between two partial sums per row (a row's 81 FP32 products in one
+// running sum lose enough to slow FP32 CG by a fifth), added at the end.
+template <typename T, typename TV = T>
+__device__ __forceinline__ void sell3_row_part(const MatArgs& A, const TV* __restrict__ vals,
+                                               const T* __restrict__ x, int base, int len, int cb, int lane, int h,
+                                               int W, T (&acc)[3]) {
+  constexpr int D = 4;
+  const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col) + base + lane;
+  const TV* v = vals + (size_t)9 * base + lane;  // block j, value t: v[(9 j + t) * 64]
+  const T* xs = x + 3 * (size_t)cb;
+  T a[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  auto add = [&](const TV (&vb)[9], int c, T (&ar)[3]) {
+    const T* xp = xs + 3 * c;
+    const T x0 = xp[0], x1 = xp[1], x2 = xp[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) ar[r] += ((T)vb[3 * r] * x0 + (T)vb[3 * r + 1] * x1) + (T)vb[3 * r + 2] * x2;
+  };
+  int j = h * D;
+  for (; j + D <= len; j += W * D) {
+    int ci[D];
+    TV vi[D][9];
+#pragma unroll
+    for (int u = 0; u < D; ++u) ci[u] = (int)stream_load(c16 + (j + u) * kSlice);
+#pragma unroll
+    for (int u = 0; u < D; ++u)
+#pragma unroll
+      for (int t = 0; t < 9; ++t) vi[u][t] = stream_load(v + (size_t)(9 * (j + u) + t) * kSlice);
+#pragma unroll
+    for (int u = 0; u < D; ++u) add(vi[u], ci[u], a[u & 1]);
+  }
+  for (int k = j; k < len && k < j + D; ++k) {
+    TV vb[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) vb[t] = stream_load(v + (size_t)(9 * k + t) * kSlice);
+    add(vb, (int)stream_load(c16 + k * kSlice), a[(k - j) & 1]);
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) acc[r] = a[0][r] + a[1][r];
+}
+
+// SELL3 form of for_each_row: f(row, (A x)_row) for the three rows 3k + c of
+// each node row k.
+template <typename T, typename TV, typename F>
+__device__ __forceinline__ void for_each_node_row(const MatArgs& A, const TV* __restrict__ vals,
+                                                  const T* __restrict__ x, F&& f) {
+  const int wpb = blockDim.x >> 6, lane = threadIdx.x & 63;
+  auto emit = [&](int s, const T (&ax)[3]) {
+    const int row = 3 * (s * kSlice + lane);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (row + c < A.n) f(row + c, ax[c]);
+  };
+  if (A.order) {
+    // scheduled, as the scalar SELL path: W wavefronts share a slice and their
+    // partial sums meet in LDS, added in a fixed order by the first of them
+    __shared__ T part[3][16][kSlice];
+    const int W = min(A.split, wpb), spb = wpb / W;
+    const int wave = threadIdx.x >> 6, g = wave / W, h = wave - g * W;
+    const int xcd = blockIdx.x % kXcds, bpx = gridDim.x / kXcds;
+    const int4* ord = reinterpret_cast<const int4*>(A.order) + (size_t)xcd * A.span;
+    const int passes = (A.span + bpx * spb - 1) / (bpx * spb);
+    for (int ps = 0; ps < passes; ++ps) {
+      const int k = (ps * bpx + blockIdx.x / kXcds) * spb + g;
+      int4 slot = make_int4(-1, 0, 0, 0);
+      if (k < A.span) slot = ord[k];
+      const int s = slot.x;
+      T ax[3] = {(T)0, (T)0, (T)0};
+      if (s >= 0 && x) sell3_row_part<T, TV>(A, vals, x, slot.y, slot.z, slot.w, lane, W == 1 ? 0 : h, W, ax);
+      if (W == 1) {
+        if (s >= 0) emit(s, ax);
+        continue;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) part[c][wave][lane] = ax[c];
+      __syncthreads();
+      if (h == 0 && s >= 0) {
+        T t[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          t[c] = part[c][wave][lane];
+          for (int q = 1; q < W; ++q) t[c] += part[c][wave + q][lane];
+        }
+        emit(s, t);
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int s = blockIdx.x * wpb + (threadIdx.x >> 6); s < A.n_slices; s += gridDim.x * wpb) {
+      T ax[3] = {(T)0, (T)0, (T)0};
+      if (x) {
+        const int base = A.ptr[s];
+        sell3_row_part<T, TV>(A, vals, x, base, (A.ptr[s + 1] - base) / kSlice, A.cbase[s], lane, 0, 1, ax);
+      }
+      emit(s, ax);
+    }
+  }
+}
+
 // Calls f(row, (A x)_row) once per row (by one lane), over the whole grid.
 template <typename T, int FMT, int L, typename TV = T, typename F>
 __device__ __forceinline__ void for_each_row(const MatArgs& A, const TV* __restrict__ vals, const T* __restrict__ x,
                                              F&& f) {
-  if constexpr (FMT == MIFE_FMT_SELL || FMT == kFmtSell16) {
+  if constexpr (FMT == kFmtSell3) {
+    for_each_node_row<T, TV>(A, vals, x, f);
+  } else if constexpr (FMT == MIFE_FMT_SELL || FMT == kFmtSell16) {
     const int wpb = blockDim.x >> 6, lane = threadIdx.x & 63;
     auto one_slice = [&](int s) {
       T ax = (T)0;
@@ -313,6 +427,9 @@ inline int row_grid(const MatArgs& A, int L) {
       STMT;                                                                     \
     } else if ((A).fmt == kFmtSell16) {                                         \
       constexpr int FMT_ = kFmtSell16, L_ = 64;                                 \
+      STMT;                                                                     \
+    } else if ((A).fmt == kFmtSell3) {                                          \
+      constexpr int FMT_ = kFmtSell3, L_ = 64;                                  \
       STMT;                                                                     \
     } else {                                                                    \
       switch (L) {                                                              \

@@ -299,11 +299,12 @@ def _linear_setup(es, rm: RegularMesh, dm: DofMap, fixed: np.ndarray, couple=Non
     else:
         c.red = fem.CoupledReduction(c.csr, *couple)
         c.lead = c.red.lead
-    # a vector field on one mesh component: rows node by node (16-bit SELL columns)
+    # a vector field on one mesh component: rows node by node (16-bit SELL columns);
+    # three components whose free DOFs are whole nodes: 3x3 node blocks (SELL3)
     ncomp = len(dm.comp_mesh)
     nper = rm.num_nodes(dm.comp_mesh[0])
     inter = (ncomp, nper) if ncomp > 1 and dm.num_dofs == ncomp * nper else None
-    c.S = fem.SolverMatrix(c.red, interleave=inter)
+    c.S = fem.SolverMatrix(c.red, interleave=inter, block=ncomp == 3)
     c.ws = fem.KrylovWorkspace(c.red.n_free, dev)
     es._device_cache = c
     return c

@@ -316,15 +316,20 @@ class SolverMatrix:
     slices (one per XCD), each walked longest slice first.  ``interleave =
     (ncomp, n_per_comp)`` orders the rows of a vector field (iron's separated DOF
     order: component-major) node by node before the length sort, so that the
-    columns of a slice stay within a narrow range (16-bit columns).
+    columns of a slice stay within a narrow range (16-bit columns).  With
+    ``block=True`` and ``interleave = (3, n_per_comp)`` a field whose free DOFs are
+    whole nodes, the three rows of each node alike, is stored as SELL3: SELL over
+    node rows with 3x3 blocks (one 16-bit node column per nine values, three
+    contiguous x reads per block); other fields keep the scalar layout.
     """
 
     def __init__(self, red: DirichletReduction, fmt: str = "sell", sigma: int = 1024, compress: bool = True,
-                 schedule: bool = True, interleave: tuple | None = None):
+                 schedule: bool = True, interleave: tuple | None = None, block: bool = False):
         dev = red.row_ptr.device
         self.red = red
         self.n = red.n_free
         self.fmt = fmt
+        self.block = 1  # rows per stored row of the layout (SELL3: 3)
         if fmt == "csr":
             self.src = red.src_pos
             self.stored = red.nnz
@@ -333,6 +338,10 @@ class SolverMatrix:
             return
         if fmt != "sell":
             raise ValueError(f"unknown solver matrix format {fmt!r}")
+        if (block and compress and interleave is not None and int(interleave[0]) == 3 and self.n > 0
+                and not getattr(red, "coupled", False) and self._build_sell3(red, int(interleave[1]), sigma,
+                                                                             schedule)):
+            return
         n = self.n
         if n == 0:  # every DOF fixed: an empty matrix (valid, non-null pointers)
             z = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -398,6 +407,100 @@ class SolverMatrix:
         self.desc = MatrixDesc(n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), colp, ptr(self.perm), n_slices,
                                self.stored, ptr(self.col_base), ptr(self.slice_order), span.value)
 
+    def _build_sell3(self, red, nper: int, sigma: int, schedule: bool) -> bool:
+        """SELL3 storage (see the class notes); False, with nothing kept, when the
+        field does not qualify."""
+        dev = red.row_ptr.device
+        n = self.n
+        if n % 3:
+            return False
+        nf = n // 3
+        # free DOFs in whole nodes: component c of the k-th free node is free DOF c nf + k
+        free = red.free_idx.long()
+        fk = free[:nf]
+        if not (torch.equal(free[nf:2 * nf], fk + nper) and torch.equal(free[2 * nf:], fk + 2 * nper)):
+            return False
+        rp, col = red.row_ptr.long(), red.col.long()
+        l3 = rp[1:] - rp[:-1]
+        if bool((l3 % 3).any()) or not (torch.equal(l3[nf:2 * nf], l3[:nf]) and torch.equal(l3[2 * nf:], l3[:nf])):
+            return False
+        lk = l3[:nf] // 3
+        nptr = torch.zeros(nf + 1, dtype=torch.int64, device=dev)
+        nptr[1:] = torch.cumsum(lk, 0)
+        nnz_n = int(nptr[-1].item())
+        if nnz_n == 0:
+            return False
+        rowk = torch.repeat_interleave(torch.arange(nf, device=dev), lk)
+        el = torch.arange(nnz_n, device=dev) - nptr[rowk]
+        ncol = col[rp[rowk] + el]
+        if int(ncol.max().item()) >= nf:
+            return False
+        # row c nf + k holds, for each column component t in turn, t nf + (the node columns of k)
+        for c in range(3):
+            base_c = rp[c * nf + rowk] + el
+            for t in range(3):
+                if not torch.equal(col[base_c + t * lk[rowk]], t * nf + ncol):
+                    return False
+        nptr32, ncol32 = nptr.to(torch.int32), ncol.to(torch.int32)
+        perm_n = torch.empty(nf, dtype=torch.int32, device=dev)
+        call("mife_sell_permutation", nf, ptr(nptr32), None, int(sigma), ptr(perm_n), _stream())
+        pn = perm_n.long()
+        iperm_n = torch.empty_like(perm_n)
+        iperm_n[pn] = torch.arange(nf, dtype=torch.int32, device=dev)
+        ns = (nf + 63) // 64
+        lp = torch.zeros(ns * 64, dtype=torch.int64, device=dev)
+        lp[:nf] = lk[pn]
+        sp = torch.zeros(ns + 1, dtype=torch.int64, device=dev)
+        sp[1:] = torch.cumsum(lp.view(ns, 64).amax(dim=1) * 64, 0)
+        nblk = int(sp[-1].item())
+        if 9 * nblk >= 2 ** 31:
+            return False
+        slice_ptr = sp.to(torch.int32)
+        bcol = torch.empty(nblk, dtype=torch.int32, device=dev)
+        bsrc = torch.empty(nblk, dtype=torch.int32, device=dev)
+        call("mife_sell_build", nf, ptr(nptr32), ptr(ncol32), None, ptr(perm_n), ptr(iperm_n), ns, ptr(slice_ptr),
+             ptr(bcol), ptr(bsrc), _stream())
+        col16 = torch.empty(nblk, dtype=torch.int16, device=dev)
+        cbase = torch.empty(ns, dtype=torch.int32, device=dev)
+        ok = C.c_int(0)
+        call("mife_sell_compress", ns, ptr(slice_ptr), ptr(bcol), ptr(cbase), ptr(col16), C.byref(ok), _stream())
+        if not ok.value:
+            return False
+        # value t = 3 r + c of block b (node row k, node column at position e of
+        # row k): solver entry (r nf + k, c nf + column), stored t-major per 64-block chunk
+        q = bsrc.long()
+        valid = q >= 0
+        qv = q.clamp(min=0)
+        kk = rowk[qv]
+        ek = qv - nptr[kk]
+        b = torch.arange(nblk, device=dev)
+        lane = b % 64
+        dst0 = (b - lane) * 9 + lane
+        src_pos = red.src_pos.long()
+        vsrc = torch.full((9 * nblk,), -1, dtype=torch.int32, device=dev)
+        for t in range(9):
+            r, c = divmod(t, 3)
+            pos = rp[r * nf + kk] + c * lk[kk] + ek
+            vsrc[dst0 + 64 * t] = torch.where(valid, src_pos[pos], -1).to(torch.int32)
+        # vector position 3 k + c is solver row c nf + perm_n[k]
+        perm = (torch.arange(3, device=dev).view(1, 3) * nf + pn.view(-1, 1)).reshape(-1).to(torch.int32)
+        self.perm = perm
+        self.iperm = torch.empty_like(perm)
+        self.iperm[perm.long()] = torch.arange(n, dtype=torch.int32, device=dev)
+        self.slice_ptr, self.col, self.col16, self.col_base, self.src = slice_ptr, bcol, col16, cbase, vsrc
+        self.stored = 9 * nblk
+        self.padding = self.stored / max(red.nnz, 1) - 1.0
+        self.block = 3
+        self.fmt = "sell3"
+        self.slice_order = None
+        span = C.c_int(0)
+        if schedule:
+            self.slice_order = torch.empty(_lib.lib().mife_sell_order_len(ns), dtype=torch.int32, device=dev)
+            call("mife_sell_order", ns, ptr(slice_ptr), ptr(cbase), ptr(self.slice_order), C.byref(span), _stream())
+        self.desc = MatrixDesc(n, _lib.MIFE_FMT_SELL3, ptr(slice_ptr), ptr(col16), ptr(perm), ns, self.stored,
+                               ptr(cbase), ptr(self.slice_order), span.value)
+        return True
+
     def gather(self, vals: torch.Tensor, prec=F64, out: torch.Tensor | None = None) -> torch.Tensor:
         """Solver-matrix values (this storage) from equations-CSR values (FP64 or FP32)."""
         if getattr(self.red, "coupled", False):
@@ -441,13 +544,16 @@ MIXED_MIN_STORED = 16_000_000
 
 def auto_precision(n_components: int, stored: int) -> str:
     """The precision "auto" picks for one solve: "mixed" (FP32 Jacobi CG with FP64
-    residual replacement) when the solver matrix holds at least MIXED_MIN_STORED
-    stored entries, else "fp64" (``n_components`` is kept for callers that scope the
-    rule).  Measured on MI355X, full passes: 48^3 triquadratic diffusion (54 M
-    entries) mixed 37 ms against 52 ms in FP64; 96x32x32 trilinear elasticity (26 M,
-    three components) 70 ms against 82 ms; 16^3 trilinear Laplace (0.1 M) 2.0 ms
-    against 0.5 ms, where launches rather than bytes bound an iteration."""
-    return "mixed" if stored >= MIXED_MIN_STORED else "fp64"
+    residual replacement) for a scalar field whose solver matrix holds at least
+    MIXED_MIN_STORED stored entries, else "fp64".  Measured on MI355X, full passes:
+    48^3 triquadratic diffusion (54 M entries) mixed 36 ms against 51 ms in FP64;
+    16^3 trilinear Laplace (0.1 M) 2.0 ms against 0.4 ms, where launches rather
+    than bytes bound an iteration.  A three-component field (96x32x32 trilinear
+    elasticity, 3x3 node blocks) runs about 50% more iterations on the rounded
+    FP32 operator than FP64 CG (1,570-1,870 against 1,212, whatever the inner
+    vectors' precision), which eats the halved value stream: mixed 62-69 ms
+    against FP64 64-69 ms, so vector fields stay in FP64."""
+    return "mixed" if n_components == 1 and stored >= MIXED_MIN_STORED else "fp64"
 
 
 @dataclass
