@@ -112,6 +112,33 @@ def test_sell3_cg_and_mixed_cg_match_scalar():
     assert torch.equal(x, x3)
 
 
+def test_sell3_gmres_and_minres_match_scalar():
+    from iron_amd import fem
+    print("case SELL3 GMRES / MINRES 12x4x4", flush=True)
+    fem, red, v, bvec, N = _elastic((12, 4, 4), (3.0, 1.0, 1.0), seed=3)
+    S3 = fem.SolverMatrix(red, interleave=(3, N), block=True)
+    Ss = fem.SolverMatrix(red, interleave=(3, N))
+    assert S3.block == 3
+    ws = fem.KrylovWorkspace(red.n_free, "cuda")
+    out = {}
+    for name, S in (("sell3", S3), ("sell", Ss)):
+        vals = S.gather(v)
+        xg = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+        ig = fem.gmres(S, vals, bvec, xg, 1e-10, 20000, ws, restart=30)
+        xm = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+        im = fem.minres(S, S.gather(v, fem.F32), bvec, xm, 1e-9, 20000, ws)
+        print(f"{name}: GMRES {ig.iterations} its relres {ig.relres:.1e}, MINRES (FP32 values) {im.iterations} its "
+              f"relres {im.relres:.1e}", flush=True)
+        assert ig.relres <= 1e-10
+        out[name] = (xg, xm, ig.iterations)
+    scale = float(out["sell"][0].abs().max())
+    assert float((out["sell3"][0] - out["sell"][0]).abs().max()) <= 1e-7 * scale
+    # MINRES on the FP32 operator: both storages hold the same rounded values
+    assert float((out["sell3"][1] - out["sell"][1]).abs().max()) <= 1e-5 * scale
+    # restarted GMRES(30) follows the rounding of the products closely: within 5%
+    assert abs(out["sell3"][2] - out["sell"][2]) <= max(3, out["sell"][2] // 20)
+
+
 def test_sell3_declines_partial_nodes():
     from iron_amd import fem
     print("case SELL3 declined for rollers", flush=True)
