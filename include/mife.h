@@ -287,13 +287,18 @@ int mife_assemble_grid(const mife_element_desc* desc, const int32_t* n_elem, int
  *   [n_slices + 1], sell_src [stored] (equations-CSR position, -1: padding) and
  *   either sell_col16 [stored] + col_base [n_slices] (col16 = 1) or sell_col
  *   [stored].  Every slice has width `width` (the longest reduced row); padding
- *   entries hold the row's own column and no source. */
+ *   entries hold the row's own column and no source.
+ * With block = 1 and n_comp = 3 (counts.block = 1) the free DOFs are numbered
+ * node by node (solver index 3 k + c for the k-th free node, x fastest) and
+ * the reduced matrix is SELL3 (MIFE_FMT_SELL3): n_slices = ceil(free nodes /
+ * 64), width in node blocks, slice_ptr in blocks, sell_col16 [stored / 9] node
+ * columns with col_base [n_slices], sell_src [stored] (per block value). */
 typedef struct {
   int32_t n_elem[3];
   int32_t order;   /* 1 trilinear, 2 triquadratic */
   int32_t n_comp;  /* 1 (scalar field) or 3 (displacement) */
   int32_t fix_lo, fix_hi;
-  int32_t pad;
+  int32_t block;   /* 1: SELL3 storage for n_comp = 3 (when its 16-bit columns fit) */
   double extent[3];
   double u_lo, u_hi;
 } mife_box_desc;
@@ -301,7 +306,7 @@ typedef struct {
 typedef struct {
   int64_t n_nodes, n_elem, nodes_per_elem, n_dofs, nnz;
   int64_t n_free, red_nnz, n_slices, width, stored;
-  int32_t col16, pad;
+  int32_t col16, block; /* block: the reduced matrix is SELL3 */
 } mife_box_counts;
 
 typedef struct {

@@ -64,7 +64,7 @@ class ElementDesc(C.Structure):
 class BoxDesc(C.Structure):
     """mife_box_desc: a generated box, its order, components and Dirichlet x-planes."""
     _fields_ = [("n_elem", C.c_int32 * 3), ("order", C.c_int32), ("n_comp", C.c_int32), ("fix_lo", C.c_int32),
-                ("fix_hi", C.c_int32), ("pad", C.c_int32), ("extent", C.c_double * 3), ("u_lo", C.c_double),
+                ("fix_hi", C.c_int32), ("block", C.c_int32), ("extent", C.c_double * 3), ("u_lo", C.c_double),
                 ("u_hi", C.c_double)]
 
 
@@ -72,7 +72,7 @@ class BoxCounts(C.Structure):
     """mife_box_counts: sizes of a box's structures (closed forms, host)."""
     _fields_ = [("n_nodes", C.c_int64), ("n_elem", C.c_int64), ("nodes_per_elem", C.c_int64), ("n_dofs", C.c_int64),
                 ("nnz", C.c_int64), ("n_free", C.c_int64), ("red_nnz", C.c_int64), ("n_slices", C.c_int64),
-                ("width", C.c_int64), ("stored", C.c_int64), ("col16", C.c_int32), ("pad", C.c_int32)]
+                ("width", C.c_int64), ("stored", C.c_int64), ("col16", C.c_int32), ("block", C.c_int32)]
 
 
 class BoxArrays(C.Structure):

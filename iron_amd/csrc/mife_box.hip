@@ -31,7 +31,8 @@ struct Box {
   int P, NC, nx, ny, nz, Nx, Ny, Nz, N;
   int lo, hi, nfx;  // fixed x-planes (0/1) and free nodes per x-line
   int Sx, Sy, Sz;   // coupled (node, neighbour) pairs per x-, y- and z-line
-  int W;            // SELL width: the longest reduced row
+  int W;            // SELL width: the longest reduced row (SELL3: in node blocks)
+  int block;        // SELL3: free DOFs node by node, 3x3 node blocks
   long long n_free, nnz;
 };
 
@@ -82,11 +83,16 @@ static int make_box(const mife_box_desc* d, Box* B, mife_box_counts* out) {
   }
   for (int i = 0; i < b.Ny; ++i) my = neighbour_count(i, b.P, b.ny) > my ? neighbour_count(i, b.P, b.ny) : my;
   for (int i = 0; i < b.Nz; ++i) mz = neighbour_count(i, b.P, b.nz) > mz ? neighbour_count(i, b.P, b.nz) : mz;
-  b.W = b.NC * mx * my * mz;
   b.n_free = (long long)b.NC * b.nfx * b.Ny * b.Nz;
   b.nnz = (long long)b.NC * b.NC * b.Sx * (long long)b.Sy * b.Sz;
-  const long long n_slices = (b.n_free + kWave - 1) / kWave;
-  const long long stored = n_slices * kWave * b.W;
+  // a slice's columns (solver numbering of one component, or node numbering)
+  // lie within P (nfx Ny + nfx + 1) of its rows
+  const bool fits16 = 63 + 2LL * b.P * ((long long)b.nfx * b.Ny + b.nfx + 1) < 65536;
+  b.block = (d->block == 1 && b.NC == 3 && fits16) ? 1 : 0;
+  b.W = b.block ? mx * my * mz : b.NC * mx * my * mz;
+  const long long rows = b.block ? b.n_free / 3 : b.n_free;  // SELL rows: nodes or DOFs
+  const long long n_slices = (rows + kWave - 1) / kWave;
+  const long long stored = n_slices * kWave * b.W * (b.block ? 9 : 1);
   MIFE_REQUIRE(b.nnz < (1LL << 31) && stored < (1LL << 31), MIFE_ERR_ARG,
                "box matrix of %lld entries (%lld stored) exceeds the int32 range", b.nnz, stored);
   if (B) *B = b;
@@ -101,9 +107,8 @@ static int make_box(const mife_box_desc* d, Box* B, mife_box_counts* out) {
     out->n_slices = n_slices;
     out->width = b.W;
     out->stored = stored;
-    // a slice's columns lie within P (nfx Ny + nfx + 1) of its rows (solver
-    // numbering, one component)
-    out->col16 = (b.NC == 1 && 63 + 2LL * b.P * ((long long)b.nfx * b.Ny + b.nfx + 1) < 65536) ? 1 : 0;
+    out->col16 = (b.block || (b.NC == 1 && fits16)) ? 1 : 0;
+    out->block = b.block;
   }
   return MIFE_OK;
 }
@@ -117,9 +122,16 @@ __device__ __forceinline__ void node_ijk(const Box& B, int node, int& ix, int& i
   iz = node / (B.Nx * B.Ny);
 }
 
-// Solver index of DOF (c, ix, iy, iz), or -1 when fixed.
+// Free node index of node (ix, iy, iz) (x fastest over the free x-planes).
+__device__ __forceinline__ int box_free_node(const Box& B, int ix, int iy, int iz) {
+  return (iz * B.Ny + iy) * B.nfx + (ix - B.lo);
+}
+
+// Solver index of DOF (c, ix, iy, iz), or -1 when fixed: component by component
+// (free DOFs ascending), or node by node with SELL3.
 __device__ __forceinline__ int box_new_idx(const Box& B, int c, int ix, int iy, int iz) {
   if (box_x_fixed(B.lo, B.hi, B.Nx, ix)) return -1;
+  if (B.block) return 3 * box_free_node(B, ix, iy, iz) + c;
   return (c * B.Nz + iz) * B.Ny * B.nfx + iy * B.nfx + (ix - B.lo);
 }
 
@@ -265,6 +277,73 @@ __global__ __launch_bounds__(256) void box_sell_kernel(Box B, int n_slices, int3
   for (; j < B.W; ++j) put(j, pad, -1);
 }
 
+// SELL3 of the reduced matrix (NC = 3, free DOFs node by node): one wavefront
+// per slice of 64 free nodes, one node row per lane.  Block j of the row is
+// its j-th free neighbour node in (z, y, x) order, stored as a 16-bit offset
+// from the slice's smallest neighbour; its nine values come from the
+// equations-CSR entries (row (r, node), column (c, neighbour)), t = 3 r + c,
+// at src[9 (base + 64 j) + 64 t + lane].  Padding blocks (up to the uniform
+// width) point at the row's own node and have no source.
+__global__ __launch_bounds__(256) void box_sell3_kernel(Box B, int n_slices, int32_t* __restrict__ slice_ptr,
+                                                        uint16_t* __restrict__ col16, int32_t* __restrict__ col_base,
+                                                        int32_t* __restrict__ src) {
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (s >= n_slices) return;
+  const int nfn = (int)(B.n_free / 3);
+  const int k = s * 64 + lane;
+  const int base = s * 64 * B.W;  // in blocks
+  if (lane == 0) {
+    slice_ptr[s] = base;
+    if (s == n_slices - 1) slice_ptr[n_slices] = base + 64 * B.W;
+  }
+  const bool real = k < nfn;
+  int ix = 0, iy = 0, iz = 0;
+  if (real) {
+    ix = k % B.nfx + B.lo;
+    iy = (k / B.nfx) % B.Ny;
+    iz = k / (B.nfx * B.Ny);
+  }
+  int lx, hx, ly, hy, lz, hz;
+  elem_range(ix, B.P, B.nx, lx, hx);
+  elem_range(iy, B.P, B.ny, ly, hy);
+  elem_range(iz, B.P, B.nz, lz, hz);
+  const int cx = B.P * (hx - lx) + B.P + 1, cy = B.P * (hy - ly) + B.P + 1, cz = B.P * (hz - lz) + B.P + 1;
+  int jx_first = B.P * lx;
+  while (box_x_fixed(B.lo, B.hi, B.Nx, jx_first)) ++jx_first;
+  const int first = real ? box_free_node(B, jx_first, B.P * ly, B.P * lz) : nfn - 1;
+  int cb = first;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) cb = min(cb, __shfl_xor(cb, off, 64));
+  if (lane == 0) col_base[s] = cb;
+  const int per_c = cz * cy * cx;  // equations-CSR entries per column component in the row
+  int rs[3] = {0, 0, 0};
+  if (real)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) rs[r] = box_row_start(B, r, ix, iy, iz);
+  int j = 0;
+  if (real) {
+    for (int qz = 0; qz < cz; ++qz)
+      for (int qy = 0; qy < cy; ++qy)
+        for (int qx = 0; qx < cx; ++qx) {
+          const int jx = B.P * lx + qx;
+          if (box_x_fixed(B.lo, B.hi, B.Nx, jx)) continue;
+          const int b = base + j * 64 + lane;
+          col16[b] = (uint16_t)(box_free_node(B, jx, B.P * ly + qy, B.P * lz + qz) - cb);
+          const int p = (qz * cy + qy) * cx + qx;
+          int32_t* sv = src + (size_t)9 * (base + j * 64) + lane;
+#pragma unroll
+          for (int t = 0; t < 9; ++t) sv[64 * t] = rs[t / 3] + (t % 3) * per_c + p;
+          ++j;
+        }
+  }
+  for (; j < B.W; ++j) {
+    col16[base + j * 64 + lane] = (uint16_t)((real ? k : nfn - 1) - cb);
+    int32_t* sv = src + (size_t)9 * (base + j * 64) + lane;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) sv[64 * t] = -1;
+  }
+}
+
 }  // namespace
 }  // namespace mife
 
@@ -296,9 +375,13 @@ int mife_box_build(const mife_box_desc* desc, const mife_box_arrays* a, void* st
   const long long en = n.n_elem * n.nodes_per_elem;
   box_elems_kernel<<<(int)std::min<long long>((en + 255) / 256, 4096), 256, 0, s>>>(B, a->elem_nodes);
   box_csr_kernel<<<div_up(n.n_dofs, 4), 256, 0, s>>>(B, a->row_ptr, a->col);
-  box_sell_kernel<<<div_up(n.n_slices, 4), 256, 0, s>>>(B, (int)n.n_slices, a->slice_ptr,
-                                                        n.col16 ? nullptr : a->sell_col,
-                                                        n.col16 ? a->sell_col16 : nullptr, a->col_base, a->sell_src);
+  if (B.block)
+    box_sell3_kernel<<<div_up(n.n_slices, 4), 256, 0, s>>>(B, (int)n.n_slices, a->slice_ptr, a->sell_col16,
+                                                           a->col_base, a->sell_src);
+  else
+    box_sell_kernel<<<div_up(n.n_slices, 4), 256, 0, s>>>(B, (int)n.n_slices, a->slice_ptr,
+                                                          n.col16 ? nullptr : a->sell_col,
+                                                          n.col16 ? a->sell_col16 : nullptr, a->col_base, a->sell_src);
   MIFE_CHECK_HIP(hipGetLastError());
   return MIFE_OK;
 }
