@@ -649,14 +649,15 @@ typedef struct mife_segment {
   int32_t n;
   int32_t maxit;
   int32_t slot;
-  int32_t format;   /* MIFE_FMT_CSR or MIFE_FMT_SELL */
+  int32_t format;   /* MIFE_FMT_CSR, MIFE_FMT_SELL or MIFE_FMT_SELL3 (16-bit block columns at col16 + nz_off) */
   int64_t row_off;
   int64_t rp_off;   /* CSR: row_ptr[n+1]; SELL: slice_ptr[n_slices+1] (entry offsets, multiples of 64) */
   int64_t nz_off;
   double rtol;
   int32_t n_slices; /* SELL: ceil(n / 64) */
   int32_t prec;     /* MIFE_F64 or MIFE_F32 */
-  int64_t cb_off;   /* SELL: offset of the per-slice base columns (16-bit columns), or -1 (int32 columns) */
+  int64_t cb_off;   /* SELL: offset of the per-slice base columns (16-bit columns), or -1 (int32 columns);
+                       SELL3: required */
 } mife_segment;
 
 /* Shared storage of the grouped segments: row / slice pointers, int32 columns,

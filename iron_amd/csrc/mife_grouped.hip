@@ -115,6 +115,24 @@ __device__ __forceinline__ T slice_dot(const View& V, const mife_segment& S, con
 template <typename T, typename F>
 __device__ __forceinline__ void block_rows(const View& V, const Block& B, const mife_segment& S,
                                            const T* __restrict__ vpool, const T* __restrict__ x, F&& f) {
+  if (S.format == MIFE_FMT_SELL3) {
+    // node rows: lane k of slice s gives rows 3 (64 s + k) + c; block columns in
+    // the 16-bit column pool at the segment's value offset
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave < B.rows) {
+      const int s = B.row0 + wave;
+      const int32_t* sp = V.rp + S.rp_off;
+      const int base = sp[s], len = (sp[s + 1] - base) / kSlice;
+      T ax[3];
+      sell3_sum<T, T>(V.col16 + S.nz_off + base + lane, vpool + S.nz_off + (size_t)9 * base + lane,
+                      x + S.row_off + 3 * (size_t)V.cbase[S.cb_off + s], len, 0, 1, ax);
+      const int row = 3 * (s * kSlice + lane);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (row + c < S.n) f(row + c, ax[c]);
+    }
+    return;
+  }
   if (S.format == MIFE_FMT_SELL) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (wave < B.rows) {
@@ -134,7 +152,10 @@ __device__ __forceinline__ void block_rows(const View& V, const Block& B, const 
 
 // First row and row count of block B (vector kernels: one row per thread).
 __device__ __forceinline__ void block_row_range(const Block& B, const mife_segment& S, int& r0, int& r1) {
-  if (S.format == MIFE_FMT_SELL) {
+  if (S.format == MIFE_FMT_SELL3) {
+    r0 = B.row0 * 3 * kSlice;
+    r1 = min(S.n, (B.row0 + B.rows) * 3 * kSlice);
+  } else if (S.format == MIFE_FMT_SELL) {
     r0 = B.row0 * kSlice;
     r1 = min(S.n, (B.row0 + B.rows) * kSlice);
   } else {
@@ -165,6 +186,30 @@ using namespace grouped;
 template <typename T>
 __device__ __forceinline__ void diag_body(const View& V, const Block& B, const mife_segment& S,
                                           const T* __restrict__ vpool, T* __restrict__ dinv) {
+  if (S.format == MIFE_FMT_SELL3) {
+    // node row k: its block at node column k (before the padding, whose column
+    // is also k), values 0, 4 and 8
+    const int32_t* sp = V.rp + S.rp_off;
+    const T* val = vpool + S.nz_off;
+    for (int t = threadIdx.x; t < B.rows * kSlice; t += blockDim.x) {
+      const int s = B.row0 + t / kSlice, lane = t % kSlice;
+      const int k = s * kSlice + lane;
+      if (3 * k >= S.n) continue;
+      const int base = sp[s], len = (sp[s + 1] - base) / kSlice, cb = V.cbase[S.cb_off + s];
+      int jd = -1;
+      for (int j = 0; j < len; ++j)
+        if (cb + (int)V.col16[S.nz_off + base + j * kSlice + lane] == k) {
+          jd = j;
+          break;
+        }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T d = jd >= 0 ? val[(size_t)9 * (base + jd * kSlice) + 4 * c * kSlice + lane] : (T)0;
+        dinv[S.row_off + 3 * k + c] = d != (T)0 ? (T)1 / d : (T)1;
+      }
+    }
+    return;
+  }
   int r0, r1;
   block_row_range(B, S, r0, r1);
   const int row = r0 + (int)threadIdx.x;
@@ -547,13 +592,15 @@ int mife_grouped_setup(const mife_segment* segs_host, int n_seg, int max_segment
     const mife_segment& S = segs_host[k];
     MIFE_REQUIRE(S.n >= 0, MIFE_ERR_ARG, "segment %d has %d rows", k, S.n);
     MIFE_REQUIRE(S.slot >= 0 && S.slot < max_segments, MIFE_ERR_ARG, "segment slot %d out of range", S.slot);
-    MIFE_REQUIRE(S.format == MIFE_FMT_CSR || (S.format == MIFE_FMT_SELL && S.n_slices == div_up(S.n, kSlice)),
+    MIFE_REQUIRE(S.format == MIFE_FMT_CSR || (S.format == MIFE_FMT_SELL && S.n_slices == div_up(S.n, kSlice)) ||
+                     (S.format == MIFE_FMT_SELL3 && S.n % 3 == 0 && S.n_slices == div_up(S.n / 3, kSlice) &&
+                      S.cb_off >= 0),
                  MIFE_ERR_ARG, "segment %d: format %d / n_slices %d", k, S.format, S.n_slices);
     MIFE_REQUIRE(S.prec == MIFE_F64 || S.prec == MIFE_F32, MIFE_ERR_ARG, "segment %d: precision %d", k, S.prec);
     H.segs[S.slot] = S;
     H.seg_parts[k] = nb;
     H.seg_ids[k] = S.slot;
-    const bool sell = S.format == MIFE_FMT_SELL;
+    const bool sell = S.format != MIFE_FMT_CSR;
     const int per = sell ? kSlicesPerBlock : 4 * kRowsPerBlock;
     const int units = sell ? S.n_slices : S.n;
     int r0 = 0;

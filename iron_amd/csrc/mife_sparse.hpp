@@ -225,20 +225,16 @@ __device__ __forceinline__ T sell_row_part(const MatArgs& A, const TV* __restric
   return (a0 + a1) + (a2 + a3);
 }
 
-// Part h of W of the three row sums of a SELL3 node row (lane `lane` of the
-// slice whose blocks start at `base`, `len` per node row, base node cb): the
+// Part h of W of the three row sums of a SELL3 node row: c16 and v point at
+// the row's first block column and first value (lane offset included; block j
+// value t at v[(9 j + t) * 64]), xs at x of the slice's base node.  The
 // wavefront takes chunks h, h + W, ... of four blocks.  A block adds to its
 // rows the dot of its row with the three x values of its node column; blocks
-// alternate between two partial sums per row (a row's 81 FP32 products in one
-// running sum lose enough to slow FP32 CG by a fifth), added at the end.
+// alternate between two partial sums per row, added at the end.
 template <typename T, typename TV = T>
-__device__ __forceinline__ void sell3_row_part(const MatArgs& A, const TV* __restrict__ vals,
-                                               const T* __restrict__ x, int base, int len, int cb, int lane, int h,
-                                               int W, T (&acc)[3]) {
+__device__ __forceinline__ void sell3_sum(const uint16_t* __restrict__ c16, const TV* __restrict__ v,
+                                          const T* __restrict__ xs, int len, int h, int W, T (&acc)[3]) {
   constexpr int D = 4;
-  const uint16_t* c16 = reinterpret_cast<const uint16_t*>(A.col) + base + lane;
-  const TV* v = vals + (size_t)9 * base + lane;  // block j, value t: v[(9 j + t) * 64]
-  const T* xs = x + 3 * (size_t)cb;
   T a[2][3] = {{0, 0, 0}, {0, 0, 0}};
   auto add = [&](const TV (&vb)[9], int c, T (&ar)[3]) {
     const T* xp = xs + 3 * c;
@@ -267,6 +263,16 @@ __device__ __forceinline__ void sell3_row_part(const MatArgs& A, const TV* __res
   }
 #pragma unroll
   for (int r = 0; r < 3; ++r) acc[r] = a[0][r] + a[1][r];
+}
+
+// sell3_sum for lane `lane` of the slice whose blocks start at `base` (`len`
+// per node row, base node cb) of matrix A.
+template <typename T, typename TV = T>
+__device__ __forceinline__ void sell3_row_part(const MatArgs& A, const TV* __restrict__ vals,
+                                               const T* __restrict__ x, int base, int len, int cb, int lane, int h,
+                                               int W, T (&acc)[3]) {
+  sell3_sum<T, TV>(reinterpret_cast<const uint16_t*>(A.col) + base + lane, vals + (size_t)9 * base + lane,
+                   x + 3 * (size_t)cb, len, h, W, acc);
 }
 
 // SELL3 form of for_each_row: f(row, (A x)_row) for the three rows 3k + c of

@@ -90,18 +90,22 @@ def _q1(device) -> fem.DeviceBasis:
 
 
 class _BoxSolverMatrix:
-    """The reduced matrix's SELL-64 storage of a box (solver order, uniform slice
-    width): what the grouped server copies into its pools, and a solver matrix of
-    its own for the single-solve path (identity permutation)."""
+    """The reduced matrix's storage of a box (solver order, uniform slice width):
+    SELL-64, or SELL3 node blocks for an elasticity box (free DOFs node by node).
+    What the grouped server copies into its pools, and a solver matrix of its own
+    for the single-solve path (identity permutation)."""
 
     def __init__(self, cnt: _lib.BoxCounts, device):
         self.n = int(cnt.n_free)
         self.stored = int(cnt.stored)
+        self.block = 3 if cnt.block else 1
         ns = int(cnt.n_slices)
         self.slice_ptr = torch.empty(ns + 1, dtype=torch.int32, device=device)
         self.src = torch.empty(self.stored, dtype=torch.int32, device=device)
+        # SELL3: one 16-bit node column per block of nine values
+        self.n_col = self.stored // self.block
         if cnt.col16:
-            self.col16 = torch.empty(self.stored, dtype=torch.int16, device=device)
+            self.col16 = torch.empty(self.n_col, dtype=torch.int16, device=device)
             self.col_base = torch.empty(ns, dtype=torch.int32, device=device)
             self.col = None
         else:
@@ -109,7 +113,8 @@ class _BoxSolverMatrix:
             self.col16 = self.col_base = None
         self.perm = torch.arange(self.n, dtype=torch.int32, device=device)
         colp = ptr(self.col16) if self.col16 is not None else ptr(self.col)
-        self.desc = _lib.MatrixDesc(self.n, _lib.MIFE_FMT_SELL, ptr(self.slice_ptr), colp, ptr(self.perm), ns,
+        self.format = _lib.MIFE_FMT_SELL3 if self.block == 3 else _lib.MIFE_FMT_SELL
+        self.desc = _lib.MatrixDesc(self.n, self.format, ptr(self.slice_ptr), colp, ptr(self.perm), ns,
                                     self.stored, ptr(self.col_base), None, 0)
 
     def gather(self, vals, prec=fem.F64, out=None):
@@ -147,8 +152,11 @@ class _Structure:
     """Device structures of one request shape (cached by (kind, n_elem)), built by
     mife_box_build from closed forms of the box: trilinear elements, the
     equations CSR, the Dirichlet reduction (diffusion: u = 0 on x = 0 and u = 1 on
-    x = 1; elasticity: clamped at x = 0) and the reduced matrix's SELL-64
-    storage.  The sizes come from the box alone, so nothing waits for the device."""
+    x = 1; elasticity: clamped at x = 0) and the reduced matrix's storage (SELL-64;
+    elasticity boxes SELL3 node blocks while ``block`` is set).  The sizes come from
+    the box alone, so nothing waits for the device."""
+
+    block = True
 
     def __init__(self, kind: str, n_elem, device):
         self.kind = kind
@@ -157,8 +165,8 @@ class _Structure:
         ncomp = 1 if kind == "diffusion" else 3
         self.ncomp = ncomp
         extent = (1.0, 1.0, 1.0) if kind == "diffusion" else (3.0, 1.0, 1.0)
-        desc = _lib.BoxDesc(self.n_elem, 1, ncomp, 1, 1 if kind == "diffusion" else 0, 0, extent, 0.0,
-                            1.0 if kind == "diffusion" else 0.0)
+        desc = _lib.BoxDesc(self.n_elem, 1, ncomp, 1, 1 if kind == "diffusion" else 0,
+                            1 if ncomp == 3 and self.block else 0, extent, 0.0, 1.0 if kind == "diffusion" else 0.0)
         cnt = _lib.BoxCounts()
         call("mife_box_sizes", C.byref(desc), C.byref(cnt))
         self.num_elements = int(cnt.n_elem)
@@ -439,14 +447,14 @@ class Server:
             call("mife_gather_values", stored, ptr(S.src), ptr(vals), fem.F64, ptr(self.val32) + 4 * nzo, fem.F32, s)
         self.rp[rpo:rpo + ns + 1].copy_(S.slice_ptr[: ns + 1])
         if c16:
-            self.col16[nzo:nzo + stored].copy_(S.col16[:stored])
+            self.col16[nzo:nzo + S.n_col].copy_(S.col16[:S.n_col])
             self.cbase[cbo:cbo + ns].copy_(S.col_base[:ns])
         else:
             self.col[nzo:nzo + stored].copy_(S.col[:stored])
         call("mife_solver_rhs", nf, ptr(red.free_idx), ptr(st.row_ptr), ptr(st.col), ptr(vals),
              ptr(red.new_idx), ptr(st.u0), None, ptr(st.rhs), None, ptr(self.b) + 8 * ro, s)
         self.x[ro:ro + nf].zero_()
-        e["seg"] = Segment(nf, 200000, e["slot"], _lib.MIFE_FMT_SELL, ro, rpo, nzo, e["req"].rtol, ns,
+        e["seg"] = Segment(nf, 200000, e["slot"], S.format, ro, rpo, nzo, e["req"].rtol, ns,
                            fem.F32 if e["mixed"] else fem.F64, cbo if c16 else -1)
         e["cbo_used"] = c16
 
