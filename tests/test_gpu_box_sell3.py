@@ -92,3 +92,38 @@ def test_box_sell3_equals_scalar_box(ne, order, lo, hi):
     y1, _ = _product(c1, t1, _lib.MIFE_FMT_SELL3, vals_eq, x1)
     scale = float(y0.abs().max())
     assert float((y1[i1] - y0[i0]).abs().max()) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("policy", ["fp64", "mixed"])
+def test_served_stream_sell3_equals_scalar_layout(policy):
+    """A small served stream (diffusion and elasticity boxes) with the elasticity
+    boxes in SELL3 node blocks against the same stream on scalar SELL: every
+    request's solution, iteration count and true residual."""
+    from iron_amd import serving
+    print(f"case served stream SELL3 vs SELL, policy {policy}", flush=True)
+    res = {}
+    for block in (True, False):
+        serving._Structure.block = block
+        try:
+            srv = serving.Server(hbm_budget=4e9, max_segments=64, chunk=16, use_graph=False, precision=policy)
+            out = srv.run(serving.make_stream(12, seed=77, lo=6, hi=12))
+        finally:
+            serving._Structure.block = True
+        torch.cuda.synchronize()
+        res[block] = {(r.kind, r.n_elem, r.seed): r for r in out}
+    assert any(k[0] == "elasticity" for k in res[True])
+    for k, rb in res[True].items():
+        rs = res[False][k]
+        assert rb.relres <= 1e-8 and rs.relres <= 1e-8
+        scale = max(float(np.abs(rs.u).max()), 1e-300)
+        if k[0] == "diffusion":
+            # untouched by the layout, and independent of the group: bit for bit
+            assert rb.iterations == rs.iterations and np.array_equal(rb.u, rs.u), k
+        elif policy == "fp64":
+            # the same FP64 CG up to the order of each row's sums
+            assert float(np.abs(rb.u - rs.u).max()) <= 1e-9 * scale, k
+            assert abs(rb.iterations - rs.iterations) <= max(3, rs.iterations // 20), k
+        else:
+            # FP32 recurrences on the rounded operator take their own paths: both meet
+            # the FP64 residual bound above and agree to the conditioning's reach
+            assert float(np.abs(rb.u - rs.u).max()) <= 1e-4 * scale, k
