@@ -568,12 +568,16 @@ class Server:
             self.timing["status"] += time.perf_counter() - ts
             for e in active:
                 # algorithmic bytes of the grouped products this chunk ran for the segment:
-                # per nonzero its value and column (16-bit or int32), per row z (gathered
-                # once), p and q read and written
+                # per nonzero its value and column (16-bit or int32; SELL3: one 16-bit
+                # column per nine values), per row z (gathered once), p and q read and
+                # written
                 ran = int(it[e["seg"].slot]) - e.get("it_seen", 0)
                 e["it_seen"] = int(it[e["seg"].slot])
                 self.stats["iterations"] += ran
-                cb = 2.0 if e["cbo_used"] else 4.0
+                if e["seg"].format == _lib.MIFE_FMT_SELL3:
+                    cb = 2.0 / 9.0
+                else:
+                    cb = 2.0 if e["cbo_used"] else 4.0
                 vb = 4.0 if e["mixed"] else 8.0
                 self.stats["spmv_bytes"] += ran * ((vb + cb) * e["nnz"] + 5.0 * vb * e["nf"])
             finished = [e for e in active if dn[e["seg"].slot]]
