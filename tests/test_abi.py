@@ -123,6 +123,43 @@ def test_argument_errors_without_gpu():
     assert L.mife_sell_order_len(13397) == 4 * 8 * 1676   # ceil(13397 / 8) = 1675 -> 1676
 
 
+def test_sell3_descriptors_without_gpu():
+    """The node-block (SELL3) storage: its format tag and the box builder's block
+    fields agree between the header and the bindings; malformed SELL3
+    descriptors are refused before any device work; the box sizes report the
+    node-block layout for three components only."""
+    import re
+    from iron_amd import _lib
+    text = (ROOT / "include" / "mife.h").read_text()
+    assert int(re.search(r"#define MIFE_FMT_SELL3 (\d+)", text).group(1)) == _lib.MIFE_FMT_SELL3
+    assert "block" in [f[0] for f in _lib.BoxDesc._fields_]
+    assert "block" in [f[0] for f in _lib.BoxCounts._fields_]
+    L = _lib.lib()
+    M = _lib.MatrixDesc
+    F = _lib.MIFE_FMT_SELL3
+    # rows not a multiple of three
+    d = M(4, F, 1, 1, 1, 1, 36, 1, None, 0)
+    assert L.mife_spmv(ctypes.byref(d), 1, 1, 1, 0, None) == 2 and "SELL3" in _lib.last_error()
+    # n_slices must be ceil(n / 192)
+    d = M(6, F, 1, 1, 1, 2, 36, 1, None, 0)
+    assert L.mife_cg(ctypes.byref(d), 1, 1, 1, 0, 1e-8, 10, 1, 1 << 30, None, None, None) == 2
+    assert "SELL3" in _lib.last_error()
+    # 16-bit node columns are required
+    d = M(6, F, 1, 1, 1, 1, 36, None, None, 0)
+    assert L.mife_spmv(ctypes.byref(d), 1, 1, 1, 0, None) == 2
+    # box sizes: an elasticity box clamped at x = 0 asks for blocks and gets them
+    c = _lib.BoxCounts()
+    _lib.call("mife_box_sizes", ctypes.byref(_lib.BoxDesc((4, 3, 3), 1, 3, 1, 0, 1, (1.0, 1.0, 1.0), 0.0, 0.0)),
+              ctypes.byref(c))
+    free_nodes = 4 * 4 * 4   # x = 0 plane fixed: 4 x-planes of 4 x 4 nodes
+    assert c.block == 1 and c.col16 == 1 and c.n_free == 3 * free_nodes
+    assert c.n_slices == (free_nodes + 63) // 64 and c.stored == 9 * 64 * c.n_slices * c.width
+    # a scalar box ignores the request
+    _lib.call("mife_box_sizes", ctypes.byref(_lib.BoxDesc((4, 3, 3), 1, 1, 1, 1, 1, (1.0, 1.0, 1.0), 0.0, 1.0)),
+              ctypes.byref(c))
+    assert c.block == 0
+
+
 def test_one_hip_runtime_in_the_process():
     """torch bundles a HIP runtime with the soname the library links against; the
     binding loads torch's first so that the process holds one runtime (two runtimes
