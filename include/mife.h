@@ -743,6 +743,13 @@ int mife_graphs_enable(int on);
 /* Number of instantiated CG chunk graphs held by the library (at most 16). */
 int mife_graphs_cached(void);
 
+/* Classical Gram-Schmidt refinement of mife_gmres, process-wide: 0 never (one
+ * pass per iteration), 1 always (a second pass with the correction).  Mirrors
+ * PETSc's KSPGMRESSetCGSRefinementType (KSP_GMRES_CGS_REFINE_NEVER /
+ * _ALWAYS), which iron leaves at PETSc's default (src/solver_routines.F90
+ * sets only the restart, via Petsc_KSPGMRESSetRestart, src/cmiss_petsc.F90:2659). */
+int mife_gmres_cgs_refinement(int mode);
+
 #ifdef __cplusplus
 }
 #endif

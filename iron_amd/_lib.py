@@ -178,6 +178,7 @@ _SIG = {
     "mife_grouped_cg_status": (_I, [_I, _P, _P, _P, _P, _P]),
     "mife_profile_enable": (_I, [_I]),
     "mife_graphs_enable": (_I, [_I]),
+    "mife_gmres_cgs_refinement": (_I, [_I]),
     "mife_graphs_cached": (_I, []),
     "mife_profile_reset": (_I, []),
     "mife_profile_read": (_I, [_I, C.c_char_p, _P, _P, C.POINTER(_I)]),
@@ -245,6 +246,11 @@ def call(name: str, *args):
 def graphs_enable(on: bool = True):
     """HIP-graph replay of CG iteration chunks on small systems (default on)."""
     call("mife_graphs_enable", 1 if on else 0)
+
+
+def gmres_cgs_refinement(mode: int):
+    """GMRES classical Gram-Schmidt refinement: 0 never (one pass), 1 always."""
+    call("mife_gmres_cgs_refinement", int(mode))
 
 
 def profile_enable(on: bool = True):

@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) void gmres_givens_kernel(int k, int m, const d
   double q = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) q += nparts[b];
   const double nrm2 = block_sum(q, red);
-  for (int j = threadIdx.x; j <= k; j += blockDim.x) col[j] = dot_coef(h1, j) + dot_coef(h2, j);
+  for (int j = threadIdx.x; j <= k; j += blockDim.x) col[j] = dot_coef(h1, j) + (h2 ? dot_coef(h2, j) : 0.0);
   for (int j = threadIdx.x; j < k; j += blockDim.x) {
     c_s[j] = cs[j];
     s_s[j] = sn[j];
@@ -961,6 +961,8 @@ constexpr int kGraphMaxRows = 131072;  // launch-bound below this many rows
 constexpr int kGraphCache = 16;
 
 static bool g_graphs_on = true;
+// GMRES classical Gram-Schmidt refinement: 0 never (one pass), 1 always (two)
+static int g_gmres_refine = 0;
 
 struct GraphKey {
   int dev, tsize, n, fmt, n_slices, span, split, L;
@@ -1334,6 +1336,12 @@ int mife_graphs_enable(int on) {
   return MIFE_OK;
 }
 
+int mife_gmres_cgs_refinement(int mode) {
+  MIFE_REQUIRE(mode == 0 || mode == 1, MIFE_ERR_ARG, "CGS refinement mode %d (0 never, 1 always)", mode);
+  g_gmres_refine = mode;
+  return MIFE_OK;
+}
+
 int mife_graphs_cached(void) {
   std::lock_guard<std::mutex> lock(g_graph_mu);
   return (int)g_graphs.size();
@@ -1565,6 +1573,7 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
   hipStream_t s = (hipStream_t)stream;
   const int n = M.a.n;
   const int m = restart;
+  const bool refine = g_gmres_refine != 0;
   Work w{(char*)work, 0, work_bytes};
   GmresBufs B;
   gmres_layout(w, n, m, B);
@@ -1604,15 +1613,19 @@ int mife_gmres(const mife_matrix* A, const double* vals, const double* b_in, dou
       for (int c = 0; c < cnt; ++c) {
         const int kk = k + c;
         launch_spmv<double>(M.a, vals, t, wv, M.L, s);   // t = D^-1 V_kk (written with V_kk)
-        // classical Gram-Schmidt, two passes (coefficients h, then the
-        // correction y); the second update also yields the norm partials, which
-        // the rotation kernel reduces
+        // classical Gram-Schmidt: coefficients h, then (refinement "always") a
+        // second pass with the correction y; the last update also yields the
+        // norm partials, which the rotation kernel reduces
         gs_dots_kernel<<<dim3(kDotChunks, kk + 1), 256, 0, s>>>(n, V, ldv, wv, hdev);
-        gs_axpy_kernel<false><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, hdev, wv, nullptr);
-        gs_dots_kernel<<<dim3(kDotChunks, kk + 1), 256, 0, s>>>(n, V, ldv, wv, ydev);
-        gs_axpy_kernel<true><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, ydev, wv, ndev);
-        gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, ydev, ndev, ag, Hdev, csdev, sndev, gdev, cy, rtol * bn,
-                                             (it + c + 1 >= maxit || kk + 1 == m) ? 1 : 0);
+        if (refine) {
+          gs_axpy_kernel<false><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, hdev, wv, nullptr);
+          gs_dots_kernel<<<dim3(kDotChunks, kk + 1), 256, 0, s>>>(n, V, ldv, wv, ydev);
+          gs_axpy_kernel<true><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, ydev, wv, ndev);
+        } else {
+          gs_axpy_kernel<true><<<ag, 64 * kAxpyWaves, 0, s>>>(n, kk + 1, V, ldv, hdev, wv, ndev);
+        }
+        gmres_givens_kernel<<<1, 256, 0, s>>>(kk, m, hdev, refine ? ydev : nullptr, ndev, ag, Hdev, csdev, sndev,
+                                             gdev, cy, rtol * bn, (it + c + 1 >= maxit || kk + 1 == m) ? 1 : 0);
         gmres_next_basis_kernel<<<vg, 256, 0, s>>>(n, wv, V + (kk + 1) * ldv, dinv, t, cy);
       }
       MIFE_CHECK_HIP(hipGetLastError());

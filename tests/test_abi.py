@@ -160,6 +160,17 @@ def test_sell3_descriptors_without_gpu():
     assert c.block == 0
 
 
+def test_gmres_refinement_mode_checked_without_gpu():
+    """The GMRES Gram-Schmidt refinement setter takes 0 (never, the default) or 1
+    (always) and refuses anything else."""
+    from iron_amd import _lib
+    L = _lib.lib()
+    assert L.mife_gmres_cgs_refinement(2) == 2 and "refinement" in _lib.last_error()
+    assert L.mife_gmres_cgs_refinement(-1) == 2
+    assert L.mife_gmres_cgs_refinement(1) == 0
+    assert L.mife_gmres_cgs_refinement(0) == 0
+
+
 def test_one_hip_runtime_in_the_process():
     """torch bundles a HIP runtime with the soname the library links against; the
     binding loads torch's first so that the process holds one runtime (two runtimes

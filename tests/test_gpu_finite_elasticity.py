@@ -436,12 +436,19 @@ def test_minres_and_gmres_solve_saddle_point_jacobian(fmt):
     b = rng.standard_normal(len(fr))
     ref = np.linalg.solve(Kff, b)
     bt = torch.as_tensor(b, device=sysg.device)
-    for solver in ("minres", "gmres"):
+    from iron_amd import _lib
+    # GMRES with classical Gram-Schmidt refined always (two passes) and never (one
+    # pass, PETSc's default and the library's)
+    for solver in ("minres", "gmres-refine-always", "gmres"):
         dx = torch.zeros(len(fr), dtype=torch.float64, device=sysg.device)
         if solver == "minres":
             info = fem.minres(S, vals, bt, dx, 1e-12, 20000, newton.ws, pc_inv=newton.pc_pos)
         else:
-            info = fem.gmres(S, vals, bt, dx, 1e-12, 20000, newton.ws, restart=100, pc_inv=newton.pc_neg)
+            _lib.gmres_cgs_refinement(1 if solver == "gmres-refine-always" else 0)
+            try:
+                info = fem.gmres(S, vals, bt, dx, 1e-12, 20000, newton.ws, restart=100, pc_inv=newton.pc_neg)
+            finally:
+                _lib.gmres_cgs_refinement(0)
         sol = dx.cpu().numpy()
         assert info.relres < 1e-9, (solver, info)
         assert np.abs(sol - ref).max() < 1e-7 * np.abs(ref).max(), solver
