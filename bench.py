@@ -507,14 +507,17 @@ def mr_leg(runs, cpu_seconds):
         return x, infos
 
     def profiled(opts, reps):
+        """reps timed solves with the library's timing events off, then one more with
+        them on for the per-kernel times (the events cost the solve ~25% on GMRES)."""
         run(opts)
-        _lib.profile_enable(True)
-        _lib.profile_reset()
         times = []
         for _ in range(reps):
             t = time.perf_counter()
             x, infos = run(opts)
             times.append(time.perf_counter() - t)
+        _lib.profile_enable(True)
+        _lib.profile_reset()
+        run(opts)
         prof = _lib.profile_read()
         _lib.profile_enable(False)
         return x, infos, times, prof
