@@ -153,3 +153,33 @@ def test_sell3_declines_partial_nodes():
     fem, red, v, _, N = _elastic((6, 4, 4), (1.0, 1.0, 1.0), fixed_fn=rollers)
     S = fem.SolverMatrix(red, interleave=(3, N), block=True)
     assert S.block == 1 and S.fmt == "sell"
+
+
+def test_sell3_graph_replay_and_noalloc_bitwise():
+    """On SELL3 the CG result is the same bit for bit whether its iteration chunks
+    replay cached HIP graphs or launch kernel by kernel, and through the
+    entry point that allocates nothing."""
+    import ctypes as C
+    from iron_amd import _lib
+    print("case SELL3 graphs / no-alloc 24x8x8", flush=True)
+    fem, red, v, bvec, N = _elastic((24, 8, 8), (3.0, 1.0, 1.0), seed=9)
+    S = fem.SolverMatrix(red, interleave=(3, N), block=True)
+    assert S.block == 3 and red.n_free <= 131072   # the graph path's size
+    vals = S.gather(v)
+    ws = fem.KrylovWorkspace(red.n_free, "cuda")
+    xg = torch.zeros(red.n_free, dtype=torch.float64, device="cuda")
+    ig = fem.cg(S, vals, bvec, xg, 1e-10, 100000, ws, prec=fem.F64)
+    _lib.graphs_enable(False)
+    try:
+        xe = torch.zeros_like(xg)
+        ie = fem.cg(S, vals, bvec, xe, 1e-10, 100000, ws, prec=fem.F64)
+    finally:
+        _lib.graphs_enable(True)
+    assert ig.iterations == ie.iterations and torch.equal(xg, xe)
+    xn = torch.zeros_like(xg)
+    it, rr = C.c_int(0), C.c_double(0.0)
+    rc = _lib.lib().mife_cg_noalloc(C.byref(S.desc), _lib.ptr(vals), _lib.ptr(bvec), _lib.ptr(xn), fem.F64, 1e-10,
+                                    100000, C.byref(it), C.byref(rr), _lib.stream_ptr())
+    torch.cuda.synchronize()
+    assert rc == 0, _lib.last_error()
+    assert it.value == ie.iterations and torch.equal(xn, xe)
