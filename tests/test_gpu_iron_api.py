@@ -295,6 +295,8 @@ def test_linear_elasticity_cantilever_traction():
                    iron.BoundaryConditionsTypes.NEUMANN_INTEGRATED_ONLY, 0.0)
     se.BoundaryConditionsCreateFinish()
     problem.Solve()
+    # whole nodes clamped: the engine stores the solver matrix as 3x3 node blocks
+    assert es._device_cache.S.fmt == "sell3"
     u = dep.ParameterSetDataGet(iron.FieldVariableTypes.U, 1).copy()
     # oracle pipeline
     ob = fo.Basis(3, 1, 2)
